@@ -1,0 +1,393 @@
+"""arcanefem_amd -- MI355X-native P1 assembly + PCG path of ArcaneFEM.
+
+The compute path is libafem.so (hand-written HIP kernels for gfx950 behind the
+C ABI of include/afem.h).  This module is a thin ctypes front-end used by the
+tests and bench.py; it mirrors the reference's names (BSRFormat,
+DoFLinearSystem) and raises AfemError on any non-zero status.  There is no
+fallback: if libafem.so is missing or no GPU is present, calls fail loudly.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafem.so")
+_lib = None
+
+# status codes (afem_status)
+OK = 0
+ERR_ARGUMENT, ERR_NOT_IMPLEMENTED, ERR_STATE, ERR_NOT_FOUND, ERR_MESH = 1, 2, 3, 4, 5
+ERR_CAPACITY, ERR_NOT_CONVERGED, ERR_DEVICE, ERR_HIP, ERR_OUT_OF_MEMORY = 6, 7, 8, 9, 10
+
+DIRICHLET_PENALTY, DIRICHLET_ROW_ELIMINATION, DIRICHLET_ROW_COLUMN_ELIMINATION = 0, 1, 2
+SOLVER_PCG_FP64, SOLVER_PCG_MIXED, SOLVER_CG_FP64 = 0, 1, 2
+PRECOND_JACOBI, PRECOND_NONE = 0, 1
+
+
+class AfemError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"afem status {status}: {msg}")
+        self.status = status
+
+
+class BsrInfo(ctypes.Structure):
+    _fields_ = [("nb_row", ctypes.c_int32), ("nb_block", ctypes.c_int32), ("nb_col", ctypes.c_int64),
+                ("nb_non_zero_value", ctypes.c_int64), ("order_values_per_block", ctypes.c_int32),
+                ("max_row_length", ctypes.c_int32)]
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("preconditioner", ctypes.c_int), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("max_iter", ctypes.c_int), ("use_initial_guess", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("reliable_delta", ctypes.c_double), ("spmv_lanes", ctypes.c_int)]
+
+
+class SolveResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("reliable_updates", ctypes.c_int),
+                ("relres", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+
+def _declare(L):
+    vp, i32, i64, dbl, ci = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "afem_last_error": (ctypes.c_char_p, []),
+        "afem_version": (ci, []),
+        "afem_init": (ci, [ci]),
+        "afem_finalize": (ci, []),
+        "afem_synchronize": (ci, []),
+        "afem_malloc": (ci, [pp, ctypes.c_size_t]),
+        "afem_free": (ci, [vp]),
+        "afem_memcpy_htod": (ci, [vp, vp, ctypes.c_size_t]),
+        "afem_memcpy_dtoh": (ci, [vp, vp, ctypes.c_size_t]),
+        "afem_memcpy_dtod": (ci, [vp, vp, ctypes.c_size_t]),
+        "afem_memset": (ci, [vp, ci, ctypes.c_size_t]),
+        "afem_profile_enable": (ci, [ci]),
+        "afem_profile_reset": (ci, []),
+        "afem_profile_query": (ci, [ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(dbl)]),
+        "afem_box_mesh_size": (ci, [ci, ci, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "afem_box_mesh": (ci, [ci, ci, dbl, ctypes.c_uint64, vp, vp]),
+        "afem_mesh_create": (ci, [ci, i32, vp, i32, i32, vp, pp]),
+        "afem_mesh_destroy": (ci, [vp]),
+        "afem_mesh_node_cells": (ci, [vp, vp, vp]),
+        "afem_bsr_create": (ci, [vp, ci, ci, ci, pp]),
+        "afem_bsr_destroy": (ci, [vp]),
+        "afem_bsr_compute_sparsity": (ci, [vp]),
+        "afem_bsr_get_info": (ci, [vp, ctypes.POINTER(BsrInfo)]),
+        "afem_bsr_reset_values": (ci, [vp]),
+        "afem_bsr_assemble_laplace": (ci, [vp, vp, dbl, vp]),
+        "afem_bsr_assemble_elasticity": (ci, [vp, dbl, dbl]),
+        "afem_bsr_device_arrays": (ci, [vp, pp, pp, pp, pp]),
+        "afem_bsr_download": (ci, [vp, vp, vp, vp, vp]),
+        "afem_bsr_to_csr": (ci, [vp, pp, pp, ctypes.POINTER(i32)]),
+        "afem_rhs_constant_source": (ci, [vp, dbl, vp, ci, ci, vp]),
+        "afem_rhs_body_force": (ci, [vp, vp, vp]),
+        "afem_system_create": (ci, [i32, pp]),
+        "afem_system_destroy": (ci, [vp]),
+        "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
+        "afem_system_set_bsr": (ci, [vp, vp]),
+        "afem_system_rhs": (ci, [vp, pp]),
+        "afem_system_solution": (ci, [vp, pp]),
+        "afem_system_forced": (ci, [vp, pp, pp]),
+        "afem_system_elimination": (ci, [vp, pp, pp]),
+        "afem_system_apply_dirichlet": (ci, [vp, ci, i32, vp, ci, ci, dbl, dbl]),
+        "afem_system_apply_matrix_transformation": (ci, [vp]),
+        "afem_system_apply_rhs_transformation": (ci, [vp]),
+        "afem_system_clear_values": (ci, [vp]),
+        "afem_solver_default_options": (None, [ctypes.POINTER(SolverOptions)]),
+        "afem_system_solve": (ci, [vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveResult)]),
+        "afem_system_spmv": (ci, [vp, vp, vp]),
+        "afem_system_spmv_fp32": (ci, [vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return sig
+
+
+def lib():
+    """Loads libafem.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AfemError(ERR_DEVICE, f"{LIB_PATH} is missing: run python -m arcanefem_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise AfemError(rc, lib().afem_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---- device memory --------------------------------------------------------------------
+
+class DeviceArray:
+    """Device allocation of a fixed dtype/shape with numpy upload/download."""
+
+    def __init__(self, shape, dtype, ptr=None, owner=True):
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.owner = owner
+        if ptr is None:
+            p = ctypes.c_void_p()
+            check(lib().afem_malloc(ctypes.byref(p), self.nbytes))
+            self.ptr = p.value
+        else:
+            self.ptr = ptr
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    @classmethod
+    def view(cls, ptr, shape, dtype):
+        return cls(shape, dtype, ptr=ptr, owner=False)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().afem_memcpy_htod(ctypes.c_void_p(self.ptr), _ptr(a), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().afem_memcpy_dtoh(_ptr(out), ctypes.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().afem_memset(ctypes.c_void_p(self.ptr), 0, self.nbytes))
+
+    @property
+    def c(self):
+        return ctypes.c_void_p(self.ptr)
+
+    def free(self):
+        if self.owner and self.ptr:
+            check(lib().afem_free(ctypes.c_void_p(self.ptr)))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.owner and self.ptr and _lib is not None:
+                _lib.afem_free(ctypes.c_void_p(self.ptr))
+        except Exception:
+            pass
+
+
+def init(device=0):
+    check(lib().afem_init(device))
+
+
+def synchronize():
+    check(lib().afem_synchronize())
+
+
+# ---- synthetic meshes (host) -------------------------------------------------------------
+
+def box_mesh(dim, n, jitter=0.0, seed=0):
+    """Jittered structured tri (dim 2) / Kuhn-tet (dim 3) mesh of the unit box."""
+    nn, nc = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().afem_box_mesh_size(dim, n, ctypes.byref(nn), ctypes.byref(nc)))
+    coords = np.zeros((nn.value, 3), np.float64)
+    cells = np.zeros((nc.value, 4), np.int32)
+    check(lib().afem_box_mesh(dim, n, float(jitter), ctypes.c_uint64(seed), _ptr(coords), _ptr(cells)))
+    return coords, cells
+
+
+# ---- mesh / BSRFormat / DoFLinearSystem ------------------------------------------------------
+
+class Mesh:
+    def __init__(self, dim, coords, cells):
+        coords = np.ascontiguousarray(coords, np.float64)
+        cells = np.ascontiguousarray(cells, np.int32)
+        assert coords.ndim == 2 and coords.shape[1] == 3 and cells.ndim == 2 and cells.shape[1] == 4
+        self.dim = dim
+        self.nb_node = coords.shape[0]
+        self.nb_cell = cells.shape[0]
+        self.npc = dim + 1
+        h = ctypes.c_void_p()
+        check(lib().afem_mesh_create(dim, self.nb_node, _ptr(coords), self.nb_cell, self.npc, _ptr(cells),
+                                     ctypes.byref(h)))
+        self.h = h
+
+    def node_cells(self):
+        off = np.zeros(self.nb_node + 1, np.int32)
+        check(lib().afem_mesh_node_cells(self.h, _ptr(off), None))
+        items = np.zeros(max(1, int(off[-1])), np.int32)
+        check(lib().afem_mesh_node_cells(self.h, None, _ptr(items)))
+        return off, items[: off[-1]]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.afem_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+
+class BSRFormat:
+    """Mirror of femutils/BSRFormat.h: initialize in the constructor."""
+
+    def __init__(self, mesh, nb_dof=1, use_csr=False, use_atomic_free=True):
+        self.mesh = mesh
+        h = ctypes.c_void_p()
+        check(lib().afem_bsr_create(mesh.h, nb_dof, 1 if use_csr else 0, 1 if use_atomic_free else 0,
+                                    ctypes.byref(h)))
+        self.h = h
+
+    def info(self):
+        i = BsrInfo()
+        check(lib().afem_bsr_get_info(self.h, ctypes.byref(i)))
+        return i
+
+    def compute_sparsity(self):
+        check(lib().afem_bsr_compute_sparsity(self.h))
+
+    def reset_values(self):
+        check(lib().afem_bsr_reset_values(self.h))
+
+    def assemble_laplace(self, cell_lambda=None, source_f=0.0, rhs=None):
+        lam = cell_lambda.c if cell_lambda is not None else None
+        check(lib().afem_bsr_assemble_laplace(self.h, lam, float(source_f), rhs.c if rhs is not None else None))
+
+    def assemble_elasticity(self, lame_lambda, lame_mu):
+        check(lib().afem_bsr_assemble_elasticity(self.h, float(lame_lambda), float(lame_mu)))
+
+    def download(self):
+        i = self.info()
+        row = np.zeros(i.nb_row + 1, np.int32)
+        col = np.zeros(i.nb_col, np.int32)
+        val = np.zeros(i.nb_non_zero_value, np.float64)
+        nzr = np.zeros(i.nb_row, np.int32)
+        check(lib().afem_bsr_download(self.h, _ptr(row), _ptr(col), _ptr(val), _ptr(nzr)))
+        return row, col, val, nzr
+
+    def device_arrays(self):
+        r, c, v, z = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().afem_bsr_device_arrays(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(v),
+                                           ctypes.byref(z)))
+        return r.value, c.value, v.value, z.value
+
+    def to_csr(self):
+        r, c, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
+        check(lib().afem_bsr_to_csr(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(n)))
+        i = self.info()
+        row = DeviceArray.view(r.value, n.value + 1, np.int32).numpy()
+        col = DeviceArray.view(c.value, i.nb_non_zero_value, np.int32).numpy()
+        return row, col
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.afem_bsr_destroy(self.h)
+        except Exception:
+            pass
+
+
+def solver_options(**kw):
+    o = SolverOptions()
+    lib().afem_solver_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class DoFLinearSystem:
+    """Mirror of the IDoFLinearSystemImpl created by the MI355X factory."""
+
+    def __init__(self, nb_dof):
+        h = ctypes.c_void_p()
+        check(lib().afem_system_create(nb_dof, ctypes.byref(h)))
+        self.h = h
+        self.nb_dof = nb_dof
+
+    def _arr(self, fn, dtype):
+        p = ctypes.c_void_p()
+        check(fn(self.h, ctypes.byref(p)))
+        return DeviceArray.view(p.value, self.nb_dof, dtype)
+
+    def rhs(self):
+        return self._arr(lib().afem_system_rhs, np.float64)
+
+    def solution(self):
+        return self._arr(lib().afem_system_solution, np.float64)
+
+    def forced(self):
+        i, v = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().afem_system_forced(self.h, ctypes.byref(i), ctypes.byref(v)))
+        return DeviceArray.view(i.value, self.nb_dof, np.uint8), DeviceArray.view(v.value, self.nb_dof, np.float64)
+
+    def elimination(self):
+        i, v = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().afem_system_elimination(self.h, ctypes.byref(i), ctypes.byref(v)))
+        return DeviceArray.view(i.value, self.nb_dof, np.uint8), DeviceArray.view(v.value, self.nb_dof, np.float64)
+
+    def set_csr_values(self, row, col, val):
+        """row/col/val are DeviceArray (kept alive by the caller until solve)."""
+        check(lib().afem_system_set_csr_values(self.h, self.nb_dof, int(np.prod(col.shape)), row.c, col.c, val.c))
+
+    def set_bsr(self, bsr):
+        check(lib().afem_system_set_bsr(self.h, bsr.h))
+
+    def apply_dirichlet(self, method, nodes, nb_dof_per_node=1, dof_index=0, value=0.0, penalty=1e30):
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        check(lib().afem_system_apply_dirichlet(self.h, method, len(nodes), _ptr(nodes), nb_dof_per_node,
+                                                dof_index, float(value), float(penalty)))
+
+    def apply_matrix_transformation(self):
+        check(lib().afem_system_apply_matrix_transformation(self.h))
+
+    def apply_rhs_transformation(self):
+        check(lib().afem_system_apply_rhs_transformation(self.h))
+
+    def clear_values(self):
+        check(lib().afem_system_clear_values(self.h))
+
+    def solve(self, raise_on_failure=True, **kw):
+        o = solver_options(**kw)
+        r = SolveResult()
+        rc = lib().afem_system_solve(self.h, ctypes.byref(o), ctypes.byref(r))
+        if rc != OK and (raise_on_failure or rc != ERR_NOT_CONVERGED):
+            check(rc)
+        return r
+
+    def spmv(self, x, y):
+        check(lib().afem_system_spmv(self.h, x.c, y.c))
+
+    def spmv_fp32(self, x, y):
+        check(lib().afem_system_spmv_fp32(self.h, x.c, y.c))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.afem_system_destroy(self.h)
+        except Exception:
+            pass
+
+
+def profile_enable(on=True):
+    check(lib().afem_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().afem_profile_reset())
+
+
+def profile_query(kernel):
+    n, t = ctypes.c_int64(), ctypes.c_double()
+    check(lib().afem_profile_query(kernel.encode(), ctypes.byref(n), ctypes.byref(t)))
+    return n.value, t.value

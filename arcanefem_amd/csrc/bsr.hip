@@ -1,0 +1,686 @@
+// bsr.hip -- BSRFormat on the MI355X: atomic-free sparsity pattern built from
+// the node->cell connectivity, and node-wise gather assembly of the P1
+// Laplace / conductivity / elasticity bilinear forms with the row staged in
+// LDS (no floating-point atomics anywhere), plus the scalar-CSR expansion.
+//
+// Reference behaviour: femutils/BSRFormat.h:143-565, femutils/BSRFormat.cc:48-1053.
+#include "common.h"
+#include "device_utils.h"
+
+#include <algorithm>
+
+namespace afem {
+
+// ---- sparsity -------------------------------------------------------------------
+// One thread per node.  The thread walks its node->cell list and inserts the
+// other nodes of each cell into a sorted, duplicate-free list held in LDS
+// (column-major per block so that lanes hit different banks).  Count pass:
+// writes the row length.  Fill pass: writes the sorted row.  A list that
+// would exceed CAP raises the overflow flag; the host then retries with a
+// larger CAP.
+
+template <int CAP, int BS>
+__global__ void __launch_bounds__(BS)
+k_pattern_rows(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+               const int32_t* __restrict__ cells, int npc, int32_t nb_node,
+               const int32_t* __restrict__ row_index, int32_t* __restrict__ out,
+               int32_t* __restrict__ overflow)
+{
+  __shared__ int32_t list[CAP * BS];
+  const int tid = threadIdx.x;
+  const int32_t i = blockIdx.x * BS + tid;
+  if (i >= nb_node)
+    return;
+  int m = 1;
+  list[tid] = i; // the diagonal
+  bool ovf = false;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int32_t c = it >> 2;
+    const int s = it & 3;
+    for (int k = 0; k < npc; ++k) {
+      if (k == s)
+        continue;
+      const int32_t v = cells[4 * c + k];
+      int pos = 0;
+      while (pos < m && list[pos * BS + tid] < v)
+        ++pos;
+      if (pos < m && list[pos * BS + tid] == v)
+        continue;
+      if (m == CAP) {
+        ovf = true;
+        continue;
+      }
+      for (int q = m; q > pos; --q)
+        list[q * BS + tid] = list[(q - 1) * BS + tid];
+      list[pos * BS + tid] = v;
+      ++m;
+    }
+  }
+  if (ovf) {
+    atomicOr(overflow, 1);
+    return;
+  }
+  if (row_index == nullptr) {
+    out[i] = m;
+  }
+  else {
+    const int32_t base = row_index[i];
+    for (int q = 0; q < m; ++q)
+      out[base + q] = list[q * BS + tid];
+  }
+}
+
+// Runs one pass (count if row_index == nullptr, fill otherwise) with the
+// smallest LDS capacity that holds every row.  Returns false on overflow at
+// the largest capacity.
+static bool pattern_pass(const afem_mesh& m, const int32_t* row_index, int32_t* out, int cap_hint)
+{
+  DBuf<int32_t> ovf(1);
+  auto run = [&](auto cap_tag, auto bs_tag) {
+    constexpr int CAP = decltype(cap_tag)::value;
+    constexpr int BS = decltype(bs_tag)::value;
+    ovf.zero();
+    hipLaunchKernelGGL((k_pattern_rows<CAP, BS>), dim3(grid_for(m.nb_node, BS)), dim3(BS), 0, stream(),
+                       m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.nb_node, row_index, out, ovf.p);
+    check_launch("pattern_rows");
+    int32_t flag = 0;
+    ovf.download(&flag, 1);
+    return flag == 0;
+  };
+  if (cap_hint <= 32 && run(std::integral_constant<int, 32>{}, std::integral_constant<int, 256>{}))
+    return true;
+  if (cap_hint <= 128 && run(std::integral_constant<int, 128>{}, std::integral_constant<int, 64>{}))
+    return true;
+  return run(std::integral_constant<int, 512>{}, std::integral_constant<int, 16>{});
+}
+
+static int row_cap_hint(const afem_mesh& m)
+{
+  int64_t bound = 1 + (int64_t)(m.npc - 1) * m.max_degree;
+  return bound > 512 ? 512 : (int)bound;
+}
+
+// nb_col of BSRFormat::computeNbColumns (femutils/BSRFormat.cc:256-345): for P1
+// simplices the number of node pairs sharing a cell (N + 2E).
+static void bsr_count_columns(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->nnz_per_row.alloc((size_t)m.nb_node);
+  if (!pattern_pass(m, nullptr, b->nnz_per_row.p, row_cap_hint(m)))
+    fail(AFEM_ERR_CAPACITY, "a node has more than 511 neighbours");
+  b->row_index.alloc((size_t)m.nb_node + 1);
+  int64_t total = exclusive_scan_i32(b->nnz_per_row.p, b->row_index.p, m.nb_node);
+  b->nb_col = total;
+  AFEM_REQUIRE(total * b->nb_dof * b->nb_dof < (1LL << 31), AFEM_ERR_ARGUMENT,
+               "matrix too large for 32-bit value offsets");
+}
+
+__global__ void k_max_row(const int32_t* __restrict__ len, int32_t n, int32_t* __restrict__ out)
+{
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = (i < n) ? len[i] : 0;
+  v = max(v, __shfl_xor(v, 32, 64));
+  v = max(v, __shfl_xor(v, 16, 64));
+  v = max(v, __shfl_xor(v, 8, 64));
+  v = max(v, __shfl_xor(v, 4, 64));
+  v = max(v, __shfl_xor(v, 2, 64));
+  v = max(v, __shfl_xor(v, 1, 64));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(out, v);
+}
+
+static void bsr_compute_sparsity(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->columns.alloc((size_t)b->nb_col);
+  if (!pattern_pass(m, b->row_index.p, b->columns.p, row_cap_hint(m)))
+    fail(AFEM_ERR_CAPACITY, "a node has more than 511 neighbours");
+  DBuf<int32_t> mx(1);
+  mx.zero();
+  hipLaunchKernelGGL(k_max_row, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), b->nnz_per_row.p,
+                     m.nb_node, mx.p);
+  check_launch("max_row");
+  mx.download(&b->max_row, 1);
+  b->values.alloc((size_t)b->nb_col * b->nb_dof * b->nb_dof);
+  b->values.zero();
+  b->has_sparsity = true;
+}
+
+// ---- element kernels ---------------------------------------------------------------
+
+struct d3
+{
+  double x, y, z;
+};
+
+__device__ __forceinline__ d3 load_node(const double* __restrict__ coords, int32_t n)
+{
+  const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)n);
+  return { xy.x, xy.y, coords[4 * (int64_t)n + 2] };
+}
+
+__device__ __forceinline__ d3 sub(d3 a, d3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ __forceinline__ d3 cross(d3 a, d3 b)
+{
+  return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+}
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Unnormalised P1 gradients of a tetrahedron: grad N_q = C[q] / det with
+// det = (p1-p0) . ((p2-p0) x (p3-p0)); returns |det| (= 6 |T|).
+__device__ __forceinline__ double tet_cofactors(const double* __restrict__ coords, const int32_t* cn, d3 C[4])
+{
+  const d3 p0 = load_node(coords, cn[0]);
+  const d3 e1 = sub(load_node(coords, cn[1]), p0);
+  const d3 e2 = sub(load_node(coords, cn[2]), p0);
+  const d3 e3 = sub(load_node(coords, cn[3]), p0);
+  C[1] = cross(e2, e3);
+  C[2] = cross(e3, e1);
+  C[3] = cross(e1, e2);
+  C[0] = { -(C[1].x + C[2].x + C[3].x), -(C[1].y + C[2].y + C[3].y), -(C[1].z + C[2].z + C[3].z) };
+  return fabs(dot(e1, C[1]));
+}
+
+// Triangle: grad N_q = G[q] / det, det = twice the signed area; returns |det|.
+__device__ __forceinline__ double tri_cofactors(const double* __restrict__ coords, const int32_t* cn, d3 G[3])
+{
+  const d3 p0 = load_node(coords, cn[0]);
+  const d3 e1 = sub(load_node(coords, cn[1]), p0);
+  const d3 e2 = sub(load_node(coords, cn[2]), p0);
+  G[1] = { e2.y, -e2.x, 0.0 };
+  G[2] = { -e1.y, e1.x, 0.0 };
+  G[0] = { -(G[1].x + G[2].x), -(G[1].y + G[2].y), 0.0 };
+  return fabs(e1.x * e2.y - e2.x * e1.y);
+}
+
+// Position of column v inside a sorted row held in LDS.
+__device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int32_t v)
+{
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cols[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Node-wise gather assembly of the P1 Laplace form, one thread per row.
+// The block's rows are consecutive, so their values form one contiguous
+// range [row_index[r0], row_index[r1]) that is staged in LDS (columns +
+// accumulators), zero-filled, accumulated by each row's owner thread without
+// atomics, and written back with coalesced stores.  Each (row, cell) pair
+// recomputes the element row from the 4 (3) node coordinates: the element
+// kernel is recomputed npc times per cell, which costs FP64 ALU instead of
+// HBM bytes.  Optionally the same pass writes the constant-source RHS.
+template <int NPC>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                   const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                   const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                   int32_t nb_node, int rows_per_block,
+                   const double* __restrict__ cell_lambda, double source_f,
+                   double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t tile = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t r0 = tile * rows_per_block;
+  const int32_t r1 = min(r0 + rows_per_block, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  double* s_val = reinterpret_cast<double*>(smem);
+  int32_t* s_col = reinterpret_cast<int32_t*>(s_val + cnt);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    s_col[k] = columns[a + k];
+    s_val[k] = 0.0;
+  }
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (threadIdx.x < rows_per_block && i < r1) {
+    const int32_t rb = row_index[i] - a;
+    const int len = row_index[i + 1] - row_index[i];
+    const int32_t* rc = s_col + rb;
+    double* rv = s_val + rb;
+    const int dpos = find_col(rc, len, i);
+    double b_acc = 0.0;
+    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+      const int32_t it = nc_items[e];
+      const int32_t c = it >> 2;
+      const int s = it & 3;
+      const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+      const int32_t cn[4] = { cq.x, cq.y, cq.z, cq.w };
+      const double lam = cell_lambda ? cell_lambda[c] : 1.0;
+      if constexpr (NPC == 4) {
+        d3 C[4];
+        const double adet = tet_cofactors(coords, cn, C);
+        const double scale = lam / (6.0 * adet);
+        const d3 cs = C[s];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
+          if (pos < len)
+            rv[pos] += scale * dot(cs, C[q]);
+        }
+        b_acc += source_f * (adet / 6.0) / 4;
+      }
+      else {
+        d3 G[3];
+        const double adet = tri_cofactors(coords, cn, G);
+        const double scale = lam / (2.0 * adet);
+        const d3 gs = G[s];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
+          if (pos < len)
+            rv[pos] += scale * (gs.x * G[q].x + gs.y * G[q].y);
+        }
+        b_acc += source_f * (adet / 2.0) / 3;
+      }
+    }
+    if (rhs)
+      rhs[i] = b_acc;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    values[a + k] = s_val[k];
+}
+
+// Node-wise gather assembly of the 12x12 P1 elasticity element into 3x3
+// blocks: K_ab = (1/(6|det|)) [lambda C_a C_b^T + mu C_b C_a^T + mu (C_a.C_b) I],
+// the closed form of modules/elasticity/ElementMatrix.h:151-211.  Blocks are
+// staged in LDS (9 doubles each, row-major) and written in the requested
+// order: per block (block*9 + i*3 + j) or per row (scalar CSR order,
+// row_index[a]*9 + 3*(x + i*nnz_row) + j).
+__global__ void __launch_bounds__(256)
+k_assemble_elasticity(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                      const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                      const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                      int32_t nb_node, int rows_per_block, double lame_lambda, double lame_mu,
+                      int per_block, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t tile = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t r0 = tile * rows_per_block;
+  const int32_t r1 = min(r0 + rows_per_block, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  double* s_val = reinterpret_cast<double*>(smem);
+  int32_t* s_col = reinterpret_cast<int32_t*>(s_val + 9 * (int64_t)cnt);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    s_col[k] = columns[a + k];
+  for (int k = threadIdx.x; k < 9 * cnt; k += blockDim.x)
+    s_val[k] = 0.0;
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (threadIdx.x < rows_per_block && i < r1) {
+    const int32_t rb = row_index[i] - a;
+    const int len = row_index[i + 1] - row_index[i];
+    const int32_t* rc = s_col + rb;
+    double* rv = s_val + 9 * (int64_t)rb;
+    const int dpos = find_col(rc, len, i);
+    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+      const int32_t it = nc_items[e];
+      const int32_t c = it >> 2;
+      const int s = it & 3;
+      const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+      const int32_t cn[4] = { cq.x, cq.y, cq.z, cq.w };
+      d3 C[4];
+      const double adet = tet_cofactors(coords, cn, C);
+      const double inv6 = 1.0 / (6.0 * adet);
+      const double la = lame_lambda * inv6, mu = lame_mu * inv6;
+      const double ca[3] = { C[s].x, C[s].y, C[s].z };
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
+        if (pos >= len)
+          continue;
+        const double cb[3] = { C[q].x, C[q].y, C[q].z };
+        const double dd = mu * (ca[0] * cb[0] + ca[1] * cb[1] + ca[2] * cb[2]);
+        double* blk = rv + 9 * pos;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int t = 0; t < 3; ++t)
+            blk[r * 3 + t] += la * ca[r] * cb[t] + mu * cb[r] * ca[t] + (r == t ? dd : 0.0);
+      }
+    }
+  }
+  __syncthreads();
+  if (per_block) {
+    for (int k = threadIdx.x; k < 9 * cnt; k += blockDim.x)
+      values[9 * (int64_t)a + k] = s_val[k];
+  }
+  else {
+    // one thread per row: scatter the row's blocks into scalar-CSR order
+    if (threadIdx.x < rows_per_block && i < r1) {
+      const int32_t rb = row_index[i] - a;
+      const int len = row_index[i + 1] - row_index[i];
+      const int64_t base = 9 * (int64_t)row_index[i];
+      for (int r = 0; r < 3; ++r)
+        for (int x = 0; x < len; ++x)
+          for (int t = 0; t < 3; ++t)
+            values[base + 3 * (x + (int64_t)r * len) + t] = s_val[9 * (int64_t)(rb + x) + r * 3 + t];
+    }
+  }
+}
+
+// Rows per block and LDS bytes for an assembly launch; the block's values
+// range is bounded by rows_per_block * max_row.
+static void assembly_shape(const afem_bsr& b, int bytes_per_entry, int* rows_per_block, size_t* lds)
+{
+  int rows = 256;
+  const size_t lds_budget = 64 * 1024;
+  while (rows > 16 && (size_t)rows * b.max_row * bytes_per_entry > lds_budget)
+    rows >>= 1;
+  size_t bytes = (size_t)rows * b.max_row * bytes_per_entry;
+  AFEM_REQUIRE(bytes <= 160 * 1024, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
+  *rows_per_block = rows;
+  *lds = bytes;
+}
+
+static void assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs)
+{
+  AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
+  AFEM_REQUIRE(b->nb_dof == 1, AFEM_ERR_ARGUMENT, "the Laplace element needs nb_dof = 1");
+  const afem_mesh& m = *b->mesh;
+  int rows;
+  size_t lds;
+  assembly_shape(*b, 12, &rows, &lds);
+  const unsigned grid = grid_for(m.nb_node, rows);
+  ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
+  if (m.npc == 4)
+    hipLaunchKernelGGL(k_assemble_laplace<4>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+                       m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
+                       rhs, b->values.p);
+  else
+    hipLaunchKernelGGL(k_assemble_laplace<3>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+                       m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
+                       rhs, b->values.p);
+  check_launch("assemble_laplace");
+}
+
+static void assemble_elasticity(afem_bsr* b, double lam, double mu)
+{
+  AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
+  AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4, AFEM_ERR_ARGUMENT,
+               "the elasticity element needs a tetrahedral mesh and nb_dof = 3");
+  const afem_mesh& m = *b->mesh;
+  int rows;
+  size_t lds;
+  assembly_shape(*b, 9 * 8 + 4, &rows, &lds);
+  const unsigned grid = grid_for(m.nb_node, rows);
+  ProfScope prof("assemble_elasticity_tet");
+  hipLaunchKernelGGL(k_assemble_elasticity, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+                     m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, lam, mu,
+                     b->per_block ? 1 : 0, b->values.p);
+  check_launch("assemble_elasticity");
+}
+
+// ---- right-hand side (node-wise gather) -----------------------------------------------
+
+__global__ void k_rhs_source(const double* __restrict__ coords, const int32_t* __restrict__ cells, int npc,
+                             const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                             int32_t nb_node, double f, const double* __restrict__ cell_scale, int nb_dof,
+                             int dof_index, double* __restrict__ rhs)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  double sum = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t c = nc_items[e] >> 2;
+    const int32_t* cn = cells + 4 * (int64_t)c;
+    double measure;
+    if (npc == 4) {
+      const d3 p0 = load_node(coords, cn[0]);
+      const d3 e1 = sub(load_node(coords, cn[1]), p0);
+      const d3 e2 = sub(load_node(coords, cn[2]), p0);
+      const d3 e3 = sub(load_node(coords, cn[3]), p0);
+      measure = fabs(dot(e1, cross(e2, e3))) / 6.0;
+    }
+    else {
+      const d3 p0 = load_node(coords, cn[0]);
+      const d3 e1 = sub(load_node(coords, cn[1]), p0);
+      const d3 e2 = sub(load_node(coords, cn[2]), p0);
+      const d3 v = cross(e1, e2);
+      measure = sqrt(dot(v, v)) / 2.0;
+    }
+    const double q = cell_scale ? f * cell_scale[c] : f;
+    sum += q * measure / npc;
+  }
+  rhs[(int64_t)i * nb_dof + dof_index] = sum;
+}
+
+__global__ void k_rhs_body_force(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                                 const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                                 int32_t nb_node, double fx, double fy, double fz, double* __restrict__ rhs)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  double vsum = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t* cn = cells + 4 * (int64_t)(nc_items[e] >> 2);
+    const d3 p0 = load_node(coords, cn[0]);
+    const d3 e1 = sub(load_node(coords, cn[1]), p0);
+    const d3 e2 = sub(load_node(coords, cn[2]), p0);
+    const d3 e3 = sub(load_node(coords, cn[3]), p0);
+    vsum += fabs(dot(e1, cross(e2, e3))) / 6.0 / 4;
+  }
+  rhs[3 * (int64_t)i + 0] += fx * vsum;
+  rhs[3 * (int64_t)i + 1] += fy * vsum;
+  rhs[3 * (int64_t)i + 2] += fz * vsum;
+}
+
+// ---- BSRMatrix::toCsr on the device ----------------------------------------------------
+
+__global__ void k_csr_rows(const int32_t* __restrict__ row_index, int32_t nb_node, int nb, int32_t* __restrict__ csr_row)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nb_node)
+    return;
+  if (i == nb_node) {
+    csr_row[(int64_t)nb * nb_node] = row_index[nb_node] * nb * nb;
+    return;
+  }
+  const int32_t len = row_index[i + 1] - row_index[i];
+  for (int r = 0; r < nb; ++r)
+    csr_row[(int64_t)nb * i + r] = row_index[i] * nb * nb + r * len * nb;
+}
+
+__global__ void k_csr_cols(const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns, int32_t nb_node,
+                           int nb, int32_t* __restrict__ csr_col)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t b0 = row_index[i], len = row_index[i + 1] - b0;
+  for (int r = 0; r < nb; ++r)
+    for (int x = 0; x < len; ++x)
+      for (int k = 0; k < nb; ++k)
+        csr_col[(int64_t)b0 * nb * nb + (int64_t)r * len * nb + x * nb + k] = columns[b0 + x] * nb + k;
+}
+
+static void bsr_to_csr(afem_bsr* b)
+{
+  AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called first");
+  const int nb = b->nb_dof;
+  const int32_t n = b->nb_row;
+  if (b->csr_row.p)
+    return;
+  b->csr_row.alloc((size_t)n * nb + 1);
+  b->csr_col.alloc((size_t)b->nb_col * nb * nb);
+  hipLaunchKernelGGL(k_csr_rows, dim3(grid_for((int64_t)n + 1, 256)), dim3(256), 0, stream(), b->row_index.p, n, nb,
+                     b->csr_row.p);
+  check_launch("csr_rows");
+  hipLaunchKernelGGL(k_csr_cols, dim3(grid_for(n, 256)), dim3(256), 0, stream(), b->row_index.p, b->columns.p, n, nb,
+                     b->csr_col.p);
+  check_launch("csr_cols");
+}
+
+} // namespace afem
+
+using namespace afem;
+
+extern "C" {
+
+AFEM_API int afem_bsr_create(afem_mesh_t mesh, int nb_dof, int does_linear_system_use_csr, int use_atomic_free,
+                             afem_bsr_t* out)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && out, AFEM_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  // femutils/BSRFormat.cc:51-55
+  AFEM_REQUIRE(nb_dof > 0, AFEM_ERR_ARGUMENT, "BSRMatrix(initialize): block_size should be positive");
+  AFEM_REQUIRE(nb_dof <= mesh->nb_node, AFEM_ERR_ARGUMENT, "BSRMatrix(initialize): block_size should be less than nb_row");
+  AFEM_REQUIRE(nb_dof == 1 || nb_dof == 3, AFEM_ERR_NOT_IMPLEMENTED, "block sizes 1 and 3 are supported");
+  auto* b = new afem_bsr();
+  try {
+    b->mesh = mesh;
+    b->nb_dof = nb_dof;
+    b->use_csr = does_linear_system_use_csr != 0;
+    b->per_block = !b->use_csr;
+    b->atomic_free = use_atomic_free != 0;
+    b->nb_row = mesh->nb_node;
+    bsr_count_columns(b);
+  }
+  catch (...) {
+    delete b;
+    throw;
+  }
+  *out = b;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_destroy(afem_bsr_t bsr)
+{
+  AFEM_GUARD_BEGIN
+  if (bsr) {
+    AFEM_HIP(hipStreamSynchronize(stream()));
+    delete bsr;
+  }
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_compute_sparsity(afem_bsr_t bsr)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  bsr_compute_sparsity(bsr);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_get_info(afem_bsr_t bsr, afem_bsr_info* info)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr && info, AFEM_ERR_ARGUMENT, "null argument");
+  info->nb_row = bsr->nb_row;
+  info->nb_block = bsr->nb_dof;
+  info->nb_col = bsr->nb_col;
+  info->nb_non_zero_value = bsr->nb_col * bsr->nb_dof * bsr->nb_dof;
+  info->order_values_per_block = bsr->per_block ? 1 : 0;
+  info->max_row_length = bsr->max_row;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_reset_values(afem_bsr_t bsr)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr && bsr->has_sparsity, AFEM_ERR_STATE, "no sparsity yet");
+  bsr->values.zero();
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda, double source_f, double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  assemble_laplace(bsr, cell_lambda, source_f, rhs);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  assemble_elasticity(bsr, lame_lambda, lame_mu);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_device_arrays(afem_bsr_t bsr, const int32_t** row_index, const int32_t** columns, double** values,
+                                    const int32_t** nnz_per_row)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  if (row_index)
+    *row_index = bsr->row_index.p;
+  if (columns)
+    *columns = bsr->columns.p;
+  if (values)
+    *values = bsr->values.p;
+  if (nnz_per_row)
+    *nnz_per_row = bsr->nnz_per_row.p;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_download(afem_bsr_t bsr, int32_t* row_index, int32_t* columns, double* values,
+                               int32_t* nnz_per_row)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  if (row_index)
+    bsr->row_index.download(row_index, (size_t)bsr->nb_row + 1);
+  if (nnz_per_row)
+    bsr->nnz_per_row.download(nnz_per_row, (size_t)bsr->nb_row);
+  if (columns || values)
+    AFEM_REQUIRE(bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity not called");
+  if (columns)
+    bsr->columns.download(columns, (size_t)bsr->nb_col);
+  if (values)
+    bsr->values.download(values, bsr->values.n);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int32_t** csr_col, int32_t* nb_csr_row)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  bsr_to_csr(bsr);
+  if (csr_row)
+    *csr_row = bsr->csr_row.p;
+  if (csr_col)
+    *csr_col = bsr->csr_col.p;
+  if (nb_csr_row)
+    *nb_csr_row = bsr->nb_row * bsr->nb_dof;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* cell_scale, int nb_dof, int dof_index,
+                                      double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && rhs, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  ProfScope prof("rhs_constant_source");
+  hipLaunchKernelGGL(k_rhs_source, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
+                     mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, f, cell_scale,
+                     nb_dof, dof_index, rhs);
+  check_launch("rhs_source");
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_rhs_body_force(afem_mesh_t mesh, const double* f, double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && f && rhs, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(mesh->npc == 4, AFEM_ERR_NOT_IMPLEMENTED, "body force needs a tetrahedral mesh");
+  hipLaunchKernelGGL(k_rhs_body_force, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
+                     mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, f[0], f[1], f[2], rhs);
+  check_launch("rhs_body_force");
+  AFEM_GUARD_END
+}
+
+} // extern "C"

@@ -1,0 +1,243 @@
+// common.h -- shared internals of libafem: error handling, the library
+// stream, device buffers, per-kernel event timing and the handle structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "afem.h"
+
+namespace afem {
+
+// Error carried from internal code to the C ABI, where it becomes a status
+// code plus the thread-local message of afem_last_error().
+struct Error : std::runtime_error
+{
+  int status;
+  Error(int s, const std::string& msg)
+  : std::runtime_error(msg)
+  , status(s)
+  {}
+};
+
+[[noreturn]] void fail(int status, const std::string& msg);
+void set_last_error(const std::string& msg);
+
+#define AFEM_HIP(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      ::afem::fail(e_ == hipErrorOutOfMemory ? AFEM_ERR_OUT_OF_MEMORY : AFEM_ERR_HIP,   \
+                   std::string(#call) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+#define AFEM_REQUIRE(cond, status, msg)                                                 \
+  do {                                                                                  \
+    if (!(cond))                                                                        \
+      ::afem::fail((status), (msg));                                                    \
+  } while (0)
+
+// Library runtime: one device, one stream of default priority.
+struct Runtime
+{
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  bool profiling = false;
+};
+Runtime& rt();
+void ensure_init();
+inline hipStream_t stream()
+{
+  ensure_init();
+  return rt().stream;
+}
+
+// Owning device buffer.
+template <class T>
+struct DBuf
+{
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept
+  : p(o.p)
+  , n(o.n)
+  {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept
+  {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count)
+  {
+    release();
+    if (count) {
+      ensure_init();
+      AFEM_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    n = count;
+  }
+  // grow-only reallocation (contents not preserved)
+  void reserve(size_t count)
+  {
+    if (count > n)
+      alloc(count);
+  }
+  void release()
+  {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void zero() { zero_n(n); }
+  void zero_n(size_t count)
+  {
+    if (count)
+      AFEM_HIP(hipMemsetAsync(p, 0, count * sizeof(T), stream()));
+  }
+  // synchronous: the host array may be a temporary
+  void upload(const T* host, size_t count)
+  {
+    if (count) {
+      AFEM_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+      AFEM_HIP(hipStreamSynchronize(stream()));
+    }
+  }
+  void download(T* host, size_t count) const
+  {
+    if (count) {
+      AFEM_HIP(hipMemcpyAsync(host, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+      AFEM_HIP(hipStreamSynchronize(stream()));
+    }
+  }
+};
+
+// Event pair recorded around one launch when profiling is on.
+struct ProfScope
+{
+  const char* name;
+  hipEvent_t start = nullptr;
+  explicit ProfScope(const char* kernel_name);
+  ~ProfScope();
+};
+
+// Checked grid size for 1-D launches.
+inline unsigned grid_for(int64_t work, int block)
+{
+  int64_t g = (work + block - 1) / block;
+  if (g < 1)
+    g = 1;
+  if (g > 0x7fffffffLL)
+    fail(AFEM_ERR_ARGUMENT, "launch too large");
+  return (unsigned)g;
+}
+
+inline void check_launch(const char* what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    fail(AFEM_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
+}
+
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total).  Returns
+// the total (synchronous read-back).  The total must fit in int32.
+int64_t exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n);
+
+} // namespace afem
+
+// C-ABI bodies: exceptions become status codes + afem_last_error().
+#define AFEM_GUARD_BEGIN try {
+#define AFEM_GUARD_END                                                                    \
+  }                                                                                       \
+  catch (const afem::Error& e)                                                            \
+  {                                                                                       \
+    afem::set_last_error(e.what());                                                       \
+    return e.status;                                                                      \
+  }                                                                                       \
+  catch (const std::bad_alloc&)                                                           \
+  {                                                                                       \
+    afem::set_last_error("host allocation failed");                                      \
+    return AFEM_ERR_OUT_OF_MEMORY;                                                        \
+  }                                                                                       \
+  catch (const std::exception& e)                                                         \
+  {                                                                                       \
+    afem::set_last_error(e.what());                                                       \
+    return AFEM_ERR_HIP;                                                                  \
+  }                                                                                       \
+  return AFEM_OK;
+
+// ---- handle structs --------------------------------------------------------
+
+struct afem_mesh
+{
+  int dim = 0;
+  int npc = 0;
+  int32_t nb_node = 0;
+  int32_t nb_cell = 0;
+  int32_t max_degree = 0;            // largest node->cell list
+  afem::DBuf<double> coords;         // nb_node * 4 (x, y, z, 0)
+  afem::DBuf<int32_t> cells;         // nb_cell * 4
+  afem::DBuf<int32_t> nc_offsets;    // nb_node + 1
+  afem::DBuf<int32_t> nc_items;      // cell*4 + slot, sorted by cell per node
+};
+
+struct afem_bsr
+{
+  afem_mesh* mesh = nullptr;
+  int nb_dof = 1;
+  bool per_block = true;             // value order
+  bool use_csr = false;
+  bool atomic_free = false;
+  bool has_sparsity = false;
+  int32_t nb_row = 0;
+  int64_t nb_col = 0;                // stored blocks
+  int32_t max_row = 0;
+  afem::DBuf<int32_t> row_index;     // nb_row + 1
+  afem::DBuf<int32_t> columns;       // nb_col
+  afem::DBuf<int32_t> nnz_per_row;   // nb_row
+  afem::DBuf<double> values;         // nb_col * nb_dof^2
+  afem::DBuf<int32_t> csr_row;       // scalar CSR (toCsr), built on demand
+  afem::DBuf<int32_t> csr_col;
+};
+
+struct afem_solver_ws;
+
+struct afem_system
+{
+  int32_t nb_dof = 0;
+  afem::DBuf<double> rhs, x, forced_value, elim_value;
+  afem::DBuf<uint8_t> forced_info, elim_info;
+  // matrix view (non-owning)
+  bool has_matrix = false;
+  int32_t nb_row = 0;
+  int64_t nnz = 0;
+  const int32_t* row_index = nullptr;
+  const int32_t* columns = nullptr;
+  double* values = nullptr;
+  int block = 1;                     // 3 when the view is a per-block BSR
+  afem_bsr* bsr = nullptr;
+  bool matrix_changed = true;        // invalidates derived data (FP32 copy, diagonal)
+  bool has_row_column = false;       // row-column eliminations present
+  afem::DBuf<double> saved_values;   // values before the row-column elimination
+  afem_solver_ws* ws = nullptr;
+};

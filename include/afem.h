@@ -1,0 +1,273 @@
+/*
+ * afem.h -- C ABI of the MI355X-native P1 assembly + PCG path.
+ *
+ * This is the drop-in boundary for ArcaneFEM's hot path: the BSR/CSR
+ * assembly entry points a physics module calls (femutils/BSRFormat.h:170-236)
+ * and the linear-system implementation an IDoFLinearSystemFactory hands back
+ * (femutils/IDoFLinearSystemFactory.h:42-43,
+ * femutils/internal/IDoFLinearSystemImpl.h:45-69).  Each entry point below
+ * cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns an afem_status (0 = AFEM_OK).  Errors never cross
+ *    the ABI as exceptions; afem_last_error() gives the message of the last
+ *    failing call on the calling thread.  The C++ adapter in
+ *    arcanefem_amd/csrc/femutils/ turns them back into exceptions, the
+ *    reference's convention (ARCANE_FATAL / ARCANE_THROW).
+ *  - Pointers documented [host] are host memory; [device] are device memory of
+ *    the current device (afem_malloc or any hipMalloc'ed buffer).
+ *  - All work is ordered on one library stream; calls return once the work
+ *    is enqueued unless documented as synchronous.  afem_synchronize() waits.
+ *  - Handles are not thread-safe (as DoFLinearSystem::matrixAddValue is not,
+ *    femutils/AlephDoFLinearSystem.cc:125-135).
+ *  - Local ids: node lid i has DoFs i*nb_dof + k (femutils/FemDoFsOnNodes.cc:101-112);
+ *    row arrays carry nb_row+1 offsets.
+ */
+#ifndef AFEM_H
+#define AFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFEM_API __attribute__((visibility("default")))
+
+typedef enum afem_status {
+  AFEM_OK = 0,
+  AFEM_ERR_ARGUMENT = 1,        /* ARCANE_THROW(ArgumentException, ...) */
+  AFEM_ERR_NOT_IMPLEMENTED = 2, /* ARCANE_THROW(NotImplementedException, ...) */
+  AFEM_ERR_STATE = 3,           /* call made out of order */
+  AFEM_ERR_NOT_FOUND = 4,       /* BSRMatrix::findValueIndex "Value not found" */
+  AFEM_ERR_MESH = 5,            /* invalid connectivity or a degenerate cell */
+  AFEM_ERR_CAPACITY = 6,        /* a row exceeds every kernel variant's capacity */
+  AFEM_ERR_NOT_CONVERGED = 7,   /* solver hit max_iter before the tolerance */
+  AFEM_ERR_DEVICE = 8,          /* no usable device */
+  AFEM_ERR_HIP = 9,             /* a HIP runtime call failed */
+  AFEM_ERR_OUT_OF_MEMORY = 10
+} afem_status;
+
+/* ---- runtime -------------------------------------------------------------- */
+
+AFEM_API const char* afem_last_error(void);
+AFEM_API int afem_version(void);
+/* Selects the device and creates the library stream (default priority). */
+AFEM_API int afem_init(int device);
+AFEM_API int afem_finalize(void);
+AFEM_API int afem_synchronize(void);
+
+/* Plain device-memory plumbing for callers without an allocator of their own. */
+AFEM_API int afem_malloc(void** dptr, size_t bytes);
+AFEM_API int afem_free(void* dptr);
+AFEM_API int afem_memcpy_htod(void* dst_device, const void* src_host, size_t bytes);
+AFEM_API int afem_memcpy_dtoh(void* dst_host, const void* src_device, size_t bytes);
+AFEM_API int afem_memcpy_dtod(void* dst_device, const void* src_device, size_t bytes);
+AFEM_API int afem_memset(void* dptr, int value, size_t bytes);
+
+/* Per-kernel timing with HIP events recorded on the library stream around each
+ * launch.  query returns the number of recorded launches of the named kernel
+ * and their summed duration in milliseconds (synchronous). */
+AFEM_API int afem_profile_enable(int on);
+AFEM_API int afem_profile_reset(void);
+AFEM_API int afem_profile_query(const char* kernel, int64_t* launches, double* total_ms);
+
+/* ---- synthetic meshes [host] ------------------------------------------------
+ * Structured unit square (dim 2: each quad split into 2 triangles along the
+ * (0,0)-(1,1) diagonal) or unit cube (dim 3: each hex split into 6 Kuhn tets
+ * sharing the (0,0,0)-(1,1,1) diagonal) with n cells per side.  Nodes are
+ * numbered lexicographically (x fastest); interior nodes are moved by
+ * jitter*h*U(-1,1) per coordinate from a counter-based generator keyed by seed.
+ * Cells are written with stride 4 (triangles: slot 3 = -1) and positive
+ * orientation.  Stand-in for the reference's mesh files (meshes/msh). */
+AFEM_API int afem_box_mesh_size(int dim, int n, int32_t* nb_node, int32_t* nb_cell);
+AFEM_API int afem_box_mesh(int dim, int n, double jitter, uint64_t seed,
+                           double* coords /* nb_node*3 */, int32_t* cells /* nb_cell*4 */);
+
+/* ---- mesh ---------------------------------------------------------------------
+ * Device copy of the node coordinates and the cell->node connectivity, plus
+ * the node->cell connectivity built on the device (sorted by cell id).  It
+ * stands in for Arcane's IMesh and UnstructuredMeshConnectivityView
+ * (cellNode / nodeCell) that BSRFormat::initialize receives
+ * (femutils/BSRFormat.cc:350-372). */
+typedef struct afem_mesh* afem_mesh_t;
+
+/* coords [host] nb_node*3, cells [host] nb_cell*4 (stride 4).  nodes_per_cell
+ * is 3 (dim 2) or 4 (dim 3).  Fails with AFEM_ERR_MESH on out-of-range node
+ * ids. */
+AFEM_API int afem_mesh_create(int dim, int32_t nb_node, const double* coords,
+                              int32_t nb_cell, int32_t nodes_per_cell, const int32_t* cells,
+                              afem_mesh_t* mesh);
+AFEM_API int afem_mesh_destroy(afem_mesh_t mesh);
+/* node->cell lists as (cell*4 + local slot) [host]: offsets nb_node+1, items 4*nb_cell max. */
+AFEM_API int afem_mesh_node_cells(afem_mesh_t mesh, int32_t* offsets, int32_t* items);
+
+/* ---- BSR matrix (femutils/BSRFormat.h:143-236, BSRMatrix :77-137) ----------- */
+typedef struct afem_bsr* afem_bsr_t;
+
+typedef struct afem_bsr_info {
+  int32_t nb_row;            /* block rows = nodes (BSRMatrix::nbRow) */
+  int32_t nb_block;          /* block size = DoFs per node (BSRMatrix::nbBlock) */
+  int64_t nb_col;            /* stored blocks = N + 2E for P1 (BSRMatrix::nbColumn) */
+  int64_t nb_non_zero_value; /* nb_col * nb_block^2 (BSRMatrix::nbNonZero) */
+  int32_t order_values_per_block; /* BSRMatrix::orderValuePerBlock */
+  int32_t max_row_length;    /* longest block row */
+} afem_bsr_info;
+
+/* BSRFormat::initialize(mesh, nb_dof, does_linear_system_use_csr, use_atomic_free)
+ * (femutils/BSRFormat.cc:350-372).  Values are ordered per row (scalar CSR
+ * order) when does_linear_system_use_csr, per block otherwise.  The
+ * atomic-free flag is accepted for API parity; the MI355X path always builds
+ * the pattern and assembles without floating-point atomics. */
+AFEM_API int afem_bsr_create(afem_mesh_t mesh, int nb_dof, int does_linear_system_use_csr,
+                             int use_atomic_free, afem_bsr_t* bsr);
+AFEM_API int afem_bsr_destroy(afem_bsr_t bsr);
+/* BSRFormat::computeSparsity (femutils/BSRFormat.cc:1011-1018): rows sorted
+ * ascending (the canonical form of the reference's column set). */
+AFEM_API int afem_bsr_compute_sparsity(afem_bsr_t bsr);
+AFEM_API int afem_bsr_get_info(afem_bsr_t bsr, afem_bsr_info* info);
+/* BSRFormat::resetMatrixValues (femutils/BSRFormat.cc:1032-1035). */
+AFEM_API int afem_bsr_reset_values(afem_bsr_t bsr);
+
+/* BSRFormat::assembleBilinearAtomic / assembleBilinearAtomicFree with the P1
+ * Laplace element: modules/poisson/ElementMatrix.h:41-63, 93-118, with a
+ * per-cell conductivity modules/fourier/ElementMatrix.h:38-95.
+ *   cell_lambda [device] nb_cell values or NULL (lambda = 1);
+ *   rhs [device] nb_row values or NULL: when given, the same kernel also
+ *   writes the constant-source right-hand side b_i = sum_T f*|T|/npc
+ *   (femutils/ArcaneFemFunctionsGpu.h:675-708), overwriting rhs.
+ * Requires nb_dof = 1.  Values are overwritten (no prior reset needed). */
+AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda,
+                                       double source_f, double* rhs);
+
+/* Same with the 12x12 P1 linear-elasticity element
+ * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3. */
+AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu);
+
+/* Device arrays of the matrix: row_index (nb_row+1), columns (nb_col),
+ * values (nb_non_zero_value), nnz_per_row (nb_row).  Owned by the handle. */
+AFEM_API int afem_bsr_device_arrays(afem_bsr_t bsr, const int32_t** row_index,
+                                    const int32_t** columns, double** values,
+                                    const int32_t** nnz_per_row);
+/* Copy to host (synchronous); any pointer may be NULL. */
+AFEM_API int afem_bsr_download(afem_bsr_t bsr, int32_t* row_index, int32_t* columns,
+                               double* values, int32_t* nnz_per_row);
+/* BSRMatrix::toCsr (femutils/BSRFormat.cc:110-172) done on the device: scalar
+ * CSR rows (nb_row*nb_block+1) and columns (nb_non_zero_value) owned by the
+ * handle; the values array is shared unchanged (per-row order required for
+ * nb_block > 1). */
+AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int32_t** csr_col,
+                             int32_t* nb_csr_row);
+
+/* ---- right-hand side ---------------------------------------------------------- */
+
+/* BoundaryConditions::applyConstantSourceToRhs, node-wise gather
+ * (femutils/ArcaneFemFunctionsGpu.h:675-708): rhs[n*nb_dof+dof_index] =
+ * sum over cells T of n of f*scale_T*|T|/npc.  cell_scale [device] or NULL. */
+AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* cell_scale,
+                                      int nb_dof, int dof_index, double* rhs);
+/* Elasticity body force, modules/elasticity/BodyForce.h:149-161, gather form:
+ * rhs[3n+c] += sum over tets T of n of f[c]*|T|/4.  f [host] 3 values. */
+AFEM_API int afem_rhs_body_force(afem_mesh_t mesh, const double* f, double* rhs);
+
+/* ---- linear system (IDoFLinearSystemImpl, femutils/internal/IDoFLinearSystemImpl.h:37-70) */
+typedef struct afem_system* afem_system_t;
+
+enum afem_dirichlet_method {
+  AFEM_DIRICHLET_PENALTY = 0,               /* femutils/ArcaneFemFunctionsGpu.cc:54-78 */
+  AFEM_DIRICHLET_ROW_ELIMINATION = 1,       /* :83-105 with ELIMINATE_ROW */
+  AFEM_DIRICHLET_ROW_COLUMN_ELIMINATION = 2 /* :83-105 with ELIMINATE_ROW_COLUMN */
+};
+
+/* IDoFLinearSystemFactory::createInstance(sd, dof_family, solver_name):
+ * nb_dof is the size of the DoF family.  The instance owns the RHS, solution,
+ * forced and elimination DoF arrays (femutils/DoFLinearSystemImplBase.cc:26-36),
+ * zero-initialised. */
+AFEM_API int afem_system_create(int32_t nb_dof, afem_system_t* sys);
+AFEM_API int afem_system_destroy(afem_system_t sys);
+/* IDoFLinearSystemImpl::setCSRValues: stores a non-owning view of device
+ * arrays that must stay valid until solve (femutils/DoFLinearSystem.h:318-325).
+ * row_index has nb_row+1 entries. */
+AFEM_API int afem_system_set_csr_values(afem_system_t sys, int32_t nb_row, int64_t nnz,
+                                        const int32_t* row_index, const int32_t* columns,
+                                        double* values);
+/* setCSRValues from a BSR handle (BSRFormat::toLinearSystem, femutils/BSRFormat.cc:377-395).
+ * With nb_block > 1 the solver keeps the block structure for its SpMV and
+ * block-Jacobi preconditioner. */
+AFEM_API int afem_system_set_bsr(afem_system_t sys, afem_bsr_t bsr);
+/* rhsVariable(), solutionVariable(), getForcedInfo/Value(), getEliminationInfo/Value()
+ * as device arrays of nb_dof entries. */
+AFEM_API int afem_system_rhs(afem_system_t sys, double** rhs);
+AFEM_API int afem_system_solution(afem_system_t sys, double** x);
+AFEM_API int afem_system_forced(afem_system_t sys, uint8_t** info, double** value);
+AFEM_API int afem_system_elimination(afem_system_t sys, uint8_t** info, double** value);
+/* BoundaryConditions::applyDirichletToLhsAndRhs for one node group and one DoF
+ * component (femutils/ArcaneFemFunctionsGpu.cc:115-147):
+ *   penalty: forced_info = 1, forced_value = penalty, rhs = penalty*value;
+ *   row / row-column elimination: elimination info and value.
+ * nodes [host] node local ids; DoF = node*nb_dof_per_node + dof_index. */
+AFEM_API int afem_system_apply_dirichlet(afem_system_t sys, int method, int32_t nb_node,
+                                         const int32_t* nodes, int nb_dof_per_node,
+                                         int dof_index, double value, double penalty);
+/* CsrDoFLinearSystemImpl::applyMatrixTransformation / applyRHSTransformation
+ * (femutils/CsrDoFLinearSystemImpl.cc:235-253), including the reference's
+ * row-column elimination rule that leaves column index 0 untouched
+ * (femutils/CsrDoFLinearSystemImpl.cc:111). */
+AFEM_API int afem_system_apply_matrix_transformation(afem_system_t sys);
+AFEM_API int afem_system_apply_rhs_transformation(afem_system_t sys);
+/* clearValues (femutils/DoFLinearSystemImplBase.cc:42-48): forced and
+ * elimination arrays reset and the CSR view dropped. */
+AFEM_API int afem_system_clear_values(afem_system_t sys);
+
+enum afem_solver_method {
+  AFEM_SOLVER_PCG_FP64 = 0,      /* FP64 PCG (the Jacobi-PCG of PETSc cg+jacobi) */
+  AFEM_SOLVER_PCG_MIXED = 1,     /* FP32 SpMV/preconditioner/vectors, FP64 dots and
+                                    FP64 reliable-update residual r = b - A x */
+  AFEM_SOLVER_CG_FP64 = 2        /* unpreconditioned FP64 CG */
+};
+enum afem_precond {
+  AFEM_PRECOND_JACOBI = 0,       /* diagonal (scalar) or 3x3 block-diagonal inverse */
+  AFEM_PRECOND_NONE = 1
+};
+
+typedef struct afem_solver_options {
+  int method;            /* afem_solver_method */
+  int preconditioner;    /* afem_precond */
+  double rtol;           /* stop when ||b - A x||_2 <= max(rtol*||b||_2, atol) */
+  double atol;
+  int max_iter;
+  int use_initial_guess; /* 1: start from the solution array; 0: x0 = 0 except
+                            eliminated rows, which start at their value */
+  int check_every;       /* iterations between host convergence polls (0 = 16) */
+  double reliable_delta; /* mixed: residual reduction factor between FP64
+                            reliable updates (0 = 0.1) */
+  int spmv_lanes;        /* lanes per row of the CSR SpMV (0 = autotuned choice) */
+} afem_solver_options;
+
+typedef struct afem_solve_result {
+  int iterations;
+  int converged;
+  int reliable_updates;
+  double relres;         /* FP64 true relative residual ||b - A x||_2 / ||b||_2 */
+  double solve_ms;       /* wall time of the solve on the library stream */
+} afem_solve_result;
+
+AFEM_API void afem_solver_default_options(afem_solver_options* opt);
+/* IDoFLinearSystemImpl::solve.  Synchronous.  Returns AFEM_ERR_NOT_CONVERGED
+ * (with the result filled) when max_iter is reached. */
+AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt,
+                               afem_solve_result* result);
+
+/* ---- sparse kernels exposed for testing and benchmarking ---------------------- */
+
+/* y = A x with the system's matrix (FP64). x, y [device] nb_dof values. */
+AFEM_API int afem_system_spmv(afem_system_t sys, const double* x, double* y);
+/* y = A32 x32 with the FP32 copy of the system's matrix (made on first use). */
+AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFEM_H */

@@ -1,0 +1,227 @@
+"""ctypes front-end of the CPU FP64 oracle (oracle/liboracle.so).
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this package, and only as the checker or as the
+timed CPU baseline.  The product library never calls it.
+
+Each wrapper mirrors one C function of afem_oracle.h, which cites the
+reference file:line it restates.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+ELIMINATE_NONE, ELIMINATE_ROW, ELIMINATE_ROW_COLUMN = 0, 1, 2
+LAPLACE, ELASTICITY = 0, 1
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _P(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        L.afo_area_tria3.restype = dbl
+        L.afo_area_tria3.argtypes = [vp, vp, vp]
+        L.afo_volume_tetra4.restype = dbl
+        L.afo_volume_tetra4.argtypes = [vp, vp, vp, vp]
+        L.afo_grad_tria3.argtypes = [vp, vp, vp]
+        L.afo_grad_tetra4.argtypes = [vp, vp, vp, vp]
+        L.afo_elem_laplace_tria3.argtypes = [vp, dbl, vp]
+        L.afo_elem_laplace_tetra4.argtypes = [vp, dbl, vp]
+        L.afo_elem_elasticity_tetra4.argtypes = [vp, dbl, dbl, vp]
+        L.afo_lame.argtypes = [dbl, dbl, vp, vp]
+        L.afo_node_cells.restype = ctypes.c_int
+        L.afo_node_cells.argtypes = [i32, i32, i32, vp, vp, vp]
+        L.afo_pattern.restype = i64
+        L.afo_pattern.argtypes = [i32, i32, i32, vp, vp, vp, i64]
+        L.afo_assemble.restype = ctypes.c_int
+        L.afo_assemble.argtypes = [ctypes.c_int, i32, vp, i32, i32, vp, vp, vp,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, dbl, dbl, vp]
+        L.afo_bsr_to_csr.argtypes = [i32, ctypes.c_int, vp, vp, vp, vp]
+        L.afo_rhs_constant_source.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl,
+                                              ctypes.c_int, ctypes.c_int, vp]
+        L.afo_rhs_body_force.argtypes = [vp, i32, vp, vp, vp]
+        L.afo_apply_transformations.restype = ctypes.c_int
+        L.afo_apply_transformations.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.afo_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.afo_pcg_jacobi.restype = ctypes.c_int
+        L.afo_pcg_jacobi.argtypes = [i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int, vp]
+        L.afo_count_differences.restype = i64
+        L.afo_count_differences.argtypes = [i64, vp, vp, dbl, dbl]
+        L.afo_wtime.restype = dbl
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def area_tria3(n0, n1, n2):
+    return lib().afo_area_tria3(_P(_f64(n0)), _P(_f64(n1)), _P(_f64(n2)))
+
+
+def volume_tetra4(n0, n1, n2, n3):
+    return lib().afo_volume_tetra4(_P(_f64(n0)), _P(_f64(n1)), _P(_f64(n2)), _P(_f64(n3)))
+
+
+def grad_tria3(X):
+    X = _f64(X)
+    dx, dy = np.zeros(3), np.zeros(3)
+    lib().afo_grad_tria3(_P(X), _P(dx), _P(dy))
+    return dx, dy
+
+
+def grad_tetra4(X):
+    X = _f64(X)
+    dx, dy, dz = np.zeros(4), np.zeros(4), np.zeros(4)
+    lib().afo_grad_tetra4(_P(X), _P(dx), _P(dy), _P(dz))
+    return dx, dy, dz
+
+
+def elem_laplace(X, lam=1.0):
+    X = _f64(X)
+    n = X.shape[0]
+    K = np.zeros((n, n))
+    if n == 3:
+        lib().afo_elem_laplace_tria3(_P(X), lam, _P(K))
+    else:
+        lib().afo_elem_laplace_tetra4(_P(X), lam, _P(K))
+    return K
+
+
+def elem_elasticity(X, lame_lambda, lame_mu):
+    X = _f64(X)
+    K = np.zeros((12, 12))
+    lib().afo_elem_elasticity_tetra4(_P(X), lame_lambda, lame_mu, _P(K))
+    return K
+
+
+def lame(E, nu):
+    lam, mu = np.zeros(1), np.zeros(1)
+    lib().afo_lame(E, nu, _P(lam), _P(mu))
+    return float(lam[0]), float(mu[0])
+
+
+def node_cells(nb_node, cells, npc):
+    cells = _i32(cells)
+    off = np.zeros(nb_node + 1, np.int32)
+    items = np.zeros(max(1, cells.shape[0] * npc), np.int32)
+    rc = lib().afo_node_cells(nb_node, cells.shape[0], npc, _P(cells), _P(off), _P(items))
+    if rc != 0:
+        raise ValueError("node id out of range")
+    return off, items[: off[-1]]
+
+
+def pattern(nb_node, cells, npc):
+    cells = _i32(cells)
+    row = np.zeros(nb_node + 1, np.int32)
+    cap = int(cells.shape[0]) * npc * npc + nb_node
+    cols = np.zeros(max(cap, 1), np.int32)
+    nnz = lib().afo_pattern(nb_node, cells.shape[0], npc, _P(cells), _P(row), _P(cols), cap)
+    if nnz < 0:
+        raise ValueError("pattern build failed")
+    return row, cols[:nnz].copy()
+
+
+def assemble(dim, coords, cells, npc, row, cols, element=LAPLACE, nb_dof=1, order_per_block=True,
+             cell_lambda=None, lame_lambda=0.0, lame_mu=0.0):
+    coords, cells, row, cols = _f64(coords), _i32(cells), _i32(row), _i32(cols)
+    lamv = None if cell_lambda is None else _f64(cell_lambda)
+    vals = np.zeros(int(row[-1]) * nb_dof * nb_dof)
+    rc = lib().afo_assemble(dim, coords.shape[0], _P(coords), cells.shape[0], npc, _P(cells),
+                            _P(row), _P(cols), element, nb_dof, 1 if order_per_block else 0,
+                            _P(lamv), lame_lambda, lame_mu, _P(vals))
+    if rc != 0:
+        raise ValueError("assembly: value not found in pattern")
+    return vals
+
+
+def bsr_to_csr(nb_node, nb, row, cols):
+    row, cols = _i32(row), _i32(cols)
+    csr_row = np.zeros(nb_node * nb + 1, np.int32)
+    csr_col = np.zeros(int(row[-1]) * nb * nb, np.int32)
+    lib().afo_bsr_to_csr(nb_node, nb, _P(row), _P(cols), _P(csr_row), _P(csr_col))
+    return csr_row, csr_col
+
+
+def rhs_constant_source(dim, coords, cells, npc, f, nb_dof=1, dof_index=0, cell_scale=None, rhs=None):
+    coords, cells = _f64(coords), _i32(cells)
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * nb_dof)
+    sc = None if cell_scale is None else _f64(cell_scale)
+    lib().afo_rhs_constant_source(dim, _P(coords), cells.shape[0], npc, _P(cells), _P(sc), f,
+                                  nb_dof, dof_index, _P(rhs))
+    return rhs
+
+
+def rhs_body_force(coords, cells, f, rhs=None):
+    coords, cells = _f64(coords), _i32(cells)
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * 3)
+    fv = _f64(f)
+    lib().afo_rhs_body_force(_P(coords), cells.shape[0], _P(cells), _P(fv), _P(rhs))
+    return rhs
+
+
+def apply_transformations(row, cols, vals, rhs, forced_info=None, forced_value=None,
+                          elim_info=None, elim_value=None):
+    """In place on vals and rhs (copies are made for non-contiguous inputs)."""
+    n = len(row) - 1
+    row, cols = _i32(row), _i32(cols)
+    fi = None if forced_info is None else np.ascontiguousarray(forced_info, np.uint8)
+    fv = None if forced_value is None else _f64(forced_value)
+    ei = None if elim_info is None else np.ascontiguousarray(elim_info, np.uint8)
+    ev = None if elim_value is None else _f64(elim_value)
+    rc = lib().afo_apply_transformations(n, _P(row), _P(cols), _P(vals), _P(rhs),
+                                         _P(fi), _P(fv), _P(ei), _P(ev))
+    if rc != 0:
+        raise ValueError("forced diagonal missing from the pattern")
+    return vals, rhs
+
+
+def spmv(row, cols, vals, x):
+    row, cols, vals, x = _i32(row), _i32(cols), _f64(vals), _f64(x)
+    y = np.zeros(len(row) - 1)
+    lib().afo_spmv(len(row) - 1, _P(row), _P(cols), _P(vals), _P(x), _P(y))
+    return y
+
+
+def pcg_jacobi(row, cols, vals, b, rtol=1e-10, max_iter=100000, x0=None):
+    row, cols, vals, b = _i32(row), _i32(cols), _f64(vals), _f64(b)
+    x = np.zeros(len(b)) if x0 is None else np.array(x0, dtype=np.float64)
+    rr = np.zeros(1)
+    it = lib().afo_pcg_jacobi(len(b), _P(row), _P(cols), _P(vals), _P(b), _P(x), rtol, max_iter, _P(rr))
+    return x, it, float(rr[0])
+
+
+def count_differences(ref, v, epsilon, min_value=0.0):
+    ref, v = _f64(ref).ravel(), _f64(v).ravel()
+    return int(lib().afo_count_differences(ref.size, _P(ref), _P(v), epsilon, min_value))
+
+
+def wtime():
+    return lib().afo_wtime()

@@ -1,0 +1,576 @@
+/*
+ * afem_oracle.c -- CPU FP64 oracle for the P1 assembly + PCG path.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT CODE.  See afem_oracle.h for the contract
+ * and the reference file:line of each function.
+ */
+#define _POSIX_C_SOURCE 199309L
+#include "afem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ---- small vector helpers (Arcane Real3 semantics) ------------------------ */
+
+static void sub3(const double* a, const double* b, double* r)
+{
+  r[0] = a[0] - b[0];
+  r[1] = a[1] - b[1];
+  r[2] = a[2] - b[2];
+}
+
+/* math::cross */
+static void cross3(const double* a, const double* b, double* r)
+{
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot3(const double* a, const double* b)
+{
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* ---- geometry -------------------------------------------------------------- */
+
+/* femutils/ArcaneFemFunctionsGpu.h:76-86 */
+double afo_area_tria3(const double* n0, const double* n1, const double* n2)
+{
+  double a[3], b[3], v[3];
+  sub3(n1, n0, a);
+  sub3(n2, n0, b);
+  cross3(a, b, v);
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / 2.0;
+}
+
+/* femutils/ArcaneFemFunctionsGpu.h:113-128 */
+double afo_volume_tetra4(const double* n0, const double* n1, const double* n2, const double* n3)
+{
+  double v0[3], v1[3], v2[3], c[3];
+  sub3(n1, n0, v0);
+  sub3(n2, n0, v1);
+  sub3(n3, n0, v2);
+  cross3(v1, v2, c);
+  return fabs(dot3(v0, c)) / 6.0;
+}
+
+/* femutils/ArcaneFemFunctionsGpu.h:241-281 */
+void afo_grad_tria3(const double* X, double dx[3], double dy[3])
+{
+  const double* n0 = X;
+  const double* n1 = X + 3;
+  const double* n2 = X + 6;
+  double A2 = ((n1[0] - n0[0]) * (n2[1] - n0[1]) - (n2[0] - n0[0]) * (n1[1] - n0[1]));
+  dx[0] = (n1[1] - n2[1]) / A2;
+  dx[1] = (n2[1] - n0[1]) / A2;
+  dx[2] = (n0[1] - n1[1]) / A2;
+  dy[0] = (n2[0] - n1[0]) / A2;
+  dy[1] = (n0[0] - n2[0]) / A2;
+  dy[2] = (n1[0] - n0[0]) / A2;
+}
+
+/* femutils/ArcaneFemFunctionsGpu.h:414-535 (x = 0, y = 1, z = 2) */
+void afo_grad_tetra4(const double* X, double dx[4], double dy[4], double dz[4])
+{
+  const double* n0 = X;
+  const double* n1 = X + 3;
+  const double* n2 = X + 6;
+  const double* n3 = X + 9;
+  double v0[3], v1[3], v2[3], c[3];
+  sub3(n1, n0, v0);
+  sub3(n2, n0, v1);
+  sub3(n3, n0, v2);
+  cross3(v1, v2, c);
+  double V6 = fabs(dot3(v0, c));
+  enum { X_ = 0, Y_ = 1, Z_ = 2 };
+  dx[0] = (n1[Y_] * (n3[Z_] - n2[Z_]) + n2[Y_] * (n1[Z_] - n3[Z_]) + n3[Y_] * (n2[Z_] - n1[Z_])) / V6;
+  dx[1] = (n0[Y_] * (n2[Z_] - n3[Z_]) + n2[Y_] * (n3[Z_] - n0[Z_]) + n3[Y_] * (n0[Z_] - n2[Z_])) / V6;
+  dx[2] = (n0[Y_] * (n3[Z_] - n1[Z_]) + n1[Y_] * (n0[Z_] - n3[Z_]) + n3[Y_] * (n1[Z_] - n0[Z_])) / V6;
+  dx[3] = (n0[Y_] * (n1[Z_] - n2[Z_]) + n1[Y_] * (n2[Z_] - n0[Z_]) + n2[Y_] * (n0[Z_] - n1[Z_])) / V6;
+
+  dy[0] = (n1[Z_] * (n3[X_] - n2[X_]) + n2[Z_] * (n1[X_] - n3[X_]) + n3[Z_] * (n2[X_] - n1[X_])) / V6;
+  dy[1] = (n0[Z_] * (n2[X_] - n3[X_]) + n2[Z_] * (n3[X_] - n0[X_]) + n3[Z_] * (n0[X_] - n2[X_])) / V6;
+  dy[2] = (n0[Z_] * (n3[X_] - n1[X_]) + n1[Z_] * (n0[X_] - n3[X_]) + n3[Z_] * (n1[X_] - n0[X_])) / V6;
+  dy[3] = (n0[Z_] * (n1[X_] - n2[X_]) + n1[Z_] * (n2[X_] - n0[X_]) + n2[Z_] * (n0[X_] - n1[X_])) / V6;
+
+  dz[0] = (n1[X_] * (n3[Y_] - n2[Y_]) + n2[X_] * (n1[Y_] - n3[Y_]) + n3[X_] * (n2[Y_] - n1[Y_])) / V6;
+  dz[1] = (n0[X_] * (n2[Y_] - n3[Y_]) + n2[X_] * (n3[Y_] - n0[Y_]) + n3[X_] * (n0[Y_] - n2[Y_])) / V6;
+  dz[2] = (n0[X_] * (n3[Y_] - n1[Y_]) + n1[X_] * (n0[Y_] - n3[Y_]) + n3[X_] * (n1[Y_] - n0[Y_])) / V6;
+  dz[3] = (n0[X_] * (n1[Y_] - n2[Y_]) + n1[X_] * (n2[Y_] - n0[Y_]) + n2[X_] * (n0[Y_] - n1[Y_])) / V6;
+}
+
+/* ---- element matrices ------------------------------------------------------- */
+
+/* modules/poisson/ElementMatrix.h:41-49; modules/fourier/ElementMatrix.h:38-47:
+ *   area * lambda * (dxU ^ dxU) + area * lambda * (dyU ^ dyU) */
+void afo_elem_laplace_tria3(const double* X, double lambda, double K[9])
+{
+  double area = afo_area_tria3(X, X + 3, X + 6);
+  double dx[3], dy[3];
+  afo_grad_tria3(X, dx, dy);
+  double s = area * lambda;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      K[i * 3 + j] = s * (dx[i] * dx[j]) + s * (dy[i] * dy[j]);
+}
+
+/* modules/poisson/ElementMatrix.h:93-102; modules/fourier/ElementMatrix.h:85-95 */
+void afo_elem_laplace_tetra4(const double* X, double lambda, double K[16])
+{
+  double vol = afo_volume_tetra4(X, X + 3, X + 6, X + 9);
+  double dx[4], dy[4], dz[4];
+  afo_grad_tetra4(X, dx, dy, dz);
+  double s = vol * lambda;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      K[i * 4 + j] = s * (dx[i] * dx[j]) + s * (dy[i] * dy[j]) + s * (dz[i] * dz[j]);
+}
+
+/* modules/elasticity/FemModule.cc:171-172 */
+void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
+{
+  *lame_mu = (E / (2 * (1 + nu)));
+  *lame_lambda = E * nu / ((1 + nu) * (1 - 2 * nu));
+}
+
+/* modules/elasticity/ElementMatrix.h:151-189.  The 12-vectors are the
+ * reference's dxUx, dyUx, ... ; '^' is the outer product
+ * (femutils/FemUtils.h:258-281) and each sum of matrices is evaluated left to
+ * right, then scaled by the Lame coefficient and by the volume. */
+void afo_elem_elasticity_tetra4(const double* X, double lam, double mu, double K[144])
+{
+  double dxu[4], dyu[4], dzu[4];
+  afo_grad_tetra4(X, dxu, dyu, dzu);
+  double volume = afo_volume_tetra4(X, X + 3, X + 6, X + 9);
+
+  double dxUx[12] = { 0 }, dyUx[12] = { 0 }, dzUx[12] = { 0 };
+  double dxUy[12] = { 0 }, dyUy[12] = { 0 }, dzUy[12] = { 0 };
+  double dxUz[12] = { 0 }, dyUz[12] = { 0 }, dzUz[12] = { 0 };
+  for (int a = 0; a < 4; ++a) {
+    dxUx[3 * a] = dxu[a];
+    dyUx[3 * a] = dyu[a];
+    dzUx[3 * a] = dzu[a];
+    dxUy[3 * a + 1] = dxu[a];
+    dyUy[3 * a + 1] = dyu[a];
+    dzUy[3 * a + 1] = dzu[a];
+    dxUz[3 * a + 2] = dxu[a];
+    dyUz[3 * a + 2] = dyu[a];
+    dzUz[3 * a + 2] = dzu[a];
+  }
+  double s1[12], s2[12], s3[12], t1[12], t2[12], t3[12];
+  for (int p = 0; p < 12; ++p) {
+    s1[p] = dxUy[p] + dyUx[p]; /* (dxUy + dyUx) */
+    t1[p] = dyUx[p] + dxUy[p]; /* (dyUx + dxUy) */
+    s2[p] = dzUy[p] + dyUz[p];
+    t2[p] = dyUz[p] + dzUy[p];
+    s3[p] = dxUz[p] + dzUx[p];
+    t3[p] = dzUx[p] + dxUz[p];
+  }
+  for (int p = 0; p < 12; ++p) {
+    for (int q = 0; q < 12; ++q) {
+      double normal = (lam + 2 * mu) * ((dxUx[p] * dxUx[q] + dyUy[p] * dyUy[q]) + dzUz[p] * dzUz[q]);
+      normal = normal * volume;
+      double comp = dyUy[p] * dxUx[q] + dxUx[p] * dyUy[q];
+      comp = comp + dzUz[p] * dxUx[q];
+      comp = comp + dxUx[p] * dzUz[q];
+      comp = comp + dyUy[p] * dzUz[q];
+      comp = comp + dzUz[p] * dyUy[q];
+      comp = (lam * comp) * volume;
+      double shear = (s1[p] * t1[q] + s2[p] * t2[q]) + s3[p] * t3[q];
+      shear = (mu * shear) * volume;
+      K[p * 12 + q] = (normal + comp) + shear;
+    }
+  }
+}
+
+/* ---- connectivity -------------------------------------------------------------- */
+
+int afo_node_cells(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                   int32_t* offsets, int32_t* items)
+{
+  memset(offsets, 0, sizeof(int32_t) * ((size_t)nb_node + 1));
+  for (int64_t c = 0; c < nb_cell; ++c)
+    for (int k = 0; k < npc; ++k) {
+      int32_t n = cells[c * 4 + k];
+      if (n < 0 || n >= nb_node)
+        return -1;
+      offsets[n + 1]++;
+    }
+  for (int32_t i = 0; i < nb_node; ++i)
+    offsets[i + 1] += offsets[i];
+  int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb_node > 0 ? nb_node : 1));
+  memcpy(fill, offsets, sizeof(int32_t) * (size_t)nb_node);
+  /* cells visited in increasing id order: each list comes out sorted by cell */
+  for (int64_t c = 0; c < nb_cell; ++c)
+    for (int k = 0; k < npc; ++k) {
+      int32_t n = cells[c * 4 + k];
+      items[fill[n]++] = (int32_t)(c * 4 + k);
+    }
+  free(fill);
+  return 0;
+}
+
+/* ---- sparsity ------------------------------------------------------------------- */
+
+static int cmp_i32(const void* a, const void* b)
+{
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+int64_t afo_pattern(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                    int32_t* row_index, int32_t* columns, int64_t capacity)
+{
+  int32_t* off = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nb_node + 1));
+  int32_t* items = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nb_cell * npc + 1));
+  if (afo_node_cells(nb_node, nb_cell, npc, cells, off, items) != 0) {
+    free(off);
+    free(items);
+    return -1;
+  }
+  int32_t maxdeg = 0;
+  for (int32_t i = 0; i < nb_node; ++i)
+    if (off[i + 1] - off[i] > maxdeg)
+      maxdeg = off[i + 1] - off[i];
+  int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * ((size_t)maxdeg * npc + 1));
+  int64_t nnz = 0;
+  int overflow = 0;
+  row_index[0] = 0;
+  for (int32_t i = 0; i < nb_node; ++i) {
+    int32_t m = 0;
+    for (int32_t e = off[i]; e < off[i + 1]; ++e) {
+      int32_t c = items[e] >> 2;
+      for (int k = 0; k < npc; ++k)
+        buf[m++] = cells[(int64_t)c * 4 + k];
+    }
+    if (m == 0)
+      buf[m++] = i; /* isolated node keeps its diagonal */
+    qsort(buf, (size_t)m, sizeof(int32_t), cmp_i32);
+    int32_t u = 0;
+    for (int32_t k = 0; k < m; ++k)
+      if (k == 0 || buf[k] != buf[k - 1])
+        buf[u++] = buf[k];
+    if (!overflow && nnz + u <= capacity)
+      memcpy(columns + nnz, buf, sizeof(int32_t) * (size_t)u);
+    else
+      overflow = 1;
+    nnz += u;
+    row_index[i + 1] = (int32_t)nnz;
+  }
+  free(buf);
+  free(off);
+  free(items);
+  return overflow ? -1 : nnz;
+}
+
+/* ---- assembly ------------------------------------------------------------------- */
+
+/* Linear search of femutils/BSRFormat.h:281-289: returns the position x of
+ * column col inside row r, or -1. */
+static int32_t find_in_row(const int32_t* row_index, const int32_t* columns, int32_t r, int32_t col)
+{
+  for (int32_t k = row_index[r]; k < row_index[r + 1]; ++k)
+    if (columns[k] == col)
+      return k - row_index[r];
+  return -1;
+}
+
+int afo_assemble(int dim, int32_t nb_node, const double* coords,
+                 int32_t nb_cell, int32_t npc, const int32_t* cells,
+                 const int32_t* row_index, const int32_t* columns,
+                 int element, int nb_dof, int order_per_block,
+                 const double* cell_lambda, double lame_lambda, double lame_mu,
+                 double* values)
+{
+  (void)dim;
+  const int nb = nb_dof;
+  const int nb2 = nb * nb;
+  const int64_t nnz = row_index[nb_node];
+  memset(values, 0, sizeof(double) * (size_t)(nnz * nb2));
+  double X[12];
+  double K[144];
+  const int ke = npc * nb; /* element matrix order */
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + c * 4;
+    for (int k = 0; k < npc; ++k)
+      memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+    double lambda = cell_lambda ? cell_lambda[c] : 1.0;
+    if (element == AFO_LAPLACE) {
+      if (npc == 3)
+        afo_elem_laplace_tria3(X, lambda, K);
+      else
+        afo_elem_laplace_tetra4(X, lambda, K);
+    }
+    else {
+      afo_elem_elasticity_tetra4(X, lame_lambda, lame_mu, K);
+    }
+    for (int a = 0; a < npc; ++a) {
+      int32_t r = cn[a];
+      for (int b = 0; b < npc; ++b) {
+        int32_t x = find_in_row(row_index, columns, r, cn[b]);
+        if (x < 0)
+          return -1;
+        int64_t blk = (int64_t)row_index[r] + x;
+        int32_t nzr = row_index[r + 1] - row_index[r];
+        for (int i = 0; i < nb; ++i)
+          for (int j = 0; j < nb; ++j) {
+            double v = K[(nb * a + i) * ke + nb * b + j];
+            int64_t idx;
+            if (order_per_block)
+              idx = blk * nb2 + (i * nb + j);
+            else
+              idx = (int64_t)row_index[r] * nb2 + nb * (x + (int64_t)i * nzr) + j;
+            values[idx] += v;
+          }
+      }
+    }
+  }
+  return 0;
+}
+
+/* femutils/BSRFormat.cc:110-172 */
+void afo_bsr_to_csr(int32_t nb_node, int nb, const int32_t* row_index, const int32_t* columns,
+                    int32_t* csr_row, int32_t* csr_col)
+{
+  csr_row[0] = 0;
+  int64_t off = 1;
+  for (int32_t i = 0; i < nb_node; ++i)
+    for (int j = 0; j < nb; ++j) {
+      int32_t len = (row_index[i + 1] - row_index[i]) * nb;
+      csr_row[off] = csr_row[off - 1] + len;
+      ++off;
+    }
+  int64_t o = 0;
+  for (int32_t i = 0; i < nb_node; ++i)
+    for (int j = 0; j < nb; ++j)
+      for (int32_t b = row_index[i]; b < row_index[i + 1]; ++b)
+        for (int k = 0; k < nb; ++k)
+          csr_col[o++] = columns[b] * nb + k;
+}
+
+/* ---- right-hand side ------------------------------------------------------------ */
+
+/* femutils/ArcaneFemFunctions.h:1601-1611 (tet) and :2531-2541 (tri) */
+void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int32_t npc,
+                             const int32_t* cells, const double* cell_scale,
+                             double f, int nb_dof, int dof_index, double* rhs)
+{
+  (void)dim;
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + c * 4;
+    double m;
+    if (npc == 3)
+      m = afo_area_tria3(coords + 3 * (int64_t)cn[0], coords + 3 * (int64_t)cn[1], coords + 3 * (int64_t)cn[2]);
+    else
+      m = afo_volume_tetra4(coords + 3 * (int64_t)cn[0], coords + 3 * (int64_t)cn[1],
+                            coords + 3 * (int64_t)cn[2], coords + 3 * (int64_t)cn[3]);
+    double q = cell_scale ? f * cell_scale[c] : f;
+    for (int k = 0; k < npc; ++k)
+      rhs[(int64_t)cn[k] * nb_dof + dof_index] += q * m / npc;
+  }
+}
+
+/* modules/elasticity/BodyForce.h:149-161 */
+void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* cells,
+                        const double f[3], double* rhs)
+{
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + c * 4;
+    double volume = afo_volume_tetra4(coords + 3 * (int64_t)cn[0], coords + 3 * (int64_t)cn[1],
+                                      coords + 3 * (int64_t)cn[2], coords + 3 * (int64_t)cn[3]);
+    for (int k = 0; k < 4; ++k) {
+      rhs[(int64_t)cn[k] * 3 + 0] += f[0] * volume / 4;
+      rhs[(int64_t)cn[k] * 3 + 1] += f[1] * volume / 4;
+      rhs[(int64_t)cn[k] * 3 + 2] += f[2] * volume / 4;
+    }
+  }
+}
+
+/* ---- transformations ------------------------------------------------------------ */
+
+int afo_apply_transformations(int32_t nb_row, const int32_t* row_index, const int32_t* columns,
+                              double* values, double* rhs,
+                              const uint8_t* forced_info, const double* forced_value,
+                              const uint8_t* elim_info, const double* elim_value)
+{
+  /* _fillRowColumnEliminationInfos (CsrDoFLinearSystemImpl.cc:187-230): the
+   * (row, col) values touching a row-column eliminated DoF, captured before the
+   * matrix is modified.  Kept in CSR order, which is the (row, col) order of
+   * the reference's ordered map when columns are sorted. */
+  int has_rc = 0;
+  if (elim_info)
+    for (int32_t i = 0; i < nb_row; ++i)
+      if (elim_info[i] == AFO_ELIMINATE_ROW_COLUMN) {
+        has_rc = 1;
+        break;
+      }
+  double* saved = NULL;
+  if (has_rc) {
+    int64_t nnz = row_index[nb_row];
+    saved = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    memcpy(saved, values, sizeof(double) * (size_t)nnz);
+  }
+  if (elim_info) {
+    /* _applyRowEliminationOnMatrix (:126-152) */
+    for (int32_t i = 0; i < nb_row; ++i)
+      if (elim_info[i] == AFO_ELIMINATE_ROW)
+        for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k)
+          values[k] = (columns[k] == i) ? 1.0 : 0.0;
+    /* _applyRowColumnEliminationOnMatrix (:88-121); note the reference's
+     * 'column_index > 0' test, which leaves column 0 untouched. */
+    if (has_rc)
+      for (int32_t i = 0; i < nb_row; ++i) {
+        int row_elim = elim_info[i] == AFO_ELIMINATE_ROW || elim_info[i] == AFO_ELIMINATE_ROW_COLUMN;
+        for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k) {
+          int32_t col = columns[k];
+          if (col > 0) {
+            int col_elim = elim_info[col] == AFO_ELIMINATE_ROW || elim_info[col] == AFO_ELIMINATE_ROW_COLUMN;
+            if (row_elim || col_elim)
+              values[k] = (col == i) ? 1.0 : 0.0;
+          }
+        }
+      }
+  }
+  /* _applyForcedValuesToLhs (:50-72) */
+  if (forced_info)
+    for (int32_t i = 0; i < nb_row; ++i)
+      if (forced_info[i]) {
+        int32_t x = find_in_row(row_index, columns, i, i);
+        if (x < 0) {
+          free(saved);
+          return -1;
+        }
+        values[row_index[i] + x] = forced_value[i];
+      }
+  /* applyRHSTransformation: _applyRowColumnEliminationToRHS
+   * (DoFLinearSystemImplBase.cc:55-88) then _applyRowOrRowColumnEliminationOnRHS
+   * (CsrDoFLinearSystemImpl.cc:157-182). */
+  if (has_rc)
+    for (int32_t i = 0; i < nb_row; ++i)
+      for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k) {
+        int32_t col = columns[k];
+        if (col < 0 || col == i)
+          continue;
+        if (!(elim_info[i] == AFO_ELIMINATE_ROW_COLUMN || elim_info[col] == AFO_ELIMINATE_ROW_COLUMN))
+          continue;
+        if (elim_info[i] == AFO_ELIMINATE_ROW_COLUMN)
+          rhs[col] = rhs[col] - saved[k] * elim_value[i];
+      }
+  if (elim_info)
+    for (int32_t i = 0; i < nb_row; ++i)
+      if (elim_info[i] == AFO_ELIMINATE_ROW || elim_info[i] == AFO_ELIMINATE_ROW_COLUMN)
+        rhs[i] = elim_value[i];
+  free(saved);
+  return 0;
+}
+
+/* ---- sparse kernels and PCG ------------------------------------------------------- */
+
+void afo_spmv(int32_t nb_row, const int32_t* row_index, const int32_t* columns,
+              const double* values, const double* x, double* y)
+{
+  for (int32_t i = 0; i < nb_row; ++i) {
+    double s = 0.0;
+    for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k)
+      s += values[k] * x[columns[k]];
+    y[i] = s;
+  }
+}
+
+static double dot_n(int32_t n, const double* a, const double* b)
+{
+  double s = 0.0;
+  for (int32_t i = 0; i < n; ++i)
+    s += a[i] * b[i];
+  return s;
+}
+
+int afo_pcg_jacobi(int32_t n, const int32_t* row_index, const int32_t* columns,
+                   const double* values, const double* b, double* x,
+                   double rtol, int max_iter, double* true_relres)
+{
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* z = (double*)malloc(sizeof(double) * (size_t)n);
+  double* p = (double*)malloc(sizeof(double) * (size_t)n);
+  double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dinv = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int32_t i = 0; i < n; ++i) {
+    double d = 0.0;
+    for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k)
+      if (columns[k] == i)
+        d = values[k];
+    dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
+  }
+  afo_spmv(n, row_index, columns, values, x, q);
+  for (int32_t i = 0; i < n; ++i) {
+    r[i] = b[i] - q[i];
+    z[i] = dinv[i] * r[i];
+    p[i] = z[i];
+  }
+  double bnorm = sqrt(dot_n(n, b, b));
+  double rho = dot_n(n, r, z);
+  int it = 0;
+  int converged = (bnorm == 0.0) || (sqrt(dot_n(n, r, r)) <= rtol * bnorm);
+  while (!converged && it < max_iter) {
+    afo_spmv(n, row_index, columns, values, p, q);
+    double alpha = rho / dot_n(n, p, q);
+    for (int32_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      z[i] = dinv[i] * r[i];
+    }
+    ++it;
+    double rho_new = dot_n(n, r, z);
+    double rnorm = sqrt(dot_n(n, r, r));
+    if (rnorm <= rtol * bnorm) {
+      converged = 1;
+      break;
+    }
+    double beta = rho_new / rho;
+    rho = rho_new;
+    for (int32_t i = 0; i < n; ++i)
+      p[i] = z[i] + beta * p[i];
+  }
+  afo_spmv(n, row_index, columns, values, x, q);
+  double rs = 0.0;
+  for (int32_t i = 0; i < n; ++i)
+    rs += (b[i] - q[i]) * (b[i] - q[i]);
+  if (true_relres)
+    *true_relres = bnorm > 0 ? sqrt(rs) / bnorm : sqrt(rs);
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  free(dinv);
+  return converged ? it : -1;
+}
+
+/* ---- comparison --------------------------------------------------------------- */
+
+int64_t afo_count_differences(int64_t n, const double* ref, const double* v,
+                              double epsilon, double min_value)
+{
+  int64_t nb = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double a = ref[i], b = v[i];
+    if (fabs(a) < min_value && fabs(b) < min_value)
+      continue;
+    double d = a - b;
+    if (d == 0.0)
+      continue;
+    double s = fabs(a) + fabs(b);
+    if (!(fabs(d) / s < epsilon))
+      ++nb;
+  }
+  return nb;
+}
+
+double afo_wtime(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}

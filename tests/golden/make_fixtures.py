@@ -1,0 +1,205 @@
+"""Build the golden fixtures under tests/golden/ from the reference's own data.
+
+Run in a checkout that has the arcanefem sources at /root/reference (or pass
+--reference).  It reads three binary Gmsh 4.1 meshes and the reference's
+golden solution files, and writes one .npz per case with:
+
+  coords      (N, 3) float64   node coordinates, nodes ordered by Gmsh tag
+  node_uid    (N,)   int64     Gmsh node tag = Arcane node uniqueId
+  cells       (C, 4) int32     cell -> local node ids (triangles: slot 3 = -1)
+  npc         ()     int32     nodes per cell (3 or 4)
+  dim         ()     int32
+  group_<name>(k,)   int32     local ids of the nodes of the boundary faces
+                               carrying the physical name <name>
+  ref_uid     (M,)   int64     uids listed in the golden file
+  ref_value   (M, d) float64   golden values (d = 1 or 3)
+
+Golden cases (test names and tolerances from the reference's CMake files):
+  circle_2D   modules/poisson/inputs/circle.2D.arc, check/poisson_test_ref_circle_2D.txt
+              (f=5.5, Dirichlet 0.5 on 'horizontal', eps 1e-4: FemModule.cc:508)
+  sphere_3D   modules/poisson/inputs/sphere.3D.arc, check/poisson_test_ref_sphere_3D.txt
+  bar_3D      modules/elasticity/inputs/bar.3D.Dirichlet.bodyForce.arc,
+              check/bar.3D.Dirichlet.bodyForce.txt (eps 1e-3, min 1e-10)
+
+The raw .msh files are copied beside the .npz files as data fixtures.
+"""
+
+import argparse
+import os
+import shutil
+import struct
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+CASES = {
+    "circle_2D": ("meshes/msh/circle_cut.msh", "modules/poisson/check/poisson_test_ref_circle_2D.txt", 1),
+    "sphere_3D": ("meshes/msh/sphere_cut.msh", "modules/poisson/check/poisson_test_ref_sphere_3D.txt", 1),
+    "bar_3D": ("meshes/msh/bar_dynamic_3D.msh", "modules/elasticity/check/bar.3D.Dirichlet.bodyForce.txt", 3),
+}
+
+# Gmsh element type -> number of nodes (only the P1 simplex family is needed)
+GMSH_NODES = {15: 1, 1: 2, 2: 3, 4: 4}
+
+
+class _Reader:
+    def __init__(self, data):
+        self.d = data
+        self.p = 0
+
+    def line(self):
+        e = self.d.index(b"\n", self.p)
+        s = self.d[self.p:e].decode("ascii").strip()
+        self.p = e + 1
+        return s
+
+    def take(self, fmt, count=1):
+        size = struct.calcsize("<" + fmt) * count
+        out = np.frombuffer(self.d, dtype=np.dtype("<" + fmt), count=count, offset=self.p)
+        self.p += size
+        return out
+
+    def i32(self):
+        return int(self.take("i")[0])
+
+    def u64(self):
+        return int(self.take("Q")[0])
+
+
+def read_msh41_binary(path):
+    """Binary Gmsh 4.1 reader (size_t = 8) for point/line/triangle/tet elements."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    names = {}
+    entity_phys = {}  # (dim, tag) -> list of physical tags
+    nodes_tag = []
+    nodes_xyz = []
+    blocks = []  # (dim, entity_tag, etype, tags, conn)
+    while r.p < len(r.d):
+        head = r.line()
+        if head == "":
+            continue
+        if head == "$MeshFormat":
+            ver, ftype, dsize = r.line().split()
+            assert ver == "4.1" and ftype == "1" and dsize == "8", (ver, ftype, dsize)
+            one = r.i32()
+            assert one == 1, "endianness"
+            r.p += 1  # newline after the int
+            assert r.line() == "$EndMeshFormat"
+        elif head == "$PhysicalNames":
+            n = int(r.line())
+            for _ in range(n):
+                parts = r.line().split(maxsplit=2)
+                names[(int(parts[0]), int(parts[1]))] = parts[2].strip('"')
+            assert r.line() == "$EndPhysicalNames"
+        elif head == "$Entities":
+            counts = [r.u64() for _ in range(4)]
+            for dim, cnt in enumerate(counts):
+                for _ in range(cnt):
+                    tag = r.i32()
+                    if dim == 0:
+                        r.take("d", 3)
+                    else:
+                        r.take("d", 6)
+                    nphys = r.u64()
+                    phys = [int(v) for v in r.take("i", nphys)] if nphys else []
+                    entity_phys[(dim, tag)] = phys
+                    if dim > 0:
+                        nb = r.u64()
+                        if nb:
+                            r.take("i", nb)
+            r.line()
+            assert r.line() == "$EndEntities"
+        elif head == "$Nodes":
+            nblocks, nnodes, _, _ = (r.u64() for _ in range(4))
+            for _ in range(nblocks):
+                _edim = r.i32()
+                _etag = r.i32()
+                param = r.i32()
+                assert param == 0, "parametric nodes unsupported"
+                n = r.u64()
+                tags = r.take("Q", n).astype(np.int64)
+                xyz = r.take("d", 3 * n).reshape(n, 3)
+                nodes_tag.append(tags)
+                nodes_xyz.append(xyz)
+            r.line()
+            assert r.line() == "$EndNodes"
+            assert sum(len(t) for t in nodes_tag) == nnodes
+        elif head == "$Elements":
+            nblocks, _, _, _ = (r.u64() for _ in range(4))
+            for _ in range(nblocks):
+                edim = r.i32()
+                etag = r.i32()
+                etype = r.i32()
+                n = r.u64()
+                k = GMSH_NODES[etype]
+                raw = r.take("Q", n * (1 + k)).reshape(n, 1 + k).astype(np.int64)
+                blocks.append((edim, etag, etype, raw[:, 0], raw[:, 1:]))
+            r.line()
+            assert r.line() == "$EndElements"
+        else:
+            # skip an unknown section
+            end = "$End" + head[1:]
+            while r.line() != end:
+                pass
+    tags = np.concatenate(nodes_tag)
+    xyz = np.concatenate(nodes_xyz)
+    order = np.argsort(tags, kind="stable")
+    tags = tags[order]
+    xyz = xyz[order]
+    return names, entity_phys, tags, xyz, blocks
+
+
+def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
+    mesh_path = os.path.join(ref_root, mesh_rel)
+    names, entity_phys, tags, xyz, blocks = read_msh41_binary(mesh_path)
+    tag_to_lid = {int(t): i for i, t in enumerate(tags)}
+    dim = max(b[0] for b in blocks)
+    cells = []
+    for edim, _etag, _etype, _etags, conn in blocks:
+        if edim == dim:
+            cells.append(conn)
+    cells = np.concatenate(cells)
+    npc = cells.shape[1]
+    lid = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(cells)
+    cells4 = -np.ones((lid.shape[0], 4), dtype=np.int32)
+    cells4[:, :npc] = lid
+    groups = {}
+    for edim, etag, _etype, _etags, conn in blocks:
+        if edim != dim - 1:
+            continue
+        for ptag in entity_phys.get((edim, etag), []):
+            gname = names.get((edim, ptag))
+            if gname is None:
+                continue
+            ids = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(conn.ravel())
+            groups.setdefault(gname, []).append(ids)
+    payload = {
+        "coords": xyz.astype(np.float64),
+        "node_uid": tags.astype(np.int64),
+        "cells": cells4,
+        "npc": np.int32(npc),
+        "dim": np.int32(dim),
+    }
+    for gname, lst in groups.items():
+        payload["group_" + gname] = np.unique(np.concatenate(lst)).astype(np.int32)
+    ref = np.loadtxt(os.path.join(ref_root, check_rel), ndmin=2)
+    payload["ref_uid"] = ref[:, 0].astype(np.int64)
+    payload["ref_value"] = ref[:, 1:1 + ncomp].astype(np.float64)
+    np.savez(os.path.join(out_dir, name + ".npz"), **payload)
+    shutil.copyfile(mesh_path, os.path.join(out_dir, os.path.basename(mesh_rel)))
+    print(f"{name}: N={len(tags)} cells={cells4.shape[0]} npc={npc} groups={sorted(groups)}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--out", default=HERE)
+    args = ap.parse_args()
+    for name, (mesh_rel, check_rel, ncomp) in CASES.items():
+        build_case(args.reference, name, mesh_rel, check_rel, ncomp, args.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+"""CPU-only checks of the product library: it loads, exports every entry point
+declared in include/afem.h, and its host-side pieces (synthetic meshes,
+argument validation) behave.  No GPU is touched here."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+import arcanefem_amd as A
+import oracle as O
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "afem.h")).read()
+    return sorted(set(re.findall(r"AFEM_API\s+[\w\s\*]+?\b(afem_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 40
+    L = A.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_no_device_is_reported_not_faked():
+    # In this container there is no GPU: init must fail with a clear status.
+    import ctypes
+    try:
+        import subprocess
+        out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=30)
+        has_gpu = "gfx950" in out.stdout
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    rc = A.lib().afem_init(0)
+    assert rc == A.ERR_DEVICE
+    assert b"device" in A.lib().afem_last_error()
+    p = ctypes.c_void_p()
+    assert A.lib().afem_malloc(ctypes.byref(p), 16) == A.ERR_DEVICE
+
+
+@pytest.mark.parametrize("dim,n,nn,nc", [(2, 256, 66049, 131072), (3, 128, 2146689, 12582912),
+                                         (3, 96, 912673, 5308416), (3, 64, 274625, 1572864)])
+def test_box_mesh_sizes_match_configs(dim, n, nn, nc):
+    import ctypes
+    a, b = ctypes.c_int32(), ctypes.c_int32()
+    A.check(A.lib().afem_box_mesh_size(dim, n, ctypes.byref(a), ctypes.byref(b)))
+    assert (a.value, b.value) == (nn, nc)
+
+
+@pytest.mark.parametrize("dim,n,alpha", [(2, 32, 0.2), (3, 12, 0.15)])
+def test_box_mesh_geometry(dim, n, alpha):
+    X, C = A.box_mesh(dim, n, alpha, seed=7)
+    h = 1.0 / n
+    idx = np.stack(np.meshgrid(*[np.arange(n + 1)] * dim, indexing="ij"), -1).reshape(-1, dim)[:, ::-1]
+    grid = idx * h  # lexicographic, x fastest
+    disp = X[:, :dim] - grid
+    assert np.abs(disp).max() <= alpha * h * (1 + 1e-12)
+    boundary = np.any((idx == 0) | (idx == n), axis=1)
+    assert np.all(disp[boundary] == 0.0)
+    assert np.abs(disp[~boundary]).max() > 0.5 * alpha * h
+    if dim == 2:
+        P = X[C[:, :3]]
+        det = (P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) - (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1])
+        assert (C[:, 3] == -1).all()
+        assert det.min() > 0 and det.sum() / 2 == pytest.approx(1.0, rel=1e-12)
+    else:
+        P = X[C]
+        det = np.linalg.det(P[:, 1:] - P[:, :1])
+        assert det.min() > 0 and det.sum() / 6 == pytest.approx(1.0, rel=1e-12)
+    # conforming: every interior face is shared by exactly two cells
+    npc = dim + 1
+    faces = np.sort(np.concatenate([np.delete(C[:, :npc], k, axis=1) for k in range(npc)]), axis=1)
+    _, counts = np.unique(faces, axis=0, return_counts=True)
+    assert counts.max() == 2
+
+
+def test_box_mesh_seeded():
+    X1, _ = A.box_mesh(3, 5, 0.15, seed=1)
+    X2, _ = A.box_mesh(3, 5, 0.15, seed=1)
+    X3, _ = A.box_mesh(3, 5, 0.15, seed=2)
+    assert np.array_equal(X1, X2) and not np.array_equal(X1, X3)
+
+
+def test_box_mesh_argument_errors():
+    with pytest.raises(A.AfemError) as e:
+        A.box_mesh(4, 3)
+    assert e.value.status == A.ERR_ARGUMENT
+    with pytest.raises(A.AfemError):
+        A.box_mesh(3, 3, jitter=0.7)
+
+
+def test_box_mesh_pattern_counts_via_oracle():
+    n = 5
+    X, C = A.box_mesh(3, n, 0.15, seed=0)
+    row, _ = O.pattern(X.shape[0], C, 4)
+    E = 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+    assert row[-1] == X.shape[0] + 2 * E

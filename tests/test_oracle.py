@@ -1,0 +1,471 @@
+"""Pins the CPU oracle (oracle/) before any GPU result is compared with it.
+
+Every oracle function is checked against something other than its own
+formula: the reference's golden solution files (tests/golden, produced by
+tests/golden/make_fixtures.py from the reference's meshes and check files),
+closed forms, invariants, textbook identities and brute force on tiny
+inputs.  Each check is chosen so that a plausible mistake (a dropped term, a
+wrong sign or index, a transposed operand) makes it fail.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import oracle as O
+
+from conftest import GOLDEN
+
+RNG = np.random.default_rng(1234)
+
+
+def unit_tri():
+    return np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+
+
+def unit_tet():
+    return np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+
+
+def random_rotation():
+    q, _ = np.linalg.qr(RNG.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def random_tet():
+    while True:
+        X = RNG.normal(size=(4, 3))
+        if abs(np.linalg.det(X[1:] - X[0])) > 0.2:
+            return X
+
+
+def random_tri():
+    while True:
+        X = np.zeros((3, 3))
+        X[:, :2] = RNG.normal(size=(3, 2))
+        if abs(np.linalg.det(X[1:, :2] - X[0, :2])) > 0.2:
+            return X
+
+
+# ---- geometry and gradients ------------------------------------------------------------
+
+def test_measures_known_answers():
+    assert O.area_tria3(*unit_tri()) == pytest.approx(0.5, abs=0)
+    assert O.volume_tetra4(*unit_tet()) == pytest.approx(1.0 / 6.0, rel=1e-15)
+    # scaling by s multiplies the measure by s^d; orientation does not matter
+    X = random_tet()
+    v = O.volume_tetra4(*X)
+    assert O.volume_tetra4(*(2.5 * X)) == pytest.approx(2.5 ** 3 * v, rel=1e-13)
+    assert O.volume_tetra4(X[1], X[0], X[2], X[3]) == pytest.approx(v, rel=1e-14)
+    assert v == pytest.approx(abs(np.linalg.det(X[1:] - X[0])) / 6, rel=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradients_reproduce_linear_fields(dim):
+    X = random_tri() if dim == 2 else random_tet()
+    if dim == 2:
+        g = np.array(O.grad_tria3(X))
+    else:
+        g = np.array(O.grad_tetra4(X))
+    a = RNG.normal(size=dim)
+    f = X[:, :dim] @ a + 0.7  # linear field sampled at the nodes
+    # sum_i f_i grad N_i = grad f.  The tet formulas divide by |6V|
+    # (femutils/ArcaneFemFunctionsGpu.h:437), so a negatively oriented tet
+    # gets sign-flipped gradients; the triangle ones use the signed 2A.
+    sign = 1.0
+    if dim == 3:
+        sign = np.sign(np.linalg.det(X[1:] - X[0]))
+        assert np.allclose(g @ f, sign * a, rtol=0, atol=1e-12)
+        Xp = X[[1, 0, 2, 3]] if sign < 0 else X  # positive orientation: exact gradient
+        assert np.allclose(np.array(O.grad_tetra4(Xp)) @ (Xp[:, :3] @ a), a, rtol=0, atol=1e-12)
+    else:
+        assert np.allclose(g @ f, a, rtol=0, atol=1e-12)
+    assert np.allclose(g.sum(axis=1), 0.0, atol=1e-13)  # partition of unity
+
+
+# ---- element matrices ---------------------------------------------------------------------
+
+def test_laplace_unit_element_known_answers():
+    Kt = O.elem_laplace(unit_tri())
+    assert np.allclose(Kt, 0.5 * np.array([[2, -1, -1], [-1, 1, 0], [-1, 0, 1]]), atol=1e-15)
+    K4 = O.elem_laplace(unit_tet())
+    ref = np.array([[3, -1, -1, -1], [-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]]) / 6.0
+    assert np.allclose(K4, ref, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_laplace_element_invariants(dim):
+    X = random_tri() if dim == 2 else random_tet()
+    lam = 3.7
+    K = O.elem_laplace(X, lam)
+    n = dim + 1
+    assert np.allclose(K, K.T, atol=1e-13)
+    assert np.allclose(K @ np.ones(n), 0.0, atol=1e-12)
+    assert np.linalg.eigvalsh(K).min() > -1e-12
+    # energy identity: u^T K u = lam |T| |grad u|^2 for linear u
+    a = RNG.normal(size=dim)
+    u = X[:, :dim] @ a
+    meas = O.area_tria3(*X) if dim == 2 else O.volume_tetra4(*X)
+    assert u @ K @ u == pytest.approx(lam * meas * a @ a, rel=1e-12)
+    if dim == 3:
+        R = random_rotation()
+        assert np.allclose(O.elem_laplace(X @ R.T + 5.0, lam), K, rtol=0, atol=1e-12)
+
+
+def rigid_modes(X):
+    modes = []
+    for d in range(3):
+        t = np.zeros((4, 3))
+        t[:, d] = 1
+        modes.append(t.ravel())
+    for w in np.eye(3):
+        modes.append(np.cross(w, X).ravel())
+    return np.array(modes).T
+
+
+def test_lame_parameters():
+    lam, mu = O.lame(21e5, 0.28)
+    assert mu == pytest.approx(21e5 / (2 * 1.28), rel=1e-15)
+    assert lam == pytest.approx(21e5 * 0.28 / (1.28 * 0.44), rel=1e-15)
+
+
+def test_elasticity_element_invariants():
+    X = random_tet()
+    lam, mu = O.lame(21e5, 0.28)
+    K = O.elem_elasticity(X, lam, mu)
+    assert np.allclose(K, K.T, rtol=0, atol=1e-9 * abs(K).max())
+    # the 6 rigid-body modes span the kernel, nothing else does
+    assert np.abs(K @ rigid_modes(X)).max() < 1e-9 * abs(K).max()
+    ev = np.linalg.eigvalsh(K)
+    assert (np.abs(ev) < 1e-9 * ev.max()).sum() == 6
+    assert ev.min() > -1e-9 * ev.max()
+    # energy identity for a linear displacement u(x) = B x:
+    # u^T K u = V (lambda tr(eps)^2 + 2 mu eps:eps), eps = sym(B)
+    B = RNG.normal(size=(3, 3))
+    u = (X @ B.T).ravel()
+    eps = 0.5 * (B + B.T)
+    V = O.volume_tetra4(*X)
+    assert u @ K @ u == pytest.approx(V * (lam * np.trace(eps) ** 2 + 2 * mu * np.sum(eps * eps)), rel=1e-11)
+
+
+def test_elasticity_element_matches_block_closed_form():
+    """K_ab = V [lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I]: the textbook P1 form."""
+    X = random_tet()
+    lam, mu = 1.3, 0.7
+    K = O.elem_elasticity(X, lam, mu)
+    g = np.array(O.grad_tetra4(X)).T  # 4 x 3
+    V = O.volume_tetra4(*X)
+    for a in range(4):
+        for b in range(4):
+            blk = V * (lam * np.outer(g[a], g[b]) + mu * np.outer(g[b], g[a]) + mu * (g[a] @ g[b]) * np.eye(3))
+            assert np.allclose(K[3 * a:3 * a + 3, 3 * b:3 * b + 3], blk, rtol=1e-12, atol=1e-12 * abs(K).max())
+
+
+# ---- connectivity and pattern -------------------------------------------------------------
+
+def box(dim, n, jitter=0.2, seed=3):
+    import arcanefem_amd as A  # host-only functions of the product library
+    return A.box_mesh(dim, n, jitter, seed)
+
+
+def brute_pattern(N, cells, npc):
+    M = np.zeros((N, N), bool)
+    for c in cells:
+        for a in c[:npc]:
+            for b in c[:npc]:
+                M[a, b] = True
+    return M
+
+
+@pytest.mark.parametrize("dim,n", [(2, 5), (3, 3)])
+def test_node_cells_and_pattern_bruteforce(dim, n):
+    X, C = box(dim, n)
+    npc = dim + 1
+    N = X.shape[0]
+    off, items = O.node_cells(N, C, npc)
+    for i in range(N):
+        lst = items[off[i]:off[i + 1]]
+        assert list(lst) == sorted(lst)
+        for it in lst:
+            assert C[it >> 2][it & 3] == i
+    assert off[-1] == C.shape[0] * npc
+    row, col = O.pattern(N, C, npc)
+    M = brute_pattern(N, C, npc)
+    for i in range(N):
+        assert list(col[row[i]:row[i + 1]]) == list(np.nonzero(M[i])[0])
+
+
+@pytest.mark.parametrize("dim,n", [(2, 16), (3, 6)])
+def test_pattern_closed_form_nnz(dim, n):
+    X, C = box(dim, n)
+    N = X.shape[0]
+    row, col = O.pattern(N, C, dim + 1)
+    if dim == 2:
+        E = 2 * n * (n + 1) + n * n
+        interior_len = 7
+    else:
+        E = 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+        interior_len = 15
+    assert row[-1] == N + 2 * E
+    lens = np.diff(row)
+    assert lens.max() == interior_len
+
+
+# ---- assembly -----------------------------------------------------------------------------
+
+def dense_assembly(X, C, npc, elem):
+    N = X.shape[0]
+    nb = elem(X[C[0][:npc]]).shape[0] // npc
+    A = np.zeros((N * nb, N * nb))
+    for c in C:
+        ids = c[:npc]
+        K = elem(X[ids])
+        dofs = (ids[:, None] * nb + np.arange(nb)).ravel()
+        A[np.ix_(dofs, dofs)] += K
+    return A
+
+
+def csr_to_dense(row, col, val, n):
+    return sp.csr_matrix((val, col, row), shape=(n, n)).toarray()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 4), (3, 3)])
+def test_laplace_assembly_bruteforce_and_identities(dim, n):
+    X, C = box(dim, n)
+    npc = dim + 1
+    N = X.shape[0]
+    lam = np.exp(RNG.normal(size=C.shape[0]))
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col, cell_lambda=lam)
+    A = csr_to_dense(row, col, val, N)
+    Ad = np.zeros((N, N))
+    for k, c in enumerate(C):
+        ids = c[:npc]
+        Ad[np.ix_(ids, ids)] += O.elem_laplace(X[ids], lam[k])
+    assert np.allclose(A, Ad, rtol=0, atol=1e-14 * abs(Ad).max())
+    # constant lambda: 1^T A = 0 and u^T A u = |a|^2 |Omega| for linear u
+    val1 = O.assemble(dim, X, C, npc, row, col)
+    A1 = csr_to_dense(row, col, val1, N)
+    assert np.abs(A1 @ np.ones(N)).max() < 1e-12
+    a = RNG.normal(size=dim)
+    u = X[:, :dim] @ a
+    assert u @ A1 @ u == pytest.approx(a @ a, rel=1e-11)
+
+
+def test_elasticity_assembly_orders_agree_with_dense():
+    X, C = box(3, 2)
+    N = X.shape[0]
+    lam, mu = O.lame(21e5, 0.28)
+    row, col = O.pattern(N, C, 4)
+    Ad = dense_assembly(X, C, 4, lambda Y: O.elem_elasticity(Y, lam, mu))
+    vb = O.assemble(3, X, C, 4, row, col, element=O.ELASTICITY, nb_dof=3, order_per_block=True,
+                    lame_lambda=lam, lame_mu=mu)
+    Ab = sp.bsr_matrix((vb.reshape(-1, 3, 3), col, row), shape=(3 * N, 3 * N)).toarray()
+    assert np.allclose(Ab, Ad, rtol=0, atol=1e-12 * abs(Ad).max())
+    vr = O.assemble(3, X, C, 4, row, col, element=O.ELASTICITY, nb_dof=3, order_per_block=False,
+                    lame_lambda=lam, lame_mu=mu)
+    crow, ccol = O.bsr_to_csr(N, 3, row, col)
+    Ar = csr_to_dense(crow, ccol, vr, 3 * N)
+    assert np.allclose(Ar, Ad, rtol=0, atol=1e-12 * abs(Ad).max())
+    # toCsr expansion equals scipy's BSR -> CSR structure
+    ref = sp.bsr_matrix((np.ones((len(col), 3, 3)), col, row), shape=(3 * N, 3 * N)).tocsr()
+    ref.sort_indices()
+    assert np.array_equal(crow, ref.indptr)
+    assert np.array_equal(np.sort(ccol[crow[5]:crow[6]]), ref.indices[ref.indptr[5]:ref.indptr[6]])
+
+
+def test_rhs_sources_integrate_to_domain_measure():
+    for dim, n in [(2, 6), (3, 4)]:
+        X, C = box(dim, n)
+        b = O.rhs_constant_source(dim, X, C, dim + 1, 2.5)
+        assert b.sum() == pytest.approx(2.5, rel=1e-13)  # f |Omega|, |Omega| = 1
+        assert (b > 0).all()
+    X, C = box(3, 3)
+    bf = O.rhs_body_force(X, C, [1.0, -2.0, 0.5]).reshape(-1, 3)
+    assert np.allclose(bf.sum(axis=0), [1.0, -2.0, 0.5], rtol=1e-13)
+
+
+# ---- linear-system transformations --------------------------------------------------------
+
+def small_system(n=7, seed=0):
+    r = np.random.default_rng(seed)
+    M = r.normal(size=(n, n))
+    A = M @ M.T + n * np.eye(n)
+    A[np.abs(A) < 0.8] = 0.0
+    A = 0.5 * (A + A.T)
+    np.fill_diagonal(A, np.abs(np.diag(A)) + n)
+    S = sp.csr_matrix(A)
+    S.sort_indices()
+    return S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy(), A, r.normal(size=n)
+
+
+def test_penalty_and_row_elimination_definitions():
+    row, col, val, A, b = small_system()
+    n = len(b)
+    fi = np.zeros(n, np.uint8)
+    fv = np.zeros(n)
+    fi[2], fv[2] = 1, 1e30
+    ei = np.zeros(n, np.uint8)
+    ev = np.zeros(n)
+    ei[4], ev[4] = O.ELIMINATE_ROW, 0.25
+    v2, b2 = val.copy(), b.copy()
+    O.apply_transformations(row, col, v2, b2, fi, fv, ei, ev)
+    A2 = csr_to_dense(row, col, v2, n)
+    expect = A.copy()
+    expect[4, :] = 0.0
+    expect[4, 4] = 1.0
+    expect[2, 2] = 1e30
+    mask = A != 0
+    mask[4, 4] = True
+    assert np.array_equal(A2[mask], expect[mask])
+    assert b2[4] == 0.25 and np.array_equal(np.delete(b2, 4), np.delete(b, 4))
+
+
+def test_row_column_elimination_definition_and_column0_rule():
+    row, col, val, A, b = small_system(seed=3)
+    n = len(b)
+    ei = np.zeros(n, np.uint8)
+    ev = np.zeros(n)
+    for i, g in [(1, 0.5), (5, -2.0)]:
+        ei[i], ev[i] = O.ELIMINATE_ROW_COLUMN, g
+    v2, b2 = val.copy(), b.copy()
+    O.apply_transformations(row, col, v2, b2, None, None, ei, ev)
+    A2 = csr_to_dense(row, col, v2, n)
+    elim = [1, 5]
+    expect = A.copy()
+    bexp = b.copy()
+    for i in elim:
+        bexp -= A[i, :] * ev[i] * (np.arange(n) != i)  # lift, uses row i values
+    for i in elim:
+        expect[i, :] = 0.0
+        expect[:, i] = 0.0
+        expect[i, i] = 1.0
+        bexp[i] = ev[i]
+    expect[:, 0] = A[:, 0]  # column index 0 is never cleared (CsrDoFLinearSystemImpl.cc:111)
+    nz = A != 0
+    for i in elim:
+        nz[i, i] = True
+    assert np.allclose(A2[nz], expect[nz], rtol=0, atol=0)
+    assert np.allclose(b2, bexp, rtol=1e-14, atol=1e-14)
+    # the reference never clears column index 0 in this step
+    ei0 = np.zeros(n, np.uint8)
+    ev0 = np.zeros(n)
+    ei0[0] = O.ELIMINATE_ROW_COLUMN
+    v3, b3 = val.copy(), b.copy()
+    O.apply_transformations(row, col, v3, b3, None, None, ei0, ev0)
+    A3 = csr_to_dense(row, col, v3, n)
+    assert np.array_equal(A3[1:, 0], A[1:, 0])
+    # ... not even the diagonal of an eliminated DoF 0, which keeps A[0, 0]
+    assert A3[0, 0] == A[0, 0] and np.all(A3[0, 1:] == 0)
+
+
+# ---- sparse kernels and the solve -----------------------------------------------------------
+
+def test_spmv_and_pcg_against_scipy():
+    X, C = box(3, 6)
+    N = X.shape[0]
+    row, col = O.pattern(N, C, 4)
+    val = O.assemble(3, X, C, 4, row, col)
+    rhs = O.rhs_constant_source(3, X, C, 4, 1.0)
+    bottom = np.nonzero(X[:, 2] == 0.0)[0]
+    ei = np.zeros(N, np.uint8)
+    ei[bottom] = O.ELIMINATE_ROW
+    O.apply_transformations(row, col, val, rhs, None, None, ei, np.zeros(N))
+    S = sp.csr_matrix((val, col, row), shape=(N, N))
+    x = RNG.normal(size=N)
+    assert np.allclose(O.spmv(row, col, val, x), S @ x, rtol=1e-14, atol=1e-14)
+    sol, it, rr = O.pcg_jacobi(row, col, val, rhs, rtol=1e-10)
+    assert it > 0 and rr < 1e-9
+    direct = spl.spsolve(S.tocsc(), rhs)
+    assert np.abs(sol - direct).max() < 1e-8 * np.abs(direct).max()
+
+
+def test_count_differences_semantics():
+    ref = np.array([1.0, 2.0, 0.0, 1e-12, 5.0])
+    v = np.array([1.0, 2.0001, 0.0, -1e-12, 5.1])
+    assert O.count_differences(ref, v, 1e-4, 1e-10) == 1  # only 5.0 vs 5.1
+    assert O.count_differences(ref, v, 1e-4, 0.0) == 2  # +-1e-12 differ once min_value is 0
+    assert O.count_differences(ref, v, 1e-1, 0.0) == 1
+
+
+# ---- golden vectors of the reference ----------------------------------------------------------
+
+def load_case(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def by_uid(d, x, ncomp=1):
+    m = dict(zip(d["node_uid"].tolist(), x.reshape(-1, ncomp)))
+    return np.array([m[u] for u in d["ref_uid"].tolist()])
+
+
+@pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
+def test_golden_poisson_penalty(case):
+    """modules/poisson tests 2D_bsr / 3D_bsr: f=5.5, u=0.5 on 'horizontal', penalty 1e30, eps 1e-4."""
+    d = load_case(case)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = O.rhs_constant_source(dim, X, C, npc, 5.5)
+    g = d["group_horizontal"]
+    fi = np.zeros(N, np.uint8)
+    fv = np.zeros(N)
+    fi[g], fv[g] = 1, 1e30
+    rhs[g] = 1e30 * 0.5
+    O.apply_transformations(row, col, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
+
+
+@pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
+def test_golden_poisson_row_elimination_pcg(case):
+    d = load_case(case)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = O.rhs_constant_source(dim, X, C, npc, 5.5)
+    g = d["group_horizontal"]
+    ei = np.zeros(N, np.uint8)
+    ev = np.zeros(N)
+    ei[g], ev[g] = O.ELIMINATE_ROW, 0.5
+    O.apply_transformations(row, col, val, rhs, None, None, ei, ev)
+    x0 = np.where(ei > 0, ev, 0.0)
+    x, it, rr = O.pcg_jacobi(row, col, val, rhs, rtol=1e-12, x0=x0)
+    assert it > 0 and rr < 1e-11
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
+
+
+def elasticity_bar_system(order_per_block):
+    d = load_case("bar_3D")
+    X, C = d["coords"], d["cells"]
+    N = X.shape[0]
+    lam, mu = O.lame(21e5, 0.28)
+    row, col = O.pattern(N, C, 4)
+    val = O.assemble(3, X, C, 4, row, col, element=O.ELASTICITY, nb_dof=3, order_per_block=order_per_block,
+                     lame_lambda=lam, lame_mu=mu)
+    # <f>-1.0</f> parses as the one-entry string[] "-1.0": f = (-1, 0, 0)
+    # (modules/elasticity/BodyForce.h:35-43)
+    rhs = O.rhs_body_force(X, C, [-1.0, 0.0, 0.0])
+    fi = np.zeros(3 * N, np.uint8)
+    fv = np.zeros(3 * N)
+    for n in d["group_surfaceleft"]:
+        for c in range(3):
+            fi[3 * n + c], fv[3 * n + c], rhs[3 * n + c] = 1, 1e30, 0.0
+    for n in d["group_surfaceright"]:
+        fi[3 * n + 1], fv[3 * n + 1], rhs[3 * n + 1] = 1, 1e30, 1e30 * 1.0
+    return d, N, row, col, val, rhs, fi, fv
+
+
+def test_golden_elasticity_bar():
+    """modules/elasticity test 3D_Dirichlet_bodyforce_bsr: eps 1e-3, min value 1e-10."""
+    d, N, row, col, val, rhs, fi, fv = elasticity_bar_system(False)
+    crow, ccol = O.bsr_to_csr(N, 3, row, col)
+    O.apply_transformations(crow, ccol, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, ccol, crow), shape=(3 * N, 3 * N)).tocsc(), rhs)
+    got = by_uid(d, x, 3)
+    assert O.count_differences(d["ref_value"], got, 1e-3, 1e-10) == 0
