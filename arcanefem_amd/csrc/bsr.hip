@@ -167,33 +167,6 @@ __device__ __forceinline__ d3 cross(d3 a, d3 b)
 }
 __device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-// Unnormalised P1 gradients of a tetrahedron: grad N_q = C[q] / det with
-// det = (p1-p0) . ((p2-p0) x (p3-p0)); returns |det| (= 6 |T|).
-__device__ __forceinline__ double tet_cofactors(const double* __restrict__ coords, const int32_t* cn, d3 C[4])
-{
-  const d3 p0 = load_node(coords, cn[0]);
-  const d3 e1 = sub(load_node(coords, cn[1]), p0);
-  const d3 e2 = sub(load_node(coords, cn[2]), p0);
-  const d3 e3 = sub(load_node(coords, cn[3]), p0);
-  C[1] = cross(e2, e3);
-  C[2] = cross(e3, e1);
-  C[3] = cross(e1, e2);
-  C[0] = { -(C[1].x + C[2].x + C[3].x), -(C[1].y + C[2].y + C[3].y), -(C[1].z + C[2].z + C[3].z) };
-  return fabs(dot(e1, C[1]));
-}
-
-// Triangle: grad N_q = G[q] / det, det = twice the signed area; returns |det|.
-__device__ __forceinline__ double tri_cofactors(const double* __restrict__ coords, const int32_t* cn, d3 G[3])
-{
-  const d3 p0 = load_node(coords, cn[0]);
-  const d3 e1 = sub(load_node(coords, cn[1]), p0);
-  const d3 e2 = sub(load_node(coords, cn[2]), p0);
-  G[1] = { e2.y, -e2.x, 0.0 };
-  G[2] = { -e1.y, e1.x, 0.0 };
-  G[0] = { -(G[1].x + G[2].x), -(G[1].y + G[2].y), 0.0 };
-  return fabs(e1.x * e2.y - e2.x * e1.y);
-}
-
 // Position of column v inside a sorted row held in LDS.
 __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int32_t v)
 {
@@ -208,15 +181,35 @@ __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int le
   return lo;
 }
 
+// The other nodes of a cell seen from its node at slot s, in cell order.
+__device__ __forceinline__ int3 other_nodes(int4 cq, int s)
+{
+  return { (s == 0) ? cq.y : cq.x, (s <= 1) ? cq.z : cq.y, (s <= 2) ? cq.w : cq.z };
+}
+
+__device__ __forceinline__ void lds_add(const int32_t* __restrict__ rc, double* rv, int len, int32_t col, double v)
+{
+  const int pos = find_col(rc, len, col);
+  if (pos < len)
+    rv[pos] += v;
+}
+
 // Node-wise gather assembly of the P1 Laplace form, one thread per row.
 // The block's rows are consecutive, so their values form one contiguous
 // range [row_index[r0], row_index[r1]) that is staged in LDS (columns +
 // accumulators), zero-filled, accumulated by each row's owner thread without
-// atomics, and written back with coalesced stores.  Each (row, cell) pair
-// recomputes the element row from the 4 (3) node coordinates: the element
-// kernel is recomputed npc times per cell, which costs FP64 ALU instead of
-// HBM bytes.  Optionally the same pass writes the constant-source RHS.
-template <int NPC>
+// atomics, and written back with coalesced stores.
+//
+// Each (row, cell) pair recomputes the element row: the element kernel runs
+// npc times per cell, which costs FP64 ALU instead of HBM bytes.  The row
+// node is taken as the origin: with e_k = p_k - p (the other nodes in cell
+// order), C_k = e_{k+1} x e_{k+2} are det * grad N_k, det = e_a . C_a and
+// C_p = -(C_a + C_b + C_c); then K_pq = lambda (C_p . C_q) / (6 |det|).  The
+// cells of a row are processed U at a time with all their index and
+// coordinate loads issued before any arithmetic, so a thread keeps U
+// dependent load chains in flight.  The same pass writes the constant-source
+// RHS b_p = sum f |T| / npc when rhs is given.
+template <int NPC, int U>
 __global__ void __launch_bounds__(256)
 k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict__ cells,
                    const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
@@ -245,41 +238,64 @@ k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict_
     const int32_t* rc = s_col + rb;
     double* rv = s_val + rb;
     const int dpos = find_col(rc, len, i);
-    double b_acc = 0.0;
-    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
-      const int32_t it = nc_items[e];
-      const int32_t c = it >> 2;
-      const int s = it & 3;
-      const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
-      const int32_t cn[4] = { cq.x, cq.y, cq.z, cq.w };
-      const double lam = cell_lambda ? cell_lambda[c] : 1.0;
-      if constexpr (NPC == 4) {
-        d3 C[4];
-        const double adet = tet_cofactors(coords, cn, C);
-        const double scale = lam / (6.0 * adet);
-        const d3 cs = C[s];
+    const d3 p = load_node(coords, i);
+    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    double diag = 0.0, b_acc = 0.0;
+    for (int32_t e = e0; e < e1; e += U) {
+      int32_t it[U];
+      int4 cq[U];
+      double lam[U];
+      d3 pa[U], pb[U], pc[U];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
-          if (pos < len)
-            rv[pos] += scale * dot(cs, C[q]);
-        }
-        b_acc += source_f * (adet / 6.0) / 4;
+      for (int u = 0; u < U; ++u)
+        it[u] = nc_items[min(e + u, e1 - 1)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t c = it[u] >> 2;
+        cq[u] = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+        lam[u] = cell_lambda ? cell_lambda[c] : 1.0;
       }
-      else {
-        d3 G[3];
-        const double adet = tri_cofactors(coords, cn, G);
-        const double scale = lam / (2.0 * adet);
-        const d3 gs = G[s];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
-          if (pos < len)
-            rv[pos] += scale * (gs.x * G[q].x + gs.y * G[q].y);
+      for (int u = 0; u < U; ++u) {
+        const int3 o = other_nodes(cq[u], it[u] & 3);
+        pa[u] = load_node(coords, o.x);
+        pb[u] = load_node(coords, o.y);
+        if constexpr (NPC == 4)
+          pc[u] = load_node(coords, o.z);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= e1)
+          break;
+        const int3 o = other_nodes(cq[u], it[u] & 3);
+        if constexpr (NPC == 4) {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p), ec = sub(pc[u], p);
+          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+          const double adet = fabs(dot(ea, Ca));
+          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+          const double scale = lam[u] / (6.0 * adet);
+          diag += scale * dot(Cp, Cp);
+          lds_add(rc, rv, len, o.x, scale * dot(Cp, Ca));
+          lds_add(rc, rv, len, o.y, scale * dot(Cp, Cb));
+          lds_add(rc, rv, len, o.z, scale * dot(Cp, Cc));
+          b_acc += source_f * (adet / 6.0) / 4;
         }
-        b_acc += source_f * (adet / 2.0) / 3;
+        else {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p);
+          const double det = ea.x * eb.y - eb.x * ea.y;
+          const double adet = fabs(det);
+          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+          const double gpx = -(gax + gbx), gpy = -(gay + gby);
+          const double scale = lam[u] / (2.0 * adet);
+          diag += scale * (gpx * gpx + gpy * gpy);
+          lds_add(rc, rv, len, o.x, scale * (gpx * gax + gpy * gay));
+          lds_add(rc, rv, len, o.y, scale * (gpx * gbx + gpy * gby));
+          b_acc += source_f * (adet / 2.0) / 3;
+        }
       }
     }
+    if (dpos < len)
+      rv[dpos] += diag;
     if (rhs)
       rhs[i] = b_acc;
   }
@@ -293,7 +309,26 @@ k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict_
 // the closed form of modules/elasticity/ElementMatrix.h:151-211.  Blocks are
 // staged in LDS (9 doubles each, row-major) and written in the requested
 // order: per block (block*9 + i*3 + j) or per row (scalar CSR order,
-// row_index[a]*9 + 3*(x + i*nnz_row) + j).
+// row_index[a]*9 + 3*(x + i*nnz_row) + j).  Same origin-at-the-row-node
+// cofactors and U-cell batching as the Laplace kernel.
+__device__ __forceinline__ void lds_add_block(const int32_t* __restrict__ rc, double* rv, int len, int32_t col,
+                                              const d3& cp, const d3& cq, double la, double mu)
+{
+  const int pos = find_col(rc, len, col);
+  if (pos >= len)
+    return;
+  const double a[3] = { cp.x, cp.y, cp.z };
+  const double b[3] = { cq.x, cq.y, cq.z };
+  const double dd = mu * (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+  double* blk = rv + 9 * pos;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+      blk[r * 3 + t] += la * a[r] * b[t] + mu * b[r] * a[t] + (r == t ? dd : 0.0);
+}
+
+template <int U>
 __global__ void __launch_bounds__(256)
 k_assemble_elasticity(const double* __restrict__ coords, const int32_t* __restrict__ cells,
                       const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
@@ -320,31 +355,40 @@ k_assemble_elasticity(const double* __restrict__ coords, const int32_t* __restri
     const int len = row_index[i + 1] - row_index[i];
     const int32_t* rc = s_col + rb;
     double* rv = s_val + 9 * (int64_t)rb;
-    const int dpos = find_col(rc, len, i);
-    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
-      const int32_t it = nc_items[e];
-      const int32_t c = it >> 2;
-      const int s = it & 3;
-      const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
-      const int32_t cn[4] = { cq.x, cq.y, cq.z, cq.w };
-      d3 C[4];
-      const double adet = tet_cofactors(coords, cn, C);
-      const double inv6 = 1.0 / (6.0 * adet);
-      const double la = lame_lambda * inv6, mu = lame_mu * inv6;
-      const double ca[3] = { C[s].x, C[s].y, C[s].z };
+    const d3 p = load_node(coords, i);
+    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    for (int32_t e = e0; e < e1; e += U) {
+      int32_t it[U];
+      int4 cq[U];
+      d3 pa[U], pb[U], pc[U];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int pos = (q == s) ? dpos : find_col(rc, len, cn[q]);
-        if (pos >= len)
-          continue;
-        const double cb[3] = { C[q].x, C[q].y, C[q].z };
-        const double dd = mu * (ca[0] * cb[0] + ca[1] * cb[1] + ca[2] * cb[2]);
-        double* blk = rv + 9 * pos;
+      for (int u = 0; u < U; ++u)
+        it[u] = nc_items[min(e + u, e1 - 1)];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+      for (int u = 0; u < U; ++u)
+        cq[u] = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)(it[u] >> 2));
 #pragma unroll
-          for (int t = 0; t < 3; ++t)
-            blk[r * 3 + t] += la * ca[r] * cb[t] + mu * cb[r] * ca[t] + (r == t ? dd : 0.0);
+      for (int u = 0; u < U; ++u) {
+        const int3 o = other_nodes(cq[u], it[u] & 3);
+        pa[u] = load_node(coords, o.x);
+        pb[u] = load_node(coords, o.y);
+        pc[u] = load_node(coords, o.z);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= e1)
+          break;
+        const int3 o = other_nodes(cq[u], it[u] & 3);
+        const d3 ea = sub(pa[u], p), eb = sub(pb[u], p), ec = sub(pc[u], p);
+        const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+        const double adet = fabs(dot(ea, Ca));
+        const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+        const double inv6 = 1.0 / (6.0 * adet);
+        const double la = lame_lambda * inv6, mu = lame_mu * inv6;
+        lds_add_block(rc, rv, len, i, Cp, Cp, la, mu);
+        lds_add_block(rc, rv, len, o.x, Cp, Ca, la, mu);
+        lds_add_block(rc, rv, len, o.y, Cp, Cb, la, mu);
+        lds_add_block(rc, rv, len, o.z, Cp, Cc, la, mu);
       }
     }
   }
@@ -392,11 +436,11 @@ static void assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, d
   const unsigned grid = grid_for(m.nb_node, rows);
   ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
   if (m.npc == 4)
-    hipLaunchKernelGGL(k_assemble_laplace<4>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+    hipLaunchKernelGGL((k_assemble_laplace<4, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
                        m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
                        rhs, b->values.p);
   else
-    hipLaunchKernelGGL(k_assemble_laplace<3>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+    hipLaunchKernelGGL((k_assemble_laplace<3, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
                        m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
                        rhs, b->values.p);
   check_launch("assemble_laplace");
@@ -413,7 +457,7 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
   assembly_shape(*b, 9 * 8 + 4, &rows, &lds);
   const unsigned grid = grid_for(m.nb_node, rows);
   ProfScope prof("assemble_elasticity_tet");
-  hipLaunchKernelGGL(k_assemble_elasticity, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+  hipLaunchKernelGGL(k_assemble_elasticity<2>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
                      m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, lam, mu,
                      b->per_block ? 1 : 0, b->values.p);
   check_launch("assemble_elasticity");

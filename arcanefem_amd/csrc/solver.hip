@@ -4,20 +4,26 @@
 // femutils/PetscDoFLinearSystem.cc:427-572): PCG with a relative 2-norm test and a
 // Jacobi preconditioner (femutils/PetscDoFLinearSystemFactory.axl:5-10).
 //
-// Layout of one PCG iteration (all scalars live on the device, so the host
-// only polls every check_every iterations):
-//   spmv   q = A p, fused dot p.q               -> last block: alpha = rho / p.q
-//   update x += alpha p, r -= alpha q, z = M r, dots r.z, r.r
-//                                                -> last block: beta, rho, convergence
-//   pupd   p = z + beta p
-// The mixed variant keeps r, z, p, q, A and M in FP32 and x in FP64; when the
-// FP32 residual has dropped by reliable_delta (or reaches the tolerance) a
-// reliable update recomputes r = b - A x with the FP64 matrix, and only that
-// FP64 residual can declare convergence.  Dots are always accumulated in FP64.
+// One PCG iteration k is three kernels (four for the mixed solver):
+//   spmv     q = A p, per-block partials of p.q
+//   update   alpha = rho_{k-1} / p.q; x += alpha p, r -= alpha q, z = M r;
+//            partials of r.z (parity buffer k&1) and r.r
+//  [reliable update, mixed only: r64 = b - A64 x when the FP32 residual has
+//            dropped by reliable_delta or reached the tolerance]
+//   pupdate  beta = rho_k / rho_{k-1}; convergence test; p = z + beta p
+// Reductions never use fences or atomics: a producing kernel writes one
+// partial per block, and every block of a consuming kernel re-reduces all
+// partials in the same order (deterministic, and identical in every block).
+// Values read across kernels are parity-indexed by the host-supplied
+// iteration index k, so no kernel reads what another block of the same
+// kernel writes.  The mixed variant keeps r, z, p, q, A and M in FP32 and x
+// in FP64; only the FP64 residual of a reliable update can declare
+// convergence.  Dots are accumulated in FP64.
 //
-// Reductions: each block writes its partial sums, the last block to finish
-// (atomic ticket) adds the partials in block order -- deterministic -- and
-// updates the scalars.
+// SpMV formats: CSR with G lanes per row (wavefront-group row reductions) and
+// SELL-64 (slices of 64 rows stored column-major, one lane per row, fully
+// coalesced value/column streams), scalar or with 3x3 blocks.  The first
+// solve on a matrix times the candidates and keeps the fastest.
 #include "common.h"
 #include "device_utils.h"
 #include "linsys.h"
@@ -27,217 +33,116 @@
 
 namespace afem {
 
-struct Scalars
-{
-  double rho, alpha, beta, rr, bb, tol2, rr_ref, delta2, rtol2, atol2;
-  int iter, max_iter, done, converged, ru, nb_ru;
-};
-
-struct RedCtx
-{
-  double* partials;
-  unsigned* counter;
-  Scalars* sc;
-};
-
 constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 1024;
 
-// Reduction of NV per-thread values over the grid; the last block calls fin.
-template <int NV, class Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NV], const RedCtx& ctx, Fin fin)
+struct State
 {
-  __shared__ double red[NV][kBlock / kWave];
-  __shared__ int is_last;
-  const int wave = threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
+  double rtol2, atol2, delta2, tol2, bb, rr_out, rr_true;
+  double rr_ref[2];
+  int ru[2];
+  int max_iter, iter, done, converged, breakdown, nb_ru;
+};
+
+// per-block partial sums, kMaxBlocks entries each
+struct Parts
+{
+  double* pq;
+  double* rz[2];
+  double* rzru[2];
+  double* rr;
+  double* rr64;
+  double* bb;
+};
+
+// ---- block reductions --------------------------------------------------------------------
+
+// Sum of n partials, identical in every block; all threads get the result.
+__device__ __forceinline__ double sum_parts(const double* __restrict__ p, int n)
+{
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock)
+    s += p[i];
+  s = wave_reduce_sum(s);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = s;
+  __syncthreads();
+  double t = 0.0;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = wave_reduce_sum(v[k]);
-    if (lane == 0)
-      red[k][wave] = s;
-  }
+  for (int w = 0; w < kBlock / kWave; ++w)
+    t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// Writes the block sum of v to out[blockIdx.x].
+__device__ __forceinline__ void put_part(double v, double* __restrict__ out)
+{
+  __shared__ double lds[kBlock / kWave];
+  v = wave_reduce_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
+    double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w)
-        s += red[k][w];
-      __hip_atomic_store(&ctx.partials[(int64_t)blockIdx.x * NV + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    const unsigned prev = atomicAdd(ctx.counter, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last)
-    return;
-  __threadfence();
-  double t[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-    t[k] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      t[k] += __hip_atomic_load(&ctx.partials[(int64_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = wave_reduce_sum(t[k]);
-    if (lane == 0)
-      red[k][wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      s[k] = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w)
-        s[k] += red[k][w];
-    }
-    fin(ctx.sc, s);
-    *ctx.counter = 0u;
-    __threadfence();
+    for (int w = 0; w < kBlock / kWave; ++w)
+      t += lds[w];
+    out[blockIdx.x] = t;
   }
 }
 
-// Gates: which iterations a kernel runs in.
-enum Gate
+// Contiguous chunk of tiles owned by this block (XCD-aware order).
+__device__ __forceinline__ void chunk_of(int64_t ntiles, int64_t& t0, int64_t& t1)
+{
+  const int64_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
+  const int64_t L = xcd_tile(blockIdx.x, gridDim.x);
+  t0 = L * chunk;
+  t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+}
+
+// ---- gates -------------------------------------------------------------------------------
+
+enum GateMode
 {
   kAlways = 0,
   kUntilDone = 1,
-  kReliableUpdate = 2
+  kReliable = 2 // runs only when iteration k decides a reliable update
 };
 
-__device__ __forceinline__ bool gate_closed(const Scalars* sc, int gate)
+__device__ __forceinline__ bool reliable_decision(const State* st, double rr, int k)
 {
-  if (gate == kAlways)
-    return false;
-  if (sc->done)
+  return rr <= st->delta2 * st->rr_ref[(k - 1) & 1] || rr <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr);
+}
+
+struct Gate
+{
+  const State* st;
+  int mode;
+  int k;
+  const double* rr_parts;
+  int nparts;
+  // evaluated by every thread of a block (uses __syncthreads through sum_parts)
+  __device__ bool open() const
+  {
+    if (mode == kAlways)
+      return true;
+    if (st->done || st->breakdown)
+      return false;
+    if (mode == kReliable)
+      return reliable_decision(st, sum_parts(rr_parts, nparts), k);
     return true;
-  return gate == kReliableUpdate && !sc->ru;
-}
-
-// ---- finalizers (run by one thread of the last block) ---------------------------
-
-struct FinAlpha
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    const double pq = s[0];
-    if (pq > 0.0 && isfinite(pq)) {
-      sc->alpha = sc->rho / pq;
-    }
-    else { // breakdown: the operator is not positive definite on p
-      sc->alpha = 0.0;
-      sc->done = 1;
-    }
   }
 };
 
-struct FinTrueResidual
-{
-  __device__ void operator()(Scalars* sc, const double* s) const { sc->rr = s[0]; }
-};
-
-// After a reliable update: the FP64 residual is the judge.
-struct FinReliable
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    sc->rr = s[0];
-    sc->rr_ref = s[0];
-    sc->nb_ru += 1;
-    if (s[0] <= sc->tol2) {
-      sc->done = 1;
-      sc->converged = 1;
-    }
-    else if (sc->iter >= sc->max_iter) {
-      sc->done = 1;
-    }
-  }
-};
-
-struct FinInit
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    sc->rho = s[0];
-    sc->rr = s[1];
-    sc->rr_ref = s[1];
-    sc->bb = s[2];
-    sc->tol2 = fmax(sc->rtol2 * s[2], sc->atol2);
-    if (s[1] <= sc->tol2 || s[0] == 0.0) {
-      sc->done = 1;
-      sc->converged = 1;
-    }
-  }
-};
-
-// update step of the FP64 solver: the recursive residual decides.
-struct FinUpdate
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    const double rho_new = s[0];
-    sc->beta = (sc->rho != 0.0) ? rho_new / sc->rho : 0.0;
-    sc->rho = rho_new;
-    sc->rr = s[1];
-    sc->iter += 1;
-    if (s[1] <= sc->tol2) {
-      sc->done = 1;
-      sc->converged = 1;
-    }
-    else if (sc->iter >= sc->max_iter || !isfinite(s[1])) {
-      sc->done = 1;
-    }
-  }
-};
-
-// update step of the mixed solver: the FP32 residual only requests a
-// reliable update.
-struct FinUpdateMixed
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    const double rho_new = s[0];
-    sc->beta = (sc->rho != 0.0) ? rho_new / sc->rho : 0.0;
-    sc->rho = rho_new;
-    sc->rr = s[1];
-    sc->iter += 1;
-    if (s[1] <= sc->delta2 * sc->rr_ref || s[1] <= sc->tol2 || sc->iter >= sc->max_iter || !isfinite(s[1]))
-      sc->ru = 1;
-  }
-};
-
-struct FinRho
-{
-  __device__ void operator()(Scalars* sc, const double* s) const
-  {
-    sc->rho = s[0];
-    sc->ru = 0;
-  }
-};
-
-// Contiguous chunk of work units for this block (XCD-aware).
-__device__ __forceinline__ void block_chunk(int64_t nunits, int64_t chunk, int64_t& u0, int64_t& u1)
-{
-  const int64_t L = xcd_tile(blockIdx.x, gridDim.x);
-  u0 = L * chunk;
-  u1 = u0 + chunk;
-  if (u1 > nunits)
-    u1 = nunits;
-}
-
-// ---- SpMV ------------------------------------------------------------------------------
+// ---- SpMV ----------------------------------------------------------------------------------
 
 enum SpmvMode
 {
   kPlain = 0,
-  kPQ = 1,      // y = A x and dot(x, y) -> alpha
-  kResidual = 2 // y = b - A x and dot(y, y); finalizer chosen by template
+  kPQ = 1,      // y = A x; partial x.y
+  kResidual = 2 // y = b - A x; partial y.y
 };
 
 template <class VT>
@@ -251,25 +156,38 @@ struct Acc<float>
   using type = float;
 };
 
-// G lanes per row, 256/G rows per tile; each block owns a contiguous chunk of
-// tiles.  Lanes of a group read consecutive entries of their row, so a
-// wavefront streams 64/G consecutive rows with coalesced value/column loads.
-template <int G, class VT, class XT, int MODE, class Fin>
+template <int MODE, class XT>
+__device__ __forceinline__ void spmv_epilogue(int64_t row, double sum, const XT* __restrict__ x, XT* __restrict__ y,
+                                              const double* __restrict__ b, double& part)
+{
+  if constexpr (MODE == kResidual) {
+    const double r = b[row] - sum;
+    y[row] = (XT)r;
+    part += r * r;
+  }
+  else {
+    y[row] = (XT)sum;
+    if constexpr (MODE == kPQ)
+      part += sum * (double)x[row];
+  }
+}
+
+// CSR, G lanes per row, 256/G rows per tile.
+template <int G, class VT, class XT, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const VT* __restrict__ v,
-           const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, int64_t chunk, int gate,
-           RedCtx ctx, Fin fin)
+           const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
+           double* __restrict__ part_out)
 {
-  if (MODE != kPlain && gate_closed(ctx.sc, gate))
+  if (!gate.open())
     return;
   using A = typename Acc<VT>::type;
   constexpr int RPT = kBlock / G;
-  const int64_t ntiles = ((int64_t)n + RPT - 1) / RPT;
   int64_t t0, t1;
-  block_chunk(ntiles, chunk, t0, t1);
+  chunk_of(((int64_t)n + RPT - 1) / RPT, t0, t1);
   const int lane = threadIdx.x & (G - 1);
   const int sub = threadIdx.x / G;
-  double acc = 0.0;
+  double part = 0.0;
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t row = t * RPT + sub;
     if (row < n) {
@@ -278,43 +196,70 @@ k_spmv_csr(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict_
       for (int32_t k = s + lane; k < e; k += G)
         sum += (A)v[k] * (A)x[ci[k]];
       sum = group_reduce_sum<G>(sum);
-      if (lane == 0) {
-        if constexpr (MODE == kResidual) {
-          const double r = b[row] - (double)sum;
-          y[row] = (XT)r;
-          acc += r * r;
-        }
-        else {
-          y[row] = (XT)sum;
-          if constexpr (MODE == kPQ)
-            acc += (double)sum * (double)x[row];
-        }
-      }
+      if (lane == 0)
+        spmv_epilogue<MODE>(row, (double)sum, x, y, b, part);
     }
   }
-  if constexpr (MODE != kPlain) {
-    double vv[1] = { acc };
-    grid_reduce<1>(vv, ctx, fin);
-  }
+  if constexpr (MODE != kPlain)
+    put_part(part, part_out);
 }
 
-// BSR with 3x3 blocks stored per block (9 contiguous values, row-major).
-template <int G, class VT, class XT, int MODE, class Fin>
+// SELL-64: slice s holds rows [64s, 64s+64); its slots (j, lane) live at
+// (off[s] + j) * 64 + lane for j < off[s+1] - off[s]; padding slots carry a
+// zero value and the row's own column.
+template <class VT, class XT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ col, const VT* __restrict__ v,
+            const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
+            double* __restrict__ part_out)
+{
+  if (!gate.open())
+    return;
+  using A = typename Acc<VT>::type;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  double part = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row < n) {
+      const int64_t s = row >> 6;
+      const int lane = (int)(row & 63);
+      const int64_t base = (int64_t)off[s] * 64 + lane;
+      const int w = off[s + 1] - off[s];
+      A s0 = 0, s1 = 0;
+      int j = 0;
+      for (; j + 1 < w; j += 2) {
+        const int64_t i0 = base + (int64_t)j * 64, i1 = i0 + 64;
+        s0 += (A)v[i0] * (A)x[col[i0]];
+        s1 += (A)v[i1] * (A)x[col[i1]];
+      }
+      if (j < w) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        s0 += (A)v[i0] * (A)x[col[i0]];
+      }
+      spmv_epilogue<MODE>(row, (double)(s0 + s1), x, y, b, part);
+    }
+  }
+  if constexpr (MODE != kPlain)
+    put_part(part, part_out);
+}
+
+// BSR 3x3 blocks stored per block (9 contiguous values), G lanes per node row.
+template <int G, class VT, class XT, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_bsr3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const VT* __restrict__ v,
-            const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, int64_t chunk, int gate,
-            RedCtx ctx, Fin fin)
+            const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
+            double* __restrict__ part_out)
 {
-  if (MODE != kPlain && gate_closed(ctx.sc, gate))
+  if (!gate.open())
     return;
   using A = typename Acc<VT>::type;
   constexpr int RPT = kBlock / G;
-  const int64_t ntiles = ((int64_t)nb_node + RPT - 1) / RPT;
   int64_t t0, t1;
-  block_chunk(ntiles, chunk, t0, t1);
+  chunk_of(((int64_t)nb_node + RPT - 1) / RPT, t0, t1);
   const int lane = threadIdx.x & (G - 1);
   const int sub = threadIdx.x / G;
-  double acc = 0.0;
+  double part = 0.0;
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t a = t * RPT + sub;
     if (a < nb_node) {
@@ -332,26 +277,100 @@ k_spmv_bsr3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __re
       s1 = group_reduce_sum<G>(s1);
       s2 = group_reduce_sum<G>(s2);
       if (lane == 0) {
-        const A sv[3] = { s0, s1, s2 };
-        for (int i = 0; i < 3; ++i) {
-          const int64_t row = 3 * a + i;
-          if constexpr (MODE == kResidual) {
-            const double r = b[row] - (double)sv[i];
-            y[row] = (XT)r;
-            acc += r * r;
-          }
-          else {
-            y[row] = (XT)sv[i];
-            if constexpr (MODE == kPQ)
-              acc += (double)sv[i] * (double)x[row];
-          }
-        }
+        spmv_epilogue<MODE>(3 * a, (double)s0, x, y, b, part);
+        spmv_epilogue<MODE>(3 * a + 1, (double)s1, x, y, b, part);
+        spmv_epilogue<MODE>(3 * a + 2, (double)s2, x, y, b, part);
       }
     }
   }
-  if constexpr (MODE != kPlain) {
-    double vv[1] = { acc };
-    grid_reduce<1>(vv, ctx, fin);
+  if constexpr (MODE != kPlain)
+    put_part(part, part_out);
+}
+
+// SELL-64 with 3x3 blocks: slot (j, lane) of slice s has its column at
+// (off[s]+j)*64 + lane and value q (row-major in the block) at
+// ((off[s]+j)*9 + q)*64 + lane.
+template <class VT, class XT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_sell3(int32_t nb_node, const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+             const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
+             Gate gate, double* __restrict__ part_out)
+{
+  if (!gate.open())
+    return;
+  using A = typename Acc<VT>::type;
+  int64_t t0, t1;
+  chunk_of(((int64_t)nb_node + kBlock - 1) / kBlock, t0, t1);
+  double part = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t a = t * kBlock + threadIdx.x;
+    if (a < nb_node) {
+      const int64_t s = a >> 6;
+      const int lane = (int)(a & 63);
+      const int64_t o = off[s];
+      const int w = off[s + 1] - off[s];
+      A s0 = 0, s1 = 0, s2 = 0;
+      for (int j = 0; j < w; ++j) {
+        const int64_t slot = o + j;
+        const int64_t c = 3 * (int64_t)col[slot * 64 + lane];
+        const VT* vb = v + slot * 9 * 64 + lane;
+        const A x0 = (A)x[c], x1 = (A)x[c + 1], x2 = (A)x[c + 2];
+        s0 += (A)vb[0] * x0 + (A)vb[64] * x1 + (A)vb[128] * x2;
+        s1 += (A)vb[192] * x0 + (A)vb[256] * x1 + (A)vb[320] * x2;
+        s2 += (A)vb[384] * x0 + (A)vb[448] * x1 + (A)vb[512] * x2;
+      }
+      spmv_epilogue<MODE>(3 * a, (double)s0, x, y, b, part);
+      spmv_epilogue<MODE>(3 * a + 1, (double)s1, x, y, b, part);
+      spmv_epilogue<MODE>(3 * a + 2, (double)s2, x, y, b, part);
+    }
+  }
+  if constexpr (MODE != kPlain)
+    put_part(part, part_out);
+}
+
+// ---- SELL construction -------------------------------------------------------------------------
+
+__global__ void k_sell_width(int32_t nrows, const int32_t* __restrict__ rp, int32_t* __restrict__ width)
+{
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t nslice = (nrows + 63) / 64;
+  if (s >= nslice)
+    return;
+  int32_t w = 0;
+  for (int32_t r = 64 * s; r < 64 * s + 64 && r < nrows; ++r)
+    w = max(w, rp[r + 1] - rp[r]);
+  width[s] = w;
+}
+
+// Columns and FP64/FP32 values of the SELL copy; NB = 1 (scalar) or 3.
+template <int NB>
+__global__ void k_sell_fill(int32_t nrows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const double* __restrict__ v, const int32_t* __restrict__ off, int32_t* __restrict__ col,
+                            double* __restrict__ v64, float* __restrict__ v32, int fill_columns)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nslots_rows = ((int64_t)nrows + 63) / 64 * 64;
+  if (r >= nslots_rows)
+    return;
+  const int64_t s = r >> 6;
+  const int lane = (int)(r & 63);
+  const int64_t o = off[s];
+  const int w = off[s + 1] - off[s];
+  const int32_t len = (r < nrows) ? rp[r + 1] - rp[r] : 0;
+  const int32_t b0 = (r < nrows) ? rp[r] : 0;
+  for (int j = 0; j < w; ++j) {
+    const int64_t slot = o + j;
+    const bool real = j < len;
+    if (fill_columns)
+      col[slot * 64 + lane] = real ? ci[b0 + j] : (int32_t)(r < nrows ? r : 0);
+    for (int q = 0; q < NB * NB; ++q) {
+      const double val = real ? v[(int64_t)(b0 + j) * NB * NB + q] : 0.0;
+      const int64_t idx = (NB == 1) ? slot * 64 + lane : (slot * 9 + q) * 64 + lane;
+      if (v64)
+        v64[idx] = val;
+      if (v32)
+        v32[idx] = (float)val;
+    }
   }
 }
 
@@ -430,11 +449,15 @@ __global__ void k_initial_guess(int32_t n, const uint8_t* __restrict__ elim, con
 
 // ---- vector kernels ------------------------------------------------------------------------
 
-// z = M r for the unit u (a row when NB = 1, a node when NB = 3).
 template <int NB, class T>
-__device__ __forceinline__ void apply_jacobi(int64_t u, const T* __restrict__ dinv, const T* rv, T* zv)
+__device__ __forceinline__ void apply_jacobi(int64_t u, const T* __restrict__ dinv, const T* rv, T* zv, int precond)
 {
-  if constexpr (NB == 1) {
+  if (!precond) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      zv[i] = rv[i];
+  }
+  else if constexpr (NB == 1) {
     zv[0] = dinv[u] * rv[0];
   }
   else {
@@ -445,113 +468,23 @@ __device__ __forceinline__ void apply_jacobi(int64_t u, const T* __restrict__ di
   }
 }
 
-// x += alpha p, r -= alpha q, z = M r; partial dots (r.z, r.r).
-template <int NB, class T, class Fin>
-__global__ void __launch_bounds__(kBlock)
-k_update(int64_t nunits, int64_t chunk, double* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-         const T* __restrict__ p, const T* __restrict__ q, const T* __restrict__ dinv, int precond, RedCtx ctx, Fin fin)
+// rho_{k-1}: r.z of iteration k-1, or of the reliable update that ended it.
+__device__ __forceinline__ double rho_prev(const State* st, const Parts& P, int nparts, int k)
 {
-  if (gate_closed(ctx.sc, kUntilDone))
-    return;
-  const double alpha = ctx.sc->alpha;
-  const T alpha_t = (T)alpha;
-  int64_t u0, u1;
-  block_chunk((nunits + kBlock - 1) / kBlock, chunk, u0, u1);
-  double rz = 0.0, rr = 0.0;
-  for (int64_t t = u0; t < u1; ++t) {
-    const int64_t u = t * kBlock + threadIdx.x;
-    if (u >= nunits)
-      continue;
-    T rv[NB], zv[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const int64_t row = u * NB + i;
-      x[row] += alpha * (double)p[row];
-      rv[i] = r[row] - alpha_t * q[row];
-      r[row] = rv[i];
-    }
-    if (precond)
-      apply_jacobi<NB, T>(u, dinv, rv, zv);
-    else
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        zv[i] = rv[i];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      z[u * NB + i] = zv[i];
-      rz += (double)rv[i] * (double)zv[i];
-      rr += (double)rv[i] * (double)rv[i];
-    }
-  }
-  double vv[2] = { rz, rr };
-  grid_reduce<2>(vv, ctx, fin);
+  const int q = (k - 1) & 1;
+  return st->ru[q] ? sum_parts(P.rzru[q], nparts) : sum_parts(P.rz[q], nparts);
 }
 
-// p = z + beta p.  After a reliable update (mixed solver) r and z are first
-// replaced by the FP64 residual and its preconditioned value, and rho = r.z is
-// recomputed.
+// r0 = b - A x0 is in r64.  z = M r, p = z; partials r.z (parity 1), r.r, b.b.
 template <int NB, class T>
 __global__ void __launch_bounds__(kBlock)
-k_pupdate(int64_t nunits, int64_t chunk, T* __restrict__ p, T* __restrict__ z, T* __restrict__ r,
-          const double* __restrict__ r64, const T* __restrict__ dinv, int precond, RedCtx ctx)
+k_init(int64_t nunits, const double* r64, const double* __restrict__ b, T* r, T* __restrict__ z, T* __restrict__ p,
+       const T* __restrict__ dinv, int precond, Parts P)
 {
-  if (gate_closed(ctx.sc, kUntilDone))
-    return;
-  const T beta = (T)ctx.sc->beta;
-  const bool ru = ctx.sc->ru != 0;
-  int64_t u0, u1;
-  block_chunk((nunits + kBlock - 1) / kBlock, chunk, u0, u1);
-  double rz = 0.0;
-  for (int64_t t = u0; t < u1; ++t) {
-    const int64_t u = t * kBlock + threadIdx.x;
-    if (u >= nunits)
-      continue;
-    T zv[NB];
-    if (ru) {
-      T rv[NB];
-#pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        rv[i] = (T)r64[u * NB + i];
-        r[u * NB + i] = rv[i];
-      }
-      if (precond)
-        apply_jacobi<NB, T>(u, dinv, rv, zv);
-      else
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-          zv[i] = rv[i];
-#pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        z[u * NB + i] = zv[i];
-        rz += (double)rv[i] * (double)zv[i];
-      }
-    }
-    else {
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        zv[i] = z[u * NB + i];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-      p[u * NB + i] = zv[i] + beta * p[u * NB + i];
-  }
-  if (ru) {
-    double vv[1] = { rz };
-    grid_reduce<1>(vv, ctx, FinRho{});
-  }
-}
-
-// Initial z = M r, p = z from the residual r64 = b - A x0 (already computed);
-// dots (r.z, r.r, b.b).  r may alias r64 (FP64 solver).
-template <int NB, class T>
-__global__ void __launch_bounds__(kBlock)
-k_init(int64_t nunits, int64_t chunk, const double* r64, const double* __restrict__ b, T* r, T* __restrict__ z,
-       T* __restrict__ p, const T* __restrict__ dinv, int precond, RedCtx ctx)
-{
-  int64_t u0, u1;
-  block_chunk((nunits + kBlock - 1) / kBlock, chunk, u0, u1);
+  int64_t t0, t1;
+  chunk_of((nunits + kBlock - 1) / kBlock, t0, t1);
   double rz = 0.0, rr = 0.0, bb = 0.0;
-  for (int64_t t = u0; t < u1; ++t) {
+  for (int64_t t = t0; t < t1; ++t) {
     const int64_t u = t * kBlock + threadIdx.x;
     if (u >= nunits)
       continue;
@@ -565,12 +498,7 @@ k_init(int64_t nunits, int64_t chunk, const double* r64, const double* __restric
       const double bv = b[u * NB + i];
       bb += bv * bv;
     }
-    if (precond)
-      apply_jacobi<NB, T>(u, dinv, rv, zv);
-    else
-#pragma unroll
-      for (int i = 0; i < NB; ++i)
-        zv[i] = rv[i];
+    apply_jacobi<NB, T>(u, dinv, rv, zv, precond);
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       z[u * NB + i] = zv[i];
@@ -578,36 +506,219 @@ k_init(int64_t nunits, int64_t chunk, const double* r64, const double* __restric
       rz += (double)rv[i] * (double)zv[i];
     }
   }
-  double vv[3] = { rz, rr, bb };
-  grid_reduce<3>(vv, ctx, FinInit{});
+  put_part(rz, P.rz[1]);
+  put_part(rr, P.rr);
+  put_part(bb, P.bb);
 }
 
-struct NoFin
+__global__ void __launch_bounds__(kBlock) k_init_finalize(Parts P, int nparts, State* st)
 {
-  __device__ void operator()(Scalars*, const double*) const {}
-};
+  const double rz = sum_parts(P.rz[1], nparts);
+  const double rr = sum_parts(P.rr, nparts);
+  const double bb = sum_parts(P.bb, nparts);
+  if (threadIdx.x == 0) {
+    st->bb = bb;
+    st->tol2 = fmax(st->rtol2 * bb, st->atol2);
+    st->rr_ref[1] = rr;
+    st->ru[1] = 0;
+    st->rr_out = rr;
+    st->iter = 0;
+    st->done = (rr <= st->tol2 || rz == 0.0) ? 1 : 0;
+    st->converged = st->done;
+  }
+}
+
+// x += alpha p, r -= alpha q, z = M r; partials r.z (parity k&1) and r.r.
+template <int NB, class T>
+__global__ void __launch_bounds__(kBlock)
+k_update(int k, int64_t nunits, double* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+         const T* __restrict__ p, const T* __restrict__ q, const T* __restrict__ dinv, int precond, Parts P,
+         int nparts, State* st)
+{
+  if (st->done || st->breakdown)
+    return;
+  const double pq = sum_parts(P.pq, nparts);
+  const double rho = rho_prev(st, P, nparts, k);
+  if (!(pq > 0.0) || !isfinite(pq)) { // the operator is not positive definite on p
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      st->breakdown = 1;
+    return;
+  }
+  const double alpha = rho / pq;
+  const T alpha_t = (T)alpha;
+  int64_t t0, t1;
+  chunk_of((nunits + kBlock - 1) / kBlock, t0, t1);
+  double rz = 0.0, rr = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t u = t * kBlock + threadIdx.x;
+    if (u >= nunits)
+      continue;
+    T rv[NB], zv[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int64_t row = u * NB + i;
+      x[row] += alpha * (double)p[row];
+      rv[i] = r[row] - alpha_t * q[row];
+      r[row] = rv[i];
+    }
+    apply_jacobi<NB, T>(u, dinv, rv, zv, precond);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      z[u * NB + i] = zv[i];
+      rz += (double)rv[i] * (double)zv[i];
+      rr += (double)rv[i] * (double)rv[i];
+    }
+  }
+  put_part(rz, P.rz[k & 1]);
+  put_part(rr, P.rr);
+}
+
+// Convergence test and p = z + beta p.  Mixed solver: after a reliable
+// update r and z are replaced by the FP64 residual and its preconditioned
+// value, and the new r.z goes to the rzru parity buffer.
+template <int NB, class T>
+__global__ void __launch_bounds__(kBlock)
+k_pupdate(int k, int mixed, int64_t nunits, T* __restrict__ p, T* __restrict__ z, T* __restrict__ r,
+          const double* __restrict__ r64, const T* __restrict__ dinv, int precond, Parts P, int nparts, State* st)
+{
+  if (st->done)
+    return;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (st->breakdown) {
+    if (lead) {
+      st->done = 1;
+      st->iter = k;
+    }
+    return;
+  }
+  const double rz = sum_parts(P.rz[k & 1], nparts);
+  const double rr = sum_parts(P.rr, nparts);
+  const double rho = rho_prev(st, P, nparts, k);
+  const double beta = (rho != 0.0) ? rz / rho : 0.0;
+  const int q = k & 1;
+  bool ru = false;
+  if (mixed) {
+    ru = reliable_decision(st, rr, k);
+    if (ru) {
+      const double rr64 = sum_parts(P.rr64, nparts);
+      if (rr64 <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr64)) {
+        if (lead) {
+          st->done = 1;
+          st->converged = rr64 <= st->tol2;
+          st->iter = k + 1;
+          st->rr_out = rr64;
+          st->nb_ru += 1;
+        }
+        return;
+      }
+      if (lead) {
+        st->ru[q] = 1;
+        st->rr_ref[q] = rr64;
+        st->rr_out = rr64;
+        st->nb_ru += 1;
+      }
+    }
+    else if (lead) {
+      st->ru[q] = 0;
+      st->rr_ref[q] = st->rr_ref[(k - 1) & 1];
+      st->rr_out = rr;
+    }
+  }
+  else {
+    if (rr <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr)) {
+      if (lead) {
+        st->done = 1;
+        st->converged = rr <= st->tol2;
+        st->iter = k + 1;
+        st->rr_out = rr;
+      }
+      return;
+    }
+    if (lead) {
+      st->ru[q] = 0;
+      st->rr_out = rr;
+    }
+  }
+  if (lead)
+    st->iter = k + 1;
+  const T beta_t = (T)beta;
+  int64_t t0, t1;
+  chunk_of((nunits + kBlock - 1) / kBlock, t0, t1);
+  double rzn = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t u = t * kBlock + threadIdx.x;
+    if (u >= nunits)
+      continue;
+    T zv[NB];
+    if (ru) {
+      T rv[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        rv[i] = (T)r64[u * NB + i];
+        r[u * NB + i] = rv[i];
+      }
+      apply_jacobi<NB, T>(u, dinv, rv, zv, precond);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        z[u * NB + i] = zv[i];
+        rzn += (double)rv[i] * (double)zv[i];
+      }
+    }
+    else {
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        zv[i] = z[u * NB + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      p[u * NB + i] = zv[i] + beta_t * p[u * NB + i];
+  }
+  if (ru)
+    put_part(rzn, P.rzru[q]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_true_residual_finalize(Parts P, int nparts, State* st)
+{
+  const double rr = sum_parts(P.rr64, nparts);
+  if (threadIdx.x == 0)
+    st->rr_true = rr;
+}
 
 } // namespace afem
 
 // ---- workspace --------------------------------------------------------------------------------
 
+enum SpmvFormat
+{
+  kFmtCsr = 0,
+  kFmtSell = 1
+};
+
 struct afem_solver_ws
 {
   int32_t n = 0;
-  int64_t nnz = 0;
   int block = 1;
+  const int32_t* rp = nullptr; // pattern the derived data was built from
+  const int32_t* ci = nullptr;
+  int64_t nnz = 0;
   afem::DBuf<double> r64, z64, p64, q64, dinv64;
   afem::DBuf<float> r32, z32, p32, q32, dinv32, vals32;
-  afem::DBuf<double> partials;
-  afem::DBuf<unsigned> counter;
-  afem::DBuf<afem::Scalars> sc;
-  afem::Scalars* host_sc = nullptr; // pinned
+  afem::DBuf<double> parts;
+  afem::DBuf<afem::State> st;
+  afem::State* host_st = nullptr; // pinned
+  // SELL copy
+  afem::DBuf<int32_t> sell_off, sell_col;
+  afem::DBuf<double> sell_v64;
+  afem::DBuf<float> sell_v32;
   bool has_fp32 = false;
-  bool derived_valid = false;
+  bool values_valid = false;
+  bool pattern_valid = false;
+  int format = -1; // chosen SpMV format
+  int lanes = 16;
   ~afem_solver_ws()
   {
-    if (host_sc)
-      (void)hipHostFree(host_sc);
+    if (host_st)
+      (void)hipHostFree(host_st);
   }
 };
 
@@ -619,11 +730,27 @@ void system_release_workspace(afem_system* s)
   s->ws = nullptr;
 }
 
-static int grid_blocks()
+static int solver_blocks(int64_t units)
 {
-  ensure_init();
-  int cu = rt().num_cu > 0 ? rt().num_cu : 256;
-  return cu * 8;
+  int64_t tiles = (units + kBlock - 1) / kBlock;
+  if (tiles < 1)
+    tiles = 1;
+  return (int)(tiles < kMaxBlocks ? tiles : kMaxBlocks);
+}
+
+static Parts parts_of(afem_solver_ws* w)
+{
+  double* b = w->parts.p;
+  Parts P;
+  P.pq = b;
+  P.rz[0] = b + kMaxBlocks;
+  P.rz[1] = b + 2 * kMaxBlocks;
+  P.rzru[0] = b + 3 * kMaxBlocks;
+  P.rzru[1] = b + 4 * kMaxBlocks;
+  P.rr = b + 5 * kMaxBlocks;
+  P.rr64 = b + 6 * kMaxBlocks;
+  P.bb = b + 7 * kMaxBlocks;
+  return P;
 }
 
 static afem_solver_ws* workspace(afem_system* s)
@@ -641,24 +768,65 @@ static afem_solver_ws* workspace(afem_system* s)
     w->p64.alloc(n);
     w->q64.alloc(n);
     w->dinv64.alloc((size_t)n * s->block);
+    w->dinv32.alloc((size_t)n * s->block);
     w->has_fp32 = false;
-    w->derived_valid = false;
+    w->values_valid = false;
+    w->pattern_valid = false;
+    w->format = -1;
   }
-  if (!w->partials.p) {
-    w->partials.alloc((size_t)grid_blocks() * 4);
-    w->counter.alloc(1);
-    w->counter.zero();
-    w->sc.alloc(1);
-    AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w->host_sc), sizeof(Scalars), hipHostMallocDefault));
+  if (!w->parts.p) {
+    w->parts.alloc((size_t)8 * kMaxBlocks);
+    w->parts.zero();
+    w->st.alloc(1);
+    AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w->host_st), sizeof(State), hipHostMallocDefault));
+  }
+  if (w->rp != s->row_index || w->ci != s->columns || w->nnz != s->nnz) {
+    w->rp = s->row_index;
+    w->ci = s->columns;
+    w->nnz = s->nnz;
+    w->pattern_valid = false;
+    w->values_valid = false;
+    w->format = -1;
   }
   if (s->matrix_changed) {
-    w->derived_valid = false;
+    w->values_valid = false;
     s->matrix_changed = false;
   }
   return w;
 }
 
-// Diagonal (or block-diagonal) inverse and the FP32 copy of the matrix.
+// SELL-64 pattern (slice offsets and columns) from the system's CSR/BSR view.
+static void build_sell_pattern(afem_system* s, afem_solver_ws* w)
+{
+  const int32_t nrows = s->nb_dof / s->block;
+  const int32_t nslice = (nrows + 63) / 64;
+  DBuf<int32_t> width((size_t)nslice);
+  hipLaunchKernelGGL(k_sell_width, dim3(grid_for(nslice, 256)), dim3(256), 0, stream(), nrows, s->row_index, width.p);
+  check_launch("sell_width");
+  w->sell_off.alloc((size_t)nslice + 1);
+  const int64_t cols = exclusive_scan_i32(width.p, w->sell_off.p, nslice);
+  AFEM_REQUIRE(cols * 64 * s->block * s->block < (1LL << 31), AFEM_ERR_ARGUMENT, "matrix too large for SELL offsets");
+  w->sell_col.alloc((size_t)cols * 64);
+  w->sell_v64.alloc((size_t)cols * 64 * s->block * s->block);
+  w->sell_v32.release();
+}
+
+static void fill_sell(afem_system* s, afem_solver_ws* w, bool columns)
+{
+  const int32_t nrows = s->nb_dof / s->block;
+  const int64_t rows_pad = ((int64_t)nrows + 63) / 64 * 64;
+  float* v32 = w->has_fp32 ? w->sell_v32.p : nullptr;
+  if (s->block == 3)
+    hipLaunchKernelGGL(k_sell_fill<3>, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows, s->row_index,
+                       s->columns, s->values, w->sell_off.p, w->sell_col.p, w->sell_v64.p, v32, columns ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_sell_fill<1>, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows, s->row_index,
+                       s->columns, s->values, w->sell_off.p, w->sell_col.p, w->sell_v64.p, v32, columns ? 1 : 0);
+  check_launch("sell_fill");
+}
+
+// Diagonal (or block-diagonal) inverse, FP32 copies and the SELL copy of
+// the current matrix values.
 static void ensure_derived(afem_system* s, afem_solver_ws* w, bool need_fp32)
 {
   const int32_t n = s->nb_dof;
@@ -667,155 +835,231 @@ static void ensure_derived(afem_system* s, afem_solver_ws* w, bool need_fp32)
     w->z32.alloc(n);
     w->p32.alloc(n);
     w->q32.alloc(n);
+    w->vals32.alloc((size_t)s->nnz);
+    if (w->sell_v64.p)
+      w->sell_v32.alloc(w->sell_v64.n);
     w->has_fp32 = true;
-    w->derived_valid = false;
+    w->values_valid = false;
   }
-  if (w->derived_valid && w->nnz == s->nnz)
+  if (w->values_valid)
     return;
-  w->dinv32.reserve((size_t)n * s->block);
-  if (w->has_fp32)
-    w->vals32.reserve((size_t)s->nnz);
   if (s->block == 3) {
-    const int32_t nb_node = s->nb_dof / 3;
+    const int32_t nb_node = n / 3;
     hipLaunchKernelGGL(k_block_inverse3, dim3(grid_for(nb_node, 256)), dim3(256), 0, stream(), nb_node, s->row_index,
                        s->columns, s->values, w->dinv64.p, w->dinv32.p);
   }
   else {
-    hipLaunchKernelGGL(k_diag_inverse, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
-                       s->row_index, s->columns, s->values, w->dinv64.p, w->dinv32.p);
+    hipLaunchKernelGGL(k_diag_inverse, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, s->row_index, s->columns,
+                       s->values, w->dinv64.p, w->dinv32.p);
   }
   check_launch("jacobi_setup");
   if (w->has_fp32) {
     hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, stream(), (int64_t)s->nnz, s->values, w->vals32.p);
     check_launch("to_float");
   }
-  w->nnz = s->nnz;
-  w->derived_valid = true;
+  if (w->pattern_valid && w->sell_v64.p) {
+    if (w->has_fp32 && !w->sell_v32.p)
+      w->sell_v32.alloc(w->sell_v64.n);
+    fill_sell(s, w, false);
+  }
+  w->values_valid = true;
+}
+
+static void ensure_sell(afem_system* s, afem_solver_ws* w)
+{
+  if (w->pattern_valid && w->sell_v64.p)
+    return;
+  build_sell_pattern(s, w);
+  if (w->has_fp32)
+    w->sell_v32.alloc(w->sell_v64.n);
+  fill_sell(s, w, true);
+  w->pattern_valid = true;
 }
 
 // ---- SpMV dispatch ---------------------------------------------------------------------------
 
-template <class VT, class XT, int MODE, class Fin>
-static void spmv_launch(const afem_system* s, const VT* vals, const XT* x, XT* y, const double* b, int gate,
-                        const RedCtx& ctx, Fin fin, int lanes, const char* prof_name)
+template <class VT, class XT, int MODE>
+static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* x, XT* y, const double* b, Gate gate,
+                        double* part_out, int nblk, const char* prof_name)
 {
-  const int nblk = grid_blocks();
   ProfScope prof(prof_name);
+  const VT* csr_vals;
+  const VT* sell_vals;
+  if constexpr (sizeof(VT) == 4) {
+    csr_vals = w->vals32.p;
+    sell_vals = w->sell_v32.p;
+  }
+  else {
+    csr_vals = s->values;
+    sell_vals = w->sell_v64.p;
+  }
+  (void)fp32;
   if (s->block == 3) {
     const int32_t nb_node = s->nb_dof / 3;
-    constexpr int G = 16;
-    const int64_t ntiles = ((int64_t)nb_node + kBlock / G - 1) / (kBlock / G);
-    const int64_t grid = ntiles < nblk ? ntiles : nblk;
-    const int64_t chunk = (ntiles + grid - 1) / grid;
-    hipLaunchKernelGGL((k_spmv_bsr3<G, VT, XT, MODE, Fin>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nb_node,
-                       s->row_index, s->columns, vals, x, y, b, chunk, gate, ctx, fin);
+    if (w->format == kFmtSell)
+      hipLaunchKernelGGL((k_spmv_sell3<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node, w->sell_off.p,
+                         w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+    else
+      hipLaunchKernelGGL((k_spmv_bsr3<16, VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node,
+                         s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
+  }
+  else if (w->format == kFmtSell) {
+    hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof, w->sell_off.p,
+                       w->sell_col.p, sell_vals, x, y, b, gate, part_out);
   }
   else {
     auto go = [&](auto gtag) {
       constexpr int G = decltype(gtag)::value;
-      const int64_t ntiles = ((int64_t)s->nb_dof + kBlock / G - 1) / (kBlock / G);
-      const int64_t grid = ntiles < nblk ? ntiles : nblk;
-      const int64_t chunk = (ntiles + grid - 1) / grid;
-      hipLaunchKernelGGL((k_spmv_csr<G, VT, XT, MODE, Fin>), dim3((unsigned)grid), dim3(kBlock), 0, stream(),
-                         s->nb_dof, s->row_index, s->columns, vals, x, y, b, chunk, gate, ctx, fin);
+      hipLaunchKernelGGL((k_spmv_csr<G, VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                         s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
     };
-    switch (lanes) {
+    switch (w->lanes) {
     case 4: go(std::integral_constant<int, 4>{}); break;
     case 8: go(std::integral_constant<int, 8>{}); break;
     case 32: go(std::integral_constant<int, 32>{}); break;
-    case 64: go(std::integral_constant<int, 64>{}); break;
     default: go(std::integral_constant<int, 16>{}); break;
     }
   }
   check_launch(prof_name);
 }
 
-// Lanes per row from the mean row length (power of two, 4..64).
-static int auto_lanes(const afem_system* s)
+static int spmv_blocks(afem_system* s)
 {
-  const double mean = (double)s->nnz / (double)(s->nb_dof > 0 ? s->nb_dof : 1);
-  int g = 4;
-  while (g < 64 && g < mean)
-    g <<= 1;
-  return g;
+  return solver_blocks(s->nb_dof / s->block);
+}
+
+// Picks the SpMV format: times FP64 SpMV with each candidate (3 launches,
+// best of) and keeps the fastest.  A positive spmv_lanes forces CSR with that
+// many lanes per row, -1 forces SELL-64.
+static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
+{
+  if (forced > 0) {
+    w->format = kFmtCsr;
+    w->lanes = forced;
+    return;
+  }
+  ensure_sell(s, w);
+  if (forced < 0) {
+    w->format = kFmtSell;
+    return;
+  }
+  if (w->format >= 0)
+    return;
+  struct Cand
+  {
+    int format, lanes;
+  };
+  std::vector<Cand> cands;
+  if (s->block == 3)
+    cands = { { kFmtCsr, 16 }, { kFmtSell, 0 } };
+  else
+    cands = { { kFmtCsr, 8 }, { kFmtCsr, 16 }, { kFmtSell, 0 } };
+  hipEvent_t e0, e1;
+  AFEM_HIP(hipEventCreate(&e0));
+  AFEM_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  Cand pick = cands.back();
+  const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
+  for (const Cand& c : cands) {
+    w->format = c.format;
+    w->lanes = c.lanes;
+    float t_best = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      AFEM_HIP(hipEventRecord(e0, stream()));
+      spmv_launch<double, double, kPlain>(s, w, false, w->p64.p, w->q64.p, nullptr, always, nullptr, spmv_blocks(s),
+                                          "spmv_tune");
+      AFEM_HIP(hipEventRecord(e1, stream()));
+      AFEM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      AFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0 && ms < t_best)
+        t_best = ms;
+    }
+    if (t_best < best) {
+      best = t_best;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  w->format = pick.format;
+  w->lanes = pick.lanes;
 }
 
 // ---- solve ----------------------------------------------------------------------------------
 
 template <int NB>
-static void launch_init(afem_system* s, afem_solver_ws* w, bool mixed, int precond, const RedCtx& ctx)
+static void launch_init(afem_system* s, afem_solver_ws* w, bool mixed, int precond, const Parts& P, int nblk)
 {
   const int64_t nunits = s->nb_dof / NB;
-  const int64_t tiles = (nunits + kBlock - 1) / kBlock;
-  const int64_t grid = tiles < grid_blocks() ? tiles : grid_blocks();
-  const int64_t chunk = (tiles + grid - 1) / grid;
   if (mixed)
-    hipLaunchKernelGGL((k_init<NB, float>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nunits, chunk, w->r64.p,
-                       s->rhs.p, w->r32.p, w->z32.p, w->p32.p, w->dinv32.p, precond, ctx);
+    hipLaunchKernelGGL((k_init<NB, float>), dim3(nblk), dim3(kBlock), 0, stream(), nunits, w->r64.p, s->rhs.p,
+                       w->r32.p, w->z32.p, w->p32.p, w->dinv32.p, precond, P);
   else
-    hipLaunchKernelGGL((k_init<NB, double>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nunits, chunk, w->r64.p,
-                       s->rhs.p, w->r64.p, w->z64.p, w->p64.p, w->dinv64.p, precond, ctx);
+    hipLaunchKernelGGL((k_init<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), nunits, w->r64.p, s->rhs.p,
+                       w->r64.p, w->z64.p, w->p64.p, w->dinv64.p, precond, P);
   check_launch("pcg_init");
+  hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kBlock), 0, stream(), P, nblk, w->st.p);
+  check_launch("pcg_init_finalize");
 }
 
 template <int NB>
-static void launch_iteration(afem_system* s, afem_solver_ws* w, bool mixed, int precond, int lanes, const RedCtx& ctx)
+static void launch_iteration(afem_system* s, afem_solver_ws* w, int k, bool mixed, int precond, const Parts& P,
+                             int nblk)
 {
   const int64_t nunits = s->nb_dof / NB;
-  const int64_t tiles = (nunits + kBlock - 1) / kBlock;
-  const int64_t grid = tiles < grid_blocks() ? tiles : grid_blocks();
-  const int64_t chunk = (tiles + grid - 1) / grid;
+  const Gate until_done{ w->st.p, kUntilDone, k, nullptr, nblk };
   if (!mixed) {
-    spmv_launch<double, double, kPQ>(s, s->values, w->p64.p, w->q64.p, nullptr, kUntilDone, ctx, FinAlpha{}, lanes,
-                                     "spmv_fp64");
+    spmv_launch<double, double, kPQ>(s, w, false, w->p64.p, w->q64.p, nullptr, until_done, P.pq, nblk, "spmv_fp64");
     {
       ProfScope prof("pcg_update_fp64");
-      hipLaunchKernelGGL((k_update<NB, double, FinUpdate>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nunits,
-                         chunk, s->x.p, w->r64.p, w->z64.p, w->p64.p, w->q64.p, w->dinv64.p, precond, ctx, FinUpdate{});
+      hipLaunchKernelGGL((k_update<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), k, nunits, s->x.p, w->r64.p,
+                         w->z64.p, w->p64.p, w->q64.p, w->dinv64.p, precond, P, nblk, w->st.p);
       check_launch("pcg_update");
     }
     {
       ProfScope prof("pcg_pupdate_fp64");
-      hipLaunchKernelGGL((k_pupdate<NB, double>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nunits, chunk,
-                         w->p64.p, w->z64.p, w->r64.p, (const double*)nullptr, w->dinv64.p, precond, ctx);
+      hipLaunchKernelGGL((k_pupdate<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), k, 0, nunits, w->p64.p,
+                         w->z64.p, w->r64.p, (const double*)nullptr, w->dinv64.p, precond, P, nblk, w->st.p);
       check_launch("pcg_pupdate");
     }
   }
   else {
-    spmv_launch<float, float, kPQ>(s, w->vals32.p, w->p32.p, w->q32.p, nullptr, kUntilDone, ctx, FinAlpha{}, lanes,
-                                   "spmv_fp32");
+    spmv_launch<float, float, kPQ>(s, w, true, w->p32.p, w->q32.p, nullptr, until_done, P.pq, nblk, "spmv_fp32");
     {
       ProfScope prof("pcg_update_fp32");
-      hipLaunchKernelGGL((k_update<NB, float, FinUpdateMixed>), dim3((unsigned)grid), dim3(kBlock), 0, stream(),
-                         nunits, chunk, s->x.p, w->r32.p, w->z32.p, w->p32.p, w->q32.p, w->dinv32.p, precond, ctx,
-                         FinUpdateMixed{});
+      hipLaunchKernelGGL((k_update<NB, float>), dim3(nblk), dim3(kBlock), 0, stream(), k, nunits, s->x.p, w->r32.p,
+                         w->z32.p, w->p32.p, w->q32.p, w->dinv32.p, precond, P, nblk, w->st.p);
       check_launch("pcg_update_mixed");
     }
-    spmv_launch<double, double, kResidual>(s, s->values, s->x.p, w->r64.p, s->rhs.p, kReliableUpdate, ctx,
-                                           FinReliable{}, lanes, "spmv_residual_fp64");
+    const Gate reliable{ w->st.p, kReliable, k, P.rr, nblk };
+    spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, reliable, P.rr64, nblk,
+                                           "spmv_residual_fp64");
     {
       ProfScope prof("pcg_pupdate_fp32");
-      hipLaunchKernelGGL((k_pupdate<NB, float>), dim3((unsigned)grid), dim3(kBlock), 0, stream(), nunits, chunk,
-                         w->p32.p, w->z32.p, w->r32.p, (const double*)w->r64.p, w->dinv32.p, precond, ctx);
+      hipLaunchKernelGGL((k_pupdate<NB, float>), dim3(nblk), dim3(kBlock), 0, stream(), k, 1, nunits, w->p32.p,
+                         w->z32.p, w->r32.p, (const double*)w->r64.p, w->dinv32.p, precond, P, nblk, w->st.p);
       check_launch("pcg_pupdate_mixed");
     }
   }
 }
 
-static void read_scalars(afem_solver_ws* w)
+static void read_state(afem_solver_ws* w)
 {
-  AFEM_HIP(hipMemcpyAsync(w->host_sc, w->sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipMemcpyAsync(w->host_st, w->st.p, sizeof(State), hipMemcpyDeviceToHost, stream()));
   AFEM_HIP(hipStreamSynchronize(stream()));
 }
 
-static double true_relres(afem_system* s, afem_solver_ws* w, const RedCtx& ctx, int lanes)
+// ||b - A x||^2 with the FP64 matrix, into st->rr_true (r64 overwritten).
+static double true_rr(afem_system* s, afem_solver_ws* w, const Parts& P, int nblk)
 {
-  spmv_launch<double, double, kResidual>(s, s->values, s->x.p, w->r64.p, s->rhs.p, kAlways, ctx, FinTrueResidual{},
-                                         lanes, "spmv_residual_fp64");
-  read_scalars(w);
-  const double bb = w->host_sc->bb;
-  const double rr = w->host_sc->rr;
-  return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  const Gate always{ w->st.p, kAlways, 0, nullptr, nblk };
+  spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, always, P.rr64, nblk,
+                                         "spmv_residual_fp64");
+  hipLaunchKernelGGL(k_true_residual_finalize, dim3(1), dim3(kBlock), 0, stream(), P, nblk, w->st.p);
+  check_launch("true_residual_finalize");
+  read_state(w);
+  return w->host_st->rr_true;
 }
 
 static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result* res)
@@ -825,62 +1069,64 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   AFEM_REQUIRE(o.preconditioner == AFEM_PRECOND_JACOBI || o.preconditioner == AFEM_PRECOND_NONE, AFEM_ERR_ARGUMENT,
                "unknown preconditioner");
   AFEM_REQUIRE(o.rtol >= 0.0 && o.atol >= 0.0 && o.max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
+  auto t_start = std::chrono::steady_clock::now();
   afem_solver_ws* w = workspace(s);
   const bool mixed = o.method == AFEM_SOLVER_PCG_MIXED;
   const int precond = (o.method == AFEM_SOLVER_CG_FP64 || o.preconditioner == AFEM_PRECOND_NONE) ? 0 : 1;
-  const int lanes = o.spmv_lanes > 0 ? o.spmv_lanes : auto_lanes(s);
   const int check_every = o.check_every > 0 ? o.check_every : 16;
   const double delta = o.reliable_delta > 0.0 ? o.reliable_delta : 0.1;
   ensure_derived(s, w, mixed);
-  RedCtx ctx{ w->partials.p, w->counter.p, w->sc.p };
+  choose_format(s, w, o.spmv_lanes);
+  ensure_derived(s, w, mixed);
+  const Parts P = parts_of(w);
+  const int nblk = spmv_blocks(s);
 
-  auto t_start = std::chrono::steady_clock::now();
   if (!o.use_initial_guess) {
     hipLaunchKernelGGL(k_initial_guess, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
                        s->elim_info.p, s->elim_value.p, s->x.p);
     check_launch("initial_guess");
   }
-  Scalars init{};
+  State init{};
   init.rtol2 = o.rtol * o.rtol;
   init.atol2 = o.atol * o.atol;
   init.delta2 = delta * delta;
-  init.max_iter = o.max_iter;
-  int total_iter = 0;
-  int restarts = 0;
-  int nb_ru = 0;
+  int total_iter = 0, restarts = 0, nb_ru = 0;
   bool converged = false;
   double relres = 0.0;
+  double bnorm = 0.0;
   for (;;) {
-    init.iter = 0;
     init.max_iter = o.max_iter - total_iter;
-    *w->host_sc = init;
-    AFEM_HIP(hipMemcpyAsync(w->sc.p, w->host_sc, sizeof(Scalars), hipMemcpyHostToDevice, stream()));
-    // r64 = b - A x0
-    spmv_launch<double, double, kResidual>(s, s->values, s->x.p, w->r64.p, s->rhs.p, kAlways, ctx, FinTrueResidual{},
-                                           lanes, "spmv_residual_fp64");
+    *w->host_st = init;
+    AFEM_HIP(hipMemcpyAsync(w->st.p, w->host_st, sizeof(State), hipMemcpyHostToDevice, stream()));
+    const Gate always{ w->st.p, kAlways, 0, nullptr, nblk };
+    spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, always, P.rr64, nblk,
+                                           "spmv_residual_fp64");
     if (s->block == 3)
-      launch_init<3>(s, w, mixed, precond, ctx);
+      launch_init<3>(s, w, mixed, precond, P, nblk);
     else
-      launch_init<1>(s, w, mixed, precond, ctx);
-    read_scalars(w);
-    while (!w->host_sc->done && w->host_sc->iter < init.max_iter) {
-      for (int k = 0; k < check_every; ++k) {
+      launch_init<1>(s, w, mixed, precond, P, nblk);
+    read_state(w);
+    int k = 0;
+    while (!w->host_st->done && k < init.max_iter) {
+      for (int j = 0; j < check_every && k < init.max_iter; ++j, ++k) {
         if (s->block == 3)
-          launch_iteration<3>(s, w, mixed, precond, lanes, ctx);
+          launch_iteration<3>(s, w, k, mixed, precond, P, nblk);
         else
-          launch_iteration<1>(s, w, mixed, precond, lanes, ctx);
+          launch_iteration<1>(s, w, k, mixed, precond, P, nblk);
       }
-      read_scalars(w);
+      read_state(w);
     }
-    total_iter += w->host_sc->iter;
-    nb_ru += w->host_sc->nb_ru;
-    relres = true_relres(s, w, ctx, lanes);
-    const double bnorm = std::sqrt(w->host_sc->bb);
-    converged = relres <= o.rtol || relres * bnorm <= o.atol;
-    if (converged || total_iter >= o.max_iter || restarts >= 8 || !std::isfinite(relres))
+    total_iter += w->host_st->iter;
+    nb_ru += w->host_st->nb_ru;
+    bnorm = std::sqrt(w->host_st->bb);
+    const double rr = true_rr(s, w, P, nblk);
+    relres = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+    converged = relres <= o.rtol || std::sqrt(rr) <= o.atol;
+    if (converged || total_iter >= o.max_iter || restarts >= 8 || !std::isfinite(relres) ||
+        w->host_st->iter == 0)
       break;
-    // the recursive residual drifted below the tolerance while the true one
-    // did not: restart from the current iterate
+    // the recursive residual met the tolerance but the true one does not:
+    // restart from the current iterate
     ++restarts;
   }
   AFEM_HIP(hipStreamSynchronize(stream()));
@@ -937,9 +1183,11 @@ AFEM_API int afem_system_spmv(afem_system_t sys, const double* x, double* y)
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(sys && x && y, AFEM_ERR_ARGUMENT, "null argument");
   afem_solver_ws* w = workspace(sys);
-  RedCtx ctx{ w->partials.p, w->counter.p, w->sc.p };
-  spmv_launch<double, double, kPlain>(sys, sys->values, x, y, nullptr, kAlways, ctx, NoFin{}, auto_lanes(sys),
-                                      "spmv_fp64");
+  ensure_derived(sys, w, false);
+  choose_format(sys, w, 0);
+  ensure_derived(sys, w, false);
+  const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
+  spmv_launch<double, double, kPlain>(sys, w, false, x, y, nullptr, always, nullptr, spmv_blocks(sys), "spmv_fp64");
   AFEM_GUARD_END
 }
 
@@ -949,9 +1197,10 @@ AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y)
   AFEM_REQUIRE(sys && x && y, AFEM_ERR_ARGUMENT, "null argument");
   afem_solver_ws* w = workspace(sys);
   ensure_derived(sys, w, true);
-  RedCtx ctx{ w->partials.p, w->counter.p, w->sc.p };
-  spmv_launch<float, float, kPlain>(sys, w->vals32.p, x, y, nullptr, kAlways, ctx, NoFin{}, auto_lanes(sys),
-                                    "spmv_fp32");
+  choose_format(sys, w, 0);
+  ensure_derived(sys, w, true);
+  const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
+  spmv_launch<float, float, kPlain>(sys, w, true, x, y, nullptr, always, nullptr, spmv_blocks(sys), "spmv_fp32");
   AFEM_GUARD_END
 }
 

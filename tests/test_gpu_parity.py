@@ -36,7 +36,8 @@ def row_scaled_error(row, ref, got, nb=1):
     scale = np.zeros(len(lens))
     np.maximum.at(scale, rid, np.abs(ref).reshape(len(rid), -1).max(axis=1))
     err = np.abs(got - ref).reshape(len(rid), -1).max(axis=1)
-    worst = (err / np.where(scale > 0, scale, 1.0)).max()
+    s = scale[rid]
+    worst = (err / np.where(s > 0, s, 1.0)).max()
     return worst
 
 
@@ -274,15 +275,19 @@ def poisson_bottom_system(A, dim, n, alpha, lam=None):
     return X, C, m, b, s, dl
 
 
+SPMV_FORMATS = [0, -1, 8]  # autotuned, SELL-64, CSR with 8 lanes per row
+
+
+@pytest.mark.parametrize("fmt", SPMV_FORMATS)
 @pytest.mark.parametrize("method", ["fp64", "mixed", "cg"])
-def test_pcg_solves_match_direct(afem_gpu, method):
+def test_pcg_solves_match_direct(afem_gpu, method, fmt):
     A = afem_gpu
-    print(f"[case] pcg {method}", flush=True)
+    print(f"[case] pcg {method} spmv_lanes={fmt}", flush=True)
     lam = np.exp(np.random.default_rng(4).normal(size=6 * 10 ** 3))
     X, C, m, b, s, _ = poisson_bottom_system(A, 3, 10, 0.15, lam)
     N = X.shape[0]
     code = {"fp64": A.SOLVER_PCG_FP64, "mixed": A.SOLVER_PCG_MIXED, "cg": A.SOLVER_CG_FP64}[method]
-    r = s.solve(method=code, rtol=1e-10, max_iter=5000)
+    r = s.solve(method=code, rtol=1e-10, max_iter=5000, spmv_lanes=fmt)
     assert r.converged == 1 and r.relres <= 1e-10
     row, col, val, _ = b.download()
     rhs = s.rhs().numpy()
@@ -298,9 +303,10 @@ def test_pcg_solves_match_direct(afem_gpu, method):
         assert r.reliable_updates >= 1
 
 
-def test_block_jacobi_pcg_elasticity(afem_gpu):
+@pytest.mark.parametrize("fmt", [-1, 16])
+def test_block_jacobi_pcg_elasticity(afem_gpu, fmt):
     A = afem_gpu
-    print("[case] elasticity pcg", flush=True)
+    print(f"[case] elasticity pcg spmv_lanes={fmt}", flush=True)
     X, C = box(3, 6, 0.15)
     N = X.shape[0]
     lam, mu = O.lame(21e5, 0.28)
@@ -319,7 +325,7 @@ def test_block_jacobi_pcg_elasticity(afem_gpu):
     s.apply_matrix_transformation()
     s.apply_rhs_transformation()
     for method in (A.SOLVER_PCG_FP64, A.SOLVER_PCG_MIXED):
-        r = s.solve(method=method, rtol=1e-8, max_iter=20000)
+        r = s.solve(method=method, rtol=1e-8, max_iter=20000, spmv_lanes=fmt)
         assert r.converged == 1 and r.relres <= 1e-8
         row, col, val, _ = b.download()
         S = sp.bsr_matrix((val.reshape(-1, 3, 3), col, row), shape=(3 * N, 3 * N)).tocsr()
@@ -379,7 +385,8 @@ def test_golden_elasticity_bar_gpu(afem_gpu, per_block):
     s.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, d["group_surfaceright"], 3, 1, 1.0, 0.0)
     s.apply_matrix_transformation()
     s.apply_rhs_transformation()
-    r = s.solve(method=A.SOLVER_PCG_FP64, rtol=1e-13, max_iter=20000)
+    # entries ~1e6 against ||b|| ~ 1: the FP64 true residual floors near 3e-10
+    r = s.solve(method=A.SOLVER_PCG_FP64, rtol=1e-9, max_iter=20000)
     assert r.converged == 1
     got = by_uid(d, s.solution().numpy(), 3)
     assert O.count_differences(d["ref_value"], got, 1e-3, 1e-10) == 0
