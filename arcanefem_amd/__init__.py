@@ -41,7 +41,8 @@ class BsrInfo(ctypes.Structure):
 class SolverOptions(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("preconditioner", ctypes.c_int), ("rtol", ctypes.c_double),
                 ("atol", ctypes.c_double), ("max_iter", ctypes.c_int), ("use_initial_guess", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("reliable_delta", ctypes.c_double), ("spmv_lanes", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("reliable_delta", ctypes.c_double), ("spmv_lanes", ctypes.c_int),
+                ("norm", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):
@@ -79,6 +80,8 @@ def _declare(L):
         "afem_bsr_reset_values": (ci, [vp]),
         "afem_bsr_assemble_laplace": (ci, [vp, vp, dbl, vp]),
         "afem_bsr_assemble_elasticity": (ci, [vp, dbl, dbl]),
+        "afem_bsr_autotune_assembly": (ci, [vp, ctypes.POINTER(dbl), ctypes.POINTER(ci)]),
+        "afem_bsr_set_assembly_variant": (ci, [vp, ci]),
         "afem_bsr_device_arrays": (ci, [vp, pp, pp, pp, pp]),
         "afem_bsr_download": (ci, [vp, vp, vp, vp, vp]),
         "afem_bsr_to_csr": (ci, [vp, pp, pp, ctypes.POINTER(i32)]),
@@ -96,6 +99,7 @@ def _declare(L):
         "afem_system_apply_matrix_transformation": (ci, [vp]),
         "afem_system_apply_rhs_transformation": (ci, [vp]),
         "afem_system_clear_values": (ci, [vp]),
+        "afem_system_matrix_add_value": (ci, [vp, i32, i32, dbl, ci]),
         "afem_solver_default_options": (None, [ctypes.POINTER(SolverOptions)]),
         "afem_system_solve": (ci, [vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveResult)]),
         "afem_system_spmv": (ci, [vp, vp, vp]),
@@ -266,6 +270,14 @@ class BSRFormat:
 
     def assemble_elasticity(self, lame_lambda, lame_mu):
         check(lib().afem_bsr_assemble_elasticity(self.h, float(lame_lambda), float(lame_mu)))
+
+    def autotune_assembly(self):
+        ms, v = ctypes.c_double(), ctypes.c_int()
+        check(lib().afem_bsr_autotune_assembly(self.h, ctypes.byref(ms), ctypes.byref(v)))
+        return v.value, ms.value
+
+    def set_assembly_variant(self, variant):
+        check(lib().afem_bsr_set_assembly_variant(self.h, int(variant)))
 
     def download(self):
         i = self.info()

@@ -1,0 +1,247 @@
+// afem_module -- the Poisson and elasticity module flows of ArcaneFEM
+// (modules/poisson/FemModule.cc:75-146, modules/elasticity/FemModule.cc:57-154)
+// driven through the C++ DoFLinearSystem / BSRFormat layer and the MI355X
+// linear-system factory, on a reference Gmsh mesh or a structured box.
+//
+//   afem_module poisson --mesh circle_cut.msh --f 5.5 --dirichlet horizontal=0.5 \
+//       --check poisson_test_ref_circle_2D.txt --epsilon 1e-4
+//   afem_module elasticity --mesh bar_dynamic_3D.msh --E 21e5 --nu 0.28 --f -1.0 \
+//       --dirichlet surfaceleft=0.0,0.0,0.0 --dirichlet surfaceright=NULL,1.0,NULL \
+//       --check bar.3D.Dirichlet.bodyForce.txt --epsilon 1e-3 --min-value 1e-10
+//
+// Exit status 0 when the solve converged and the check (if any) passed.
+#include "femutils/BSRFormat.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
+using namespace afem::femutils;
+
+namespace {
+
+struct Args
+{
+  std::string physics, mesh, check, method = "Penalty", format = "BSR", precision = "fp64";
+  int box_dim = 0, box_n = 0;
+  double box_jitter = 0.0;
+  unsigned long long box_seed = 0;
+  std::vector<double> f;
+  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet;
+  double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
+  bool print = false;
+  bool parse_only = false; // read the mesh on the host, print a summary, exit
+};
+
+[[noreturn]] void usage(const std::string& msg)
+{
+  std::fprintf(stderr, "afem_module: %s\n", msg.c_str());
+  std::exit(2);
+}
+
+std::vector<std::string> split(const std::string& s, char sep)
+{
+  std::vector<std::string> out;
+  std::string cur;
+  for (char c : s) {
+    if (c == sep) {
+      out.push_back(cur);
+      cur.clear();
+    }
+    else
+      cur += c;
+  }
+  out.push_back(cur);
+  return out;
+}
+
+Args parse(int argc, char** argv)
+{
+  Args a;
+  if (argc < 2)
+    usage("expected 'poisson' or 'elasticity'");
+  a.physics = argv[1];
+  for (int k = 2; k < argc; ++k) {
+    std::string key = argv[k];
+    auto next = [&]() -> std::string {
+      if (k + 1 >= argc)
+        usage("missing value after " + key);
+      return argv[++k];
+    };
+    if (key == "--mesh")
+      a.mesh = next();
+    else if (key == "--box") {
+      a.box_dim = std::stoi(next());
+      a.box_n = std::stoi(next());
+      a.box_jitter = std::stod(next());
+      a.box_seed = std::stoull(next());
+    }
+    else if (key == "--f") {
+      for (auto& s : split(next(), ','))
+        a.f.push_back(s == "NULL" ? 0.0 : std::stod(s));
+    }
+    else if (key == "--dirichlet") {
+      // group=value[,value,value]; the group may itself be a plane "x=0"
+      const std::string s = next();
+      const size_t eq = s.rfind('=');
+      if (eq == std::string::npos || eq == 0 || eq + 1 == s.size())
+        usage("--dirichlet expects group=value[,value,value]");
+      a.dirichlet.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
+    }
+    else if (key == "--method")
+      a.method = next();
+    else if (key == "--format")
+      a.format = next();
+    else if (key == "--precision")
+      a.precision = next();
+    else if (key == "--E")
+      a.E = std::stod(next());
+    else if (key == "--nu")
+      a.nu = std::stod(next());
+    else if (key == "--rtol")
+      a.rtol = std::stod(next());
+    else if (key == "--check")
+      a.check = next();
+    else if (key == "--epsilon")
+      a.epsilon = std::stod(next());
+    else if (key == "--min-value")
+      a.min_value = std::stod(next());
+    else if (key == "--print")
+      a.print = true;
+    else if (key == "--parse-only")
+      a.parse_only = true;
+    else
+      usage("unknown option " + key);
+  }
+  if (a.mesh.empty() && a.box_dim == 0)
+    usage("give --mesh or --box");
+  return a;
+}
+
+// group name, or "x=0" style plane selectors for boxes
+NodeGroup group_of(const Mesh& m, const std::string& name)
+{
+  if (name.size() > 2 && name[1] == '=' && (name[0] == 'x' || name[0] == 'y' || name[0] == 'z'))
+    return m.nodesOnPlane(name[0] - 'x', std::stod(name.substr(2)));
+  return m.group(name);
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+  Args a = parse(argc, argv);
+  try {
+    if (a.parse_only) { // modules/poisson/CMakeLists.txt "2D_parse_and_exit"
+      Mesh host_mesh;
+      host_mesh.readGmsh41(a.mesh, false);
+      long long sum_uid = 0;
+      for (Int64 u : host_mesh.uniqueIds())
+        sum_uid += u;
+      double sx = 0.0;
+      for (Real x : host_mesh.coords())
+        sx += x;
+      std::printf("dim=%d nodes=%d cells=%d uid_sum=%lld coord_sum=%.17g\n", host_mesh.dimension(),
+                  host_mesh.nbNode(), host_mesh.nbCell(), sum_uid, sx);
+      for (auto& g : host_mesh.groups()) {
+        long long s = 0;
+        for (Int32 v : g.second)
+          s += v;
+        std::printf("group %s %zu %lld\n", g.first.c_str(), g.second.size(), s);
+      }
+      return 0;
+    }
+    Runner runner{ 0 };
+    check(afem_init(runner.device), "afem_init");
+    Mesh mesh;
+    if (!a.mesh.empty())
+      mesh.readGmsh41(a.mesh);
+    else
+      mesh.buildBox(a.box_dim, a.box_n, a.box_jitter, a.box_seed);
+    const bool elasticity = a.physics == "elasticity";
+    if (!elasticity && a.physics != "poisson")
+      usage("physics must be poisson or elasticity");
+    const int nb_dof = elasticity ? mesh.dimension() : 1; // modules/elasticity/FemModule.cc:174
+    const Int32 n_dofs = mesh.nbNode() * nb_dof;
+    auto t0 = std::chrono::steady_clock::now();
+
+    // compute(): linear system from the factory, BSR format with CSR values
+    Mi355xDoFLinearSystemFactory factory;
+    DoFLinearSystem ls;
+    ls.setLinearSystemFactory(&factory);
+    ls.initialize(&runner, n_dofs, "Solver");
+    CommandLineArguments solver_args;
+    char rtol_text[32]; // std::to_string would print 1e-12 as "0.000000"
+    std::snprintf(rtol_text, sizeof rtol_text, "%.17g", a.rtol);
+    solver_args.args = { "-ksp_rtol", rtol_text, "-afem_precision", a.precision };
+    ls.setSolverCommandLineArguments(solver_args);
+    BSRFormat bsr;
+    bsr.initialize(mesh, nb_dof, true, a.format == "AF-BSR");
+    bsr.computeSparsity();
+
+    // _assembleBilinearOperator
+    ElementParameters ep;
+    if (elasticity) {
+      ep.lame_mu = a.E / (2 * (1 + a.nu)); // modules/elasticity/FemModule.cc:171-172
+      ep.lame_lambda = a.E * a.nu / ((1 + a.nu) * (1 - 2 * a.nu));
+      bsr.assembleBilinear(Element::Elasticity, ep);
+    }
+    else {
+      bsr.assembleBilinear(Element::Laplace, ep);
+    }
+    bsr.toLinearSystem(ls);
+
+    // _assembleLinearOperator: source / body force, then Dirichlet
+    VariableDoFReal& rhs = ls.rhsVariable();
+    rhs.fill(0.0);
+    if (elasticity) {
+      double f3[3] = { 0.0, 0.0, 0.0 };
+      for (size_t k = 0; k < a.f.size() && k < 3; ++k)
+        f3[k] = a.f[k];
+      BoundaryConditions::applyBodyForceToRhs(f3, mesh, rhs);
+    }
+    else if (!a.f.empty()) {
+      BoundaryConditions::applyConstantSourceToRhs(a.f[0], mesh, 1, 0, rhs);
+    }
+    for (auto& d : a.dirichlet) {
+      const NodeGroup& g = group_of(mesh, d.first);
+      for (size_t c = 0; c < d.second.size() && (int)c < nb_dof; ++c) {
+        if (d.second[c] == "NULL")
+          continue;
+        BoundaryConditions::applyDirichletToLhsAndRhs(a.method, g, nb_dof, (int)c, std::stod(d.second[c]), ls);
+      }
+    }
+
+    // _solve.  Penalty rows (1e30 on the diagonal, 1e30*g in the RHS) make
+    // ||b|| meaningless, so those systems stop on the preconditioned norm
+    // ||M^-1 r|| / ||M^-1 b||, in which a penalty row weighs like any other.
+    auto* impl = dynamic_cast<Mi355xDoFLinearSystemImpl*>(ls.implementation());
+    if (a.method == "Penalty")
+      ls.setSolverCommandLineArguments(CommandLineArguments{ { "-ksp_norm_type", "preconditioned" } });
+    ls.applyLinearSystemTransformationAndSolve();
+    auto t1 = std::chrono::steady_clock::now();
+    const afem_solve_result& r = impl->lastSolve();
+    std::printf("afem_module %s: nodes=%d dofs=%d iterations=%d relres=%.3e converged=%d time_ms=%.1f\n",
+                a.physics.c_str(), mesh.nbNode(), n_dofs, r.iterations, r.relres, r.converged,
+                std::chrono::duration<double, std::milli>(t1 - t0).count());
+
+    // _updateVariables / _validateResults
+    std::vector<Real> u = ls.solutionVariable().toHost();
+    if (a.print)
+      for (Int32 i = 0; i < mesh.nbNode(); ++i) {
+        std::printf("u[%lld] =", (long long)mesh.uniqueIds()[i]);
+        for (int c = 0; c < nb_dof; ++c)
+          std::printf(" %.15g", u[(size_t)i * nb_dof + c]);
+        std::printf("\n");
+      }
+    if (!a.check.empty()) {
+      checkNodeResultFile(a.check, mesh.uniqueIds(), u, nb_dof, a.epsilon, a.min_value);
+      std::printf("CHECK OK %s\n", a.check.c_str());
+    }
+  }
+  catch (const FatalError& e) {
+    std::fprintf(stderr, "afem_module: fatal (status %d): %s\n", e.status, e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -147,6 +147,8 @@ static void bsr_compute_sparsity(afem_bsr* b)
   b->has_sparsity = true;
 }
 
+static void build_plan(afem_bsr* b);
+
 // ---- element kernels ---------------------------------------------------------------
 
 struct d3
@@ -166,6 +168,18 @@ __device__ __forceinline__ d3 cross(d3 a, d3 b)
   return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
 }
 __device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// 1/x from the hardware FP64 reciprocal estimate refined by two
+// Newton-Raphson steps, y <- y + y (1 - x y); within an ulp or two of the
+// correctly rounded quotient (x > 0 finite).
+__device__ __forceinline__ double fast_rcp(double x)
+{
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
 
 // Position of column v inside a sorted row held in LDS.
 __device__ __forceinline__ int find_col(const int32_t* __restrict__ cols, int len, int32_t v)
@@ -273,12 +287,12 @@ k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict_
           const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
           const double adet = fabs(dot(ea, Ca));
           const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
-          const double scale = lam[u] / (6.0 * adet);
+          const double scale = lam[u] * fast_rcp(6.0 * adet);
           diag += scale * dot(Cp, Cp);
           lds_add(rc, rv, len, o.x, scale * dot(Cp, Ca));
           lds_add(rc, rv, len, o.y, scale * dot(Cp, Cb));
           lds_add(rc, rv, len, o.z, scale * dot(Cp, Cc));
-          b_acc += source_f * (adet / 6.0) / 4;
+          b_acc += adet; // b_p = f * sum |det| / 24
         }
         else {
           const d3 ea = sub(pa[u], p), eb = sub(pb[u], p);
@@ -286,18 +300,18 @@ k_assemble_laplace(const double* __restrict__ coords, const int32_t* __restrict_
           const double adet = fabs(det);
           const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
           const double gpx = -(gax + gbx), gpy = -(gay + gby);
-          const double scale = lam[u] / (2.0 * adet);
+          const double scale = lam[u] * fast_rcp(2.0 * adet);
           diag += scale * (gpx * gpx + gpy * gpy);
           lds_add(rc, rv, len, o.x, scale * (gpx * gax + gpy * gay));
           lds_add(rc, rv, len, o.y, scale * (gpx * gbx + gpy * gby));
-          b_acc += source_f * (adet / 2.0) / 3;
+          b_acc += adet; // b_p = f * sum |det| / 6
         }
       }
     }
     if (dpos < len)
       rv[dpos] += diag;
     if (rhs)
-      rhs[i] = b_acc;
+      rhs[i] = source_f * b_acc / (NPC == 4 ? 24.0 : 6.0);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < cnt; k += blockDim.x)
@@ -383,7 +397,7 @@ k_assemble_elasticity(const double* __restrict__ coords, const int32_t* __restri
         const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
         const double adet = fabs(dot(ea, Ca));
         const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
-        const double inv6 = 1.0 / (6.0 * adet);
+        const double inv6 = fast_rcp(6.0 * adet);
         const double la = lame_lambda * inv6, mu = lame_mu * inv6;
         lds_add_block(rc, rv, len, i, Cp, Cp, la, mu);
         lds_add_block(rc, rv, len, o.x, Cp, Ca, la, mu);
@@ -411,39 +425,762 @@ k_assemble_elasticity(const double* __restrict__ coords, const int32_t* __restri
   }
 }
 
+// ---- assembly plan --------------------------------------------------------------------
+// Built once with the pattern: for every (row node i, cell c) pair, in
+// node->cell order, one 64-bit word
+//   bits 0-31 cell id, 32-33 slot of i in c,
+//   34-41 / 42-49 / 50-57 positions inside row i of the other nodes of c
+//   (in cell order, as returned by other_nodes),
+// and per row the position of the diagonal.  With it the assembly needs no
+// column search and no column staging.  Rows longer than 255 entries do
+// not get a plan (the searching kernels handle them).
+
+constexpr int kPlanMaxRow = 255;
+
+__device__ __forceinline__ int32_t plan_cell(uint64_t w) { return (int32_t)(w & 0xffffffffULL); }
+__device__ __forceinline__ int plan_slot(uint64_t w) { return (int)((w >> 32) & 3); }
+__device__ __forceinline__ int plan_pos(uint64_t w, int k) { return (int)((w >> (34 + 8 * k)) & 0xff); }
+
+__global__ void k_pair_plan(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                            const int32_t* __restrict__ cells, int npc, int32_t nb_node,
+                            const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                            uint64_t* __restrict__ plan, int32_t* __restrict__ diag_pos, int32_t* __restrict__ err)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t* rc = columns + row_index[i];
+  const int len = row_index[i + 1] - row_index[i];
+  const int d = find_col(rc, len, i);
+  diag_pos[i] = d;
+  bool bad = (d >= len || rc[d] != i);
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int32_t c = it >> 2;
+    const int s = it & 3;
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+    const int3 o = other_nodes(cq, s);
+    const int32_t nodes[3] = { o.x, o.y, o.z };
+    uint64_t w = (uint64_t)(uint32_t)c | ((uint64_t)s << 32);
+    for (int k = 0; k < npc - 1; ++k) {
+      const int pos = find_col(rc, len, nodes[k]);
+      bad = bad || pos >= len || rc[pos] != nodes[k] || pos > kPlanMaxRow;
+      w |= (uint64_t)(pos & 0xff) << (34 + 8 * k);
+    }
+    plan[e] = w;
+  }
+  if (bad)
+    atomicOr(err, 1);
+}
+
+// Compact plan: the other nodes as signed 16-bit deltas from the row node and
+// their 5-bit row positions, all in one word,
+//   bits 0-15 / 16-31 / 32-47 deltas, 48-52 / 53-57 / 58-62 positions,
+// so the assembly reads one word per (row, cell) pair and then the
+// coordinates, without touching the cell connectivity.  Built only when every
+// delta fits in 16 bits and every row has at most 32 entries.
+constexpr int kCompactMaxRow = 32;
+
+__device__ __forceinline__ int32_t cplan_node(uint64_t w, int32_t row, int k)
+{
+  return row + (int32_t)(int16_t)((w >> (16 * k)) & 0xffff);
+}
+__device__ __forceinline__ int cplan_pos(uint64_t w, int k) { return (int)((w >> (48 + 5 * k)) & 31); }
+
+__global__ void k_compact_plan(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+                               const int32_t* __restrict__ cells, int npc, int32_t nb_node,
+                               uint64_t* __restrict__ cplan, int32_t* __restrict__ unfit)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  bool bad = false;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const uint64_t w = plan[e];
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)plan_cell(w));
+    const int3 o = other_nodes(cq, plan_slot(w));
+    const int32_t nodes[3] = { o.x, o.y, o.z };
+    uint64_t c = 0;
+    for (int k = 0; k < 3; ++k) {
+      const int32_t d = (k < npc - 1) ? nodes[k] - i : 0;
+      const int pos = (k < npc - 1) ? plan_pos(w, k) : 0;
+      bad = bad || d < -32768 || d > 32767 || pos >= kCompactMaxRow;
+      c |= (uint64_t)(uint16_t)(int16_t)d << (16 * k);
+      c |= (uint64_t)(pos & 31) << (48 + 5 * k);
+    }
+    cplan[e] = c;
+  }
+  if (bad)
+    atomicOr(unfit, 1);
+}
+
+static void build_compact_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->cplan.release();
+  if (!b->plan.p || b->max_row > kCompactMaxRow)
+    return;
+  b->cplan.alloc(b->plan.n);
+  DBuf<int32_t> unfit(1);
+  unfit.zero();
+  hipLaunchKernelGGL(k_compact_plan, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.nc_offsets.p,
+                     b->plan.p, m.cells.p, m.npc, m.nb_node, b->cplan.p, unfit.p);
+  check_launch("compact_plan");
+  int32_t h = 0;
+  unfit.download(&h, 1);
+  if (h)
+    b->cplan.release();
+}
+
+// Laplace assembly driven by the compact plan: per (row, cell) pair one
+// 8-byte word, then the three (two) neighbour coordinates.  The plan words of
+// a row are contiguous and loaded U at a time; the cell id (for a per-cell
+// conductivity) comes from the node->cell list only when lambda is given.
+template <int NPC, int U>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace_cplan(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
+                         const int32_t* __restrict__ nc_items, const uint64_t* __restrict__ cplan,
+                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
+                         int32_t nb_node, int rows_per_block, const double* __restrict__ cell_lambda,
+                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t tile = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t r0 = tile * rows_per_block;
+  const int32_t r1 = min(r0 + rows_per_block, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  double* s_val = reinterpret_cast<double*>(smem);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    s_val[k] = 0.0;
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (threadIdx.x < rows_per_block && i < r1) {
+    double* rv = s_val + (row_index[i] - a);
+    const d3 p = load_node(coords, i);
+    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    double diag = 0.0, det_sum = 0.0;
+    for (int32_t e = e0; e < e1; e += U) {
+      uint64_t w[U];
+      double lam[U];
+      d3 pa[U], pb[U], pc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        w[u] = cplan[min(e + u, e1 - 1)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        lam[u] = cell_lambda ? cell_lambda[nc_items[min(e + u, e1 - 1)] >> 2] : 1.0;
+        pa[u] = load_node(coords, cplan_node(w[u], i, 0));
+        pb[u] = load_node(coords, cplan_node(w[u], i, 1));
+        if constexpr (NPC == 4)
+          pc[u] = load_node(coords, cplan_node(w[u], i, 2));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= e1)
+          break;
+        if constexpr (NPC == 4) {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p), ec = sub(pc[u], p);
+          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+          const double adet = fabs(dot(ea, Ca));
+          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+          const double scale = lam[u] * fast_rcp(6.0 * adet);
+          diag += scale * dot(Cp, Cp);
+          rv[cplan_pos(w[u], 0)] += scale * dot(Cp, Ca);
+          rv[cplan_pos(w[u], 1)] += scale * dot(Cp, Cb);
+          rv[cplan_pos(w[u], 2)] += scale * dot(Cp, Cc);
+          det_sum += adet;
+        }
+        else {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p);
+          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
+          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+          const double gpx = -(gax + gbx), gpy = -(gay + gby);
+          const double scale = lam[u] * fast_rcp(2.0 * adet);
+          diag += scale * (gpx * gpx + gpy * gpy);
+          rv[cplan_pos(w[u], 0)] += scale * (gpx * gax + gpy * gay);
+          rv[cplan_pos(w[u], 1)] += scale * (gpx * gbx + gpy * gby);
+          det_sum += adet;
+        }
+      }
+    }
+    rv[diag_pos[i]] += diag;
+    if (rhs)
+      rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    values[a + k] = s_val[k];
+}
+
+// ---- block-local plan ---------------------------------------------------------------------
+// Rows are split into blocks of kLocalRows consecutive rows.  The nodes a
+// block's assembly touches are exactly the union of its rows' columns; that
+// sorted set is the block's node table (at most kLocalMaxNodes entries).  The
+// local plan word of a (row, cell) pair holds the table indices of the other
+// nodes (11 bits each) and their row positions (5 bits each):
+//   bits 0-10 / 11-21 / 22-32 table indices, 33-37 / 38-42 / 43-47 positions.
+// The assembly stages the block's coordinates in LDS once and then reads one
+// word per pair from HBM.
+constexpr int kLocalRows = 256;
+constexpr int kLocalMaxNodes = 2048;
+constexpr int kLocalSort = 8192;
+
+__device__ __forceinline__ int lplan_idx(uint64_t w, int k) { return (int)((w >> (11 * k)) & 2047); }
+
+// a += v on an LDS double owned by the calling thread alone: the LDS unit
+// does the read-modify-write (ds_add_f64), so the thread does not wait for
+// the old value; with a single writer the summation order is program order.
+__device__ __forceinline__ void lds_accumulate(double* a, double v)
+{
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lplan_pos(uint64_t w, int k) { return (int)((w >> (33 + 5 * k)) & 31); }
+
+// Sorted unique columns of each block of rows (bitonic sort in LDS, then a
+// compaction with a block-wide scan).  count < 0 marks a block whose rows
+// hold more than kLocalSort entries.
+__global__ void __launch_bounds__(256)
+k_block_nodes(const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns, int32_t nb_node,
+              int32_t* __restrict__ table, int32_t* __restrict__ count)
+{
+  __shared__ int32_t buf[kLocalSort];
+  __shared__ int32_t scan[256];
+  const int tid = threadIdx.x;
+  const int32_t blk = blockIdx.x;
+  const int32_t r0 = blk * kLocalRows;
+  const int32_t r1 = min(r0 + kLocalRows, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  if (cnt > kLocalSort) {
+    if (tid == 0)
+      count[blk] = -1;
+    return;
+  }
+  int n2 = 256;
+  while (n2 < cnt)
+    n2 <<= 1;
+  for (int k = tid; k < n2; k += 256)
+    buf[k] = (k < cnt) ? columns[a + k] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = tid; k < n2 / 2; k += 256) {
+        const int i = (k / stride) * 2 * stride + (k % stride);
+        const int j = i + stride;
+        const bool asc = (i & size) == 0;
+        const int32_t x = buf[i], y = buf[j];
+        if ((x > y) == asc) {
+          buf[i] = y;
+          buf[j] = x;
+        }
+      }
+      __syncthreads();
+    }
+  // each thread owns a contiguous chunk; count first occurrences, scan, write
+  const int chunk = n2 / 256;
+  const int k0 = tid * chunk;
+  int mine = 0;
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < cnt && (k == 0 || buf[k] != buf[k - 1]))
+      ++mine;
+  scan[tid] = mine;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = (tid >= off) ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  int pos = scan[tid] - mine;
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < cnt && (k == 0 || buf[k] != buf[k - 1])) {
+      if (pos < kLocalMaxNodes)
+        table[(int64_t)blk * kLocalMaxNodes + pos] = buf[k];
+      ++pos;
+    }
+  if (tid == 255)
+    count[blk] = scan[255];
+}
+
+__device__ __forceinline__ int find_node(const int32_t* __restrict__ tab, int n, int32_t v)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_local_plan(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+             const int32_t* __restrict__ cells, int npc, int32_t nb_node, const int32_t* __restrict__ table,
+             const int32_t* __restrict__ count, uint64_t* __restrict__ lplan, int32_t* __restrict__ err)
+{
+  __shared__ int32_t tab[kLocalMaxNodes];
+  const int32_t blk = blockIdx.x;
+  const int tn = count[blk];
+  if (tn < 0 || tn > kLocalMaxNodes) {
+    if (threadIdx.x == 0)
+      atomicOr(err, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < tn; k += 256)
+    tab[k] = table[(int64_t)blk * kLocalMaxNodes + k];
+  __syncthreads();
+  const int32_t i = blk * kLocalRows + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  bool bad = false;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const uint64_t w = plan[e];
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)plan_cell(w));
+    const int3 o = other_nodes(cq, plan_slot(w));
+    const int32_t nodes[3] = { o.x, o.y, o.z };
+    uint64_t lw = 0;
+    for (int k = 0; k < npc - 1; ++k) {
+      const int l = find_node(tab, tn, nodes[k]);
+      const int pos = plan_pos(w, k);
+      bad = bad || l >= tn || tab[l] != nodes[k] || pos >= kCompactMaxRow;
+      lw |= (uint64_t)(l & 2047) << (11 * k);
+      lw |= (uint64_t)(pos & 31) << (33 + 5 * k);
+    }
+    lplan[e] = lw;
+  }
+  if (bad)
+    atomicOr(err, 1);
+}
+
+static void build_local_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->lplan.release();
+  b->block_nodes.release();
+  b->block_count.release();
+  b->block_nodes_max = 0;
+  if (!b->plan.p || b->max_row > kCompactMaxRow)
+    return;
+  const int32_t nblk = (m.nb_node + kLocalRows - 1) / kLocalRows;
+  b->block_nodes.alloc((size_t)nblk * kLocalMaxNodes);
+  b->block_count.alloc((size_t)nblk);
+  hipLaunchKernelGGL(k_block_nodes, dim3(nblk), dim3(256), 0, stream(), b->row_index.p, b->columns.p, m.nb_node,
+                     b->block_nodes.p, b->block_count.p);
+  check_launch("block_nodes");
+  std::vector<int32_t> counts((size_t)nblk);
+  b->block_count.download(counts.data(), counts.size());
+  int32_t mx = 0;
+  for (int32_t c : counts) {
+    if (c < 0 || c > kLocalMaxNodes) {
+      b->block_nodes.release();
+      b->block_count.release();
+      return;
+    }
+    mx = std::max(mx, c);
+  }
+  b->lplan.alloc(b->plan.n);
+  DBuf<int32_t> err(1);
+  err.zero();
+  hipLaunchKernelGGL(k_local_plan, dim3(nblk), dim3(256), 0, stream(), m.nc_offsets.p, b->plan.p, m.cells.p, m.npc,
+                     m.nb_node, b->block_nodes.p, b->block_count.p, b->lplan.p, err.p);
+  check_launch("local_plan");
+  int32_t h = 0;
+  err.download(&h, 1);
+  if (h) {
+    b->lplan.release();
+    b->block_nodes.release();
+    b->block_count.release();
+    return;
+  }
+  b->block_nodes_max = mx;
+}
+
+// Laplace assembly with the block's node coordinates staged in LDS (SoA):
+// per (row, cell) pair one word from HBM, then only LDS reads.
+template <int NPC, int U>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
+                         const int32_t* __restrict__ nc_items, const uint64_t* __restrict__ lplan,
+                         const int32_t* __restrict__ table, const int32_t* __restrict__ count,
+                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
+                         int32_t nb_node, int val_cap, int node_cap, const double* __restrict__ cell_lambda,
+                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t blk = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t r0 = blk * kLocalRows;
+  const int32_t r1 = min(r0 + kLocalRows, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  const int tn = count[blk];
+  double* s_val = reinterpret_cast<double*>(smem);
+  double2* s_xy = reinterpret_cast<double2*>(s_val + val_cap); // val_cap is even
+  double* s_z = reinterpret_cast<double*>(s_xy + node_cap);
+  const int32_t* tab = table + (int64_t)blk * kLocalMaxNodes;
+  for (int k = threadIdx.x; k < tn; k += blockDim.x) {
+    const int64_t g = tab[k];
+    s_xy[k] = *reinterpret_cast<const double2*>(coords + 4 * g);
+    s_z[k] = coords[4 * g + 2];
+  }
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    s_val[k] = 0.0;
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (i < r1) {
+    double* rv = s_val + (row_index[i] - a);
+    const d3 p = load_node(coords, i);
+    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    double diag = 0.0, det_sum = 0.0;
+    for (int32_t e = e0; e < e1; e += U) {
+      uint64_t w[U];
+      double lam[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        w[u] = lplan[min(e + u, e1 - 1)];
+        lam[u] = cell_lambda ? cell_lambda[nc_items[min(e + u, e1 - 1)] >> 2] : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= e1)
+          break;
+        const int la = lplan_idx(w[u], 0), lb = lplan_idx(w[u], 1);
+        const double2 xa = s_xy[la], xb = s_xy[lb];
+        const d3 ea = { xa.x - p.x, xa.y - p.y, s_z[la] - p.z };
+        const d3 eb = { xb.x - p.x, xb.y - p.y, s_z[lb] - p.z };
+        if constexpr (NPC == 4) {
+          const int lc = lplan_idx(w[u], 2);
+          const double2 xc = s_xy[lc];
+          const d3 ec = { xc.x - p.x, xc.y - p.y, s_z[lc] - p.z };
+          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+          const double adet = fabs(dot(ea, Ca));
+          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+          const double scale = lam[u] * fast_rcp(6.0 * adet);
+          diag += scale * dot(Cp, Cp);
+          lds_accumulate(rv + lplan_pos(w[u], 0), scale * dot(Cp, Ca));
+          lds_accumulate(rv + lplan_pos(w[u], 1), scale * dot(Cp, Cb));
+          lds_accumulate(rv + lplan_pos(w[u], 2), scale * dot(Cp, Cc));
+          det_sum += adet;
+        }
+        else {
+          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
+          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+          const double gpx = -(gax + gbx), gpy = -(gay + gby);
+          const double scale = lam[u] * fast_rcp(2.0 * adet);
+          diag += scale * (gpx * gpx + gpy * gpy);
+          lds_accumulate(rv + lplan_pos(w[u], 0), scale * (gpx * gax + gpy * gay));
+          lds_accumulate(rv + lplan_pos(w[u], 1), scale * (gpx * gbx + gpy * gby));
+          det_sum += adet;
+        }
+      }
+    }
+    rv[diag_pos[i]] += diag;
+    if (rhs)
+      rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    values[a + k] = s_val[k];
+}
+
+static void build_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->plan.release();
+  b->cplan.release();
+  b->diag_pos.release();
+  if (b->max_row > kPlanMaxRow)
+    return;
+  b->plan.alloc(m.nc_items.n);
+  b->diag_pos.alloc((size_t)m.nb_node);
+  DBuf<int32_t> err(1);
+  err.zero();
+  hipLaunchKernelGGL(k_pair_plan, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.nc_offsets.p,
+                     m.nc_items.p, m.cells.p, m.npc, m.nb_node, b->row_index.p, b->columns.p, b->plan.p,
+                     b->diag_pos.p, err.p);
+  check_launch("pair_plan");
+  int32_t h = 0;
+  err.download(&h, 1);
+  AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix: a cell coupling is missing from the pattern");
+  build_compact_plan(b);
+  build_local_plan(b);
+}
+
+// Laplace assembly driven by the plan: only the values are staged in LDS.
+template <int NPC, int U>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace_plan(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                        const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+                        const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
+                        int32_t nb_node, int rows_per_block, const double* __restrict__ cell_lambda,
+                        double source_f, double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t tile = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t r0 = tile * rows_per_block;
+  const int32_t r1 = min(r0 + rows_per_block, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  double* s_val = reinterpret_cast<double*>(smem);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    s_val[k] = 0.0;
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (threadIdx.x < rows_per_block && i < r1) {
+    double* rv = s_val + (row_index[i] - a);
+    const d3 p = load_node(coords, i);
+    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    double diag = 0.0, b_acc = 0.0;
+    for (int32_t e = e0; e < e1; e += U) {
+      uint64_t w[U];
+      int4 cq[U];
+      double lam[U];
+      d3 pa[U], pb[U], pc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        w[u] = plan[min(e + u, e1 - 1)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t c = plan_cell(w[u]);
+        cq[u] = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+        lam[u] = cell_lambda ? cell_lambda[c] : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int3 o = other_nodes(cq[u], plan_slot(w[u]));
+        pa[u] = load_node(coords, o.x);
+        pb[u] = load_node(coords, o.y);
+        if constexpr (NPC == 4)
+          pc[u] = load_node(coords, o.z);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= e1)
+          break;
+        if constexpr (NPC == 4) {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p), ec = sub(pc[u], p);
+          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+          const double adet = fabs(dot(ea, Ca));
+          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+          const double scale = lam[u] * fast_rcp(6.0 * adet);
+          diag += scale * dot(Cp, Cp);
+          rv[plan_pos(w[u], 0)] += scale * dot(Cp, Ca);
+          rv[plan_pos(w[u], 1)] += scale * dot(Cp, Cb);
+          rv[plan_pos(w[u], 2)] += scale * dot(Cp, Cc);
+          b_acc += adet; // b_p = f * sum |det| / 24
+        }
+        else {
+          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p);
+          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
+          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+          const double gpx = -(gax + gbx), gpy = -(gay + gby);
+          const double scale = lam[u] * fast_rcp(2.0 * adet);
+          diag += scale * (gpx * gpx + gpy * gpy);
+          rv[plan_pos(w[u], 0)] += scale * (gpx * gax + gpy * gay);
+          rv[plan_pos(w[u], 1)] += scale * (gpx * gbx + gpy * gby);
+          b_acc += adet; // b_p = f * sum |det| / 6
+        }
+      }
+    }
+    rv[diag_pos[i]] += diag;
+    if (rhs)
+      rhs[i] = source_f * b_acc / (NPC == 4 ? 24.0 : 6.0);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    values[a + k] = s_val[k];
+}
+
 // Rows per block and LDS bytes for an assembly launch; the block's values
 // range is bounded by rows_per_block * max_row.
-static void assembly_shape(const afem_bsr& b, int bytes_per_entry, int* rows_per_block, size_t* lds)
+static bool assembly_shape(const afem_bsr& b, int bytes_per_entry, int max_rows, int* rows_per_block, size_t* lds)
 {
-  int rows = 256;
+  int rows = max_rows;
   const size_t lds_budget = 64 * 1024;
   while (rows > 16 && (size_t)rows * b.max_row * bytes_per_entry > lds_budget)
     rows >>= 1;
   size_t bytes = (size_t)rows * b.max_row * bytes_per_entry;
-  AFEM_REQUIRE(bytes <= 160 * 1024, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
   *rows_per_block = rows;
   *lds = bytes;
+  return bytes <= 160 * 1024;
+}
+
+// Laplace assembly variants: searching kernel (no plan), plan kernels and
+// compact-plan kernels over (U, rows per block).
+enum PlanKind
+{
+  kSearch = 0,
+  kPlan = 1,
+  kCompact = 2,
+  kLocal = 3
+};
+struct LaplaceVariant
+{
+  int kind;
+  int u;
+  int rows;
+};
+static const LaplaceVariant kLaplaceVariants[] = {
+  { kSearch, 4, 256 },  { kPlan, 4, 256 },     { kPlan, 8, 256 },  { kCompact, 4, 256 },
+  { kCompact, 8, 128 }, { kLocal, 4, 256 },    { kLocal, 8, 256 }, { kLocal, 16, 256 },
+};
+constexpr int kNumLaplaceVariants = sizeof(kLaplaceVariants) / sizeof(kLaplaceVariants[0]);
+constexpr int kDefaultLaplaceVariant = 6;
+
+template <int NPC, int U>
+static void launch_laplace_lplan(const afem_bsr* b, const double* lam, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  const int val_cap = (kLocalRows * b->max_row + 1) & ~1; // keeps the coordinate table 16-byte aligned
+  const int node_cap = b->block_nodes_max;
+  const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap) * sizeof(double);
+  hipLaunchKernelGGL((k_assemble_laplace_lplan<NPC, U>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
+                     lds, stream(), m.coords.p, m.nc_offsets.p, m.nc_items.p, b->lplan.p, b->block_nodes.p,
+                     b->block_count.p, b->diag_pos.p, b->row_index.p, m.nb_node, val_cap, node_cap, lam, f, rhs,
+                     b->values.p);
+}
+
+template <int NPC, int U>
+static void launch_laplace_plan(const afem_bsr* b, int rows, size_t lds, const double* lam, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  hipLaunchKernelGGL((k_assemble_laplace_plan<NPC, U>), dim3(grid_for(m.nb_node, rows)), dim3(rows), lds, stream(),
+                     m.coords.p, m.cells.p, m.nc_offsets.p, b->plan.p, b->diag_pos.p, b->row_index.p, m.nb_node,
+                     rows, lam, f, rhs, b->values.p);
+}
+
+template <int NPC, int U>
+static void launch_laplace_cplan(const afem_bsr* b, int rows, size_t lds, const double* lam, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  hipLaunchKernelGGL((k_assemble_laplace_cplan<NPC, U>), dim3(grid_for(m.nb_node, rows)), dim3(rows), lds, stream(),
+                     m.coords.p, m.nc_offsets.p, m.nc_items.p, b->cplan.p, b->diag_pos.p, b->row_index.p, m.nb_node,
+                     rows, lam, f, rhs, b->values.p);
+}
+
+// The variant actually run: falls back to a plan kind the handle has.
+static LaplaceVariant effective_variant(const afem_bsr* b, int v)
+{
+  LaplaceVariant var = kLaplaceVariants[v];
+  const size_t local_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->block_nodes_max) * sizeof(double);
+  if (var.kind == kLocal && (!b->lplan.p || local_lds > 160 * 1024))
+    var.kind = kCompact;
+  if (var.kind == kCompact && !b->cplan.p)
+    var.kind = kPlan;
+  if (var.kind == kPlan && !b->plan.p)
+    var.kind = kSearch;
+  return var;
+}
+
+static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lambda, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  const LaplaceVariant var = effective_variant(b, v);
+  int rows;
+  size_t lds;
+  const bool fits = assembly_shape(*b, var.kind == kSearch ? 12 : 8, var.rows, &rows, &lds);
+  AFEM_REQUIRE(fits, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
+  ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
+  if (var.kind == kLocal) {
+    if (m.npc == 4) {
+      switch (var.u) {
+      case 4: launch_laplace_lplan<4, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<4, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<4, 8>(b, cell_lambda, f, rhs); break;
+      }
+    }
+    else {
+      switch (var.u) {
+      case 4: launch_laplace_lplan<3, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<3, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<3, 8>(b, cell_lambda, f, rhs); break;
+      }
+    }
+  }
+  else if (var.kind == kCompact) {
+    if (m.npc == 4) {
+      switch (var.u) {
+      case 4: launch_laplace_cplan<4, 4>(b, rows, lds, cell_lambda, f, rhs); break;
+      default: launch_laplace_cplan<4, 8>(b, rows, lds, cell_lambda, f, rhs); break;
+      }
+    }
+    else {
+      switch (var.u) {
+      case 4: launch_laplace_cplan<3, 4>(b, rows, lds, cell_lambda, f, rhs); break;
+      default: launch_laplace_cplan<3, 8>(b, rows, lds, cell_lambda, f, rhs); break;
+      }
+    }
+  }
+  else if (var.kind == kSearch) {
+    const unsigned grid = grid_for(m.nb_node, rows);
+    if (m.npc == 4)
+      hipLaunchKernelGGL((k_assemble_laplace<4, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+                         m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
+                         rhs, b->values.p);
+    else
+      hipLaunchKernelGGL((k_assemble_laplace<3, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
+                         m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
+                         rhs, b->values.p);
+  }
+  else if (m.npc == 4) {
+    if (var.u == 8)
+      launch_laplace_plan<4, 8>(b, rows, lds, cell_lambda, f, rhs);
+    else
+      launch_laplace_plan<4, 4>(b, rows, lds, cell_lambda, f, rhs);
+  }
+  else {
+    if (var.u == 8)
+      launch_laplace_plan<3, 8>(b, rows, lds, cell_lambda, f, rhs);
+    else
+      launch_laplace_plan<3, 4>(b, rows, lds, cell_lambda, f, rhs);
+  }
+  check_launch("assemble_laplace");
 }
 
 static void assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs)
 {
   AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
   AFEM_REQUIRE(b->nb_dof == 1, AFEM_ERR_ARGUMENT, "the Laplace element needs nb_dof = 1");
-  const afem_mesh& m = *b->mesh;
-  int rows;
-  size_t lds;
-  assembly_shape(*b, 12, &rows, &lds);
-  const unsigned grid = grid_for(m.nb_node, rows);
-  ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
-  if (m.npc == 4)
-    hipLaunchKernelGGL((k_assemble_laplace<4, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
-                       m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
-                       rhs, b->values.p);
-  else
-    hipLaunchKernelGGL((k_assemble_laplace<3, 4>), dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
-                       m.nc_offsets.p, m.nc_items.p, b->row_index.p, b->columns.p, m.nb_node, rows, cell_lambda, f,
-                       rhs, b->values.p);
-  check_launch("assemble_laplace");
+  assemble_laplace_variant(b, b->variant >= 0 ? b->variant : kDefaultLaplaceVariant, cell_lambda, f, rhs);
+}
+
+// Times every Laplace variant (best of 3 after one warm-up launch) on the
+// handle's own matrix and keeps the fastest.  Values are overwritten.
+static void autotune_laplace(afem_bsr* b, double* best_ms, int* best_variant)
+{
+  hipEvent_t e0, e1;
+  AFEM_HIP(hipEventCreate(&e0));
+  AFEM_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int pick = kDefaultLaplaceVariant;
+  for (int v = 0; v < kNumLaplaceVariants; ++v) {
+    if (effective_variant(b, v).kind != kLaplaceVariants[v].kind)
+      continue; // this handle lacks the plan the variant needs
+    float t_best = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      AFEM_HIP(hipEventRecord(e0, stream()));
+      assemble_laplace_variant(b, v, nullptr, 1.0, nullptr);
+      AFEM_HIP(hipEventRecord(e1, stream()));
+      AFEM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      AFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0 && ms < t_best)
+        t_best = ms;
+    }
+    if (t_best < best) {
+      best = t_best;
+      pick = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->variant = pick;
+  if (best_ms)
+    *best_ms = best;
+  if (best_variant)
+    *best_variant = pick;
 }
 
 static void assemble_elasticity(afem_bsr* b, double lam, double mu)
@@ -454,7 +1191,8 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
   const afem_mesh& m = *b->mesh;
   int rows;
   size_t lds;
-  assembly_shape(*b, 9 * 8 + 4, &rows, &lds);
+  AFEM_REQUIRE(assembly_shape(*b, 9 * 8 + 4, 256, &rows, &lds), AFEM_ERR_CAPACITY,
+               "matrix rows too long for the LDS-staged assembly");
   const unsigned grid = grid_for(m.nb_node, rows);
   ProfScope prof("assemble_elasticity_tet");
   hipLaunchKernelGGL(k_assemble_elasticity<2>, dim3(grid), dim3(rows), lds, stream(), m.coords.p, m.cells.p,
@@ -614,6 +1352,25 @@ AFEM_API int afem_bsr_compute_sparsity(afem_bsr_t bsr)
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
   bsr_compute_sparsity(bsr);
+  build_plan(bsr);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* variant)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr && bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called first");
+  AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_NOT_IMPLEMENTED, "assembly autotuning covers the Laplace element");
+  autotune_laplace(bsr, best_ms, variant);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  AFEM_REQUIRE(variant >= -1 && variant < kNumLaplaceVariants, AFEM_ERR_ARGUMENT, "unknown assembly variant");
+  bsr->variant = variant;
   AFEM_GUARD_END
 }
 

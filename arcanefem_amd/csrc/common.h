@@ -218,6 +218,16 @@ struct afem_bsr
   afem::DBuf<double> values;         // nb_col * nb_dof^2
   afem::DBuf<int32_t> csr_row;       // scalar CSR (toCsr), built on demand
   afem::DBuf<int32_t> csr_col;
+  // assembly plan: per (node, cell) pair the cell, the slot and the row
+  // positions of the other nodes; per row the diagonal position
+  afem::DBuf<uint64_t> plan;
+  afem::DBuf<uint64_t> cplan;        // compact plan (node deltas + positions), when it fits
+  afem::DBuf<uint64_t> lplan;        // block-local plan (node-table indices + positions), when it fits
+  afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
+  afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes
+  int32_t block_nodes_max = 0;       // largest node table
+  afem::DBuf<int32_t> diag_pos;
+  int variant = -1;                  // assembly kernel variant (-1: default)
 };
 
 struct afem_solver_ws;
@@ -238,6 +248,7 @@ struct afem_system
   afem_bsr* bsr = nullptr;
   bool matrix_changed = true;        // invalidates derived data (FP32 copy, diagonal)
   bool has_row_column = false;       // row-column eliminations present
+  bool dof0_row_column = false;      // DoF 0 is row-column eliminated (column 0 kept: non-symmetric)
   afem::DBuf<double> saved_values;   // values before the row-column elimination
   afem_solver_ws* ws = nullptr;
 };

@@ -1,0 +1,191 @@
+// BSRFormat.hpp -- the module-side assembly entry points of ArcaneFEM
+// (femutils/BSRFormat.h:143-236) and the GPU boundary-condition helpers
+// (femutils/ArcaneFemFunctionsGpu.h:729-816), over the afem C ABI.
+//
+// The reference's assembleBilinearAtomic / assembleBilinearAtomicFree take a
+// device lambda computing the element matrix; a C ABI cannot carry one, so
+// the element is named instead (P1 Laplace with an optional per-cell
+// conductivity -- the Poisson and Fourier modules -- or P1 linear elasticity).
+// Both entry points run the same atomic-free node-wise gather kernels.
+#pragma once
+
+#include "DoFLinearSystem.hpp"
+
+#include <cmath>
+#include <iomanip>
+#include <iostream>
+
+namespace afem::femutils {
+
+enum class Element
+{
+  Laplace,   // modules/poisson/ElementMatrix.h, modules/fourier/ElementMatrix.h
+  Elasticity // modules/elasticity/ElementMatrix.h:151-301
+};
+
+struct ElementParameters
+{
+  const Real* cell_lambda = nullptr; // device array, nb_cell values, or null
+  Real source_f = 0.0;               // fused constant-source RHS (Laplace)
+  Real* rhs = nullptr;               // device array written when source_f is used
+  Real lame_lambda = 0.0;
+  Real lame_mu = 0.0;
+};
+
+class BSRFormat
+{
+ public:
+  BSRFormat() = default;
+  BSRFormat(const BSRFormat&) = delete;
+  BSRFormat& operator=(const BSRFormat&) = delete;
+  ~BSRFormat()
+  {
+    if (m_bsr)
+      afem_bsr_destroy(m_bsr);
+  }
+
+  // femutils/BSRFormat.cc:350-372
+  void initialize(const Mesh& mesh, int nb_dof, bool does_linear_system_use_csr, bool use_atomic_free = false)
+  {
+    if (m_bsr)
+      afem_bsr_destroy(m_bsr);
+    m_bsr = nullptr;
+    check(afem_bsr_create(mesh.handle(), nb_dof, does_linear_system_use_csr ? 1 : 0, use_atomic_free ? 1 : 0, &m_bsr),
+          "BSRFormat(initialize)");
+    m_use_csr = does_linear_system_use_csr;
+  }
+  void computeSparsity() { check(afem_bsr_compute_sparsity(handle()), "BSRFormat(computeSparsity)"); }
+  void resetMatrixValues() { check(afem_bsr_reset_values(handle()), "BSRFormat(resetMatrixValues)"); }
+
+  void assembleBilinearAtomic(Element e, const ElementParameters& p) { assemble(e, p); }
+  void assembleBilinearAtomicFree(Element e, const ElementParameters& p) { assemble(e, p); }
+  void assembleBilinear(Element e, const ElementParameters& p) { assemble(e, p); }
+
+  // femutils/BSRFormat.cc:377-395: hands the (scalar-CSR-ordered) values to a
+  // CSR-capable linear system as a non-owning view.
+  void toLinearSystem(DoFLinearSystem& ls)
+  {
+    if (!ls.hasSetCSRValues())
+      throw FatalError(AFEM_ERR_ARGUMENT,
+                       "BSRFormat(toLinearSystem): Linear system was set to use CSR but is incompatible");
+    afem_bsr_info i = info();
+    CSRFormatView v;
+    v.bsr = handle();
+    v.nb_row = i.nb_row * i.nb_block;
+    v.nb_value = i.nb_non_zero_value;
+    const int32_t* r = nullptr;
+    const int32_t* c = nullptr;
+    check(afem_bsr_device_arrays(handle(), &r, &c, &v.values, nullptr), "toLinearSystem");
+    v.rows = r;
+    v.columns = c;
+    ls.setCSRValues(v);
+  }
+
+  afem_bsr_info info() const
+  {
+    afem_bsr_info i{};
+    check(afem_bsr_get_info(handle(), &i), "BSRFormat(info)");
+    return i;
+  }
+  afem_bsr_t handle() const
+  {
+    if (!m_bsr)
+      throw FatalError(AFEM_ERR_STATE, "BSRFormat: initialize() not called");
+    return m_bsr;
+  }
+
+ private:
+  void assemble(Element e, const ElementParameters& p)
+  {
+    if (e == Element::Laplace)
+      check(afem_bsr_assemble_laplace(handle(), p.cell_lambda, p.source_f, p.rhs), "BSRFormat(assembleBilinear)");
+    else
+      check(afem_bsr_assemble_elasticity(handle(), p.lame_lambda, p.lame_mu), "BSRFormat(assembleBilinear)");
+  }
+  afem_bsr_t m_bsr = nullptr;
+  bool m_use_csr = false;
+};
+
+namespace BoundaryConditions {
+
+  // femutils/ArcaneFemFunctionsGpu.h:675-708 (node-wise gather form)
+  inline void applyConstantSourceToRhs(Real qdot, const Mesh& mesh, int nb_dof, int dof_index, VariableDoFReal& rhs)
+  {
+    check(afem_rhs_constant_source(mesh.handle(), qdot, nullptr, nb_dof, dof_index, rhs.devicePointer()),
+          "applyConstantSourceToRhs");
+  }
+
+  // modules/elasticity/BodyForce.h:149-161
+  inline void applyBodyForceToRhs(const Real f[3], const Mesh& mesh, VariableDoFReal& rhs)
+  {
+    check(afem_rhs_body_force(mesh.handle(), f, rhs.devicePointer()), "applyBodyForceToRhs");
+  }
+
+  // femutils/ArcaneFemFunctionsGpu.cc:115-147 for one component: method is
+  // "Penalty" (default penalty 1e30, femutils/FemBoundaryConditions.axl:104-109),
+  // "RowElimination" or "RowColumnElimination".
+  inline void applyDirichletToLhsAndRhs(const std::string& method, const NodeGroup& nodes, int nb_dof_per_node,
+                                        int dof_index, Real value, DoFLinearSystem& ls, Real penalty = 1.0e30)
+  {
+    int m;
+    if (method == "Penalty")
+      m = AFEM_DIRICHLET_PENALTY;
+    else if (method == "RowElimination")
+      m = AFEM_DIRICHLET_ROW_ELIMINATION;
+    else if (method == "RowColumnElimination")
+      m = AFEM_DIRICHLET_ROW_COLUMN_ELIMINATION;
+    else
+      throw FatalError(AFEM_ERR_ARGUMENT, "Unknown method to enforce Dirichlet BC: '" + method + "'");
+    auto* impl = dynamic_cast<Mi355xDoFLinearSystemImpl*>(ls.implementation());
+    if (!impl)
+      throw FatalError(AFEM_ERR_ARGUMENT, "applyDirichletToLhsAndRhs needs the MI355X linear system");
+    check(afem_system_apply_dirichlet(impl->handle(), m, (int32_t)nodes.size(), nodes.data(), nb_dof_per_node,
+                                      dof_index, value, penalty),
+          "applyDirichletToLhsAndRhs");
+  }
+
+} // namespace BoundaryConditions
+
+// femutils/FemUtils.cc:85-171 (checkNodeResultFile): reads "uid v..." lines
+// and counts relative differences (Arcane's isNearlyEqualWithEpsilon:
+// equal, or |a-b| / (|a|+|b|) < epsilon), skipping pairs below min_value.
+// Returns the number of differing values and throws if any.
+inline Int64 checkNodeResultFile(const std::string& filename, const std::vector<Int64>& uids,
+                                 const std::vector<Real>& values, int ncomp, double epsilon, double min_value = 0.0)
+{
+  std::ifstream f(filename);
+  if (!f)
+    throw FatalError(AFEM_ERR_ARGUMENT, "Invalid result file '" + filename + "'");
+  std::map<Int64, std::vector<Real>> ref;
+  Int64 uid;
+  while (f >> uid) {
+    std::vector<Real> v((size_t)ncomp);
+    for (auto& x : v)
+      if (!(f >> x))
+        throw FatalError(AFEM_ERR_ARGUMENT, "Error during parsing of file '" + filename + "'");
+    ref[uid] = v;
+  }
+  Int64 nb_error = 0;
+  for (size_t n = 0; n < uids.size(); ++n) {
+    auto it = ref.find(uids[n]);
+    if (it == ref.end())
+      continue;
+    for (int c = 0; c < ncomp; ++c) {
+      const double a = it->second[c], b = values[n * ncomp + c];
+      if (std::fabs(a) < min_value && std::fabs(b) < min_value)
+        continue;
+      const double d = a - b;
+      if (d == 0.0)
+        continue;
+      if (!(std::fabs(d) / (std::fabs(a) + std::fabs(b)) < epsilon)) {
+        if (++nb_error < 50)
+          std::cout << "ERROR: uid=" << uids[n] << " ref=" << a << " v=" << b << " diff=" << d << "\n";
+      }
+    }
+  }
+  if (nb_error > 0)
+    throw FatalError(AFEM_ERR_NOT_FOUND, "Error checking values nb_error=" + std::to_string(nb_error));
+  return nb_error;
+}
+
+} // namespace afem::femutils

@@ -1,0 +1,313 @@
+// Mesh.hpp -- C++ stand-in for the Arcane mesh that ArcaneFEM's modules hand to
+// BSRFormat (IMesh + UnstructuredMeshConnectivityView + node groups), over the
+// afem C ABI, plus a reader for the binary Gmsh 4.1 files the reference's test
+// cases use (meshes/msh/*.msh, "$MeshFormat 4.1 1 8").
+//
+// Header-only; errors are thrown as afem::femutils::FatalError, the
+// counterpart of ARCANE_FATAL.
+#pragma once
+
+#include "afem.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <iterator>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace afem::femutils {
+
+using Real = double;
+using Int32 = std::int32_t;
+using Int64 = std::int64_t;
+using Byte = std::uint8_t;
+
+struct FatalError : std::runtime_error
+{
+  int status;
+  FatalError(int s, const std::string& m)
+  : std::runtime_error(m)
+  , status(s)
+  {}
+};
+
+inline void check(int rc, const char* where)
+{
+  if (rc != AFEM_OK)
+    throw FatalError(rc, std::string(where) + ": " + afem_last_error());
+}
+
+// Node ids of one named boundary group (the nodes of the faces carrying a
+// physical name), local ids.
+using NodeGroup = std::vector<Int32>;
+
+class Mesh
+{
+ public:
+  Mesh() = default;
+  Mesh(const Mesh&) = delete;
+  Mesh& operator=(const Mesh&) = delete;
+  ~Mesh()
+  {
+    if (m_handle)
+      afem_mesh_destroy(m_handle);
+  }
+
+  // Host arrays: coordinates (N x 3), cells (C x 4, stride 4), node unique
+  // ids; upload = false keeps the mesh on the host (parse-only use).
+  void build(int dim, std::vector<Real> coords, std::vector<Int32> cells, std::vector<Int64> uids,
+             bool upload = true)
+  {
+    m_dim = dim;
+    m_coords = std::move(coords);
+    m_cells = std::move(cells);
+    m_uids = std::move(uids);
+    if (m_handle)
+      afem_mesh_destroy(m_handle);
+    m_handle = nullptr;
+    if (upload)
+      check(afem_mesh_create(dim, nbNode(), m_coords.data(), nbCell(), dim + 1, m_cells.data(), &m_handle),
+            "afem_mesh_create");
+  }
+  const std::map<std::string, NodeGroup>& groups() const { return m_groups; }
+
+  // Structured unit box (see afem_box_mesh); uids = lid + 1 like Gmsh tags.
+  void buildBox(int dim, int n, double jitter, std::uint64_t seed)
+  {
+    Int32 nn = 0, nc = 0;
+    check(afem_box_mesh_size(dim, n, &nn, &nc), "afem_box_mesh_size");
+    std::vector<Real> x((size_t)nn * 3);
+    std::vector<Int32> c((size_t)nc * 4);
+    check(afem_box_mesh(dim, n, jitter, seed, x.data(), c.data()), "afem_box_mesh");
+    std::vector<Int64> uid((size_t)nn);
+    for (Int32 i = 0; i < nn; ++i)
+      uid[i] = i + 1;
+    build(dim, std::move(x), std::move(c), std::move(uid));
+  }
+
+  // Binary Gmsh 4.1 (size_t = 8): nodes, the cells of the highest dimension
+  // (tri3 or tet4) and the node groups of the (dim-1) physical names.
+  void readGmsh41(const std::string& path, bool upload = true)
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+      throw FatalError(AFEM_ERR_ARGUMENT, "cannot open mesh file '" + path + "'");
+    std::string data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    size_t p = 0;
+    auto line = [&]() {
+      size_t e = data.find('\n', p);
+      if (e == std::string::npos)
+        throw FatalError(AFEM_ERR_ARGUMENT, "truncated mesh file");
+      std::string s = data.substr(p, e - p);
+      p = e + 1;
+      while (!s.empty() && (s.back() == '\r' || s.back() == ' '))
+        s.pop_back();
+      return s;
+    };
+    auto take = [&](void* dst, size_t n) {
+      if (p + n > data.size())
+        throw FatalError(AFEM_ERR_ARGUMENT, "truncated mesh file");
+      std::memcpy(dst, data.data() + p, n);
+      p += n;
+    };
+    auto i32 = [&]() {
+      int32_t v;
+      take(&v, 4);
+      return v;
+    };
+    auto u64 = [&]() {
+      uint64_t v;
+      take(&v, 8);
+      return v;
+    };
+    std::map<std::pair<int, int>, std::string> names;
+    std::map<std::pair<int, int>, std::vector<int>> entity_phys;
+    std::vector<Int64> tags;
+    std::vector<Real> xyz;
+    struct Block
+    {
+      int dim, tag, k;
+      std::vector<uint64_t> conn;
+    };
+    std::vector<Block> blocks;
+    static const std::map<int, int> nodes_of = { { 15, 1 }, { 1, 2 }, { 2, 3 }, { 4, 4 } };
+    while (p < data.size()) {
+      std::string head = line();
+      if (head.empty())
+        continue;
+      if (head == "$MeshFormat") {
+        std::string fmt = line();
+        if (fmt.rfind("4.1 1 8", 0) != 0)
+          throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "only binary Gmsh 4.1 with 8-byte size_t is supported");
+        if (i32() != 1)
+          throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "big-endian Gmsh file");
+        line();
+        line(); // $EndMeshFormat
+      }
+      else if (head == "$PhysicalNames") {
+        int n = std::stoi(line());
+        for (int k = 0; k < n; ++k) {
+          std::string s = line();
+          size_t a = s.find(' '), b = s.find(' ', a + 1);
+          int d = std::stoi(s.substr(0, a)), t = std::stoi(s.substr(a + 1, b - a - 1));
+          std::string nm = s.substr(b + 1);
+          if (nm.size() >= 2 && nm.front() == '"')
+            nm = nm.substr(1, nm.size() - 2);
+          names[{ d, t }] = nm;
+        }
+        line();
+      }
+      else if (head == "$Entities") {
+        uint64_t counts[4];
+        for (auto& c : counts)
+          c = u64();
+        for (int d = 0; d < 4; ++d)
+          for (uint64_t k = 0; k < counts[d]; ++k) {
+            int tag = i32();
+            double box[6];
+            take(box, d == 0 ? 24 : 48);
+            uint64_t np = u64();
+            std::vector<int> phys(np);
+            for (auto& v : phys)
+              v = i32();
+            entity_phys[{ d, tag }] = phys;
+            if (d > 0) {
+              uint64_t nb = u64();
+              p += 4 * nb;
+            }
+          }
+        line();
+        line();
+      }
+      else if (head == "$Nodes") {
+        uint64_t nblocks = u64(), nnodes = u64();
+        u64();
+        u64();
+        for (uint64_t b = 0; b < nblocks; ++b) {
+          i32();
+          i32();
+          if (i32() != 0)
+            throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "parametric Gmsh nodes");
+          uint64_t n = u64();
+          size_t t0 = tags.size();
+          for (uint64_t k = 0; k < n; ++k)
+            tags.push_back((Int64)u64());
+          xyz.resize((t0 + n) * 3);
+          take(xyz.data() + t0 * 3, n * 24);
+        }
+        if (tags.size() != nnodes)
+          throw FatalError(AFEM_ERR_MESH, "node count mismatch");
+        line();
+        line();
+      }
+      else if (head == "$Elements") {
+        uint64_t nblocks = u64();
+        u64();
+        u64();
+        u64();
+        for (uint64_t b = 0; b < nblocks; ++b) {
+          Block blk;
+          blk.dim = i32();
+          blk.tag = i32();
+          int type = i32();
+          auto it = nodes_of.find(type);
+          if (it == nodes_of.end())
+            throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "Gmsh element type " + std::to_string(type));
+          blk.k = it->second;
+          uint64_t n = u64();
+          blk.conn.resize(n * (blk.k + 1));
+          take(blk.conn.data(), blk.conn.size() * 8);
+          blocks.push_back(std::move(blk));
+        }
+        line();
+        line();
+      }
+      else {
+        std::string end = "$End" + head.substr(1);
+        while (line() != end) {
+        }
+      }
+    }
+    // nodes ordered by tag; local id = rank of the tag
+    std::vector<size_t> order(tags.size());
+    for (size_t k = 0; k < order.size(); ++k)
+      order[k] = k;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tags[a] < tags[b]; });
+    std::map<Int64, Int32> lid;
+    std::vector<Real> coords(tags.size() * 3);
+    std::vector<Int64> uids(tags.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+      lid[tags[order[k]]] = (Int32)k;
+      uids[k] = tags[order[k]];
+      for (int d = 0; d < 3; ++d)
+        coords[3 * k + d] = xyz[3 * order[k] + d];
+    }
+    int dim = 0;
+    for (auto& b : blocks)
+      dim = std::max(dim, b.dim);
+    std::vector<Int32> cells;
+    m_groups.clear();
+    for (auto& b : blocks) {
+      const size_t n = b.conn.size() / (b.k + 1);
+      if (b.dim == dim) {
+        for (size_t e = 0; e < n; ++e)
+          for (int s = 0; s < 4; ++s)
+            cells.push_back(s < b.k ? lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]) : -1);
+      }
+      else if (b.dim == dim - 1) {
+        for (int ph : entity_phys[{ b.dim, b.tag }]) {
+          auto nm = names.find({ b.dim, ph });
+          if (nm == names.end())
+            continue;
+          auto& g = m_groups[nm->second];
+          for (size_t e = 0; e < n; ++e)
+            for (int s = 0; s < b.k; ++s)
+              g.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
+        }
+      }
+    }
+    for (auto& kv : m_groups) {
+      std::sort(kv.second.begin(), kv.second.end());
+      kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+    }
+    build(dim, std::move(coords), std::move(cells), std::move(uids), upload);
+  }
+
+  afem_mesh_t handle() const { return m_handle; }
+  int dimension() const { return m_dim; }
+  Int32 nbNode() const { return (Int32)(m_coords.size() / 3); }
+  Int32 nbCell() const { return (Int32)(m_cells.size() / 4); }
+  const std::vector<Real>& coords() const { return m_coords; }
+  const std::vector<Int32>& cells() const { return m_cells; }
+  const std::vector<Int64>& uniqueIds() const { return m_uids; }
+  const NodeGroup& group(const std::string& name) const
+  {
+    auto it = m_groups.find(name);
+    if (it == m_groups.end())
+      throw FatalError(AFEM_ERR_ARGUMENT, "no boundary group '" + name + "'");
+    return it->second;
+  }
+  // nodes whose coordinate `axis` equals `value` (structured boxes)
+  NodeGroup nodesOnPlane(int axis, double value) const
+  {
+    NodeGroup g;
+    for (Int32 i = 0; i < nbNode(); ++i)
+      if (m_coords[3 * (size_t)i + axis] == value)
+        g.push_back(i);
+    return g;
+  }
+
+ private:
+  int m_dim = 0;
+  std::vector<Real> m_coords;
+  std::vector<Int32> m_cells;
+  std::vector<Int64> m_uids;
+  std::map<std::string, NodeGroup> m_groups;
+  afem_mesh_t m_handle = nullptr;
+};
+
+} // namespace afem::femutils

@@ -162,6 +162,28 @@ __global__ void k_dirichlet(const int32_t* __restrict__ nodes, int32_t count, in
   }
 }
 
+// One (row, column) entry of the matrix view: add or set; found = 0 when the
+// entry is not in the pattern.
+template <int NB>
+__global__ void k_matrix_entry(RowEntries<NB> m, double* __restrict__ vals, int32_t row, int32_t col, double v,
+                               int set, int32_t* __restrict__ found)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0)
+    return;
+  int32_t b, e;
+  int i;
+  m.range(row, b, e, i);
+  for (int32_t blk = b; blk < e; ++blk)
+    for (int j = 0; j < NB; ++j)
+      if (m.col(blk, j) == col) {
+        double& a = vals[m.vidx(blk, i, j)];
+        a = set ? v : a + v;
+        *found = 1;
+        return;
+      }
+  *found = 0;
+}
+
 static bool any_equal(const DBuf<uint8_t>& a, int32_t n, uint8_t v)
 {
   DBuf<int32_t> flag(1);
@@ -185,7 +207,12 @@ static void matrix_transformation(afem_system* s)
   const int32_t n = s->nb_dof;
   const unsigned grid = grid_for(n, 256);
   s->has_row_column = any_equal(s->elim_info, n, kEliminateRowColumn);
+  s->dof0_row_column = false;
   if (s->has_row_column) {
+    uint8_t info0 = 0;
+    AFEM_HIP(hipMemcpyAsync(&info0, s->elim_info.p, 1, hipMemcpyDeviceToHost, stream()));
+    AFEM_HIP(hipStreamSynchronize(stream()));
+    s->dof0_row_column = n > 0 && info0 == kEliminateRowColumn;
     s->saved_values.reserve((size_t)s->nnz);
     AFEM_HIP(hipMemcpyAsync(s->saved_values.p, s->values, sizeof(double) * s->nnz, hipMemcpyDeviceToDevice, stream()));
   }
@@ -425,6 +452,28 @@ AFEM_API int afem_system_apply_rhs_transformation(afem_system_t sys)
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(sys, AFEM_ERR_ARGUMENT, "null handle");
   system_rhs_transformation(sys);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_system_matrix_add_value(afem_system_t sys, int32_t row, int32_t column, double value, int set)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys, AFEM_ERR_ARGUMENT, "null handle");
+  system_require_matrix(sys);
+  AFEM_REQUIRE(row >= 0 && row < sys->nb_dof && column >= 0 && column < sys->nb_dof, AFEM_ERR_ARGUMENT,
+               "matrix entry outside the DoF family");
+  DBuf<int32_t> found(1);
+  if (sys->block == 3)
+    hipLaunchKernelGGL(k_matrix_entry<3>, dim3(1), dim3(64), 0, stream(), RowEntries<3>{ sys->row_index, sys->columns },
+                       sys->values, row, column, value, set, found.p);
+  else
+    hipLaunchKernelGGL(k_matrix_entry<1>, dim3(1), dim3(64), 0, stream(), RowEntries<1>{ sys->row_index, sys->columns },
+                       sys->values, row, column, value, set, found.p);
+  check_launch("matrix_entry");
+  int32_t h = 0;
+  found.download(&h, 1);
+  AFEM_REQUIRE(h == 1, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
+  sys->matrix_changed = true;
   AFEM_GUARD_END
 }
 

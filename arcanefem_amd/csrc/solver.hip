@@ -42,6 +42,7 @@ struct State
   double rr_ref[2];
   int ru[2];
   int max_iter, iter, done, converged, breakdown, nb_ru;
+  int natural; // stopping norm: 0 r.r, 1 r.z (natural), 2 z.z (preconditioned; rr holds z.z, bb (M b).(M b))
 };
 
 // per-block partial sums, kMaxBlocks entries each
@@ -479,7 +480,7 @@ __device__ __forceinline__ double rho_prev(const State* st, const Parts& P, int 
 template <int NB, class T>
 __global__ void __launch_bounds__(kBlock)
 k_init(int64_t nunits, const double* r64, const double* __restrict__ b, T* r, T* __restrict__ z, T* __restrict__ p,
-       const T* __restrict__ dinv, int precond, Parts P)
+       const T* __restrict__ dinv, int precond, int pnorm, Parts P)
 {
   int64_t t0, t1;
   chunk_of((nunits + kBlock - 1) / kBlock, t0, t1);
@@ -488,22 +489,24 @@ k_init(int64_t nunits, const double* r64, const double* __restrict__ b, T* r, T*
     const int64_t u = t * kBlock + threadIdx.x;
     if (u >= nunits)
       continue;
-    T rv[NB], zv[NB];
+    T rv[NB], zv[NB], bv[NB], mb[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const double rd = r64[u * NB + i];
       rv[i] = (T)rd;
       r[u * NB + i] = rv[i];
-      rr += rd * rd;
-      const double bv = b[u * NB + i];
-      bb += bv * bv;
+      bv[i] = (T)b[u * NB + i];
     }
     apply_jacobi<NB, T>(u, dinv, rv, zv, precond);
+    if (pnorm)
+      apply_jacobi<NB, T>(u, dinv, bv, mb, precond);
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       z[u * NB + i] = zv[i];
       p[u * NB + i] = zv[i];
       rz += (double)rv[i] * (double)zv[i];
+      rr += pnorm ? (double)zv[i] * (double)zv[i] : r64[u * NB + i] * r64[u * NB + i];
+      bb += pnorm ? (double)mb[i] * (double)mb[i] : b[u * NB + i] * b[u * NB + i];
     }
   }
   put_part(rz, P.rz[1]);
@@ -518,12 +521,12 @@ __global__ void __launch_bounds__(kBlock) k_init_finalize(Parts P, int nparts, S
   const double bb = sum_parts(P.bb, nparts);
   if (threadIdx.x == 0) {
     st->bb = bb;
-    st->tol2 = fmax(st->rtol2 * bb, st->atol2);
+    st->tol2 = st->natural == 1 ? st->rtol2 * rz : fmax(st->rtol2 * bb, st->atol2);
     st->rr_ref[1] = rr;
     st->ru[1] = 0;
     st->rr_out = rr;
     st->iter = 0;
-    st->done = (rr <= st->tol2 || rz == 0.0) ? 1 : 0;
+    st->done = ((st->natural == 1 ? rz : rr) <= st->tol2 || rz == 0.0) ? 1 : 0;
     st->converged = st->done;
   }
 }
@@ -546,6 +549,7 @@ k_update(int k, int64_t nunits, double* __restrict__ x, T* __restrict__ r, T* __
   }
   const double alpha = rho / pq;
   const T alpha_t = (T)alpha;
+  const bool pnorm = st->natural == 2;
   int64_t t0, t1;
   chunk_of((nunits + kBlock - 1) / kBlock, t0, t1);
   double rz = 0.0, rr = 0.0;
@@ -566,7 +570,7 @@ k_update(int k, int64_t nunits, double* __restrict__ x, T* __restrict__ r, T* __
     for (int i = 0; i < NB; ++i) {
       z[u * NB + i] = zv[i];
       rz += (double)rv[i] * (double)zv[i];
-      rr += (double)rv[i] * (double)rv[i];
+      rr += pnorm ? (double)zv[i] * (double)zv[i] : (double)rv[i] * (double)rv[i];
     }
   }
   put_part(rz, P.rz[k & 1]);
@@ -625,10 +629,11 @@ k_pupdate(int k, int mixed, int64_t nunits, T* __restrict__ p, T* __restrict__ z
     }
   }
   else {
-    if (rr <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr)) {
+    const double crit = st->natural == 1 ? rz : rr;
+    if (crit <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr)) {
       if (lead) {
         st->done = 1;
-        st->converged = rr <= st->tol2;
+        st->converged = crit <= st->tol2;
         st->iter = k + 1;
         st->rr_out = rr;
       }
@@ -989,15 +994,16 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
 // ---- solve ----------------------------------------------------------------------------------
 
 template <int NB>
-static void launch_init(afem_system* s, afem_solver_ws* w, bool mixed, int precond, const Parts& P, int nblk)
+static void launch_init(afem_system* s, afem_solver_ws* w, bool mixed, int precond, int pnorm, const Parts& P,
+                        int nblk)
 {
   const int64_t nunits = s->nb_dof / NB;
   if (mixed)
     hipLaunchKernelGGL((k_init<NB, float>), dim3(nblk), dim3(kBlock), 0, stream(), nunits, w->r64.p, s->rhs.p,
-                       w->r32.p, w->z32.p, w->p32.p, w->dinv32.p, precond, P);
+                       w->r32.p, w->z32.p, w->p32.p, w->dinv32.p, precond, pnorm, P);
   else
     hipLaunchKernelGGL((k_init<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), nunits, w->r64.p, s->rhs.p,
-                       w->r64.p, w->z64.p, w->p64.p, w->dinv64.p, precond, P);
+                       w->r64.p, w->z64.p, w->p64.p, w->dinv64.p, precond, pnorm, P);
   check_launch("pcg_init");
   hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kBlock), 0, stream(), P, nblk, w->st.p);
   check_launch("pcg_init_finalize");
@@ -1069,6 +1075,17 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   AFEM_REQUIRE(o.preconditioner == AFEM_PRECOND_JACOBI || o.preconditioner == AFEM_PRECOND_NONE, AFEM_ERR_ARGUMENT,
                "unknown preconditioner");
   AFEM_REQUIRE(o.rtol >= 0.0 && o.atol >= 0.0 && o.max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
+  AFEM_REQUIRE(o.norm >= 0 && o.norm <= 2, AFEM_ERR_ARGUMENT, "unknown stopping norm");
+  AFEM_REQUIRE(o.norm == 0 || o.method != AFEM_SOLVER_PCG_MIXED, AFEM_ERR_NOT_IMPLEMENTED,
+               "the natural-norm and preconditioned-norm stopping tests are available for the FP64 solvers");
+  // The reference's row-column elimination never touches column 0
+  // (femutils/CsrDoFLinearSystemImpl.cc:111), so with DoF 0 eliminated the
+  // matrix keeps A[i,0] in every row while the RHS is lifted by A[0,i] g
+  // (femutils/DoFLinearSystemImplBase.cc:65-87): a non-symmetric system
+  // that no conjugate-gradient method solves.
+  AFEM_REQUIRE(!s->dof0_row_column, AFEM_ERR_NOT_IMPLEMENTED,
+               "DoF 0 is row-column eliminated: the transformed matrix keeps column 0 and is not symmetric, "
+               "so PCG does not apply (use RowElimination or Penalty for that DoF)");
   auto t_start = std::chrono::steady_clock::now();
   afem_solver_ws* w = workspace(s);
   const bool mixed = o.method == AFEM_SOLVER_PCG_MIXED;
@@ -1090,6 +1107,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   init.rtol2 = o.rtol * o.rtol;
   init.atol2 = o.atol * o.atol;
   init.delta2 = delta * delta;
+  init.natural = o.norm;
   int total_iter = 0, restarts = 0, nb_ru = 0;
   bool converged = false;
   double relres = 0.0;
@@ -1102,9 +1120,9 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, always, P.rr64, nblk,
                                            "spmv_residual_fp64");
     if (s->block == 3)
-      launch_init<3>(s, w, mixed, precond, P, nblk);
+      launch_init<3>(s, w, mixed, precond, o.norm == 2, P, nblk);
     else
-      launch_init<1>(s, w, mixed, precond, P, nblk);
+      launch_init<1>(s, w, mixed, precond, o.norm == 2, P, nblk);
     read_state(w);
     int k = 0;
     while (!w->host_st->done && k < init.max_iter) {
@@ -1119,8 +1137,15 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     total_iter += w->host_st->iter;
     nb_ru += w->host_st->nb_ru;
     bnorm = std::sqrt(w->host_st->bb);
+    const bool device_converged = w->host_st->converged != 0;
     const double rr = true_rr(s, w, P, nblk);
     relres = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+    if (o.norm == 2)
+      relres = bnorm > 0.0 ? std::sqrt(w->host_st->rr_out) / bnorm : std::sqrt(w->host_st->rr_out);
+    if (o.norm != 0) {
+      converged = device_converged && std::isfinite(relres);
+      break; // the natural and preconditioned tests have no true-residual restart
+    }
     converged = relres <= o.rtol || std::sqrt(rr) <= o.atol;
     if (converged || total_iter >= o.max_iter || restarts >= 8 || !std::isfinite(relres) ||
         w->host_st->iter == 0)
@@ -1160,6 +1185,7 @@ AFEM_API void afem_solver_default_options(afem_solver_options* opt)
   opt->check_every = 16;
   opt->reliable_delta = 0.1;
   opt->spmv_lanes = 0;
+  opt->norm = 0;
 }
 
 AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt, afem_solve_result* result)

@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one 256 MiB device-to-device copy first (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -70,6 +72,14 @@ def main():
     import arcanefem_amd as A
 
     A.init(0)
+    if args.calibrate:
+        nbytes = 256 << 20
+        src, dst = A.DeviceArray(nbytes, np.uint8), A.DeviceArray(nbytes, np.uint8)
+        src.zero()
+        A.check(A.lib().afem_memcpy_dtod(dst.c, src.c, nbytes))
+        A.synchronize()
+        log(f"[bench] calibration copy of {nbytes} bytes done")
+        del src, dst
     n = args.n
     t0 = time.time()
     X, C = A.box_mesh(3, n, 0.15, seed=args.seed)
@@ -87,6 +97,8 @@ def main():
     log(f"[bench] pattern nnz={nnz} ({nnz / N:.2f}/row) in {pattern_ms:.1f} ms")
     sysm = A.DoFLinearSystem(N)
     rhs = sysm.rhs()
+    variant, tune_ms = bsr.autotune_assembly()
+    log(f"[bench] assembly autotuner: variant {variant} ({tune_ms:.3f} ms)")
 
     # ---- timed assembly steps -------------------------------------------------------------
     for _ in range(args.warmup):
@@ -132,7 +144,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": kname, "kernel_ms": k_ms, "algorithmic_bytes": alg_bytes,
-            "bytes_formula": "16*T + 24*N + 12*nnz + 4*(N+1)",
+            "bytes_formula": "16*T + 24*N + 12*nnz + 4*(N+1)", "assembly_variant": variant,
         },
         "pattern_ms": pattern_ms,
     }

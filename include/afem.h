@@ -141,6 +141,15 @@ AFEM_API int afem_bsr_reset_values(afem_bsr_t bsr);
 AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda,
                                        double source_f, double* rhs);
 
+/* Assembly kernel variants of the Laplace element (no counterpart in the
+ * reference, which has one lambda per format): variant 0 searches each
+ * column in the LDS-staged row; the others read the per-(node, cell) plan
+ * built by afem_bsr_compute_sparsity, with different batch depths and rows
+ * per workgroup.  autotune times each variant on this matrix, keeps the
+ * fastest and reports it; set_assembly_variant pins one (-1 = default). */
+AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* variant);
+AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant);
+
 /* Same with the 12x12 P1 linear-elasticity element
  * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3. */
 AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu);
@@ -216,6 +225,12 @@ AFEM_API int afem_system_apply_dirichlet(afem_system_t sys, int method, int32_t 
  * (femutils/CsrDoFLinearSystemImpl.cc:111). */
 AFEM_API int afem_system_apply_matrix_transformation(afem_system_t sys);
 AFEM_API int afem_system_apply_rhs_transformation(afem_system_t sys);
+/* CsrDoFLinearSystemImpl::matrixAddValue / matrixSetValue
+ * (femutils/internal/CsrDoFLinearSystemImpl.h:54-63): finds (row, column) in
+ * the CSR view and adds to (set = 0) or overwrites (set = 1) its value.
+ * AFEM_ERR_NOT_FOUND when the entry is not in the pattern. */
+AFEM_API int afem_system_matrix_add_value(afem_system_t sys, int32_t row, int32_t column, double value, int set);
+
 /* clearValues (femutils/DoFLinearSystemImplBase.cc:42-48): forced and
  * elimination arrays reset and the CSR view dropped. */
 AFEM_API int afem_system_clear_values(afem_system_t sys);
@@ -242,14 +257,23 @@ typedef struct afem_solver_options {
   int check_every;       /* iterations between host convergence polls (0 = 16) */
   double reliable_delta; /* mixed: residual reduction factor between FP64
                             reliable updates (0 = 0.1) */
-  int spmv_lanes;        /* lanes per row of the CSR SpMV (0 = autotuned choice) */
+  int spmv_lanes;        /* SpMV format: 0 = autotuned, > 0 = CSR with that many
+                            lanes per row (4, 8, 16, 32), -1 = SELL-64 */
+  int norm;              /* stopping norm: 0 = ||b - A x||_2 <= max(rtol ||b||_2, atol);
+                            1 = natural norm sqrt(r.M^-1 r) <= rtol * its initial value;
+                            2 = preconditioned norm ||M^-1 r||_2 <= max(rtol ||M^-1 b||_2,
+                            atol) (PETSc's default test for CG, KSP_NORM_PRECONDITIONED;
+                            it scales penalty rows by 1/1e30, so it is the one to use
+                            when penalty rows make ||b|| meaningless).  1 and 2 have no
+                            true-residual restart and are FP64 PCG only. */
 } afem_solver_options;
 
 typedef struct afem_solve_result {
   int iterations;
   int converged;
   int reliable_updates;
-  double relres;         /* FP64 true relative residual ||b - A x||_2 / ||b||_2 */
+  double relres;         /* FP64 true relative residual ||b - A x||_2 / ||b||_2
+                            (norm 2: ||M^-1 r||_2 / ||M^-1 b||_2 of the last iteration) */
   double solve_ms;       /* wall time of the solve on the library stream */
 } afem_solve_result;
 

@@ -21,7 +21,8 @@ Golden cases (test names and tolerances from the reference's CMake files):
   bar_3D      modules/elasticity/inputs/bar.3D.Dirichlet.bodyForce.arc,
               check/bar.3D.Dirichlet.bodyForce.txt (eps 1e-3, min 1e-10)
 
-The raw .msh files are copied beside the .npz files as data fixtures.
+The raw .msh files and the golden solution files are copied beside the .npz
+files as data fixtures (the C++ module driver reads them directly).
 """
 
 import argparse
@@ -189,6 +190,7 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     payload["ref_value"] = ref[:, 1:1 + ncomp].astype(np.float64)
     np.savez(os.path.join(out_dir, name + ".npz"), **payload)
     shutil.copyfile(mesh_path, os.path.join(out_dir, os.path.basename(mesh_rel)))
+    shutil.copyfile(os.path.join(ref_root, check_rel), os.path.join(out_dir, os.path.basename(check_rel)))
     print(f"{name}: N={len(tags)} cells={cells4.shape[0]} npc={npc} groups={sorted(groups)}")
 
 

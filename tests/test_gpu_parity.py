@@ -83,17 +83,22 @@ def test_node_cells_and_pattern_bit_exact(afem_gpu, spec):
     assert info.nb_col == len(ocol) and info.max_row_length == np.diff(orow).max()
 
 
+N_LAPLACE_VARIANTS = 8
+
+
 @pytest.mark.parametrize("spec", MESHES, ids=[m[0] for m in MESHES])
 @pytest.mark.parametrize("with_lambda", [False, True])
-def test_laplace_assembly_and_fused_rhs(afem_gpu, spec, with_lambda):
+@pytest.mark.parametrize("variant", list(range(N_LAPLACE_VARIANTS)))
+def test_laplace_assembly_and_fused_rhs(afem_gpu, spec, with_lambda, variant):
     A = afem_gpu
     dim, X, C = get_mesh(spec)
-    print(f"[case] laplace {spec[0]} lambda={with_lambda}", flush=True)
+    print(f"[case] laplace {spec[0]} lambda={with_lambda} variant={variant}", flush=True)
     N = X.shape[0]
     lam = np.exp(np.random.default_rng(5).normal(size=C.shape[0])) if with_lambda else None
     m = A.Mesh(dim, X, C)
     b = A.BSRFormat(m, 1)
     b.compute_sparsity()
+    b.set_assembly_variant(variant)
     dlam = A.DeviceArray.from_numpy(lam) if lam is not None else None
     rhs = A.DeviceArray(N, np.float64)
     b.assemble_laplace(dlam, 2.5, rhs)
@@ -110,6 +115,21 @@ def test_laplace_assembly_and_fused_rhs(afem_gpu, spec, with_lambda):
     b.assemble_laplace(dlam, 0.0, None)
     _, _, val2, _ = b.download()
     assert np.array_equal(val, val2)
+
+
+def test_assembly_autotune_keeps_parity(afem_gpu):
+    A = afem_gpu
+    print("[case] assembly autotune", flush=True)
+    X, C = box(3, 12, 0.15)
+    m = A.Mesh(3, X, C)
+    b = A.BSRFormat(m, 1)
+    b.compute_sparsity()
+    variant, ms = b.autotune_assembly()
+    assert 0 <= variant < N_LAPLACE_VARIANTS and ms > 0
+    rhs = A.DeviceArray(X.shape[0], np.float64)
+    b.assemble_laplace(None, 1.0, rhs)
+    row, col, val, _ = b.download()
+    assert row_scaled_error(row, O.assemble(3, X, C, 4, row, col), val) <= ROW_TOL
 
 
 @pytest.mark.parametrize("per_block", [True, False])
@@ -361,6 +381,39 @@ def test_golden_poisson_gpu(afem_gpu, case, method):
     assert r.converged == 1
     x = s.solution().numpy()
     assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
+
+
+@pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
+@pytest.mark.parametrize("norm", [1, 2])
+def test_golden_poisson_penalty_gpu(afem_gpu, case, norm):
+    """The reference's default Dirichlet method (Penalty, 1e30) on the golden
+    cases.  With x0 = 0 the penalty rows dominate both ||b|| and r.z, so the
+    2-norm and natural tests stop early; the preconditioned norm
+    ||M^-1 r|| / ||M^-1 b|| (PETSc's CG default) weighs a penalty row like
+    any other.  The natural test must therefore fail the golden check and
+    the preconditioned one pass it."""
+    A = afem_gpu
+    print(f"[case] golden penalty {case} norm={norm}", flush=True)
+    d, X, C, npc, dim = load_case(case)
+    m, b, s, _ = build_poisson(A, dim, X, C, 5.5)
+    s.apply_dirichlet(A.DIRICHLET_PENALTY, d["group_horizontal"], 1, 0, 0.5, 1e30)
+    s.apply_matrix_transformation()
+    s.apply_rhs_transformation()
+    r = s.solve(method=A.SOLVER_PCG_FP64, rtol=1e-12, max_iter=10000, norm=norm)
+    assert r.converged == 1
+    nerr = O.count_differences(d["ref_value"][:, 0], by_uid(d, s.solution().numpy())[:, 0], 1e-4)
+    if norm == 2:
+        assert nerr == 0 and r.iterations > 5 and r.relres <= 1e-12
+    else:
+        assert nerr > 0
+
+
+def test_preconditioned_norm_rejected_for_mixed(afem_gpu):
+    A = afem_gpu
+    d, X, C, npc, dim = load_case("circle_2D")
+    m, b, s, _ = build_poisson(A, dim, X, C, 5.5)
+    with pytest.raises(A.AfemError):
+        s.solve(method=A.SOLVER_PCG_MIXED, norm=2)
 
 
 @pytest.mark.parametrize("per_block", [True, False])
