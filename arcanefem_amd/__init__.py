@@ -104,6 +104,7 @@ def _declare(L):
         "afem_system_solve": (ci, [vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveResult)]),
         "afem_system_spmv": (ci, [vp, vp, vp]),
         "afem_system_spmv_fp32": (ci, [vp, vp, vp]),
+        "afem_system_spmv_format": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -382,6 +383,14 @@ class DoFLinearSystem:
 
     def spmv_fp32(self, x, y):
         check(lib().afem_system_spmv_fp32(self.h, x.c, y.c))
+
+    def spmv_format(self):
+        """('csr', lanes) or ('sell', 64) as chosen by the SpMV autotuner; None before any SpMV."""
+        f, lanes = ctypes.c_int(), ctypes.c_int()
+        check(lib().afem_system_spmv_format(self.h, ctypes.byref(f), ctypes.byref(lanes)))
+        if f.value < 0:
+            return None
+        return ("sell", 64) if f.value == 1 else ("csr", lanes.value)
 
     def __del__(self):
         try:

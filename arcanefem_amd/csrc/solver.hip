@@ -1230,4 +1230,14 @@ AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y)
   AFEM_GUARD_END
 }
 
+AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys && format && lanes, AFEM_ERR_ARGUMENT, "null argument");
+  afem_solver_ws* w = sys->ws;
+  *format = w ? w->format : -1;
+  *lanes = w ? w->lanes : 0;
+  AFEM_GUARD_END
+}
+
 } // extern "C"

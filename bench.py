@@ -31,6 +31,29 @@ sys.path.insert(0, ROOT)
 METRIC = "assembled DoFs/s; solve time to 1e-8 relres, learned-PCG vs plain CG"
 HBM_PEAK_GBS = 8000.0
 
+# assembly variant id (afem_bsr_set_assembly_variant) -> kernel instantiation
+ASSEMBLY_KERNELS = {
+    0: "k_assemble_laplace<4, 4>", 1: "k_assemble_laplace_plan<4, 4>", 2: "k_assemble_laplace_plan<4, 8>",
+    3: "k_assemble_laplace_cplan<4, 4>", 4: "k_assemble_laplace_cplan<4, 8>",
+    5: "k_assemble_laplace_lplan<4, 4>", 6: "k_assemble_laplace_lplan<4, 8>", 7: "k_assemble_laplace_lplan<4, 16>",
+}
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+# workload (scaled by a copy of known size), written by arcanefem_amd.profiling
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "assembly_spmv_tet128.pmc.json")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary, or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            summ = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if summ.get("workload") != workload:
+        return None
+    k = summ.get("kernels", {}).get(kernel)
+    return k["traffic_bytes"] if k else None
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -60,6 +83,7 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="hexes per side")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=9, help="timed repetitions of the CPU baseline")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 256 MiB device-to-device copy first (reference for HBM byte counters)")
@@ -122,6 +146,8 @@ def main():
     log(f"[bench] assembly: {ms_per_step:.3f} ms/step (kernel {k_ms:.3f} ms x{k_n}), {value / 1e9:.2f} GDoF/s, "
         f"{achieved:.0f} GB/s algorithmic")
 
+    workload_key = f"tet{n}_seed{args.seed}"
+    akernel = ASSEMBLY_KERNELS.get(variant)
     out = {
         "metric": METRIC,
         "value": value,
@@ -142,9 +168,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": kname, "kernel_ms": k_ms, "algorithmic_bytes": alg_bytes,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload_key, akernel),
+            "kernel": akernel, "kernel_ms": k_ms, "algorithmic_bytes": alg_bytes,
             "bytes_formula": "16*T + 24*N + 12*nnz + 4*(N+1)", "assembly_variant": variant,
+            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT),
         },
         "pattern_ms": pattern_ms,
     }
@@ -175,11 +202,17 @@ def main():
     s32, _ = kernel_avg_ms(A, "spmv_fp32")
     b64 = 12 * nnz + 20 * N
     b32 = 8 * nnz + 12 * N
+    fmt = sysm.spmv_format()
+    k64 = "k_spmv_sell<double, double, 0>" if fmt[0] == "sell" else f"k_spmv_csr<{fmt[1]}, double, double, 0>"
+    k32 = "k_spmv_sell<float, float, 0>" if fmt[0] == "sell" else f"k_spmv_csr<{fmt[1]}, float, float, 0>"
     out["spmv"] = {
+        "format": fmt[0] if fmt[0] == "sell" else f"csr{fmt[1]}",
         "fp64": {"kernel_ms": s64, "algorithmic_bytes": b64, "GBs": b64 / (s64 * 1e-3) / 1e9,
-                 "frac": b64 / (s64 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                 "frac": b64 / (s64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": k64,
+                 "traffic": measured_traffic(workload_key, k64)},
         "fp32": {"kernel_ms": s32, "algorithmic_bytes": b32, "GBs": b32 / (s32 * 1e-3) / 1e9,
-                 "frac": b32 / (s32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                 "frac": b32 / (s32 * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": k32,
+                 "traffic": measured_traffic(workload_key, k32)},
     }
     log(f"[bench] spmv fp64 {s64:.3f} ms ({out['spmv']['fp64']['GBs']:.0f} GB/s), "
         f"fp32 {s32:.3f} ms ({out['spmv']['fp32']['GBs']:.0f} GB/s)")
@@ -204,7 +237,8 @@ def main():
         import oracle as O
         row, col, _, _ = bsr.download()
         times = []
-        for _ in range(3):
+        O.assemble(3, X, C, 4, row, col)  # warm-up (page-in)
+        for _ in range(args.cpu_reps):
             t0 = O.wtime()
             O.assemble(3, X, C, 4, row, col)
             O.rhs_constant_source(3, X, C, 4, 1.0)
@@ -212,8 +246,9 @@ def main():
         t = float(np.median(times))
         out["cpu_baseline"] = {
             "value": N / t, "unit": "DoF/s", "cores": 1, "kind": "port",
-            "sample": f"full {n}^3 assembly (matrix values with row search + RHS), median of 3, "
-                      f"{t:.2f} s each, on {cpu_model()}",
+            "sample": f"full {n}^3 assembly (cell-wise, matrix values with per-entry row search + RHS), "
+                      f"1 warm-up + median of {args.cpu_reps} ({t:.2f} s each, {sum(times):.1f} s total), "
+                      f"one thread on {cpu_model()}",
         }
         log(f"[bench] cpu baseline: {N / t / 1e6:.2f} MDoF/s ({t:.2f} s)")
 

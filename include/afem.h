@@ -289,6 +289,10 @@ AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt
 AFEM_API int afem_system_spmv(afem_system_t sys, const double* x, double* y);
 /* y = A32 x32 with the FP32 copy of the system's matrix (made on first use). */
 AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y);
+/* SpMV format the autotuner chose for the system's matrix: *format = 0 (CSR,
+ * *lanes lanes per row, or per 3x3 block row when the matrix has blocks) or
+ * 1 (SELL-64); -1 while no SpMV or solve has run on the current matrix. */
+AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes);
 
 #ifdef __cplusplus
 }
