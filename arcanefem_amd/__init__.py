@@ -22,7 +22,7 @@ ERR_ARGUMENT, ERR_NOT_IMPLEMENTED, ERR_STATE, ERR_NOT_FOUND, ERR_MESH = 1, 2, 3,
 ERR_CAPACITY, ERR_NOT_CONVERGED, ERR_DEVICE, ERR_HIP, ERR_OUT_OF_MEMORY = 6, 7, 8, 9, 10
 
 DIRICHLET_PENALTY, DIRICHLET_ROW_ELIMINATION, DIRICHLET_ROW_COLUMN_ELIMINATION = 0, 1, 2
-SOLVER_PCG_FP64, SOLVER_PCG_MIXED, SOLVER_CG_FP64 = 0, 1, 2
+SOLVER_PCG_FP64, SOLVER_PCG_MIXED, SOLVER_CG_FP64, SOLVER_FPCG_SPAI = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_NONE = 0, 1
 
 
@@ -47,7 +47,18 @@ class SolverOptions(ctypes.Structure):
 
 class SolveResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("reliable_updates", ctypes.c_int),
-                ("relres", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+                ("relres", ctypes.c_double), ("solve_ms", ctypes.c_double), ("fallback", ctypes.c_int)]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("reliable_delta", ctypes.c_double), ("chunk", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("use_initial_guess", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("reliable_updates", ctypes.c_int),
+                ("relres", ctypes.c_double)]
 
 
 def _declare(L):
@@ -105,6 +116,26 @@ def _declare(L):
         "afem_system_spmv": (ci, [vp, vp, vp]),
         "afem_system_spmv_fp32": (ci, [vp, vp, vp]),
         "afem_system_spmv_format": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+        "afem_system_set_spai": (ci, [vp, vp]),
+        "afem_rff_parameters": (ci, [ci, ci, dbl, ctypes.c_uint64, vp, vp]),
+        "afem_lognormal_field": (ci, [vp, dbl, dbl, ci, ctypes.c_uint64, vp]),
+        "afem_spai_num_params": (ci, []),
+        "afem_spai_create": (ci, [ctypes.c_uint64, pp]),
+        "afem_spai_destroy": (ci, [vp]),
+        "afem_spai_get_params": (ci, [vp, vp, ci]),
+        "afem_spai_set_params": (ci, [vp, vp, ci]),
+        "afem_spai_loss": (ci, [vp, vp, ctypes.c_uint64, ctypes.POINTER(dbl)]),
+        "afem_spai_loss_grad": (ci, [vp, vp, ctypes.c_uint64, ctypes.POINTER(dbl), vp]),
+        "afem_spai_adam_step": (ci, [vp, dbl]),
+        "afem_spai_train_step": (ci, [vp, vp, ctypes.c_uint64, dbl, ctypes.POINTER(dbl)]),
+        "afem_spai_apply": (ci, [vp, vp]),
+        "afem_spai_inspect": (ci, [vp, ci, vp, i64]),
+        "afem_batch_default_options": (None, [ctypes.POINTER(BatchOptions)]),
+        "afem_batch_create": (ci, [vp, ci, vp, pp]),
+        "afem_batch_destroy": (ci, [vp]),
+        "afem_batch_set_tiles": (ci, [vp, ci, ci]),
+        "afem_batch_autotune": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(dbl)]),
+        "afem_batch_solve": (ci, [vp, ctypes.POINTER(BatchOptions), vp, ctypes.POINTER(dbl)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +415,10 @@ class DoFLinearSystem:
     def spmv_fp32(self, x, y):
         check(lib().afem_system_spmv_fp32(self.h, x.c, y.c))
 
+    def set_spai(self, values):
+        """Installs a sparse preconditioner with the matrix pattern (DeviceArray of nnz float32)."""
+        check(lib().afem_system_set_spai(self.h, values.c))
+
     def spmv_format(self):
         """('csr', lanes) or ('sell', 64) as chosen by the SpMV autotuner; None before any SpMV."""
         f, lanes = ctypes.c_int(), ctypes.c_int()
@@ -398,6 +433,133 @@ class DoFLinearSystem:
                 _lib.afem_system_destroy(self.h)
         except Exception:
             pass
+
+
+def rff_parameters(dim, nfeatures, corr_length, seed):
+    """(omega (K, 3), phase (K,)) of the random-Fourier-feature field generator."""
+    om = np.zeros((nfeatures, 3))
+    ph = np.zeros(nfeatures)
+    check(lib().afem_rff_parameters(dim, nfeatures, corr_length, seed, _ptr(om), _ptr(ph)))
+    return om, ph
+
+
+def lognormal_field(mesh, sigma=1.0, corr_length=0.1, nfeatures=512, seed=0, out=None):
+    """Per-cell log-normal conductivity exp(g) on the device (DeviceArray of nb_cell doubles)."""
+    if out is None:
+        out = DeviceArray(mesh.nb_cell, np.float64)
+    check(lib().afem_lognormal_field(mesh.h, sigma, corr_length, nfeatures, seed, out.c))
+    return out
+
+
+class LearnedSpai:
+    """The learned sparse approximate inverse (message-passing network on the
+    Jacobi-scaled matrix graph; BF16 MFMA forward/backward kernels)."""
+
+    INSPECT = {"shat": 0, "features": 1, "h3": 2, "u": 3, "mhat": 4, "m": 5, "scale": 6, "h0": 7}
+
+    def __init__(self, seed=0):
+        h = ctypes.c_void_p()
+        check(lib().afem_spai_create(seed, ctypes.byref(h)))
+        self.h = h
+        self.n_params = lib().afem_spai_num_params()
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        check(lib().afem_spai_get_params(self.h, _ptr(p), self.n_params))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().afem_spai_set_params(self.h, _ptr(p), p.size))
+
+    def loss(self, sysm, probe_seed=0):
+        out = ctypes.c_double()
+        check(lib().afem_spai_loss(self.h, sysm.h, probe_seed, ctypes.byref(out)))
+        return out.value
+
+    def loss_grad(self, sysm, probe_seed=0):
+        out = ctypes.c_double()
+        g = np.zeros(self.n_params, np.float32)
+        check(lib().afem_spai_loss_grad(self.h, sysm.h, probe_seed, ctypes.byref(out), _ptr(g)))
+        return out.value, g
+
+    def adam_step(self, lr):
+        check(lib().afem_spai_adam_step(self.h, lr))
+
+    def train_step(self, sysm, probe_seed, lr):
+        out = ctypes.c_double()
+        check(lib().afem_spai_train_step(self.h, sysm.h, probe_seed, lr, ctypes.byref(out)))
+        return out.value
+
+    def apply(self, sysm):
+        check(lib().afem_spai_apply(self.h, sysm.h))
+
+    def inspect(self, what, count, dtype=np.float32):
+        out = np.zeros(count, np.float32)
+        check(lib().afem_spai_inspect(self.h, self.INSPECT[what], _ptr(out), count))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.afem_spai_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+def union_mesh(meshes):
+    """Disjoint union of (coords, cells) pairs: concatenated arrays with the
+    node ids of each mesh offset, and the node offsets of the parts."""
+    coords, cells, offs = [], [], [0]
+    for X, C in meshes:
+        Cn = C.copy()
+        Cn[Cn >= 0] += offs[-1]
+        coords.append(X)
+        cells.append(Cn)
+        offs.append(offs[-1] + X.shape[0])
+    return np.concatenate(coords), np.concatenate(cells), np.array(offs, np.int32)
+
+
+class BatchSolver:
+    """Independent systems on the diagonal blocks of one system's matrix,
+    solved together (segmented mixed-precision PCG, hipGraph chunks)."""
+
+    def __init__(self, sysm, segment_rows):
+        seg = np.ascontiguousarray(segment_rows, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().afem_batch_create(sysm.h, len(seg) - 1, _ptr(seg), ctypes.byref(h)))
+        self.h = h
+        self.nb_segment = len(seg) - 1
+        self._sys = sysm
+
+    def set_tiles(self, lanes, tiles):
+        check(lib().afem_batch_set_tiles(self.h, lanes, tiles))
+
+    def autotune(self):
+        lanes, tiles, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().afem_batch_autotune(self.h, ctypes.byref(lanes), ctypes.byref(tiles), ctypes.byref(ms)))
+        return lanes.value, tiles.value, ms.value
+
+    def solve(self, raise_on_failure=True, **kw):
+        o = BatchOptions()
+        lib().afem_batch_default_options(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        res = (BatchResult * self.nb_segment)()
+        ms = ctypes.c_double()
+        rc = lib().afem_batch_solve(self.h, ctypes.byref(o), ctypes.cast(res, ctypes.c_void_p), ctypes.byref(ms))
+        if rc != OK and (raise_on_failure or rc != ERR_NOT_CONVERGED):
+            check(rc)
+        return list(res), ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.afem_batch_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
 
 
 def profile_enable(on=True):

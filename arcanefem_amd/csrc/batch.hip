@@ -1,0 +1,811 @@
+// batch.hip -- many independent scalar systems solved together.
+//
+// No counterpart in the reference, which solves one system per module
+// instance (femutils/DoFLinearSystem.cc:154-156 and the Hypre / PETSc
+// back-ends).  The systems are the diagonal blocks of one block-diagonal
+// matrix -- typically the Laplace matrix of the disjoint union of several
+// meshes, so the existing pattern and assembly kernels build all of them in
+// one launch each.  This file solves them with per-system scalars:
+// mixed-precision Jacobi PCG (FP32 matrix, r, z, p, q; FP64 x, FP64
+// reliable-update residual r = b - A x, which alone declares convergence).
+//
+// Each workgroup owns rows of exactly one system (a segment), so the
+// fence-free reduction scheme of solver.hip carries over per segment:
+// producers write one partial per workgroup, every consumer workgroup of a
+// segment re-reduces that segment's partials in the same order.  Kernels
+// take the iteration parity as an argument and read the absolute iteration
+// number from the segment state (parity-indexed, written one iteration
+// ahead), so a chunk of iterations can be captured once in a hipGraph and
+// replayed; converged segments make their workgroups return at once.
+//
+// Tuning: lanes per row (G) and row tiles per workgroup (T) are timed on
+// the batch and the fastest pair is kept.
+#include "common.h"
+#include "device_utils.h"
+#include "linsys.h"
+
+#include <algorithm>
+#include <type_traits>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace afem {
+namespace batch {
+
+constexpr int kBT = 256;
+
+struct SegState
+{
+  double rtol2, atol2, delta2, tol2, bb, rr_out, rr_true;
+  double rr_ref[2];
+  int iter_par[2]; // iteration number seen by kernels of parity q
+  int ru[2];
+  int max_iter, iter, done, converged, breakdown, nb_ru, active;
+};
+
+struct BParts
+{
+  double* pq;
+  double* rz[2];
+  double* rzru[2];
+  double* rr;
+  double* rr64;
+  double* bb;
+};
+
+struct Seg
+{
+  const int32_t* rows;   // nb_seg + 1 row offsets
+  const int32_t* blocks; // nb_seg + 1 workgroup offsets
+  const int32_t* owner;  // segment of each workgroup
+  int rows_per_block;
+};
+
+struct Ctx
+{
+  int s;        // segment
+  int b0, b1;   // its workgroups
+  int32_t r0, r1; // rows of this workgroup
+};
+
+__device__ __forceinline__ Ctx context(const Seg& g)
+{
+  Ctx c;
+  c.s = g.owner[blockIdx.x];
+  c.b0 = g.blocks[c.s];
+  c.b1 = g.blocks[c.s + 1];
+  const int32_t sr0 = g.rows[c.s], sr1 = g.rows[c.s + 1];
+  c.r0 = sr0 + (int32_t)(blockIdx.x - c.b0) * g.rows_per_block;
+  c.r1 = min(sr1, c.r0 + g.rows_per_block);
+  return c;
+}
+
+// sum of the segment's partials p[b0..b1), identical in every workgroup
+__device__ __forceinline__ double seg_sum(const double* __restrict__ p, int b0, int b1)
+{
+  __shared__ double lds[kBT / kWave];
+  double s = 0.0;
+  for (int i = b0 + threadIdx.x; i < b1; i += kBT)
+    s += p[i];
+  s = wave_reduce_sum(s);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = s;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBT / kWave; ++w)
+    t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ void put(double v, double* __restrict__ out)
+{
+  __shared__ double lds[kBT / kWave];
+  v = wave_reduce_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBT / kWave; ++w)
+      t += lds[w];
+    out[blockIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool running(const SegState& st) { return st.active && !st.done && !st.breakdown; }
+
+__device__ __forceinline__ bool reliable(const SegState& st, double rr, int k)
+{
+  return rr <= st.delta2 * st.rr_ref[(k - 1) & 1] || rr <= st.tol2 || k + 1 >= st.max_iter || !isfinite(rr);
+}
+
+// Row product of G lanes: returns the row sum in lane 0 of the group.
+template <int G, class VT, class XT>
+__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                          const VT* __restrict__ v, const XT* __restrict__ x, int32_t row, int lane)
+{
+  using A = typename std::conditional<sizeof(VT) == 4, float, double>::type;
+  A sum = 0;
+  for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += G)
+    sum += (A)v[k] * (A)x[ci[k]];
+  return (double)group_reduce_sum<G>(sum);
+}
+
+// q = A32 p; partial p.q
+template <int G>
+__global__ void __launch_bounds__(kBT)
+k_spmv_pq(Seg g, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const float* __restrict__ v,
+          const float* __restrict__ p, float* __restrict__ q, const SegState* __restrict__ st, BParts P)
+{
+  const Ctx c = context(g);
+  if (!running(st[c.s]))
+    return;
+  const int lane = threadIdx.x & (G - 1);
+  double part = 0.0;
+  for (int32_t r = c.r0 + threadIdx.x / G; r < c.r1; r += kBT / G) {
+    const double y = row_dot<G>(rp, ci, v, p, r, lane);
+    if (lane == 0) {
+      q[r] = (float)y;
+      part += y * (double)p[r];
+    }
+  }
+  put(part, P.pq);
+}
+
+__device__ __forceinline__ double rho_prev(const SegState& st, const BParts& P, const Ctx& c, int par)
+{
+  const int q = par ^ 1;
+  return st.ru[q] ? seg_sum(P.rzru[q], c.b0, c.b1) : seg_sum(P.rz[q], c.b0, c.b1);
+}
+
+// x += alpha p; r -= alpha q; z = D^-1 r; partials r.z (parity par) and r.r
+__global__ void __launch_bounds__(kBT)
+k_update(Seg g, double* __restrict__ x, float* __restrict__ r, float* __restrict__ z, const float* __restrict__ p,
+         const float* __restrict__ q, const float* __restrict__ dinv, SegState* __restrict__ st, BParts P, int par)
+{
+  const Ctx c = context(g);
+  if (!running(st[c.s]))
+    return;
+  const double pq = seg_sum(P.pq, c.b0, c.b1);
+  const double rho = rho_prev(st[c.s], P, c, par);
+  if (!(pq > 0.0) || !isfinite(pq)) {
+    if (blockIdx.x == c.b0 && threadIdx.x == 0)
+      st[c.s].breakdown = 1;
+    return;
+  }
+  const double alpha = rho / pq;
+  const float af = (float)alpha;
+  double rz = 0.0, rr = 0.0;
+  for (int32_t i = c.r0 + threadIdx.x; i < c.r1; i += kBT) {
+    x[i] += alpha * (double)p[i];
+    const float rv = r[i] - af * q[i];
+    r[i] = rv;
+    const float zv = dinv[i] * rv;
+    z[i] = zv;
+    rz += (double)rv * (double)zv;
+    rr += (double)rv * (double)rv;
+  }
+  put(rz, P.rz[par]);
+  put(rr, P.rr);
+}
+
+// r64 = b - A64 x on the segments that take a reliable update this
+// iteration (or all active segments when always != 0); partial r64.r64.
+template <int G>
+__global__ void __launch_bounds__(kBT)
+k_residual(Seg g, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const double* __restrict__ v,
+           const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r64,
+           const SegState* __restrict__ st, BParts P, int par, int always)
+{
+  const Ctx c = context(g);
+  const SegState& S = st[c.s];
+  if (always) {
+    if (!S.active)
+      return;
+  }
+  else {
+    if (!running(S))
+      return;
+    if (!reliable(S, seg_sum(P.rr, c.b0, c.b1), S.iter_par[par]))
+      return;
+  }
+  const int lane = threadIdx.x & (G - 1);
+  double part = 0.0;
+  for (int32_t r = c.r0 + threadIdx.x / G; r < c.r1; r += kBT / G) {
+    const double y = row_dot<G>(rp, ci, v, x, r, lane);
+    if (lane == 0) {
+      const double rv = b[r] - y;
+      r64[r] = rv;
+      part += rv * rv;
+    }
+  }
+  put(part, P.rr64);
+}
+
+// Convergence, reliable-update bookkeeping and p = z + beta p.
+__global__ void __launch_bounds__(kBT)
+k_pupdate(Seg g, float* __restrict__ p, float* __restrict__ z, float* __restrict__ r, const double* __restrict__ r64,
+          const float* __restrict__ dinv, SegState* __restrict__ st, BParts P, int par)
+{
+  const Ctx c = context(g);
+  SegState& S = st[c.s];
+  if (!S.active || S.done)
+    return;
+  const bool lead = blockIdx.x == c.b0 && threadIdx.x == 0;
+  const int k = S.iter_par[par];
+  if (S.breakdown) {
+    if (lead) {
+      S.done = 1;
+      S.iter = k;
+    }
+    return;
+  }
+  const double rz = seg_sum(P.rz[par], c.b0, c.b1);
+  const double rr = seg_sum(P.rr, c.b0, c.b1);
+  const double rho = rho_prev(S, P, c, par);
+  const double beta = (rho != 0.0) ? rz / rho : 0.0;
+  const bool ru = reliable(S, rr, k);
+  if (ru) {
+    const double rr64 = seg_sum(P.rr64, c.b0, c.b1);
+    if (rr64 <= S.tol2 || k + 1 >= S.max_iter || !isfinite(rr64)) {
+      if (lead) {
+        S.done = 1;
+        S.converged = rr64 <= S.tol2;
+        S.iter = k + 1;
+        S.rr_out = rr64;
+        S.nb_ru += 1;
+      }
+      return;
+    }
+    if (lead) {
+      S.ru[par] = 1;
+      S.rr_ref[par] = rr64;
+      S.rr_out = rr64;
+      S.nb_ru += 1;
+    }
+  }
+  else if (lead) {
+    S.ru[par] = 0;
+    S.rr_ref[par] = S.rr_ref[par ^ 1];
+    S.rr_out = rr;
+  }
+  if (lead) {
+    S.iter = k + 1;
+    S.iter_par[par ^ 1] = k + 1;
+  }
+  const float bt = (float)beta;
+  double rzn = 0.0;
+  for (int32_t i = c.r0 + threadIdx.x; i < c.r1; i += kBT) {
+    float zv;
+    if (ru) {
+      const float rv = (float)r64[i];
+      r[i] = rv;
+      zv = dinv[i] * rv;
+      z[i] = zv;
+      rzn += (double)rv * (double)zv;
+    }
+    else {
+      zv = z[i];
+    }
+    p[i] = zv + bt * p[i];
+  }
+  if (ru)
+    put(rzn, P.rzru[par]);
+}
+
+// r = r64; z = D^-1 r; p = z; partials r.z (parity 1), r.r, b.b
+__global__ void __launch_bounds__(kBT)
+k_init(Seg g, const double* __restrict__ r64, const double* __restrict__ b, float* __restrict__ r,
+       float* __restrict__ z, float* __restrict__ p, const float* __restrict__ dinv, const SegState* __restrict__ st,
+       BParts P)
+{
+  const Ctx c = context(g);
+  if (!st[c.s].active)
+    return;
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  for (int32_t i = c.r0 + threadIdx.x; i < c.r1; i += kBT) {
+    const double rd = r64[i];
+    const float rv = (float)rd;
+    const float zv = dinv[i] * rv;
+    r[i] = rv;
+    z[i] = zv;
+    p[i] = zv;
+    rz += (double)rv * (double)zv;
+    rr += rd * rd;
+    bb += b[i] * b[i];
+  }
+  put(rz, P.rz[1]);
+  put(rr, P.rr);
+  put(bb, P.bb);
+}
+
+// one workgroup per segment
+__global__ void __launch_bounds__(kBT)
+k_init_finalize(Seg g, SegState* __restrict__ st, BParts P)
+{
+  const int s = blockIdx.x;
+  SegState& S = st[s];
+  if (!S.active)
+    return;
+  const int b0 = g.blocks[s], b1 = g.blocks[s + 1];
+  const double rz = seg_sum(P.rz[1], b0, b1);
+  const double rr = seg_sum(P.rr, b0, b1);
+  const double bb = seg_sum(P.bb, b0, b1);
+  if (threadIdx.x == 0) {
+    S.bb = bb;
+    S.tol2 = fmax(S.rtol2 * bb, S.atol2);
+    S.rr_ref[1] = rr;
+    S.ru[1] = 0;
+    S.rr_out = rr;
+    S.iter = 0;
+    S.iter_par[0] = 0;
+    S.iter_par[1] = 0;
+    S.done = (rr <= S.tol2 || rz == 0.0) ? 1 : 0;
+    S.converged = S.done;
+  }
+}
+
+__global__ void __launch_bounds__(kBT)
+k_true_finalize(Seg g, SegState* __restrict__ st, BParts P)
+{
+  const int s = blockIdx.x;
+  SegState& S = st[s];
+  if (!S.active)
+    return;
+  const double rr = seg_sum(P.rr64, g.blocks[s], g.blocks[s + 1]);
+  if (threadIdx.x == 0)
+    S.rr_true = rr;
+}
+
+__global__ void k_initial_guess(int32_t n, const uint8_t* __restrict__ elim, const double* __restrict__ elim_value,
+                                double* __restrict__ x)
+{
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n)
+    x[r] = (elim[r] == 1 || elim[r] == 2) ? elim_value[r] : 0.0;
+}
+
+__global__ void k_prepare(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                          const double* __restrict__ v, int64_t nnz, float* __restrict__ v32,
+                          float* __restrict__ dinv)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t k = t; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+    v32[k] = (float)v[k];
+  for (int64_t r = t; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+      if (ci[k] == r)
+        d = v[k];
+    dinv[r] = (float)(d != 0.0 ? 1.0 / d : 1.0);
+  }
+}
+
+} // namespace batch
+} // namespace afem
+
+using namespace afem;
+using namespace afem::batch;
+
+struct afem_batch
+{
+  afem_system* sys = nullptr;
+  int nb_seg = 0;
+  std::vector<int32_t> seg_rows;
+  DBuf<int32_t> d_rows, d_blocks, d_owner;
+  int lanes = 8, tiles = 1, nblk = 0;
+  std::vector<int32_t> blocks_host;
+  DBuf<float> v32, dinv, r32, z32, p32, q32;
+  DBuf<double> r64, parts;
+  DBuf<SegState> st;
+  SegState* host_st = nullptr;
+  bool prepared = false;
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  ~afem_batch()
+  {
+    if (graph)
+      (void)hipGraphExecDestroy(graph);
+    if (host_st)
+      (void)hipHostFree(host_st);
+  }
+};
+
+namespace {
+
+// rows per workgroup: `tiles` strips of 256 rows (the SpMV kernels give each
+// row `lanes` lanes and walk the strip 256 / lanes rows at a time)
+int rows_per_block(const afem_batch* b) { return b->tiles * kBT; }
+
+void drop_graph(afem_batch* b)
+{
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->graph_key = -1;
+}
+
+void layout(afem_batch* b)
+{
+  const int rpb = rows_per_block(b);
+  b->blocks_host.assign(b->nb_seg + 1, 0);
+  for (int s = 0; s < b->nb_seg; ++s) {
+    const int32_t rows = b->seg_rows[s + 1] - b->seg_rows[s];
+    b->blocks_host[s + 1] = b->blocks_host[s] + std::max(1, (rows + rpb - 1) / rpb);
+  }
+  b->nblk = b->blocks_host[b->nb_seg];
+  std::vector<int32_t> owner(b->nblk);
+  for (int s = 0; s < b->nb_seg; ++s)
+    for (int k = b->blocks_host[s]; k < b->blocks_host[s + 1]; ++k)
+      owner[k] = s;
+  b->d_blocks.alloc(b->nb_seg + 1);
+  b->d_blocks.upload(b->blocks_host.data(), b->blocks_host.size());
+  b->d_owner.alloc(b->nblk);
+  b->d_owner.upload(owner.data(), owner.size());
+  b->parts.alloc((size_t)8 * b->nblk);
+  drop_graph(b);
+}
+
+Seg seg_of(const afem_batch* b)
+{
+  return Seg{ b->d_rows.p, b->d_blocks.p, b->d_owner.p, rows_per_block(b) };
+}
+
+BParts parts_of(afem_batch* b)
+{
+  double* p = b->parts.p;
+  const int n = b->nblk;
+  return BParts{ p, { p + n, p + 2 * n }, { p + 3 * n, p + 4 * n }, p + 5 * n, p + 6 * n, p + 7 * n };
+}
+
+template <class F>
+void with_lanes(int lanes, F&& f)
+{
+  switch (lanes) {
+  case 4: f(std::integral_constant<int, 4>{}); break;
+  case 16: f(std::integral_constant<int, 16>{}); break;
+  default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
+
+void launch_residual(afem_batch* b, int par, int always)
+{
+  afem_system* s = b->sys;
+  const Seg g = seg_of(b);
+  const BParts P = parts_of(b);
+  with_lanes(b->lanes, [&](auto tag) {
+    constexpr int G = decltype(tag)::value;
+    hipLaunchKernelGGL((k_residual<G>), dim3(b->nblk), dim3(kBT), 0, stream(), g, s->row_index, s->columns,
+                       s->values, s->x.p, s->rhs.p, b->r64.p, b->st.p, P, par, always);
+  });
+}
+
+void launch_iteration(afem_batch* b, int par)
+{
+  afem_system* s = b->sys;
+  const Seg g = seg_of(b);
+  const BParts P = parts_of(b);
+  with_lanes(b->lanes, [&](auto tag) {
+    constexpr int G = decltype(tag)::value;
+    hipLaunchKernelGGL((k_spmv_pq<G>), dim3(b->nblk), dim3(kBT), 0, stream(), g, s->row_index, s->columns, b->v32.p,
+                       b->p32.p, b->q32.p, b->st.p, P);
+  });
+  hipLaunchKernelGGL(k_update, dim3(b->nblk), dim3(kBT), 0, stream(), g, s->x.p, b->r32.p, b->z32.p, b->p32.p,
+                     b->q32.p, b->dinv.p, b->st.p, P, par);
+  launch_residual(b, par, 0);
+  hipLaunchKernelGGL(k_pupdate, dim3(b->nblk), dim3(kBT), 0, stream(), g, b->p32.p, b->z32.p, b->r32.p, b->r64.p,
+                     b->dinv.p, b->st.p, P, par);
+}
+
+void read_states(afem_batch* b)
+{
+  AFEM_HIP(hipMemcpyAsync(b->host_st, b->st.p, sizeof(SegState) * b->nb_seg, hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+}
+
+void write_states(afem_batch* b)
+{
+  AFEM_HIP(hipMemcpyAsync(b->st.p, b->host_st, sizeof(SegState) * b->nb_seg, hipMemcpyHostToDevice, stream()));
+}
+
+void prepare(afem_batch* b)
+{
+  afem_system* s = b->sys;
+  const int32_t n = s->nb_dof;
+  if (b->v32.n != (size_t)s->nnz) {
+    drop_graph(b);
+    b->v32.alloc((size_t)s->nnz);
+  }
+  if (b->r32.n != (size_t)n) {
+    drop_graph(b);
+    b->dinv.alloc(n);
+    b->r32.alloc(n);
+    b->z32.alloc(n);
+    b->p32.alloc(n);
+    b->q32.alloc(n);
+    b->r64.alloc(n);
+  }
+  hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, stream(), n, s->row_index, s->columns, s->values, s->nnz,
+                     b->v32.p, b->dinv.p);
+  check_launch("batch_prepare");
+  b->prepared = true;
+}
+
+// the iterations of one chunk, captured once and replayed
+void run_chunk(afem_batch* b, int chunk, bool use_graph)
+{
+  if (!use_graph) {
+    for (int j = 0; j < chunk; ++j)
+      launch_iteration(b, j & 1);
+    check_launch("batch_iteration");
+    return;
+  }
+  const int key = chunk * 1000 + b->lanes * 10 + b->tiles;
+  if (!b->graph || b->graph_key != key) {
+    drop_graph(b);
+    hipGraph_t gr;
+    AFEM_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < chunk; ++j)
+      launch_iteration(b, j & 1);
+    AFEM_HIP(hipStreamEndCapture(stream(), &gr));
+    const hipError_t e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    AFEM_HIP(e);
+    b->graph_key = key;
+  }
+  AFEM_HIP(hipGraphLaunch(b->graph, stream()));
+}
+
+struct RunStats
+{
+  int chunks = 0;
+};
+
+// Solves the active segments (host_st[s].active) from the current x.
+RunStats run(afem_batch* b, const afem_batch_options& o, bool use_graph)
+{
+  RunStats rs;
+  const Seg g = seg_of(b);
+  const BParts P = parts_of(b);
+  afem_system* s = b->sys;
+  launch_residual(b, 0, 1);
+  hipLaunchKernelGGL(k_init, dim3(b->nblk), dim3(kBT), 0, stream(), g, b->r64.p, s->rhs.p, b->r32.p, b->z32.p,
+                     b->p32.p, b->dinv.p, b->st.p, P);
+  hipLaunchKernelGGL(k_init_finalize, dim3(b->nb_seg), dim3(kBT), 0, stream(), g, b->st.p, P);
+  check_launch("batch_init");
+  const int chunk = o.chunk > 0 ? (o.chunk + 1) / 2 * 2 : 32;
+  int done_iters = 0;
+  for (;;) {
+    read_states(b);
+    bool all = true;
+    for (int k = 0; k < b->nb_seg; ++k)
+      if (b->host_st[k].active && !b->host_st[k].done)
+        all = false;
+    if (all || done_iters >= o.max_iter)
+      break;
+    run_chunk(b, chunk, use_graph);
+    done_iters += chunk;
+    rs.chunks += 1;
+  }
+  return rs;
+}
+
+} // namespace
+
+extern "C" {
+
+AFEM_API void afem_batch_default_options(afem_batch_options* o)
+{
+  if (!o)
+    return;
+  o->rtol = 1e-8;
+  o->atol = 0.0;
+  o->max_iter = 100000;
+  o->reliable_delta = 0.1;
+  o->chunk = 32;
+  o->use_graph = 1;
+  o->use_initial_guess = 0;
+}
+
+AFEM_API int afem_batch_create(afem_system_t sys, int nb_segment, const int32_t* segment_rows, afem_batch_t* out)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys && segment_rows && out, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_segment >= 1, AFEM_ERR_ARGUMENT, "at least one segment");
+  AFEM_REQUIRE(segment_rows[0] == 0 && segment_rows[nb_segment] == sys->nb_dof, AFEM_ERR_ARGUMENT,
+               "segment offsets must start at 0 and end at the number of DoFs");
+  for (int k = 0; k < nb_segment; ++k)
+    AFEM_REQUIRE(segment_rows[k + 1] > segment_rows[k], AFEM_ERR_ARGUMENT, "empty or decreasing segment");
+  auto* b = new afem_batch();
+  b->sys = sys;
+  b->nb_seg = nb_segment;
+  b->seg_rows.assign(segment_rows, segment_rows + nb_segment + 1);
+  b->d_rows.alloc(nb_segment + 1);
+  b->d_rows.upload(b->seg_rows.data(), b->seg_rows.size());
+  b->st.alloc(nb_segment);
+  AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->host_st), sizeof(SegState) * nb_segment, hipHostMallocDefault));
+  layout(b);
+  *out = b;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_batch_destroy(afem_batch_t b)
+{
+  AFEM_GUARD_BEGIN
+  delete b;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_batch_set_tiles(afem_batch_t b, int lanes, int tiles)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(b, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16, AFEM_ERR_ARGUMENT, "lanes must be 4, 8 or 16");
+  AFEM_REQUIRE(tiles >= 1 && tiles <= 16, AFEM_ERR_ARGUMENT, "tiles must be in [1, 16]");
+  b->lanes = lanes;
+  b->tiles = tiles;
+  layout(b);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_batch_solve(afem_batch_t b, const afem_batch_options* opt, afem_batch_result* results,
+                              double* total_ms)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(b && opt, AFEM_ERR_ARGUMENT, "null argument");
+  afem_system* s = b->sys;
+  system_require_matrix(s);
+  AFEM_REQUIRE(s->block == 1, AFEM_ERR_NOT_IMPLEMENTED, "batched solves are for scalar systems");
+  AFEM_REQUIRE(opt->rtol >= 0.0 && opt->atol >= 0.0 && opt->max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
+  const auto t0 = std::chrono::steady_clock::now();
+  prepare(b);
+  if (!opt->use_initial_guess) {
+    hipLaunchKernelGGL(k_initial_guess, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
+                       s->elim_info.p, s->elim_value.p, s->x.p);
+    check_launch("batch_initial_guess");
+  }
+  const double delta = opt->reliable_delta > 0.0 ? opt->reliable_delta : 0.1;
+  std::vector<int> iters(b->nb_seg, 0), nru(b->nb_seg, 0), conv(b->nb_seg, 0);
+  std::vector<double> relres(b->nb_seg, 0.0);
+  std::vector<char> want(b->nb_seg, 1);
+  for (int round = 0; round < 5; ++round) {
+    bool any = false;
+    for (int k = 0; k < b->nb_seg; ++k) {
+      SegState z{};
+      z.rtol2 = opt->rtol * opt->rtol;
+      z.atol2 = opt->atol * opt->atol;
+      z.delta2 = delta * delta;
+      z.max_iter = std::max(0, opt->max_iter - iters[k]);
+      z.active = (want[k] && z.max_iter > 0) ? 1 : 0;
+      any = any || z.active;
+      b->host_st[k] = z;
+    }
+    if (!any)
+      break;
+    write_states(b);
+    run(b, *opt, opt->use_graph != 0);
+    // FP64 true residual of every segment that ran
+    read_states(b);
+    std::vector<SegState> after(b->host_st, b->host_st + b->nb_seg);
+    launch_residual(b, 0, 1);
+    hipLaunchKernelGGL(k_true_finalize, dim3(b->nb_seg), dim3(kBT), 0, stream(), seg_of(b), b->st.p, parts_of(b));
+    check_launch("batch_true_residual");
+    read_states(b);
+    bool again = false;
+    for (int k = 0; k < b->nb_seg; ++k) {
+      if (!after[k].active)
+        continue;
+      iters[k] += after[k].iter;
+      nru[k] += after[k].nb_ru;
+      const double bn = std::sqrt(after[k].bb);
+      const double tr = std::sqrt(b->host_st[k].rr_true);
+      relres[k] = bn > 0.0 ? tr / bn : tr;
+      conv[k] = (relres[k] <= opt->rtol || tr <= opt->atol) ? 1 : 0;
+      // the recursive residual met the tolerance but the true one does not:
+      // run that segment again from its current iterate
+      want[k] = (!conv[k] && after[k].converged && after[k].iter > 0 && std::isfinite(relres[k])) ? 1 : 0;
+      again = again || want[k];
+    }
+    if (!again)
+      break;
+  }
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  const auto t1 = std::chrono::steady_clock::now();
+  bool all = true;
+  for (int k = 0; k < b->nb_seg; ++k) {
+    if (results) {
+      results[k].iterations = iters[k];
+      results[k].converged = conv[k];
+      results[k].reliable_updates = nru[k];
+      results[k].relres = relres[k];
+    }
+    all = all && conv[k];
+  }
+  if (total_ms)
+    *total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return all ? AFEM_OK : AFEM_ERR_NOT_CONVERGED;
+  AFEM_GUARD_END
+}
+
+// Times one chunk of iterations for each (lanes, tiles) candidate from the
+// initial state and keeps the fastest; x is left at the initial guess.
+AFEM_API int afem_batch_autotune(afem_batch_t b, int* lanes, int* tiles, double* best_ms)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(b, AFEM_ERR_ARGUMENT, "null argument");
+  afem_system* s = b->sys;
+  system_require_matrix(s);
+  prepare(b);
+  struct Cand
+  {
+    int lanes, tiles;
+  };
+  const Cand cands[] = { { 4, 1 }, { 4, 2 }, { 8, 1 }, { 8, 2 }, { 8, 4 }, { 16, 1 }, { 16, 2 } };
+  hipEvent_t e0, e1;
+  AFEM_HIP(hipEventCreate(&e0));
+  AFEM_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  Cand pick = cands[2];
+  afem_batch_options o;
+  afem_batch_default_options(&o);
+  for (const Cand& c : cands) {
+    b->lanes = c.lanes;
+    b->tiles = c.tiles;
+    layout(b);
+    float t_best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      hipLaunchKernelGGL(k_initial_guess, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
+                         s->elim_info.p, s->elim_value.p, s->x.p);
+      for (int k = 0; k < b->nb_seg; ++k) {
+        SegState z{};
+        z.rtol2 = 0.0; // run every iteration of the chunk
+        z.delta2 = 0.01;
+        z.max_iter = 1 << 30;
+        z.active = 1;
+        b->host_st[k] = z;
+      }
+      write_states(b);
+      const Seg g = seg_of(b);
+      const BParts P = parts_of(b);
+      launch_residual(b, 0, 1);
+      hipLaunchKernelGGL(k_init, dim3(b->nblk), dim3(kBT), 0, stream(), g, b->r64.p, s->rhs.p, b->r32.p, b->z32.p,
+                         b->p32.p, b->dinv.p, b->st.p, P);
+      hipLaunchKernelGGL(k_init_finalize, dim3(b->nb_seg), dim3(kBT), 0, stream(), g, b->st.p, P);
+      AFEM_HIP(hipEventRecord(e0, stream()));
+      run_chunk(b, 16, true);
+      AFEM_HIP(hipEventRecord(e1, stream()));
+      AFEM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      AFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0 && ms < t_best)
+        t_best = ms;
+    }
+    if (t_best < best) {
+      best = t_best;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->lanes = pick.lanes;
+  b->tiles = pick.tiles;
+  layout(b);
+  hipLaunchKernelGGL(k_initial_guess, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
+                     s->elim_info.p, s->elim_value.p, s->x.p);
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  if (lanes)
+    *lanes = pick.lanes;
+  if (tiles)
+    *tiles = pick.tiles;
+  if (best_ms)
+    *best_ms = best / 16.0;
+  AFEM_GUARD_END
+}
+
+} // extern "C"

@@ -54,6 +54,7 @@ struct Parts
   double* rr;
   double* rr64;
   double* bb;
+  double* zq; // flexible PCG: z_{k+1}.q_k
 };
 
 // ---- block reductions --------------------------------------------------------------------
@@ -92,6 +93,7 @@ __device__ __forceinline__ void put_part(double v, double* __restrict__ out)
       t += lds[w];
     out[blockIdx.x] = t;
   }
+  __syncthreads(); // the next call reuses lds
 }
 
 // Contiguous chunk of tiles owned by this block (XCD-aware order).
@@ -689,6 +691,258 @@ __global__ void __launch_bounds__(kBlock) k_true_residual_finalize(Parts P, int 
     st->rr_true = rr;
 }
 
+// ---- flexible PCG with a sparse preconditioner matrix ------------------------------------------
+//
+// z = M r is an SpMV with the preconditioner's values (FP32, pattern of A),
+// so an iteration is: q = A p (p.q); x += alpha p, r -= alpha q (r.r);
+// [reliable update: r64 = b - A64 x]; z = M r (r.z, z.q); p = z + beta p
+// with the Polak-Ribiere beta = z_{k+1}.(r_{k+1} - r_k) / rho_k
+// = -alpha z_{k+1}.q_k / rho_k, which keeps CG's convergence when M is only
+// approximately symmetric.  After a reliable update r is replaced and
+// beta = rho_{k+1} / rho_k.
+
+// Input of z = M r on iteration k: the FP32 residual, or the FP64 one on an
+// iteration that takes a reliable update (always on the first call).
+struct PrecIn
+{
+  const float* r32;
+  const double* r64;
+  bool from64;
+  __device__ __forceinline__ float at(int64_t c) const { return from64 ? (float)r64[c] : r32[c]; }
+};
+
+template <int INIT>
+__device__ __forceinline__ void prec_row(int64_t row, float zrow, const PrecIn& in, float* __restrict__ r32,
+                                         float* __restrict__ z, float* __restrict__ p, const float* __restrict__ q,
+                                         const double* __restrict__ b, double& rz, double& zq, double& rr,
+                                         double& bb)
+{
+  const float rin = in.at(row);
+  z[row] = zrow;
+  if (in.from64)
+    r32[row] = rin; // no other thread reads r32 on such an iteration
+  rz += (double)zrow * (double)rin;
+  if constexpr (INIT) {
+    p[row] = zrow;
+    const double bv = b[row], r64v = in.r64[row];
+    bb += bv * bv;
+    rr += r64v * r64v;
+  }
+  else {
+    zq += (double)zrow * (double)q[row];
+  }
+}
+
+template <int INIT>
+__device__ __forceinline__ bool prec_open(const State* st, const Parts& P, int nparts, int k, PrecIn& in)
+{
+  if constexpr (INIT) {
+    in.from64 = true;
+    return true;
+  }
+  else {
+    if (st->done || st->breakdown)
+      return false;
+    in.from64 = reliable_decision(st, sum_parts(P.rr, nparts), k);
+    return true;
+  }
+}
+
+template <int INIT>
+__device__ __forceinline__ void prec_parts(const Parts& P, int k, double rz, double zq, double rr, double bb)
+{
+  put_part(rz, P.rz[k & 1]);
+  if constexpr (INIT) {
+    put_part(rr, P.rr);
+    put_part(bb, P.bb);
+  }
+  else {
+    put_part(zq, P.zq);
+  }
+}
+
+template <int G, int INIT>
+__global__ void __launch_bounds__(kBlock)
+k_prec_csr(int k, int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+           const float* __restrict__ mv, PrecIn in, float* __restrict__ r32, float* __restrict__ z,
+           float* __restrict__ p, const float* __restrict__ q, const double* __restrict__ b, Parts P, int nparts,
+           const State* st)
+{
+  if (!prec_open<INIT>(st, P, nparts, k, in))
+    return;
+  constexpr int RPT = kBlock / G;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + RPT - 1) / RPT, t0, t1);
+  const int lane = threadIdx.x & (G - 1);
+  const int sub = threadIdx.x / G;
+  double rz = 0.0, zq = 0.0, rr = 0.0, bb = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * RPT + sub;
+    if (row < n) {
+      float sum = 0.f;
+      for (int32_t e = rp[row] + lane; e < rp[row + 1]; e += G)
+        sum += mv[e] * in.at(ci[e]);
+      sum = group_reduce_sum<G>(sum);
+      if (lane == 0)
+        prec_row<INIT>(row, sum, in, r32, z, p, q, b, rz, zq, rr, bb);
+    }
+  }
+  prec_parts<INIT>(P, k, rz, zq, rr, bb);
+}
+
+template <int INIT>
+__global__ void __launch_bounds__(kBlock)
+k_prec_sell(int k, int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+            const float* __restrict__ mv, PrecIn in, float* __restrict__ r32, float* __restrict__ z,
+            float* __restrict__ p, const float* __restrict__ q, const double* __restrict__ b, Parts P, int nparts,
+            const State* st)
+{
+  if (!prec_open<INIT>(st, P, nparts, k, in))
+    return;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  double rz = 0.0, zq = 0.0, rr = 0.0, bb = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row < n) {
+      const int64_t s = row >> 6;
+      const int64_t base = (int64_t)off[s] * 64 + (row & 63);
+      const int w = off[s + 1] - off[s];
+      float s0 = 0.f, s1 = 0.f;
+      int j = 0;
+      for (; j + 1 < w; j += 2) {
+        const int64_t i0 = base + (int64_t)j * 64, i1 = i0 + 64;
+        s0 += mv[i0] * in.at(col[i0]);
+        s1 += mv[i1] * in.at(col[i1]);
+      }
+      if (j < w) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        s0 += mv[i0] * in.at(col[i0]);
+      }
+      prec_row<INIT>(row, s0 + s1, in, r32, z, p, q, b, rz, zq, rr, bb);
+    }
+  }
+  prec_parts<INIT>(P, k, rz, zq, rr, bb);
+}
+
+// x += alpha p, r -= alpha q; partial r.r.
+__global__ void __launch_bounds__(kBlock)
+k_update_flex(int k, int32_t n, double* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+              const float* __restrict__ q, Parts P, int nparts, State* st)
+{
+  if (st->done || st->breakdown)
+    return;
+  const double pq = sum_parts(P.pq, nparts);
+  const double rho = sum_parts(P.rz[(k - 1) & 1], nparts);
+  if (!(pq > 0.0) || !isfinite(pq) || !(rho > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      st->breakdown = 1;
+    return;
+  }
+  const double alpha = rho / pq;
+  const float alpha_t = (float)alpha;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  double rr = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row >= n)
+      continue;
+    x[row] += alpha * (double)p[row];
+    const float rv = r[row] - alpha_t * q[row];
+    r[row] = rv;
+    rr += (double)rv * (double)rv;
+  }
+  put_part(rr, P.rr);
+}
+
+// Convergence test (FP64 residual of a reliable update), beta, p = z + beta p.
+__global__ void __launch_bounds__(kBlock)
+k_pupdate_flex(int k, int32_t n, float* __restrict__ p, const float* __restrict__ z, Parts P, int nparts, State* st)
+{
+  if (st->done)
+    return;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (st->breakdown) {
+    if (lead) {
+      st->done = 1;
+      st->iter = k;
+    }
+    return;
+  }
+  const double rr = sum_parts(P.rr, nparts);
+  const double rho = sum_parts(P.rz[(k - 1) & 1], nparts);
+  const double pq = sum_parts(P.pq, nparts);
+  const double rz = sum_parts(P.rz[k & 1], nparts);
+  const int q = k & 1;
+  const bool ru = reliable_decision(st, rr, k);
+  double beta;
+  if (ru) {
+    const double rr64 = sum_parts(P.rr64, nparts);
+    if (rr64 <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr64)) {
+      if (lead) {
+        st->done = 1;
+        st->converged = rr64 <= st->tol2;
+        st->iter = k + 1;
+        st->rr_out = rr64;
+        st->nb_ru += 1;
+      }
+      return;
+    }
+    if (lead) {
+      st->rr_ref[q] = rr64;
+      st->rr_out = rr64;
+      st->nb_ru += 1;
+    }
+    beta = rz / rho;
+  }
+  else {
+    const double zq = sum_parts(P.zq, nparts);
+    if (lead) {
+      st->rr_ref[q] = st->rr_ref[(k - 1) & 1];
+      st->rr_out = rr;
+    }
+    beta = -(rho / pq) * zq / rho;
+  }
+  if (!(rz > 0.0) || !isfinite(rz) || !isfinite(beta)) { // M is not positive on this residual
+    if (lead) {
+      st->breakdown = 1;
+      st->done = 1;
+      st->iter = k + 1;
+    }
+    return;
+  }
+  if (lead) {
+    st->ru[q] = 0;
+    st->iter = k + 1;
+  }
+  const float beta_t = (float)beta;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row < n)
+      p[row] = z[row] + beta_t * p[row];
+  }
+}
+
+// SELL copy of FP32 values given in CSR order (same slice layout as A's).
+__global__ void k_sell_fill_f32(int32_t nrows, const int32_t* __restrict__ rp, const float* __restrict__ v,
+                                const int32_t* __restrict__ off, float* __restrict__ out)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t rows_pad = ((int64_t)nrows + 63) / 64 * 64;
+  if (r >= rows_pad)
+    return;
+  const int64_t s = r >> 6;
+  const int lane = (int)(r & 63);
+  const int w = off[s + 1] - off[s];
+  const int32_t len = (r < nrows) ? rp[r + 1] - rp[r] : 0;
+  const int32_t b0 = (r < nrows) ? rp[r] : 0;
+  for (int j = 0; j < w; ++j)
+    out[((int64_t)off[s] + j) * 64 + lane] = (j < len) ? v[b0 + j] : 0.f;
+}
+
 } // namespace afem
 
 // ---- workspace --------------------------------------------------------------------------------
@@ -720,6 +974,10 @@ struct afem_solver_ws
   bool pattern_valid = false;
   int format = -1; // chosen SpMV format
   int lanes = 16;
+  // sparse preconditioner matrix (pattern of A, FP32), CSR and SELL order
+  afem::DBuf<float> spai32, spai_sell32;
+  bool has_spai = false;
+  bool spai_sell_valid = false;
   ~afem_solver_ws()
   {
     if (host_st)
@@ -755,6 +1013,7 @@ static Parts parts_of(afem_solver_ws* w)
   P.rr = b + 5 * kMaxBlocks;
   P.rr64 = b + 6 * kMaxBlocks;
   P.bb = b + 7 * kMaxBlocks;
+  P.zq = b + 8 * kMaxBlocks;
   return P;
 }
 
@@ -778,9 +1037,11 @@ static afem_solver_ws* workspace(afem_system* s)
     w->values_valid = false;
     w->pattern_valid = false;
     w->format = -1;
+    w->has_spai = false;
+    w->spai_sell_valid = false;
   }
   if (!w->parts.p) {
-    w->parts.alloc((size_t)8 * kMaxBlocks);
+    w->parts.alloc((size_t)9 * kMaxBlocks);
     w->parts.zero();
     w->st.alloc(1);
     AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w->host_st), sizeof(State), hipHostMallocDefault));
@@ -792,6 +1053,8 @@ static afem_solver_ws* workspace(afem_system* s)
     w->pattern_valid = false;
     w->values_valid = false;
     w->format = -1;
+    w->has_spai = false;
+    w->spai_sell_valid = false;
   }
   if (s->matrix_changed) {
     w->values_valid = false;
@@ -1068,15 +1331,85 @@ static double true_rr(afem_system* s, afem_solver_ws* w, const Parts& P, int nbl
   return w->host_st->rr_true;
 }
 
+static void ensure_spai_sell(afem_system* s, afem_solver_ws* w)
+{
+  if (w->format != kFmtSell || w->spai_sell_valid)
+    return;
+  w->spai_sell32.alloc(w->sell_col.n);
+  const int64_t rows_pad = ((int64_t)s->nb_dof + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), s->nb_dof, s->row_index,
+                     w->spai32.p, w->sell_off.p, w->spai_sell32.p);
+  check_launch("spai_sell_fill");
+  w->spai_sell_valid = true;
+}
+
+template <int INIT>
+static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk)
+{
+  ProfScope prof("fpcg_precond_spmv");
+  const PrecIn in{ w->r32.p, w->r64.p, true };
+  if (w->format == kFmtSell) {
+    hipLaunchKernelGGL((k_prec_sell<INIT>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, w->sell_off.p,
+                       w->sell_col.p, w->spai_sell32.p, in, w->r32.p, w->z32.p, w->p32.p, w->q32.p, s->rhs.p, P, nblk,
+                       w->st.p);
+  }
+  else {
+    auto go = [&](auto gtag) {
+      constexpr int G = decltype(gtag)::value;
+      hipLaunchKernelGGL((k_prec_csr<G, INIT>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, s->row_index,
+                         s->columns, w->spai32.p, in, w->r32.p, w->z32.p, w->p32.p, w->q32.p, s->rhs.p, P, nblk,
+                         w->st.p);
+    };
+    switch (w->lanes) {
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 32: go(std::integral_constant<int, 32>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+    }
+  }
+  check_launch("fpcg_precond_spmv");
+}
+
+static void launch_flex_init(afem_system* s, afem_solver_ws* w, const Parts& P, int nblk)
+{
+  prec_launch<1>(s, w, -1, P, nblk);
+  hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kBlock), 0, stream(), P, nblk, w->st.p);
+  check_launch("fpcg_init_finalize");
+}
+
+static void launch_flex_iteration(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk)
+{
+  const Gate until_done{ w->st.p, kUntilDone, k, nullptr, nblk };
+  spmv_launch<float, float, kPQ>(s, w, true, w->p32.p, w->q32.p, nullptr, until_done, P.pq, nblk, "spmv_fp32");
+  {
+    ProfScope prof("fpcg_update");
+    hipLaunchKernelGGL(k_update_flex, dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, s->x.p, w->r32.p,
+                       w->p32.p, w->q32.p, P, nblk, w->st.p);
+    check_launch("fpcg_update");
+  }
+  const Gate reliable{ w->st.p, kReliable, k, P.rr, nblk };
+  spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, reliable, P.rr64, nblk,
+                                         "spmv_residual_fp64");
+  prec_launch<0>(s, w, k, P, nblk);
+  {
+    ProfScope prof("fpcg_pupdate");
+    hipLaunchKernelGGL(k_pupdate_flex, dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, w->p32.p, w->z32.p, P,
+                       nblk, w->st.p);
+    check_launch("fpcg_pupdate");
+  }
+}
+
 static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result* res)
 {
-  AFEM_REQUIRE(o.method == AFEM_SOLVER_PCG_FP64 || o.method == AFEM_SOLVER_PCG_MIXED || o.method == AFEM_SOLVER_CG_FP64,
+  AFEM_REQUIRE(o.method == AFEM_SOLVER_PCG_FP64 || o.method == AFEM_SOLVER_PCG_MIXED ||
+                 o.method == AFEM_SOLVER_CG_FP64 || o.method == AFEM_SOLVER_FPCG_SPAI,
                AFEM_ERR_ARGUMENT, "unknown solver method");
   AFEM_REQUIRE(o.preconditioner == AFEM_PRECOND_JACOBI || o.preconditioner == AFEM_PRECOND_NONE, AFEM_ERR_ARGUMENT,
                "unknown preconditioner");
   AFEM_REQUIRE(o.rtol >= 0.0 && o.atol >= 0.0 && o.max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
   AFEM_REQUIRE(o.norm >= 0 && o.norm <= 2, AFEM_ERR_ARGUMENT, "unknown stopping norm");
-  AFEM_REQUIRE(o.norm == 0 || o.method != AFEM_SOLVER_PCG_MIXED, AFEM_ERR_NOT_IMPLEMENTED,
+  AFEM_REQUIRE(o.norm == 0 || (o.method != AFEM_SOLVER_PCG_MIXED && o.method != AFEM_SOLVER_FPCG_SPAI),
+               AFEM_ERR_NOT_IMPLEMENTED,
                "the natural-norm and preconditioned-norm stopping tests are available for the FP64 solvers");
   // The reference's row-column elimination never touches column 0
   // (femutils/CsrDoFLinearSystemImpl.cc:111), so with DoF 0 eliminated the
@@ -1088,13 +1421,21 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
                "so PCG does not apply (use RowElimination or Penalty for that DoF)");
   auto t_start = std::chrono::steady_clock::now();
   afem_solver_ws* w = workspace(s);
-  const bool mixed = o.method == AFEM_SOLVER_PCG_MIXED;
+  const bool flex = o.method == AFEM_SOLVER_FPCG_SPAI;
+  if (flex) {
+    AFEM_REQUIRE(s->block == 1, AFEM_ERR_NOT_IMPLEMENTED, "the sparse preconditioner is for scalar systems");
+    AFEM_REQUIRE(w->has_spai, AFEM_ERR_STATE,
+                 "no preconditioner values: call afem_system_set_spai or afem_spai_apply after the matrix transformation");
+  }
+  const bool mixed = o.method == AFEM_SOLVER_PCG_MIXED || flex;
   const int precond = (o.method == AFEM_SOLVER_CG_FP64 || o.preconditioner == AFEM_PRECOND_NONE) ? 0 : 1;
   const int check_every = o.check_every > 0 ? o.check_every : 16;
   const double delta = o.reliable_delta > 0.0 ? o.reliable_delta : 0.1;
   ensure_derived(s, w, mixed);
   choose_format(s, w, o.spmv_lanes);
   ensure_derived(s, w, mixed);
+  if (flex)
+    ensure_spai_sell(s, w);
   const Parts P = parts_of(w);
   const int nblk = spmv_blocks(s);
 
@@ -1109,7 +1450,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   init.delta2 = delta * delta;
   init.natural = o.norm;
   int total_iter = 0, restarts = 0, nb_ru = 0;
-  bool converged = false;
+  bool converged = false, breakdown = false;
   double relres = 0.0;
   double bnorm = 0.0;
   for (;;) {
@@ -1119,7 +1460,9 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     const Gate always{ w->st.p, kAlways, 0, nullptr, nblk };
     spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, always, P.rr64, nblk,
                                            "spmv_residual_fp64");
-    if (s->block == 3)
+    if (flex)
+      launch_flex_init(s, w, P, nblk);
+    else if (s->block == 3)
       launch_init<3>(s, w, mixed, precond, o.norm == 2, P, nblk);
     else
       launch_init<1>(s, w, mixed, precond, o.norm == 2, P, nblk);
@@ -1127,7 +1470,9 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     int k = 0;
     while (!w->host_st->done && k < init.max_iter) {
       for (int j = 0; j < check_every && k < init.max_iter; ++j, ++k) {
-        if (s->block == 3)
+        if (flex)
+          launch_flex_iteration(s, w, k, P, nblk);
+        else if (s->block == 3)
           launch_iteration<3>(s, w, k, mixed, precond, P, nblk);
         else
           launch_iteration<1>(s, w, k, mixed, precond, P, nblk);
@@ -1138,6 +1483,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     nb_ru += w->host_st->nb_ru;
     bnorm = std::sqrt(w->host_st->bb);
     const bool device_converged = w->host_st->converged != 0;
+    breakdown = w->host_st->breakdown != 0;
     const double rr = true_rr(s, w, P, nblk);
     relres = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
     if (o.norm == 2)
@@ -1148,11 +1494,30 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     }
     converged = relres <= o.rtol || std::sqrt(rr) <= o.atol;
     if (converged || total_iter >= o.max_iter || restarts >= 8 || !std::isfinite(relres) ||
-        w->host_st->iter == 0)
+        w->host_st->iter == 0 || (flex && breakdown))
       break;
     // the recursive residual met the tolerance but the true one does not:
     // restart from the current iterate
     ++restarts;
+  }
+  int fallback = 0;
+  if (flex && !converged && breakdown && total_iter < o.max_iter && std::isfinite(relres)) {
+    // the preconditioner is not positive on some residual: finish with
+    // Jacobi mixed-precision PCG from the current iterate
+    afem_solver_options jo = o;
+    jo.method = AFEM_SOLVER_PCG_MIXED;
+    jo.preconditioner = AFEM_PRECOND_JACOBI;
+    jo.use_initial_guess = 1;
+    jo.max_iter = o.max_iter - total_iter;
+    afem_solve_result jr{};
+    const int rc = solve(s, jo, &jr);
+    if (rc != AFEM_OK && rc != AFEM_ERR_NOT_CONVERGED)
+      return rc;
+    total_iter += jr.iterations;
+    nb_ru += jr.reliable_updates;
+    relres = jr.relres;
+    converged = jr.converged != 0;
+    fallback = 1;
   }
   AFEM_HIP(hipStreamSynchronize(stream()));
   auto t_end = std::chrono::steady_clock::now();
@@ -1162,6 +1527,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     res->reliable_updates = nb_ru;
     res->relres = relres;
     res->solve_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    res->fallback = fallback;
   }
   return converged ? AFEM_OK : AFEM_ERR_NOT_CONVERGED;
 }
@@ -1227,6 +1593,21 @@ AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y)
   ensure_derived(sys, w, true);
   const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
   spmv_launch<float, float, kPlain>(sys, w, true, x, y, nullptr, always, nullptr, spmv_blocks(sys), "spmv_fp32");
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_system_set_spai(afem_system_t sys, const float* values)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys && values, AFEM_ERR_ARGUMENT, "null argument");
+  afem_solver_ws* w = workspace(sys);
+  AFEM_REQUIRE(sys->block == 1, AFEM_ERR_NOT_IMPLEMENTED, "the sparse preconditioner is for scalar systems");
+  if (w->spai32.n != (size_t)sys->nnz)
+    w->spai32.alloc((size_t)sys->nnz);
+  AFEM_HIP(hipMemcpyAsync(w->spai32.p, values, sizeof(float) * (size_t)sys->nnz, hipMemcpyDeviceToDevice, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  w->has_spai = true;
+  w->spai_sell_valid = false;
   AFEM_GUARD_END
 }
 

@@ -239,7 +239,12 @@ enum afem_solver_method {
   AFEM_SOLVER_PCG_FP64 = 0,      /* FP64 PCG (the Jacobi-PCG of PETSc cg+jacobi) */
   AFEM_SOLVER_PCG_MIXED = 1,     /* FP32 SpMV/preconditioner/vectors, FP64 dots and
                                     FP64 reliable-update residual r = b - A x */
-  AFEM_SOLVER_CG_FP64 = 2        /* unpreconditioned FP64 CG */
+  AFEM_SOLVER_CG_FP64 = 2,       /* unpreconditioned FP64 CG */
+  AFEM_SOLVER_FPCG_SPAI = 3      /* flexible PCG (Polak-Ribiere beta) with the sparse
+                                    preconditioner matrix given by afem_system_set_spai /
+                                    afem_spai_apply, FP32 SpMVs, FP64 reliable updates;
+                                    falls back to Jacobi mixed PCG from the current iterate
+                                    if the preconditioner is not positive on a residual */
 };
 enum afem_precond {
   AFEM_PRECOND_JACOBI = 0,       /* diagonal (scalar) or 3x3 block-diagonal inverse */
@@ -275,6 +280,7 @@ typedef struct afem_solve_result {
   double relres;         /* FP64 true relative residual ||b - A x||_2 / ||b||_2
                             (norm 2: ||M^-1 r||_2 / ||M^-1 b||_2 of the last iteration) */
   double solve_ms;       /* wall time of the solve on the library stream */
+  int fallback;          /* AFEM_SOLVER_FPCG_SPAI finished with Jacobi PCG */
 } afem_solve_result;
 
 AFEM_API void afem_solver_default_options(afem_solver_options* opt);
@@ -289,10 +295,102 @@ AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt
 AFEM_API int afem_system_spmv(afem_system_t sys, const double* x, double* y);
 /* y = A32 x32 with the FP32 copy of the system's matrix (made on first use). */
 AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y);
+/* Installs a sparse preconditioner M with the pattern of the system's matrix:
+ * values [device] nnz FP32 in the order of the matrix values; z = M r.  Used
+ * by AFEM_SOLVER_FPCG_SPAI until the matrix pattern changes.  No counterpart
+ * in the reference (its preconditioners live inside Hypre / PETSc). */
+AFEM_API int afem_system_set_spai(afem_system_t sys, const float* values);
+
 /* SpMV format the autotuner chose for the system's matrix: *format = 0 (CSR,
  * *lanes lanes per row, or per 3x3 block row when the matrix has blocks) or
  * 1 (SELL-64); -1 while no SpMV or solve has run on the current matrix. */
 AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes);
+
+/* ---- heterogeneous coefficients (no counterpart in the reference) ------------------- */
+
+/* Random-Fourier-feature parameters of a Gaussian random field with
+ * squared-exponential covariance sigma^2 exp(-|r|^2 / (2 l^2)):
+ * omega [host] nfeatures x 3 (w_k ~ N(0, I / l^2); zero z-component in 2D),
+ * phase [host] nfeatures (phi_k ~ U[0, 2 pi)).  Host only. */
+AFEM_API int afem_rff_parameters(int dim, int nfeatures, double corr_length, uint64_t seed,
+                                 double* omega, double* phase);
+/* lambda_c = exp(g(x_c)) at the cell centroids x_c [device, nb_cell values],
+ * g(x) = sigma sqrt(2/K) sum_k cos(w_k . x + phi_k) with the parameters above
+ * (phase evaluated in FP32).  The per-cell conductivity of
+ * afem_bsr_assemble_laplace (modules/fourier/ElementMatrix.h:38-95). */
+AFEM_API int afem_lognormal_field(afem_mesh_t mesh, double sigma, double corr_length,
+                                  int nfeatures, uint64_t seed, double* cell_lambda);
+
+/* ---- learned sparse approximate inverse (no counterpart in the reference) ------------
+ * A 3-layer message-passing network (width 32, BF16 weights and activations,
+ * FP32 accumulation on MFMA) on the graph of the Jacobi-scaled matrix
+ * S = D^-1/2 A D^-1/2 predicts Mh with the pattern of A; the preconditioner
+ * is M = D^-1/2 Mh D^-1/2.  Training minimises (1/8N) sum_p ||z_p - Mh S z_p||^2
+ * over 8 Rademacher probes per system.  Systems must be scalar, with the
+ * Dirichlet transformation applied (those rows are taken as identity). */
+typedef struct afem_spai* afem_spai_t;
+/* Number of FP32 parameters of the model. */
+AFEM_API int afem_spai_num_params(void);
+/* New model; the untrained model is the Jacobi preconditioner. */
+AFEM_API int afem_spai_create(uint64_t seed, afem_spai_t* spai);
+AFEM_API int afem_spai_destroy(afem_spai_t spai);
+AFEM_API int afem_spai_get_params(afem_spai_t spai, float* params, int n);
+/* Replaces the parameters and resets the optimiser state. */
+AFEM_API int afem_spai_set_params(afem_spai_t spai, const float* params, int n);
+/* Probe loss of the system (forward only). */
+AFEM_API int afem_spai_loss(afem_spai_t spai, afem_system_t sys, uint64_t probe_seed, double* loss);
+/* Loss and its gradient (kept for afem_spai_adam_step; copied to grad [host,
+ * afem_spai_num_params() floats] when grad is not NULL). */
+AFEM_API int afem_spai_loss_grad(afem_spai_t spai, afem_system_t sys, uint64_t probe_seed,
+                                 double* loss, float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
+AFEM_API int afem_spai_adam_step(afem_spai_t spai, double lr);
+/* afem_spai_loss_grad followed by afem_spai_adam_step. */
+AFEM_API int afem_spai_train_step(afem_spai_t spai, afem_system_t sys, uint64_t probe_seed,
+                                  double lr, double* loss);
+/* Forward pass on the system and afem_system_set_spai with the resulting M. */
+AFEM_API int afem_spai_apply(afem_spai_t spai, afem_system_t sys);
+/* Arrays of the last forward pass [host floats], for testing: 0 = S values
+ * (nnz), 1 = node features (N x 16), 2 = h3 (N x 32), 3 = u (N x 32),
+ * 4 = Mh (nnz), 5 = M (nnz), 6 = D^-1/2 (N), 7 = h0 (N x 32). */
+AFEM_API int afem_spai_inspect(afem_spai_t spai, int what, float* out, int64_t count);
+
+/* ---- batched systems (no counterpart in the reference) ------------------------------
+ * Independent scalar systems that are the diagonal blocks of one system's
+ * matrix (e.g. the Laplace matrix of a disjoint union of meshes, assembled in
+ * one launch by afem_bsr_assemble_laplace).  They are solved together by
+ * mixed-precision Jacobi PCG with per-system scalars: FP32 matrix and vectors,
+ * FP64 x, FP64 reliable-update residual deciding convergence, chunks of
+ * iterations captured in a hipGraph and replayed. */
+typedef struct afem_batch* afem_batch_t;
+typedef struct afem_batch_options {
+  double rtol;            /* per system: ||b_s - A_s x_s|| <= max(rtol ||b_s||, atol) */
+  double atol;
+  int max_iter;
+  double reliable_delta;  /* reliable-update reduction factor (0 = 0.1) */
+  int chunk;              /* iterations per captured graph launch (even; 0 = 32) */
+  int use_graph;          /* 1: replay a hipGraph per chunk; 0: plain launches */
+  int use_initial_guess;  /* 1: start from the system's solution array */
+} afem_batch_options;
+typedef struct afem_batch_result {
+  int iterations;
+  int converged;
+  int reliable_updates;
+  double relres;          /* FP64 true relative residual of the system */
+} afem_batch_result;
+AFEM_API void afem_batch_default_options(afem_batch_options* opt);
+/* segment_rows [host] nb_segment + 1 row offsets: 0 = r_0 < r_1 < ... = nb_dof. */
+AFEM_API int afem_batch_create(afem_system_t sys, int nb_segment, const int32_t* segment_rows,
+                               afem_batch_t* batch);
+AFEM_API int afem_batch_destroy(afem_batch_t batch);
+/* Lanes per matrix row (4, 8, 16) and 256-row strips per workgroup (1..16). */
+AFEM_API int afem_batch_set_tiles(afem_batch_t batch, int lanes, int tiles);
+/* Times the (lanes, tiles) candidates on one chunk and keeps the fastest. */
+AFEM_API int afem_batch_autotune(afem_batch_t batch, int* lanes, int* tiles, double* ms_per_iteration);
+/* Solves every system; results [host] nb_segment entries (may be NULL).
+ * AFEM_ERR_NOT_CONVERGED when a system misses the tolerance. */
+AFEM_API int afem_batch_solve(afem_batch_t batch, const afem_batch_options* opt,
+                              afem_batch_result* results, double* total_ms);
 
 #ifdef __cplusplus
 }
