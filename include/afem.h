@@ -146,7 +146,9 @@ AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda
  * column in the LDS-staged row; the others read the per-(node, cell) plan
  * built by afem_bsr_compute_sparsity, with different batch depths and rows
  * per workgroup.  autotune times each variant on this matrix, keeps the
- * fastest and reports it; set_assembly_variant pins one (-1 = default). */
+ * fastest and reports it; set_assembly_variant pins one (-1 = default).
+ * The autotuner assembles with lambda = 1 and no RHS: it overwrites the
+ * matrix values, so assemble the system afterwards. */
 AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* variant);
 AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant);
 
