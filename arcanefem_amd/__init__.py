@@ -42,7 +42,7 @@ class SolverOptions(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("preconditioner", ctypes.c_int), ("rtol", ctypes.c_double),
                 ("atol", ctypes.c_double), ("max_iter", ctypes.c_int), ("use_initial_guess", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("reliable_delta", ctypes.c_double), ("spmv_lanes", ctypes.c_int),
-                ("norm", ctypes.c_int)]
+                ("norm", ctypes.c_int), ("precond_bf16", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):
@@ -420,12 +420,13 @@ class DoFLinearSystem:
         check(lib().afem_system_set_spai(self.h, values.c))
 
     def spmv_format(self):
-        """('csr', lanes) or ('sell', 64) as chosen by the SpMV autotuner; None before any SpMV."""
+        """('csr', lanes), ('sell', 64) or ('sell16', 64) as chosen by the SpMV autotuner;
+        None before any SpMV."""
         f, lanes = ctypes.c_int(), ctypes.c_int()
         check(lib().afem_system_spmv_format(self.h, ctypes.byref(f), ctypes.byref(lanes)))
         if f.value < 0:
             return None
-        return ("sell", 64) if f.value == 1 else ("csr", lanes.value)
+        return {1: ("sell", 64), 2: ("sell16", 64)}.get(f.value, ("csr", lanes.value))
 
     def __del__(self):
         try:

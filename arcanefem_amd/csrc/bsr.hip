@@ -798,16 +798,66 @@ static void build_local_plan(afem_bsr* b)
   b->block_nodes_max = mx;
 }
 
+// Sliced copy of the local plan and of the cell ids: word j of row r of
+// block blk at slice[blk] + j * 256 + (r - r0).
+__global__ void __launch_bounds__(256)
+k_slice_plan(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+             const uint64_t* __restrict__ lplan, const int64_t* __restrict__ slice, int32_t nb_node,
+             uint64_t* __restrict__ out_plan, int32_t* __restrict__ out_cell)
+{
+  const int32_t blk = blockIdx.x;
+  const int32_t i = blk * kLocalRows + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t e0 = nc_off[i], n = nc_off[i + 1] - e0;
+  const int64_t base = slice[blk] + threadIdx.x;
+  for (int32_t j = 0; j < n; ++j) {
+    out_plan[base + (int64_t)j * kLocalRows] = lplan[e0 + j];
+    out_cell[base + (int64_t)j * kLocalRows] = nc_items[e0 + j] >> 2;
+  }
+}
+
+static void build_sliced_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->lplan_sl.release();
+  b->lcell_sl.release();
+  b->slice.release();
+  if (!b->lplan.p)
+    return;
+  const int32_t nblk = (m.nb_node + kLocalRows - 1) / kLocalRows;
+  std::vector<int32_t> off((size_t)m.nb_node + 1);
+  m.nc_offsets.download(off.data(), off.size());
+  std::vector<int64_t> slice((size_t)nblk + 1, 0);
+  for (int32_t k = 0; k < nblk; ++k) {
+    int32_t w = 0;
+    for (int32_t i = k * kLocalRows; i < std::min(m.nb_node, (k + 1) * kLocalRows); ++i)
+      w = std::max(w, off[i + 1] - off[i]);
+    slice[k + 1] = slice[k] + (int64_t)w * kLocalRows;
+  }
+  b->slice.alloc(slice.size());
+  b->slice.upload(slice.data(), slice.size());
+  b->lplan_sl.alloc((size_t)slice[nblk]);
+  b->lcell_sl.alloc((size_t)slice[nblk]);
+  hipLaunchKernelGGL(k_slice_plan, dim3(nblk), dim3(kLocalRows), 0, stream(), m.nc_offsets.p, m.nc_items.p,
+                     b->lplan.p, b->slice.p, m.nb_node, b->lplan_sl.p, b->lcell_sl.p);
+  check_launch("slice_plan");
+}
+
 // Laplace assembly with the block's node coordinates staged in LDS (SoA):
-// per (row, cell) pair one word from HBM, then only LDS reads.
-template <int NPC, int U>
+// per (row, cell) pair one word from HBM, then only LDS reads.  SLICED:
+// the words (and cell ids) are stored per block of rows with word j of row
+// r at slice[blk] + j * 256 + (r - r0), so one wave load is one contiguous
+// 512-byte read; otherwise they follow the node->cell lists (nc_off).
+template <int NPC, int U, bool SLICED>
 __global__ void __launch_bounds__(256)
 k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
                          const int32_t* __restrict__ nc_items, const uint64_t* __restrict__ lplan,
-                         const int32_t* __restrict__ table, const int32_t* __restrict__ count,
-                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
-                         int32_t nb_node, int val_cap, int node_cap, const double* __restrict__ cell_lambda,
-                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
+                         const int64_t* __restrict__ slice, const int32_t* __restrict__ table,
+                         const int32_t* __restrict__ count, const int32_t* __restrict__ diag_pos,
+                         const int32_t* __restrict__ row_index, int32_t nb_node, int val_cap, int node_cap,
+                         const double* __restrict__ cell_lambda, double source_f, double* __restrict__ rhs,
+                         double* __restrict__ values)
 {
   extern __shared__ unsigned char smem[];
   const int32_t blk = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
@@ -832,19 +882,27 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
   if (i < r1) {
     double* rv = s_val + (row_index[i] - a);
     const d3 p = load_node(coords, i);
-    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
+    const int32_t e0 = nc_off[i];
+    const int32_t ncell = nc_off[i + 1] - e0;
+    // word j of this row (and its cell id for the conductivity)
+    const int64_t wbase = SLICED ? slice[blk] + threadIdx.x : (int64_t)e0;
+    const int wstride = SLICED ? kLocalRows : 1;
     double diag = 0.0, det_sum = 0.0;
-    for (int32_t e = e0; e < e1; e += U) {
+    for (int32_t e = 0; e < ncell; e += U) {
       uint64_t w[U];
       double lam[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        w[u] = lplan[min(e + u, e1 - 1)];
-        lam[u] = cell_lambda ? cell_lambda[nc_items[min(e + u, e1 - 1)] >> 2] : 1.0;
+        const int64_t at = wbase + (int64_t)min(e + u, ncell - 1) * wstride;
+        w[u] = lplan[at];
+        if (cell_lambda)
+          lam[u] = cell_lambda[SLICED ? nc_items[at] : (nc_items[at] >> 2)];
+        else
+          lam[u] = 1.0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (e + u >= e1)
+        if (e + u >= ncell)
           break;
         const int la = lplan_idx(w[u], 0), lb = lplan_idx(w[u], 1);
         const double2 xa = s_xy[la], xb = s_xy[lb];
@@ -906,6 +964,7 @@ static void build_plan(afem_bsr* b)
   AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix: a cell coupling is missing from the pattern");
   build_compact_plan(b);
   build_local_plan(b);
+  build_sliced_plan(b);
 }
 
 // Laplace assembly driven by the plan: only the values are staged in LDS.
@@ -1014,7 +1073,8 @@ enum PlanKind
   kSearch = 0,
   kPlan = 1,
   kCompact = 2,
-  kLocal = 3
+  kLocal = 3,
+  kLocalSliced = 4
 };
 struct LaplaceVariant
 {
@@ -1023,23 +1083,25 @@ struct LaplaceVariant
   int rows;
 };
 static const LaplaceVariant kLaplaceVariants[] = {
-  { kSearch, 4, 256 },  { kPlan, 4, 256 },     { kPlan, 8, 256 },  { kCompact, 4, 256 },
-  { kCompact, 8, 128 }, { kLocal, 4, 256 },    { kLocal, 8, 256 }, { kLocal, 16, 256 },
+  { kSearch, 4, 256 },  { kPlan, 4, 256 },  { kPlan, 8, 256 },        { kCompact, 4, 256 },
+  { kCompact, 8, 128 }, { kLocal, 4, 256 }, { kLocal, 8, 256 },       { kLocal, 16, 256 },
+  { kLocalSliced, 4, 256 }, { kLocalSliced, 8, 256 },
 };
 constexpr int kNumLaplaceVariants = sizeof(kLaplaceVariants) / sizeof(kLaplaceVariants[0]);
 constexpr int kDefaultLaplaceVariant = 6;
 
-template <int NPC, int U>
+template <int NPC, int U, bool SLICED>
 static void launch_laplace_lplan(const afem_bsr* b, const double* lam, double f, double* rhs)
 {
   const afem_mesh& m = *b->mesh;
   const int val_cap = (kLocalRows * b->max_row + 1) & ~1; // keeps the coordinate table 16-byte aligned
   const int node_cap = b->block_nodes_max;
   const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap) * sizeof(double);
-  hipLaunchKernelGGL((k_assemble_laplace_lplan<NPC, U>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
-                     lds, stream(), m.coords.p, m.nc_offsets.p, m.nc_items.p, b->lplan.p, b->block_nodes.p,
-                     b->block_count.p, b->diag_pos.p, b->row_index.p, m.nb_node, val_cap, node_cap, lam, f, rhs,
-                     b->values.p);
+  hipLaunchKernelGGL((k_assemble_laplace_lplan<NPC, U, SLICED>), dim3(grid_for(m.nb_node, kLocalRows)),
+                     dim3(kLocalRows), lds, stream(), m.coords.p, m.nc_offsets.p,
+                     SLICED ? b->lcell_sl.p : m.nc_items.p, SLICED ? b->lplan_sl.p : b->lplan.p, b->slice.p,
+                     b->block_nodes.p, b->block_count.p, b->diag_pos.p, b->row_index.p, m.nb_node, val_cap, node_cap,
+                     lam, f, rhs, b->values.p);
 }
 
 template <int NPC, int U>
@@ -1065,7 +1127,9 @@ static LaplaceVariant effective_variant(const afem_bsr* b, int v)
 {
   LaplaceVariant var = kLaplaceVariants[v];
   const size_t local_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->block_nodes_max) * sizeof(double);
-  if (var.kind == kLocal && (!b->lplan.p || local_lds > 160 * 1024))
+  if (var.kind == kLocalSliced && !b->lplan_sl.p)
+    var.kind = kLocal;
+  if ((var.kind == kLocal || var.kind == kLocalSliced) && (!b->lplan.p || local_lds > 160 * 1024))
     var.kind = kCompact;
   if (var.kind == kCompact && !b->cplan.p)
     var.kind = kPlan;
@@ -1083,19 +1147,33 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
   const bool fits = assembly_shape(*b, var.kind == kSearch ? 12 : 8, var.rows, &rows, &lds);
   AFEM_REQUIRE(fits, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
   ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
-  if (var.kind == kLocal) {
+  if (var.kind == kLocalSliced) {
+    if (m.npc == 4) {
+      if (var.u == 4)
+        launch_laplace_lplan<4, 4, true>(b, cell_lambda, f, rhs);
+      else
+        launch_laplace_lplan<4, 8, true>(b, cell_lambda, f, rhs);
+    }
+    else {
+      if (var.u == 4)
+        launch_laplace_lplan<3, 4, true>(b, cell_lambda, f, rhs);
+      else
+        launch_laplace_lplan<3, 8, true>(b, cell_lambda, f, rhs);
+    }
+  }
+  else if (var.kind == kLocal) {
     if (m.npc == 4) {
       switch (var.u) {
-      case 4: launch_laplace_lplan<4, 4>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<4, 16>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<4, 8>(b, cell_lambda, f, rhs); break;
+      case 4: launch_laplace_lplan<4, 4, false>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<4, 16, false>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<4, 8, false>(b, cell_lambda, f, rhs); break;
       }
     }
     else {
       switch (var.u) {
-      case 4: launch_laplace_lplan<3, 4>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<3, 16>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<3, 8>(b, cell_lambda, f, rhs); break;
+      case 4: launch_laplace_lplan<3, 4, false>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<3, 16, false>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<3, 8, false>(b, cell_lambda, f, rhs); break;
       }
     }
   }

@@ -223,6 +223,9 @@ struct afem_bsr
   afem::DBuf<uint64_t> plan;
   afem::DBuf<uint64_t> cplan;        // compact plan (node deltas + positions), when it fits
   afem::DBuf<uint64_t> lplan;        // block-local plan (node-table indices + positions), when it fits
+  afem::DBuf<uint64_t> lplan_sl;     // the same words sliced per block of rows (coalesced reads)
+  afem::DBuf<int32_t> lcell_sl;      // cell id of each sliced word
+  afem::DBuf<int64_t> slice;         // per block of rows: first sliced word
   afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
   afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes
   int32_t block_nodes_max = 0;       // largest node table

@@ -247,6 +247,53 @@ k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restric
     put_part(part, part_out);
 }
 
+// SELL-64 with 16-bit column offsets: slot (j, lane) of slice s stores
+// col - row at (off[s] + j) * 64 + lane (padding slots: 0 with a zero value).
+// The value and offset streams are read once, so they are loaded non-temporal
+// to leave the L2 to the gathered vector.
+template <class VT, class XT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
+              const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
+              Gate gate, double* __restrict__ part_out)
+{
+  if (!gate.open())
+    return;
+  using A = typename Acc<VT>::type;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  double part = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row < n) {
+      const int64_t s = row >> 6;
+      const int64_t base = (int64_t)off[s] * 64 + (row & 63);
+      const int w = off[s + 1] - off[s];
+      const XT* xr = x + row;
+      A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      int j = 0;
+      for (; j + 3 < w; j += 4) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        const int d0 = __builtin_nontemporal_load(dcol + i0);
+        const int d1 = __builtin_nontemporal_load(dcol + i0 + 64);
+        const int d2 = __builtin_nontemporal_load(dcol + i0 + 128);
+        const int d3 = __builtin_nontemporal_load(dcol + i0 + 192);
+        s0 += (A)__builtin_nontemporal_load(v + i0) * (A)xr[d0];
+        s1 += (A)__builtin_nontemporal_load(v + i0 + 64) * (A)xr[d1];
+        s2 += (A)__builtin_nontemporal_load(v + i0 + 128) * (A)xr[d2];
+        s3 += (A)__builtin_nontemporal_load(v + i0 + 192) * (A)xr[d3];
+      }
+      for (; j < w; ++j) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        s0 += (A)__builtin_nontemporal_load(v + i0) * (A)xr[__builtin_nontemporal_load(dcol + i0)];
+      }
+      spmv_epilogue<MODE>(row, (double)((s0 + s1) + (s2 + s3)), x, y, b, part);
+    }
+  }
+  if constexpr (MODE != kPlain)
+    put_part(part, part_out);
+}
+
 // BSR 3x3 blocks stored per block (9 contiguous values), G lanes per node row.
 template <int G, class VT, class XT, int MODE>
 __global__ void __launch_bounds__(kBlock)
@@ -375,6 +422,29 @@ __global__ void k_sell_fill(int32_t nrows, const int32_t* __restrict__ rp, const
         v32[idx] = (float)val;
     }
   }
+}
+
+// 16-bit offsets col - row of the SELL columns (one workgroup of 64 per
+// slice); *overflow = 1 when an offset does not fit.
+__global__ void k_sell_col16(int32_t nrows, const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                             int16_t* __restrict__ dcol, int32_t* __restrict__ overflow)
+{
+  const int64_t s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t row = s * 64 + lane;
+  const int w = off[s + 1] - off[s];
+  bool bad = false;
+  for (int j = 0; j < w; ++j) {
+    const int64_t e = ((int64_t)off[s] + j) * 64 + lane;
+    int64_t d = (row < nrows) ? (int64_t)col[e] - row : 0;
+    if (d < -32768 || d > 32767) {
+      bad = true;
+      d = 0;
+    }
+    dcol[e] = (int16_t)d;
+  }
+  if (bad)
+    atomicOr(overflow, 1);
 }
 
 // ---- preconditioner set-up ----------------------------------------------------------------
@@ -825,6 +895,68 @@ k_prec_sell(int k, int32_t n, const int32_t* __restrict__ off, const int32_t* __
   prec_parts<INIT>(P, k, rz, zq, rr, bb);
 }
 
+// Preconditioner values as FP32 or BF16 (the high half of an FP32).
+__device__ __forceinline__ float mval_at(const float* __restrict__ m, int64_t i)
+{
+  return __builtin_nontemporal_load(m + i);
+}
+__device__ __forceinline__ float mval_at(const uint16_t* __restrict__ m, int64_t i)
+{
+  return __uint_as_float((uint32_t)__builtin_nontemporal_load(m + i) << 16);
+}
+
+template <int INIT, class MT>
+__global__ void __launch_bounds__(kBlock)
+k_prec_sell16(int k, int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
+              const MT* __restrict__ mv, PrecIn in, float* __restrict__ r32, float* __restrict__ z,
+              float* __restrict__ p, const float* __restrict__ q, const double* __restrict__ b, Parts P, int nparts,
+              const State* st)
+{
+  if (!prec_open<INIT>(st, P, nparts, k, in))
+    return;
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  double rz = 0.0, zq = 0.0, rr = 0.0, bb = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row < n) {
+      const int64_t s = row >> 6;
+      const int64_t base = (int64_t)off[s] * 64 + (row & 63);
+      const int w = off[s + 1] - off[s];
+      float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+      int j = 0;
+      for (; j + 3 < w; j += 4) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        const int d0 = __builtin_nontemporal_load(dcol + i0);
+        const int d1 = __builtin_nontemporal_load(dcol + i0 + 64);
+        const int d2 = __builtin_nontemporal_load(dcol + i0 + 128);
+        const int d3 = __builtin_nontemporal_load(dcol + i0 + 192);
+        s0 += mval_at(mv, i0) * in.at(row + d0);
+        s1 += mval_at(mv, i0 + 64) * in.at(row + d1);
+        s2 += mval_at(mv, i0 + 128) * in.at(row + d2);
+        s3 += mval_at(mv, i0 + 192) * in.at(row + d3);
+      }
+      for (; j < w; ++j) {
+        const int64_t i0 = base + (int64_t)j * 64;
+        s0 += mval_at(mv, i0) * in.at(row + __builtin_nontemporal_load(dcol + i0));
+      }
+      prec_row<INIT>(row, (s0 + s1) + (s2 + s3), in, r32, z, p, q, b, rz, zq, rr, bb);
+    }
+  }
+  prec_parts<INIT>(P, k, rz, zq, rr, bb);
+}
+
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ in, uint16_t* __restrict__ out)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t u = __float_as_uint(in[i]);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+      out[i] = 0x7fc0;
+    else
+      out[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
+}
+
 // x += alpha p, r -= alpha q; partial r.r.
 __global__ void __launch_bounds__(kBlock)
 k_update_flex(int k, int32_t n, double* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
@@ -950,7 +1082,8 @@ __global__ void k_sell_fill_f32(int32_t nrows, const int32_t* __restrict__ rp, c
 enum SpmvFormat
 {
   kFmtCsr = 0,
-  kFmtSell = 1
+  kFmtSell = 1,
+  kFmtSell16 = 2 // SELL-64 with 16-bit column offsets
 };
 
 struct afem_solver_ws
@@ -967,6 +1100,8 @@ struct afem_solver_ws
   afem::State* host_st = nullptr; // pinned
   // SELL copy
   afem::DBuf<int32_t> sell_off, sell_col;
+  afem::DBuf<int16_t> sell_dcol; // col - row, when every offset fits 16 bits
+  bool has_dcol = false;
   afem::DBuf<double> sell_v64;
   afem::DBuf<float> sell_v32;
   bool has_fp32 = false;
@@ -976,8 +1111,10 @@ struct afem_solver_ws
   int lanes = 16;
   // sparse preconditioner matrix (pattern of A, FP32), CSR and SELL order
   afem::DBuf<float> spai32, spai_sell32;
+  afem::DBuf<uint16_t> spai_sell_bf16;
   bool has_spai = false;
   bool spai_sell_valid = false;
+  bool spai_bf16_valid = false;
   ~afem_solver_ws()
   {
     if (host_st)
@@ -1142,6 +1279,23 @@ static void ensure_sell(afem_system* s, afem_solver_ws* w)
     w->sell_v32.alloc(w->sell_v64.n);
   fill_sell(s, w, true);
   w->pattern_valid = true;
+  // 16-bit column offsets (scalar matrices whose bandwidth fits)
+  w->has_dcol = false;
+  w->sell_dcol.release();
+  if (s->block == 1) {
+    const int32_t nslice = (s->nb_dof + 63) / 64;
+    w->sell_dcol.alloc(w->sell_col.n);
+    DBuf<int32_t> overflow(1);
+    overflow.zero();
+    hipLaunchKernelGGL(k_sell_col16, dim3(nslice), dim3(64), 0, stream(), s->nb_dof, w->sell_off.p, w->sell_col.p,
+                       w->sell_dcol.p, overflow.p);
+    check_launch("sell_col16");
+    int32_t h = 0;
+    overflow.download(&h, 1);
+    w->has_dcol = h == 0;
+    if (!w->has_dcol)
+      w->sell_dcol.release();
+  }
 }
 
 // ---- SpMV dispatch ---------------------------------------------------------------------------
@@ -1175,6 +1329,10 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
     hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof, w->sell_off.p,
                        w->sell_col.p, sell_vals, x, y, b, gate, part_out);
   }
+  else if (w->format == kFmtSell16) {
+    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                       w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+  }
   else {
     auto go = [&](auto gtag) {
       constexpr int G = decltype(gtag)::value;
@@ -1207,6 +1365,10 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     return;
   }
   ensure_sell(s, w);
+  if (forced == -2 && w->has_dcol) {
+    w->format = kFmtSell16;
+    return;
+  }
   if (forced < 0) {
     w->format = kFmtSell;
     return;
@@ -1222,6 +1384,8 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     cands = { { kFmtCsr, 16 }, { kFmtSell, 0 } };
   else
     cands = { { kFmtCsr, 8 }, { kFmtCsr, 16 }, { kFmtSell, 0 } };
+  if (s->block == 1 && w->has_dcol)
+    cands.push_back({ kFmtSell16, 0 });
   hipEvent_t e0, e1;
   AFEM_HIP(hipEventCreate(&e0));
   AFEM_HIP(hipEventCreate(&e1));
@@ -1331,24 +1495,44 @@ static double true_rr(afem_system* s, afem_solver_ws* w, const Parts& P, int nbl
   return w->host_st->rr_true;
 }
 
-static void ensure_spai_sell(afem_system* s, afem_solver_ws* w)
+static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
 {
-  if (w->format != kFmtSell || w->spai_sell_valid)
+  if (w->format != kFmtSell && w->format != kFmtSell16)
     return;
-  w->spai_sell32.alloc(w->sell_col.n);
-  const int64_t rows_pad = ((int64_t)s->nb_dof + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), s->nb_dof, s->row_index,
-                     w->spai32.p, w->sell_off.p, w->spai_sell32.p);
-  check_launch("spai_sell_fill");
-  w->spai_sell_valid = true;
+  if (!w->spai_sell_valid) {
+    w->spai_sell32.alloc(w->sell_col.n);
+    const int64_t rows_pad = ((int64_t)s->nb_dof + 63) / 64 * 64;
+    hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), s->nb_dof,
+                       s->row_index, w->spai32.p, w->sell_off.p, w->spai_sell32.p);
+    check_launch("spai_sell_fill");
+    w->spai_sell_valid = true;
+    w->spai_bf16_valid = false;
+  }
+  if (bf16 && !w->spai_bf16_valid) {
+    w->spai_sell_bf16.alloc(w->spai_sell32.n);
+    hipLaunchKernelGGL(k_to_bf16, dim3(2048), dim3(256), 0, stream(), (int64_t)w->spai_sell32.n, w->spai_sell32.p,
+                       w->spai_sell_bf16.p);
+    check_launch("spai_to_bf16");
+    w->spai_bf16_valid = true;
+  }
 }
 
 template <int INIT>
-static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk)
+static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk, bool bf16)
 {
   ProfScope prof("fpcg_precond_spmv");
   const PrecIn in{ w->r32.p, w->r64.p, true };
-  if (w->format == kFmtSell) {
+  if (w->format == kFmtSell16) {
+    if (bf16)
+      hipLaunchKernelGGL((k_prec_sell16<INIT, uint16_t>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof,
+                         w->sell_off.p, w->sell_dcol.p, w->spai_sell_bf16.p, in, w->r32.p, w->z32.p, w->p32.p,
+                         w->q32.p, s->rhs.p, P, nblk, w->st.p);
+    else
+      hipLaunchKernelGGL((k_prec_sell16<INIT, float>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof,
+                         w->sell_off.p, w->sell_dcol.p, w->spai_sell32.p, in, w->r32.p, w->z32.p, w->p32.p, w->q32.p,
+                         s->rhs.p, P, nblk, w->st.p);
+  }
+  else if (w->format == kFmtSell) {
     hipLaunchKernelGGL((k_prec_sell<INIT>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, w->sell_off.p,
                        w->sell_col.p, w->spai_sell32.p, in, w->r32.p, w->z32.p, w->p32.p, w->q32.p, s->rhs.p, P, nblk,
                        w->st.p);
@@ -1370,14 +1554,14 @@ static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P
   check_launch("fpcg_precond_spmv");
 }
 
-static void launch_flex_init(afem_system* s, afem_solver_ws* w, const Parts& P, int nblk)
+static void launch_flex_init(afem_system* s, afem_solver_ws* w, const Parts& P, int nblk, bool bf16)
 {
-  prec_launch<1>(s, w, -1, P, nblk);
+  prec_launch<1>(s, w, -1, P, nblk, bf16);
   hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kBlock), 0, stream(), P, nblk, w->st.p);
   check_launch("fpcg_init_finalize");
 }
 
-static void launch_flex_iteration(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk)
+static void launch_flex_iteration(afem_system* s, afem_solver_ws* w, int k, const Parts& P, int nblk, bool bf16)
 {
   const Gate until_done{ w->st.p, kUntilDone, k, nullptr, nblk };
   spmv_launch<float, float, kPQ>(s, w, true, w->p32.p, w->q32.p, nullptr, until_done, P.pq, nblk, "spmv_fp32");
@@ -1390,7 +1574,7 @@ static void launch_flex_iteration(afem_system* s, afem_solver_ws* w, int k, cons
   const Gate reliable{ w->st.p, kReliable, k, P.rr, nblk };
   spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, reliable, P.rr64, nblk,
                                          "spmv_residual_fp64");
-  prec_launch<0>(s, w, k, P, nblk);
+  prec_launch<0>(s, w, k, P, nblk, bf16);
   {
     ProfScope prof("fpcg_pupdate");
     hipLaunchKernelGGL(k_pupdate_flex, dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, w->p32.p, w->z32.p, P,
@@ -1434,8 +1618,11 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   ensure_derived(s, w, mixed);
   choose_format(s, w, o.spmv_lanes);
   ensure_derived(s, w, mixed);
+  // BF16 preconditioner values need a SELL layout; CSR keeps FP32
+  const bool bf16 = flex && o.precond_bf16 && (w->format == kFmtSell || w->format == kFmtSell16);
+  const bool bf16_used = bf16 && w->format == kFmtSell16;
   if (flex)
-    ensure_spai_sell(s, w);
+    ensure_spai_sell(s, w, bf16_used);
   const Parts P = parts_of(w);
   const int nblk = spmv_blocks(s);
 
@@ -1461,7 +1648,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     spmv_launch<double, double, kResidual>(s, w, false, s->x.p, w->r64.p, s->rhs.p, always, P.rr64, nblk,
                                            "spmv_residual_fp64");
     if (flex)
-      launch_flex_init(s, w, P, nblk);
+      launch_flex_init(s, w, P, nblk, bf16_used);
     else if (s->block == 3)
       launch_init<3>(s, w, mixed, precond, o.norm == 2, P, nblk);
     else
@@ -1471,7 +1658,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
     while (!w->host_st->done && k < init.max_iter) {
       for (int j = 0; j < check_every && k < init.max_iter; ++j, ++k) {
         if (flex)
-          launch_flex_iteration(s, w, k, P, nblk);
+          launch_flex_iteration(s, w, k, P, nblk, bf16_used);
         else if (s->block == 3)
           launch_iteration<3>(s, w, k, mixed, precond, P, nblk);
         else
@@ -1552,6 +1739,7 @@ AFEM_API void afem_solver_default_options(afem_solver_options* opt)
   opt->reliable_delta = 0.1;
   opt->spmv_lanes = 0;
   opt->norm = 0;
+  opt->precond_bf16 = 0;
 }
 
 AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt, afem_solve_result* result)

@@ -265,7 +265,9 @@ typedef struct afem_solver_options {
   double reliable_delta; /* mixed: residual reduction factor between FP64
                             reliable updates (0 = 0.1) */
   int spmv_lanes;        /* SpMV format: 0 = autotuned, > 0 = CSR with that many
-                            lanes per row (4, 8, 16, 32), -1 = SELL-64 */
+                            lanes per row (4, 8, 16, 32), -1 = SELL-64,
+                            -2 = SELL-64 with 16-bit column offsets (scalar
+                            matrices whose |col - row| fits; else SELL-64) */
   int norm;              /* stopping norm: 0 = ||b - A x||_2 <= max(rtol ||b||_2, atol);
                             1 = natural norm sqrt(r.M^-1 r) <= rtol * its initial value;
                             2 = preconditioned norm ||M^-1 r||_2 <= max(rtol ||M^-1 b||_2,
@@ -273,6 +275,8 @@ typedef struct afem_solver_options {
                             it scales penalty rows by 1/1e30, so it is the one to use
                             when penalty rows make ||b|| meaningless).  1 and 2 have no
                             true-residual restart and are FP64 PCG only. */
+  int precond_bf16;      /* AFEM_SOLVER_FPCG_SPAI: store the preconditioner values as
+                            BF16 (with the 16-bit-offset SELL format; FP32 otherwise) */
 } afem_solver_options;
 
 typedef struct afem_solve_result {
@@ -304,8 +308,9 @@ AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y);
 AFEM_API int afem_system_set_spai(afem_system_t sys, const float* values);
 
 /* SpMV format the autotuner chose for the system's matrix: *format = 0 (CSR,
- * *lanes lanes per row, or per 3x3 block row when the matrix has blocks) or
- * 1 (SELL-64); -1 while no SpMV or solve has run on the current matrix. */
+ * *lanes lanes per row, or per 3x3 block row when the matrix has blocks),
+ * 1 (SELL-64) or 2 (SELL-64 with 16-bit column offsets); -1 while no SpMV
+ * or solve has run on the current matrix. */
 AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes);
 
 /* ---- heterogeneous coefficients (no counterpart in the reference) ------------------- */

@@ -83,7 +83,7 @@ def test_node_cells_and_pattern_bit_exact(afem_gpu, spec):
     assert info.nb_col == len(ocol) and info.max_row_length == np.diff(orow).max()
 
 
-N_LAPLACE_VARIANTS = 8
+N_LAPLACE_VARIANTS = 10
 
 
 @pytest.mark.parametrize("spec", MESHES, ids=[m[0] for m in MESHES])
@@ -295,7 +295,30 @@ def poisson_bottom_system(A, dim, n, alpha, lam=None):
     return X, C, m, b, s, dl
 
 
-SPMV_FORMATS = [0, -1, 8]  # autotuned, SELL-64, CSR with 8 lanes per row
+SPMV_FORMATS = [0, -1, -2, 8]  # autotuned, SELL-64, SELL-64 with 16-bit offsets, CSR with 8 lanes per row
+
+
+@pytest.mark.parametrize("fmt", [-1, -2, 8])
+def test_spmv_forced_formats(afem_gpu, fmt):
+    """Each SpMV format against the oracle (the format is pinned by a one-iteration solve)."""
+    A = afem_gpu
+    print(f"[case] spmv format {fmt}", flush=True)
+    X, C = box(3, 12, 0.15)
+    N = X.shape[0]
+    m, b, s, _ = build_poisson(A, 3, X, C)
+    s.solve(raise_on_failure=False, method=A.SOLVER_PCG_MIXED, max_iter=1, spmv_lanes=fmt)
+    want = {-1: ("sell", 64), -2: ("sell16", 64), 8: ("csr", 8)}[fmt]
+    assert s.spmv_format() == want
+    row, col, val, _ = b.download()
+    x = np.random.default_rng(3).normal(size=N)
+    dy = A.DeviceArray(N, np.float64)
+    s.spmv(A.DeviceArray.from_numpy(x), dy)
+    ref = O.spmv(row, col, val, x)
+    scale = np.abs(val).max() * np.abs(x).max()
+    assert np.abs(dy.numpy() - ref).max() <= 1e-13 * scale
+    dy32 = A.DeviceArray(N, np.float32)
+    s.spmv_fp32(A.DeviceArray.from_numpy(x.astype(np.float32)), dy32)
+    assert np.abs(dy32.numpy() - ref).max() <= 2e-5 * scale
 
 
 @pytest.mark.parametrize("fmt", SPMV_FORMATS)

@@ -175,6 +175,26 @@ def test_fpcg_matches_direct_solve(afem_gpu):
     assert np.linalg.norm(x - x_ref) <= 1e-7 * np.linalg.norm(x_ref)
 
 
+@pytest.mark.parametrize("fmt,bf16", [(-1, 0), (-2, 0), (-2, 1), (8, 1)])
+def test_fpcg_formats_and_bf16(afem_gpu, fmt, bf16):
+    """The preconditioner SpMV in every format, with FP32 or BF16 values
+    (BF16 is used with the 16-bit-offset SELL format only)."""
+    A = afem_gpu
+    print(f"[case] fpcg format {fmt} bf16 {bf16}", flush=True)
+    case = build_system(A, 3, 12, seed=21)
+    s = case["sys"]
+    net = A.LearnedSpai(seed=2)
+    for k in range(30):
+        net.train_step(s, probe_seed=k, lr=3e-3)
+    net.apply(s)
+    r = s.solve(method=A.SOLVER_FPCG_SPAI, rtol=1e-9, max_iter=5000, spmv_lanes=fmt, precond_bf16=bf16)
+    assert r.converged and r.relres <= 1e-9 and not r.fallback
+    row, col, val, _ = case["bsr"].download()
+    n = len(row) - 1
+    x_ref = spla.spsolve(sp.csr_matrix((val, col, row), shape=(n, n)).tocsc(), s.rhs().numpy())
+    assert np.linalg.norm(s.solution().numpy() - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+
+
 def test_fallback_to_jacobi(afem_gpu):
     A = afem_gpu
     print("[case] fpcg fallback 2D n=16", flush=True)
