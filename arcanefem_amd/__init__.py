@@ -420,13 +420,13 @@ class DoFLinearSystem:
         check(lib().afem_system_set_spai(self.h, values.c))
 
     def spmv_format(self):
-        """('csr', lanes), ('sell', 64) or ('sell16', 64) as chosen by the SpMV autotuner;
+        """('csr', lanes), ('sell', 64), ('sell16', 64) or ('sell16x8', 64) as chosen by the SpMV autotuner;
         None before any SpMV."""
         f, lanes = ctypes.c_int(), ctypes.c_int()
         check(lib().afem_system_spmv_format(self.h, ctypes.byref(f), ctypes.byref(lanes)))
         if f.value < 0:
             return None
-        return {1: ("sell", 64), 2: ("sell16", 64)}.get(f.value, ("csr", lanes.value))
+        return {1: ("sell", 64), 2: ("sell16", 64), 3: ("sell16x8", 64)}.get(f.value, ("csr", lanes.value))
 
     def __del__(self):
         try:

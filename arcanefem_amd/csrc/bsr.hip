@@ -798,6 +798,270 @@ static void build_local_plan(afem_bsr* b)
   b->block_nodes_max = mx;
 }
 
+// ---- brick-ordered local plan ---------------------------------------------------------------
+// Rows are processed in bricks of kLocalRows nodes that are consecutive in
+// the Morton order of the node coordinates (8 x 8 x 4 nodes on a structured
+// grid), so a brick's node table is its nodes plus a one-layer halo (about
+// 600 nodes instead of about 1800 for 256 lexicographically consecutive
+// rows).  Row numbering is unchanged: each thread owns one row of its brick,
+// accumulates it in LDS at the row's offset in the brick, and writes the
+// row's segment of the value array at the end.
+
+// Row offsets of each brick slot in LDS (exclusive scan of the row lengths)
+// and the brick's total; total < 0 when it exceeds kLocalSort.
+__global__ void __launch_bounds__(256)
+k_brick_offsets(const int32_t* __restrict__ perm, const int32_t* __restrict__ row_index, int32_t nb_node,
+                int32_t* __restrict__ val_off, int32_t* __restrict__ total)
+{
+  __shared__ int32_t scan[256];
+  const int tid = threadIdx.x;
+  const int64_t slot = (int64_t)blockIdx.x * kLocalRows + tid;
+  int32_t len = 0;
+  if (slot < nb_node) {
+    const int32_t r = perm[slot];
+    len = row_index[r + 1] - row_index[r];
+  }
+  scan[tid] = len;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = (tid >= off) ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  if (slot < nb_node)
+    val_off[slot] = scan[tid] - len;
+  if (tid == 255)
+    total[blockIdx.x] = scan[255] <= kLocalSort ? scan[255] : -1;
+}
+
+// Sorted unique columns of each brick's rows (as k_block_nodes).
+__global__ void __launch_bounds__(256)
+k_brick_nodes(const int32_t* __restrict__ perm, const int32_t* __restrict__ val_off,
+              const int32_t* __restrict__ total, const int32_t* __restrict__ row_index,
+              const int32_t* __restrict__ columns, int32_t nb_node, int32_t* __restrict__ table,
+              int32_t* __restrict__ count)
+{
+  __shared__ int32_t buf[kLocalSort];
+  __shared__ int32_t scan[256];
+  const int tid = threadIdx.x;
+  const int32_t blk = blockIdx.x;
+  const int32_t cnt = total[blk];
+  if (cnt < 0) {
+    if (tid == 0)
+      count[blk] = -1;
+    return;
+  }
+  int n2 = 256;
+  while (n2 < cnt)
+    n2 <<= 1;
+  for (int k = tid; k < n2; k += 256)
+    buf[k] = 0x7fffffff;
+  __syncthreads();
+  const int64_t slot = (int64_t)blk * kLocalRows + tid;
+  if (slot < nb_node) {
+    const int32_t r = perm[slot];
+    const int32_t o = val_off[slot];
+    for (int32_t k = row_index[r]; k < row_index[r + 1]; ++k)
+      buf[o + (k - row_index[r])] = columns[k];
+  }
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = tid; k < n2 / 2; k += 256) {
+        const int i = (k / stride) * 2 * stride + (k % stride);
+        const int j = i + stride;
+        const bool asc = (i & size) == 0;
+        const int32_t x = buf[i], y = buf[j];
+        if ((x > y) == asc) {
+          buf[i] = y;
+          buf[j] = x;
+        }
+      }
+      __syncthreads();
+    }
+  const int chunk = n2 / 256;
+  const int k0 = tid * chunk;
+  int mine = 0;
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < cnt && (k == 0 || buf[k] != buf[k - 1]))
+      ++mine;
+  scan[tid] = mine;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = (tid >= off) ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  int pos = scan[tid] - mine;
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < cnt && (k == 0 || buf[k] != buf[k - 1])) {
+      if (pos < kLocalMaxNodes)
+        table[(int64_t)blk * kLocalMaxNodes + pos] = buf[k];
+      ++pos;
+    }
+  if (tid == 255)
+    count[blk] = scan[255];
+}
+
+// Sliced local-plan words of each brick (table indices + row positions, as
+// k_local_plan) and the cell ids, word j of slot t at slice[blk] + j*256 + t.
+__global__ void __launch_bounds__(256)
+k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+             const int32_t* __restrict__ cells, int npc, int32_t nb_node, const int32_t* __restrict__ table,
+             const int32_t* __restrict__ count, const int64_t* __restrict__ slice, uint64_t* __restrict__ out_plan,
+             int32_t* __restrict__ out_cell, int32_t* __restrict__ err)
+{
+  __shared__ int32_t tab[kLocalMaxNodes];
+  const int32_t blk = blockIdx.x;
+  const int tn = count[blk];
+  if (tn < 0 || tn > kLocalMaxNodes) {
+    if (threadIdx.x == 0)
+      atomicOr(err, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < tn; k += 256)
+    tab[k] = table[(int64_t)blk * kLocalMaxNodes + k];
+  __syncthreads();
+  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
+  if (slot >= nb_node)
+    return;
+  const int32_t i = perm[slot];
+  const int64_t base = slice[blk] + threadIdx.x;
+  bool bad = false;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const uint64_t w = plan[e];
+    const int32_t cell = plan_cell(w);
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)cell);
+    const int3 o = other_nodes(cq, plan_slot(w));
+    const int32_t nodes[3] = { o.x, o.y, o.z };
+    uint64_t lw = 0;
+    for (int k = 0; k < npc - 1; ++k) {
+      const int l = find_node(tab, tn, nodes[k]);
+      const int pos = plan_pos(w, k);
+      bad = bad || l >= tn || tab[l] != nodes[k] || pos >= kCompactMaxRow;
+      lw |= (uint64_t)(l & 2047) << (11 * k);
+      lw |= (uint64_t)(pos & 31) << (33 + 5 * k);
+    }
+    const int64_t at = base + (int64_t)(e - nc_off[i]) * kLocalRows;
+    out_plan[at] = lw;
+    out_cell[at] = cell;
+  }
+  if (bad)
+    atomicOr(err, 1);
+}
+
+static uint64_t morton_spread(uint64_t v) // 21 bits -> every third bit
+{
+  v &= 0x1fffffULL;
+  v = (v | (v << 32)) & 0x1f00000000ffffULL;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+  v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+  v = (v | (v << 2)) & 0x1249249249249249ULL;
+  return v;
+}
+
+static void build_brick_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  auto release_all = [b]() {
+    b->bperm.release();
+    b->bval_off.release();
+    b->btotal.release();
+    b->btable.release();
+    b->bcount.release();
+    b->bslice.release();
+    b->bplan.release();
+    b->bcell.release();
+    b->brick_val_max = 0;
+    b->brick_nodes_max = 0;
+  };
+  release_all();
+  if (!b->plan.p || b->max_row > kCompactMaxRow)
+    return;
+  const int32_t n = m.nb_node;
+  // Morton order of the node coordinates (21 bits per axis in the bounding box)
+  std::vector<double> xyz((size_t)n * 4);
+  m.coords.download(xyz.data(), xyz.size());
+  double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+  for (int32_t i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], xyz[4 * (size_t)i + d]);
+      hi[d] = std::max(hi[d], xyz[4 * (size_t)i + d]);
+    }
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+  for (int32_t i = 0; i < n; ++i) {
+    uint64_t code = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double ext = hi[d] - lo[d];
+      const double t = ext > 0.0 ? (xyz[4 * (size_t)i + d] - lo[d]) / ext : 0.0;
+      const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+      code |= morton_spread(q) << d;
+    }
+    key[i] = { code, i };
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm((size_t)n);
+  for (int32_t i = 0; i < n; ++i)
+    perm[i] = key[i].second;
+  const int32_t nblk = (n + kLocalRows - 1) / kLocalRows;
+  b->bperm.alloc(n);
+  b->bperm.upload(perm.data(), perm.size());
+  b->bval_off.alloc(n);
+  b->btotal.alloc(nblk);
+  hipLaunchKernelGGL(k_brick_offsets, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, b->row_index.p, n,
+                     b->bval_off.p, b->btotal.p);
+  check_launch("brick_offsets");
+  b->btable.alloc((size_t)nblk * kLocalMaxNodes);
+  b->bcount.alloc(nblk);
+  hipLaunchKernelGGL(k_brick_nodes, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, b->bval_off.p, b->btotal.p,
+                     b->row_index.p, b->columns.p, n, b->btable.p, b->bcount.p);
+  check_launch("brick_nodes");
+  std::vector<int32_t> totals((size_t)nblk), counts((size_t)nblk);
+  b->btotal.download(totals.data(), totals.size());
+  b->bcount.download(counts.data(), counts.size());
+  int32_t vmax = 0, cmax = 0;
+  for (int32_t k = 0; k < nblk; ++k) {
+    if (totals[k] < 0 || counts[k] < 0 || counts[k] > kLocalMaxNodes) {
+      release_all();
+      return;
+    }
+    vmax = std::max(vmax, totals[k]);
+    cmax = std::max(cmax, counts[k]);
+  }
+  // slices: width = the brick's longest node->cell list
+  std::vector<int32_t> off((size_t)n + 1);
+  m.nc_offsets.download(off.data(), off.size());
+  std::vector<int64_t> slice((size_t)nblk + 1, 0);
+  for (int32_t k = 0; k < nblk; ++k) {
+    int32_t w = 0;
+    for (int32_t t = k * kLocalRows; t < std::min(n, (k + 1) * kLocalRows); ++t) {
+      const int32_t r = perm[t];
+      w = std::max(w, off[r + 1] - off[r]);
+    }
+    slice[k + 1] = slice[k] + (int64_t)w * kLocalRows;
+  }
+  b->bslice.alloc(slice.size());
+  b->bslice.upload(slice.data(), slice.size());
+  b->bplan.alloc((size_t)slice[nblk]);
+  b->bcell.alloc((size_t)slice[nblk]);
+  DBuf<int32_t> err(1);
+  err.zero();
+  hipLaunchKernelGGL(k_brick_plan, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, m.nc_offsets.p, b->plan.p,
+                     m.cells.p, m.npc, n, b->btable.p, b->bcount.p, b->bslice.p, b->bplan.p, b->bcell.p, err.p);
+  check_launch("brick_plan");
+  int32_t h = 0;
+  err.download(&h, 1);
+  if (h) {
+    release_all();
+    return;
+  }
+  b->brick_val_max = vmax;
+  b->brick_nodes_max = cmax;
+}
+
 // Sliced copy of the local plan and of the cell ids: word j of row r of
 // block blk at slice[blk] + j * 256 + (r - r0).
 __global__ void __launch_bounds__(256)
@@ -943,6 +1207,101 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
     values[a + k] = s_val[k];
 }
 
+// Brick-ordered Laplace assembly: thread t of brick blk owns row
+// perm[blk * 256 + t] and writes its segment of the value array at the end;
+// otherwise as k_assemble_laplace_lplan (SLICED).  The row accumulators are
+// interleaved in LDS -- slot k of row t at (t / 64) * 64 * max_row + k * 64 +
+// t % 64 -- so the 64 lanes of a wave always hit 64 consecutive doubles
+// whatever their column positions (no bank conflicts).
+template <int NPC, int U>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace_brick(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
+                         const int32_t* __restrict__ perm, const int32_t* __restrict__ bval_off,
+                         const int32_t* __restrict__ total, const uint64_t* __restrict__ bplan,
+                         const int32_t* __restrict__ bcell, const int64_t* __restrict__ slice,
+                         const int32_t* __restrict__ table, const int32_t* __restrict__ count,
+                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
+                         int32_t nb_node, int val_cap, int node_cap, const double* __restrict__ cell_lambda,
+                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t blk = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int cnt = val_cap; // kLocalRows * max_row interleaved slots
+  const int max_row = val_cap / kLocalRows;
+  const int tn = count[blk];
+  double* s_val = reinterpret_cast<double*>(smem);
+  double2* s_xy = reinterpret_cast<double2*>(s_val + val_cap); // val_cap is even
+  double* s_z = reinterpret_cast<double*>(s_xy + node_cap);
+  const int32_t* tab = table + (int64_t)blk * kLocalMaxNodes;
+  for (int k = threadIdx.x; k < tn; k += blockDim.x) {
+    const int64_t g = tab[k];
+    s_xy[k] = *reinterpret_cast<const double2*>(coords + 4 * g);
+    s_z[k] = coords[4 * g + 2];
+  }
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+    s_val[k] = 0.0;
+  __syncthreads();
+  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
+  if (slot < nb_node) {
+    const int32_t i = perm[slot];
+    double* rv = s_val + (threadIdx.x >> 6) * (64 * max_row) + (threadIdx.x & 63); // slot k at rv[64 k]
+    const d3 p = load_node(coords, i);
+    const int32_t ncell = nc_off[i + 1] - nc_off[i];
+    const int64_t wbase = slice[blk] + threadIdx.x;
+    double diag = 0.0, det_sum = 0.0;
+    for (int32_t e = 0; e < ncell; e += U) {
+      uint64_t w[U];
+      double lam[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t at = wbase + (int64_t)min(e + u, ncell - 1) * kLocalRows;
+        w[u] = bplan[at];
+        lam[u] = cell_lambda ? cell_lambda[bcell[at]] : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= ncell)
+          break;
+        const int la = lplan_idx(w[u], 0), lb = lplan_idx(w[u], 1);
+        const double2 xa = s_xy[la], xb = s_xy[lb];
+        const d3 ea = { xa.x - p.x, xa.y - p.y, s_z[la] - p.z };
+        const d3 eb = { xb.x - p.x, xb.y - p.y, s_z[lb] - p.z };
+        if constexpr (NPC == 4) {
+          const int lc = lplan_idx(w[u], 2);
+          const double2 xc = s_xy[lc];
+          const d3 ec = { xc.x - p.x, xc.y - p.y, s_z[lc] - p.z };
+          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+          const double adet = fabs(dot(ea, Ca));
+          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+          const double scale = lam[u] * fast_rcp(6.0 * adet);
+          diag += scale * dot(Cp, Cp);
+          lds_accumulate(rv + 64 * lplan_pos(w[u], 0), scale * dot(Cp, Ca));
+          lds_accumulate(rv + 64 * lplan_pos(w[u], 1), scale * dot(Cp, Cb));
+          lds_accumulate(rv + 64 * lplan_pos(w[u], 2), scale * dot(Cp, Cc));
+          det_sum += adet;
+        }
+        else {
+          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
+          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+          const double gpx = -(gax + gbx), gpy = -(gay + gby);
+          const double scale = lam[u] * fast_rcp(2.0 * adet);
+          diag += scale * (gpx * gpx + gpy * gpy);
+          lds_accumulate(rv + 64 * lplan_pos(w[u], 0), scale * (gpx * gax + gpy * gay));
+          lds_accumulate(rv + 64 * lplan_pos(w[u], 1), scale * (gpx * gbx + gpy * gby));
+          det_sum += adet;
+        }
+      }
+    }
+    rv[64 * diag_pos[i]] += diag;
+    if (rhs)
+      rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
+    // only this thread writes into rv, so its row is complete here
+    const int32_t a = row_index[i], len = row_index[i + 1] - a;
+    for (int k = 0; k < len; ++k)
+      values[a + k] = rv[64 * k];
+  }
+}
+
 static void build_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
@@ -965,6 +1324,7 @@ static void build_plan(afem_bsr* b)
   build_compact_plan(b);
   build_local_plan(b);
   build_sliced_plan(b);
+  build_brick_plan(b);
 }
 
 // Laplace assembly driven by the plan: only the values are staged in LDS.
@@ -1074,7 +1434,8 @@ enum PlanKind
   kPlan = 1,
   kCompact = 2,
   kLocal = 3,
-  kLocalSliced = 4
+  kLocalSliced = 4,
+  kBrick = 5
 };
 struct LaplaceVariant
 {
@@ -1085,7 +1446,7 @@ struct LaplaceVariant
 static const LaplaceVariant kLaplaceVariants[] = {
   { kSearch, 4, 256 },  { kPlan, 4, 256 },  { kPlan, 8, 256 },        { kCompact, 4, 256 },
   { kCompact, 8, 128 }, { kLocal, 4, 256 }, { kLocal, 8, 256 },       { kLocal, 16, 256 },
-  { kLocalSliced, 4, 256 }, { kLocalSliced, 8, 256 },
+  { kLocalSliced, 4, 256 }, { kLocalSliced, 8, 256 }, { kBrick, 4, 256 }, { kBrick, 8, 256 },
 };
 constexpr int kNumLaplaceVariants = sizeof(kLaplaceVariants) / sizeof(kLaplaceVariants[0]);
 constexpr int kDefaultLaplaceVariant = 6;
@@ -1102,6 +1463,19 @@ static void launch_laplace_lplan(const afem_bsr* b, const double* lam, double f,
                      SLICED ? b->lcell_sl.p : m.nc_items.p, SLICED ? b->lplan_sl.p : b->lplan.p, b->slice.p,
                      b->block_nodes.p, b->block_count.p, b->diag_pos.p, b->row_index.p, m.nb_node, val_cap, node_cap,
                      lam, f, rhs, b->values.p);
+}
+
+template <int NPC, int U>
+static void launch_laplace_brick(const afem_bsr* b, const double* lam, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  const int val_cap = kLocalRows * b->max_row; // interleaved row slots (even)
+  const int node_cap = b->brick_nodes_max;
+  const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap) * sizeof(double);
+  hipLaunchKernelGGL((k_assemble_laplace_brick<NPC, U>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
+                     lds, stream(), m.coords.p, m.nc_offsets.p, b->bperm.p, b->bval_off.p, b->btotal.p, b->bplan.p,
+                     b->bcell.p, b->bslice.p, b->btable.p, b->bcount.p, b->diag_pos.p, b->row_index.p, m.nb_node,
+                     val_cap, node_cap, lam, f, rhs, b->values.p);
 }
 
 template <int NPC, int U>
@@ -1127,6 +1501,9 @@ static LaplaceVariant effective_variant(const afem_bsr* b, int v)
 {
   LaplaceVariant var = kLaplaceVariants[v];
   const size_t local_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->block_nodes_max) * sizeof(double);
+  const size_t brick_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->brick_nodes_max) * sizeof(double);
+  if (var.kind == kBrick && (!b->bplan.p || brick_lds > 160 * 1024))
+    var.kind = kLocalSliced;
   if (var.kind == kLocalSliced && !b->lplan_sl.p)
     var.kind = kLocal;
   if ((var.kind == kLocal || var.kind == kLocalSliced) && (!b->lplan.p || local_lds > 160 * 1024))
@@ -1147,7 +1524,21 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
   const bool fits = assembly_shape(*b, var.kind == kSearch ? 12 : 8, var.rows, &rows, &lds);
   AFEM_REQUIRE(fits, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
   ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
-  if (var.kind == kLocalSliced) {
+  if (var.kind == kBrick) {
+    if (m.npc == 4) {
+      if (var.u == 4)
+        launch_laplace_brick<4, 4>(b, cell_lambda, f, rhs);
+      else
+        launch_laplace_brick<4, 8>(b, cell_lambda, f, rhs);
+    }
+    else {
+      if (var.u == 4)
+        launch_laplace_brick<3, 4>(b, cell_lambda, f, rhs);
+      else
+        launch_laplace_brick<3, 8>(b, cell_lambda, f, rhs);
+    }
+  }
+  else if (var.kind == kLocalSliced) {
     if (m.npc == 4) {
       if (var.u == 4)
         launch_laplace_lplan<4, 4, true>(b, cell_lambda, f, rhs);

@@ -226,6 +226,15 @@ struct afem_bsr
   afem::DBuf<uint64_t> lplan_sl;     // the same words sliced per block of rows (coalesced reads)
   afem::DBuf<int32_t> lcell_sl;      // cell id of each sliced word
   afem::DBuf<int64_t> slice;         // per block of rows: first sliced word
+  // brick order (Morton order of the nodes, blocks of 256 rows)
+  afem::DBuf<int32_t> bperm;         // rows in brick order
+  afem::DBuf<int32_t> bval_off;      // LDS offset of each row within its brick
+  afem::DBuf<int32_t> btotal;        // values per brick
+  afem::DBuf<int32_t> btable, bcount; // per brick: sorted referenced nodes (fixed stride), count
+  afem::DBuf<int64_t> bslice;        // per brick: first sliced plan word
+  afem::DBuf<uint64_t> bplan;        // sliced local-plan words of the bricks
+  afem::DBuf<int32_t> bcell;         // cell id of each brick plan word
+  int32_t brick_val_max = 0, brick_nodes_max = 0;
   afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
   afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes
   int32_t block_nodes_max = 0;       // largest node table

@@ -251,7 +251,7 @@ k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restric
 // col - row at (off[s] + j) * 64 + lane (padding slots: 0 with a zero value).
 // The value and offset streams are read once, so they are loaded non-temporal
 // to leave the L2 to the gathered vector.
-template <class VT, class XT, int MODE>
+template <class VT, class XT, int MODE, int UNR = 4>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
               const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
@@ -272,6 +272,23 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
       const XT* xr = x + row;
       A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
       int j = 0;
+      if constexpr (UNR == 8) {
+        // all offsets and values of 8 slots first, then the 8 gathers
+        for (; j + 7 < w; j += 8) {
+          const int64_t i0 = base + (int64_t)j * 64;
+          int d[8];
+          VT m[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            d[q] = __builtin_nontemporal_load(dcol + i0 + 64 * q);
+            m[q] = __builtin_nontemporal_load(v + i0 + 64 * q);
+          }
+          s0 += (A)m[0] * (A)xr[d[0]] + (A)m[4] * (A)xr[d[4]];
+          s1 += (A)m[1] * (A)xr[d[1]] + (A)m[5] * (A)xr[d[5]];
+          s2 += (A)m[2] * (A)xr[d[2]] + (A)m[6] * (A)xr[d[6]];
+          s3 += (A)m[3] * (A)xr[d[3]] + (A)m[7] * (A)xr[d[7]];
+        }
+      }
       for (; j + 3 < w; j += 4) {
         const int64_t i0 = base + (int64_t)j * 64;
         const int d0 = __builtin_nontemporal_load(dcol + i0);
@@ -1083,7 +1100,8 @@ enum SpmvFormat
 {
   kFmtCsr = 0,
   kFmtSell = 1,
-  kFmtSell16 = 2 // SELL-64 with 16-bit column offsets
+  kFmtSell16 = 2, // SELL-64 with 16-bit column offsets
+  kFmtSell16x8 = 3 // the same, 8 slots per step
 };
 
 struct afem_solver_ws
@@ -1330,7 +1348,11 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
                        w->sell_col.p, sell_vals, x, y, b, gate, part_out);
   }
   else if (w->format == kFmtSell16) {
-    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, 4>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                       w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+  }
+  else if (w->format == kFmtSell16x8) {
+    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, 8>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
                        w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
   }
   else {
@@ -1365,8 +1387,8 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     return;
   }
   ensure_sell(s, w);
-  if (forced == -2 && w->has_dcol) {
-    w->format = kFmtSell16;
+  if ((forced == -2 || forced == -3) && w->has_dcol) {
+    w->format = forced == -2 ? kFmtSell16 : kFmtSell16x8;
     return;
   }
   if (forced < 0) {
@@ -1384,8 +1406,10 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     cands = { { kFmtCsr, 16 }, { kFmtSell, 0 } };
   else
     cands = { { kFmtCsr, 8 }, { kFmtCsr, 16 }, { kFmtSell, 0 } };
-  if (s->block == 1 && w->has_dcol)
+  if (s->block == 1 && w->has_dcol) {
     cands.push_back({ kFmtSell16, 0 });
+    cands.push_back({ kFmtSell16x8, 0 });
+  }
   hipEvent_t e0, e1;
   AFEM_HIP(hipEventCreate(&e0));
   AFEM_HIP(hipEventCreate(&e1));
@@ -1497,7 +1521,7 @@ static double true_rr(afem_system* s, afem_solver_ws* w, const Parts& P, int nbl
 
 static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
 {
-  if (w->format != kFmtSell && w->format != kFmtSell16)
+  if (w->format != kFmtSell && w->format != kFmtSell16 && w->format != kFmtSell16x8)
     return;
   if (!w->spai_sell_valid) {
     w->spai_sell32.alloc(w->sell_col.n);
@@ -1522,7 +1546,7 @@ static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P
 {
   ProfScope prof("fpcg_precond_spmv");
   const PrecIn in{ w->r32.p, w->r64.p, true };
-  if (w->format == kFmtSell16) {
+  if (w->format == kFmtSell16 || w->format == kFmtSell16x8) {
     if (bf16)
       hipLaunchKernelGGL((k_prec_sell16<INIT, uint16_t>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof,
                          w->sell_off.p, w->sell_dcol.p, w->spai_sell_bf16.p, in, w->r32.p, w->z32.p, w->p32.p,
@@ -1619,8 +1643,7 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   choose_format(s, w, o.spmv_lanes);
   ensure_derived(s, w, mixed);
   // BF16 preconditioner values need a SELL layout; CSR keeps FP32
-  const bool bf16 = flex && o.precond_bf16 && (w->format == kFmtSell || w->format == kFmtSell16);
-  const bool bf16_used = bf16 && w->format == kFmtSell16;
+  const bool bf16_used = flex && o.precond_bf16 && (w->format == kFmtSell16 || w->format == kFmtSell16x8);
   if (flex)
     ensure_spai_sell(s, w, bf16_used);
   const Parts P = parts_of(w);

@@ -267,7 +267,8 @@ typedef struct afem_solver_options {
   int spmv_lanes;        /* SpMV format: 0 = autotuned, > 0 = CSR with that many
                             lanes per row (4, 8, 16, 32), -1 = SELL-64,
                             -2 = SELL-64 with 16-bit column offsets (scalar
-                            matrices whose |col - row| fits; else SELL-64) */
+                            matrices whose |col - row| fits; else SELL-64),
+                            -3 = the same with 8 slots per step */
   int norm;              /* stopping norm: 0 = ||b - A x||_2 <= max(rtol ||b||_2, atol);
                             1 = natural norm sqrt(r.M^-1 r) <= rtol * its initial value;
                             2 = preconditioned norm ||M^-1 r||_2 <= max(rtol ||M^-1 b||_2,
@@ -309,8 +310,8 @@ AFEM_API int afem_system_set_spai(afem_system_t sys, const float* values);
 
 /* SpMV format the autotuner chose for the system's matrix: *format = 0 (CSR,
  * *lanes lanes per row, or per 3x3 block row when the matrix has blocks),
- * 1 (SELL-64) or 2 (SELL-64 with 16-bit column offsets); -1 while no SpMV
- * or solve has run on the current matrix. */
+ * 1 (SELL-64), 2 (SELL-64 with 16-bit column offsets) or 3 (the same, 8
+ * slots per step); -1 while no SpMV or solve has run on the current matrix. */
 AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes);
 
 /* ---- heterogeneous coefficients (no counterpart in the reference) ------------------- */

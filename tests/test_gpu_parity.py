@@ -83,7 +83,7 @@ def test_node_cells_and_pattern_bit_exact(afem_gpu, spec):
     assert info.nb_col == len(ocol) and info.max_row_length == np.diff(orow).max()
 
 
-N_LAPLACE_VARIANTS = 10
+N_LAPLACE_VARIANTS = 12
 
 
 @pytest.mark.parametrize("spec", MESHES, ids=[m[0] for m in MESHES])
@@ -295,10 +295,10 @@ def poisson_bottom_system(A, dim, n, alpha, lam=None):
     return X, C, m, b, s, dl
 
 
-SPMV_FORMATS = [0, -1, -2, 8]  # autotuned, SELL-64, SELL-64 with 16-bit offsets, CSR with 8 lanes per row
+SPMV_FORMATS = [0, -1, -2, -3, 8]  # autotuned, SELL-64, SELL-64 with 16-bit offsets (4 / 8 slots per step), CSR 8
 
 
-@pytest.mark.parametrize("fmt", [-1, -2, 8])
+@pytest.mark.parametrize("fmt", [-1, -2, -3, 8])
 def test_spmv_forced_formats(afem_gpu, fmt):
     """Each SpMV format against the oracle (the format is pinned by a one-iteration solve)."""
     A = afem_gpu
@@ -307,7 +307,7 @@ def test_spmv_forced_formats(afem_gpu, fmt):
     N = X.shape[0]
     m, b, s, _ = build_poisson(A, 3, X, C)
     s.solve(raise_on_failure=False, method=A.SOLVER_PCG_MIXED, max_iter=1, spmv_lanes=fmt)
-    want = {-1: ("sell", 64), -2: ("sell16", 64), 8: ("csr", 8)}[fmt]
+    want = {-1: ("sell", 64), -2: ("sell16", 64), -3: ("sell16x8", 64), 8: ("csr", 8)}[fmt]
     assert s.spmv_format() == want
     row, col, val, _ = b.download()
     x = np.random.default_rng(3).normal(size=N)
