@@ -110,7 +110,7 @@ def stats_markdown(path, command, top=20):
     return "\n".join(lines) + "\n"
 
 
-def pmc_markdown(summary, command, workload, top=25):
+def pmc_markdown(summary, command, workload, top=60):
     lines = [
         f"# HBM traffic per dispatch ({workload})",
         "",
