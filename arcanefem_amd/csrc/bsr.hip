@@ -991,21 +991,36 @@ static void build_brick_plan(afem_bsr* b)
       lo[d] = std::min(lo[d], xyz[4 * (size_t)i + d]);
       hi[d] = std::max(hi[d], xyz[4 * (size_t)i + d]);
     }
-  std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+  std::vector<uint64_t> code((size_t)n), code2((size_t)n);
   for (int32_t i = 0; i < n; ++i) {
-    uint64_t code = 0;
+    uint64_t c = 0;
     for (int d = 0; d < 3; ++d) {
       const double ext = hi[d] - lo[d];
       const double t = ext > 0.0 ? (xyz[4 * (size_t)i + d] - lo[d]) / ext : 0.0;
       const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
-      code |= morton_spread(q) << d;
+      c |= morton_spread(q) << d;
     }
-    key[i] = { code, i };
+    code[i] = c;
   }
-  std::sort(key.begin(), key.end());
-  std::vector<int32_t> perm((size_t)n);
+  // stable LSD radix sort of (code, node) by code: four 16-bit digits
+  std::vector<int32_t> perm((size_t)n), perm2((size_t)n);
   for (int32_t i = 0; i < n; ++i)
-    perm[i] = key[i].second;
+    perm[i] = i;
+  std::vector<int64_t> hist(65536 + 1);
+  for (int shift = 0; shift < 64; shift += 16) {
+    std::fill(hist.begin(), hist.end(), 0);
+    for (int32_t i = 0; i < n; ++i)
+      ++hist[((code[i] >> shift) & 0xffff) + 1];
+    for (int k = 0; k < 65536; ++k)
+      hist[k + 1] += hist[k];
+    for (int32_t i = 0; i < n; ++i) {
+      const int64_t at = hist[(code[i] >> shift) & 0xffff]++;
+      code2[at] = code[i];
+      perm2[at] = perm[i];
+    }
+    code.swap(code2);
+    perm.swap(perm2);
+  }
   const int32_t nblk = (n + kLocalRows - 1) / kLocalRows;
   b->bperm.alloc(n);
   b->bperm.upload(perm.data(), perm.size());

@@ -300,9 +300,20 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
         s2 += (A)__builtin_nontemporal_load(v + i0 + 128) * (A)xr[d2];
         s3 += (A)__builtin_nontemporal_load(v + i0 + 192) * (A)xr[d3];
       }
-      for (; j < w; ++j) {
+      if (j < w) { // the last 1-3 slots, loads issued together
         const int64_t i0 = base + (int64_t)j * 64;
-        s0 += (A)__builtin_nontemporal_load(v + i0) * (A)xr[__builtin_nontemporal_load(dcol + i0)];
+        const int rem = w - j;
+        int d[3] = { 0, 0, 0 };
+        VT m[3] = { 0, 0, 0 };
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+          if (u < rem) {
+            d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
+            m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
+          }
+        s0 += (A)m[0] * (A)xr[d[0]];
+        s1 += (A)m[1] * (A)xr[d[1]];
+        s2 += (A)m[2] * (A)xr[d[2]];
       }
       spmv_epilogue<MODE>(row, (double)((s0 + s1) + (s2 + s3)), x, y, b, part);
     }
@@ -922,6 +933,89 @@ __device__ __forceinline__ float mval_at(const uint16_t* __restrict__ m, int64_t
   return __uint_as_float((uint32_t)__builtin_nontemporal_load(m + i) << 16);
 }
 
+// z = M r_in over the rows of this workgroup with the input fixed at compile
+// time (FROM64: the FP64 residual of a reliable update, converted).
+template <int INIT, class MT, bool FROM64>
+__device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __restrict__ off,
+                                                 const int16_t* __restrict__ dcol, const MT* __restrict__ mv,
+                                                 const PrecIn& in, float* __restrict__ r32, float* __restrict__ z,
+                                                 float* __restrict__ p, const float* __restrict__ q,
+                                                 const double* __restrict__ b, double& rz, double& zq, double& rr,
+                                                 double& bb)
+{
+  auto xin = [&](int64_t c) -> float { return FROM64 ? (float)in.r64[c] : in.r32[c]; };
+  int64_t t0, t1;
+  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t row = t * kBlock + threadIdx.x;
+    if (row >= n)
+      continue;
+    const int64_t s = row >> 6;
+    const int64_t base = (int64_t)off[s] * 64 + (row & 63);
+    const int w = off[s + 1] - off[s];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int j = 0;
+    for (; j + 7 < w; j += 8) {
+      const int64_t i0 = base + (int64_t)j * 64;
+      int d[8];
+      float m[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
+        m[u] = mval_at(mv, i0 + 64 * u);
+      }
+      s0 += m[0] * xin(row + d[0]) + m[4] * xin(row + d[4]);
+      s1 += m[1] * xin(row + d[1]) + m[5] * xin(row + d[5]);
+      s2 += m[2] * xin(row + d[2]) + m[6] * xin(row + d[6]);
+      s3 += m[3] * xin(row + d[3]) + m[7] * xin(row + d[7]);
+    }
+    for (; j + 3 < w; j += 4) {
+      const int64_t i0 = base + (int64_t)j * 64;
+      int d[4];
+      float m[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
+        m[u] = mval_at(mv, i0 + 64 * u);
+      }
+      s0 += m[0] * xin(row + d[0]);
+      s1 += m[1] * xin(row + d[1]);
+      s2 += m[2] * xin(row + d[2]);
+      s3 += m[3] * xin(row + d[3]);
+    }
+    if (j < w) { // the last 1-3 slots, loads issued together
+      const int64_t i0 = base + (int64_t)j * 64;
+      const int rem = w - j;
+      int d[3] = { 0, 0, 0 };
+      float m[3] = { 0.f, 0.f, 0.f };
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (u < rem) {
+          d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
+          m[u] = mval_at(mv, i0 + 64 * u);
+        }
+      s0 += m[0] * xin(row + d[0]);
+      s1 += m[1] * xin(row + d[1]);
+      s2 += m[2] * xin(row + d[2]);
+    }
+    const float zrow = (s0 + s1) + (s2 + s3);
+    const float rin = xin(row);
+    z[row] = zrow;
+    if (FROM64)
+      r32[row] = rin; // no other thread reads r32 on such an iteration
+    rz += (double)zrow * (double)rin;
+    if constexpr (INIT) {
+      p[row] = zrow;
+      const double bv = b[row], r64v = in.r64[row];
+      bb += bv * bv;
+      rr += r64v * r64v;
+    }
+    else {
+      zq += (double)zrow * (double)q[row];
+    }
+  }
+}
+
 template <int INIT, class MT>
 __global__ void __launch_bounds__(kBlock)
 k_prec_sell16(int k, int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
@@ -931,35 +1025,11 @@ k_prec_sell16(int k, int32_t n, const int32_t* __restrict__ off, const int16_t* 
 {
   if (!prec_open<INIT>(st, P, nparts, k, in))
     return;
-  int64_t t0, t1;
-  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
   double rz = 0.0, zq = 0.0, rr = 0.0, bb = 0.0;
-  for (int64_t t = t0; t < t1; ++t) {
-    const int64_t row = t * kBlock + threadIdx.x;
-    if (row < n) {
-      const int64_t s = row >> 6;
-      const int64_t base = (int64_t)off[s] * 64 + (row & 63);
-      const int w = off[s + 1] - off[s];
-      float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-      int j = 0;
-      for (; j + 3 < w; j += 4) {
-        const int64_t i0 = base + (int64_t)j * 64;
-        const int d0 = __builtin_nontemporal_load(dcol + i0);
-        const int d1 = __builtin_nontemporal_load(dcol + i0 + 64);
-        const int d2 = __builtin_nontemporal_load(dcol + i0 + 128);
-        const int d3 = __builtin_nontemporal_load(dcol + i0 + 192);
-        s0 += mval_at(mv, i0) * in.at(row + d0);
-        s1 += mval_at(mv, i0 + 64) * in.at(row + d1);
-        s2 += mval_at(mv, i0 + 128) * in.at(row + d2);
-        s3 += mval_at(mv, i0 + 192) * in.at(row + d3);
-      }
-      for (; j < w; ++j) {
-        const int64_t i0 = base + (int64_t)j * 64;
-        s0 += mval_at(mv, i0) * in.at(row + __builtin_nontemporal_load(dcol + i0));
-      }
-      prec_row<INIT>(row, (s0 + s1) + (s2 + s3), in, r32, z, p, q, b, rz, zq, rr, bb);
-    }
-  }
+  if (in.from64)
+    prec_sell16_rows<INIT, MT, true>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
+  else if constexpr (!INIT)
+    prec_sell16_rows<INIT, MT, false>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
   prec_parts<INIT>(P, k, rz, zq, rr, bb);
 }
 

@@ -406,17 +406,29 @@ k_gemm_dw(int32_t n, int32_t nodes_per_wave, const uint16_t* __restrict__ X1, co
   }
 }
 
-// out[q] (+)= sum_b partial[b * stride + q] for q < len, in block order.
-__global__ void k_reduce_partials(int nparts, int stride, int len, const float* __restrict__ partial,
-                                  float* __restrict__ out, int accumulate)
+// out[q] (+)= sum_b partial[b * stride + q] for q < len: 32 outputs per
+// workgroup, 8 groups of lanes take every 8th partial, and the 8 group sums
+// are added in a fixed order (deterministic).
+__global__ void __launch_bounds__(256)
+k_reduce_partials(int nparts, int stride, int len, const float* __restrict__ partial, float* __restrict__ out,
+                  int accumulate)
 {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= len)
-    return;
+  __shared__ double red[8][32];
+  const int ql = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int q = blockIdx.x * 32 + ql;
   double s = 0.0;
-  for (int b = 0; b < nparts; ++b)
-    s += partial[(int64_t)b * stride + q];
-  out[q] = accumulate ? out[q] + (float)s : (float)s;
+  if (q < len)
+    for (int b = g; b < nparts; b += 8)
+      s += partial[(int64_t)b * stride + q];
+  red[g][ql] = s;
+  __syncthreads();
+  if (g == 0 && q < len) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      t += red[k][ql];
+    out[q] = accumulate ? out[q] + (float)t : (float)t;
+  }
 }
 
 // ---- message passing and heads ------------------------------------------------------------------
@@ -642,20 +654,28 @@ k_head_backward(int32_t n, const int32_t* __restrict__ rp, const int32_t* __rest
   block_sum_write<kHeadParts>(pv, part + (int64_t)blockIdx.x * kHeadParts);
 }
 
-__global__ void k_reduce_head(int nparts, const double* __restrict__ part, float* __restrict__ grad)
+// one workgroup per head parameter q
+__global__ void __launch_bounds__(256)
+k_reduce_head(int nparts, const double* __restrict__ part, float* __restrict__ grad)
 {
-  const int q = threadIdx.x;
-  if (q >= kHeadParts)
-    return;
+  __shared__ double lds[4];
+  const int q = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nparts; ++b)
+  for (int b = threadIdx.x; b < nparts; b += 256)
     s += part[(int64_t)b * kHeadParts + q];
-  if (q < kH)
-    grad[oWd + q] = (float)s;
-  else if (q == kH)
-    grad[oBd] = (float)s;
-  else
-    grad[oC] = (float)s;
+  s = wave_reduce_sum(s);
+  if ((threadIdx.x & 63) == 0)
+    lds[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = lds[0] + lds[1] + lds[2] + lds[3];
+    if (q < kH)
+      grad[oWd + q] = (float)t;
+    else if (q == kH)
+      grad[oBd] = (float)t;
+    else
+      grad[oC] = (float)t;
+  }
 }
 
 __global__ void k_sum_double(int nparts, const double* __restrict__ part, double* __restrict__ out)
@@ -875,11 +895,11 @@ void weight_grad(afem_spai* m, int32_t n, const uint16_t* X1, const uint16_t* X2
   hipLaunchKernelGGL(k_gemm_dw<K>, dim3(kDwBlocks), dim3(256), 0, stream(), n, per, X1, X2, dY, pre, m->dwp.p);
   check_launch("spai_gemm_dw");
   const int stride = K * 32 + 32;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((K * 32 + 255) / 256), dim3(256), 0, stream(), kDwBlocks, stride,
+  hipLaunchKernelGGL(k_reduce_partials, dim3((K * 32 + 31) / 32), dim3(256), 0, stream(), kDwBlocks, stride,
                      K * 32, m->dwp.p, m->grad.p + w_off, 0);
   if (b_off >= 0)
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(32), 0, stream(), kDwBlocks, stride, 32, m->dwp.p + K * 32,
-                       m->grad.p + b_off, 0);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, stream(), kDwBlocks, stride, 32,
+                       m->dwp.p + K * 32, m->grad.p + b_off, 0);
   check_launch("spai_reduce_dw");
 }
 
@@ -895,7 +915,7 @@ void backward(afem_spai* m, afem_system* s)
   hipLaunchKernelGGL(k_head_backward, dim3(g), dim3(256), 0, stream(), n, s->row_index, s->columns, m->shat.p,
                      m->ident.p, m->h[kL].p, m->u.p, m->pre_d.p, m->params.p, m->y.p, m->r.p, coef, m->du.p, dh,
                      m->part.p);
-  hipLaunchKernelGGL(k_reduce_head, dim3(1), dim3(64), 0, stream(), (int)g, m->part.p, m->grad.p);
+  hipLaunchKernelGGL(k_reduce_head, dim3(kHeadParts), dim3(256), 0, stream(), (int)g, m->part.p, m->grad.p);
   check_launch("spai_head_backward");
   // u = h3 Wu
   weight_grad<kH>(m, n, m->h[kL].p, nullptr, m->du.p, nullptr, oWu, -1);
