@@ -1022,6 +1022,10 @@ static void build_brick_plan(afem_bsr* b)
     perm.swap(perm2);
   }
   const int32_t nblk = (n + kLocalRows - 1) / kLocalRows;
+  // inside a brick the rows go in increasing order, so that the row segments a
+  // wave writes at the end lie in runs of consecutive rows
+  for (int32_t k = 0; k < nblk; ++k)
+    std::sort(perm.begin() + (size_t)k * kLocalRows, perm.begin() + std::min<size_t>(n, (size_t)(k + 1) * kLocalRows));
   b->bperm.alloc(n);
   b->bperm.upload(perm.data(), perm.size());
   b->bval_off.alloc(n);
@@ -1462,6 +1466,7 @@ static const LaplaceVariant kLaplaceVariants[] = {
   { kSearch, 4, 256 },  { kPlan, 4, 256 },  { kPlan, 8, 256 },        { kCompact, 4, 256 },
   { kCompact, 8, 128 }, { kLocal, 4, 256 }, { kLocal, 8, 256 },       { kLocal, 16, 256 },
   { kLocalSliced, 4, 256 }, { kLocalSliced, 8, 256 }, { kBrick, 4, 256 }, { kBrick, 8, 256 },
+  { kLocalSliced, 16, 256 }, { kBrick, 16, 256 },
 };
 constexpr int kNumLaplaceVariants = sizeof(kLaplaceVariants) / sizeof(kLaplaceVariants[0]);
 constexpr int kDefaultLaplaceVariant = 6;
@@ -1541,30 +1546,34 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
   ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
   if (var.kind == kBrick) {
     if (m.npc == 4) {
-      if (var.u == 4)
-        launch_laplace_brick<4, 4>(b, cell_lambda, f, rhs);
-      else
-        launch_laplace_brick<4, 8>(b, cell_lambda, f, rhs);
+      switch (var.u) {
+      case 4: launch_laplace_brick<4, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_brick<4, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_brick<4, 8>(b, cell_lambda, f, rhs); break;
+      }
     }
     else {
-      if (var.u == 4)
-        launch_laplace_brick<3, 4>(b, cell_lambda, f, rhs);
-      else
-        launch_laplace_brick<3, 8>(b, cell_lambda, f, rhs);
+      switch (var.u) {
+      case 4: launch_laplace_brick<3, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_brick<3, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_brick<3, 8>(b, cell_lambda, f, rhs); break;
+      }
     }
   }
   else if (var.kind == kLocalSliced) {
     if (m.npc == 4) {
-      if (var.u == 4)
-        launch_laplace_lplan<4, 4, true>(b, cell_lambda, f, rhs);
-      else
-        launch_laplace_lplan<4, 8, true>(b, cell_lambda, f, rhs);
+      switch (var.u) {
+      case 4: launch_laplace_lplan<4, 4, true>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<4, 16, true>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<4, 8, true>(b, cell_lambda, f, rhs); break;
+      }
     }
     else {
-      if (var.u == 4)
-        launch_laplace_lplan<3, 4, true>(b, cell_lambda, f, rhs);
-      else
-        launch_laplace_lplan<3, 8, true>(b, cell_lambda, f, rhs);
+      switch (var.u) {
+      case 4: launch_laplace_lplan<3, 4, true>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<3, 16, true>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<3, 8, true>(b, cell_lambda, f, rhs); break;
+      }
     }
   }
   else if (var.kind == kLocal) {

@@ -83,7 +83,7 @@ def test_node_cells_and_pattern_bit_exact(afem_gpu, spec):
     assert info.nb_col == len(ocol) and info.max_row_length == np.diff(orow).max()
 
 
-N_LAPLACE_VARIANTS = 12
+N_LAPLACE_VARIANTS = 14
 
 
 @pytest.mark.parametrize("spec", MESHES, ids=[m[0] for m in MESHES])
