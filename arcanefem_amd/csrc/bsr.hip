@@ -637,6 +637,110 @@ __device__ __forceinline__ void lds_accumulate(double* a, double v)
 }
 __device__ __forceinline__ int lplan_pos(uint64_t w, int k) { return (int)((w >> (33 + 5 * k)) & 31); }
 
+// LDS coordinates of the other nodes of one local-plan word.
+template <int NPC>
+struct PairNodes
+{
+  double2 xy[NPC - 1];
+  double z[NPC - 1];
+};
+
+template <int NPC>
+__device__ __forceinline__ PairNodes<NPC> pair_nodes(const double2* __restrict__ s_xy, const double* __restrict__ s_z,
+                                                     uint64_t w)
+{
+  PairNodes<NPC> n;
+#pragma unroll
+  for (int k = 0; k < NPC - 1; ++k) {
+    const int l = lplan_idx(w, k);
+    n.xy[k] = s_xy[l];
+    n.z[k] = NPC == 4 ? s_z[l] : 0.0;
+  }
+  return n;
+}
+
+// Row p of one element from a local-plan word: the diagonal entry and |det|
+// are summed in registers, the other entries added into the row's LDS slots
+// (slot k at rv[stride * k]).  Edges relative to the row node p (as in
+// k_assemble_laplace).
+template <int NPC>
+__device__ __forceinline__ void pair_element(const PairNodes<NPC>& n, d3 p, double lam, uint64_t w, double* rv,
+                                             int stride, double& diag, double& det_sum)
+{
+  const d3 ea = { n.xy[0].x - p.x, n.xy[0].y - p.y, n.z[0] - p.z };
+  const d3 eb = { n.xy[1].x - p.x, n.xy[1].y - p.y, n.z[1] - p.z };
+  if constexpr (NPC == 4) {
+    const d3 ec = { n.xy[2].x - p.x, n.xy[2].y - p.y, n.z[2] - p.z };
+    const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
+    const double adet = fabs(dot(ea, Ca));
+    const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
+    const double scale = lam * fast_rcp(6.0 * adet);
+    diag += scale * dot(Cp, Cp);
+    lds_accumulate(rv + stride * lplan_pos(w, 0), scale * dot(Cp, Ca));
+    lds_accumulate(rv + stride * lplan_pos(w, 1), scale * dot(Cp, Cb));
+    lds_accumulate(rv + stride * lplan_pos(w, 2), scale * dot(Cp, Cc));
+    det_sum += adet;
+  }
+  else {
+    const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
+    const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
+    const double gpx = -(gax + gbx), gpy = -(gay + gby);
+    const double scale = lam * fast_rcp(2.0 * adet);
+    diag += scale * (gpx * gpx + gpy * gpy);
+    lds_accumulate(rv + stride * lplan_pos(w, 0), scale * (gpx * gax + gpy * gay));
+    lds_accumulate(rv + stride * lplan_pos(w, 1), scale * (gpx * gbx + gpy * gby));
+    det_sum += adet;
+  }
+}
+
+// All the (row, cell) pairs of one row from its local-plan words, word j at
+// plan[wbase + j * wstride] (cell id at cell_of[...]).  Full batches of U
+// words run without branches, the coordinates of pair u + 1 read from LDS
+// before pair u is computed; the last partial batch stops at ncell.
+template <int NPC, int U, bool CELL_SHIFT>
+__device__ __forceinline__ void row_pairs(const uint64_t* __restrict__ plan, const int32_t* __restrict__ cell_of,
+                                          int64_t wbase, int wstride, int32_t ncell,
+                                          const double* __restrict__ cell_lambda, const double2* __restrict__ s_xy,
+                                          const double* __restrict__ s_z, d3 p, double* rv, int stride, double& diag,
+                                          double& det_sum)
+{
+  int32_t e = 0;
+  for (; e + U <= ncell; e += U) {
+    uint64_t w[U];
+    double lam[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t at = wbase + (int64_t)(e + u) * wstride;
+      w[u] = plan[at];
+      lam[u] = cell_lambda ? cell_lambda[CELL_SHIFT ? (cell_of[at] >> 2) : cell_of[at]] : 1.0;
+    }
+    PairNodes<NPC> next = pair_nodes<NPC>(s_xy, s_z, w[0]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const PairNodes<NPC> cur = next;
+      if (u + 1 < U)
+        next = pair_nodes<NPC>(s_xy, s_z, w[u + 1]);
+      pair_element<NPC>(cur, p, lam[u], w[u], rv, stride, diag, det_sum);
+    }
+  }
+  if (e < ncell) {
+    uint64_t w[U];
+    double lam[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t at = wbase + (int64_t)min(e + u, ncell - 1) * wstride;
+      w[u] = plan[at];
+      lam[u] = cell_lambda ? cell_lambda[CELL_SHIFT ? (cell_of[at] >> 2) : cell_of[at]] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (e + u >= ncell)
+        break;
+      pair_element<NPC>(pair_nodes<NPC>(s_xy, s_z, w[u]), p, lam[u], w[u], rv, stride, diag, det_sum);
+    }
+  }
+}
+
 // Sorted unique columns of each block of rows (bitonic sort in LDS, then a
 // compaction with a block-wide scan).  count < 0 marks a block whose rows
 // hold more than kLocalSort entries.
@@ -1171,52 +1275,8 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
     const int64_t wbase = SLICED ? slice[blk] + threadIdx.x : (int64_t)e0;
     const int wstride = SLICED ? kLocalRows : 1;
     double diag = 0.0, det_sum = 0.0;
-    for (int32_t e = 0; e < ncell; e += U) {
-      uint64_t w[U];
-      double lam[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t at = wbase + (int64_t)min(e + u, ncell - 1) * wstride;
-        w[u] = lplan[at];
-        if (cell_lambda)
-          lam[u] = cell_lambda[SLICED ? nc_items[at] : (nc_items[at] >> 2)];
-        else
-          lam[u] = 1.0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (e + u >= ncell)
-          break;
-        const int la = lplan_idx(w[u], 0), lb = lplan_idx(w[u], 1);
-        const double2 xa = s_xy[la], xb = s_xy[lb];
-        const d3 ea = { xa.x - p.x, xa.y - p.y, s_z[la] - p.z };
-        const d3 eb = { xb.x - p.x, xb.y - p.y, s_z[lb] - p.z };
-        if constexpr (NPC == 4) {
-          const int lc = lplan_idx(w[u], 2);
-          const double2 xc = s_xy[lc];
-          const d3 ec = { xc.x - p.x, xc.y - p.y, s_z[lc] - p.z };
-          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
-          const double adet = fabs(dot(ea, Ca));
-          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
-          const double scale = lam[u] * fast_rcp(6.0 * adet);
-          diag += scale * dot(Cp, Cp);
-          lds_accumulate(rv + lplan_pos(w[u], 0), scale * dot(Cp, Ca));
-          lds_accumulate(rv + lplan_pos(w[u], 1), scale * dot(Cp, Cb));
-          lds_accumulate(rv + lplan_pos(w[u], 2), scale * dot(Cp, Cc));
-          det_sum += adet;
-        }
-        else {
-          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
-          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
-          const double gpx = -(gax + gbx), gpy = -(gay + gby);
-          const double scale = lam[u] * fast_rcp(2.0 * adet);
-          diag += scale * (gpx * gpx + gpy * gpy);
-          lds_accumulate(rv + lplan_pos(w[u], 0), scale * (gpx * gax + gpy * gay));
-          lds_accumulate(rv + lplan_pos(w[u], 1), scale * (gpx * gbx + gpy * gby));
-          det_sum += adet;
-        }
-      }
-    }
+    row_pairs<NPC, U, !SLICED>(lplan, nc_items, wbase, wstride, ncell, cell_lambda, s_xy, s_z, p, rv, 1, diag,
+                               det_sum);
     rv[diag_pos[i]] += diag;
     if (rhs)
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
@@ -1268,49 +1328,8 @@ k_assemble_laplace_brick(const double* __restrict__ coords, const int32_t* __res
     const int32_t ncell = nc_off[i + 1] - nc_off[i];
     const int64_t wbase = slice[blk] + threadIdx.x;
     double diag = 0.0, det_sum = 0.0;
-    for (int32_t e = 0; e < ncell; e += U) {
-      uint64_t w[U];
-      double lam[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t at = wbase + (int64_t)min(e + u, ncell - 1) * kLocalRows;
-        w[u] = bplan[at];
-        lam[u] = cell_lambda ? cell_lambda[bcell[at]] : 1.0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (e + u >= ncell)
-          break;
-        const int la = lplan_idx(w[u], 0), lb = lplan_idx(w[u], 1);
-        const double2 xa = s_xy[la], xb = s_xy[lb];
-        const d3 ea = { xa.x - p.x, xa.y - p.y, s_z[la] - p.z };
-        const d3 eb = { xb.x - p.x, xb.y - p.y, s_z[lb] - p.z };
-        if constexpr (NPC == 4) {
-          const int lc = lplan_idx(w[u], 2);
-          const double2 xc = s_xy[lc];
-          const d3 ec = { xc.x - p.x, xc.y - p.y, s_z[lc] - p.z };
-          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
-          const double adet = fabs(dot(ea, Ca));
-          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
-          const double scale = lam[u] * fast_rcp(6.0 * adet);
-          diag += scale * dot(Cp, Cp);
-          lds_accumulate(rv + 64 * lplan_pos(w[u], 0), scale * dot(Cp, Ca));
-          lds_accumulate(rv + 64 * lplan_pos(w[u], 1), scale * dot(Cp, Cb));
-          lds_accumulate(rv + 64 * lplan_pos(w[u], 2), scale * dot(Cp, Cc));
-          det_sum += adet;
-        }
-        else {
-          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
-          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
-          const double gpx = -(gax + gbx), gpy = -(gay + gby);
-          const double scale = lam[u] * fast_rcp(2.0 * adet);
-          diag += scale * (gpx * gpx + gpy * gpy);
-          lds_accumulate(rv + 64 * lplan_pos(w[u], 0), scale * (gpx * gax + gpy * gay));
-          lds_accumulate(rv + 64 * lplan_pos(w[u], 1), scale * (gpx * gbx + gpy * gby));
-          det_sum += adet;
-        }
-      }
-    }
+    row_pairs<NPC, U, false>(bplan, bcell, wbase, kLocalRows, ncell, cell_lambda, s_xy, s_z, p, rv, 64, diag,
+                             det_sum);
     rv[64 * diag_pos[i]] += diag;
     if (rhs)
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
