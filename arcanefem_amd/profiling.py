@@ -141,6 +141,7 @@ def main():
     s.add_argument("--csv", required=True)
     s.add_argument("--command", required=True)
     s.add_argument("--out", required=True)
+    s.add_argument("--top", type=int, default=60)
     p = sub.add_parser("pmc")
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
@@ -151,7 +152,7 @@ def main():
     a = ap.parse_args()
     if a.cmd == "stats":
         with open(a.out, "w") as f:
-            f.write(stats_markdown(a.csv, a.command))
+            f.write(stats_markdown(a.csv, a.command, a.top))
     else:
         summ = pmc_summary(a.fetch, a.write, a.copy_bytes)
         summ["workload"] = a.workload
