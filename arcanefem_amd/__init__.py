@@ -116,6 +116,7 @@ def _declare(L):
         "afem_system_spmv": (ci, [vp, vp, vp]),
         "afem_system_spmv_fp32": (ci, [vp, vp, vp]),
         "afem_system_spmv_format": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+        "afem_system_spmv_tuning": (ci, [vp] + [ctypes.POINTER(ci)] * 4),
         "afem_system_set_spai": (ci, [vp, vp]),
         "afem_rff_parameters": (ci, [ci, ci, dbl, ctypes.c_uint64, vp, vp]),
         "afem_lognormal_field": (ci, [vp, dbl, dbl, ci, ctypes.c_uint64, vp]),
@@ -427,6 +428,13 @@ class DoFLinearSystem:
         if f.value < 0:
             return None
         return {1: ("sell", 64), 2: ("sell16", 64), 3: ("sell16x8", 64)}.get(f.value, ("csr", lanes.value))
+
+    def spmv_tuning(self):
+        """((slots, workgroup) of the FP64 SpMV, (slots, workgroup) of the FP32 SpMV) for the
+        16-bit-offset SELL formats; zeros otherwise."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().afem_system_spmv_tuning(self.h, *[ctypes.byref(c) for c in v]))
+        return (v[0].value, v[1].value), (v[2].value, v[3].value)
 
     def __del__(self):
         try:

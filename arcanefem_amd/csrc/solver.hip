@@ -59,12 +59,15 @@ struct Parts
 
 // ---- block reductions --------------------------------------------------------------------
 
-// Sum of n partials, identical in every block; all threads get the result.
+// Sum of n partials, identical in every block of BS threads; all threads get
+// the result.  The order of the additions depends on BS only through the
+// final per-wave sums, which are added in a fixed order.
+template <int BS = kBlock>
 __device__ __forceinline__ double sum_parts(const double* __restrict__ p, int n)
 {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[BS / kWave];
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock)
+  for (int i = threadIdx.x; i < n; i += BS)
     s += p[i];
   s = wave_reduce_sum(s);
   if ((threadIdx.x & (kWave - 1)) == 0)
@@ -72,16 +75,17 @@ __device__ __forceinline__ double sum_parts(const double* __restrict__ p, int n)
   __syncthreads();
   double t = 0.0;
 #pragma unroll
-  for (int w = 0; w < kBlock / kWave; ++w)
+  for (int w = 0; w < BS / kWave; ++w)
     t += lds[w];
   __syncthreads();
   return t;
 }
 
-// Writes the block sum of v to out[blockIdx.x].
+// Writes the sum of v over the BS threads of the block to out[blockIdx.x].
+template <int BS = kBlock>
 __device__ __forceinline__ void put_part(double v, double* __restrict__ out)
 {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[BS / kWave];
   v = wave_reduce_sum(v);
   if ((threadIdx.x & (kWave - 1)) == 0)
     lds[threadIdx.x / kWave] = v;
@@ -89,7 +93,7 @@ __device__ __forceinline__ void put_part(double v, double* __restrict__ out)
   if (threadIdx.x == 0) {
     double t = 0.0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; ++w)
+    for (int w = 0; w < BS / kWave; ++w)
       t += lds[w];
     out[blockIdx.x] = t;
   }
@@ -126,7 +130,9 @@ struct Gate
   int k;
   const double* rr_parts;
   int nparts;
-  // evaluated by every thread of a block (uses __syncthreads through sum_parts)
+  // evaluated by every thread of a block of BS threads (uses __syncthreads
+  // through sum_parts)
+  template <int BS = kBlock>
   __device__ bool open() const
   {
     if (mode == kAlways)
@@ -134,7 +140,7 @@ struct Gate
     if (st->done || st->breakdown)
       return false;
     if (mode == kReliable)
-      return reliable_decision(st, sum_parts(rr_parts, nparts), k);
+      return reliable_decision(st, sum_parts<BS>(rr_parts, nparts), k);
     return true;
   }
 };
@@ -250,21 +256,23 @@ k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restric
 // SELL-64 with 16-bit column offsets: slot (j, lane) of slice s stores
 // col - row at (off[s] + j) * 64 + lane (padding slots: 0 with a zero value).
 // The value and offset streams are read once, so they are loaded non-temporal
-// to leave the L2 to the gathered vector.
-template <class VT, class XT, int MODE, int UNR = 4>
-__global__ void __launch_bounds__(kBlock)
+// to leave the L2 to the gathered vector.  BS threads per workgroup (256 or
+// 512: the grid is capped at kMaxBlocks workgroups, so 512 doubles the waves
+// in flight without more reduction partials).
+template <class VT, class XT, int MODE, int UNR = 4, int BS = kBlock>
+__global__ void __launch_bounds__(BS)
 k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
               const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
               Gate gate, double* __restrict__ part_out)
 {
-  if (!gate.open())
+  if (!gate.open<BS>())
     return;
   using A = typename Acc<VT>::type;
   int64_t t0, t1;
-  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  chunk_of(((int64_t)n + BS - 1) / BS, t0, t1);
   double part = 0.0;
   for (int64_t t = t0; t < t1; ++t) {
-    const int64_t row = t * kBlock + threadIdx.x;
+    const int64_t row = t * BS + threadIdx.x;
     if (row < n) {
       const int64_t s = row >> 6;
       const int64_t base = (int64_t)off[s] * 64 + (row & 63);
@@ -319,7 +327,7 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
     }
   }
   if constexpr (MODE != kPlain)
-    put_part(part, part_out);
+    put_part<BS>(part, part_out);
 }
 
 // BSR 3x3 blocks stored per block (9 contiguous values), G lanes per node row.
@@ -831,7 +839,7 @@ __device__ __forceinline__ void prec_row(int64_t row, float zrow, const PrecIn& 
   }
 }
 
-template <int INIT>
+template <int INIT, int BS = kBlock>
 __device__ __forceinline__ bool prec_open(const State* st, const Parts& P, int nparts, int k, PrecIn& in)
 {
   if constexpr (INIT) {
@@ -841,21 +849,21 @@ __device__ __forceinline__ bool prec_open(const State* st, const Parts& P, int n
   else {
     if (st->done || st->breakdown)
       return false;
-    in.from64 = reliable_decision(st, sum_parts(P.rr, nparts), k);
+    in.from64 = reliable_decision(st, sum_parts<BS>(P.rr, nparts), k);
     return true;
   }
 }
 
-template <int INIT>
+template <int INIT, int BS = kBlock>
 __device__ __forceinline__ void prec_parts(const Parts& P, int k, double rz, double zq, double rr, double bb)
 {
-  put_part(rz, P.rz[k & 1]);
+  put_part<BS>(rz, P.rz[k & 1]);
   if constexpr (INIT) {
-    put_part(rr, P.rr);
-    put_part(bb, P.bb);
+    put_part<BS>(rr, P.rr);
+    put_part<BS>(bb, P.bb);
   }
   else {
-    put_part(zq, P.zq);
+    put_part<BS>(zq, P.zq);
   }
 }
 
@@ -923,19 +931,27 @@ k_prec_sell(int k, int32_t n, const int32_t* __restrict__ off, const int32_t* __
   prec_parts<INIT>(P, k, rz, zq, rr, bb);
 }
 
-// Preconditioner values as FP32 or BF16 (the high half of an FP32).
-__device__ __forceinline__ float mval_at(const float* __restrict__ m, int64_t i)
+// Slot i of the preconditioner in SELL order: its column offset and value.
+// FP32 values come with the separate 16-bit offset stream; BF16 values are
+// packed with the offset in one 32-bit word (value in the high half, offset
+// in the low half), so a slot costs one load instead of two.
+__device__ __forceinline__ void prec_slot(const int16_t* __restrict__ dcol, const float* __restrict__ m, int64_t i,
+                                          int& d, float& v)
 {
-  return __builtin_nontemporal_load(m + i);
+  d = __builtin_nontemporal_load(dcol + i);
+  v = __builtin_nontemporal_load(m + i);
 }
-__device__ __forceinline__ float mval_at(const uint16_t* __restrict__ m, int64_t i)
+__device__ __forceinline__ void prec_slot(const int16_t* __restrict__, const uint32_t* __restrict__ m, int64_t i,
+                                          int& d, float& v)
 {
-  return __uint_as_float((uint32_t)__builtin_nontemporal_load(m + i) << 16);
+  const uint32_t w = __builtin_nontemporal_load(m + i);
+  d = (int)(int16_t)(w & 0xffffu);
+  v = __uint_as_float(w & 0xffff0000u);
 }
 
 // z = M r_in over the rows of this workgroup with the input fixed at compile
 // time (FROM64: the FP64 residual of a reliable update, converted).
-template <int INIT, class MT, bool FROM64>
+template <int INIT, class MT, bool FROM64, int BS>
 __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __restrict__ off,
                                                  const int16_t* __restrict__ dcol, const MT* __restrict__ mv,
                                                  const PrecIn& in, float* __restrict__ r32, float* __restrict__ z,
@@ -945,9 +961,9 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
 {
   auto xin = [&](int64_t c) -> float { return FROM64 ? (float)in.r64[c] : in.r32[c]; };
   int64_t t0, t1;
-  chunk_of(((int64_t)n + kBlock - 1) / kBlock, t0, t1);
+  chunk_of(((int64_t)n + BS - 1) / BS, t0, t1);
   for (int64_t t = t0; t < t1; ++t) {
-    const int64_t row = t * kBlock + threadIdx.x;
+    const int64_t row = t * BS + threadIdx.x;
     if (row >= n)
       continue;
     const int64_t s = row >> 6;
@@ -960,10 +976,8 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
       int d[8];
       float m[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
-        m[u] = mval_at(mv, i0 + 64 * u);
-      }
+      for (int u = 0; u < 8; ++u)
+        prec_slot(dcol, mv, i0 + 64 * u, d[u], m[u]);
       s0 += m[0] * xin(row + d[0]) + m[4] * xin(row + d[4]);
       s1 += m[1] * xin(row + d[1]) + m[5] * xin(row + d[5]);
       s2 += m[2] * xin(row + d[2]) + m[6] * xin(row + d[6]);
@@ -974,10 +988,8 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
       int d[4];
       float m[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
-        m[u] = mval_at(mv, i0 + 64 * u);
-      }
+      for (int u = 0; u < 4; ++u)
+        prec_slot(dcol, mv, i0 + 64 * u, d[u], m[u]);
       s0 += m[0] * xin(row + d[0]);
       s1 += m[1] * xin(row + d[1]);
       s2 += m[2] * xin(row + d[2]);
@@ -990,10 +1002,8 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
       float m[3] = { 0.f, 0.f, 0.f };
 #pragma unroll
       for (int u = 0; u < 3; ++u)
-        if (u < rem) {
-          d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
-          m[u] = mval_at(mv, i0 + 64 * u);
-        }
+        if (u < rem)
+          prec_slot(dcol, mv, i0 + 64 * u, d[u], m[u]);
       s0 += m[0] * xin(row + d[0]);
       s1 += m[1] * xin(row + d[1]);
       s2 += m[2] * xin(row + d[2]);
@@ -1016,31 +1026,32 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
   }
 }
 
-template <int INIT, class MT>
-__global__ void __launch_bounds__(kBlock)
+template <int INIT, class MT, int BS = kBlock>
+__global__ void __launch_bounds__(BS)
 k_prec_sell16(int k, int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
               const MT* __restrict__ mv, PrecIn in, float* __restrict__ r32, float* __restrict__ z,
               float* __restrict__ p, const float* __restrict__ q, const double* __restrict__ b, Parts P, int nparts,
               const State* st)
 {
-  if (!prec_open<INIT>(st, P, nparts, k, in))
+  if (!prec_open<INIT, BS>(st, P, nparts, k, in))
     return;
   double rz = 0.0, zq = 0.0, rr = 0.0, bb = 0.0;
   if (in.from64)
-    prec_sell16_rows<INIT, MT, true>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
+    prec_sell16_rows<INIT, MT, true, BS>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
   else if constexpr (!INIT)
-    prec_sell16_rows<INIT, MT, false>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
-  prec_parts<INIT>(P, k, rz, zq, rr, bb);
+    prec_sell16_rows<INIT, MT, false, BS>(n, off, dcol, mv, in, r32, z, p, q, b, rz, zq, rr, bb);
+  prec_parts<INIT, BS>(P, k, rz, zq, rr, bb);
 }
 
-__global__ void k_to_bf16(int64_t n, const float* __restrict__ in, uint16_t* __restrict__ out)
+// BF16 (round to nearest even) of the FP32 preconditioner values, packed
+// with the 16-bit column offsets: word = bf16 << 16 | (uint16_t)offset.
+__global__ void k_pack_bf16(int64_t n, const float* __restrict__ in, const int16_t* __restrict__ dcol,
+                            uint32_t* __restrict__ out)
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t u = __float_as_uint(in[i]);
-    if ((u & 0x7fffffffu) > 0x7f800000u)
-      out[i] = 0x7fc0;
-    else
-      out[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    const uint32_t u = __float_as_uint(in[i]);
+    const uint32_t h = (u & 0x7fffffffu) > 0x7f800000u ? 0x7fc0u : (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    out[i] = (h << 16) | (uint32_t)(uint16_t)dcol[i];
   }
 }
 
@@ -1197,9 +1208,14 @@ struct afem_solver_ws
   bool pattern_valid = false;
   int format = -1; // chosen SpMV format
   int lanes = 16;
+  // SELL16 kernels: workgroup size of the FP64 SpMV, and slots per step and
+  // workgroup size of the FP32 SpMV and the preconditioner (unr32 = 0: not
+  // tuned, as the FP64 choice)
+  int bs64 = 256, unr32 = 0, bs32 = 256;
+  bool forced_format = false;
   // sparse preconditioner matrix (pattern of A, FP32), CSR and SELL order
   afem::DBuf<float> spai32, spai_sell32;
-  afem::DBuf<uint16_t> spai_sell_bf16;
+  afem::DBuf<uint32_t> spai_sell_pk; // BF16 values packed with the 16-bit column offsets
   bool has_spai = false;
   bool spai_sell_valid = false;
   bool spai_bf16_valid = false;
@@ -1262,6 +1278,8 @@ static afem_solver_ws* workspace(afem_system* s)
     w->values_valid = false;
     w->pattern_valid = false;
     w->format = -1;
+    w->bs64 = 256;
+    w->unr32 = 0;
     w->has_spai = false;
     w->spai_sell_valid = false;
   }
@@ -1278,6 +1296,8 @@ static afem_solver_ws* workspace(afem_system* s)
     w->pattern_valid = false;
     w->values_valid = false;
     w->format = -1;
+    w->bs64 = 256;
+    w->unr32 = 0;
     w->has_spai = false;
     w->spai_sell_valid = false;
   }
@@ -1417,13 +1437,23 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
     hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof, w->sell_off.p,
                        w->sell_col.p, sell_vals, x, y, b, gate, part_out);
   }
-  else if (w->format == kFmtSell16) {
-    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, 4>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
-                       w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
-  }
-  else if (w->format == kFmtSell16x8) {
-    hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, 8>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
-                       w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+  else if (w->format == kFmtSell16 || w->format == kFmtSell16x8) {
+    const int fmt_unr = w->format == kFmtSell16x8 ? 8 : 4;
+    const int unr = (sizeof(VT) == 4 && w->unr32) ? w->unr32 : fmt_unr;
+    const int bs = (sizeof(VT) == 4 && w->unr32) ? w->bs32 : w->bs64;
+    auto go = [&](auto utag, auto btag) {
+      constexpr int U = decltype(utag)::value, BS = decltype(btag)::value;
+      hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, U, BS>), dim3(nblk), dim3(BS), 0, stream(), s->nb_dof,
+                         w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    using B256 = std::integral_constant<int, 256>;
+    using B512 = std::integral_constant<int, 512>;
+    if (unr == 8)
+      bs == 512 ? go(I8{}, B512{}) : go(I8{}, B256{});
+    else
+      bs == 512 ? go(I4{}, B512{}) : go(I4{}, B256{});
   }
   else {
     auto go = [&](auto gtag) {
@@ -1451,14 +1481,18 @@ static int spmv_blocks(afem_system* s)
 // many lanes per row, -1 forces SELL-64.
 static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
 {
+  if (forced != 0)
+    w->forced_format = true; // until the next autotuning
   if (forced > 0) {
     w->format = kFmtCsr;
     w->lanes = forced;
     return;
   }
   ensure_sell(s, w);
-  if ((forced == -2 || forced == -3) && w->has_dcol) {
-    w->format = forced == -2 ? kFmtSell16 : kFmtSell16x8;
+  if (forced <= -2 && forced >= -5 && w->has_dcol) {
+    w->format = (forced == -2 || forced == -4) ? kFmtSell16 : kFmtSell16x8;
+    w->bs64 = forced <= -4 ? 512 : 256;
+    w->unr32 = 0;
     return;
   }
   if (forced < 0) {
@@ -1469,16 +1503,18 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     return;
   struct Cand
   {
-    int format, lanes;
+    int format, lanes, bs;
   };
   std::vector<Cand> cands;
   if (s->block == 3)
-    cands = { { kFmtCsr, 16 }, { kFmtSell, 0 } };
+    cands = { { kFmtCsr, 16, 256 }, { kFmtSell, 0, 256 } };
   else
-    cands = { { kFmtCsr, 8 }, { kFmtCsr, 16 }, { kFmtSell, 0 } };
+    cands = { { kFmtCsr, 8, 256 }, { kFmtCsr, 16, 256 }, { kFmtSell, 0, 256 } };
   if (s->block == 1 && w->has_dcol) {
-    cands.push_back({ kFmtSell16, 0 });
-    cands.push_back({ kFmtSell16x8, 0 });
+    for (int bs : { 256, 512 }) {
+      cands.push_back({ kFmtSell16, 0, bs });
+      cands.push_back({ kFmtSell16x8, 0, bs });
+    }
   }
   hipEvent_t e0, e1;
   AFEM_HIP(hipEventCreate(&e0));
@@ -1486,9 +1522,12 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
   float best = 1e30f;
   Cand pick = cands.back();
   const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
+  w->unr32 = 0;
+  w->forced_format = false;
   for (const Cand& c : cands) {
     w->format = c.format;
     w->lanes = c.lanes;
+    w->bs64 = c.bs;
     float t_best = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {
       AFEM_HIP(hipEventRecord(e0, stream()));
@@ -1510,6 +1549,49 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
   (void)hipEventDestroy(e1);
   w->format = pick.format;
   w->lanes = pick.lanes;
+  w->bs64 = pick.bs;
+}
+
+// The FP32 SELL16 SpMV (mixed solvers, preconditioner) is tuned on its own:
+// slots per step and workgroup size, timed like choose_format.  Needs the FP32
+// values; after a forced format (the last choose_format call) the FP32 kernels
+// keep the FP64 shape.
+static void choose_fp32(afem_system* s, afem_solver_ws* w)
+{
+  if (w->forced_format || w->unr32 || (w->format != kFmtSell16 && w->format != kFmtSell16x8) || !w->sell_v32.p)
+    return;
+  hipEvent_t e0, e1;
+  AFEM_HIP(hipEventCreate(&e0));
+  AFEM_HIP(hipEventCreate(&e1));
+  const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
+  float best = 1e30f;
+  int pick_u = 4, pick_b = 256;
+  for (int u : { 4, 8 })
+    for (int bs : { 256, 512 }) {
+      w->unr32 = u;
+      w->bs32 = bs;
+      float t_best = 1e30f;
+      for (int rep = 0; rep < 4; ++rep) {
+        AFEM_HIP(hipEventRecord(e0, stream()));
+        spmv_launch<float, float, kPlain>(s, w, true, w->p32.p, w->q32.p, nullptr, always, nullptr, spmv_blocks(s),
+                                          "spmv_tune_fp32");
+        AFEM_HIP(hipEventRecord(e1, stream()));
+        AFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        AFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0 && ms < t_best)
+          t_best = ms;
+      }
+      if (t_best < best) {
+        best = t_best;
+        pick_u = u;
+        pick_b = bs;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  w->unr32 = pick_u;
+  w->bs32 = pick_b;
 }
 
 // ---- solve ----------------------------------------------------------------------------------
@@ -1603,9 +1685,10 @@ static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
     w->spai_bf16_valid = false;
   }
   if (bf16 && !w->spai_bf16_valid) {
-    w->spai_sell_bf16.alloc(w->spai_sell32.n);
-    hipLaunchKernelGGL(k_to_bf16, dim3(2048), dim3(256), 0, stream(), (int64_t)w->spai_sell32.n, w->spai_sell32.p,
-                       w->spai_sell_bf16.p);
+    AFEM_REQUIRE(w->has_dcol, AFEM_ERR_STATE, "BF16 preconditioner values need the 16-bit offset layout");
+    w->spai_sell_pk.alloc(w->spai_sell32.n);
+    hipLaunchKernelGGL(k_pack_bf16, dim3(2048), dim3(256), 0, stream(), (int64_t)w->spai_sell32.n, w->spai_sell32.p,
+                       w->sell_dcol.p, w->spai_sell_pk.p);
     check_launch("spai_to_bf16");
     w->spai_bf16_valid = true;
   }
@@ -1617,14 +1700,22 @@ static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P
   ProfScope prof("fpcg_precond_spmv");
   const PrecIn in{ w->r32.p, w->r64.p, true };
   if (w->format == kFmtSell16 || w->format == kFmtSell16x8) {
-    if (bf16)
-      hipLaunchKernelGGL((k_prec_sell16<INIT, uint16_t>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof,
-                         w->sell_off.p, w->sell_dcol.p, w->spai_sell_bf16.p, in, w->r32.p, w->z32.p, w->p32.p,
-                         w->q32.p, s->rhs.p, P, nblk, w->st.p);
+    const int bs = w->unr32 ? w->bs32 : w->bs64;
+    auto go = [&](auto btag) {
+      constexpr int BS = decltype(btag)::value;
+      if (bf16)
+        hipLaunchKernelGGL((k_prec_sell16<INIT, uint32_t, BS>), dim3(nblk), dim3(BS), 0, stream(), k, s->nb_dof,
+                           w->sell_off.p, w->sell_dcol.p, w->spai_sell_pk.p, in, w->r32.p, w->z32.p, w->p32.p,
+                           w->q32.p, s->rhs.p, P, nblk, w->st.p);
+      else
+        hipLaunchKernelGGL((k_prec_sell16<INIT, float, BS>), dim3(nblk), dim3(BS), 0, stream(), k, s->nb_dof,
+                           w->sell_off.p, w->sell_dcol.p, w->spai_sell32.p, in, w->r32.p, w->z32.p, w->p32.p,
+                           w->q32.p, s->rhs.p, P, nblk, w->st.p);
+    };
+    if (bs == 512)
+      go(std::integral_constant<int, 512>{});
     else
-      hipLaunchKernelGGL((k_prec_sell16<INIT, float>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof,
-                         w->sell_off.p, w->sell_dcol.p, w->spai_sell32.p, in, w->r32.p, w->z32.p, w->p32.p, w->q32.p,
-                         s->rhs.p, P, nblk, w->st.p);
+      go(std::integral_constant<int, 256>{});
   }
   else if (w->format == kFmtSell) {
     hipLaunchKernelGGL((k_prec_sell<INIT>), dim3(nblk), dim3(kBlock), 0, stream(), k, s->nb_dof, w->sell_off.p,
@@ -1712,6 +1803,8 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   ensure_derived(s, w, mixed);
   choose_format(s, w, o.spmv_lanes);
   ensure_derived(s, w, mixed);
+  if (mixed)
+    choose_fp32(s, w);
   // BF16 preconditioner values need a SELL layout; CSR keeps FP32
   const bool bf16_used = flex && o.precond_bf16 && (w->format == kFmtSell16 || w->format == kFmtSell16x8);
   if (flex)
@@ -1872,6 +1965,7 @@ AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y)
   ensure_derived(sys, w, true);
   choose_format(sys, w, 0);
   ensure_derived(sys, w, true);
+  choose_fp32(sys, w);
   const Gate always{ w->st.p, kAlways, 0, nullptr, 0 };
   spmv_launch<float, float, kPlain>(sys, w, true, x, y, nullptr, always, nullptr, spmv_blocks(sys), "spmv_fp32");
   AFEM_GUARD_END
@@ -1899,6 +1993,20 @@ AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes)
   afem_solver_ws* w = sys->ws;
   *format = w ? w->format : -1;
   *lanes = w ? w->lanes : 0;
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_system_spmv_tuning(afem_system_t sys, int* fp64_slots, int* fp64_block, int* fp32_slots,
+                                     int* fp32_block)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys && fp64_slots && fp64_block && fp32_slots && fp32_block, AFEM_ERR_ARGUMENT, "null argument");
+  afem_solver_ws* w = sys->ws;
+  const bool sell16 = w && (w->format == kFmtSell16 || w->format == kFmtSell16x8);
+  *fp64_slots = sell16 ? (w->format == kFmtSell16x8 ? 8 : 4) : 0;
+  *fp64_block = sell16 ? w->bs64 : 0;
+  *fp32_slots = sell16 ? (w->unr32 ? w->unr32 : *fp64_slots) : 0;
+  *fp32_block = sell16 ? (w->unr32 ? w->bs32 : w->bs64) : 0;
   AFEM_GUARD_END
 }
 

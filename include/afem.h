@@ -268,7 +268,8 @@ typedef struct afem_solver_options {
                             lanes per row (4, 8, 16, 32), -1 = SELL-64,
                             -2 = SELL-64 with 16-bit column offsets (scalar
                             matrices whose |col - row| fits; else SELL-64),
-                            -3 = the same with 8 slots per step */
+                            -3 = the same with 8 slots per step, -4 / -5 =
+                            -2 / -3 with 512-thread workgroups */
   int norm;              /* stopping norm: 0 = ||b - A x||_2 <= max(rtol ||b||_2, atol);
                             1 = natural norm sqrt(r.M^-1 r) <= rtol * its initial value;
                             2 = preconditioned norm ||M^-1 r||_2 <= max(rtol ||M^-1 b||_2,
@@ -313,6 +314,14 @@ AFEM_API int afem_system_set_spai(afem_system_t sys, const float* values);
  * 1 (SELL-64), 2 (SELL-64 with 16-bit column offsets) or 3 (the same, 8
  * slots per step); -1 while no SpMV or solve has run on the current matrix. */
 AFEM_API int afem_system_spmv_format(afem_system_t sys, int* format, int* lanes);
+
+/* Kernel shape of the SELL-64 16-bit-offset SpMV (formats 2 and 3): slots per
+ * step and workgroup size (256 or 512 threads) of the FP64 SpMV, and of the
+ * FP32 SpMV and preconditioner product, which the mixed solvers and
+ * afem_system_spmv_fp32 tune separately.  All zero for the other formats.
+ * No counterpart in the reference. */
+AFEM_API int afem_system_spmv_tuning(afem_system_t sys, int* fp64_slots, int* fp64_block, int* fp32_slots,
+                                     int* fp32_block);
 
 /* ---- heterogeneous coefficients (no counterpart in the reference) ------------------- */
 

@@ -295,10 +295,12 @@ def poisson_bottom_system(A, dim, n, alpha, lam=None):
     return X, C, m, b, s, dl
 
 
-SPMV_FORMATS = [0, -1, -2, -3, 8]  # autotuned, SELL-64, SELL-64 with 16-bit offsets (4 / 8 slots per step), CSR 8
+# autotuned, SELL-64, SELL-64 with 16-bit offsets (4 / 8 slots per step; 256- or
+# 512-thread workgroups), CSR 8
+SPMV_FORMATS = [0, -1, -2, -3, -4, -5, 8]
 
 
-@pytest.mark.parametrize("fmt", [-1, -2, -3, 8])
+@pytest.mark.parametrize("fmt", [-1, -2, -3, -4, -5, 8])
 def test_spmv_forced_formats(afem_gpu, fmt):
     """Each SpMV format against the oracle (the format is pinned by a one-iteration solve)."""
     A = afem_gpu
@@ -307,8 +309,12 @@ def test_spmv_forced_formats(afem_gpu, fmt):
     N = X.shape[0]
     m, b, s, _ = build_poisson(A, 3, X, C)
     s.solve(raise_on_failure=False, method=A.SOLVER_PCG_MIXED, max_iter=1, spmv_lanes=fmt)
-    want = {-1: ("sell", 64), -2: ("sell16", 64), -3: ("sell16x8", 64), 8: ("csr", 8)}[fmt]
+    want = {-1: ("sell", 64), -2: ("sell16", 64), -3: ("sell16x8", 64), -4: ("sell16", 64), -5: ("sell16x8", 64),
+            8: ("csr", 8)}[fmt]
     assert s.spmv_format() == want
+    if fmt <= -2:
+        slots, block = (4 if fmt in (-2, -4) else 8), (512 if fmt <= -4 else 256)
+        assert s.spmv_tuning() == ((slots, block), (slots, block))
     row, col, val, _ = b.download()
     x = np.random.default_rng(3).normal(size=N)
     dy = A.DeviceArray(N, np.float64)

@@ -175,7 +175,7 @@ def test_fpcg_matches_direct_solve(afem_gpu):
     assert np.linalg.norm(x - x_ref) <= 1e-7 * np.linalg.norm(x_ref)
 
 
-@pytest.mark.parametrize("fmt,bf16", [(-1, 0), (-2, 0), (-2, 1), (8, 1)])
+@pytest.mark.parametrize("fmt,bf16", [(-1, 0), (-2, 0), (-2, 1), (-5, 1), (8, 1)])
 def test_fpcg_formats_and_bf16(afem_gpu, fmt, bf16):
     """The preconditioner SpMV in every format, with FP32 or BF16 values
     (BF16 is used with the 16-bit-offset SELL format only)."""
