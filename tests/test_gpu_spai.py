@@ -195,6 +195,33 @@ def test_fpcg_formats_and_bf16(afem_gpu, fmt, bf16):
     assert np.linalg.norm(s.solution().numpy() - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
 
 
+def test_fpcg_formats_at_scale(afem_gpu):
+    """Flexible PCG on a grid of 1024 workgroups in SELL-64 and in the 16-bit
+    offset layout (256- and 512-thread A products): same convergence, no
+    fallback, bit-identical repeats."""
+    A = afem_gpu
+    print("[case] fpcg formats 3D n=64", flush=True)
+    case = build_system(A, 3, 64, seed=23)
+    s = case["sys"]
+    net = A.LearnedSpai(seed=4)
+    for k in range(40):
+        net.train_step(s, probe_seed=k, lr=3e-3)
+    net.apply(s)
+    its = {}
+    for fmt in (-1, -2, -5):
+        r = s.solve(method=A.SOLVER_FPCG_SPAI, rtol=1e-8, max_iter=5000, spmv_lanes=fmt, precond_bf16=0)
+        assert r.converged and r.relres <= 1e-8 and not r.fallback, (fmt, r.iterations, r.fallback)
+        its[fmt] = r.iterations
+        if fmt != -1:
+            # deterministic: a race on the reduction partials would show here
+            x1 = s.solution().numpy().copy()
+            r2 = s.solve(method=A.SOLVER_FPCG_SPAI, rtol=1e-8, max_iter=5000, spmv_lanes=fmt, precond_bf16=0)
+            assert r2.iterations == r.iterations and np.array_equal(s.solution().numpy(), x1), fmt
+    # FP32 products in another summation order move the count by a few percent
+    for fmt in (-2, -5):
+        assert abs(its[fmt] - its[-1]) <= 0.08 * its[-1], its
+
+
 def test_fallback_to_jacobi(afem_gpu):
     A = afem_gpu
     print("[case] fpcg fallback 2D n=16", flush=True)
