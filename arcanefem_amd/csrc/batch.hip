@@ -19,7 +19,11 @@
 // replayed; converged segments make their workgroups return at once.
 //
 // Tuning: lanes per row (G) and row tiles per workgroup (T) are timed on
-// the batch and the fastest pair is kept.
+// the batch and the fastest pair is kept; G = 1 selects a SELL-64 copy of
+// the matrix whose slices start at every segment's first row (16-bit column
+// offsets, one thread per row).  Between chunks the grid is cut down to the
+// workgroups of the segments still running (through a workgroup list), so
+// the tail of the longest solves does not pay for the finished ones.
 #include "common.h"
 #include "device_utils.h"
 #include "linsys.h"
@@ -60,23 +64,31 @@ struct Seg
   const int32_t* blocks; // nb_seg + 1 workgroup offsets
   const int32_t* owner;  // segment of each workgroup
   int rows_per_block;
+  const int32_t* list;   // launched workgroup -> workgroup of the layout (nullptr: the same)
+  // segment-aligned SELL-64: first slice of each segment, slot offset of each
+  // slice (nb_slice + 1), column offsets col - row
+  const int32_t* sslice;
+  const int32_t* soff;
+  const int16_t* sdcol;
 };
 
 struct Ctx
 {
-  int s;        // segment
-  int b0, b1;   // its workgroups
+  int s;          // segment
+  int gb;         // workgroup of the layout
+  int b0, b1;     // the segment's workgroups
   int32_t r0, r1; // rows of this workgroup
 };
 
 __device__ __forceinline__ Ctx context(const Seg& g)
 {
   Ctx c;
-  c.s = g.owner[blockIdx.x];
+  c.gb = g.list ? g.list[blockIdx.x] : (int)blockIdx.x;
+  c.s = g.owner[c.gb];
   c.b0 = g.blocks[c.s];
   c.b1 = g.blocks[c.s + 1];
   const int32_t sr0 = g.rows[c.s], sr1 = g.rows[c.s + 1];
-  c.r0 = sr0 + (int32_t)(blockIdx.x - c.b0) * g.rows_per_block;
+  c.r0 = sr0 + (int32_t)(c.gb - c.b0) * g.rows_per_block;
   c.r1 = min(sr1, c.r0 + g.rows_per_block);
   return c;
 }
@@ -100,7 +112,8 @@ __device__ __forceinline__ double seg_sum(const double* __restrict__ p, int b0, 
   return t;
 }
 
-__device__ __forceinline__ void put(double v, double* __restrict__ out)
+// out[gb] = the sum of v over the workgroup
+__device__ __forceinline__ void put(double v, double* __restrict__ out, int gb)
 {
   __shared__ double lds[kBT / kWave];
   v = wave_reduce_sum(v);
@@ -112,7 +125,7 @@ __device__ __forceinline__ void put(double v, double* __restrict__ out)
 #pragma unroll
     for (int w = 0; w < kBT / kWave; ++w)
       t += lds[w];
-    out[blockIdx.x] = t;
+    out[gb] = t;
   }
   __syncthreads();
 }
@@ -136,6 +149,70 @@ __device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const 
   return (double)group_reduce_sum<G>(sum);
 }
 
+// Row r of the segment-aligned SELL-64 copy (4 slots per step; the last 1-3
+// slots loaded together).
+template <class VT, class XT>
+__device__ __forceinline__ double sell_row(const Seg& g, const VT* __restrict__ v, const XT* __restrict__ x,
+                                           int32_t r, int32_t sr0, int32_t sl0)
+{
+  using A = typename std::conditional<sizeof(VT) == 4, float, double>::type;
+  const int32_t local = r - sr0;
+  const int32_t sl = sl0 + (local >> 6);
+  const int64_t base = (int64_t)g.soff[sl] * 64 + (local & 63);
+  const int w = g.soff[sl + 1] - g.soff[sl];
+  const XT* xr = x + r;
+  A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int j = 0;
+  for (; j + 3 < w; j += 4) {
+    const int64_t i0 = base + (int64_t)j * 64;
+    int d[4];
+    VT m[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      d[u] = __builtin_nontemporal_load(g.sdcol + i0 + 64 * u);
+      m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
+    }
+    s0 += (A)m[0] * (A)xr[d[0]];
+    s1 += (A)m[1] * (A)xr[d[1]];
+    s2 += (A)m[2] * (A)xr[d[2]];
+    s3 += (A)m[3] * (A)xr[d[3]];
+  }
+  if (j < w) {
+    const int64_t i0 = base + (int64_t)j * 64;
+    const int rem = w - j;
+    int d[3] = { 0, 0, 0 };
+    VT m[3] = { 0, 0, 0 };
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (u < rem) {
+        d[u] = __builtin_nontemporal_load(g.sdcol + i0 + 64 * u);
+        m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
+      }
+    s0 += (A)m[0] * (A)xr[d[0]];
+    s1 += (A)m[1] * (A)xr[d[1]];
+    s2 += (A)m[2] * (A)xr[d[2]];
+  }
+  return (double)((s0 + s1) + (s2 + s3));
+}
+
+// q = A32 p on the segment-aligned SELL-64 copy; partial p.q
+__global__ void __launch_bounds__(kBT)
+k_spmv_pq_sell(Seg g, const float* __restrict__ v, const float* __restrict__ p, float* __restrict__ q,
+               const SegState* __restrict__ st, BParts P)
+{
+  const Ctx c = context(g);
+  if (!running(st[c.s]))
+    return;
+  const int32_t sr0 = g.rows[c.s], sl0 = g.sslice[c.s];
+  double part = 0.0;
+  for (int32_t r = c.r0 + threadIdx.x; r < c.r1; r += kBT) {
+    const double y = sell_row(g, v, p, r, sr0, sl0);
+    q[r] = (float)y;
+    part += y * (double)p[r];
+  }
+  put(part, P.pq, c.gb);
+}
+
 // q = A32 p; partial p.q
 template <int G>
 __global__ void __launch_bounds__(kBT)
@@ -154,7 +231,7 @@ k_spmv_pq(Seg g, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
       part += y * (double)p[r];
     }
   }
-  put(part, P.pq);
+  put(part, P.pq, c.gb);
 }
 
 __device__ __forceinline__ double rho_prev(const SegState& st, const BParts& P, const Ctx& c, int par)
@@ -174,7 +251,7 @@ k_update(Seg g, double* __restrict__ x, float* __restrict__ r, float* __restrict
   const double pq = seg_sum(P.pq, c.b0, c.b1);
   const double rho = rho_prev(st[c.s], P, c, par);
   if (!(pq > 0.0) || !isfinite(pq)) {
-    if (blockIdx.x == c.b0 && threadIdx.x == 0)
+    if (c.gb == c.b0 && threadIdx.x == 0)
       st[c.s].breakdown = 1;
     return;
   }
@@ -190,8 +267,8 @@ k_update(Seg g, double* __restrict__ x, float* __restrict__ r, float* __restrict
     rz += (double)rv * (double)zv;
     rr += (double)rv * (double)rv;
   }
-  put(rz, P.rz[par]);
-  put(rr, P.rr);
+  put(rz, P.rz[par], c.gb);
+  put(rr, P.rr, c.gb);
 }
 
 // r64 = b - A64 x on the segments that take a reliable update this
@@ -224,7 +301,97 @@ k_residual(Seg g, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci
       part += rv * rv;
     }
   }
-  put(part, P.rr64);
+  put(part, P.rr64, c.gb);
+}
+
+// k_residual on the segment-aligned SELL-64 copy (FP64 values)
+__global__ void __launch_bounds__(kBT)
+k_residual_sell(Seg g, const double* __restrict__ v, const double* __restrict__ x, const double* __restrict__ b,
+                double* __restrict__ r64, const SegState* __restrict__ st, BParts P, int par, int always)
+{
+  const Ctx c = context(g);
+  const SegState& S = st[c.s];
+  if (always) {
+    if (!S.active)
+      return;
+  }
+  else {
+    if (!running(S))
+      return;
+    if (!reliable(S, seg_sum(P.rr, c.b0, c.b1), S.iter_par[par]))
+      return;
+  }
+  const int32_t sr0 = g.rows[c.s], sl0 = g.sslice[c.s];
+  double part = 0.0;
+  for (int32_t r = c.r0 + threadIdx.x; r < c.r1; r += kBT) {
+    const double rv = b[r] - sell_row(g, v, x, r, sr0, sl0);
+    r64[r] = rv;
+    part += rv * rv;
+  }
+  put(part, P.rr64, c.gb);
+}
+
+// Slot count (longest row) of each slice of the segment-aligned SELL copy.
+__global__ void k_sell_width(int32_t nslice, const int32_t* __restrict__ slice_row, const int32_t* __restrict__ rp,
+                             int32_t* __restrict__ width)
+{
+  const int32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslice)
+    return;
+  int32_t w = 0;
+  for (int32_t r = slice_row[sl]; r < slice_row[sl + 1]; ++r)
+    w = max(w, rp[r + 1] - rp[r]);
+  width[sl] = w;
+}
+
+// Column offsets of the SELL copy (one thread per slice lane); *overflow = 1
+// when an offset does not fit in 16 bits.  Padding slots: offset 0.
+__global__ void k_sell_columns(int32_t nslice, const int32_t* __restrict__ slice_row,
+                               const int32_t* __restrict__ soff, const int32_t* __restrict__ rp,
+                               const int32_t* __restrict__ ci, int16_t* __restrict__ dcol, int32_t* __restrict__ overflow)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nslice * 64)
+    return;
+  const int32_t sl = (int32_t)(t >> 6);
+  const int lane = (int)(t & 63);
+  const int32_t r = slice_row[sl] + lane;
+  const bool real_row = r < slice_row[sl + 1];
+  const int32_t len = real_row ? rp[r + 1] - rp[r] : 0;
+  bool bad = false;
+  for (int j = 0; j < soff[sl + 1] - soff[sl]; ++j) {
+    int64_t d = 0;
+    if (j < len) {
+      d = (int64_t)ci[rp[r] + j] - r;
+      if (d < -32768 || d > 32767) {
+        bad = true;
+        d = 0;
+      }
+    }
+    dcol[((int64_t)soff[sl] + j) * 64 + lane] = (int16_t)d;
+  }
+  if (bad)
+    atomicOr(overflow, 1);
+}
+
+// FP64 and FP32 values of the SELL copy (padding slots: 0).
+__global__ void k_sell_values(int32_t nslice, const int32_t* __restrict__ slice_row, const int32_t* __restrict__ soff,
+                              const int32_t* __restrict__ rp, const double* __restrict__ v, double* __restrict__ v64,
+                              float* __restrict__ v32)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nslice * 64)
+    return;
+  const int32_t sl = (int32_t)(t >> 6);
+  const int lane = (int)(t & 63);
+  const int32_t r = slice_row[sl] + lane;
+  const int32_t len = r < slice_row[sl + 1] ? rp[r + 1] - rp[r] : 0;
+  for (int j = 0; j < soff[sl + 1] - soff[sl]; ++j) {
+    const double val = j < len ? v[rp[r] + j] : 0.0;
+    const int64_t e = ((int64_t)soff[sl] + j) * 64 + lane;
+    v64[e] = val;
+    v32[e] = (float)val;
+  }
 }
 
 // Convergence, reliable-update bookkeeping and p = z + beta p.
@@ -236,7 +403,7 @@ k_pupdate(Seg g, float* __restrict__ p, float* __restrict__ z, float* __restrict
   SegState& S = st[c.s];
   if (!S.active || S.done)
     return;
-  const bool lead = blockIdx.x == c.b0 && threadIdx.x == 0;
+  const bool lead = c.gb == c.b0 && threadIdx.x == 0;
   const int k = S.iter_par[par];
   if (S.breakdown) {
     if (lead) {
@@ -295,7 +462,7 @@ k_pupdate(Seg g, float* __restrict__ p, float* __restrict__ z, float* __restrict
     p[i] = zv + bt * p[i];
   }
   if (ru)
-    put(rzn, P.rzru[par]);
+    put(rzn, P.rzru[par], c.gb);
 }
 
 // r = r64; z = D^-1 r; p = z; partials r.z (parity 1), r.r, b.b
@@ -319,9 +486,9 @@ k_init(Seg g, const double* __restrict__ r64, const double* __restrict__ b, floa
     rr += rd * rd;
     bb += b[i] * b[i];
   }
-  put(rz, P.rz[1]);
-  put(rr, P.rr);
-  put(bb, P.bb);
+  put(rz, P.rz[1], c.gb);
+  put(rr, P.rr, c.gb);
+  put(bb, P.bb, c.gb);
 }
 
 // one workgroup per segment
@@ -400,13 +567,24 @@ struct afem_batch
   DBuf<int32_t> d_rows, d_blocks, d_owner;
   int lanes = 8, tiles = 1, nblk = 0;
   std::vector<int32_t> blocks_host;
+  // launched workgroups: nlaunch > 0 -> the first nlaunch entries of d_list
+  DBuf<int32_t> d_list;
+  int nlaunch = 0;
+  // segment-aligned SELL-64 copy (built with the first solve of a pattern)
+  const int32_t* sell_rp = nullptr;
+  bool sell_ok = false;
+  DBuf<int32_t> d_sslice, d_soff, d_slice_row;
+  DBuf<int16_t> sdcol;
+  DBuf<double> sv64;
+  DBuf<float> sv32;
+  int nslice = 0;
   DBuf<float> v32, dinv, r32, z32, p32, q32;
   DBuf<double> r64, parts;
   DBuf<SegState> st;
   SegState* host_st = nullptr;
   bool prepared = false;
   hipGraphExec_t graph = nullptr;
-  int graph_key = -1;
+  int64_t graph_key = -1;
   ~afem_batch()
   {
     if (graph)
@@ -431,6 +609,11 @@ void drop_graph(afem_batch* b)
   b->graph_key = -1;
 }
 
+// lanes = 1 runs the SELL kernels when the copy exists, else CSR with 8 lanes
+bool use_sell(const afem_batch* b) { return b->lanes == 1 && b->sell_ok; }
+
+int grid(const afem_batch* b) { return b->nlaunch > 0 ? b->nlaunch : b->nblk; }
+
 void layout(afem_batch* b)
 {
   const int rpb = rows_per_block(b);
@@ -448,13 +631,75 @@ void layout(afem_batch* b)
   b->d_blocks.upload(b->blocks_host.data(), b->blocks_host.size());
   b->d_owner.alloc(b->nblk);
   b->d_owner.upload(owner.data(), owner.size());
+  b->d_list.alloc(b->nblk);
+  b->nlaunch = 0;
   b->parts.alloc((size_t)8 * b->nblk);
   drop_graph(b);
 }
 
 Seg seg_of(const afem_batch* b)
 {
-  return Seg{ b->d_rows.p, b->d_blocks.p, b->d_owner.p, rows_per_block(b) };
+  return Seg{ b->d_rows.p,  b->d_blocks.p, b->d_owner.p, rows_per_block(b), b->nlaunch > 0 ? b->d_list.p : nullptr,
+              b->d_sslice.p, b->d_soff.p,   b->sdcol.p };
+}
+
+// Launches only the workgroups of the segments still running (host states
+// just read) once they are at most half of the current launch.
+void shrink_grid(afem_batch* b)
+{
+  std::vector<int32_t> list;
+  for (int k = 0; k < b->nb_seg; ++k)
+    if (b->host_st[k].active && !b->host_st[k].done && !b->host_st[k].breakdown)
+      for (int w = b->blocks_host[k]; w < b->blocks_host[k + 1]; ++w)
+        list.push_back(w);
+  if (list.empty() || 2 * list.size() > (size_t)grid(b))
+    return;
+  b->d_list.upload(list.data(), list.size());
+  b->nlaunch = (int)list.size();
+}
+
+// The segment-aligned SELL-64 layout of the pattern: slices of 64 rows
+// starting at every segment's first row, 16-bit column offsets.
+void build_sell(afem_batch* b)
+{
+  afem_system* s = b->sys;
+  b->sell_rp = s->row_index;
+  b->sell_ok = false;
+  std::vector<int32_t> slice_row, sslice(b->nb_seg + 1, 0);
+  for (int k = 0; k < b->nb_seg; ++k) {
+    sslice[k] = (int32_t)slice_row.size();
+    for (int32_t r = b->seg_rows[k]; r < b->seg_rows[k + 1]; r += 64)
+      slice_row.push_back(r);
+  }
+  sslice[b->nb_seg] = (int32_t)slice_row.size();
+  slice_row.push_back(s->nb_dof);
+  b->nslice = sslice[b->nb_seg];
+  b->d_sslice.alloc(sslice.size());
+  b->d_sslice.upload(sslice.data(), sslice.size());
+  b->d_slice_row.alloc(slice_row.size());
+  b->d_slice_row.upload(slice_row.data(), slice_row.size());
+  DBuf<int32_t> width(b->nslice);
+  hipLaunchKernelGGL(k_sell_width, dim3(grid_for(b->nslice, 256)), dim3(256), 0, stream(), b->nslice,
+                     b->d_slice_row.p, s->row_index, width.p);
+  check_launch("batch_sell_width");
+  std::vector<int32_t> w(b->nslice), off(b->nslice + 1, 0);
+  width.download(w.data(), w.size());
+  for (int k = 0; k < b->nslice; ++k)
+    off[k + 1] = off[k] + w[k];
+  b->d_soff.alloc(off.size());
+  b->d_soff.upload(off.data(), off.size());
+  const size_t slots = (size_t)off[b->nslice] * 64;
+  b->sdcol.alloc(std::max<size_t>(slots, 1));
+  b->sv64.alloc(std::max<size_t>(slots, 1));
+  b->sv32.alloc(std::max<size_t>(slots, 1));
+  DBuf<int32_t> overflow(1);
+  overflow.zero();
+  hipLaunchKernelGGL(k_sell_columns, dim3(grid_for((int64_t)b->nslice * 64, 256)), dim3(256), 0, stream(),
+                     b->nslice, b->d_slice_row.p, b->d_soff.p, s->row_index, s->columns, b->sdcol.p, overflow.p);
+  check_launch("batch_sell_columns");
+  int32_t h = 0;
+  overflow.download(&h, 1);
+  b->sell_ok = h == 0;
 }
 
 BParts parts_of(afem_batch* b)
@@ -479,9 +724,14 @@ void launch_residual(afem_batch* b, int par, int always)
   afem_system* s = b->sys;
   const Seg g = seg_of(b);
   const BParts P = parts_of(b);
+  if (use_sell(b)) {
+    hipLaunchKernelGGL(k_residual_sell, dim3(grid(b)), dim3(kBT), 0, stream(), g, b->sv64.p, s->x.p, s->rhs.p,
+                       b->r64.p, b->st.p, P, par, always);
+    return;
+  }
   with_lanes(b->lanes, [&](auto tag) {
     constexpr int G = decltype(tag)::value;
-    hipLaunchKernelGGL((k_residual<G>), dim3(b->nblk), dim3(kBT), 0, stream(), g, s->row_index, s->columns,
+    hipLaunchKernelGGL((k_residual<G>), dim3(grid(b)), dim3(kBT), 0, stream(), g, s->row_index, s->columns,
                        s->values, s->x.p, s->rhs.p, b->r64.p, b->st.p, P, par, always);
   });
 }
@@ -491,15 +741,21 @@ void launch_iteration(afem_batch* b, int par)
   afem_system* s = b->sys;
   const Seg g = seg_of(b);
   const BParts P = parts_of(b);
-  with_lanes(b->lanes, [&](auto tag) {
-    constexpr int G = decltype(tag)::value;
-    hipLaunchKernelGGL((k_spmv_pq<G>), dim3(b->nblk), dim3(kBT), 0, stream(), g, s->row_index, s->columns, b->v32.p,
-                       b->p32.p, b->q32.p, b->st.p, P);
-  });
-  hipLaunchKernelGGL(k_update, dim3(b->nblk), dim3(kBT), 0, stream(), g, s->x.p, b->r32.p, b->z32.p, b->p32.p,
+  if (use_sell(b)) {
+    hipLaunchKernelGGL(k_spmv_pq_sell, dim3(grid(b)), dim3(kBT), 0, stream(), g, b->sv32.p, b->p32.p, b->q32.p,
+                       b->st.p, P);
+  }
+  else {
+    with_lanes(b->lanes, [&](auto tag) {
+      constexpr int G = decltype(tag)::value;
+      hipLaunchKernelGGL((k_spmv_pq<G>), dim3(grid(b)), dim3(kBT), 0, stream(), g, s->row_index, s->columns,
+                         b->v32.p, b->p32.p, b->q32.p, b->st.p, P);
+    });
+  }
+  hipLaunchKernelGGL(k_update, dim3(grid(b)), dim3(kBT), 0, stream(), g, s->x.p, b->r32.p, b->z32.p, b->p32.p,
                      b->q32.p, b->dinv.p, b->st.p, P, par);
   launch_residual(b, par, 0);
-  hipLaunchKernelGGL(k_pupdate, dim3(b->nblk), dim3(kBT), 0, stream(), g, b->p32.p, b->z32.p, b->r32.p, b->r64.p,
+  hipLaunchKernelGGL(k_pupdate, dim3(grid(b)), dim3(kBT), 0, stream(), g, b->p32.p, b->z32.p, b->r32.p, b->r64.p,
                      b->dinv.p, b->st.p, P, par);
 }
 
@@ -534,6 +790,15 @@ void prepare(afem_batch* b)
   hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, stream(), n, s->row_index, s->columns, s->values, s->nnz,
                      b->v32.p, b->dinv.p);
   check_launch("batch_prepare");
+  if (b->sell_rp != s->row_index) {
+    drop_graph(b);
+    build_sell(b);
+  }
+  if (b->sell_ok) {
+    hipLaunchKernelGGL(k_sell_values, dim3(grid_for((int64_t)b->nslice * 64, 256)), dim3(256), 0, stream(),
+                       b->nslice, b->d_slice_row.p, b->d_soff.p, s->row_index, s->values, b->sv64.p, b->sv32.p);
+    check_launch("batch_sell_values");
+  }
   b->prepared = true;
 }
 
@@ -546,7 +811,7 @@ void run_chunk(afem_batch* b, int chunk, bool use_graph)
     check_launch("batch_iteration");
     return;
   }
-  const int key = chunk * 1000 + b->lanes * 10 + b->tiles;
+  const int64_t key = ((int64_t)grid(b) << 32) | (chunk * 1000 + b->lanes * 10 + b->tiles);
   if (!b->graph || b->graph_key != key) {
     drop_graph(b);
     hipGraph_t gr;
@@ -571,6 +836,7 @@ struct RunStats
 RunStats run(afem_batch* b, const afem_batch_options& o, bool use_graph)
 {
   RunStats rs;
+  b->nlaunch = 0;
   const Seg g = seg_of(b);
   const BParts P = parts_of(b);
   afem_system* s = b->sys;
@@ -589,10 +855,12 @@ RunStats run(afem_batch* b, const afem_batch_options& o, bool use_graph)
         all = false;
     if (all || done_iters >= o.max_iter)
       break;
+    shrink_grid(b);
     run_chunk(b, chunk, use_graph);
     done_iters += chunk;
     rs.chunks += 1;
   }
+  b->nlaunch = 0; // the full layout again (true residual, next run)
   return rs;
 }
 
@@ -646,7 +914,8 @@ AFEM_API int afem_batch_set_tiles(afem_batch_t b, int lanes, int tiles)
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(b, AFEM_ERR_ARGUMENT, "null argument");
-  AFEM_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16, AFEM_ERR_ARGUMENT, "lanes must be 4, 8 or 16");
+  AFEM_REQUIRE(lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16, AFEM_ERR_ARGUMENT,
+               "lanes must be 1 (SELL), 4, 8 or 16");
   AFEM_REQUIRE(tiles >= 1 && tiles <= 16, AFEM_ERR_ARGUMENT, "tiles must be in [1, 16]");
   b->lanes = lanes;
   b->tiles = tiles;
@@ -746,7 +1015,10 @@ AFEM_API int afem_batch_autotune(afem_batch_t b, int* lanes, int* tiles, double*
   {
     int lanes, tiles;
   };
-  const Cand cands[] = { { 4, 1 }, { 4, 2 }, { 8, 1 }, { 8, 2 }, { 8, 4 }, { 16, 1 }, { 16, 2 } };
+  std::vector<Cand> cands = { { 4, 1 }, { 4, 2 }, { 8, 1 }, { 8, 2 }, { 8, 4 }, { 16, 1 }, { 16, 2 } };
+  if (b->sell_ok)
+    for (int t : { 1, 2, 4 })
+      cands.push_back({ 1, t });
   hipEvent_t e0, e1;
   AFEM_HIP(hipEventCreate(&e0));
   AFEM_HIP(hipEventCreate(&e1));

@@ -400,7 +400,10 @@ AFEM_API void afem_batch_default_options(afem_batch_options* opt);
 AFEM_API int afem_batch_create(afem_system_t sys, int nb_segment, const int32_t* segment_rows,
                                afem_batch_t* batch);
 AFEM_API int afem_batch_destroy(afem_batch_t batch);
-/* Lanes per matrix row (4, 8, 16) and 256-row strips per workgroup (1..16). */
+/* Lanes per matrix row (4, 8, 16 over the CSR arrays; 1 = one thread per row
+ * over a SELL-64 copy whose slices start at every segment's first row, with
+ * 16-bit column offsets -- CSR with 8 lanes when an offset does not fit) and
+ * 256-row strips per workgroup (1..16). */
 AFEM_API int afem_batch_set_tiles(afem_batch_t batch, int lanes, int tiles);
 /* Times the (lanes, tiles) candidates on one chunk and keeps the fastest. */
 AFEM_API int afem_batch_autotune(afem_batch_t batch, int* lanes, int* tiles, double* ms_per_iteration);

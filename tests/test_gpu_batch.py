@@ -118,13 +118,40 @@ def test_batch_autotune_and_tiles(afem_gpu):
     case = build_batch(A, [12, 40, 25, 8], seed=2)
     bs = A.BatchSolver(case["sys"], case["offs"])
     lanes, tiles, ms = bs.autotune()
-    assert lanes in (4, 8, 16) and 1 <= tiles <= 16 and ms > 0
+    assert lanes in (1, 4, 8, 16) and 1 <= tiles <= 16 and ms > 0
     res, _ = bs.solve(rtol=1e-8)
     assert all(r.converged and r.relres <= 1e-8 for r in res)
-    for lanes, tiles in ((4, 1), (16, 2)):
+    for lanes, tiles in ((4, 1), (16, 2), (1, 1), (1, 4)):
         bs.set_tiles(lanes, tiles)
         res2, _ = bs.solve(rtol=1e-8)
         assert all(r.converged for r in res2)
+
+
+def test_batch_sell_and_shrinking_grid(afem_gpu):
+    """One large system among many small ones: once the small ones converge
+    the grid shrinks to the large one's workgroups.  The SELL-64 layout
+    (lanes = 1) and CSR (8 lanes) both match the direct solves, and graph
+    replays equal plain launches."""
+    A = afem_gpu
+    print("[case] batch sell + shrinking grid", flush=True)
+    case = build_batch(A, [40, 6, 7, 8, 9, 6, 7, 8, 9], seed=5)
+    bs = A.BatchSolver(case["sys"], case["offs"])
+    sols = {}
+    for lanes in (1, 8):
+        bs.set_tiles(lanes, 1)
+        res, _ = bs.solve(rtol=1e-10, max_iter=5000)
+        x = case["sys"].solution().numpy().copy()
+        for s_, r in enumerate(res):
+            assert r.converged and r.relres <= 1e-10, (lanes, s_, r.relres)
+            ref = block_direct(case, s_)
+            xs = x[case["offs"][s_]:case["offs"][s_ + 1]]
+            assert np.linalg.norm(xs - ref) <= 1e-8 * np.linalg.norm(ref), (lanes, s_)
+        assert res[0].iterations > 2 * max(r.iterations for r in res[1:])  # the grid did shrink
+        sols[lanes] = (res, x)
+    bs.set_tiles(1, 1)
+    r2, _ = bs.solve(rtol=1e-10, max_iter=5000, use_graph=0)
+    assert [r.iterations for r in r2] == [r.iterations for r in sols[1][0]]
+    assert np.array_equal(case["sys"].solution().numpy(), sols[1][1])
 
 
 def test_batch_argument_errors(afem_gpu):
