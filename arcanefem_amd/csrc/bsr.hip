@@ -1340,6 +1340,242 @@ k_assemble_laplace_brick(const double* __restrict__ coords, const int32_t* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Edge plan: for every off-diagonal block (p, q) of the pattern, the cells
+// that contain both p and q, taken from the pair plan (the row positions of
+// the other nodes of each (row, cell) pair).  The rows are packed into
+// workgroups of at most 256 blocks; cell j of block t of workgroup w is at
+// eslice[w] + 256 j + t.  Each row thread writes only its own row's blocks,
+// so counting and filling need no atomics and keep the node->cell order.
+
+__global__ void __launch_bounds__(256)
+k_edge_count(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+             const int32_t* __restrict__ row_index, int npc, int32_t nb_node, int32_t* __restrict__ cnt)
+{
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nb_node)
+    return;
+  const int32_t r0 = row_index[p];
+  for (int32_t e = nc_off[p]; e < nc_off[p + 1]; ++e) {
+    const uint64_t w = plan[e];
+    for (int k = 0; k < npc - 1; ++k)
+      cnt[r0 + plan_pos(w, k)] += 1;
+  }
+}
+
+// widest block of each workgroup; *bad = 1 when a block has more than 255 cells
+__global__ void __launch_bounds__(256)
+k_edge_width(const int32_t* __restrict__ wg_row, const int32_t* __restrict__ row_index,
+             const int32_t* __restrict__ cnt, int32_t nwg, int32_t* __restrict__ width, int32_t* __restrict__ bad)
+{
+  const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwg)
+    return;
+  int32_t m = 0;
+  for (int32_t k = row_index[wg_row[w]]; k < row_index[wg_row[w + 1]]; ++k)
+    m = max(m, cnt[k]);
+  width[w] = m;
+  if (m > 255)
+    atomicOr(bad, 1);
+}
+
+__global__ void __launch_bounds__(256)
+k_edge_fill(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
+            const int32_t* __restrict__ row_index, const int32_t* __restrict__ row_wg,
+            const int32_t* __restrict__ wg_row, const int64_t* __restrict__ eslice, int npc, int32_t nb_node,
+            int32_t* __restrict__ run, int32_t* __restrict__ ecell, uint8_t* __restrict__ ecnt)
+{
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nb_node)
+    return;
+  const int32_t w = row_wg[p];
+  const int32_t b0 = row_index[wg_row[w]];
+  const int32_t r0 = row_index[p];
+  for (int32_t e = nc_off[p]; e < nc_off[p + 1]; ++e) {
+    const uint64_t wd = plan[e];
+    for (int k = 0; k < npc - 1; ++k) {
+      const int32_t blk = r0 + plan_pos(wd, k);
+      const int32_t j = run[blk]++;
+      ecell[eslice[w] + 256 * (int64_t)j + (blk - b0)] = plan_cell(wd);
+    }
+  }
+  for (int32_t k = r0; k < row_index[p + 1]; ++k)
+    ecnt[k] = (uint8_t)run[k];
+}
+
+static void build_edge_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  auto release_all = [b]() {
+    b->ewg_row.release();
+    b->eslice.release();
+    b->ecell.release();
+    b->ecnt.release();
+    b->e_nwg = 0;
+  };
+  release_all();
+  if (!b->plan.p || b->nb_dof != 3 || m.npc != 4 || b->max_row > 256)
+    return;
+  const int32_t n = m.nb_node;
+  std::vector<int32_t> rows((size_t)n + 1);
+  b->row_index.download(rows.data(), rows.size());
+  // whole rows, at most 256 blocks per workgroup
+  std::vector<int32_t> wg_row(1, 0), row_wg((size_t)n);
+  for (int32_t p = 0; p < n; ++p) {
+    if (rows[p + 1] - rows[wg_row.back()] > 256)
+      wg_row.push_back(p);
+    row_wg[p] = (int32_t)wg_row.size() - 1;
+  }
+  wg_row.push_back(n);
+  const int32_t nwg = (int32_t)wg_row.size() - 1;
+  const int64_t nnz = rows[n];
+  DBuf<int32_t> cnt((size_t)nnz), width((size_t)nwg), bad(1), d_row_wg((size_t)n);
+  cnt.zero();
+  bad.zero();
+  b->ewg_row.alloc(wg_row.size());
+  b->ewg_row.upload(wg_row.data(), wg_row.size());
+  d_row_wg.upload(row_wg.data(), row_wg.size());
+  hipLaunchKernelGGL(k_edge_count, dim3(grid_for(n, 256)), dim3(256), 0, stream(), m.nc_offsets.p, b->plan.p,
+                     b->row_index.p, m.npc, n, cnt.p);
+  check_launch("edge_count");
+  hipLaunchKernelGGL(k_edge_width, dim3(grid_for(nwg, 256)), dim3(256), 0, stream(), b->ewg_row.p, b->row_index.p,
+                     cnt.p, nwg, width.p, bad.p);
+  check_launch("edge_width");
+  int32_t h = 0;
+  bad.download(&h, 1);
+  if (h) {
+    release_all();
+    return;
+  }
+  std::vector<int32_t> wv((size_t)nwg);
+  width.download(wv.data(), wv.size());
+  std::vector<int64_t> slice((size_t)nwg + 1, 0);
+  for (int32_t w = 0; w < nwg; ++w)
+    slice[w + 1] = slice[w] + 256 * (int64_t)wv[w];
+  b->eslice.alloc(slice.size());
+  b->eslice.upload(slice.data(), slice.size());
+  b->ecell.alloc((size_t)std::max<int64_t>(slice[nwg], 1));
+  b->ecell.zero();
+  b->ecnt.alloc((size_t)nnz);
+  cnt.zero(); // running counters of the fill
+  hipLaunchKernelGGL(k_edge_fill, dim3(grid_for(n, 256)), dim3(256), 0, stream(), m.nc_offsets.p, b->plan.p,
+                     b->row_index.p, d_row_wg.p, b->ewg_row.p, b->eslice.p, m.npc, n, cnt.p, b->ecell.p, b->ecnt.p);
+  check_launch("edge_fill");
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  b->e_nwg = nwg;
+}
+
+// Elasticity assembly, one thread per block (p, q) of a workgroup of whole
+// rows: the 3x3 block lambda grad_p (x) grad_q + mu grad_q (x) grad_p +
+// mu (grad_p . grad_q) I times the volume, summed over the cells of the edge
+// (cofactors with the origin at p, as k_assemble_elasticity) in registers,
+// and written once (72 contiguous bytes in per-block order).  The diagonal
+// block is minus the sum of the row's other blocks: every row of the element
+// matrix annihilates rigid translations (the shape-function gradients of a
+// cell sum to zero), K_pp = -sum_{q != p} K_pq, summed in column order.
+__global__ void __launch_bounds__(256)
+k_assemble_elasticity_edge(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                           const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                           const int32_t* __restrict__ wg_row, const int64_t* __restrict__ eslice,
+                           const int32_t* __restrict__ ecell, const uint8_t* __restrict__ ecnt, double lame_lambda,
+                           double lame_mu, int per_block, double* __restrict__ values)
+{
+  __shared__ int32_t s_rows[257];
+  __shared__ double s_blk[9 * 256];
+  const int32_t w = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t p0 = wg_row[w], nr = wg_row[w + 1] - p0;
+  const int32_t b0 = row_index[p0];
+  for (int k = threadIdx.x; k <= nr; k += 256)
+    s_rows[k] = row_index[p0 + k] - b0;
+  __syncthreads();
+  const int nb = s_rows[nr];
+  const int t = threadIdx.x;
+  double K[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    K[i] = 0.0;
+  int pr = 0;
+  bool diag = false;
+  if (t < nb) {
+    int lo = 0, hi = nr - 1; // the row of block t: the last row starting at or before t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_rows[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    pr = lo;
+    const int32_t p = p0 + pr, q = columns[b0 + t];
+    diag = p == q;
+    const int cnum = ecnt[b0 + t];
+    const int64_t base = eslice[w] + t;
+    const d3 P = load_node(coords, p);
+    const d3 eq = sub(load_node(coords, q), P);
+    for (int j = 0; j < cnum; ++j) {
+      const int32_t c = ecell[base + 256 * (int64_t)j];
+      const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+      // the two vertices other than p and q, in cell order
+      const int32_t v[4] = { cq.x, cq.y, cq.z, cq.w };
+      int32_t o0 = -1, o1 = -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (v[k] != p && v[k] != q) {
+          if (o0 < 0)
+            o0 = v[k];
+          else
+            o1 = v[k];
+        }
+      if (o1 < 0)
+        continue; // not a cell of this edge (the plan never lists one)
+      const d3 ea = sub(load_node(coords, o0), P), eb = sub(load_node(coords, o1), P);
+      const d3 Cq = cross(ea, eb), Ca = cross(eb, eq), Cb = cross(eq, ea);
+      const double adet = fabs(dot(eq, Cq));
+      const d3 Cp = { -(Cq.x + Ca.x + Cb.x), -(Cq.y + Ca.y + Cb.y), -(Cq.z + Ca.z + Cb.z) };
+      const double inv6 = fast_rcp(6.0 * adet);
+      const double la = lame_lambda * inv6, mu = lame_mu * inv6;
+      const double av[3] = { Cp.x, Cp.y, Cp.z };
+      const double bq[3] = { Cq.x, Cq.y, Cq.z };
+      const double dd = mu * (av[0] * bq[0] + av[1] * bq[1] + av[2] * bq[2]);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; ++c2)
+          K[r * 3 + c2] += la * av[r] * bq[c2] + mu * bq[r] * av[c2] + (r == c2 ? dd : 0.0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    s_blk[i * 256 + t] = K[i];
+  __syncthreads();
+  if (t >= nb)
+    return;
+  const int x0 = s_rows[pr], len = s_rows[pr + 1] - x0;
+  if (diag) {
+    for (int u = x0; u < x0 + len; ++u)
+      if (u != t) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+          K[i] -= s_blk[i * 256 + u];
+      }
+  }
+  const int64_t blk = (int64_t)b0 + t;
+  if (per_block) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      values[9 * blk + i] = K[i];
+  }
+  else {
+    const int64_t base = 9 * ((int64_t)b0 + x0);
+    const int x = t - x0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+        values[base + 3 * (x + (int64_t)r * len) + s] = K[r * 3 + s];
+  }
+}
+
 static void build_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
@@ -1363,6 +1599,7 @@ static void build_plan(afem_bsr* b)
   build_local_plan(b);
   build_sliced_plan(b);
   build_brick_plan(b);
+  build_edge_plan(b);
 }
 
 // Laplace assembly driven by the plan: only the values are staged in LDS.
@@ -1701,6 +1938,14 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
   AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4, AFEM_ERR_ARGUMENT,
                "the elasticity element needs a tetrahedral mesh and nb_dof = 3");
   const afem_mesh& m = *b->mesh;
+  if (b->ecell.p && b->variant != 0) {
+    ProfScope prof("assemble_elasticity_tet");
+    hipLaunchKernelGGL(k_assemble_elasticity_edge, dim3(b->e_nwg), dim3(256), 0, stream(), m.coords.p, m.cells.p,
+                       b->row_index.p, b->columns.p, b->ewg_row.p, b->eslice.p, b->ecell.p, b->ecnt.p, lam, mu,
+                       b->per_block ? 1 : 0, b->values.p);
+    check_launch("assemble_elasticity_edge");
+    return;
+  }
   int rows;
   size_t lds;
   AFEM_REQUIRE(assembly_shape(*b, 9 * 8 + 4, 256, &rows, &lds), AFEM_ERR_CAPACITY,

@@ -235,6 +235,13 @@ struct afem_bsr
   afem::DBuf<uint64_t> bplan;        // sliced local-plan words of the bricks
   afem::DBuf<int32_t> bcell;         // cell id of each brick plan word
   int32_t brick_val_max = 0, brick_nodes_max = 0;
+  // per nonzero block plan (elasticity): workgroups of whole rows with at most
+  // 256 blocks, per block the cells sharing its edge (sliced per workgroup)
+  afem::DBuf<int32_t> ewg_row;       // first row of each workgroup (nb_wg + 1)
+  afem::DBuf<int64_t> eslice;        // first cell word of each workgroup
+  afem::DBuf<int32_t> ecell;         // cell j of block t of workgroup w at eslice[w] + 256 j + t
+  afem::DBuf<uint8_t> ecnt;          // cells per block
+  int32_t e_nwg = 0;
   afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
   afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes
   int32_t block_nodes_max = 0;       // largest node table

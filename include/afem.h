@@ -153,7 +153,10 @@ AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* va
 AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant);
 
 /* Same with the 12x12 P1 linear-elasticity element
- * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3. */
+ * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3.
+ * One thread per nonzero block sums the block over the cells of its edge;
+ * assembly variant 0 selects the row-per-thread kernel with a column search
+ * instead. */
 AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu);
 
 /* Device arrays of the matrix: row_index (nb_row+1), columns (nb_col),

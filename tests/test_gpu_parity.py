@@ -132,11 +132,12 @@ def test_assembly_autotune_keeps_parity(afem_gpu):
     assert row_scaled_error(row, O.assemble(3, X, C, 4, row, col), val) <= ROW_TOL
 
 
+@pytest.mark.parametrize("variant", [-1, 0])  # per-block kernel, column-search kernel
 @pytest.mark.parametrize("per_block", [True, False])
 @pytest.mark.parametrize("case", ["box3", "bar_3D"])
-def test_elasticity_assembly(afem_gpu, per_block, case):
+def test_elasticity_assembly(afem_gpu, per_block, case, variant):
     A = afem_gpu
-    print(f"[case] elasticity {case} per_block={per_block}", flush=True)
+    print(f"[case] elasticity {case} per_block={per_block} variant={variant}", flush=True)
     if case == "bar_3D":
         _, X, C, _, _ = load_case(case)
     else:
@@ -146,6 +147,7 @@ def test_elasticity_assembly(afem_gpu, per_block, case):
     m = A.Mesh(3, X, C)
     b = A.BSRFormat(m, 3, use_csr=not per_block)
     b.compute_sparsity()
+    b.set_assembly_variant(variant)
     b.assemble_elasticity(lam, mu)
     row, col, val, _ = b.download()
     oval = O.assemble(3, X, C, 4, row, col, element=O.ELASTICITY, nb_dof=3, order_per_block=per_block,
