@@ -81,6 +81,39 @@ __device__ __forceinline__ double sum_parts(const double* __restrict__ p, int n)
   return t;
 }
 
+// K partial arrays reduced in one pass (out[j] = sum_parts(p[j], n)): the
+// loads of all arrays are in flight together and one barrier round serves
+// them all; each sum is formed in exactly the order sum_parts uses.
+template <int K, int BS = kBlock>
+__device__ __forceinline__ void sum_parts_n(const double* const (&p)[K], int n, double (&out)[K])
+{
+  __shared__ double lds[K][BS / kWave];
+  double s[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    s[j] = 0.0;
+  for (int i = threadIdx.x; i < n; i += BS)
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      s[j] += p[j][i];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    s[j] = wave_reduce_sum(s[j]);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+      lds[j][threadIdx.x / kWave] = s[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / kWave; ++w)
+      t += lds[j][w];
+    out[j] = t;
+  }
+  __syncthreads();
+}
+
 // Writes the sum of v over the BS threads of the block to out[blockIdx.x].
 template <int BS = kBlock>
 __device__ __forceinline__ void put_part(double v, double* __restrict__ out)
@@ -577,11 +610,12 @@ __device__ __forceinline__ void apply_jacobi(int64_t u, const T* __restrict__ di
   }
 }
 
-// rho_{k-1}: r.z of iteration k-1, or of the reliable update that ended it.
-__device__ __forceinline__ double rho_prev(const State* st, const Parts& P, int nparts, int k)
+// rho_{k-1}, the r.z of iteration k-1 or of the reliable update that ended
+// it: the partials it is summed from.
+__device__ __forceinline__ const double* rho_prev_parts(const State* st, const Parts& P, int k)
 {
   const int q = (k - 1) & 1;
-  return st->ru[q] ? sum_parts(P.rzru[q], nparts) : sum_parts(P.rz[q], nparts);
+  return st->ru[q] ? P.rzru[q] : P.rz[q];
 }
 
 // r0 = b - A x0 is in r64.  z = M r, p = z; partials r.z (parity 1), r.r, b.b.
@@ -648,8 +682,10 @@ k_update(int k, int64_t nunits, double* __restrict__ x, T* __restrict__ r, T* __
 {
   if (st->done || st->breakdown)
     return;
-  const double pq = sum_parts(P.pq, nparts);
-  const double rho = rho_prev(st, P, nparts, k);
+  const double* const arrs[2] = { P.pq, rho_prev_parts(st, P, k) };
+  double sums[2];
+  sum_parts_n<2>(arrs, nparts, sums);
+  const double pq = sums[0], rho = sums[1];
   if (!(pq > 0.0) || !isfinite(pq)) { // the operator is not positive definite on p
     if (blockIdx.x == 0 && threadIdx.x == 0)
       st->breakdown = 1;
@@ -703,16 +739,17 @@ k_pupdate(int k, int mixed, int64_t nunits, T* __restrict__ p, T* __restrict__ z
     }
     return;
   }
-  const double rz = sum_parts(P.rz[k & 1], nparts);
-  const double rr = sum_parts(P.rr, nparts);
-  const double rho = rho_prev(st, P, nparts, k);
+  const double* const arrs[4] = { P.rz[k & 1], P.rr, rho_prev_parts(st, P, k), P.rr64 };
+  double sums[4];
+  sum_parts_n<4>(arrs, nparts, sums);
+  const double rz = sums[0], rr = sums[1], rho = sums[2];
   const double beta = (rho != 0.0) ? rz / rho : 0.0;
   const int q = k & 1;
   bool ru = false;
   if (mixed) {
     ru = reliable_decision(st, rr, k);
     if (ru) {
-      const double rr64 = sum_parts(P.rr64, nparts);
+      const double rr64 = sums[3]; // written by the reliable-update residual of this iteration
       if (rr64 <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr64)) {
         if (lead) {
           st->done = 1;
@@ -1062,8 +1099,10 @@ k_update_flex(int k, int32_t n, double* __restrict__ x, float* __restrict__ r, c
 {
   if (st->done || st->breakdown)
     return;
-  const double pq = sum_parts(P.pq, nparts);
-  const double rho = sum_parts(P.rz[(k - 1) & 1], nparts);
+  const double* const arrs[2] = { P.pq, P.rz[(k - 1) & 1] };
+  double sums[2];
+  sum_parts_n<2>(arrs, nparts, sums);
+  const double pq = sums[0], rho = sums[1];
   if (!(pq > 0.0) || !isfinite(pq) || !(rho > 0.0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0)
       st->breakdown = 1;
@@ -1100,15 +1139,15 @@ k_pupdate_flex(int k, int32_t n, float* __restrict__ p, const float* __restrict_
     }
     return;
   }
-  const double rr = sum_parts(P.rr, nparts);
-  const double rho = sum_parts(P.rz[(k - 1) & 1], nparts);
-  const double pq = sum_parts(P.pq, nparts);
-  const double rz = sum_parts(P.rz[k & 1], nparts);
+  const double* const arrs[6] = { P.rr, P.rz[(k - 1) & 1], P.pq, P.rz[k & 1], P.rr64, P.zq };
+  double sums[6];
+  sum_parts_n<6>(arrs, nparts, sums);
+  const double rr = sums[0], rho = sums[1], pq = sums[2], rz = sums[3];
   const int q = k & 1;
   const bool ru = reliable_decision(st, rr, k);
   double beta;
   if (ru) {
-    const double rr64 = sum_parts(P.rr64, nparts);
+    const double rr64 = sums[4];
     if (rr64 <= st->tol2 || k + 1 >= st->max_iter || !isfinite(rr64)) {
       if (lead) {
         st->done = 1;
@@ -1127,7 +1166,7 @@ k_pupdate_flex(int k, int32_t n, float* __restrict__ p, const float* __restrict_
     beta = rz / rho;
   }
   else {
-    const double zq = sum_parts(P.zq, nparts);
+    const double zq = sums[5];
     if (lead) {
       st->rr_ref[q] = st->rr_ref[(k - 1) & 1];
       st->rr_out = rr;
