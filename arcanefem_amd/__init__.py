@@ -98,6 +98,7 @@ def _declare(L):
         "afem_bsr_to_csr": (ci, [vp, pp, pp, ctypes.POINTER(i32)]),
         "afem_rhs_constant_source": (ci, [vp, dbl, vp, ci, ci, vp]),
         "afem_rhs_body_force": (ci, [vp, vp, vp]),
+        "afem_rhs_neumann": (ci, [vp, ci, vp, ci, vp, ci, ci, vp]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
         "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
@@ -442,6 +443,15 @@ class DoFLinearSystem:
                 _lib.afem_system_destroy(self.h)
         except Exception:
             pass
+
+
+def rhs_neumann(mesh, faces, value, rhs, nb_dof=1, dof_index=0):
+    """Adds the Neumann condition on the boundary faces `faces` ((nf, dim) node
+    ids) to the device vector `rhs`: a scalar flux or a dim-vector g."""
+    faces = np.ascontiguousarray(faces, np.int32)
+    vals = np.ascontiguousarray(np.atleast_1d(np.asarray(value, np.float64)))
+    check(lib().afem_rhs_neumann(mesh.h, int(faces.shape[0]), _ptr(faces), int(vals.size), _ptr(vals), nb_dof,
+                                 dof_index, rhs.c))
 
 
 def rff_parameters(dim, nfeatures, corr_length, seed):

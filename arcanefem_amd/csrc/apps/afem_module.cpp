@@ -4,6 +4,7 @@
 // linear-system factory, on a reference Gmsh mesh or a structured box.
 //
 //   afem_module poisson --mesh circle_cut.msh --f 5.5 --dirichlet horizontal=0.5 \
+//       [--neumann curved=-0.35,1.65] \
 //       --check poisson_test_ref_circle_2D.txt --epsilon 1e-4
 //   afem_module elasticity --mesh bar_dynamic_3D.msh --E 21e5 --nu 0.28 --f -1.0 \
 //       --dirichlet surfaceleft=0.0,0.0,0.0 --dirichlet surfaceright=NULL,1.0,NULL \
@@ -27,7 +28,7 @@ struct Args
   double box_jitter = 0.0;
   unsigned long long box_seed = 0;
   std::vector<double> f;
-  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet;
+  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann;
   double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
   bool print = false;
   bool parse_only = false; // read the mesh on the host, print a summary, exit
@@ -79,6 +80,14 @@ Args parse(int argc, char** argv)
     else if (key == "--f") {
       for (auto& s : split(next(), ','))
         a.f.push_back(s == "NULL" ? 0.0 : std::stod(s));
+    }
+    else if (key == "--neumann") {
+      // group=value (scalar flux) or group=gx,gy[,gz] (flux vector)
+      const std::string s = next();
+      const size_t eq = s.rfind('=');
+      if (eq == std::string::npos || eq == 0)
+        usage("--neumann expects group=value[,value,value]");
+      a.neumann.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
     }
     else if (key == "--dirichlet") {
       // group=value[,value,value]; the group may itself be a plane "x=0"
@@ -203,6 +212,10 @@ int main(int argc, char** argv)
     else if (!a.f.empty()) {
       BoundaryConditions::applyConstantSourceToRhs(a.f[0], mesh, 1, 0, rhs);
     }
+    if (!a.neumann.empty() && elasticity)
+      usage("--neumann is available for poisson");
+    for (auto& nm : a.neumann)
+      BoundaryConditions::applyNeumannToRhs(mesh, nm.first, nm.second, rhs);
     for (auto& d : a.dirichlet) {
       const NodeGroup& g = group_of(mesh, d.first);
       for (size_t c = 0; c < d.second.size() && (int)c < nb_dof; ++c) {

@@ -1576,6 +1576,92 @@ k_assemble_elasticity_edge(const double* __restrict__ coords, const int32_t* __r
   }
 }
 
+// ---- Neumann condition on boundary faces ---------------------------------------------
+//
+// Per face (2 nodes in 2D, 3 in 3D): its measure, its outward unit normal
+// (the face normal from its node order, turned away from the vertex of its
+// cell that is not on the face) and the value each of its nodes receives,
+// g |F| / k or (g . n) |F| / k, with the operation order of
+// femutils/ArcaneFemFunctions.h:2715-2759 (edges) and :1793-1840 (triangles).
+// *err = 1 when a face is not a face of any cell.
+__global__ void __launch_bounds__(256)
+k_neumann_faces(int dim, int32_t nface, const int32_t* __restrict__ faces, const double* __restrict__ coords,
+                const int32_t* __restrict__ cells, const int32_t* __restrict__ nc_off,
+                const int32_t* __restrict__ nc_items, int scalar, double g, double gx, double gy, double gz,
+                double* __restrict__ contrib, int32_t* __restrict__ err)
+{
+  const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nface)
+    return;
+  const int k = dim; // nodes per face
+  const int32_t* fn = faces + (int64_t)k * f;
+  int32_t opp = -1;
+  for (int32_t e = nc_off[fn[0]]; e < nc_off[fn[0] + 1] && opp < 0; ++e) {
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)(nc_items[e] >> 2));
+    const int32_t v[4] = { cq.x, cq.y, cq.z, cq.w };
+    int hits = 0;
+    for (int a = 1; a < k; ++a)
+      for (int s = 0; s < k + 1; ++s)
+        hits += v[s] == fn[a];
+    if (hits != k - 1)
+      continue;
+    for (int s = 0; s < k + 1; ++s) {
+      bool on_face = false;
+      for (int a = 0; a < k; ++a)
+        on_face = on_face || v[s] == fn[a];
+      if (!on_face)
+        opp = v[s];
+    }
+  }
+  if (opp < 0) {
+    contrib[f] = 0.0;
+    atomicOr(err, 1);
+    return;
+  }
+  const d3 x0 = load_node(coords, fn[0]), x1 = load_node(coords, fn[1]), xo = load_node(coords, opp);
+  double val;
+  if (dim == 2) {
+    const double dx = x1.x - x0.x, dy = x1.y - x0.y;
+    const double length = sqrt(dx * dx + dy * dy);
+    double nx = dy / length, ny = -dx / length;
+    if (nx * (xo.x - x0.x) + ny * (xo.y - x0.y) > 0.0) { // towards the cell: turn outward
+      nx = -nx;
+      ny = -ny;
+    }
+    val = scalar ? g * length / 2.0 : (nx * gx + ny * gy) * length / 2.0;
+  }
+  else {
+    const d3 x2 = load_node(coords, fn[2]);
+    d3 c = cross(sub(x1, x0), sub(x2, x0));
+    double norm = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+    const double area = norm / 2.0;
+    if (c.x * (xo.x - x0.x) + c.y * (xo.y - x0.y) + c.z * (xo.z - x0.z) > 0.0) {
+      // towards the cell: the normal of the face with its first two nodes swapped
+      c = cross(sub(x0, x1), sub(x2, x1));
+      norm = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+    }
+    const double nx = c.x / norm, ny = c.y / norm, nz = c.z / norm;
+    val = scalar ? g * area / 3.0 : (nx * gx + ny * gy + nz * gz) * area / 3.0;
+  }
+  contrib[f] = val;
+}
+
+// rhs[node * nb_dof + dof_index] += the values of the node's faces, in face order
+__global__ void __launch_bounds__(256)
+k_neumann_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int32_t* __restrict__ off,
+                 const int32_t* __restrict__ face_of, const double* __restrict__ contrib, int nb_dof, int dof_index,
+                 double* __restrict__ rhs)
+{
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode)
+    return;
+  double* r = rhs + (int64_t)node_ids[t] * nb_dof + dof_index;
+  double acc = *r;
+  for (int32_t j = off[t]; j < off[t + 1]; ++j)
+    acc += contrib[face_of[j]];
+  *r = acc;
+}
+
 static void build_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
@@ -2227,6 +2313,58 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
                      mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, f, cell_scale,
                      nb_dof, dof_index, rhs);
   check_launch("rhs_source");
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
+                              const double* values, int nb_dof, int dof_index, double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && rhs && (nb_face == 0 || face_nodes) && values, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_face >= 0, AFEM_ERR_ARGUMENT, "negative face count");
+  AFEM_REQUIRE(nb_value == 1 || nb_value == mesh->dim, AFEM_ERR_ARGUMENT,
+               "a Neumann value is a scalar or a vector of the mesh dimension");
+  AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  AFEM_REQUIRE(mesh->npc == mesh->dim + 1, AFEM_ERR_NOT_IMPLEMENTED, "Neumann faces of simplex meshes only");
+  if (nb_face == 0)
+    return AFEM_OK;
+  const int k = mesh->dim;
+  const size_t nf = (size_t)nb_face * k;
+  for (size_t j = 0; j < nf; ++j)
+    AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < mesh->nb_node, AFEM_ERR_ARGUMENT,
+                 "Neumann face node out of range");
+  // nodes of the faces and, per node, its faces in increasing order
+  std::vector<int32_t> ids(face_nodes, face_nodes + nf);
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  std::vector<int32_t> off(ids.size() + 1, 0), face_of(nf);
+  for (size_t j = 0; j < nf; ++j)
+    ++off[std::lower_bound(ids.begin(), ids.end(), face_nodes[j]) - ids.begin() + 1];
+  for (size_t t = 0; t < ids.size(); ++t)
+    off[t + 1] += off[t];
+  std::vector<int32_t> fill(off.begin(), off.end() - 1);
+  for (size_t j = 0; j < nf; ++j)
+    face_of[fill[std::lower_bound(ids.begin(), ids.end(), face_nodes[j]) - ids.begin()]++] = (int32_t)(j / k);
+  DBuf<int32_t> d_faces(nf), d_ids(ids.size()), d_off(off.size()), d_face_of(nf), err(1);
+  DBuf<double> contrib((size_t)nb_face);
+  d_faces.upload(face_nodes, nf);
+  d_ids.upload(ids.data(), ids.size());
+  d_off.upload(off.data(), off.size());
+  d_face_of.upload(face_of.data(), nf);
+  err.zero();
+  const bool scalar = nb_value == 1;
+  ProfScope prof("rhs_neumann");
+  hipLaunchKernelGGL(k_neumann_faces, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), mesh->dim, nb_face,
+                     d_faces.p, mesh->coords.p, mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, scalar ? 1 : 0,
+                     values[0], scalar ? 0.0 : values[0], scalar ? 0.0 : values[1],
+                     (!scalar && k == 3) ? values[2] : 0.0, contrib.p, err.p);
+  check_launch("neumann_faces");
+  int32_t h = 0;
+  err.download(&h, 1); // before rhs is touched: a failed call leaves it as it was
+  AFEM_REQUIRE(h == 0, AFEM_ERR_MESH, "a Neumann face is not a face of any cell");
+  hipLaunchKernelGGL(k_neumann_gather, dim3(grid_for((int64_t)ids.size(), 256)), dim3(256), 0, stream(),
+                     (int32_t)ids.size(), d_ids.p, d_off.p, d_face_of.p, contrib.p, nb_dof, dof_index, rhs);
+  check_launch("neumann_gather");
   AFEM_GUARD_END
 }
 

@@ -121,6 +121,29 @@ namespace BoundaryConditions {
     check(afem_rhs_body_force(mesh.handle(), f, rhs.devicePointer()), "applyBodyForceToRhs");
   }
 
+  // femutils/ArcaneFemFunctionsGpu.cc:269-310 on tri3 / tet4 meshes (edges /
+  // triangles, femutils/ArcaneFemFunctions.h:2715-2759, :1793-1840): one
+  // value is a scalar flux g, dim values ("NULL" = 0) the flux vector whose
+  // component along the outward normal enters the RHS.
+  inline void applyNeumannToRhs(const Mesh& mesh, const std::string& group, const std::vector<std::string>& values,
+                                VariableDoFReal& rhs)
+  {
+    std::vector<Real> v;
+    if (values.size() == 1) {
+      if (values[0] == "NULL")
+        return;
+      v.push_back(std::stod(values[0]));
+    }
+    else {
+      for (int d = 0; d < mesh.dimension(); ++d)
+        v.push_back((size_t)d < values.size() && values[d] != "NULL" ? std::stod(values[d]) : 0.0);
+    }
+    const std::vector<Int32>& faces = mesh.faceGroup(group);
+    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), (int)v.size(),
+                           v.data(), 1, 0, rhs.devicePointer()),
+          "applyNeumannToRhs");
+  }
+
   // femutils/ArcaneFemFunctionsGpu.cc:115-147 for one component: method is
   // "Penalty" (default penalty 1e30, femutils/FemBoundaryConditions.axl:104-109),
   // "RowElimination" or "RowColumnElimination".

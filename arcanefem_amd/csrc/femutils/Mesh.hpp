@@ -251,6 +251,7 @@ class Mesh
       dim = std::max(dim, b.dim);
     std::vector<Int32> cells;
     m_groups.clear();
+    m_face_groups.clear();
     for (auto& b : blocks) {
       const size_t n = b.conn.size() / (b.k + 1);
       if (b.dim == dim) {
@@ -267,6 +268,12 @@ class Mesh
           for (size_t e = 0; e < n; ++e)
             for (int s = 0; s < b.k; ++s)
               g.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
+          if (b.k == dim) { // simplex faces (edges in 2D, triangles in 3D), in file order
+            auto& fg = m_face_groups[nm->second];
+            for (size_t e = 0; e < n; ++e)
+              for (int s = 0; s < b.k; ++s)
+                fg.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
+          }
         }
       }
     }
@@ -291,6 +298,14 @@ class Mesh
       throw FatalError(AFEM_ERR_ARGUMENT, "no boundary group '" + name + "'");
     return it->second;
   }
+  // faces of a named boundary group: dimension() node ids per face
+  const std::vector<Int32>& faceGroup(const std::string& name) const
+  {
+    auto it = m_face_groups.find(name);
+    if (it == m_face_groups.end())
+      throw FatalError(AFEM_ERR_ARGUMENT, "no boundary face group '" + name + "'");
+    return it->second;
+  }
   // nodes whose coordinate `axis` equals `value` (structured boxes)
   NodeGroup nodesOnPlane(int axis, double value) const
   {
@@ -307,6 +322,7 @@ class Mesh
   std::vector<Int32> m_cells;
   std::vector<Int64> m_uids;
   std::map<std::string, NodeGroup> m_groups;
+  std::map<std::string, std::vector<Int32>> m_face_groups;
   afem_mesh_t m_handle = nullptr;
 };
 

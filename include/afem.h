@@ -181,6 +181,16 @@ AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int3
  * sum over cells T of n of f*scale_T*|T|/npc.  cell_scale [device] or NULL. */
 AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* cell_scale,
                                       int nb_dof, int dof_index, double* rhs);
+/* BoundaryConditions::applyNeumannToRhs for tri3 / tet4 meshes
+ * (femutils/ArcaneFemFunctions.h:2715-2759 edges, :1793-1840 triangles; GPU
+ * femutils/ArcaneFemFunctionsGpu.cc:679-738): for each boundary face F
+ * (face_nodes [host]: nb_face * dim node ids) and each node i of F,
+ * rhs[i*nb_dof+dof_index] += g |F| / dim (nb_value = 1) or (g . n_F) |F| / dim
+ * (nb_value = dim, values [host] the vector g, n_F the outward unit normal of
+ * F).  The faces' values are summed per node in face order (no atomics).
+ * AFEM_ERR_MESH when a face is not a face of any cell. */
+AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
+                              const double* values, int nb_dof, int dof_index, double* rhs);
 /* Elasticity body force, modules/elasticity/BodyForce.h:149-161, gather form:
  * rhs[3n+c] += sum over tets T of n of f[c]*|T|/4.  f [host] 3 values. */
 AFEM_API int afem_rhs_body_force(afem_mesh_t mesh, const double* f, double* rhs);

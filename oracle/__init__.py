@@ -60,6 +60,9 @@ def lib():
         L.afo_rhs_constant_source.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl,
                                               ctypes.c_int, ctypes.c_int, vp]
         L.afo_rhs_body_force.argtypes = [vp, i32, vp, vp, vp]
+        L.afo_rhs_neumann.restype = ctypes.c_int
+        L.afo_rhs_neumann.argtypes = [ctypes.c_int, vp, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
+                                      ctypes.c_int, vp]
         L.afo_apply_transformations.restype = ctypes.c_int
         L.afo_apply_transformations.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.afo_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
@@ -175,6 +178,19 @@ def rhs_constant_source(dim, coords, cells, npc, f, nb_dof=1, dof_index=0, cell_
     sc = None if cell_scale is None else _f64(cell_scale)
     lib().afo_rhs_constant_source(dim, _P(coords), cells.shape[0], npc, _P(cells), _P(sc), f,
                                   nb_dof, dof_index, _P(rhs))
+    return rhs
+
+
+def rhs_neumann(dim, coords, cells, faces, value, nb_dof=1, dof_index=0, rhs=None):
+    """faces: (nf, dim) node ids; value: a scalar g or a dim-vector."""
+    coords, cells, faces = _f64(coords), _i32(cells), _i32(faces)
+    vals = _f64(np.atleast_1d(np.asarray(value, dtype=np.float64)))
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * nb_dof)
+    rc = lib().afo_rhs_neumann(dim, _P(coords), cells.shape[0], _P(cells), faces.shape[0], _P(faces), vals.size,
+                               _P(vals), nb_dof, dof_index, _P(rhs))
+    if rc != 0:
+        raise ValueError("a Neumann face is not a face of any cell")
     return rhs
 
 

@@ -389,6 +389,93 @@ void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* ce
   }
 }
 
+/* BoundaryConditions2D::applyNeumannToRhsTria3 (femutils/ArcaneFemFunctions.h:2715-2759)
+ * and BoundaryConditions3D::applyNeumannToRhsTetra4 (:1793-1840), face by face in order.
+ * Measures and normals as computeLengthEdge2 (:375-381), computeNormalEdge2 (:392-405),
+ * computeAreaTria3 (:180-188) and computeNormalTriangle (:454-480), which swap the first
+ * two nodes of a face that is not oriented outward; here that is a face whose normal
+ * points towards the vertex of its cell that is not on the face. */
+int afo_rhs_neumann(int dim, const double* coords, int32_t nb_cell, const int32_t* cells, int32_t nb_face,
+                    const int32_t* faces, int nb_value, const double* values, int nb_dof, int dof_index,
+                    double* rhs)
+{
+  const int k = dim, npc = dim + 1;
+  const int scalar = nb_value == 1;
+  for (int32_t f = 0; f < nb_face; ++f) {
+    const int32_t* fn = faces + (int64_t)f * k;
+    int32_t opp = -1;
+    for (int32_t c = 0; c < nb_cell && opp < 0; ++c) {
+      const int32_t* cn = cells + 4 * (int64_t)c;
+      int hits = 0;
+      for (int a = 0; a < k; ++a)
+        for (int s = 0; s < npc; ++s)
+          hits += cn[s] == fn[a];
+      if (hits != k)
+        continue;
+      for (int s = 0; s < npc; ++s) {
+        int on = 0;
+        for (int a = 0; a < k; ++a)
+          on |= cn[s] == fn[a];
+        if (!on)
+          opp = cn[s];
+      }
+    }
+    if (opp < 0)
+      return -1;
+    const double* xo = coords + 3 * (int64_t)opp;
+    const double* v0 = coords + 3 * (int64_t)fn[0];
+    const double* v1 = coords + 3 * (int64_t)fn[1];
+    double rhs_value;
+    if (dim == 2) {
+      const double ex = v1[0] - v0[0], ey = v1[1] - v0[1], ez = v1[2] - v0[2];
+      const double length = sqrt(ex * ex + ey * ey + ez * ez);
+      const double* a = v0;
+      const double* b = v1;
+      double dx = b[0] - a[0], dy = b[1] - a[1];
+      double norm_n = sqrt(dx * dx + dy * dy);
+      double nx = dy / norm_n, ny = -dx / norm_n;
+      if (nx * (xo[0] - v0[0]) + ny * (xo[1] - v0[1]) > 0.0) {
+        a = v1;
+        b = v0;
+        dx = b[0] - a[0];
+        dy = b[1] - a[1];
+        norm_n = sqrt(dx * dx + dy * dy);
+        nx = dy / norm_n;
+        ny = -dx / norm_n;
+      }
+      rhs_value = scalar ? values[0] * length / 2.0 : (nx * values[0] + ny * values[1]) * length / 2.0;
+    }
+    else {
+      const double* v2 = coords + 3 * (int64_t)fn[2];
+      double e1[3], e2[3], c[3];
+      for (int d = 0; d < 3; ++d) {
+        e1[d] = v1[d] - v0[d];
+        e2[d] = v2[d] - v0[d];
+      }
+      c[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      c[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      c[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const double area = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) / 2.0;
+      if (c[0] * (xo[0] - v0[0]) + c[1] * (xo[1] - v0[1]) + c[2] * (xo[2] - v0[2]) > 0.0) {
+        for (int d = 0; d < 3; ++d) {
+          e1[d] = v0[d] - v1[d];
+          e2[d] = v2[d] - v1[d];
+        }
+        c[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        c[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        c[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      }
+      const double norm = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      const double nx = c[0] / norm, ny = c[1] / norm, nz = c[2] / norm;
+      rhs_value = scalar ? values[0] * area / 3.0
+                         : (nx * values[0] + ny * values[1] + nz * values[2]) * area / 3.0;
+    }
+    for (int a = 0; a < k; ++a)
+      rhs[(int64_t)fn[a] * nb_dof + dof_index] += rhs_value;
+  }
+  return 0;
+}
+
 /* ---- transformations ------------------------------------------------------------ */
 
 int afo_apply_transformations(int32_t nb_row, const int32_t* row_index, const int32_t* columns,

@@ -121,6 +121,14 @@ void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int
 void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* cells,
                         const double f[3], double* rhs);
 
+/* Neumann condition on boundary faces (dim nodes each): rhs[dof(n)] += g*|F|/dim
+ * (nb_value 1) or (g.n_F)*|F|/dim (nb_value dim, n_F the outward unit normal).
+ * femutils/ArcaneFemFunctions.h:2715-2759 (edges), :1793-1840 (triangles).
+ * Returns -1 when a face is not a face of any cell. */
+int afo_rhs_neumann(int dim, const double* coords, int32_t nb_cell, const int32_t* cells, int32_t nb_face,
+                    const int32_t* faces, int nb_value, const double* values, int nb_dof, int dof_index,
+                    double* rhs);
+
 /* ---- Dirichlet and linear-system transformations --------------------------- */
 
 enum afo_elim { AFO_ELIMINATE_NONE = 0, AFO_ELIMINATE_ROW = 1, AFO_ELIMINATE_ROW_COLUMN = 2 };

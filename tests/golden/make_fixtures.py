@@ -11,6 +11,7 @@ golden solution files, and writes one .npz per case with:
   dim         ()     int32
   group_<name>(k,)   int32     local ids of the nodes of the boundary faces
                                carrying the physical name <name>
+  faces_<name>(f, dim) int32   those faces (local node ids, file order)
   ref_uid     (M,)   int64     uids listed in the golden file
   ref_value   (M, d) float64   golden values (d = 1 or 3)
 
@@ -20,6 +21,11 @@ Golden cases (test names and tolerances from the reference's CMake files):
   sphere_3D   modules/poisson/inputs/sphere.3D.arc, check/poisson_test_ref_sphere_3D.txt
   bar_3D      modules/elasticity/inputs/bar.3D.Dirichlet.bodyForce.arc,
               check/bar.3D.Dirichlet.bodyForce.txt (eps 1e-3, min 1e-10)
+  circle_neumann_2D  modules/poisson/inputs/circle.neumann.2D.arc,
+              check/poisson_test_ref_circle_neumann_2D.txt (f=5.5, Dirichlet 0.5 on
+              'horizontal', Neumann flux (-0.35, 1.65) on 'curved', eps 1e-4)
+  sphere_neumann_3D  modules/poisson/inputs/sphere.neumann.3D.arc,
+              check/poisson_test_ref_sphere_neumann_3D.txt (flux (0.35, 1.65, 3.75))
 
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
@@ -38,6 +44,10 @@ CASES = {
     "circle_2D": ("meshes/msh/circle_cut.msh", "modules/poisson/check/poisson_test_ref_circle_2D.txt", 1),
     "sphere_3D": ("meshes/msh/sphere_cut.msh", "modules/poisson/check/poisson_test_ref_sphere_3D.txt", 1),
     "bar_3D": ("meshes/msh/bar_dynamic_3D.msh", "modules/elasticity/check/bar.3D.Dirichlet.bodyForce.txt", 3),
+    "circle_neumann_2D": ("meshes/msh/circle_cut.msh", "modules/poisson/check/poisson_test_ref_circle_neumann_2D.txt",
+                          1),
+    "sphere_neumann_3D": ("meshes/msh/sphere_cut.msh", "modules/poisson/check/poisson_test_ref_sphere_neumann_3D.txt",
+                          1),
 }
 
 # Gmsh element type -> number of nodes (only the P1 simplex family is needed)
@@ -166,7 +176,7 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     lid = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(cells)
     cells4 = -np.ones((lid.shape[0], 4), dtype=np.int32)
     cells4[:, :npc] = lid
-    groups = {}
+    groups, faces = {}, {}
     for edim, etag, _etype, _etags, conn in blocks:
         if edim != dim - 1:
             continue
@@ -176,6 +186,7 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
                 continue
             ids = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(conn.ravel())
             groups.setdefault(gname, []).append(ids)
+            faces.setdefault(gname, []).append(ids.reshape(len(conn), -1))
     payload = {
         "coords": xyz.astype(np.float64),
         "node_uid": tags.astype(np.int64),
@@ -185,6 +196,8 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     }
     for gname, lst in groups.items():
         payload["group_" + gname] = np.unique(np.concatenate(lst)).astype(np.int32)
+    for gname, lst in faces.items():
+        payload["faces_" + gname] = np.concatenate(lst).astype(np.int32)
     ref = np.loadtxt(os.path.join(ref_root, check_rel), ndmin=2)
     payload["ref_uid"] = ref[:, 0].astype(np.int64)
     payload["ref_value"] = ref[:, 1:1 + ncomp].astype(np.float64)

@@ -514,3 +514,38 @@ def test_error_paths(afem_gpu):
     s.apply_rhs_transformation()
     r = s.solve(raise_on_failure=False, max_iter=2, rtol=1e-14)
     assert r.converged == 0 and r.iterations <= 2
+
+
+def _boundary_faces(C, dim):
+    from collections import Counter
+    import itertools
+    cnt = Counter()
+    for c in C:
+        for f in itertools.combinations(sorted(c[:dim + 1]), dim):
+            cnt[f] += 1
+    return np.array([f for f, k in cnt.items() if k == 1], np.int32)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 9), (3, 5)])
+@pytest.mark.parametrize("value", ["scalar", "vector"])
+def test_neumann_matches_oracle(afem_gpu, dim, n, value):
+    """afem_rhs_neumann against the oracle on the whole boundary and on one
+    side, added on top of a source term."""
+    A = afem_gpu
+    print(f"[case] neumann dim={dim} {value}", flush=True)
+    X, C = box(dim, n, 0.2)
+    N = X.shape[0]
+    m = A.Mesh(dim, X, C)
+    F = _boundary_faces(C, dim)
+    g = 1.75 if value == "scalar" else [0.35, -1.65, 3.75][:dim]
+    for faces in (F, F[np.all(X[F][:, :, 0] == 0.0, axis=1)]):
+        rhs = A.DeviceArray(N, np.float64)
+        A.check(A.lib().afem_rhs_constant_source(m.h, 5.5, None, 1, 0, rhs.c))
+        A.rhs_neumann(m, faces, g, rhs)
+        ref = O.rhs_constant_source(dim, X, C, dim + 1, 5.5)
+        ref = O.rhs_neumann(dim, X, C, faces, g, rhs=ref)
+        assert np.abs(rhs.numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    before = rhs.numpy().copy()
+    with pytest.raises(A.AfemError):
+        A.rhs_neumann(m, np.array([[0, N - 1] + ([1] if dim == 3 else [])], np.int32), g, rhs)
+    assert np.array_equal(rhs.numpy(), before)

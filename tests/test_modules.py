@@ -110,6 +110,31 @@ def test_poisson_module_golden(msh, ref, method, fmt, precision):
     _check_ok(r)
 
 
+NEUMANN = [
+    ("circle_cut.msh", "curved=-0.35,1.65", "poisson_test_ref_circle_neumann_2D.txt"),
+    ("sphere_cut.msh", "curved=0.35,1.65,3.75", "poisson_test_ref_sphere_neumann_3D.txt"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msh,flux,ref", NEUMANN)
+@pytest.mark.parametrize("method,fmt,precision", SOLVES)
+def test_poisson_module_neumann_golden(msh, flux, ref, method, fmt, precision):
+    # modules/poisson/inputs/{circle.neumann.2D,sphere.neumann.3D}.arc: f = 5.5,
+    # u = 0.5 on 'horizontal', Neumann flux vector on 'curved'
+    r = run(["poisson", "--mesh", os.path.join(GOLDEN, msh), "--f", "5.5", "--dirichlet", "horizontal=0.5",
+             "--neumann", flux, "--method", method, "--format", fmt, "--precision", precision,
+             "--check", os.path.join(GOLDEN, ref), "--epsilon", "1e-4"])
+    _check_ok(r)
+
+
+@pytest.mark.gpu
+def test_module_neumann_unknown_group():
+    r = run(["poisson", "--mesh", os.path.join(GOLDEN, "circle_cut.msh"), "--f", "5.5", "--dirichlet",
+             "horizontal=0.5", "--neumann", "nosuchgroup=1.0"])
+    assert r.returncode == 1 and "nosuchgroup" in r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", GOLDEN_METHODS)
 @pytest.mark.parametrize("fmt", ["BSR", "AF-BSR"])

@@ -421,6 +421,36 @@ def test_golden_poisson_penalty(case):
     assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
 
 
+@pytest.mark.parametrize("case,flux", [("circle_neumann_2D", [-0.35, 1.65]),
+                                       ("sphere_neumann_3D", [0.35, 1.65, 3.75])])
+def test_golden_poisson_neumann(case, flux):
+    """modules/poisson tests 2D_neumann / 3D_neumann (inputs/circle.neumann.2D.arc,
+    sphere.neumann.3D.arc): f=5.5, Neumann flux on 'curved', u=0.5 on
+    'horizontal' by penalty, eps 1e-4 -- pins the oracle's face measure and
+    outward normal against the reference's own solution files."""
+    d = load_case(case)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = O.rhs_constant_source(dim, X, C, npc, 5.5)
+    rhs = O.rhs_neumann(dim, X, C, d["faces_curved"], flux, rhs=rhs)
+    g = d["group_horizontal"]
+    fi = np.zeros(N, np.uint8)
+    fv = np.zeros(N)
+    fi[g], fv[g] = 1, 1e30
+    rhs[g] = 1e30 * 0.5
+    O.apply_transformations(row, col, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
+    # the flux enters: with the opposite sign the solution misses the file
+    rhs2 = O.rhs_constant_source(dim, X, C, npc, 5.5)
+    rhs2 = O.rhs_neumann(dim, X, C, d["faces_curved"], [-v for v in flux], rhs=rhs2)
+    rhs2[g] = 1e30 * 0.5
+    x2 = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs2)
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x2)[:, 0], 1e-4) > 0
+
+
 @pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
 def test_golden_poisson_row_elimination_pcg(case):
     d = load_case(case)
@@ -469,3 +499,57 @@ def test_golden_elasticity_bar():
     x = spl.spsolve(sp.csr_matrix((val, ccol, crow), shape=(3 * N, 3 * N)).tocsc(), rhs)
     got = by_uid(d, x, 3)
     assert O.count_differences(d["ref_value"], got, 1e-3, 1e-10) == 0
+
+
+# ---- Neumann condition (femutils/ArcaneFemFunctions.h:2715-2759, :1793-1840) -------------
+
+
+def _boundary_faces(C, dim):
+    """Faces (dim nodes) that belong to exactly one cell."""
+    from collections import Counter
+    import itertools
+    npc = dim + 1
+    cnt = Counter()
+    for c in C:
+        for f in itertools.combinations(sorted(c[:npc]), dim):
+            cnt[f] += 1
+    return np.array([f for f, k in cnt.items() if k == 1], np.int32)
+
+
+def test_neumann_known_answers():
+    X = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    C = np.array([[0, 1, 2, -1]], np.int32)
+    for face in ([0, 1], [1, 0]):  # either node order: the normal is the outward one
+        r = O.rhs_neumann(2, X, C, np.array([face], np.int32), [0.0, -1.0])
+        assert np.allclose(r, [0.5, 0.5, 0.0], rtol=0, atol=1e-15)
+    r = O.rhs_neumann(2, X, C, np.array([[1, 2]], np.int32), 2.0)  # scalar: g |F| / 2
+    assert np.allclose(r, [0.0, np.sqrt(2), np.sqrt(2)], rtol=0, atol=1e-15)
+    X3 = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    C3 = np.array([[0, 1, 2, 3]], np.int32)
+    for face in ([0, 1, 2], [2, 1, 0], [1, 0, 2]):
+        r = O.rhs_neumann(3, X3, C3, np.array([face], np.int32), [0.0, 0.0, -3.0])  # n = -e_z, |F| = 1/2
+        assert np.allclose(r, [0.5, 0.5, 0.5, 0.0], rtol=0, atol=1e-15)
+    with pytest.raises(ValueError):
+        O.rhs_neumann(3, X3, C3, np.array([[0, 1, 7]], np.int32), 1.0)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 7), (3, 4)])
+def test_neumann_divergence_theorem(dim, n):
+    """Over the whole boundary: a constant vector flux sums to zero, a scalar
+    flux to g times the boundary measure, and e_x gives +1 on x = 1 and -1 on
+    x = 0."""
+    from arcanefem_amd import box_mesh
+    X, C = box_mesh(dim, n, 0.2, seed=3)
+    F = _boundary_faces(C, dim)
+    g = [0.3, -1.1, 0.7][:dim]
+    assert abs(O.rhs_neumann(dim, X, C, F, g).sum()) <= 1e-13
+    assert abs(O.rhs_neumann(dim, X, C, F, 2.5).sum() - 2.5 * 2 * dim) <= 1e-12
+    ex = [1.0, 0.0, 0.0][:dim]
+    on_x1 = np.all(X[F][:, :, 0] == 1.0, axis=1)
+    on_x0 = np.all(X[F][:, :, 0] == 0.0, axis=1)
+    assert abs(O.rhs_neumann(dim, X, C, F[on_x1], ex).sum() - 1.0) <= 1e-13
+    assert abs(O.rhs_neumann(dim, X, C, F[on_x0], ex).sum() + 1.0) <= 1e-13
+    # accumulates on top of an existing RHS, per DoF
+    base = np.arange(X.shape[0] * 2, dtype=float)
+    r = O.rhs_neumann(dim, X, C, F, 1.0, nb_dof=2, dof_index=1, rhs=base.copy())
+    assert np.array_equal(r[0::2], base[0::2])
