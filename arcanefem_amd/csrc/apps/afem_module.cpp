@@ -89,7 +89,10 @@ Args parse(int argc, char** argv)
         usage("--neumann expects group=value[,value,value]");
       a.neumann.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
     }
-    else if (key == "--dirichlet") {
+    else if (key == "--dirichlet" || key == "--dirichlet-point") {
+      // point conditions (applyPointDirichletToLhsAndRhs,
+      // femutils/ArcaneFemFunctionsGpu.cc:157-184) are node-group conditions
+      // on a named Gmsh point
       // group=value[,value,value]; the group may itself be a plane "x=0"
       const std::string s = next();
       const size_t eq = s.rfind('=');

@@ -90,7 +90,8 @@ class Mesh
   }
 
   // Binary Gmsh 4.1 (size_t = 8): nodes, the cells of the highest dimension
-  // (tri3 or tet4) and the node groups of the (dim-1) physical names.
+  // (tri3 or tet4), the node and face groups of the (dim-1) physical names and
+  // the node groups of the named physical points.
   void readGmsh41(const std::string& path, bool upload = true)
   {
     std::ifstream f(path, std::ios::binary);
@@ -259,7 +260,7 @@ class Mesh
           for (int s = 0; s < 4; ++s)
             cells.push_back(s < b.k ? lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]) : -1);
       }
-      else if (b.dim == dim - 1) {
+      else if (b.dim == dim - 1 || (b.dim == 0 && dim > 1)) { // faces, or named points
         for (int ph : entity_phys[{ b.dim, b.tag }]) {
           auto nm = names.find({ b.dim, ph });
           if (nm == names.end())
@@ -268,7 +269,7 @@ class Mesh
           for (size_t e = 0; e < n; ++e)
             for (int s = 0; s < b.k; ++s)
               g.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
-          if (b.k == dim) { // simplex faces (edges in 2D, triangles in 3D), in file order
+          if (b.dim == dim - 1 && b.k == dim) { // simplex faces (edges in 2D, triangles in 3D), in file order
             auto& fg = m_face_groups[nm->second];
             for (size_t e = 0; e < n; ++e)
               for (int s = 0; s < b.k; ++s)

@@ -26,6 +26,10 @@ Golden cases (test names and tolerances from the reference's CMake files):
               'horizontal', Neumann flux (-0.35, 1.65) on 'curved', eps 1e-4)
   sphere_neumann_3D  modules/poisson/inputs/sphere.neumann.3D.arc,
               check/poisson_test_ref_sphere_neumann_3D.txt (flux (0.35, 1.65, 3.75))
+  point_dirichlet_2D modules/poisson/inputs/perforatedSquare.pointDirichlet.2D.arc,
+              check/poisson_test_point_dirichlet_2D.txt (no source; penalty at the
+              points topLeftCorner 50, topRightCorner 20, botLeftCorner 20,
+              botRightCorner 50)
 
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
@@ -48,6 +52,7 @@ CASES = {
                           1),
     "sphere_neumann_3D": ("meshes/msh/sphere_cut.msh", "modules/poisson/check/poisson_test_ref_sphere_neumann_3D.txt",
                           1),
+    "point_dirichlet_2D": ("meshes/msh/plancher.msh", "modules/poisson/check/poisson_test_point_dirichlet_2D.txt", 1),
 }
 
 # Gmsh element type -> number of nodes (only the P1 simplex family is needed)
@@ -178,7 +183,7 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     cells4[:, :npc] = lid
     groups, faces = {}, {}
     for edim, etag, _etype, _etags, conn in blocks:
-        if edim != dim - 1:
+        if edim != dim - 1 and not (edim == 0 and dim > 1):  # faces, or named points
             continue
         for ptag in entity_phys.get((edim, etag), []):
             gname = names.get((edim, ptag))
@@ -186,7 +191,8 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
                 continue
             ids = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(conn.ravel())
             groups.setdefault(gname, []).append(ids)
-            faces.setdefault(gname, []).append(ids.reshape(len(conn), -1))
+            if edim == dim - 1:
+                faces.setdefault(gname, []).append(ids.reshape(len(conn), -1))
     payload = {
         "coords": xyz.astype(np.float64),
         "node_uid": tags.astype(np.int64),

@@ -129,6 +129,19 @@ def test_poisson_module_neumann_golden(msh, flux, ref, method, fmt, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method,fmt,precision", SOLVES)
+def test_poisson_module_point_dirichlet_golden(method, fmt, precision):
+    # modules/poisson/inputs/perforatedSquare.pointDirichlet.2D.arc: no source,
+    # Dirichlet at four named points of meshes/plancher.msh
+    r = run(["poisson", "--mesh", os.path.join(GOLDEN, "plancher.msh"),
+             "--dirichlet-point", "topLeftCorner=50.0", "--dirichlet-point", "topRightCorner=20.0",
+             "--dirichlet-point", "botLeftCorner=20.0", "--dirichlet-point", "botRightCorner=50.0",
+             "--method", method, "--format", fmt, "--precision", precision,
+             "--check", os.path.join(GOLDEN, "poisson_test_point_dirichlet_2D.txt"), "--epsilon", "1e-4"])
+    _check_ok(r)
+
+
+@pytest.mark.gpu
 def test_module_neumann_unknown_group():
     r = run(["poisson", "--mesh", os.path.join(GOLDEN, "circle_cut.msh"), "--f", "5.5", "--dirichlet",
              "horizontal=0.5", "--neumann", "nosuchgroup=1.0"])

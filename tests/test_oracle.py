@@ -451,6 +451,30 @@ def test_golden_poisson_neumann(case, flux):
     assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x2)[:, 0], 1e-4) > 0
 
 
+POINTS = {"topLeftCorner": 50.0, "topRightCorner": 20.0, "botLeftCorner": 20.0, "botRightCorner": 50.0}
+
+
+def test_golden_poisson_point_dirichlet():
+    """modules/poisson test 2D_pntDirichlet (inputs/perforatedSquare.pointDirichlet.2D.arc):
+    no source, penalty Dirichlet at four named mesh points, eps 1e-4."""
+    d = load_case("point_dirichlet_2D")
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = np.zeros(N)
+    fi = np.zeros(N, np.uint8)
+    fv = np.zeros(N)
+    for name, g in POINTS.items():
+        nodes = d["group_" + name]
+        assert len(nodes) == 1
+        fi[nodes], fv[nodes] = 1, 1e30
+        rhs[nodes] = 1e30 * g
+    O.apply_transformations(row, col, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, x)[:, 0], 1e-4) == 0
+
+
 @pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
 def test_golden_poisson_row_elimination_pcg(case):
     d = load_case(case)
