@@ -71,6 +71,9 @@ def lib():
         L.afo_count_differences.restype = i64
         L.afo_count_differences.argtypes = [i64, vp, vp, dbl, dbl]
         L.afo_wtime.restype = dbl
+        L.afo_assemble_laplace_threads.restype = ctypes.c_int
+        L.afo_assemble_laplace_threads.argtypes = [i32, vp, i32, i32, vp, vp, vp, dbl, vp, vp, ctypes.c_int]
+        L.afo_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -192,6 +195,22 @@ def rhs_neumann(dim, coords, cells, faces, value, nb_dof=1, dof_index=0, rhs=Non
     if rc != 0:
         raise ValueError("a Neumann face is not a face of any cell")
     return rhs
+
+
+def max_threads():
+    return int(lib().afo_max_threads())
+
+
+def assemble_laplace_threads(coords, cells, npc, row, cols, f, nthreads):
+    """Laplace values and constant-source RHS on nthreads threads (timing baseline)."""
+    coords, cells, row, cols = _f64(coords), _i32(cells), _i32(row), _i32(cols)
+    vals = np.empty(int(row[-1]))
+    rhs = np.empty(coords.shape[0])
+    rc = lib().afo_assemble_laplace_threads(coords.shape[0], _P(coords), cells.shape[0], npc, _P(cells), _P(row),
+                                            _P(cols), f, _P(vals), _P(rhs), int(nthreads))
+    if rc != 0:
+        raise ValueError("assembly: value not found in pattern")
+    return vals, rhs
 
 
 def rhs_body_force(coords, cells, f, rhs=None):

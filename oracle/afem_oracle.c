@@ -10,6 +10,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <time.h>
 
 /* ---- small vector helpers (Arcane Real3 semantics) ------------------------ */
@@ -329,6 +332,68 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
     }
   }
   return 0;
+}
+
+/* The same cell-wise Laplace assembly and constant source on `nthreads` host
+ * threads: cells split among the threads, every entry added with an atomic
+ * update, as the reference's atomic scatter does on its device
+ * (femutils/BSRFormat.h:257-401, doAtomic Add).  Timing baseline only; the
+ * summation order differs from afo_assemble in the last bits. */
+int afo_assemble_laplace_threads(int32_t nb_node, const double* coords, int32_t nb_cell, int32_t npc,
+                                 const int32_t* cells, const int32_t* row_index, const int32_t* columns,
+                                 double f, double* values, double* rhs, int nthreads)
+{
+  const int64_t nnz = row_index[nb_node];
+  int bad = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+#pragma omp for schedule(static)
+    for (int64_t k = 0; k < nnz; ++k)
+      values[k] = 0.0;
+#pragma omp for schedule(static)
+    for (int64_t k = 0; k < nb_node; ++k)
+      rhs[k] = 0.0;
+#pragma omp for schedule(static) reduction(| : bad)
+    for (int64_t c = 0; c < nb_cell; ++c) {
+      const int32_t* cn = cells + c * 4;
+      double X[12], K[16];
+      for (int k = 0; k < npc; ++k)
+        memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+      double m;
+      if (npc == 3) {
+        afo_elem_laplace_tria3(X, 1.0, K);
+        m = afo_area_tria3(X, X + 3, X + 6);
+      }
+      else {
+        afo_elem_laplace_tetra4(X, 1.0, K);
+        m = afo_volume_tetra4(X, X + 3, X + 6, X + 9);
+      }
+      for (int a = 0; a < npc; ++a) {
+        const int32_t r = cn[a];
+        for (int b = 0; b < npc; ++b) {
+          const int32_t x = find_in_row(row_index, columns, r, cn[b]);
+          if (x < 0) {
+            bad = 1;
+            continue;
+          }
+#pragma omp atomic
+          values[(int64_t)row_index[r] + x] += K[a * npc + b];
+        }
+#pragma omp atomic
+        rhs[r] += f * m / npc;
+      }
+    }
+  }
+  return bad ? -1 : 0;
+}
+
+int afo_max_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 /* femutils/BSRFormat.cc:110-172 */

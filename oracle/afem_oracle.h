@@ -108,6 +108,13 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords /* nb_node x 3 *
 void afo_bsr_to_csr(int32_t nb_node, int nb, const int32_t* row_index, const int32_t* columns,
                     int32_t* csr_row, int32_t* csr_col);
 
+/* Laplace values (lambda = 1) and the constant-source RHS f*|T|/npc on `nthreads`
+ * threads with atomic updates (timing baseline; see afem_oracle.c). */
+int afo_assemble_laplace_threads(int32_t nb_node, const double* coords, int32_t nb_cell, int32_t npc,
+                                 const int32_t* cells, const int32_t* row_index, const int32_t* columns,
+                                 double f, double* values, double* rhs, int nthreads);
+int afo_max_threads(void);
+
 /* ---- right-hand side ---------------------------------------------------------- */
 
 /* b[dof(n)] += f*|T|/npc for every cell T and node n of T.

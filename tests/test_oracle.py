@@ -577,3 +577,17 @@ def test_neumann_divergence_theorem(dim, n):
     base = np.arange(X.shape[0] * 2, dtype=float)
     r = O.rhs_neumann(dim, X, C, F, 1.0, nb_dof=2, dof_index=1, rhs=base.copy())
     assert np.array_equal(r[0::2], base[0::2])
+
+
+def test_threaded_assembly_matches_serial():
+    """The threaded baseline assembly (atomic scatter) equals the serial one up
+    to summation order."""
+    from arcanefem_amd import box_mesh
+    for dim, n in ((2, 9), (3, 5)):
+        X, C = box_mesh(dim, n, 0.2, seed=4)
+        row, col = O.pattern(X.shape[0], C, dim + 1)
+        val, rhs = O.assemble_laplace_threads(X, C, dim + 1, row, col, 1.5, 4)
+        ref = O.assemble(dim, X, C, dim + 1, row, col)
+        assert np.abs(val - ref).max() <= 1e-13 * np.abs(ref).max()
+        rref = O.rhs_constant_source(dim, X, C, dim + 1, 1.5)
+        assert np.abs(rhs - rref).max() <= 1e-13 * np.abs(rref).max()
