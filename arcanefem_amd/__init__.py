@@ -99,6 +99,8 @@ def _declare(L):
         "afem_rhs_constant_source": (ci, [vp, dbl, vp, ci, ci, vp]),
         "afem_rhs_body_force": (ci, [vp, vp, vp]),
         "afem_rhs_neumann": (ci, [vp, ci, vp, ci, vp, ci, ci, vp]),
+        "afem_rhs_mass_field": (ci, [vp, vp, ctypes.c_double, ci, ci, vp]),
+        "afem_bsr_assemble_heat": (ci, [vp, vp, ctypes.c_double]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
         "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
@@ -302,6 +304,11 @@ class BSRFormat:
         lam = cell_lambda.c if cell_lambda is not None else None
         check(lib().afem_bsr_assemble_laplace(self.h, lam, float(source_f), rhs.c if rhs is not None else None))
 
+    def assemble_heat(self, cell_lambda, dt):
+        """lambda K + M / dt (cell_lambda a DeviceArray or None for 1)."""
+        lam = cell_lambda.c if cell_lambda is not None else None
+        check(lib().afem_bsr_assemble_heat(self.h, lam, float(dt)))
+
     def assemble_elasticity(self, lame_lambda, lame_mu):
         check(lib().afem_bsr_assemble_elasticity(self.h, float(lame_lambda), float(lame_mu)))
 
@@ -452,6 +459,11 @@ def rhs_neumann(mesh, faces, value, rhs, nb_dof=1, dof_index=0):
     vals = np.ascontiguousarray(np.atleast_1d(np.asarray(value, np.float64)))
     check(lib().afem_rhs_neumann(mesh.h, int(faces.shape[0]), _ptr(faces), int(vals.size), _ptr(vals), nb_dof,
                                  dof_index, rhs.c))
+
+
+def rhs_mass_field(mesh, field, scale, rhs, nb_dof=1, dof_index=0):
+    """rhs += M (scale * field), field a DeviceArray of one value per node."""
+    check(lib().afem_rhs_mass_field(mesh.h, field.c, float(scale), nb_dof, dof_index, rhs.c))
 
 
 def rff_parameters(dim, nfeatures, corr_length, seed):

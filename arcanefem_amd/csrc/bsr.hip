@@ -1662,6 +1662,73 @@ k_neumann_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int3
   *r = acc;
 }
 
+// ---- heat: mass terms -----------------------------------------------------------------
+
+// |T| of cell c: |cross(n1 - n0, n2 - n0)| / 2 (tri) or |det| / 6 (tet), as
+// MeshOperation::computeAreaTria3 / computeVolumeTetra4.
+__device__ __forceinline__ double cell_measure(const double* __restrict__ coords, int4 cq, int npc)
+{
+  const d3 x0 = load_node(coords, cq.x);
+  const d3 e1 = sub(load_node(coords, cq.y), x0), e2 = sub(load_node(coords, cq.z), x0);
+  const d3 c = cross(e1, e2);
+  if (npc == 3)
+    return sqrt(c.x * c.x + c.y * c.y + c.z * c.z) / 2.0;
+  return fabs(dot(sub(load_node(coords, cq.w), x0), c)) / 6.0;
+}
+
+// values += M / dt, the consistent mass matrix of the heat element
+// (modules/heat/ElementMatrix.h:28-39, 80-91: (1/12 or 1/20) * massMatrix * |T| / dt,
+// massMatrix 2 on the diagonal and 1 elsewhere), one thread per row.
+__global__ void __launch_bounds__(256)
+k_add_mass(const double* __restrict__ coords, const int32_t* __restrict__ cells, int npc,
+           const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+           const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns, int32_t nb_node, double dt,
+           double* __restrict__ values)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t a = row_index[i], len = row_index[i + 1] - a;
+  const int32_t* rc = columns + a;
+  const double w = npc == 3 ? 1.0 / 12.0 : 1.0 / 20.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)(nc_items[e] >> 2));
+    const double m = cell_measure(coords, cq, npc);
+    const int32_t v[4] = { cq.x, cq.y, cq.z, cq.w };
+    for (int k = 0; k < npc; ++k) {
+      const int pos = find_col(rc, len, v[k]);
+      if (pos < len && rc[pos] == v[k])
+        values[a + pos] += w * (v[k] == i ? 2.0 : 1.0) * m / dt;
+    }
+  }
+}
+
+// rhs[i*nb_dof+dof_index] += (M (scale * field))_i, gathered per node over its
+// cells in node->cell order (femutils/ArcaneFemFunctions.h:2558-2590, 1628-1660:
+// per cell sum_j c_ij f_j, c = |T|/6 | |T|/12 (tri), |T|/10 | |T|/20 (tet)).
+__global__ void __launch_bounds__(256)
+k_rhs_mass_field(const double* __restrict__ coords, const int32_t* __restrict__ cells, int npc,
+                 const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+                 const double* __restrict__ field, double scale, int nb_dof, int dof_index, double* __restrict__ rhs)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  double sum = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)(nc_items[e] >> 2));
+    const int s = nc_items[e] & 3;
+    const double m = cell_measure(coords, cq, npc);
+    const double cd = npc == 3 ? m / 6.0 : m / 10.0, co = npc == 3 ? m / 12.0 : m / 20.0;
+    const int32_t v[4] = { cq.x, cq.y, cq.z, cq.w };
+    double acc = 0.0;
+    for (int j = 0; j < npc; ++j)
+      acc += (j == s ? cd : co) * (field[v[j]] * scale);
+    sum += acc;
+  }
+  rhs[(int64_t)i * nb_dof + dof_index] += sum;
+}
+
 static void build_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
@@ -2365,6 +2432,35 @@ AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face
   hipLaunchKernelGGL(k_neumann_gather, dim3(grid_for((int64_t)ids.size(), 256)), dim3(256), 0, stream(),
                      (int32_t)ids.size(), d_ids.p, d_off.p, d_face_of.p, contrib.p, nb_dof, dof_index, rhs);
   check_launch("neumann_gather");
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, double dt)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
+  AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_ARGUMENT, "the heat element needs nb_dof = 1");
+  AFEM_REQUIRE(dt > 0.0, AFEM_ERR_ARGUMENT, "the time step must be positive");
+  assemble_laplace(bsr, cell_lambda, 0.0, nullptr);
+  const afem_mesh& m = *bsr->mesh;
+  ProfScope prof("assemble_heat_mass");
+  hipLaunchKernelGGL(k_add_mass, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.coords.p, m.cells.p,
+                     m.npc, m.nc_offsets.p, m.nc_items.p, bsr->row_index.p, bsr->columns.p, m.nb_node, dt,
+                     bsr->values.p);
+  check_launch("add_mass");
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_rhs_mass_field(afem_mesh_t mesh, const double* field, double scale, int nb_dof, int dof_index,
+                                 double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && field && rhs, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  hipLaunchKernelGGL(k_rhs_mass_field, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
+                     mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, field, scale,
+                     nb_dof, dof_index, rhs);
+  check_launch("rhs_mass_field");
   AFEM_GUARD_END
 }
 

@@ -152,6 +152,12 @@ AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda
 AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* variant);
 AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant);
 
+/* Heat element lambda K + M / dt (modules/heat/ElementMatrix.h:28-39 tri,
+ * :80-91 tet; M the consistent mass matrix, femutils/FemUtils.h:341-355):
+ * the Laplace assembly (cell_lambda [device] or NULL = 1, current variant)
+ * followed by a pass adding M / dt.  Values only; nb_dof = 1, dt > 0. */
+AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, double dt);
+
 /* Same with the 12x12 P1 linear-elasticity element
  * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3.
  * One thread per nonzero block sums the block over the cells of its edge;
@@ -191,6 +197,12 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
  * AFEM_ERR_MESH when a face is not a face of any cell. */
 AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
                               const double* values, int nb_dof, int dof_index, double* rhs);
+/* integrateNodalFieldToRhs (femutils/ArcaneFemFunctions.h:2558-2590 tri,
+ * :1628-1660 tet) on scale * field, gather form: rhs[i*nb_dof+dof_index] +=
+ * (M (scale * field))_i with M the consistent P1 mass matrix.  field [device],
+ * one value per node (the heat module's M u_old / dt with scale = 1 / dt). */
+AFEM_API int afem_rhs_mass_field(afem_mesh_t mesh, const double* field, double scale, int nb_dof,
+                                 int dof_index, double* rhs);
 /* Elasticity body force, modules/elasticity/BodyForce.h:149-161, gather form:
  * rhs[3n+c] += sum over tets T of n of f[c]*|T|/4.  f [host] 3 values. */
 AFEM_API int afem_rhs_body_force(afem_mesh_t mesh, const double* f, double* rhs);

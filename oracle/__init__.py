@@ -19,7 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 ELIMINATE_NONE, ELIMINATE_ROW, ELIMINATE_ROW_COLUMN = 0, 1, 2
-LAPLACE, ELASTICITY = 0, 1
+LAPLACE, ELASTICITY, HEAT = 0, 1, 2
 
 
 def build():
@@ -60,6 +60,7 @@ def lib():
         L.afo_rhs_constant_source.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl,
                                               ctypes.c_int, ctypes.c_int, vp]
         L.afo_rhs_body_force.argtypes = [vp, i32, vp, vp, vp]
+        L.afo_rhs_mass_field.argtypes = [vp, i32, i32, vp, vp, dbl, ctypes.c_int, ctypes.c_int, vp]
         L.afo_rhs_neumann.restype = ctypes.c_int
         L.afo_rhs_neumann.argtypes = [ctypes.c_int, vp, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
                                       ctypes.c_int, vp]
@@ -211,6 +212,15 @@ def assemble_laplace_threads(coords, cells, npc, row, cols, f, nthreads):
     if rc != 0:
         raise ValueError("assembly: value not found in pattern")
     return vals, rhs
+
+
+def rhs_mass_field(coords, cells, npc, field, scale, nb_dof=1, dof_index=0, rhs=None):
+    coords, cells, field = _f64(coords), _i32(cells), _f64(field)
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * nb_dof)
+    lib().afo_rhs_mass_field(_P(coords), cells.shape[0], npc, _P(cells), _P(field), scale, nb_dof, dof_index,
+                             _P(rhs))
+    return rhs
 
 
 def rhs_body_force(coords, cells, f, rhs=None):

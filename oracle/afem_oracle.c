@@ -132,6 +132,27 @@ void afo_elem_laplace_tetra4(const double* X, double lambda, double K[16])
       K[i * 4 + j] = s * (dx[i] * dx[j]) + s * (dy[i] * dy[j]) + s * (dz[i] * dz[j]);
 }
 
+/* Heat element lambda K + M / dt, modules/heat/ElementMatrix.h:28-39 (tri:
+ * (1/12.) * massMatrix * area / dt) and :80-91 (tet: (1/20.) * ... * volume / dt);
+ * massMatrix (femutils/FemUtils.h:341-355) is 2 on the diagonal, 1 elsewhere. */
+void afo_elem_heat_tria3(const double* X, double lambda, double dt, double K[9])
+{
+  afo_elem_laplace_tria3(X, lambda, K);
+  double area = afo_area_tria3(X, X + 3, X + 6);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      K[i * 3 + j] = K[i * 3 + j] + (1.0 / 12.0) * (i == j ? 2.0 : 1.0) * area / dt;
+}
+
+void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16])
+{
+  afo_elem_laplace_tetra4(X, lambda, K);
+  double vol = afo_volume_tetra4(X, X + 3, X + 6, X + 9);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      K[i * 4 + j] = K[i * 4 + j] + (1.0 / 20.0) * (i == j ? 2.0 : 1.0) * vol / dt;
+}
+
 /* modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
 {
@@ -307,6 +328,12 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
       else
         afo_elem_laplace_tetra4(X, lambda, K);
     }
+    else if (element == AFO_HEAT) { /* lame_lambda carries dt */
+      if (npc == 3)
+        afo_elem_heat_tria3(X, lambda, lame_lambda, K);
+      else
+        afo_elem_heat_tetra4(X, lambda, lame_lambda, K);
+    }
     else {
       afo_elem_elasticity_tetra4(X, lame_lambda, lame_mu, K);
     }
@@ -435,6 +462,40 @@ void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int
     double q = cell_scale ? f * cell_scale[c] : f;
     for (int k = 0; k < npc; ++k)
       rhs[(int64_t)cn[k] * nb_dof + dof_index] += q * m / npc;
+  }
+}
+
+/* BoundaryConditions2D::integrateNodalFieldToRhsTria3 (femutils/ArcaneFemFunctions.h:2558-2590)
+ * and BoundaryConditions3D::integrateNodalFieldToRhsTetra4 (:1628-1660) on the field
+ * multiplied by scale first (the heat module does m_node_temperature_old.mult(1 / dt),
+ * modules/heat/FemModule.cc:343): per cell, node i receives sum_j c_ij f_j with
+ * c = area/6 | area/12 (tri) or volume/10 | volume/20 (tet) on | off the diagonal. */
+void afo_rhs_mass_field(const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                        const double* field, double scale, int nb_dof, int dof_index, double* rhs)
+{
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + c * 4;
+    double m, cd, co;
+    if (npc == 3) {
+      m = afo_area_tria3(coords + 3 * (int64_t)cn[0], coords + 3 * (int64_t)cn[1], coords + 3 * (int64_t)cn[2]);
+      cd = m / 6.0;
+      co = m / 12.0;
+    }
+    else {
+      m = afo_volume_tetra4(coords + 3 * (int64_t)cn[0], coords + 3 * (int64_t)cn[1], coords + 3 * (int64_t)cn[2],
+                            coords + 3 * (int64_t)cn[3]);
+      cd = m / 10.0;
+      co = m / 20.0;
+    }
+    double fv[4];
+    for (int j = 0; j < npc; ++j)
+      fv[j] = field[cn[j]] * scale;
+    for (int i = 0; i < npc; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < npc; ++j)
+        acc += (i == j ? cd : co) * fv[j];
+      rhs[(int64_t)cn[i] * nb_dof + dof_index] += acc;
+    }
   }
 }
 

@@ -81,7 +81,7 @@ int64_t afo_pattern(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t
 
 /* ---- assembly --------------------------------------------------------------- */
 
-enum afo_element { AFO_LAPLACE = 0, AFO_ELASTICITY = 1 };
+enum afo_element { AFO_LAPLACE = 0, AFO_ELASTICITY = 1, AFO_HEAT = 2 };
 
 /* Cell-wise assembly with a linear column search in each owned row, values
  * accumulated with '+=' (the reference uses atomics on GPU and map inserts on
@@ -90,7 +90,8 @@ enum afo_element { AFO_LAPLACE = 0, AFO_ELASTICITY = 1 };
  *   femutils/BSRFormat.h:309-370  (ordered per row = scalar CSR order:
  *        value = row_index[r]*nb^2 + nb*(x + i*nnz_row[r]) + j)
  *   modules/poisson/FemModule.cc:398-421 (DOK/cell loop)
- * nb_dof = 1 for AFO_LAPLACE (2D tri or 3D tet), 3 for AFO_ELASTICITY (tet).
+ * nb_dof = 1 for AFO_LAPLACE and AFO_HEAT (2D tri or 3D tet; for AFO_HEAT
+ * lame_lambda carries the time step dt), 3 for AFO_ELASTICITY (tet).
  * cell_lambda may be NULL (lambda = 1 for every cell).
  * values (nnz*nb_dof^2) are zeroed first.  Returns 0, or -1 when a column is
  * missing from its row (BSRFormat.cc:105 "Value not found"). */
@@ -123,6 +124,13 @@ int afo_max_threads(void);
 void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int32_t npc,
                              const int32_t* cells, const double* cell_scale /* NULL = 1 */,
                              double f, int nb_dof, int dof_index, double* rhs);
+
+/* Mass-matrix product of a nodal field (times scale) into the RHS:
+ * femutils/ArcaneFemFunctions.h:2558-2590 (tri), :1628-1660 (tet). */
+void afo_rhs_mass_field(const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                        const double* field, double scale, int nb_dof, int dof_index, double* rhs);
+void afo_elem_heat_tria3(const double* X, double lambda, double dt, double K[9]);
+void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16]);
 
 /* b[3n+c] += f[c]*V/4 for every tet and node.  modules/elasticity/BodyForce.h:149-161 */
 void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* cells,

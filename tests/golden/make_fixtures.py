@@ -30,6 +30,10 @@ Golden cases (test names and tolerances from the reference's CMake files):
               check/poisson_test_point_dirichlet_2D.txt (no source; penalty at the
               points topLeftCorner 50, topRightCorner 20, botLeftCorner 20,
               botRightCorner 50)
+  heat_plate_2D  modules/heat/inputs/conduction.arc, check/2d_conduction.txt (lambda 1.75,
+              dt 0.4, tmax 20, Tinit 30, penalty 1e31 Dirichlet 10 on 'left', eps 1e-4)
+  heat_cube_3D   modules/heat/inputs/3d_conduction.arc, check/3d_conduction.txt (lambda 1.75,
+              dt 0.1, tmax 1, Tinit 30, Dirichlet 100.6 on 'top' and 1.6 on 'bottom')
 
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
@@ -53,6 +57,8 @@ CASES = {
     "sphere_neumann_3D": ("meshes/msh/sphere_cut.msh", "modules/poisson/check/poisson_test_ref_sphere_neumann_3D.txt",
                           1),
     "point_dirichlet_2D": ("meshes/msh/plancher.msh", "modules/poisson/check/poisson_test_point_dirichlet_2D.txt", 1),
+    "heat_plate_2D": ("meshes/msh/plate.msh", "modules/heat/check/2d_conduction.txt", 1),
+    "heat_cube_3D": ("meshes/msh/truncated_cube.msh", "modules/heat/check/3d_conduction.txt", 1),
 }
 
 # Gmsh element type -> number of nodes (only the P1 simplex family is needed)

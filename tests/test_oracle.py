@@ -10,6 +10,8 @@ wrong sign or index, a transposed operand) makes it fail.
 
 import os
 
+import ctypes
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -591,3 +593,79 @@ def test_threaded_assembly_matches_serial():
         assert np.abs(val - ref).max() <= 1e-13 * np.abs(ref).max()
         rref = O.rhs_constant_source(dim, X, C, dim + 1, 1.5)
         assert np.abs(rhs - rref).max() <= 1e-13 * np.abs(rref).max()
+
+
+# ---- heat (modules/heat/ElementMatrix.h, FemModule.cc) ---------------------------------
+
+
+def test_heat_element_mass_identities():
+    """(K_heat - K_laplace) dt is the consistent mass matrix: it sums to |T|,
+    each row to |T|/npc, and its diagonal is twice the off-diagonal."""
+    rng = np.random.default_rng(5)
+    for npc, elem_h, elem_l, meas in ((3, O.lib().afo_elem_heat_tria3, O.lib().afo_elem_laplace_tria3, 3),
+                                      (4, O.lib().afo_elem_heat_tetra4, O.lib().afo_elem_laplace_tetra4, 4)):
+        X = rng.normal(size=(npc, 3))
+        if npc == 3:
+            X[:, 2] = 0.0
+        Xc = np.ascontiguousarray(X.ravel())
+        Kh = np.zeros(npc * npc)
+        Kl = np.zeros(npc * npc)
+        elem_h.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        elem_l.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+        elem_h(Xc.ctypes.data, 1.3, 0.25, Kh.ctypes.data)
+        elem_l(Xc.ctypes.data, 1.3, Kl.ctypes.data)
+        M = ((Kh - Kl) * 0.25).reshape(npc, npc)
+        if npc == 3:
+            size = abs(np.cross(X[1] - X[0], X[2] - X[0])[2]) / 2
+        else:
+            size = abs(np.dot(X[1] - X[0], np.cross(X[2] - X[0], X[3] - X[0]))) / 6
+        tol = 1e-14 * npc * npc * np.abs(Kl).max() * 0.25  # M is a difference of two larger matrices
+        assert abs(M.sum() - size) <= tol
+        assert np.allclose(M.sum(axis=1), size / npc, rtol=0, atol=tol)
+        assert np.allclose(np.diag(M), 2 * M[0, 1], rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 6), (3, 3)])
+def test_mass_field_integrates_linear_fields(dim, n):
+    """sum_i (M f)_i = integral of f: exact for the constant 1 and for a linear field."""
+    from arcanefem_amd import box_mesh
+    X, C = box_mesh(dim, n, 0.2, seed=6)
+    one = np.ones(X.shape[0])
+    assert abs(O.rhs_mass_field(X, C, dim + 1, one, 2.0).sum() - 2.0) <= 1e-13
+    lin = 1.0 + 2.0 * X[:, 0] - X[:, 1]
+    # integral over the unit box of 1 + 2x - y = 1 + 1 - 1/2
+    assert abs(O.rhs_mass_field(X, C, dim + 1, lin, 1.0).sum() - 1.5) <= 1e-13
+
+
+HEAT = [("heat_plate_2D", 1.75, 0.4, 20.0, 30.0, [("left", 10.0)], 1e31, 52),
+        ("heat_cube_3D", 1.75, 0.1, 1.0, 30.0, [("top", 100.6), ("bottom", 1.6)], 1e30, 12)]
+
+
+@pytest.mark.parametrize("case,lam,dt,tmax,tinit,bcs,penalty,nsolve", HEAT)
+def test_golden_heat_conduction(case, lam, dt, tmax, tinit, bcs, penalty, nsolve):
+    """modules/heat tests conduction / 3d_conduction: lambda K + M/dt, RHS = M u_old/dt,
+    penalty Dirichlet; a solve at every t < tmax (t += dt in floating point) and one
+    more at the first t >= tmax, where the file is compared (eps 1e-4)."""
+    d = load_case(case)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col, element=O.HEAT, cell_lambda=np.full(C.shape[0], lam),
+                     lame_lambda=dt)
+    fi, fv, gv = np.zeros(N, np.uint8), np.zeros(N), np.zeros(N)
+    for g, v in bcs:
+        fi[d["group_" + g]], fv[d["group_" + g]], gv[d["group_" + g]] = 1, penalty, v
+    O.apply_transformations(row, col, val, np.zeros(N), fi, fv)
+    lu = spl.splu(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc())
+    u, t, solves = np.full(N, tinit), 0.0, 0
+    while True:
+        last = t >= tmax
+        b = O.rhs_mass_field(X, C, npc, u, 1.0 / dt)
+        b[fi == 1] = penalty * gv[fi == 1]
+        u = lu.solve(b)
+        solves += 1
+        if last:
+            break
+        t += dt
+    assert solves == nsolve
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, u)[:, 0], 1e-4) == 0
