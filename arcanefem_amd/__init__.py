@@ -101,6 +101,7 @@ def _declare(L):
         "afem_rhs_neumann": (ci, [vp, ci, vp, ci, vp, ci, ci, vp]),
         "afem_rhs_mass_field": (ci, [vp, vp, ctypes.c_double, ci, ci, vp]),
         "afem_bsr_assemble_heat": (ci, [vp, vp, ctypes.c_double]),
+        "afem_bsr_add_convection": (ci, [vp, ci, vp, ctypes.c_double]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
         "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
@@ -308,6 +309,12 @@ class BSRFormat:
         """lambda K + M / dt (cell_lambda a DeviceArray or None for 1)."""
         lam = cell_lambda.c if cell_lambda is not None else None
         check(lib().afem_bsr_assemble_heat(self.h, lam, float(dt)))
+
+    def add_convection(self, faces, h):
+        """Adds h times the consistent mass matrix of each boundary face
+        ((nf, dim) node ids): the heat module's convection matrix."""
+        faces = np.ascontiguousarray(faces, np.int32)
+        check(lib().afem_bsr_add_convection(self.h, int(faces.shape[0]), _ptr(faces), float(h)))
 
     def assemble_elasticity(self, lame_lambda, lame_mu):
         check(lib().afem_bsr_assemble_elasticity(self.h, float(lame_lambda), float(lame_mu)))

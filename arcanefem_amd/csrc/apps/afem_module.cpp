@@ -1,7 +1,8 @@
-// afem_module -- the Poisson and elasticity module flows of ArcaneFEM
-// (modules/poisson/FemModule.cc:75-146, modules/elasticity/FemModule.cc:57-154)
-// driven through the C++ DoFLinearSystem / BSRFormat layer and the MI355X
-// linear-system factory, on a reference Gmsh mesh or a structured box.
+// afem_module -- the Poisson, elasticity and heat module flows of ArcaneFEM
+// (modules/poisson/FemModule.cc:75-146, modules/elasticity/FemModule.cc:57-154,
+// modules/heat/FemModule.cc:74-420) driven through the C++ DoFLinearSystem /
+// BSRFormat layer and the MI355X linear-system factory, on a reference Gmsh
+// mesh or a structured box.
 //
 //   afem_module poisson --mesh circle_cut.msh --f 5.5 --dirichlet horizontal=0.5 \
 //       [--neumann curved=-0.35,1.65] \
@@ -9,6 +10,9 @@
 //   afem_module elasticity --mesh bar_dynamic_3D.msh --E 21e5 --nu 0.28 --f -1.0 \
 //       --dirichlet surfaceleft=0.0,0.0,0.0 --dirichlet surfaceright=NULL,1.0,NULL \
 //       --check bar.3D.Dirichlet.bodyForce.txt --epsilon 1e-3 --min-value 1e-10
+//   afem_module heat --mesh plate.msh --lambda 1.75 --dt 0.4 --tmax 20 --Tinit 30 \
+//       --penalty 1e31 --dirichlet left=10.0 [--convection right=1.0,20.0] \
+//       --check 2d_conduction.txt
 //
 // Exit status 0 when the solve converged and the check (if any) passed.
 #include "femutils/BSRFormat.hpp"
@@ -28,8 +32,11 @@ struct Args
   double box_jitter = 0.0;
   unsigned long long box_seed = 0;
   std::vector<double> f;
-  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann;
+  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann, convection;
   double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
+  // heat (modules/heat/Fem.axl defaults) and the Dirichlet penalty
+  // (femutils/FemBoundaryConditions.axl:104-109)
+  double lambda = 1.75, dt = 0.1, tmax = 1.0, t_init = 0.0, penalty = 1.0e30;
   bool print = false;
   bool parse_only = false; // read the mesh on the host, print a summary, exit
 };
@@ -60,7 +67,7 @@ Args parse(int argc, char** argv)
 {
   Args a;
   if (argc < 2)
-    usage("expected 'poisson' or 'elasticity'");
+    usage("expected 'poisson', 'elasticity' or 'heat'");
   a.physics = argv[1];
   for (int k = 2; k < argc; ++k) {
     std::string key = argv[k];
@@ -100,6 +107,25 @@ Args parse(int argc, char** argv)
         usage("--dirichlet expects group=value[,value,value]");
       a.dirichlet.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
     }
+    else if (key == "--convection") {
+      // group=h,Text (modules/heat/Fem.axl convection-boundary-condition)
+      const std::string s = next();
+      const size_t eq = s.rfind('=');
+      const std::vector<std::string> v = eq == std::string::npos ? std::vector<std::string>{} : split(s.substr(eq + 1), ',');
+      if (eq == std::string::npos || eq == 0 || v.size() != 2)
+        usage("--convection expects group=h,Text");
+      a.convection.push_back({ s.substr(0, eq), v });
+    }
+    else if (key == "--lambda")
+      a.lambda = std::stod(next());
+    else if (key == "--dt")
+      a.dt = std::stod(next());
+    else if (key == "--tmax")
+      a.tmax = std::stod(next());
+    else if (key == "--Tinit")
+      a.t_init = std::stod(next());
+    else if (key == "--penalty")
+      a.penalty = std::stod(next());
     else if (key == "--method")
       a.method = next();
     else if (key == "--format")
@@ -138,6 +164,106 @@ NodeGroup group_of(const Mesh& m, const std::string& name)
   return m.group(name);
 }
 
+void apply_dirichlet(const Args& a, const Mesh& mesh, int nb_dof, DoFLinearSystem& ls)
+{
+  for (auto& d : a.dirichlet) {
+    const NodeGroup& g = group_of(mesh, d.first);
+    for (size_t c = 0; c < d.second.size() && (int)c < nb_dof; ++c) {
+      if (d.second[c] == "NULL")
+        continue;
+      BoundaryConditions::applyDirichletToLhsAndRhs(a.method, g, nb_dof, (int)c, std::stod(d.second[c]), ls,
+                                                    a.penalty);
+    }
+  }
+}
+
+// modules/heat/FemModule.cc: the matrix lambda K + M/dt (and the convection
+// matrix) is assembled at the first step only (:455-488, :302-331 under
+// t <= dt - 1e-8); every step rebuilds the RHS -- convection h Text, the old
+// temperature M u_old/dt (:350-362), Neumann, Dirichlet -- and solves.  The
+// time loop stops after the solve made at the first t >= tmax (:79-82 with
+// :128-135).  The BSR values stay in place between steps, so the matrix
+// transformation (penalty diagonal, eliminated rows) runs once.
+int run_heat(const Args& a, Mesh& mesh, Runner& runner)
+{
+  if (a.dt <= 0.0)
+    usage("--dt must be positive");
+  const Int32 n = mesh.nbNode();
+  auto t0 = std::chrono::steady_clock::now();
+  Mi355xDoFLinearSystemFactory factory;
+  DoFLinearSystem ls;
+  ls.setLinearSystemFactory(&factory);
+  ls.initialize(&runner, n, "Solver");
+  char rtol_text[32];
+  std::snprintf(rtol_text, sizeof rtol_text, "%.17g", a.rtol);
+  CommandLineArguments solver_args;
+  solver_args.args = { "-ksp_rtol", rtol_text, "-afem_precision", a.precision };
+  if (a.method == "Penalty") {
+    solver_args.args.push_back("-ksp_norm_type");
+    solver_args.args.push_back("preconditioned");
+  }
+  ls.setSolverCommandLineArguments(solver_args);
+  BSRFormat bsr;
+  bsr.initialize(mesh, 1, true, a.format == "AF-BSR");
+  bsr.computeSparsity();
+
+  OwnedDoFVariable<Real> cell_lambda(mesh.nbCell()), t_old(n); // m_cell_lambda, m_node_temperature_old
+  cell_lambda.fill(a.lambda);
+  t_old.fill(a.t_init);
+  ElementParameters ep;
+  ep.cell_lambda = cell_lambda.devicePointer();
+  ep.dt = a.dt;
+  bsr.assembleBilinear(Element::Heat, ep);
+  bsr.toLinearSystem(ls);
+  std::vector<std::pair<double, double>> conv;
+  for (auto& c : a.convection) {
+    conv.push_back({ std::stod(c.second[0]), std::stod(c.second[1]) });
+    BoundaryConditions::applyConvectionToLhs(mesh, c.first, conv.back().first, bsr);
+  }
+
+  auto* impl = dynamic_cast<Mi355xDoFLinearSystemImpl*>(ls.implementation());
+  VariableDoFReal& rhs = ls.rhsVariable();
+  double t = 0.0;
+  int nb_solve = 0, iterations = 0;
+  for (;;) {
+    const bool last = t >= a.tmax;
+    rhs.fill(0.0);
+    for (size_t k = 0; k < a.convection.size(); ++k)
+      BoundaryConditions::applyConvectionToRhs(mesh, a.convection[k].first, conv[k].first, conv[k].second, rhs);
+    BoundaryConditions::integrateNodalFieldToRhs(t_old.variable(), 1.0 / a.dt, mesh, rhs);
+    for (auto& nm : a.neumann)
+      BoundaryConditions::applyNeumannToRhs(mesh, nm.first, nm.second, rhs);
+    apply_dirichlet(a, mesh, 1, ls);
+    if (nb_solve == 0)
+      ls.applyMatrixTransformation();
+    ls.applyRHSTransformation();
+    ls.solve();
+    const afem_solve_result& r = impl->lastSolve();
+    iterations += r.iterations;
+    if (!r.converged) {
+      std::fprintf(stderr, "afem_module heat: no convergence at t=%g (relres %.3e)\n", t, r.relres);
+      return 1;
+    }
+    t_old.copyFrom(ls.solutionVariable());
+    ++nb_solve;
+    if (last)
+      break;
+    t += a.dt;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  std::printf("afem_module heat: nodes=%d dofs=%d steps=%d iterations=%d t=%g time_ms=%.1f\n", n, n, nb_solve,
+              iterations, t, std::chrono::duration<double, std::milli>(t1 - t0).count());
+  std::vector<Real> u = ls.solutionVariable().toHost();
+  if (a.print)
+    for (Int32 i = 0; i < n; ++i)
+      std::printf("u[%lld] = %.15g\n", (long long)mesh.uniqueIds()[i], u[(size_t)i]);
+  if (!a.check.empty()) {
+    checkNodeResultFile(a.check, mesh.uniqueIds(), u, 1, a.epsilon, a.min_value);
+    std::printf("CHECK OK %s\n", a.check.c_str());
+  }
+  return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -170,9 +296,13 @@ int main(int argc, char** argv)
       mesh.readGmsh41(a.mesh);
     else
       mesh.buildBox(a.box_dim, a.box_n, a.box_jitter, a.box_seed);
+    if (a.physics == "heat")
+      return run_heat(a, mesh, runner);
     const bool elasticity = a.physics == "elasticity";
     if (!elasticity && a.physics != "poisson")
-      usage("physics must be poisson or elasticity");
+      usage("physics must be poisson, elasticity or heat");
+    if (!a.convection.empty())
+      usage("--convection is available for heat");
     const int nb_dof = elasticity ? mesh.dimension() : 1; // modules/elasticity/FemModule.cc:174
     const Int32 n_dofs = mesh.nbNode() * nb_dof;
     auto t0 = std::chrono::steady_clock::now();
@@ -219,14 +349,7 @@ int main(int argc, char** argv)
       usage("--neumann is available for poisson");
     for (auto& nm : a.neumann)
       BoundaryConditions::applyNeumannToRhs(mesh, nm.first, nm.second, rhs);
-    for (auto& d : a.dirichlet) {
-      const NodeGroup& g = group_of(mesh, d.first);
-      for (size_t c = 0; c < d.second.size() && (int)c < nb_dof; ++c) {
-        if (d.second[c] == "NULL")
-          continue;
-        BoundaryConditions::applyDirichletToLhsAndRhs(a.method, g, nb_dof, (int)c, std::stod(d.second[c]), ls);
-      }
-    }
+    apply_dirichlet(a, mesh, nb_dof, ls);
 
     // _solve.  Penalty rows (1e30 on the diagonal, 1e30*g in the RHS) make
     // ||b|| meaningless, so those systems stop on the preconditioned norm

@@ -1662,6 +1662,61 @@ k_neumann_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int3
   *r = acc;
 }
 
+// |F| of each face: computeLengthEdge2 (femutils/ArcaneFemFunctions.h:375-381)
+// or computeAreaTria3 (:180-188).
+__global__ void __launch_bounds__(256)
+k_face_measure(int dim, int32_t nface, const int32_t* __restrict__ faces, const double* __restrict__ coords,
+               double* __restrict__ measure)
+{
+  const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nface)
+    return;
+  const int32_t* fn = faces + (int64_t)dim * f;
+  const d3 x0 = load_node(coords, fn[0]);
+  const d3 e1 = sub(load_node(coords, fn[1]), x0);
+  if (dim == 2) {
+    measure[f] = sqrt(e1.x * e1.x + e1.y * e1.y + e1.z * e1.z);
+  }
+  else {
+    const d3 c = cross(e1, sub(load_node(coords, fn[2]), x0));
+    measure[f] = sqrt(c.x * c.x + c.y * c.y + c.z * c.z) / 2.0;
+  }
+}
+
+// values += hf * m_ab * |F| for the pairs (node, b) of every face of the node,
+// in face order (m = 2 on the diagonal, 1 off it; hf = h / 6 or h / 12): the
+// convection matrix of modules/heat/FemModule.cc:302-331.  With check_only
+// the values are not touched and *err = 1 when a pair is missing from the
+// pattern.
+__global__ void __launch_bounds__(256)
+k_convection_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int32_t* __restrict__ off,
+                    const int32_t* __restrict__ face_of, const int32_t* __restrict__ faces, int k,
+                    const double* __restrict__ measure, double hf, const int32_t* __restrict__ row_index,
+                    const int32_t* __restrict__ columns, double* __restrict__ values, int check_only,
+                    int32_t* __restrict__ err)
+{
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode)
+    return;
+  const int32_t i = node_ids[t];
+  const int32_t a = row_index[i], len = row_index[i + 1] - a;
+  const int32_t* rc = columns + a;
+  for (int32_t j = off[t]; j < off[t + 1]; ++j) {
+    const int32_t f = face_of[j];
+    const int32_t* fn = faces + (int64_t)k * f;
+    for (int b = 0; b < k; ++b) {
+      const int pos = find_col(rc, len, fn[b]);
+      const bool found = pos < len && rc[pos] == fn[b];
+      if (check_only) {
+        if (!found)
+          atomicOr(err, 1);
+      }
+      else if (found)
+        values[a + pos] += hf * (fn[b] == i ? 2.0 : 1.0) * measure[f];
+    }
+  }
+}
+
 // ---- heat: mass terms -----------------------------------------------------------------
 
 // |T| of cell c: |cross(n1 - n0, n2 - n0)| / 2 (tri) or |det| / 6 (tet), as
@@ -2213,6 +2268,40 @@ static void bsr_to_csr(afem_bsr* b)
   check_launch("csr_cols");
 }
 
+// Boundary faces of k nodes on the device with, for each node they touch (ids,
+// sorted), its faces in increasing order: face_of[off[t]..off[t+1]).  Gather
+// kernels walk it so that every sum runs in face order, with no atomics.
+struct FaceIndex
+{
+  int32_t nb_node = 0;
+  DBuf<int32_t> faces, ids, off, face_of;
+  FaceIndex(const int32_t* face_nodes, int nb_face, int k)
+  {
+    const size_t nf = (size_t)nb_face * k;
+    std::vector<int32_t> h_ids(face_nodes, face_nodes + nf);
+    std::sort(h_ids.begin(), h_ids.end());
+    h_ids.erase(std::unique(h_ids.begin(), h_ids.end()), h_ids.end());
+    std::vector<int32_t> h_off(h_ids.size() + 1, 0), h_face_of(nf);
+    auto slot = [&](int32_t v) { return std::lower_bound(h_ids.begin(), h_ids.end(), v) - h_ids.begin(); };
+    for (size_t j = 0; j < nf; ++j)
+      ++h_off[slot(face_nodes[j]) + 1];
+    for (size_t t = 0; t < h_ids.size(); ++t)
+      h_off[t + 1] += h_off[t];
+    std::vector<int32_t> fill(h_off.begin(), h_off.end() - 1);
+    for (size_t j = 0; j < nf; ++j)
+      h_face_of[fill[slot(face_nodes[j])]++] = (int32_t)(j / k);
+    nb_node = (int32_t)h_ids.size();
+    faces.alloc(nf);
+    ids.alloc(h_ids.size());
+    off.alloc(h_off.size());
+    face_of.alloc(nf);
+    faces.upload(face_nodes, nf);
+    ids.upload(h_ids.data(), h_ids.size());
+    off.upload(h_off.data(), h_off.size());
+    face_of.upload(h_face_of.data(), nf);
+  }
+};
+
 } // namespace afem
 
 using namespace afem;
@@ -2383,6 +2472,43 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
   AFEM_GUARD_END
 }
 
+AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t* face_nodes, double h)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(bsr && (nb_face == 0 || face_nodes), AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_face >= 0, AFEM_ERR_ARGUMENT, "negative face count");
+  AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_ARGUMENT, "the convection term needs nb_dof = 1");
+  AFEM_REQUIRE(bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called first");
+  const afem_mesh& m = *bsr->mesh;
+  AFEM_REQUIRE(m.npc == m.dim + 1, AFEM_ERR_NOT_IMPLEMENTED, "convection faces of simplex meshes only");
+  if (nb_face == 0)
+    return AFEM_OK;
+  const int k = m.dim;
+  for (size_t j = 0; j < (size_t)nb_face * k; ++j)
+    AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < m.nb_node, AFEM_ERR_ARGUMENT, "convection face node out of range");
+  FaceIndex fx(face_nodes, nb_face, k);
+  DBuf<double> measure((size_t)nb_face);
+  DBuf<int32_t> err(1);
+  err.zero();
+  ProfScope prof("assemble_convection");
+  hipLaunchKernelGGL(k_face_measure, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), k, nb_face, fx.faces.p,
+                     m.coords.p, measure.p);
+  check_launch("face_measure");
+  const double hf = h * (k == 2 ? 1 / 6. : 1 / 12.);
+  for (int check_only = 1; check_only >= 0; --check_only) { // a failed call leaves the values as they were
+    hipLaunchKernelGGL(k_convection_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
+                       fx.ids.p, fx.off.p, fx.face_of.p, fx.faces.p, k, measure.p, hf, bsr->row_index.p,
+                       bsr->columns.p, bsr->values.p, check_only, err.p);
+    check_launch("convection_gather");
+    if (check_only) {
+      int32_t e = 0;
+      err.download(&e, 1);
+      AFEM_REQUIRE(e == 0, AFEM_ERR_NOT_FOUND, "a convection face pair is not in the matrix pattern");
+    }
+  }
+  AFEM_GUARD_END
+}
+
 AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
                               const double* values, int nb_dof, int dof_index, double* rhs)
 {
@@ -2400,37 +2526,22 @@ AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face
   for (size_t j = 0; j < nf; ++j)
     AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < mesh->nb_node, AFEM_ERR_ARGUMENT,
                  "Neumann face node out of range");
-  // nodes of the faces and, per node, its faces in increasing order
-  std::vector<int32_t> ids(face_nodes, face_nodes + nf);
-  std::sort(ids.begin(), ids.end());
-  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
-  std::vector<int32_t> off(ids.size() + 1, 0), face_of(nf);
-  for (size_t j = 0; j < nf; ++j)
-    ++off[std::lower_bound(ids.begin(), ids.end(), face_nodes[j]) - ids.begin() + 1];
-  for (size_t t = 0; t < ids.size(); ++t)
-    off[t + 1] += off[t];
-  std::vector<int32_t> fill(off.begin(), off.end() - 1);
-  for (size_t j = 0; j < nf; ++j)
-    face_of[fill[std::lower_bound(ids.begin(), ids.end(), face_nodes[j]) - ids.begin()]++] = (int32_t)(j / k);
-  DBuf<int32_t> d_faces(nf), d_ids(ids.size()), d_off(off.size()), d_face_of(nf), err(1);
+  FaceIndex fx(face_nodes, nb_face, k);
+  DBuf<int32_t> err(1);
   DBuf<double> contrib((size_t)nb_face);
-  d_faces.upload(face_nodes, nf);
-  d_ids.upload(ids.data(), ids.size());
-  d_off.upload(off.data(), off.size());
-  d_face_of.upload(face_of.data(), nf);
   err.zero();
   const bool scalar = nb_value == 1;
   ProfScope prof("rhs_neumann");
   hipLaunchKernelGGL(k_neumann_faces, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), mesh->dim, nb_face,
-                     d_faces.p, mesh->coords.p, mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, scalar ? 1 : 0,
+                     fx.faces.p, mesh->coords.p, mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, scalar ? 1 : 0,
                      values[0], scalar ? 0.0 : values[0], scalar ? 0.0 : values[1],
                      (!scalar && k == 3) ? values[2] : 0.0, contrib.p, err.p);
   check_launch("neumann_faces");
   int32_t h = 0;
   err.download(&h, 1); // before rhs is touched: a failed call leaves it as it was
   AFEM_REQUIRE(h == 0, AFEM_ERR_MESH, "a Neumann face is not a face of any cell");
-  hipLaunchKernelGGL(k_neumann_gather, dim3(grid_for((int64_t)ids.size(), 256)), dim3(256), 0, stream(),
-                     (int32_t)ids.size(), d_ids.p, d_off.p, d_face_of.p, contrib.p, nb_dof, dof_index, rhs);
+  hipLaunchKernelGGL(k_neumann_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
+                     fx.ids.p, fx.off.p, fx.face_of.p, contrib.p, nb_dof, dof_index, rhs);
   check_launch("neumann_gather");
   AFEM_GUARD_END
 }

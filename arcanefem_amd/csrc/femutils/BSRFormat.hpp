@@ -19,8 +19,9 @@ namespace afem::femutils {
 
 enum class Element
 {
-  Laplace,   // modules/poisson/ElementMatrix.h, modules/fourier/ElementMatrix.h
-  Elasticity // modules/elasticity/ElementMatrix.h:151-301
+  Laplace,    // modules/poisson/ElementMatrix.h, modules/fourier/ElementMatrix.h
+  Elasticity, // modules/elasticity/ElementMatrix.h:151-301
+  Heat        // lambda K + M / dt, modules/heat/ElementMatrix.h:28-39, :80-91
 };
 
 struct ElementParameters
@@ -30,6 +31,7 @@ struct ElementParameters
   Real* rhs = nullptr;               // device array written when source_f is used
   Real lame_lambda = 0.0;
   Real lame_mu = 0.0;
+  Real dt = 0.0; // time step of the heat element
 };
 
 class BSRFormat
@@ -99,6 +101,8 @@ class BSRFormat
   {
     if (e == Element::Laplace)
       check(afem_bsr_assemble_laplace(handle(), p.cell_lambda, p.source_f, p.rhs), "BSRFormat(assembleBilinear)");
+    else if (e == Element::Heat)
+      check(afem_bsr_assemble_heat(handle(), p.cell_lambda, p.dt), "BSRFormat(assembleBilinear)");
     else
       check(afem_bsr_assemble_elasticity(handle(), p.lame_lambda, p.lame_mu), "BSRFormat(assembleBilinear)");
   }
@@ -113,6 +117,16 @@ namespace BoundaryConditions {
   {
     check(afem_rhs_constant_source(mesh.handle(), qdot, nullptr, nb_dof, dof_index, rhs.devicePointer()),
           "applyConstantSourceToRhs");
+  }
+
+  // femutils/ArcaneFemFunctions.h:2558-2590 (tri3), :1628-1660 (tet4):
+  // rhs += M (scale * field), M the consistent P1 mass matrix; the heat
+  // module passes u_old with scale = 1 / dt.
+  inline void integrateNodalFieldToRhs(const VariableDoFReal& field, Real scale, const Mesh& mesh,
+                                       VariableDoFReal& rhs)
+  {
+    check(afem_rhs_mass_field(mesh.handle(), field.devicePointer(), scale, 1, 0, rhs.devicePointer()),
+          "integrateNodalFieldToRhs");
   }
 
   // modules/elasticity/BodyForce.h:149-161
@@ -142,6 +156,27 @@ namespace BoundaryConditions {
     check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), (int)v.size(),
                            v.data(), 1, 0, rhs.devicePointer()),
           "applyNeumannToRhs");
+  }
+
+  // modules/heat/FemModule.cc:302-331, matrix part (first time step only):
+  // h times the consistent mass matrix of each face of the group.
+  inline void applyConvectionToLhs(const Mesh& mesh, const std::string& group, Real h, BSRFormat& bsr)
+  {
+    const std::vector<Int32>& faces = mesh.faceGroup(group);
+    check(afem_bsr_add_convection(bsr.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), h),
+          "applyConvectionToLhs");
+  }
+
+  // modules/heat/FemModule.cc:333-337, RHS part: (h Text) |F| / dim at each
+  // node of each face -- a scalar Neumann flux h * Text.
+  inline void applyConvectionToRhs(const Mesh& mesh, const std::string& group, Real h, Real t_ext,
+                                   VariableDoFReal& rhs)
+  {
+    const std::vector<Int32>& faces = mesh.faceGroup(group);
+    const Real g = h * t_ext;
+    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), 1, &g, 1, 0,
+                           rhs.devicePointer()),
+          "applyConvectionToRhs");
   }
 
   // femutils/ArcaneFemFunctionsGpu.cc:115-147 for one component: method is

@@ -65,6 +65,37 @@ using VariableDoFReal = DoFVariable<Real>;
 using VariableDoFByte = DoFVariable<Byte>;
 using VariableDoFBool = DoFVariable<Byte>;
 
+// A device array the caller owns, seen as a DoF variable: the node and cell
+// fields of a module (the heat module's old temperature, per-cell lambda).
+template <typename T>
+class OwnedDoFVariable
+{
+ public:
+  explicit OwnedDoFVariable(Int32 n)
+  {
+    void* p = nullptr;
+    check(afem_malloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)), "afem_malloc");
+    m_var = DoFVariable<T>(static_cast<T*>(p), n);
+  }
+  OwnedDoFVariable(const OwnedDoFVariable&) = delete;
+  OwnedDoFVariable& operator=(const OwnedDoFVariable&) = delete;
+  ~OwnedDoFVariable() { afem_free(m_var.devicePointer()); }
+  DoFVariable<T>& variable() { return m_var; }
+  const DoFVariable<T>& variable() const { return m_var; }
+  T* devicePointer() const { return m_var.devicePointer(); }
+  void fill(T value) { m_var.fill(value); }
+  void copyFrom(const DoFVariable<T>& src)
+  {
+    if (src.size() != m_var.size())
+      throw FatalError(AFEM_ERR_ARGUMENT, "DoF variable size mismatch");
+    check(afem_memcpy_dtod(m_var.devicePointer(), src.devicePointer(), sizeof(T) * (size_t)src.size()),
+          "afem_memcpy_dtod");
+  }
+
+ private:
+  DoFVariable<T> m_var;
+};
+
 // Non-owning CSR view of device arrays (rows carry nb_row + 1 offsets here).
 struct CsrFormatView
 {

@@ -187,6 +187,16 @@ AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int3
  * sum over cells T of n of f*scale_T*|T|/npc.  cell_scale [device] or NULL. */
 AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* cell_scale,
                                       int nb_dof, int dof_index, double* rhs);
+/* Convection term of the heat module on boundary faces (modules/heat/
+ * FemModule.cc:302-331, the matrix part, added once before the time loop):
+ * for each face F (face_nodes [host]: nb_face * dim node ids) and each pair
+ * (a, b) of its nodes, A[a][b] += h c m_ab |F| with m_ab = 2 (a = b) or 1 and
+ * c = 1/6 (edges) or 1/12 (triangles) -- h times the face's consistent mass
+ * matrix.  Summed per entry in face order (no atomics).  The RHS part,
+ * (h Text) |F| / dim at each face node, is afem_rhs_neumann with the scalar
+ * h * Text.  nb_dof = 1; AFEM_ERR_NOT_FOUND when a pair is outside the
+ * pattern (a face that is not a face of the mesh), values untouched. */
+AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t* face_nodes, double h);
 /* BoundaryConditions::applyNeumannToRhs for tri3 / tet4 meshes
  * (femutils/ArcaneFemFunctions.h:2715-2759 edges, :1793-1840 triangles; GPU
  * femutils/ArcaneFemFunctionsGpu.cc:679-738): for each boundary face F

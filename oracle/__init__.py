@@ -64,6 +64,8 @@ def lib():
         L.afo_rhs_neumann.restype = ctypes.c_int
         L.afo_rhs_neumann.argtypes = [ctypes.c_int, vp, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
                                       ctypes.c_int, vp]
+        L.afo_convection_lhs.restype = ctypes.c_int
+        L.afo_convection_lhs.argtypes = [ctypes.c_int, vp, i32, vp, dbl, vp, vp, vp]
         L.afo_apply_transformations.restype = ctypes.c_int
         L.afo_apply_transformations.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.afo_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
@@ -196,6 +198,16 @@ def rhs_neumann(dim, coords, cells, faces, value, nb_dof=1, dof_index=0, rhs=Non
     if rc != 0:
         raise ValueError("a Neumann face is not a face of any cell")
     return rhs
+
+
+def convection_lhs(dim, coords, faces, h, row, cols, values):
+    """Adds the heat module's boundary convection matrix h * M_F to values (in place)."""
+    coords, faces, row, cols = _f64(coords), _i32(faces), _i32(row), _i32(cols)
+    assert values.dtype == np.float64 and values.flags.c_contiguous
+    rc = lib().afo_convection_lhs(dim, _P(coords), faces.shape[0], _P(faces), h, _P(row), _P(cols), _P(values))
+    if rc != 0:
+        raise ValueError("convection: a face pair is not in the pattern")
+    return values
 
 
 def max_threads():

@@ -602,6 +602,41 @@ int afo_rhs_neumann(int dim, const double* coords, int32_t nb_cell, const int32_
   return 0;
 }
 
+/* modules/heat/FemModule.cc:302-331: K_e = h * factor * massMatrix(U, U) * measure
+ * with U all ones (FemUtils.h:584-596 doubles the diagonal), measure
+ * computeLengthEdge2 (ArcaneFemFunctions.h:375-381) or computeAreaTria3
+ * (:180-188), added entry by entry with matrixAddValue. */
+int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int32_t* faces, double h,
+                       const int32_t* row_index, const int32_t* columns, double* values)
+{
+  const int k = dim;
+  const double factor = dim == 2 ? 1 / 6. : 1 / 12.;
+  for (int32_t f = 0; f < nb_face; ++f) {
+    const int32_t* fn = faces + (int64_t)f * k;
+    const double* v0 = coords + 3 * (int64_t)fn[0];
+    const double* v1 = coords + 3 * (int64_t)fn[1];
+    double e1[3], measure;
+    sub3(v1, v0, e1);
+    if (dim == 2)
+      measure = sqrt(dot3(e1, e1));
+    else {
+      double e2[3], c[3];
+      sub3(coords + 3 * (int64_t)fn[2], v0, e2);
+      cross3(e1, e2, c);
+      measure = sqrt(dot3(c, c)) / 2.0;
+    }
+    for (int a = 0; a < k; ++a)
+      for (int b = 0; b < k; ++b) {
+        const double m = a == b ? 2.0 : 1.0;
+        const int32_t x = find_in_row(row_index, columns, fn[a], fn[b]);
+        if (x < 0)
+          return -1;
+        values[row_index[fn[a]] + x] += h * factor * m * measure;
+      }
+  }
+  return 0;
+}
+
 /* ---- transformations ------------------------------------------------------------ */
 
 int afo_apply_transformations(int32_t nb_row, const int32_t* row_index, const int32_t* columns,

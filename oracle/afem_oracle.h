@@ -136,6 +136,15 @@ void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16
 void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* cells,
                         const double f[3], double* rhs);
 
+/* Convection term of the heat module on boundary faces (dim nodes each):
+ * A[a][b] += h * c * m_ab * |F| for every pair of nodes of F, with m the matrix
+ * of FemUtils.h:584-596 (1 off the diagonal, 2 on it) and c = 1/6 (edges) or
+ * 1/12 (triangles).  modules/heat/FemModule.cc:302-331.  The RHS part of the
+ * same condition, (h * Text) |F| / dim per face node, is afo_rhs_neumann with
+ * the scalar g = h * Text.  Returns -1 when a pair is not in the pattern. */
+int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int32_t* faces, double h,
+                       const int32_t* row_index, const int32_t* columns, double* values);
+
 /* Neumann condition on boundary faces (dim nodes each): rhs[dof(n)] += g*|F|/dim
  * (nb_value 1) or (g.n_F)*|F|/dim (nb_value dim, n_F the outward unit normal).
  * femutils/ArcaneFemFunctions.h:2715-2759 (edges), :1793-1840 (triangles).

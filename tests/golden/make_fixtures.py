@@ -34,6 +34,15 @@ Golden cases (test names and tolerances from the reference's CMake files):
               dt 0.4, tmax 20, Tinit 30, penalty 1e31 Dirichlet 10 on 'left', eps 1e-4)
   heat_cube_3D   modules/heat/inputs/3d_conduction.arc, check/3d_conduction.txt (lambda 1.75,
               dt 0.1, tmax 1, Tinit 30, Dirichlet 100.6 on 'top' and 1.6 on 'bottom')
+  heat_convection_2D  modules/heat/inputs/conduction.convection.arc,
+              check/2d_conduction_convection.txt (as heat_plate_2D plus convection
+              h 1, Text 20 on 'right', 'top', 'bottom')
+  heat_neumann_point_2D  modules/heat/inputs/conduction.neumann.point-dirichlet.arc,
+              check/2d_conduction_neumann_pointBC.txt (penalty 1e31 at the points
+              topLeft 1.8 and botRight 31, Neumann 9.6 on 'left')
+  heat_convection_point_3D  modules/heat/inputs/3d_conduction.pointBc.convection.arc,
+              check/3d_conduction_convection_pointBC.txt (convection h 1.9, Text 30 on
+              'top', Dirichlet 2.6 on 'bottom', point 'center' 100.6, penalty 1e30)
 
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
@@ -59,6 +68,10 @@ CASES = {
     "point_dirichlet_2D": ("meshes/msh/plancher.msh", "modules/poisson/check/poisson_test_point_dirichlet_2D.txt", 1),
     "heat_plate_2D": ("meshes/msh/plate.msh", "modules/heat/check/2d_conduction.txt", 1),
     "heat_cube_3D": ("meshes/msh/truncated_cube.msh", "modules/heat/check/3d_conduction.txt", 1),
+    "heat_convection_2D": ("meshes/msh/plate.msh", "modules/heat/check/2d_conduction_convection.txt", 1),
+    "heat_neumann_point_2D": ("meshes/msh/plate.msh", "modules/heat/check/2d_conduction_neumann_pointBC.txt", 1),
+    "heat_convection_point_3D": ("meshes/msh/truncated_cube.msh",
+                                 "modules/heat/check/3d_conduction_convection_pointBC.txt", 1),
 }
 
 # Gmsh element type -> number of nodes (only the P1 simplex family is needed)

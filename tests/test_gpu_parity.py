@@ -549,3 +549,42 @@ def test_neumann_matches_oracle(afem_gpu, dim, n, value):
     with pytest.raises(A.AfemError):
         A.rhs_neumann(m, np.array([[0, N - 1] + ([1] if dim == 3 else [])], np.int32), g, rhs)
     assert np.array_equal(rhs.numpy(), before)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 9), (3, 5)])
+def test_heat_assembly_matches_oracle(afem_gpu, dim, n):
+    """lambda K + M/dt with a per-cell lambda, every assembly variant, then the
+    convection matrix on part of the boundary and the mass-weighted old field
+    RHS: values and RHS within 1e-12 of the row's / vector's largest entry."""
+    A = afem_gpu
+    print(f"[case] heat dim={dim}", flush=True)
+    X, C = box(dim, n, 0.2)
+    N, dt = X.shape[0], 0.4
+    m = A.Mesh(dim, X, C)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    orow, ocol = O.pattern(N, C, dim + 1)
+    lam = 1.0 + np.random.default_rng(3).random(C.shape[0])
+    dlam = A.DeviceArray.from_numpy(lam)
+    oval = O.assemble(dim, X, C, dim + 1, orow, ocol, element=O.HEAT, cell_lambda=lam, lame_lambda=dt)
+    for v in range(-1, 14):
+        b.set_assembly_variant(v)
+        b.assemble_heat(dlam, dt)
+        _, _, val, _ = b.download()
+        assert row_scaled_error(orow, oval, val) <= 1e-12, v
+    F = _boundary_faces(C, dim)
+    side = F[np.all(X[F][:, :, 0] == 0.0, axis=1)]
+    b.add_convection(side, 1.9)
+    O.convection_lhs(dim, X, side, 1.9, orow, ocol, oval)
+    _, _, val, _ = b.download()
+    assert row_scaled_error(orow, oval, val) <= 1e-12
+    with pytest.raises(A.AfemError):
+        b.add_convection(np.array([[0, N - 1] + ([1] if dim == 3 else [])], np.int32), 1.0)
+    assert np.array_equal(b.download()[2], val)
+    u = 30.0 + np.sin(X[:, 0] * 3.0) * X[:, 1]
+    rhs = A.DeviceArray(N, np.float64)
+    A.rhs_neumann(m, side, 1.9 * 20.0, rhs)
+    A.rhs_mass_field(m, A.DeviceArray.from_numpy(u), 1.0 / dt, rhs)
+    ref = O.rhs_neumann(dim, X, C, side, 1.9 * 20.0)
+    ref = O.rhs_mass_field(X, C, dim + 1, u, 1.0 / dt, rhs=ref)
+    assert np.abs(rhs.numpy() - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -40,6 +40,7 @@ def test_binary_built():
     (["poisson", "--mesh"], "missing value"),
     (["poisson", "--box", "3", "4", "0", "0", "--bogus"], "unknown option"),
     (["poisson", "--mesh", "x.msh", "--dirichlet", "nogroupvalue"], "--dirichlet expects"),
+    (["heat", "--mesh", "x.msh", "--convection", "top=1.0"], "--convection expects"),
 ])
 def test_usage_errors(args, msg):
     r = run(args)
@@ -139,6 +140,47 @@ def test_poisson_module_point_dirichlet_golden(method, fmt, precision):
              "--method", method, "--format", fmt, "--precision", precision,
              "--check", os.path.join(GOLDEN, "poisson_test_point_dirichlet_2D.txt"), "--epsilon", "1e-4"])
     _check_ok(r)
+
+
+# modules/heat/inputs/*.arc (lambda 1.75, Tinit 30): mesh, dt, tmax, penalty,
+# conditions, check file, number of solves
+HEAT = [
+    ("plate.msh", "0.4", "20", "1e31", ["--dirichlet", "left=10.0"], "2d_conduction.txt", 52),
+    ("truncated_cube.msh", "0.1", "1", "1e30", ["--dirichlet", "top=100.6", "--dirichlet", "bottom=1.6"],
+     "3d_conduction.txt", 12),
+    ("plate.msh", "0.4", "20", "1e31", ["--dirichlet", "left=10.0", "--convection", "right=1.0,20.0",
+                                        "--convection", "top=1.0,20.0", "--convection", "bottom=1.0,20.0"],
+     "2d_conduction_convection.txt", 52),
+    ("plate.msh", "0.4", "20", "1e31", ["--dirichlet-point", "topLeft=1.8", "--dirichlet-point", "botRight=31.0",
+                                        "--neumann", "left=9.6"], "2d_conduction_neumann_pointBC.txt", 52),
+    ("truncated_cube.msh", "0.1", "1", "1e30", ["--convection", "top=1.9,30.0", "--dirichlet", "bottom=2.6",
+                                                "--dirichlet-point", "center=100.6"],
+     "3d_conduction_convection_pointBC.txt", 12),
+]
+HEAT_SOLVES = [("Penalty", "BSR", "fp64"), ("Penalty", "AF-BSR", "fp64"), ("RowElimination", "BSR", "fp64"),
+               ("RowElimination", "BSR", "mixed")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msh,dt,tmax,penalty,conds,ref,steps", HEAT)
+@pytest.mark.parametrize("method,fmt,precision", HEAT_SOLVES)
+def test_heat_module_golden(msh, dt, tmax, penalty, conds, ref, steps, method, fmt, precision):
+    # modules/heat: transient conduction, backward Euler; the matrix is
+    # assembled once and every step solves with the old temperature's RHS
+    r = run(["heat", "--mesh", os.path.join(GOLDEN, msh), "--lambda", "1.75", "--dt", dt, "--tmax", tmax,
+             "--Tinit", "30.0", "--penalty", penalty, "--method", method, "--format", fmt, "--precision", precision,
+             "--check", os.path.join(GOLDEN, ref), "--epsilon", "1e-4"] + conds)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "CHECK OK" in r.stdout and f"steps={steps} " in r.stdout, r.stdout
+
+
+@pytest.mark.gpu
+def test_heat_module_check_detects_missing_convection():
+    # the convection case without its convection terms must fail its check
+    r = run(["heat", "--mesh", os.path.join(GOLDEN, "plate.msh"), "--lambda", "1.75", "--dt", "0.4", "--tmax", "20",
+             "--Tinit", "30.0", "--penalty", "1e31", "--dirichlet", "left=10.0",
+             "--check", os.path.join(GOLDEN, "2d_conduction_convection.txt"), "--epsilon", "1e-4"])
+    assert r.returncode == 1 and "CHECK OK" not in r.stdout
 
 
 @pytest.mark.gpu

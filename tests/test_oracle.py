@@ -637,35 +637,98 @@ def test_mass_field_integrates_linear_fields(dim, n):
     assert abs(O.rhs_mass_field(X, C, dim + 1, lin, 1.0).sum() - 1.5) <= 1e-13
 
 
-HEAT = [("heat_plate_2D", 1.75, 0.4, 20.0, 30.0, [("left", 10.0)], 1e31, 52),
-        ("heat_cube_3D", 1.75, 0.1, 1.0, 30.0, [("top", 100.6), ("bottom", 1.6)], 1e30, 12)]
+# modules/heat/inputs/*.arc (lambda 1.75, Tinit 30 throughout): name, dt, tmax,
+# Dirichlet (group, value) incl. named points, penalty, convection (group, h,
+# Text), Neumann (group, g), number of solves
+HEAT = [
+    ("heat_plate_2D", 0.4, 20.0, [("left", 10.0)], 1e31, [], [], 52),
+    ("heat_cube_3D", 0.1, 1.0, [("top", 100.6), ("bottom", 1.6)], 1e30, [], [], 12),
+    ("heat_convection_2D", 0.4, 20.0, [("left", 10.0)], 1e31,
+     [("right", 1.0, 20.0), ("top", 1.0, 20.0), ("bottom", 1.0, 20.0)], [], 52),
+    ("heat_neumann_point_2D", 0.4, 20.0, [("topLeft", 1.8), ("botRight", 31.0)], 1e31, [], [("left", 9.6)], 52),
+    ("heat_convection_point_3D", 0.1, 1.0, [("bottom", 2.6), ("center", 100.6)], 1e30, [("top", 1.9, 30.0)], [],
+     12),
+]
 
 
-@pytest.mark.parametrize("case,lam,dt,tmax,tinit,bcs,penalty,nsolve", HEAT)
-def test_golden_heat_conduction(case, lam, dt, tmax, tinit, bcs, penalty, nsolve):
-    """modules/heat tests conduction / 3d_conduction: lambda K + M/dt, RHS = M u_old/dt,
-    penalty Dirichlet; a solve at every t < tmax (t += dt in floating point) and one
-    more at the first t >= tmax, where the file is compared (eps 1e-4)."""
-    d = load_case(case)
+def heat_oracle_run(d, dt, tmax, dirichlet, penalty, convection, neumann, lam=1.75, tinit=30.0):
+    """The heat module's time loop (modules/heat/FemModule.cc:74-110, 276-420):
+    lambda K + M/dt and the convection matrix assembled once; every step
+    rhs = convection (h Text) + M u_old/dt + Neumann, penalty rows, solve; a
+    solve at every t < tmax (t += dt in floating point) and one more at the
+    first t >= tmax.  Returns (u, number of solves)."""
     X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
     N = X.shape[0]
     row, col = O.pattern(N, C, npc)
     val = O.assemble(dim, X, C, npc, row, col, element=O.HEAT, cell_lambda=np.full(C.shape[0], lam),
                      lame_lambda=dt)
+    for g, h, _ in convection:
+        O.convection_lhs(dim, X, d["faces_" + g], h, row, col, val)
     fi, fv, gv = np.zeros(N, np.uint8), np.zeros(N), np.zeros(N)
-    for g, v in bcs:
+    for g, v in dirichlet:
         fi[d["group_" + g]], fv[d["group_" + g]], gv[d["group_" + g]] = 1, penalty, v
     O.apply_transformations(row, col, val, np.zeros(N), fi, fv)
     lu = spl.splu(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc())
     u, t, solves = np.full(N, tinit), 0.0, 0
     while True:
         last = t >= tmax
-        b = O.rhs_mass_field(X, C, npc, u, 1.0 / dt)
+        b = np.zeros(N)
+        for g, h, text in convection:
+            O.rhs_neumann(dim, X, C, d["faces_" + g], h * text, rhs=b)
+        O.rhs_mass_field(X, C, npc, u, 1.0 / dt, rhs=b)
+        for g, flux in neumann:
+            O.rhs_neumann(dim, X, C, d["faces_" + g], flux, rhs=b)
         b[fi == 1] = penalty * gv[fi == 1]
         u = lu.solve(b)
         solves += 1
         if last:
             break
         t += dt
+    return u, solves
+
+
+@pytest.mark.parametrize("case,dt,tmax,dirichlet,penalty,convection,neumann,nsolve", HEAT)
+def test_golden_heat_conduction(case, dt, tmax, dirichlet, penalty, convection, neumann, nsolve):
+    """modules/heat tests conduction, 3d_conduction, conduction_convection,
+    conduction_neumann_pointBC and 3d_conduction_convection_pointBC: the
+    reference's check files within their eps 1e-4."""
+    d = load_case(case)
+    u, solves = heat_oracle_run(d, dt, tmax, dirichlet, penalty, convection, neumann)
     assert solves == nsolve
     assert O.count_differences(d["ref_value"][:, 0], by_uid(d, u)[:, 0], 1e-4) == 0
+
+
+def test_heat_golden_needs_each_term():
+    """Dropping the convection matrix, the convection RHS or the Neumann flux
+    moves the solution off the reference's files: each term is pinned."""
+    d = load_case("heat_convection_2D")
+    u_full, _ = heat_oracle_run(d, 0.4, 20.0, [("left", 10.0)], 1e31, [("right", 1.0, 20.0)], [])
+    u_none, _ = heat_oracle_run(d, 0.4, 20.0, [("left", 10.0)], 1e31, [], [])
+    assert np.abs(u_full - u_none).max() > 1e-2
+    d = load_case("heat_neumann_point_2D")
+    u, _ = heat_oracle_run(d, 0.4, 20.0, [("topLeft", 1.8), ("botRight", 31.0)], 1e31, [], [])
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, u)[:, 0], 1e-4) > 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_convection_matrix_is_face_mass(dim):
+    """One face: the added matrix is h times the face's consistent mass matrix
+    (sums to h |F|, diagonal twice the off-diagonal); the convection RHS
+    h Text |F| / dim integrates to h Text |F|."""
+    if dim == 2:
+        X = np.array([[0.0, 0, 0], [3, 0, 0], [0, 4, 0]])
+        C = np.array([[0, 1, 2, -1]], np.int32)
+        face, meas = np.array([[1, 2]], np.int32), 5.0
+    else:
+        X = np.array([[0.0, 0, 0], [2, 0, 0], [0, 2, 0], [0, 0, 2]])
+        C = np.array([[0, 1, 2, 3]], np.int32)
+        face, meas = np.array([[0, 1, 2]], np.int32), 2.0
+    row, col = O.pattern(X.shape[0], C, dim + 1)
+    val = np.zeros(int(row[-1]))
+    O.convection_lhs(dim, X, face, 1.5, row, col, val)
+    A = sp.csr_matrix((val, col, row), shape=(X.shape[0],) * 2).toarray()
+    F = A[np.ix_(face[0], face[0])]
+    assert abs(A.sum() - 1.5 * meas) <= 1e-14 and abs(F.sum() - 1.5 * meas) <= 1e-14
+    assert np.allclose(np.diag(F), 2 * F[0, 1], rtol=1e-15)
+    b = O.rhs_neumann(dim, X, C, face, 1.5 * 20.0)
+    assert abs(b.sum() - 1.5 * 20.0 * meas) <= 1e-12
