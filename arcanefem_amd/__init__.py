@@ -212,6 +212,10 @@ class DeviceArray:
     def zero(self):
         check(lib().afem_memset(ctypes.c_void_p(self.ptr), 0, self.nbytes))
 
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes, (other.nbytes, self.nbytes)
+        check(lib().afem_memcpy_dtod(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), self.nbytes))
+
     @property
     def c(self):
         return ctypes.c_void_p(self.ptr)
@@ -598,6 +602,28 @@ class BatchSolver:
         except Exception:
             pass
         self.h = None
+
+
+def heat_batch_steps(mesh, sysm, batch, source, dt, nb_step, **solve_kw):
+    """Backward-Euler time loop of the heat module (modules/heat/FemModule.cc:
+    74-110, 344-362) over every system of a batch at once.  The matrix
+    lambda K + M/dt is assembled on the union mesh, set on sysm and
+    transformed (Dirichlet) by the caller; the solution array holds the
+    initial temperature.  Each step: rhs = source + M u_old / dt, the RHS
+    transformation, one batched solve started from u_old.  source: a
+    DeviceArray (the constant f integrated once) or None.  Returns the batch
+    results and solve time of every step."""
+    out = []
+    rhs, x = sysm.rhs(), sysm.solution()
+    for _ in range(nb_step):
+        if source is not None:
+            rhs.copy_from(source)
+        else:
+            rhs.zero()
+        rhs_mass_field(mesh, x, 1.0 / dt, rhs)
+        sysm.apply_rhs_transformation()
+        out.append(batch.solve(use_initial_guess=1, **solve_kw))
+    return out
 
 
 def profile_enable(on=True):

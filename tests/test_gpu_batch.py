@@ -164,3 +164,59 @@ def test_batch_argument_errors(afem_gpu):
     bs = A.BatchSolver(case["sys"], case["offs"])
     with pytest.raises(A.AfemError):
         bs.set_tiles(3, 1)
+
+
+def test_batch_heat_time_loop_matches_oracle(afem_gpu):
+    """The heat module's backward-Euler loop on every system of a batch at
+    once (lambda K + M/dt assembled on the union mesh, rhs = f + M u_old/dt,
+    u = 0 rows eliminated, warm-started batched solves) against the oracle's
+    loop on each mesh alone with direct solves: 1e-8 relative per system."""
+    import oracle as O
+    A = afem_gpu
+    print("[case] batch heat time loop", flush=True)
+    ns, dt, steps = [6, 11, 17, 24], 0.02, 6
+    rng = np.random.default_rng(8)
+    parts, lams, bnd = [], [], []
+    for k, n in enumerate(ns):
+        X, C = A.box_mesh(2, n, 0.2, seed=40 + k)
+        parts.append((X, C))
+        lams.append(checkerboard(X, C, np.exp(rng.uniform(np.log(0.1), np.log(10.0), 64))))
+        on = (X[:, 0] == 0.0) | (X[:, 0] == 1.0) | (X[:, 1] == 0.0) | (X[:, 1] == 1.0)
+        bnd.append(np.nonzero(on)[0])
+    X, C, offs = A.union_mesh(parts)
+    N = X.shape[0]
+    nodes = np.concatenate([b + offs[k] for k, b in enumerate(bnd)]).astype(np.int32)
+    mesh = A.Mesh(2, X, C)
+    bsr = A.BSRFormat(mesh, 1, use_csr=True)
+    bsr.compute_sparsity()
+    sysm = A.DoFLinearSystem(N)
+    bsr.assemble_heat(A.DeviceArray.from_numpy(np.concatenate(lams)), dt)
+    sysm.set_bsr(bsr)
+    sysm.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, nodes, 1, 0, 0.0, 0.0)
+    sysm.apply_matrix_transformation()
+    source = A.DeviceArray(N, np.float64)
+    A.check(A.lib().afem_rhs_constant_source(mesh.h, 1.0, None, 1, 0, source.c))
+    sysm.solution().zero()  # Tinit = 0
+    bs = A.BatchSolver(sysm, offs)
+    out = A.heat_batch_steps(mesh, sysm, bs, source, dt, steps, rtol=1e-11, max_iter=4000)
+    assert len(out) == steps and all(r.converged for res, _ in out for r in res)
+    x = sysm.solution().numpy()
+    for s, (Xs, Cs) in enumerate(parts):
+        n = Xs.shape[0]
+        row, col = O.pattern(n, Cs, 3)
+        val = O.assemble(2, Xs, Cs, 3, row, col, element=O.HEAT, cell_lambda=lams[s], lame_lambda=dt)
+        ei, ev = np.zeros(n, np.uint8), np.zeros(n)
+        ei[bnd[s]] = 1
+        O.apply_transformations(row, col, val, np.zeros(n), np.zeros(n, np.uint8), np.zeros(n), ei, ev)
+        lu = spla.splu(sp.csr_matrix((val, col, row), shape=(n, n)).tocsc())
+        f = O.rhs_constant_source(2, Xs, Cs, 3, 1.0)
+        u = np.zeros(n)
+        for _ in range(steps):
+            b = O.rhs_mass_field(Xs, Cs, 3, u, 1.0 / dt, rhs=f.copy())
+            b[bnd[s]] = 0.0
+            u = lu.solve(b)
+        xs = x[offs[s]:offs[s + 1]]
+        assert np.linalg.norm(xs - u) <= 1e-8 * np.linalg.norm(u), s
+    # warm starts: the last step needs no more iterations than the first
+    its = [max(r.iterations for r in res) for res, _ in out]
+    assert its[-1] <= its[0]
