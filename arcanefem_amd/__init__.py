@@ -253,17 +253,55 @@ def box_mesh(dim, n, jitter=0.0, seed=0):
     return coords, cells
 
 
+def box_mesh_q1(dim, n, jitter=0.0, seed=0):
+    """Jittered structured Quad4 (dim 2) / Hexa8 (dim 3) mesh of the unit box,
+    n cells per side: nodes x-fastest, quads counter-clockwise from the
+    lower-left corner, hexas the bottom face counter-clockwise then the top
+    face (Gmsh / Arcane local order).  Interior nodes move uniformly by up to
+    jitter * h per coordinate."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.0, 1.0, n + 1)
+    h = 1.0 / n
+    if dim == 2:
+        y, x = np.meshgrid(g, g, indexing="ij")
+        X = np.stack([x.ravel(), y.ravel(), np.zeros(x.size)], axis=1)
+        i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+        v = (j * (n + 1) + i).ravel()
+        C = np.stack([v, v + 1, v + n + 2, v + n + 1], axis=1)
+    else:
+        z, y, x = np.meshgrid(g, g, g, indexing="ij")
+        X = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+        k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        v = (k * (n + 1) ** 2 + j * (n + 1) + i).ravel()
+        s, p = n + 1, (n + 1) ** 2
+        C = np.stack([v, v + 1, v + s + 1, v + s, v + p, v + p + 1, v + p + s + 1, v + p + s], axis=1)
+    if jitter > 0.0:
+        inner = np.all((X > 0.0) & (X < 1.0), axis=1) if dim == 3 else np.all((X[:, :2] > 0.0) & (X[:, :2] < 1.0),
+                                                                                axis=1)
+        d = rng.uniform(-jitter * h, jitter * h, size=(X.shape[0], 3))
+        if dim == 2:
+            d[:, 2] = 0.0
+        X[inner] += d[inner]
+    return X, C.astype(np.int32)
+
+
 # ---- mesh / BSRFormat / DoFLinearSystem ------------------------------------------------------
 
 class Mesh:
-    def __init__(self, dim, coords, cells):
+    """Device mesh: tri3 / tet4 cells (dim + 1 nodes, stride 4), Quad4 (dim 2,
+    npc 4) or Hexa8 (dim 3, npc 8, stride 8).  npc defaults to dim + 1, or 8
+    for 8-column cells."""
+
+    def __init__(self, dim, coords, cells, npc=None):
         coords = np.ascontiguousarray(coords, np.float64)
         cells = np.ascontiguousarray(cells, np.int32)
-        assert coords.ndim == 2 and coords.shape[1] == 3 and cells.ndim == 2 and cells.shape[1] == 4
+        if npc is None:
+            npc = 8 if cells.shape[1] == 8 else dim + 1
+        assert coords.ndim == 2 and coords.shape[1] == 3 and cells.ndim == 2 and cells.shape[1] == max(4, npc)
         self.dim = dim
         self.nb_node = coords.shape[0]
         self.nb_cell = cells.shape[0]
-        self.npc = dim + 1
+        self.npc = npc
         h = ctypes.c_void_p()
         check(lib().afem_mesh_create(dim, self.nb_node, _ptr(coords), self.nb_cell, self.npc, _ptr(cells),
                                      ctypes.byref(h)))

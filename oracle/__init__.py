@@ -64,6 +64,10 @@ def lib():
         L.afo_rhs_neumann.restype = ctypes.c_int
         L.afo_rhs_neumann.argtypes = [ctypes.c_int, vp, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
                                       ctypes.c_int, vp]
+        L.afo_elem_laplace_q1.argtypes = [ctypes.c_int, vp, dbl, vp]
+        L.afo_source_q1.argtypes = [ctypes.c_int, vp, dbl, vp]
+        L.afo_q1_gradients.restype = dbl
+        L.afo_q1_gradients.argtypes = [ctypes.c_int, vp, dbl, dbl, dbl, vp, vp, vp, vp]
         L.afo_convection_lhs.restype = ctypes.c_int
         L.afo_convection_lhs.argtypes = [ctypes.c_int, vp, i32, vp, dbl, vp, vp, vp]
         L.afo_apply_transformations.restype = ctypes.c_int
@@ -164,6 +168,8 @@ def assemble(dim, coords, cells, npc, row, cols, element=LAPLACE, nb_dof=1, orde
     rc = lib().afo_assemble(dim, coords.shape[0], _P(coords), cells.shape[0], npc, _P(cells),
                             _P(row), _P(cols), element, nb_dof, 1 if order_per_block else 0,
                             _P(lamv), lame_lambda, lame_mu, _P(vals))
+    if rc == -2:
+        raise ValueError("assembly: Q1 cells carry the Laplace element only")
     if rc != 0:
         raise ValueError("assembly: value not found in pattern")
     return vals
@@ -198,6 +204,22 @@ def rhs_neumann(dim, coords, cells, faces, value, nb_dof=1, dof_index=0, rhs=Non
     if rc != 0:
         raise ValueError("a Neumann face is not a face of any cell")
     return rhs
+
+
+def elem_laplace_q1(dim, X, lam=1.0):
+    """Quad4 (dim 2, X (4, 3)) or Hexa8 (dim 3, X (8, 3)) Laplace element matrix."""
+    X = _f64(X)
+    n = 4 if dim == 2 else 8
+    K = np.zeros(n * n)
+    lib().afo_elem_laplace_q1(dim, _P(X), lam, _P(K))
+    return K.reshape(n, n)
+
+
+def source_q1(dim, X, qdot):
+    X = _f64(X)
+    b = np.zeros(4 if dim == 2 else 8)
+    lib().afo_source_q1(dim, _P(X), qdot, _P(b))
+    return b
 
 
 def convection_lhs(dim, coords, faces, h, row, cols, values):

@@ -153,6 +153,134 @@ void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16
       K[i * 4 + j] = K[i * 4 + j] + (1.0 / 20.0) * (i == j ? 2.0 : 1.0) * vol / dt;
 }
 
+/* ---- Q1 elements (Quad4, Hexa8) ------------------------------------------- */
+
+/* 1/sqrt(3): the 2-point Gauss abscissa of every Q1 integral in the reference
+ * (modules/poisson/ElementMatrixHexQuad.h:67-69), weights 1. */
+#define AFO_GP 0.57735026918962576451
+
+/* Shape functions and reference gradients of femutils/ShapeFunctions.h:104-129
+ * (Quad4, nodes counter-clockwise from (-1,-1)) and :291-350 (Hexa8, the
+ * bottom face counter-clockwise then the top face). */
+static void q1_shape(int dim, double xi, double eta, double zeta, double N[8], double dxi[8], double deta[8],
+                     double dzeta[8])
+{
+  static const double sx[8] = { -1, 1, 1, -1, -1, 1, 1, -1 };
+  static const double sy[8] = { -1, -1, 1, 1, -1, -1, 1, 1 };
+  static const double sz[8] = { -1, -1, -1, -1, 1, 1, 1, 1 };
+  if (dim == 2) {
+    for (int a = 0; a < 4; ++a) {
+      N[a] = 0.25 * (1.0 + sx[a] * xi) * (1.0 + sy[a] * eta);
+      dxi[a] = 0.25 * sx[a] * (1.0 + sy[a] * eta);
+      deta[a] = 0.25 * sy[a] * (1.0 + sx[a] * xi);
+      dzeta[a] = 0.0;
+    }
+    return;
+  }
+  for (int a = 0; a < 8; ++a) {
+    N[a] = 0.125 * (1.0 + sx[a] * xi) * (1.0 + sy[a] * eta) * (1.0 + sz[a] * zeta);
+    dxi[a] = 0.125 * sx[a] * (1.0 + sy[a] * eta) * (1.0 + sz[a] * zeta);
+    deta[a] = 0.125 * sy[a] * (1.0 + sx[a] * xi) * (1.0 + sz[a] * zeta);
+    dzeta[a] = 0.125 * sz[a] * (1.0 + sx[a] * xi) * (1.0 + sy[a] * eta);
+  }
+}
+
+/* Physical gradients and det J at one point: J[i][j] = sum_a dN_a/dxi_i x_a[j],
+ * grad N = J^-1 (dN/dxi, dN/deta[, dN/dzeta]) (femutils/ArcaneFemFunctionsGpu.h:
+ * 296-350 for Quad4, :553-587 for Hexa8).  Returns det J (signed). */
+double afo_q1_gradients(int dim, const double* X, double xi, double eta, double zeta, double N[8], double gx[8],
+                        double gy[8], double gz[8])
+{
+  double dxi[8], deta[8], dzeta[8];
+  q1_shape(dim, xi, eta, zeta, N, dxi, deta, dzeta);
+  const int n = dim == 2 ? 4 : 8;
+  if (dim == 2) {
+    double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+    for (int a = 0; a < n; ++a) {
+      j00 += dxi[a] * X[3 * a + 0];
+      j01 += dxi[a] * X[3 * a + 1];
+      j10 += deta[a] * X[3 * a + 0];
+      j11 += deta[a] * X[3 * a + 1];
+    }
+    const double det = j00 * j11 - j01 * j10;
+    const double i00 = j11 / det, i01 = -j01 / det, i10 = -j10 / det, i11 = j00 / det;
+    for (int a = 0; a < n; ++a) {
+      gx[a] = i00 * dxi[a] + i01 * deta[a];
+      gy[a] = i10 * dxi[a] + i11 * deta[a];
+      gz[a] = 0.0;
+    }
+    return det;
+  }
+  double J[3][3] = { { 0 } };
+  for (int a = 0; a < n; ++a)
+    for (int d = 0; d < 3; ++d) {
+      J[0][d] += dxi[a] * X[3 * a + d];
+      J[1][d] += deta[a] * X[3 * a + d];
+      J[2][d] += dzeta[a] * X[3 * a + d];
+    }
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double I[3][3];
+  I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+  I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+  I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+  I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+  I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+  I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+  I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+  I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+  I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+  for (int a = 0; a < n; ++a) {
+    gx[a] = I[0][0] * dxi[a] + I[0][1] * deta[a] + I[0][2] * dzeta[a];
+    gy[a] = I[1][0] * dxi[a] + I[1][1] * deta[a] + I[1][2] * dzeta[a];
+    gz[a] = I[2][0] * dxi[a] + I[2][1] * deta[a] + I[2][2] * dzeta[a];
+  }
+  return det;
+}
+
+/* modules/poisson/ElementMatrixHexQuad.h:33-64 (Quad4), :159-195 (Hexa8):
+ * sum over the 2x2(x2) Gauss points of (dxU^dxU + dyU^dyU [+ dzU^dzU]) detJ,
+ * times the cell's lambda. */
+void afo_elem_laplace_q1(int dim, const double* X, double lambda, double* K)
+{
+  const int n = dim == 2 ? 4 : 8, ng = dim == 2 ? 1 : 2;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int k = 0; k < n * n; ++k)
+    K[k] = 0.0;
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int iz = 0; iz < ng; ++iz) {
+        double N[8], gx[8], gy[8], gz[8];
+        const double w = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+        for (int a = 0; a < n; ++a)
+          for (int b = 0; b < n; ++b)
+            K[a * n + b] += (gx[a] * gx[b]) * w + (gy[a] * gy[b]) * w + (gz[a] * gz[b]) * w;
+      }
+  for (int k = 0; k < n * n; ++k)
+    K[k] *= lambda;
+}
+
+/* BoundaryConditions2D::applyConstantSourceToRhsQuad4 and
+ * BoundaryConditions3D::applyConstantSourceToRhsHexa8
+ * (femutils/ArcaneFemFunctionsGpu.cc:393-470, :875-960): every node of the
+ * cell receives qdot N_a detJ at each Gauss point. */
+void afo_source_q1(int dim, const double* X, double qdot, double* b)
+{
+  const int n = dim == 2 ? 4 : 8, ng = dim == 2 ? 1 : 2;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int a = 0; a < n; ++a)
+    b[a] = 0.0;
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int iz = 0; iz < ng; ++iz) {
+        double N[8], gx[8], gy[8], gz[8];
+        const double w = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+        for (int a = 0; a < n; ++a)
+          b[a] += qdot * N[a] * w;
+      }
+}
+
 /* modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
 {
@@ -215,10 +343,11 @@ void afo_elem_elasticity_tetra4(const double* X, double lam, double mu, double K
 int afo_node_cells(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t* cells,
                    int32_t* offsets, int32_t* items)
 {
+  const int st = AFO_CELL_STRIDE(npc);
   memset(offsets, 0, sizeof(int32_t) * ((size_t)nb_node + 1));
   for (int64_t c = 0; c < nb_cell; ++c)
     for (int k = 0; k < npc; ++k) {
-      int32_t n = cells[c * 4 + k];
+      int32_t n = cells[c * st + k];
       if (n < 0 || n >= nb_node)
         return -1;
       offsets[n + 1]++;
@@ -230,8 +359,8 @@ int afo_node_cells(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t*
   /* cells visited in increasing id order: each list comes out sorted by cell */
   for (int64_t c = 0; c < nb_cell; ++c)
     for (int k = 0; k < npc; ++k) {
-      int32_t n = cells[c * 4 + k];
-      items[fill[n]++] = (int32_t)(c * 4 + k);
+      int32_t n = cells[c * st + k];
+      items[fill[n]++] = (int32_t)(c * st + k);
     }
   free(fill);
   return 0;
@@ -266,9 +395,9 @@ int64_t afo_pattern(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t
   for (int32_t i = 0; i < nb_node; ++i) {
     int32_t m = 0;
     for (int32_t e = off[i]; e < off[i + 1]; ++e) {
-      int32_t c = items[e] >> 2;
+      int32_t c = items[e] / AFO_CELL_STRIDE(npc);
       for (int k = 0; k < npc; ++k)
-        buf[m++] = cells[(int64_t)c * 4 + k];
+        buf[m++] = cells[(int64_t)c * AFO_CELL_STRIDE(npc) + k];
     }
     if (m == 0)
       buf[m++] = i; /* isolated node keeps its diagonal */
@@ -309,20 +438,24 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
                  const double* cell_lambda, double lame_lambda, double lame_mu,
                  double* values)
 {
-  (void)dim;
   const int nb = nb_dof;
   const int nb2 = nb * nb;
   const int64_t nnz = row_index[nb_node];
+  const int q1 = AFO_IS_Q1(dim, npc);
+  if (q1 && element != AFO_LAPLACE)
+    return -2;
   memset(values, 0, sizeof(double) * (size_t)(nnz * nb2));
-  double X[12];
+  double X[24];
   double K[144];
   const int ke = npc * nb; /* element matrix order */
   for (int64_t c = 0; c < nb_cell; ++c) {
-    const int32_t* cn = cells + c * 4;
+    const int32_t* cn = cells + c * AFO_CELL_STRIDE(npc);
     for (int k = 0; k < npc; ++k)
       memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
     double lambda = cell_lambda ? cell_lambda[c] : 1.0;
-    if (element == AFO_LAPLACE) {
+    if (q1)
+      afo_elem_laplace_q1(dim, X, lambda, K);
+    else if (element == AFO_LAPLACE) {
       if (npc == 3)
         afo_elem_laplace_tria3(X, lambda, K);
       else
@@ -450,7 +583,18 @@ void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int
                              const int32_t* cells, const double* cell_scale,
                              double f, int nb_dof, int dof_index, double* rhs)
 {
-  (void)dim;
+  if (AFO_IS_Q1(dim, npc)) {
+    for (int64_t c = 0; c < nb_cell; ++c) {
+      const int32_t* cn = cells + c * AFO_CELL_STRIDE(npc);
+      double X[24], b[8];
+      for (int k = 0; k < npc; ++k)
+        memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+      afo_source_q1(dim, X, cell_scale ? f * cell_scale[c] : f, b);
+      for (int k = 0; k < npc; ++k)
+        rhs[(int64_t)cn[k] * nb_dof + dof_index] += b[k];
+    }
+    return;
+  }
   for (int64_t c = 0; c < nb_cell; ++c) {
     const int32_t* cn = cells + c * 4;
     double m;

@@ -83,6 +83,22 @@ int64_t afo_pattern(int32_t nb_node, int32_t nb_cell, int32_t npc, const int32_t
 
 enum afo_element { AFO_LAPLACE = 0, AFO_ELASTICITY = 1, AFO_HEAT = 2 };
 
+/* Cells are stored with stride 4 (tri3 padded with -1, tet4, quad4) or 8
+ * (hexa8); Quad4 is (dim 2, 4 nodes), Hexa8 (dim 3, 8 nodes). */
+#define AFO_CELL_STRIDE(npc) ((npc) > 4 ? (npc) : 4)
+#define AFO_IS_Q1(dim, npc) (((dim) == 2 && (npc) == 4) || ((dim) == 3 && (npc) == 8))
+
+/* Q1 elements: physical shape-function gradients and signed det J at
+ * (xi, eta[, zeta]) (femutils/ShapeFunctions.h:104-129, :291-350;
+ * femutils/ArcaneFemFunctionsGpu.h:296-350, :553-587); the Laplace element
+ * matrix with 2x2(x2) Gauss points times lambda
+ * (modules/poisson/ElementMatrixHexQuad.h:33-64, :159-195); the constant
+ * source vector (femutils/ArcaneFemFunctionsGpu.cc:393-470, :875-960). */
+double afo_q1_gradients(int dim, const double* X, double xi, double eta, double zeta, double N[8], double gx[8],
+                        double gy[8], double gz[8]);
+void afo_elem_laplace_q1(int dim, const double* X, double lambda, double* K);
+void afo_source_q1(int dim, const double* X, double qdot, double* b);
+
 /* Cell-wise assembly with a linear column search in each owned row, values
  * accumulated with '+=' (the reference uses atomics on GPU and map inserts on
  * CPU; the sum is the same up to summation order).

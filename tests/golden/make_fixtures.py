@@ -6,8 +6,9 @@ golden solution files, and writes one .npz per case with:
 
   coords      (N, 3) float64   node coordinates, nodes ordered by Gmsh tag
   node_uid    (N,)   int64     Gmsh node tag = Arcane node uniqueId
-  cells       (C, 4) int32     cell -> local node ids (triangles: slot 3 = -1)
-  npc         ()     int32     nodes per cell (3 or 4)
+  cells       (C, s) int32     cell -> local node ids, s = 4 (triangles: slot 3 = -1;
+                               tets; quads) or 8 (hexas)
+  npc         ()     int32     nodes per cell (3, 4 or 8)
   dim         ()     int32
   group_<name>(k,)   int32     local ids of the nodes of the boundary faces
                                carrying the physical name <name>
@@ -44,6 +45,15 @@ Golden cases (test names and tolerances from the reference's CMake files):
               check/3d_conduction_convection_pointBC.txt (convection h 1.9, Text 30 on
               'top', Dirichlet 2.6 on 'bottom', point 'center' 100.6, penalty 1e30)
 
+  circle_2D_quad  modules/poisson/inputs/circle.2D.quad.arc,
+              check/poisson_test_ref_circle_2D_quad.txt (Quad4; f=5.5, Dirichlet 0.5 on
+              'horizontal', penalty)
+  sphere_3D_hexa  modules/poisson/inputs/sphere.3D.hexa.arc,
+              check/poisson_test_ref_sphere_3D_hexa.txt (Hexa8; the same conditions)
+  cube_3D_hexa8  modules/poisson/inputs/cube.3D.hexa.arc on meshes/3x3x3_cube_hexa8.msh,
+              check/poisson_test_ref_cube_3D_hexa8.txt (f=9.8, Dirichlet 0.5 on 'left',
+              scalar Neumann 13.9 on 'right')
+
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
 """
@@ -72,10 +82,13 @@ CASES = {
     "heat_neumann_point_2D": ("meshes/msh/plate.msh", "modules/heat/check/2d_conduction_neumann_pointBC.txt", 1),
     "heat_convection_point_3D": ("meshes/msh/truncated_cube.msh",
                                  "modules/heat/check/3d_conduction_convection_pointBC.txt", 1),
+    "circle_2D_quad": ("meshes/msh/circle_cut.quad.msh", "modules/poisson/check/poisson_test_ref_circle_2D_quad.txt", 1),
+    "sphere_3D_hexa": ("meshes/msh/sphere_cut.hexa.msh", "modules/poisson/check/poisson_test_ref_sphere_3D_hexa.txt", 1),
+    "cube_3D_hexa8": ("meshes/msh/3x3x3_cube_hexa8.msh", "modules/poisson/check/poisson_test_ref_cube_3D_hexa8.txt", 1),
 }
 
-# Gmsh element type -> number of nodes (only the P1 simplex family is needed)
-GMSH_NODES = {15: 1, 1: 2, 2: 3, 4: 4}
+# Gmsh element type -> number of nodes (point, line, triangle, quad, tet, hexa)
+GMSH_NODES = {15: 1, 1: 2, 2: 3, 3: 4, 4: 4, 5: 8}
 
 
 class _Reader:
@@ -198,7 +211,7 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     cells = np.concatenate(cells)
     npc = cells.shape[1]
     lid = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(cells)
-    cells4 = -np.ones((lid.shape[0], 4), dtype=np.int32)
+    cells4 = -np.ones((lid.shape[0], max(4, npc)), dtype=np.int32)
     cells4[:, :npc] = lid
     groups, faces = {}, {}
     for edim, etag, _etype, _etags, conn in blocks:

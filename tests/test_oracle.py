@@ -404,9 +404,10 @@ def by_uid(d, x, ncomp=1):
     return np.array([m[u] for u in d["ref_uid"].tolist()])
 
 
-@pytest.mark.parametrize("case", ["circle_2D", "sphere_3D"])
+@pytest.mark.parametrize("case", ["circle_2D", "sphere_3D", "circle_2D_quad", "sphere_3D_hexa"])
 def test_golden_poisson_penalty(case):
-    """modules/poisson tests 2D_bsr / 3D_bsr: f=5.5, u=0.5 on 'horizontal', penalty 1e30, eps 1e-4."""
+    """modules/poisson tests 2D_bsr / 3D_bsr, 2D_quad / 3D_hexa (Quad4 and Hexa8
+    meshes): f=5.5, u=0.5 on 'horizontal', penalty 1e30, eps 1e-4."""
     d = load_case(case)
     X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
     N = X.shape[0]
@@ -732,3 +733,96 @@ def test_convection_matrix_is_face_mass(dim):
     assert np.allclose(np.diag(F), 2 * F[0, 1], rtol=1e-15)
     b = O.rhs_neumann(dim, X, C, face, 1.5 * 20.0)
     assert abs(b.sum() - 1.5 * 20.0 * meas) <= 1e-12
+
+
+# ---- Q1 elements (Quad4, Hexa8) --------------------------------------------------------
+
+def test_q1_unit_element_known_answers():
+    """Unit square: K = [4 -1 -2 -1; ...] / 6; unit cube: 1/3 on the diagonal,
+    0 along an edge, -1/12 across a face or the body (textbook values for the
+    bilinear / trilinear element, exactly integrated by 2x2(x2) Gauss)."""
+    Xq = np.array([[0.0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    Kq = O.elem_laplace_q1(2, Xq)
+    ref = np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 6.0
+    assert np.abs(Kq - ref).max() <= 1e-15
+    Xh = np.array([[0.0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    Kh = O.elem_laplace_q1(3, Xh)
+    for a in range(8):
+        for b in range(8):
+            diff = int(np.abs(Xh[a] - Xh[b]).sum())  # 0 self, 1 edge, 2 face diagonal, 3 body diagonal
+            want = {0: 1 / 3, 1: 0.0, 2: -1 / 12, 3: -1 / 12}[diff]
+            assert abs(Kh[a, b] - want) <= 1e-15, (a, b)
+    # lambda scales, a 2 x 3 rectangle: K_xx part scales by b/a, K_yy by a/b
+    K2 = O.elem_laplace_q1(2, Xq * [2.0, 3.0, 1.0], lam=1.5)
+    Kx = np.array([[2, -2, -1, 1], [-2, 2, 1, -1], [-1, 1, 2, -2], [1, -1, -2, 2]]) / 6.0
+    Ky = np.array([[2, 1, -1, -2], [1, 2, -2, -1], [-1, -2, 2, 1], [-2, -1, 1, 2]]) / 6.0
+    assert np.abs(K2 - 1.5 * (Kx * 3 / 2 + Ky * 2 / 3)).max() <= 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q1_element_identities_on_distorted_cells(dim):
+    """On jittered (non-affine) cells: symmetric, K 1 = 0, u^T K u = |a|^2 |T|
+    for u linear (2-point Gauss integrates det J exactly), and the source
+    vector sums to f |T| with every entry f int N_a."""
+    from arcanefem_amd import box_mesh_q1
+    rng = np.random.default_rng(21)
+    X, C = box_mesh_q1(dim, 3, 0.3, seed=9)
+    for c in rng.choice(C.shape[0], 5, replace=False):
+        Xc = X[C[c]]
+        K = O.elem_laplace_q1(dim, Xc, lam=0.7)
+        vol = sum(O.lib().afo_q1_gradients(dim, np.ascontiguousarray(Xc).ctypes.data, *gp,
+                                           *(np.zeros(8).ctypes.data for _ in range(4)))
+                  for gp in _gauss_points(dim))
+        assert np.abs(K - K.T).max() <= 1e-15 * np.abs(K).max() * 10
+        assert np.abs(K.sum(axis=1)).max() <= 1e-14 * np.abs(K).max()
+        a = np.array([0.3, -1.2, 0.8])[:dim]
+        u = Xc[:, :dim] @ a + 0.4
+        assert abs(u @ K @ u - 0.7 * (a @ a) * vol) <= 1e-13 * abs(u @ K @ u)
+        b = O.source_q1(dim, Xc, 2.5)
+        assert abs(b.sum() - 2.5 * vol) <= 1e-14 * abs(b.sum())
+    # the cells tile the unit box: the measures sum to 1
+    total = sum(sum(O.lib().afo_q1_gradients(dim, np.ascontiguousarray(X[C[c]]).ctypes.data, *gp,
+                                             *(np.zeros(8).ctypes.data for _ in range(4)))
+                    for gp in _gauss_points(dim)) for c in range(C.shape[0]))
+    assert abs(total - 1.0) <= 1e-13
+
+
+def _gauss_points(dim):
+    g = 1.0 / np.sqrt(3.0)
+    pts = [(x, y, 0.0) for x in (-g, g) for y in (-g, g)]
+    if dim == 3:
+        pts = [(x, y, z) for x in (-g, g) for y in (-g, g) for z in (-g, g)]
+    return pts
+
+
+@pytest.mark.parametrize("dim,n", [(2, 5), (3, 3)])
+def test_q1_pattern_and_assembly(dim, n):
+    """Pattern: every pair of nodes sharing a cell (brute force), 9 / 27 entries
+    on interior rows (edges + face and body diagonals, femutils/BSRFormat.cc:
+    284-317); assembly against a dense brute-force sum of element matrices;
+    1^T A = 0 and u^T A u = |a|^2 for linear u on the unit box."""
+    from arcanefem_amd import box_mesh_q1
+    X, C = box_mesh_q1(dim, n, 0.25, seed=4)
+    N, npc = X.shape[0], C.shape[1]
+    row, col = O.pattern(N, C, npc)
+    want = [sorted({int(v) for c in C if i in c for v in c}) for i in range(N)]
+    assert [list(col[row[i]:row[i + 1]]) for i in range(N)] == want
+    lens = np.diff(row)
+    assert lens.max() == 3 ** dim
+    lam = np.random.default_rng(2).uniform(0.5, 2.0, C.shape[0])
+    val = O.assemble(dim, X, C, npc, row, col, cell_lambda=lam)
+    dense = np.zeros((N, N))
+    for c in range(C.shape[0]):
+        dense[np.ix_(C[c], C[c])] += O.elem_laplace_q1(dim, X[C[c]], lam[c])
+    A = sp.csr_matrix((val, col, row), shape=(N, N)).toarray()
+    assert np.abs(A - dense).max() <= 1e-14 * np.abs(dense).max()
+    val1 = O.assemble(dim, X, C, npc, row, col)
+    A1 = sp.csr_matrix((val1, col, row), shape=(N, N))
+    assert np.abs(A1 @ np.ones(N)).max() <= 1e-13
+    a = np.array([1.0, -2.0, 0.5])[:dim]
+    u = X[:, :dim] @ a
+    assert abs(u @ (A1 @ u) - a @ a) <= 1e-12
+    b = O.rhs_constant_source(dim, X, C, npc, 3.0)
+    assert abs(b.sum() - 3.0) <= 1e-13
+    with pytest.raises(ValueError):
+        O.assemble(dim, X, C, npc, row, col, element=O.HEAT, lame_lambda=0.1)
