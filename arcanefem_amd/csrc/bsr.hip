@@ -22,7 +22,7 @@ namespace afem {
 template <int CAP, int BS>
 __global__ void __launch_bounds__(BS)
 k_pattern_rows(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
-               const int32_t* __restrict__ cells, int npc, int32_t nb_node,
+               const int32_t* __restrict__ cells, int npc, int slot_bits, int32_t nb_node,
                const int32_t* __restrict__ row_index, int32_t* __restrict__ out,
                int32_t* __restrict__ overflow)
 {
@@ -36,12 +36,12 @@ k_pattern_rows(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ n
   bool ovf = false;
   for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
     const int32_t it = nc_items[e];
-    const int32_t c = it >> 2;
-    const int s = it & 3;
+    const int s = it & ((1 << slot_bits) - 1);
+    const int32_t* cn = cells + (it - s); // the cell's first node: items are stride * cell + slot
     for (int k = 0; k < npc; ++k) {
       if (k == s)
         continue;
-      const int32_t v = cells[4 * c + k];
+      const int32_t v = cn[k];
       int pos = 0;
       while (pos < m && list[pos * BS + tid] < v)
         ++pos;
@@ -82,7 +82,8 @@ static bool pattern_pass(const afem_mesh& m, const int32_t* row_index, int32_t* 
     constexpr int BS = decltype(bs_tag)::value;
     ovf.zero();
     hipLaunchKernelGGL((k_pattern_rows<CAP, BS>), dim3(grid_for(m.nb_node, BS)), dim3(BS), 0, stream(),
-                       m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.nb_node, row_index, out, ovf.p);
+                       m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.slot_bits, m.nb_node, row_index, out,
+                       ovf.p);
     check_launch("pattern_rows");
     int32_t flag = 0;
     ovf.download(&flag, 1);
@@ -1790,7 +1791,7 @@ static void build_plan(afem_bsr* b)
   b->plan.release();
   b->cplan.release();
   b->diag_pos.release();
-  if (b->max_row > kPlanMaxRow)
+  if (b->max_row > kPlanMaxRow || !m.simplex) // the plans serve the simplex kernels
     return;
   b->plan.alloc(m.nc_items.n);
   b->diag_pos.alloc((size_t)m.nb_node);
@@ -2100,6 +2101,10 @@ static void assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, d
 {
   AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
   AFEM_REQUIRE(b->nb_dof == 1, AFEM_ERR_ARGUMENT, "the Laplace element needs nb_dof = 1");
+  if (!b->mesh->simplex) {
+    q1_assemble_laplace(b, cell_lambda, f, rhs);
+    return;
+  }
   assemble_laplace_variant(b, b->variant >= 0 ? b->variant : kDefaultLaplaceVariant, cell_lambda, f, rhs);
 }
 
@@ -2143,7 +2148,7 @@ static void autotune_laplace(afem_bsr* b, double* best_ms, int* best_variant)
 static void assemble_elasticity(afem_bsr* b, double lam, double mu)
 {
   AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
-  AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4, AFEM_ERR_ARGUMENT,
+  AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4 && b->mesh->dim == 3, AFEM_ERR_ARGUMENT,
                "the elasticity element needs a tetrahedral mesh and nb_dof = 3");
   const afem_mesh& m = *b->mesh;
   if (b->ecell.p && b->variant != 0) {
@@ -2360,6 +2365,7 @@ AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* va
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(bsr && bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called first");
   AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_NOT_IMPLEMENTED, "assembly autotuning covers the Laplace element");
+  AFEM_REQUIRE(bsr->mesh->simplex, AFEM_ERR_NOT_IMPLEMENTED, "Quad4 / Hexa8 meshes have one assembly kernel");
   autotune_laplace(bsr, best_ms, variant);
   AFEM_GUARD_END
 }
@@ -2464,6 +2470,10 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(mesh && rhs, AFEM_ERR_ARGUMENT, "null argument");
   AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  if (!mesh->simplex) {
+    q1_rhs_constant_source(*mesh, f, cell_scale, nb_dof, dof_index, rhs);
+    return AFEM_OK;
+  }
   ProfScope prof("rhs_constant_source");
   hipLaunchKernelGGL(k_rhs_source, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
                      mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, f, cell_scale,
@@ -2551,6 +2561,7 @@ AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, d
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
   AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_ARGUMENT, "the heat element needs nb_dof = 1");
+  AFEM_REQUIRE(bsr->mesh->simplex, AFEM_ERR_NOT_IMPLEMENTED, "the heat element is implemented for tri3 / tet4");
   AFEM_REQUIRE(dt > 0.0, AFEM_ERR_ARGUMENT, "the time step must be positive");
   assemble_laplace(bsr, cell_lambda, 0.0, nullptr);
   const afem_mesh& m = *bsr->mesh;
@@ -2567,6 +2578,7 @@ AFEM_API int afem_rhs_mass_field(afem_mesh_t mesh, const double* field, double s
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(mesh && field && rhs, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(mesh->simplex, AFEM_ERR_NOT_IMPLEMENTED, "the mass-weighted field RHS is implemented for tri3 / tet4");
   AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
   hipLaunchKernelGGL(k_rhs_mass_field, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
                      mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, field, scale,
@@ -2579,7 +2591,7 @@ AFEM_API int afem_rhs_body_force(afem_mesh_t mesh, const double* f, double* rhs)
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(mesh && f && rhs, AFEM_ERR_ARGUMENT, "null argument");
-  AFEM_REQUIRE(mesh->npc == 4, AFEM_ERR_NOT_IMPLEMENTED, "body force needs a tetrahedral mesh");
+  AFEM_REQUIRE(mesh->npc == 4 && mesh->dim == 3, AFEM_ERR_NOT_IMPLEMENTED, "body force needs a tetrahedral mesh");
   hipLaunchKernelGGL(k_rhs_body_force, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
                      mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, f[0], f[1], f[2], rhs);
   check_launch("rhs_body_force");

@@ -192,13 +192,16 @@ struct afem_mesh
 {
   int dim = 0;
   int npc = 0;
+  int stride = 4;                    // cell stride: 4 (tri3, tet4, quad4) or 8 (hexa8)
+  int slot_bits = 2;                 // log2(stride)
+  bool simplex = true;               // tri3 / tet4; false for Quad4 / Hexa8
   int32_t nb_node = 0;
   int32_t nb_cell = 0;
   int32_t max_degree = 0;            // largest node->cell list
   afem::DBuf<double> coords;         // nb_node * 4 (x, y, z, 0)
-  afem::DBuf<int32_t> cells;         // nb_cell * 4
+  afem::DBuf<int32_t> cells;         // nb_cell * stride
   afem::DBuf<int32_t> nc_offsets;    // nb_node + 1
-  afem::DBuf<int32_t> nc_items;      // cell*4 + slot, sorted by cell per node
+  afem::DBuf<int32_t> nc_items;      // cell*stride + slot, sorted by cell per node
 };
 
 struct afem_bsr
@@ -271,3 +274,13 @@ struct afem_system
   afem::DBuf<double> saved_values;   // values before the row-column elimination
   afem_solver_ws* ws = nullptr;
 };
+
+namespace afem {
+
+// Quad4 / Hexa8 meshes (q1.hip): Laplace values (and the fused constant-source
+// RHS when rhs is given) and the constant-source RHS alone.
+void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs);
+void q1_rhs_constant_source(const afem_mesh& m, double f, const double* cell_scale, int nb_dof, int dof_index,
+                            double* rhs);
+
+} // namespace afem

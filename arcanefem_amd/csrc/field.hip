@@ -62,7 +62,7 @@ constexpr int kFieldChunk = 512; // features staged in LDS per pass
 // evaluated in FP32 (|w.x| stays below ~50 on the unit cube for l >= 0.05),
 // the sum in FP32 per chunk and FP64 across chunks.
 __global__ void __launch_bounds__(kFieldBlock)
-k_lognormal(const double* __restrict__ coords, const int32_t* __restrict__ cells, int npc, int32_t nb_cell,
+k_lognormal(const double* __restrict__ coords, const int32_t* __restrict__ cells, int npc, int stride, int32_t nb_cell,
             const float4* __restrict__ feat, int nfeatures, double scale, double* __restrict__ lambda)
 {
   __shared__ float4 s_feat[kFieldChunk];
@@ -71,7 +71,7 @@ k_lognormal(const double* __restrict__ coords, const int32_t* __restrict__ cells
   if (c < nb_cell) {
     double x = 0.0, y = 0.0, z = 0.0;
     for (int k = 0; k < npc; ++k) {
-      const int64_t n = cells[4 * (int64_t)c + k];
+      const int64_t n = cells[stride * (int64_t)c + k];
       x += coords[4 * n];
       y += coords[4 * n + 1];
       z += coords[4 * n + 2];
@@ -134,7 +134,7 @@ AFEM_API int afem_lognormal_field(afem_mesh_t mesh, double sigma, double corr_le
   feat.upload(packed.data(), packed.size());
   const double scale = sigma * std::sqrt(2.0 / nfeatures);
   hipLaunchKernelGGL(k_lognormal, dim3(grid_for(mesh->nb_cell, kFieldBlock)), dim3(kFieldBlock), 0, stream(),
-                     mesh->coords.p, mesh->cells.p, mesh->npc, mesh->nb_cell,
+                     mesh->coords.p, mesh->cells.p, mesh->npc, mesh->stride, mesh->nb_cell,
                      reinterpret_cast<const float4*>(feat.p), nfeatures, scale, cell_lambda);
   check_launch("lognormal_field");
   AFEM_HIP(hipStreamSynchronize(stream())); // feat is freed on return

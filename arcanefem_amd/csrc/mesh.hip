@@ -112,7 +112,7 @@ static void box_mesh(int dim, int n, double jitter, uint64_t seed, double* coord
 
 // ---- node -> cell connectivity on the device ----------------------------------------
 
-__global__ void k_count_node_cells(const int32_t* __restrict__ cells, int32_t nb_cell, int npc,
+__global__ void k_count_node_cells(const int32_t* __restrict__ cells, int32_t nb_cell, int npc, int stride,
                                    int32_t* __restrict__ counts)
 {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,10 +120,10 @@ __global__ void k_count_node_cells(const int32_t* __restrict__ cells, int32_t nb
     return;
   int64_t c = t / npc;
   int s = (int)(t - c * npc);
-  atomicAdd(&counts[cells[4 * c + s]], 1);
+  atomicAdd(&counts[cells[stride * c + s]], 1);
 }
 
-__global__ void k_fill_node_cells(const int32_t* __restrict__ cells, int32_t nb_cell, int npc,
+__global__ void k_fill_node_cells(const int32_t* __restrict__ cells, int32_t nb_cell, int npc, int stride,
                                   const int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
                                   int32_t* __restrict__ items)
 {
@@ -132,9 +132,9 @@ __global__ void k_fill_node_cells(const int32_t* __restrict__ cells, int32_t nb_
     return;
   int64_t c = t / npc;
   int s = (int)(t - c * npc);
-  int32_t n = cells[4 * c + s];
+  int32_t n = cells[stride * c + s];
   int32_t pos = offsets[n] + atomicAdd(&cursor[n], 1);
-  items[pos] = (int32_t)(4 * c + s);
+  items[pos] = (int32_t)(stride * c + s);
 }
 
 // Insertion sort of each node's list (deterministic cell order) and the
@@ -162,14 +162,16 @@ static afem_mesh* mesh_create(int dim, int32_t nb_node, const double* coords, in
                               const int32_t* cells)
 {
   AFEM_REQUIRE(dim == 2 || dim == 3, AFEM_ERR_ARGUMENT, "dim must be 2 or 3");
-  AFEM_REQUIRE(npc == dim + 1, AFEM_ERR_NOT_IMPLEMENTED, "only P1 simplices (tri3 in 2D, tet4 in 3D) are supported");
+  AFEM_REQUIRE(npc == dim + 1 || (dim == 2 && npc == 4) || (dim == 3 && npc == 8), AFEM_ERR_NOT_IMPLEMENTED,
+               "supported cells: tri3 and quad4 in 2D, tet4 and hexa8 in 3D");
+  const int stride = npc > 4 ? 8 : 4;
   AFEM_REQUIRE(nb_node > 0 && nb_cell > 0, AFEM_ERR_ARGUMENT, "empty mesh");
-  AFEM_REQUIRE(nb_cell < (1 << 29), AFEM_ERR_ARGUMENT, "too many cells for 32-bit (cell, slot) ids");
+  AFEM_REQUIRE(nb_cell < (1 << 28), AFEM_ERR_ARGUMENT, "too many cells for 32-bit (cell, slot) ids");
   AFEM_REQUIRE(coords && cells, AFEM_ERR_ARGUMENT, "null mesh arrays");
   // validate connectivity on the host: every kernel indexes with these ids
   for (int64_t c = 0; c < nb_cell; ++c)
     for (int s = 0; s < npc; ++s) {
-      int32_t v = cells[4 * c + s];
+      int32_t v = cells[stride * c + s];
       if (v < 0 || v >= nb_node)
         fail(AFEM_ERR_MESH, "cell " + std::to_string(c) + " references node " + std::to_string(v) +
                               " outside [0, " + std::to_string(nb_node) + ")");
@@ -178,6 +180,9 @@ static afem_mesh* mesh_create(int dim, int32_t nb_node, const double* coords, in
   try {
     m->dim = dim;
     m->npc = npc;
+    m->stride = stride;
+    m->slot_bits = stride == 8 ? 3 : 2;
+    m->simplex = npc == dim + 1;
     m->nb_node = nb_node;
     m->nb_cell = nb_cell;
     std::vector<double> packed((size_t)nb_node * 4);
@@ -189,10 +194,10 @@ static afem_mesh* mesh_create(int dim, int32_t nb_node, const double* coords, in
     }
     m->coords.alloc(packed.size());
     m->coords.upload(packed.data(), packed.size());
-    std::vector<int32_t> cl((size_t)nb_cell * 4);
+    std::vector<int32_t> cl((size_t)nb_cell * stride);
     for (int64_t c = 0; c < nb_cell; ++c)
-      for (int s = 0; s < 4; ++s)
-        cl[4 * c + s] = (s < npc) ? cells[4 * c + s] : -1;
+      for (int s = 0; s < stride; ++s)
+        cl[stride * c + s] = (s < npc) ? cells[stride * c + s] : -1;
     m->cells.alloc(cl.size());
     m->cells.upload(cl.data(), cl.size());
 
@@ -200,7 +205,7 @@ static afem_mesh* mesh_create(int dim, int32_t nb_node, const double* coords, in
     counts.zero();
     int64_t work = (int64_t)nb_cell * npc;
     hipLaunchKernelGGL(k_count_node_cells, dim3(grid_for(work, 256)), dim3(256), 0, stream(), m->cells.p, nb_cell,
-                       npc, counts.p);
+                       npc, stride, counts.p);
     check_launch("count_node_cells");
     m->nc_offsets.alloc((size_t)nb_node + 1);
     int64_t total = exclusive_scan_i32(counts.p, m->nc_offsets.p, nb_node);
@@ -208,7 +213,7 @@ static afem_mesh* mesh_create(int dim, int32_t nb_node, const double* coords, in
     m->nc_items.alloc((size_t)total);
     counts.zero();
     hipLaunchKernelGGL(k_fill_node_cells, dim3(grid_for(work, 256)), dim3(256), 0, stream(), m->cells.p, nb_cell,
-                       npc, m->nc_offsets.p, counts.p, m->nc_items.p);
+                       npc, stride, m->nc_offsets.p, counts.p, m->nc_items.p);
     check_launch("fill_node_cells");
     DBuf<int32_t> maxdeg(1);
     maxdeg.zero();

@@ -1,0 +1,242 @@
+// q1.hip -- Quad4 / Hexa8 (Q1) meshes: node-wise gather assembly of the
+// Laplace bilinear form (2x2 / 2x2x2 Gauss points, an isoparametric Jacobian
+// per point) and of the constant-source right-hand side.
+//
+// Reference behaviour: modules/poisson/ElementMatrixHexQuad.h:33-64 (Quad4),
+// :159-195 (Hexa8); femutils/ShapeFunctions.h:104-129, :291-350;
+// femutils/ArcaneFemFunctionsGpu.h:296-350, :553-587 (gradients and det J);
+// femutils/ArcaneFemFunctionsGpu.cc:393-470, :875-960 (source term).  The
+// reference scatters each element matrix with atomics (assembleBilinearAtomic)
+// or recomputes it per node with a column search (assembleBilinearAtomicFree);
+// here one thread owns one matrix row: it walks the node's cells, builds the
+// element row of the node's slot in registers and adds it into its own row,
+// so no two threads ever write the same value.
+#include "common.h"
+#include "device_utils.h"
+
+namespace afem {
+
+namespace {
+
+constexpr double kGauss = 0.57735026918962576451; // 1/sqrt(3), weight 1
+
+// Shape functions, reference gradients, physical gradients and det J at the
+// Gauss point g (ixi-major: g = ixi*2 + ieta in 2D, ixi*4 + ieta*2 + izeta in
+// 3D, the reference's loop order) of a cell with node coordinates x.
+template <int DIM>
+struct Q1Point
+{
+  static constexpr int N = DIM == 2 ? 4 : 8;
+  double Nv[N], gx[N], gy[N], gz[N];
+  double det;
+
+  __device__ __forceinline__ void eval(const double (&x)[N][3], int g)
+  {
+    const double xi = (DIM == 2 ? (g >> 1) : (g >> 2)) ? kGauss : -kGauss;
+    const double eta = (DIM == 2 ? (g & 1) : ((g >> 1) & 1)) ? kGauss : -kGauss;
+    const double zeta = (DIM == 3 && (g & 1)) ? kGauss : -kGauss;
+    double dxi[N], deta[N], dzeta[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      // corner signs of node a: counter-clockwise in the (xi, eta) plane, then zeta
+      const double sx = ((a & 3) == 1 || (a & 3) == 2) ? 1.0 : -1.0;
+      const double sy = ((a & 3) >= 2) ? 1.0 : -1.0;
+      if (DIM == 2) {
+        Nv[a] = 0.25 * (1.0 + sx * xi) * (1.0 + sy * eta);
+        dxi[a] = 0.25 * sx * (1.0 + sy * eta);
+        deta[a] = 0.25 * sy * (1.0 + sx * xi);
+        dzeta[a] = 0.0;
+      }
+      else {
+        const double sz = a >= 4 ? 1.0 : -1.0;
+        Nv[a] = 0.125 * (1.0 + sx * xi) * (1.0 + sy * eta) * (1.0 + sz * zeta);
+        dxi[a] = 0.125 * sx * (1.0 + sy * eta) * (1.0 + sz * zeta);
+        deta[a] = 0.125 * sy * (1.0 + sx * xi) * (1.0 + sz * zeta);
+        dzeta[a] = 0.125 * sz * (1.0 + sx * xi) * (1.0 + sy * eta);
+      }
+    }
+    if (DIM == 2) {
+      double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        j00 += dxi[a] * x[a][0];
+        j01 += dxi[a] * x[a][1];
+        j10 += deta[a] * x[a][0];
+        j11 += deta[a] * x[a][1];
+      }
+      det = j00 * j11 - j01 * j10;
+      const double i00 = j11 / det, i01 = -j01 / det, i10 = -j10 / det, i11 = j00 / det;
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        gx[a] = i00 * dxi[a] + i01 * deta[a];
+        gy[a] = i10 * dxi[a] + i11 * deta[a];
+        gz[a] = 0.0;
+      }
+      return;
+    }
+    double J[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        J[0][d] += dxi[a] * x[a][d];
+        J[1][d] += deta[a] * x[a][d];
+        J[2][d] += dzeta[a] * x[a][d];
+      }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double r = 1.0 / det;
+    const double I00 = c00 * r, I01 = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r,
+                 I02 = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    const double I10 = c01 * r, I11 = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r,
+                 I12 = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    const double I20 = c02 * r, I21 = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r,
+                 I22 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      gx[a] = I00 * dxi[a] + I01 * deta[a] + I02 * dzeta[a];
+      gy[a] = I10 * dxi[a] + I11 * deta[a] + I12 * dzeta[a];
+      gz[a] = I20 * dxi[a] + I21 * deta[a] + I22 * dzeta[a];
+    }
+  }
+};
+
+__device__ __forceinline__ int q1_find_col(const int32_t* __restrict__ cols, int len, int32_t v)
+{
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One thread per node i.  MAT: zero row i, then for each cell of i add
+// lambda_c * sum_g (grad N_s . grad N_b) det J_g into the entries (i, node b);
+// *err = 1 when a column is missing.  RHS: rhs[i * nb_dof + dof_index] =
+// sum over the cells of f * scale_c * sum_g N_s det J_g.
+template <int DIM, bool MAT>
+__global__ void __launch_bounds__(256)
+k_assemble_q1(const double* __restrict__ coords, const int32_t* __restrict__ cells, int slot_bits,
+              const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+              const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+              const double* __restrict__ cell_lambda, double* __restrict__ values, double f,
+              const double* __restrict__ cell_scale, int nb_dof, int dof_index, double* __restrict__ rhs,
+              int32_t* __restrict__ err)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NG = DIM == 2 ? 4 : 8;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  int32_t a0 = 0, len = 0;
+  const int32_t* rc = nullptr;
+  if (MAT) {
+    a0 = row_index[i];
+    len = row_index[i + 1] - a0;
+    rc = columns + a0;
+    for (int k = 0; k < len; ++k)
+      values[a0 + k] = 0.0;
+  }
+  double src = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int s = it & ((1 << slot_bits) - 1);
+    const int32_t c = it >> slot_bits;
+    const int32_t* cn = cells + (it - s);
+    int32_t node[N];
+    double x[N][3];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      node[a] = cn[a];
+      const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)node[a]);
+      x[a][0] = xy.x;
+      x[a][1] = xy.y;
+      x[a][2] = coords[4 * (int64_t)node[a] + 2];
+    }
+    double row[N];
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+      row[b] = 0.0;
+    double ns = 0.0;
+    for (int g = 0; g < NG; ++g) {
+      Q1Point<DIM> q;
+      q.eval(x, g);
+      double sx = 0.0, sy = 0.0, sz = 0.0, sn = 0.0; // the values of slot s, selected without indexing
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        if (a == s) {
+          sx = q.gx[a];
+          sy = q.gy[a];
+          sz = q.gz[a];
+          sn = q.Nv[a];
+        }
+      if (MAT) {
+#pragma unroll
+        for (int b = 0; b < N; ++b)
+          row[b] += (sx * q.gx[b]) * q.det + (sy * q.gy[b]) * q.det + (sz * q.gz[b]) * q.det;
+      }
+      ns += sn * q.det;
+    }
+    if (rhs)
+      src += f * (cell_scale ? cell_scale[c] : 1.0) * ns;
+    if (MAT) {
+      const double lam = cell_lambda ? cell_lambda[c] : 1.0;
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        const int pos = q1_find_col(rc, len, node[b]);
+        if (pos < len && rc[pos] == node[b])
+          values[a0 + pos] += row[b] * lam;
+        else
+          atomicOr(err, 1);
+      }
+    }
+  }
+  if (rhs)
+    rhs[(int64_t)i * nb_dof + dof_index] = src;
+}
+
+template <bool MAT>
+void launch_q1(const afem_mesh& m, const int32_t* row_index, const int32_t* columns, const double* cell_lambda,
+               double* values, double f, const double* cell_scale, int nb_dof, int dof_index, double* rhs,
+               int32_t* err)
+{
+  const dim3 grid(grid_for(m.nb_node, 256)), block(256);
+  if (m.dim == 2)
+    hipLaunchKernelGGL((k_assemble_q1<2, MAT>), grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, row_index, columns, cell_lambda, values, f,
+                       cell_scale, nb_dof, dof_index, rhs, err);
+  else
+    hipLaunchKernelGGL((k_assemble_q1<3, MAT>), grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, row_index, columns, cell_lambda, values, f,
+                       cell_scale, nb_dof, dof_index, rhs, err);
+  check_launch("assemble_q1");
+}
+
+} // namespace
+
+void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  DBuf<int32_t> err(1);
+  err.zero();
+  ProfScope prof(m.dim == 2 ? "assemble_laplace_quad4" : "assemble_laplace_hexa8");
+  launch_q1<true>(m, b->row_index.p, b->columns.p, cell_lambda, b->values.p, f, nullptr, 1, 0, rhs, err.p);
+  int32_t h = 0;
+  err.download(&h, 1);
+  AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
+}
+
+void q1_rhs_constant_source(const afem_mesh& m, double f, const double* cell_scale, int nb_dof, int dof_index,
+                            double* rhs)
+{
+  ProfScope prof("rhs_constant_source_q1");
+  launch_q1<false>(m, nullptr, nullptr, nullptr, nullptr, f, cell_scale, nb_dof, dof_index, rhs, nullptr);
+}
+
+} // namespace afem

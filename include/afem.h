@@ -93,9 +93,15 @@ AFEM_API int afem_box_mesh(int dim, int n, double jitter, uint64_t seed,
  * (femutils/BSRFormat.cc:350-372). */
 typedef struct afem_mesh* afem_mesh_t;
 
-/* coords [host] nb_node*3, cells [host] nb_cell*4 (stride 4).  nodes_per_cell
- * is 3 (dim 2) or 4 (dim 3).  Fails with AFEM_ERR_MESH on out-of-range node
- * ids. */
+/* coords [host] nb_node*3, cells [host] nb_cell*stride.  nodes_per_cell is 3
+ * (tri3) or 4 (quad4) in dim 2, 4 (tet4) or 8 (hexa8) in dim 3; the stride is
+ * 4, or 8 for hexa8 (triangles pad slot 3 with -1).  Quad4 nodes go
+ * counter-clockwise, Hexa8 the bottom face counter-clockwise then the top
+ * face (the Gmsh / Arcane local order, femutils/ShapeFunctions.h:104-129,
+ * :291-350).  On Quad4 / Hexa8 meshes the sparsity, the Laplace assembly (one
+ * kernel: no variants, no autotuning) and the constant-source RHS run; the
+ * other element kernels return AFEM_ERR_NOT_IMPLEMENTED or AFEM_ERR_ARGUMENT.
+ * Fails with AFEM_ERR_MESH on out-of-range node ids. */
 AFEM_API int afem_mesh_create(int dim, int32_t nb_node, const double* coords,
                               int32_t nb_cell, int32_t nodes_per_cell, const int32_t* cells,
                               afem_mesh_t* mesh);

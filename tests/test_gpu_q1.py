@@ -1,0 +1,165 @@
+"""GPU parity of the Quad4 / Hexa8 (Q1) path through the C ABI against the
+oracle: node->cell lists and sparsity pattern bit-exact; Laplace values and
+constant-source RHS within 1e-12 of the row's / vector's largest entry; the
+reference's Q1 golden files (circle_2D_quad, sphere_3D_hexa: f = 5.5,
+u = 0.5 on 'horizontal', eps 1e-4) through a GPU solve; the element kernels
+that have no Q1 form refuse Q1 meshes."""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import oracle as O
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+ROW_TOL = 1e-12
+
+
+def row_scaled_error(row, ref, got):
+    lens = np.diff(row)
+    rid = np.repeat(np.arange(len(lens)), lens)
+    scale = np.zeros(len(lens))
+    np.maximum.at(scale, rid, np.abs(ref))
+    s = scale[rid]
+    return (np.abs(got - ref) / np.where(s > 0, s, 1.0)).max() if len(ref) else 0.0
+
+
+def q1_mesh(spec):
+    import arcanefem_amd as A
+    name, dim, n, jitter = spec
+    if dim is None:
+        d = np.load(os.path.join(GOLDEN, name + ".npz"))
+        return d, int(d["dim"]), d["coords"], d["cells"], int(d["npc"])
+    X, C = A.box_mesh_q1(dim, n, jitter, seed=13)
+    return None, dim, X, C, C.shape[1]
+
+
+SPECS = [("quad_box", 2, 19, 0.25), ("hexa_box", 3, 7, 0.2), ("circle_2D_quad", None, None, None),
+         ("sphere_3D_hexa", None, None, None)]
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=[s[0] for s in SPECS])
+def test_q1_pattern_values_rhs_match_oracle(afem_gpu, spec):
+    A = afem_gpu
+    print(f"[case] q1 parity {spec[0]}", flush=True)
+    _, dim, X, C, npc = q1_mesh(spec)
+    N = X.shape[0]
+    m = A.Mesh(dim, X, C, npc=npc)
+    off, items = m.node_cells()
+    ooff, oitems = O.node_cells(N, C, npc)
+    assert np.array_equal(off, ooff) and np.array_equal(items, oitems)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    orow, ocol = O.pattern(N, C, npc)
+    row, col, _, _ = b.download()
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+    lam = np.random.default_rng(5).uniform(0.2, 5.0, C.shape[0])
+    rhs = A.DeviceArray(N, np.float64)
+    b.assemble_laplace(A.DeviceArray.from_numpy(lam), 2.5, rhs)
+    _, _, val, _ = b.download()
+    oval = O.assemble(dim, X, C, npc, orow, ocol, cell_lambda=lam)
+    assert row_scaled_error(orow, oval, val) <= ROW_TOL
+    orhs = O.rhs_constant_source(dim, X, C, npc, 2.5)
+    assert np.abs(rhs.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max()
+    # the standalone source kernel, with a per-cell scale, into component 1 of 2
+    scale = np.random.default_rng(6).uniform(0.5, 1.5, C.shape[0])
+    rhs2 = A.DeviceArray(2 * N, np.float64)
+    rhs2.zero()
+    A.check(A.lib().afem_rhs_constant_source(m.h, 1.5, A.DeviceArray.from_numpy(scale).c, 2, 1, rhs2.c))
+    o2 = O.rhs_constant_source(dim, X, C, npc, 1.5, nb_dof=2, dof_index=1, cell_scale=scale)
+    assert np.abs(rhs2.numpy() - o2).max() <= 1e-12 * np.abs(o2).max()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 24), (3, 8)])
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_q1_solve_matches_direct(afem_gpu, dim, n, method):
+    """u = 0 on x = 0, f = 1: PCG on the GPU against a direct solve of the
+    oracle's system (1e-8 relative)."""
+    A = afem_gpu
+    print(f"[case] q1 solve dim={dim} {method}", flush=True)
+    X, C = A.box_mesh_q1(dim, n, 0.2, seed=3)
+    N, npc = X.shape[0], C.shape[1]
+    m = A.Mesh(dim, X, C, npc=npc)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    s = A.DoFLinearSystem(N)
+    b.assemble_laplace(None, 1.0, s.rhs())
+    s.set_bsr(b)
+    left = np.nonzero(X[:, 0] == 0.0)[0].astype(np.int32)
+    s.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, left, 1, 0, 0.0, 0.0)
+    s.apply_matrix_transformation()
+    s.apply_rhs_transformation()
+    code = A.SOLVER_PCG_FP64 if method == "fp64" else A.SOLVER_PCG_MIXED
+    r = s.solve(method=code, rtol=1e-11, max_iter=20000)
+    assert r.converged
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = O.rhs_constant_source(dim, X, C, npc, 1.0)
+    ei, ev = np.zeros(N, np.uint8), np.zeros(N)
+    ei[left] = 1
+    O.apply_transformations(row, col, val, rhs, np.zeros(N, np.uint8), np.zeros(N), ei, ev)
+    ref = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    x = s.solution().numpy()
+    assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def by_uid(d, x):
+    mp = dict(zip(d["node_uid"].tolist(), x))
+    return np.array([mp[u] for u in d["ref_uid"].tolist()])
+
+
+@pytest.mark.parametrize("case", ["circle_2D_quad", "sphere_3D_hexa"])
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_q1_golden_poisson_gpu(afem_gpu, case, method):
+    """modules/poisson tests 2D_quad / 3D_hexa: f = 5.5, u = 0.5 on
+    'horizontal' (row elimination), eps 1e-4."""
+    A = afem_gpu
+    print(f"[case] q1 golden {case} {method}", flush=True)
+    d, dim, X, C, npc = q1_mesh((case, None, None, None))
+    N = X.shape[0]
+    m = A.Mesh(dim, X, C, npc=npc)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    s = A.DoFLinearSystem(N)
+    b.assemble_laplace(None, 5.5, s.rhs())
+    s.set_bsr(b)
+    s.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, d["group_horizontal"], 1, 0, 0.5, 0.0)
+    s.apply_matrix_transformation()
+    s.apply_rhs_transformation()
+    code = A.SOLVER_PCG_FP64 if method == "fp64" else A.SOLVER_PCG_MIXED
+    r = s.solve(method=code, rtol=1e-12, max_iter=20000)
+    assert r.converged
+    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, s.solution().numpy()), 1e-4) == 0
+
+
+def test_q1_unsupported_kernels_refuse(afem_gpu):
+    A = afem_gpu
+    print("[case] q1 refusals", flush=True)
+    X, C = A.box_mesh_q1(3, 3, 0.1, seed=1)
+    N = X.shape[0]
+    m = A.Mesh(3, X, C)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    with pytest.raises(A.AfemError):
+        b.autotune_assembly()
+    with pytest.raises(A.AfemError):
+        b.assemble_heat(None, 0.1)
+    rhs = A.DeviceArray(N, np.float64)
+    with pytest.raises(A.AfemError):
+        A.rhs_mass_field(m, rhs, 1.0, rhs)
+    b3 = A.BSRFormat(m, 3, use_csr=False)
+    b3.compute_sparsity()
+    with pytest.raises(A.AfemError):
+        b3.assemble_elasticity(1.0, 1.0)
+    X2, C2 = A.box_mesh_q1(2, 3, 0.1, seed=1)
+    m2 = A.Mesh(2, X2, C2, npc=4)
+    f = np.array([0.0, 0.0, -1.0])
+    rhs3 = A.DeviceArray(3 * X2.shape[0], np.float64)
+    with pytest.raises(A.AfemError):
+        A.check(A.lib().afem_rhs_body_force(m2.h, f.ctypes.data_as(A.ctypes.c_void_p), rhs3.c))
