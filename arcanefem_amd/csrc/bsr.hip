@@ -1615,7 +1615,8 @@ k_neumann_faces(int dim, int32_t nface, const int32_t* __restrict__ faces, const
     }
   }
   if (opp < 0) {
-    contrib[f] = 0.0;
+    for (int a = 0; a < k; ++a)
+      contrib[(int64_t)k * f + a] = 0.0;
     atomicOr(err, 1);
     return;
   }
@@ -1644,13 +1645,14 @@ k_neumann_faces(int dim, int32_t nface, const int32_t* __restrict__ faces, const
     const double nx = c.x / norm, ny = c.y / norm, nz = c.z / norm;
     val = scalar ? g * area / 3.0 : (nx * gx + ny * gy + nz * gz) * area / 3.0;
   }
-  contrib[f] = val;
+  for (int a = 0; a < k; ++a) // every node of the face receives the same value
+    contrib[(int64_t)k * f + a] = val;
 }
 
 // rhs[node * nb_dof + dof_index] += the values of the node's faces, in face order
 __global__ void __launch_bounds__(256)
 k_neumann_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int32_t* __restrict__ off,
-                 const int32_t* __restrict__ face_of, const double* __restrict__ contrib, int nb_dof, int dof_index,
+                 const int32_t* __restrict__ pos_of, const double* __restrict__ contrib, int nb_dof, int dof_index,
                  double* __restrict__ rhs)
 {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1659,8 +1661,94 @@ k_neumann_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int3
   double* r = rhs + (int64_t)node_ids[t] * nb_dof + dof_index;
   double acc = *r;
   for (int32_t j = off[t]; j < off[t + 1]; ++j)
-    acc += contrib[face_of[j]];
+    acc += contrib[pos_of[j]];
   *r = acc;
+}
+
+// Per (face, slot) of a Quad4 mesh (edges) or a Hexa8 mesh (quadrangles, 4
+// nodes): the Gauss-point sum of femutils/ArcaneFemFunctionsGpu.cc:741-850
+// (2 points on the edge, outward normal as computeNormalFace,
+// ArcaneFemFunctionsGpu.h:158-178) and :1146-1320 (2x2 points; the normal is
+// t1 x t2 of the face's node order, not turned outward, as the reference
+// does).  *err = 1 when a face is not a face of any cell.
+__global__ void __launch_bounds__(256)
+k_neumann_faces_q1(int dim, int32_t nface, const int32_t* __restrict__ faces, const double* __restrict__ coords,
+                   const int32_t* __restrict__ cells, int npc, int slot_bits, const int32_t* __restrict__ nc_off,
+                   const int32_t* __restrict__ nc_items, int scalar, double g, double gx, double gy, double gz,
+                   double* __restrict__ contrib, int32_t* __restrict__ err)
+{
+  const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nface)
+    return;
+  const int k = dim == 2 ? 2 : 4;
+  const int32_t* fn = faces + (int64_t)k * f;
+  int32_t inner = -1; // a node of the face's cell that is not on the face
+  for (int32_t e = nc_off[fn[0]]; e < nc_off[fn[0] + 1] && inner < 0; ++e) {
+    const int32_t it = nc_items[e];
+    const int32_t* cn = cells + (it - (it & ((1 << slot_bits) - 1)));
+    int hits = 0;
+    for (int a = 1; a < k; ++a)
+      for (int s = 0; s < npc; ++s)
+        hits += cn[s] == fn[a];
+    if (hits != k - 1)
+      continue;
+    for (int s = 0; s < npc && inner < 0; ++s) {
+      bool on_face = false;
+      for (int a = 0; a < k; ++a)
+        on_face = on_face || cn[s] == fn[a];
+      if (!on_face)
+        inner = cn[s];
+    }
+  }
+  double* out = contrib + (int64_t)k * f;
+  for (int a = 0; a < k; ++a)
+    out[a] = 0.0;
+  if (inner < 0) {
+    atomicOr(err, 1);
+    return;
+  }
+  constexpr double gp[2] = { -0.57735026918962576451, 0.57735026918962576451 };
+  if (dim == 2) {
+    const d3 a = load_node(coords, fn[0]), b = load_node(coords, fn[1]), xi = load_node(coords, inner);
+    const double length = sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y));
+    double nx = (b.y - a.y) / length, ny = (a.x - b.x) / length;
+    if (nx * (xi.x - a.x) + ny * (xi.y - a.y) > 0.0) { // towards the cell: turn outward
+      nx = -nx;
+      ny = -ny;
+    }
+    const double v = scalar ? g : nx * gx + ny * gy;
+    for (int i = 0; i < 2; ++i) {
+      const double iw = 1.0 * length * 0.5;
+      out[0] += v * (0.5 * (1.0 - gp[i])) * iw;
+      out[1] += v * (0.5 * (1.0 + gp[i])) * iw;
+    }
+    return;
+  }
+  d3 x[4];
+  for (int a = 0; a < 4; ++a)
+    x[a] = load_node(coords, fn[a]);
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy) {
+      const double xi = gp[ix], eta = gp[iy];
+      const double N[4] = { 0.25 * (1.0 - xi) * (1.0 - eta), 0.25 * (1.0 + xi) * (1.0 - eta),
+                            0.25 * (1.0 + xi) * (1.0 + eta), 0.25 * (1.0 - xi) * (1.0 + eta) };
+      const double dxi[4] = { -0.25 * (1.0 - eta), 0.25 * (1.0 - eta), 0.25 * (1.0 + eta), -0.25 * (1.0 + eta) };
+      const double deta[4] = { -0.25 * (1.0 - xi), -0.25 * (1.0 + xi), 0.25 * (1.0 + xi), 0.25 * (1.0 - xi) };
+      d3 t1 = { 0.0, 0.0, 0.0 }, t2 = { 0.0, 0.0, 0.0 };
+      for (int a = 0; a < 4; ++a) {
+        t1.x += dxi[a] * x[a].x;
+        t1.y += dxi[a] * x[a].y;
+        t1.z += dxi[a] * x[a].z;
+        t2.x += deta[a] * x[a].x;
+        t2.y += deta[a] * x[a].y;
+        t2.z += deta[a] * x[a].z;
+      }
+      const d3 n = cross(t1, t2);
+      const double det = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+      const double v = scalar ? g : (n.x / det) * gx + (n.y / det) * gy + (n.z / det) * gz;
+      for (int a = 0; a < 4; ++a)
+        out[a] += v * N[a] * (1.0 * 1.0 * det);
+    }
 }
 
 // |F| of each face: computeLengthEdge2 (femutils/ArcaneFemFunctions.h:375-381)
@@ -1691,7 +1779,7 @@ k_face_measure(int dim, int32_t nface, const int32_t* __restrict__ faces, const 
 // pattern.
 __global__ void __launch_bounds__(256)
 k_convection_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int32_t* __restrict__ off,
-                    const int32_t* __restrict__ face_of, const int32_t* __restrict__ faces, int k,
+                    const int32_t* __restrict__ pos_of, const int32_t* __restrict__ faces, int k,
                     const double* __restrict__ measure, double hf, const int32_t* __restrict__ row_index,
                     const int32_t* __restrict__ columns, double* __restrict__ values, int check_only,
                     int32_t* __restrict__ err)
@@ -1703,7 +1791,7 @@ k_convection_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const i
   const int32_t a = row_index[i], len = row_index[i + 1] - a;
   const int32_t* rc = columns + a;
   for (int32_t j = off[t]; j < off[t + 1]; ++j) {
-    const int32_t f = face_of[j];
+    const int32_t f = pos_of[j] / k;
     const int32_t* fn = faces + (int64_t)k * f;
     for (int b = 0; b < k; ++b) {
       const int pos = find_col(rc, len, fn[b]);
@@ -2274,19 +2362,20 @@ static void bsr_to_csr(afem_bsr* b)
 }
 
 // Boundary faces of k nodes on the device with, for each node they touch (ids,
-// sorted), its faces in increasing order: face_of[off[t]..off[t+1]).  Gather
-// kernels walk it so that every sum runs in face order, with no atomics.
+// sorted), its (face, slot) positions f * k + slot in increasing order:
+// pos_of[off[t]..off[t+1]).  Gather kernels walk it so that every sum runs in
+// face order, with no atomics.
 struct FaceIndex
 {
   int32_t nb_node = 0;
-  DBuf<int32_t> faces, ids, off, face_of;
+  DBuf<int32_t> faces, ids, off, pos_of;
   FaceIndex(const int32_t* face_nodes, int nb_face, int k)
   {
     const size_t nf = (size_t)nb_face * k;
     std::vector<int32_t> h_ids(face_nodes, face_nodes + nf);
     std::sort(h_ids.begin(), h_ids.end());
     h_ids.erase(std::unique(h_ids.begin(), h_ids.end()), h_ids.end());
-    std::vector<int32_t> h_off(h_ids.size() + 1, 0), h_face_of(nf);
+    std::vector<int32_t> h_off(h_ids.size() + 1, 0), h_pos_of(nf);
     auto slot = [&](int32_t v) { return std::lower_bound(h_ids.begin(), h_ids.end(), v) - h_ids.begin(); };
     for (size_t j = 0; j < nf; ++j)
       ++h_off[slot(face_nodes[j]) + 1];
@@ -2294,16 +2383,16 @@ struct FaceIndex
       h_off[t + 1] += h_off[t];
     std::vector<int32_t> fill(h_off.begin(), h_off.end() - 1);
     for (size_t j = 0; j < nf; ++j)
-      h_face_of[fill[slot(face_nodes[j])]++] = (int32_t)(j / k);
+      h_pos_of[fill[slot(face_nodes[j])]++] = (int32_t)j;
     nb_node = (int32_t)h_ids.size();
     faces.alloc(nf);
     ids.alloc(h_ids.size());
     off.alloc(h_off.size());
-    face_of.alloc(nf);
+    pos_of.alloc(nf);
     faces.upload(face_nodes, nf);
     ids.upload(h_ids.data(), h_ids.size());
     off.upload(h_off.data(), h_off.size());
-    face_of.upload(h_face_of.data(), nf);
+    pos_of.upload(h_pos_of.data(), nf);
   }
 };
 
@@ -2507,7 +2596,7 @@ AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t*
   const double hf = h * (k == 2 ? 1 / 6. : 1 / 12.);
   for (int check_only = 1; check_only >= 0; --check_only) { // a failed call leaves the values as they were
     hipLaunchKernelGGL(k_convection_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
-                       fx.ids.p, fx.off.p, fx.face_of.p, fx.faces.p, k, measure.p, hf, bsr->row_index.p,
+                       fx.ids.p, fx.off.p, fx.pos_of.p, fx.faces.p, k, measure.p, hf, bsr->row_index.p,
                        bsr->columns.p, bsr->values.p, check_only, err.p);
     check_launch("convection_gather");
     if (check_only) {
@@ -2528,30 +2617,38 @@ AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face
   AFEM_REQUIRE(nb_value == 1 || nb_value == mesh->dim, AFEM_ERR_ARGUMENT,
                "a Neumann value is a scalar or a vector of the mesh dimension");
   AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
-  AFEM_REQUIRE(mesh->npc == mesh->dim + 1, AFEM_ERR_NOT_IMPLEMENTED, "Neumann faces of simplex meshes only");
   if (nb_face == 0)
     return AFEM_OK;
-  const int k = mesh->dim;
+  const int k = mesh->npc == 8 ? 4 : mesh->dim; // nodes per face
   const size_t nf = (size_t)nb_face * k;
   for (size_t j = 0; j < nf; ++j)
     AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < mesh->nb_node, AFEM_ERR_ARGUMENT,
                  "Neumann face node out of range");
   FaceIndex fx(face_nodes, nb_face, k);
   DBuf<int32_t> err(1);
-  DBuf<double> contrib((size_t)nb_face);
+  DBuf<double> contrib((size_t)nb_face * k);
   err.zero();
   const bool scalar = nb_value == 1;
   ProfScope prof("rhs_neumann");
-  hipLaunchKernelGGL(k_neumann_faces, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), mesh->dim, nb_face,
-                     fx.faces.p, mesh->coords.p, mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p, scalar ? 1 : 0,
-                     values[0], scalar ? 0.0 : values[0], scalar ? 0.0 : values[1],
-                     (!scalar && k == 3) ? values[2] : 0.0, contrib.p, err.p);
+  const double gz = (!scalar && mesh->dim == 3) ? values[2] : 0.0;
+  if (!mesh->simplex) {
+    hipLaunchKernelGGL(k_neumann_faces_q1, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), mesh->dim, nb_face,
+                       fx.faces.p, mesh->coords.p, mesh->cells.p, mesh->npc, mesh->slot_bits, mesh->nc_offsets.p,
+                       mesh->nc_items.p, scalar ? 1 : 0, values[0], scalar ? 0.0 : values[0],
+                       scalar ? 0.0 : values[1], gz, contrib.p, err.p);
+  }
+  else {
+    hipLaunchKernelGGL(k_neumann_faces, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), mesh->dim, nb_face,
+                       fx.faces.p, mesh->coords.p, mesh->cells.p, mesh->nc_offsets.p, mesh->nc_items.p,
+                       scalar ? 1 : 0, values[0], scalar ? 0.0 : values[0], scalar ? 0.0 : values[1], gz, contrib.p,
+                       err.p);
+  }
   check_launch("neumann_faces");
   int32_t h = 0;
   err.download(&h, 1); // before rhs is touched: a failed call leaves it as it was
   AFEM_REQUIRE(h == 0, AFEM_ERR_MESH, "a Neumann face is not a face of any cell");
   hipLaunchKernelGGL(k_neumann_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
-                     fx.ids.p, fx.off.p, fx.face_of.p, contrib.p, nb_dof, dof_index, rhs);
+                     fx.ids.p, fx.off.p, fx.pos_of.p, contrib.p, nb_dof, dof_index, rhs);
   check_launch("neumann_gather");
   AFEM_GUARD_END
 }

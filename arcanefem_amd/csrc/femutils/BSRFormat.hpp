@@ -135,8 +135,9 @@ namespace BoundaryConditions {
     check(afem_rhs_body_force(mesh.handle(), f, rhs.devicePointer()), "applyBodyForceToRhs");
   }
 
-  // femutils/ArcaneFemFunctionsGpu.cc:269-310 on tri3 / tet4 meshes (edges /
-  // triangles, femutils/ArcaneFemFunctions.h:2715-2759, :1793-1840): one
+  // femutils/ArcaneFemFunctionsGpu.cc:269-310 (edges / triangles,
+  // femutils/ArcaneFemFunctions.h:2715-2759, :1793-1840; the Gauss-point
+  // forms of :741-850 and :1146-1320 on Quad4 / Hexa8 meshes): one
   // value is a scalar flux g, dim values ("NULL" = 0) the flux vector whose
   // component along the outward normal enters the RHS.
   inline void applyNeumannToRhs(const Mesh& mesh, const std::string& group, const std::vector<std::string>& values,
@@ -153,7 +154,7 @@ namespace BoundaryConditions {
         v.push_back((size_t)d < values.size() && values[d] != "NULL" ? std::stod(values[d]) : 0.0);
     }
     const std::vector<Int32>& faces = mesh.faceGroup(group);
-    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), (int)v.size(),
+    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), (int)v.size(),
                            v.data(), 1, 0, rhs.devicePointer()),
           "applyNeumannToRhs");
   }
@@ -163,7 +164,7 @@ namespace BoundaryConditions {
   inline void applyConvectionToLhs(const Mesh& mesh, const std::string& group, Real h, BSRFormat& bsr)
   {
     const std::vector<Int32>& faces = mesh.faceGroup(group);
-    check(afem_bsr_add_convection(bsr.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), h),
+    check(afem_bsr_add_convection(bsr.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), h),
           "applyConvectionToLhs");
   }
 
@@ -174,7 +175,7 @@ namespace BoundaryConditions {
   {
     const std::vector<Int32>& faces = mesh.faceGroup(group);
     const Real g = h * t_ext;
-    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.dimension()), faces.data(), 1, &g, 1, 0,
+    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), 1, &g, 1, 0,
                            rhs.devicePointer()),
           "applyConvectionToRhs");
   }

@@ -57,12 +57,14 @@ class Mesh
       afem_mesh_destroy(m_handle);
   }
 
-  // Host arrays: coordinates (N x 3), cells (C x 4, stride 4), node unique
-  // ids; upload = false keeps the mesh on the host (parse-only use).
+  // Host arrays: coordinates (N x 3), cells (C x stride: 4, or 8 for Hexa8),
+  // node unique ids; npc = nodes per cell (0: dim + 1, a simplex mesh);
+  // upload = false keeps the mesh on the host (parse-only use).
   void build(int dim, std::vector<Real> coords, std::vector<Int32> cells, std::vector<Int64> uids,
-             bool upload = true)
+             bool upload = true, int npc = 0)
   {
     m_dim = dim;
+    m_npc = npc > 0 ? npc : dim + 1;
     m_coords = std::move(coords);
     m_cells = std::move(cells);
     m_uids = std::move(uids);
@@ -70,7 +72,7 @@ class Mesh
       afem_mesh_destroy(m_handle);
     m_handle = nullptr;
     if (upload)
-      check(afem_mesh_create(dim, nbNode(), m_coords.data(), nbCell(), dim + 1, m_cells.data(), &m_handle),
+      check(afem_mesh_create(dim, nbNode(), m_coords.data(), nbCell(), m_npc, m_cells.data(), &m_handle),
             "afem_mesh_create");
   }
   const std::map<std::string, NodeGroup>& groups() const { return m_groups; }
@@ -90,8 +92,9 @@ class Mesh
   }
 
   // Binary Gmsh 4.1 (size_t = 8): nodes, the cells of the highest dimension
-  // (tri3 or tet4), the node and face groups of the (dim-1) physical names and
-  // the node groups of the named physical points.
+  // (tri3 or quad4 in 2D, tet4 or hexa8 in 3D, one kind per mesh), the node
+  // and face groups of the (dim-1) physical names and the node groups of the
+  // named physical points.
   void readGmsh41(const std::string& path, bool upload = true)
   {
     std::ifstream f(path, std::ios::binary);
@@ -135,7 +138,8 @@ class Mesh
       std::vector<uint64_t> conn;
     };
     std::vector<Block> blocks;
-    static const std::map<int, int> nodes_of = { { 15, 1 }, { 1, 2 }, { 2, 3 }, { 4, 4 } };
+    // Gmsh element types: point, line, triangle, quadrangle, tetrahedron, hexahedron
+    static const std::map<int, int> nodes_of = { { 15, 1 }, { 1, 2 }, { 2, 3 }, { 3, 4 }, { 4, 4 }, { 5, 8 } };
     while (p < data.size()) {
       std::string head = line();
       if (head.empty())
@@ -253,11 +257,20 @@ class Mesh
     std::vector<Int32> cells;
     m_groups.clear();
     m_face_groups.clear();
+    m_face_nodes = 0;
+    int npc = 0;
+    for (auto& b : blocks)
+      if (b.dim == dim) {
+        if (npc != 0 && npc != b.k)
+          throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "meshes mixing cell types are not supported");
+        npc = b.k;
+      }
+    const int stride = npc > 4 ? npc : 4;
     for (auto& b : blocks) {
       const size_t n = b.conn.size() / (b.k + 1);
       if (b.dim == dim) {
         for (size_t e = 0; e < n; ++e)
-          for (int s = 0; s < 4; ++s)
+          for (int s = 0; s < stride; ++s)
             cells.push_back(s < b.k ? lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]) : -1);
       }
       else if (b.dim == dim - 1 || (b.dim == 0 && dim > 1)) { // faces, or named points
@@ -269,11 +282,31 @@ class Mesh
           for (size_t e = 0; e < n; ++e)
             for (int s = 0; s < b.k; ++s)
               g.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
-          if (b.dim == dim - 1 && b.k == dim) { // simplex faces (edges in 2D, triangles in 3D), in file order
+          if (b.dim == dim - 1 && (b.k == dim || (dim == 3 && b.k == 4))) {
+            // faces in file order: edges (2D), triangles or quadrangles (3D)
+            if (m_face_nodes != 0 && m_face_nodes != b.k)
+              throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "meshes mixing face types are not supported");
+            m_face_nodes = b.k;
             auto& fg = m_face_groups[nm->second];
-            for (size_t e = 0; e < n; ++e)
+            for (size_t e = 0; e < n; ++e) {
+              Int32 f[4];
               for (int s = 0; s < b.k; ++s)
-                fg.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
+                f[s] = lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]);
+              if (b.k == 4) { // quadrangles in Arcane's face order (smallest uid first, towards its smaller neighbour)
+                const Int64* u = uids.data();
+                int i0 = 0;
+                for (int s = 1; s < 4; ++s)
+                  if (u[f[s]] < u[f[i0]])
+                    i0 = s;
+                const int step = u[f[(i0 + 1) % 4]] < u[f[(i0 + 3) % 4]] ? 1 : 3;
+                for (int s = 0; s < 4; ++s)
+                  fg.push_back(f[(i0 + step * s) % 4]);
+              }
+              else {
+                for (int s = 0; s < b.k; ++s)
+                  fg.push_back(f[s]);
+              }
+            }
           }
         }
       }
@@ -282,13 +315,17 @@ class Mesh
       std::sort(kv.second.begin(), kv.second.end());
       kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
     }
-    build(dim, std::move(coords), std::move(cells), std::move(uids), upload);
+    build(dim, std::move(coords), std::move(cells), std::move(uids), upload, npc);
   }
 
   afem_mesh_t handle() const { return m_handle; }
   int dimension() const { return m_dim; }
   Int32 nbNode() const { return (Int32)(m_coords.size() / 3); }
-  Int32 nbCell() const { return (Int32)(m_cells.size() / 4); }
+  Int32 nbCell() const { return (Int32)(m_cells.size() / (m_npc > 4 ? m_npc : 4)); }
+  int nodesPerCell() const { return m_npc; }
+  // nodes per boundary face of the face groups: dimension() for simplices and
+  // Quad4, 4 for Hexa8
+  int nodesPerFace() const { return m_face_nodes != 0 ? m_face_nodes : (m_npc == 8 ? 4 : m_dim); }
   const std::vector<Real>& coords() const { return m_coords; }
   const std::vector<Int32>& cells() const { return m_cells; }
   const std::vector<Int64>& uniqueIds() const { return m_uids; }
@@ -299,7 +336,7 @@ class Mesh
       throw FatalError(AFEM_ERR_ARGUMENT, "no boundary group '" + name + "'");
     return it->second;
   }
-  // faces of a named boundary group: dimension() node ids per face
+  // faces of a named boundary group: nodesPerFace() node ids per face
   const std::vector<Int32>& faceGroup(const std::string& name) const
   {
     auto it = m_face_groups.find(name);
@@ -319,6 +356,8 @@ class Mesh
 
  private:
   int m_dim = 0;
+  int m_npc = 0;
+  int m_face_nodes = 0;
   std::vector<Real> m_coords;
   std::vector<Int32> m_cells;
   std::vector<Int64> m_uids;

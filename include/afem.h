@@ -203,14 +203,19 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
  * h * Text.  nb_dof = 1; AFEM_ERR_NOT_FOUND when a pair is outside the
  * pattern (a face that is not a face of the mesh), values untouched. */
 AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t* face_nodes, double h);
-/* BoundaryConditions::applyNeumannToRhs for tri3 / tet4 meshes
- * (femutils/ArcaneFemFunctions.h:2715-2759 edges, :1793-1840 triangles; GPU
- * femutils/ArcaneFemFunctionsGpu.cc:679-738): for each boundary face F
- * (face_nodes [host]: nb_face * dim node ids) and each node i of F,
- * rhs[i*nb_dof+dof_index] += g |F| / dim (nb_value = 1) or (g . n_F) |F| / dim
- * (nb_value = dim, values [host] the vector g, n_F the outward unit normal of
- * F).  The faces' values are summed per node in face order (no atomics).
- * AFEM_ERR_MESH when a face is not a face of any cell. */
+/* BoundaryConditions::applyNeumannToRhs (femutils/ArcaneFemFunctions.h:
+ * 2715-2759 edges, :1793-1840 triangles; GPU femutils/ArcaneFemFunctionsGpu.cc:
+ * 679-738, and :741-850 / :1146-1320 for Quad4 / Hexa8 meshes): for each
+ * boundary face F (face_nodes [host]: nb_face * k node ids, k = dim, or 4 on
+ * a Hexa8 mesh) and each node i of F, rhs[i*nb_dof+dof_index] += g |F| / dim
+ * (nb_value = 1) or (g . n_F) |F| / dim (nb_value = dim, values [host] the
+ * vector g, n_F the outward unit normal of F) on simplex meshes; on Quad4 /
+ * Hexa8 meshes the same integrals of g N_i over F with 2 (edge) or 2x2
+ * (quadrangle) Gauss points.  A quadrangle's normal is t1 x t2 of its node
+ * order, as the reference takes it from Arcane's face order (first the node
+ * of smallest unique id, then towards its neighbour of smaller id): pass the
+ * faces in that order.  The faces' values are summed per node in face order
+ * (no atomics).  AFEM_ERR_MESH when a face is not a face of any cell. */
 AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
                               const double* values, int nb_dof, int dof_index, double* rhs);
 /* integrateNodalFieldToRhs (femutils/ArcaneFemFunctions.h:2558-2590 tri,

@@ -68,6 +68,9 @@ def lib():
         L.afo_source_q1.argtypes = [ctypes.c_int, vp, dbl, vp]
         L.afo_q1_gradients.restype = dbl
         L.afo_q1_gradients.argtypes = [ctypes.c_int, vp, dbl, dbl, dbl, vp, vp, vp, vp]
+        L.afo_rhs_neumann_q1.restype = ctypes.c_int
+        L.afo_rhs_neumann_q1.argtypes = [ctypes.c_int, vp, i32, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, vp]
         L.afo_convection_lhs.restype = ctypes.c_int
         L.afo_convection_lhs.argtypes = [ctypes.c_int, vp, i32, vp, dbl, vp, vp, vp]
         L.afo_apply_transformations.restype = ctypes.c_int
@@ -230,6 +233,39 @@ def convection_lhs(dim, coords, faces, h, row, cols, values):
     if rc != 0:
         raise ValueError("convection: a face pair is not in the pattern")
     return values
+
+
+def arcane_face_order(faces, node_uid):
+    """Face node lists in Arcane's face order: start at the node of smallest
+    unique id, then walk towards its neighbour (in the face's cycle) of
+    smaller unique id.  Arcane builds faces this way when it reads a mesh;
+    the reference's Hexa8 Neumann normal (t1 x t2 of that order,
+    femutils/ArcaneFemFunctionsGpu.cc:1146-1320) depends on it.  Arcane is a
+    third-party dependency absent here; the rule is pinned by the reference's
+    poisson_test_ref_sphere_neumann_3D_hexa.txt."""
+    faces = np.asarray(faces)
+    uid = np.asarray(node_uid)
+    out = np.empty_like(faces)
+    k = faces.shape[1]
+    for r, f in enumerate(faces):
+        u = uid[f]
+        i = int(np.argmin(u))
+        step = 1 if uid[f[(i + 1) % k]] < uid[f[(i - 1) % k]] else -1
+        out[r] = [f[(i + step * j) % k] for j in range(k)]
+    return out
+
+
+def rhs_neumann_q1(dim, coords, cells, npc, faces, value, nb_dof=1, dof_index=0, outward=False, rhs=None):
+    """Quad4 / Hexa8 meshes: faces (nf, 2) edges or (nf, 4) quadrangles."""
+    coords, cells, faces = _f64(coords), _i32(cells), _i32(faces)
+    vals = _f64(np.atleast_1d(np.asarray(value, dtype=np.float64)))
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * nb_dof)
+    rc = lib().afo_rhs_neumann_q1(dim, _P(coords), cells.shape[0], npc, _P(cells), faces.shape[0], _P(faces),
+                                  vals.size, _P(vals), nb_dof, dof_index, 1 if outward else 0, _P(rhs))
+    if rc != 0:
+        raise ValueError("a Neumann face is not a face of any cell")
+    return rhs
 
 
 def max_threads():

@@ -781,6 +781,92 @@ int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int
   return 0;
 }
 
+/* Neumann condition on the boundary faces of a Quad4 mesh (edges, 2 nodes) or
+ * a Hexa8 mesh (quadrangles, 4 nodes), femutils/ArcaneFemFunctionsGpu.cc:
+ * 741-850 and :1146-1320: Gauss points on the face (2 on an edge, 2x2 on a
+ * quadrangle), rhs[node j] += g N_j w |J| or (g . n) N_j w |J|.  Edges take
+ * the outward normal (computeNormalFace, ArcaneFemFunctionsGpu.h:158-178);
+ * quadrangles the normal t1 x t2 of the face's node order when
+ * outward == 0, turned away from the cell when outward == 1.
+ * Returns -1 when a face is not a face of any cell. */
+int afo_rhs_neumann_q1(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                       int32_t nb_face, const int32_t* faces, int nb_value, const double* values, int nb_dof,
+                       int dof_index, int outward, double* rhs)
+{
+  const int k = dim == 2 ? 2 : 4, st = AFO_CELL_STRIDE(npc);
+  const int scalar = nb_value == 1;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int32_t f = 0; f < nb_face; ++f) {
+    const int32_t* fn = faces + (int64_t)f * k;
+    /* a node of the face's cell that is not on the face (the inner side) */
+    int32_t inner = -1;
+    for (int32_t c = 0; c < nb_cell && inner < 0; ++c) {
+      const int32_t* cn = cells + st * (int64_t)c;
+      int hits = 0;
+      for (int a = 0; a < k; ++a)
+        for (int s2 = 0; s2 < npc; ++s2)
+          hits += cn[s2] == fn[a];
+      if (hits != k)
+        continue;
+      for (int s2 = 0; s2 < npc && inner < 0; ++s2) {
+        int on = 0;
+        for (int a = 0; a < k; ++a)
+          on |= cn[s2] == fn[a];
+        if (!on)
+          inner = cn[s2];
+      }
+    }
+    if (inner < 0)
+      return -1;
+    const double* xi_ = coords + 3 * (int64_t)inner;
+    if (dim == 2) {
+      const double* a = coords + 3 * (int64_t)fn[0];
+      const double* b = coords + 3 * (int64_t)fn[1];
+      const double length = sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]));
+      double nx = (b[1] - a[1]) / length, ny = (a[0] - b[0]) / length;
+      if (nx * (xi_[0] - a[0]) + ny * (xi_[1] - a[1]) > 0.0) {
+        nx = -nx;
+        ny = -ny;
+      }
+      for (int i = 0; i < 2; ++i) {
+        const double N[2] = { 0.5 * (1.0 - gp[i]), 0.5 * (1.0 + gp[i]) };
+        const double iw = 1.0 * length * 0.5;
+        for (int j = 0; j < 2; ++j)
+          rhs[(int64_t)fn[j] * nb_dof + dof_index] +=
+            (scalar ? values[0] : nx * values[0] + ny * values[1]) * N[j] * iw;
+      }
+      continue;
+    }
+    double X[12];
+    for (int a = 0; a < 4; ++a)
+      memcpy(X + 3 * a, coords + 3 * (int64_t)fn[a], 3 * sizeof(double));
+    for (int ix = 0; ix < 2; ++ix)
+      for (int iy = 0; iy < 2; ++iy) {
+        double N[8], dxi[8], deta[8], dz[8];
+        q1_shape(2, gp[ix], gp[iy], 0.0, N, dxi, deta, dz);
+        double t1[3] = { 0, 0, 0 }, t2[3] = { 0, 0, 0 }, n[3];
+        for (int a = 0; a < 4; ++a)
+          for (int d = 0; d < 3; ++d) {
+            t1[d] += dxi[a] * X[3 * a + d];
+            t2[d] += deta[a] * X[3 * a + d];
+          }
+        cross3(t1, t2, n);
+        const double det = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        double u[3] = { n[0] / det, n[1] / det, n[2] / det };
+        if (outward && u[0] * (xi_[0] - X[0]) + u[1] * (xi_[1] - X[1]) + u[2] * (xi_[2] - X[2]) > 0.0) {
+          u[0] = -u[0];
+          u[1] = -u[1];
+          u[2] = -u[2];
+        }
+        const double iw = 1.0 * 1.0 * det;
+        for (int j = 0; j < 4; ++j)
+          rhs[(int64_t)fn[j] * nb_dof + dof_index] +=
+            (scalar ? values[0] : u[0] * values[0] + u[1] * values[1] + u[2] * values[2]) * N[j] * iw;
+      }
+  }
+  return 0;
+}
+
 /* ---- transformations ------------------------------------------------------------ */
 
 int afo_apply_transformations(int32_t nb_row, const int32_t* row_index, const int32_t* columns,

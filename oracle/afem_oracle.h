@@ -161,6 +161,15 @@ void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* ce
 int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int32_t* faces, double h,
                        const int32_t* row_index, const int32_t* columns, double* values);
 
+/* Neumann condition on the faces of a Quad4 mesh (edges) or a Hexa8 mesh
+ * (quadrangles, 4 nodes per face) with Gauss points on the face:
+ * femutils/ArcaneFemFunctionsGpu.cc:741-850 (edges, outward normal) and
+ * :1146-1320 (quadrangles: normal t1 x t2 of the face node order, or turned
+ * away from the cell when outward = 1).  -1 when a face has no cell. */
+int afo_rhs_neumann_q1(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                       int32_t nb_face, const int32_t* faces, int nb_value, const double* values, int nb_dof,
+                       int dof_index, int outward, double* rhs);
+
 /* Neumann condition on boundary faces (dim nodes each): rhs[dof(n)] += g*|F|/dim
  * (nb_value 1) or (g.n_F)*|F|/dim (nb_value dim, n_F the outward unit normal).
  * femutils/ArcaneFemFunctions.h:2715-2759 (edges), :1793-1840 (triangles).

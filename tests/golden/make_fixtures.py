@@ -15,6 +15,7 @@ golden solution files, and writes one .npz per case with:
   faces_<name>(f, dim) int32   those faces (local node ids, file order)
   ref_uid     (M,)   int64     uids listed in the golden file
   ref_value   (M, d) float64   golden values (d = 1 or 3)
+  ref_uid_<t>, ref_value_<t>   further golden files on the same mesh (EXTRA)
 
 Golden cases (test names and tolerances from the reference's CMake files):
   circle_2D   modules/poisson/inputs/circle.2D.arc, check/poisson_test_ref_circle_2D.txt
@@ -50,6 +51,13 @@ Golden cases (test names and tolerances from the reference's CMake files):
               'horizontal', penalty)
   sphere_3D_hexa  modules/poisson/inputs/sphere.3D.hexa.arc,
               check/poisson_test_ref_sphere_3D_hexa.txt (Hexa8; the same conditions)
+  circle_2D_quad extra 'neumann': inputs/circle.neumann.2D.quad.arc,
+              check/poisson_test_ref_circle_neumann_2D_quad.txt (flux (-0.35, 1.65) on
+              'curved'); 'scalar_neumann': the same input with value 2,
+              check/poisson_test_ref_circle_scalar_neumann_2D_quad.txt
+  sphere_3D_hexa extra 'neumann': inputs/sphere.neumann.3D.hexa.arc,
+              check/poisson_test_ref_sphere_neumann_3D_hexa.txt (flux (0.35, 1.65, 3.75));
+              'scalar_neumann': value 3.1, check/poisson_test_ref_sphere_scalar_neumann_3D_hexa.txt
   cube_3D_hexa8  modules/poisson/inputs/cube.3D.hexa.arc on meshes/3x3x3_cube_hexa8.msh,
               check/poisson_test_ref_cube_3D_hexa8.txt (f=9.8, Dirichlet 0.5 on 'left',
               scalar Neumann 13.9 on 'right')
@@ -85,6 +93,17 @@ CASES = {
     "circle_2D_quad": ("meshes/msh/circle_cut.quad.msh", "modules/poisson/check/poisson_test_ref_circle_2D_quad.txt", 1),
     "sphere_3D_hexa": ("meshes/msh/sphere_cut.hexa.msh", "modules/poisson/check/poisson_test_ref_sphere_3D_hexa.txt", 1),
     "cube_3D_hexa8": ("meshes/msh/3x3x3_cube_hexa8.msh", "modules/poisson/check/poisson_test_ref_cube_3D_hexa8.txt", 1),
+}
+
+EXTRA = {
+    "circle_2D_quad": {
+        "neumann": "modules/poisson/check/poisson_test_ref_circle_neumann_2D_quad.txt",
+        "scalar_neumann": "modules/poisson/check/poisson_test_ref_circle_scalar_neumann_2D_quad.txt",
+    },
+    "sphere_3D_hexa": {
+        "neumann": "modules/poisson/check/poisson_test_ref_sphere_neumann_3D_hexa.txt",
+        "scalar_neumann": "modules/poisson/check/poisson_test_ref_sphere_scalar_neumann_3D_hexa.txt",
+    },
 }
 
 # Gmsh element type -> number of nodes (point, line, triangle, quad, tet, hexa)
@@ -239,6 +258,11 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     ref = np.loadtxt(os.path.join(ref_root, check_rel), ndmin=2)
     payload["ref_uid"] = ref[:, 0].astype(np.int64)
     payload["ref_value"] = ref[:, 1:1 + ncomp].astype(np.float64)
+    for tag, rel in EXTRA.get(name, {}).items():
+        ex = np.loadtxt(os.path.join(ref_root, rel), ndmin=2)
+        payload["ref_uid_" + tag] = ex[:, 0].astype(np.int64)
+        payload["ref_value_" + tag] = ex[:, 1:1 + ncomp].astype(np.float64)
+        shutil.copyfile(os.path.join(ref_root, rel), os.path.join(out_dir, os.path.basename(rel)))
     np.savez(os.path.join(out_dir, name + ".npz"), **payload)
     shutil.copyfile(mesh_path, os.path.join(out_dir, os.path.basename(mesh_rel)))
     shutil.copyfile(os.path.join(ref_root, check_rel), os.path.join(out_dir, os.path.basename(check_rel)))

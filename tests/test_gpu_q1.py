@@ -163,3 +163,79 @@ def test_q1_unsupported_kernels_refuse(afem_gpu):
     rhs3 = A.DeviceArray(3 * X2.shape[0], np.float64)
     with pytest.raises(A.AfemError):
         A.check(A.lib().afem_rhs_body_force(m2.h, f.ctypes.data_as(A.ctypes.c_void_p), rhs3.c))
+
+
+def _box_faces(C, dim):
+    loc = [(0, 1), (1, 2), (2, 3), (3, 0)] if dim == 2 else \
+        [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    cnt = {}
+    for c in C:
+        for l in loc:
+            f = tuple(int(v) for v in c[list(l)])
+            key = tuple(sorted(f))
+            n, _ = cnt.get(key, (0, f))
+            cnt[key] = (n + 1, f)
+    return np.array([f for n, f in cnt.values() if n == 1], np.int32)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("value", ["scalar", "vector"])
+def test_q1_neumann_matches_oracle(afem_gpu, dim, value):
+    """Gauss-point Neumann on the edges / quadrangles of a jittered Q1 box,
+    whole boundary and one side, on top of a source term (1e-13 of the
+    largest entry); a face of no cell fails and leaves the RHS untouched."""
+    A = afem_gpu
+    print(f"[case] q1 neumann dim={dim} {value}", flush=True)
+    X, C = A.box_mesh_q1(dim, 5, 0.2, seed=7)
+    N, npc = X.shape[0], C.shape[1]
+    m = A.Mesh(dim, X, C, npc=npc)
+    F = _box_faces(C, dim)
+    g = 1.75 if value == "scalar" else [0.35, -1.65, 3.75][:dim]
+    for faces in (F, F[np.all(X[F][:, :, 0] == 0.0, axis=1)]):
+        rhs = A.DeviceArray(N, np.float64)
+        A.check(A.lib().afem_rhs_constant_source(m.h, 5.5, None, 1, 0, rhs.c))
+        A.rhs_neumann(m, faces, g, rhs)
+        ref = O.rhs_constant_source(dim, X, C, npc, 5.5)
+        ref = O.rhs_neumann_q1(dim, X, C, npc, faces, g, rhs=ref)
+        assert np.abs(rhs.numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    before = rhs.numpy().copy()
+    bad = np.array([[0, N - 1] + ([1, 2] if dim == 3 else [])], np.int32)
+    with pytest.raises(A.AfemError):
+        A.rhs_neumann(m, bad, g, rhs)
+    assert np.array_equal(rhs.numpy(), before)
+
+
+Q1_NEUMANN = [
+    ("circle_2D_quad", "neumann", 5.5, "curved", [-0.35, 1.65], "horizontal", 0.5),
+    ("circle_2D_quad", "scalar_neumann", 5.5, "curved", 2.0, "horizontal", 0.5),
+    ("sphere_3D_hexa", "neumann", 5.5, "curved", [0.35, 1.65, 3.75], "horizontal", 0.5),
+    ("sphere_3D_hexa", "scalar_neumann", 5.5, "curved", 3.1, "horizontal", 0.5),
+    ("cube_3D_hexa8", "", 9.8, "right", 13.9, "left", 0.5),
+]
+
+
+@pytest.mark.parametrize("case,tag,f,ngroup,value,dgroup,dvalue", Q1_NEUMANN,
+                         ids=[c[0] + "-" + (c[1] or "golden") for c in Q1_NEUMANN])
+def test_q1_golden_neumann_gpu(afem_gpu, case, tag, f, ngroup, value, dgroup, dvalue):
+    """The reference's Q1 Neumann golden files through the GPU path (faces in
+    Arcane's order, Dirichlet by row elimination, FP64 PCG), eps 1e-4."""
+    A = afem_gpu
+    print(f"[case] q1 golden neumann {case} {tag}", flush=True)
+    d, dim, X, C, npc = q1_mesh((case, None, None, None))
+    N = X.shape[0]
+    m = A.Mesh(dim, X, C, npc=npc)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    s = A.DoFLinearSystem(N)
+    b.assemble_laplace(None, f, s.rhs())
+    A.rhs_neumann(m, O.arcane_face_order(d["faces_" + ngroup], d["node_uid"]), value, s.rhs())
+    s.set_bsr(b)
+    s.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, d["group_" + dgroup], 1, 0, dvalue, 0.0)
+    s.apply_matrix_transformation()
+    s.apply_rhs_transformation()
+    r = s.solve(method=A.SOLVER_PCG_FP64, rtol=1e-12, max_iter=20000)
+    assert r.converged
+    sfx = "_" + tag if tag else ""
+    mp = dict(zip(d["node_uid"].tolist(), s.solution().numpy()))
+    got = np.array([mp[u] for u in d["ref_uid" + sfx].tolist()])
+    assert O.count_differences(d["ref_value" + sfx][:, 0], got, 1e-4) == 0

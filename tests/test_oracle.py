@@ -826,3 +826,93 @@ def test_q1_pattern_and_assembly(dim, n):
     assert abs(b.sum() - 3.0) <= 1e-13
     with pytest.raises(ValueError):
         O.assemble(dim, X, C, npc, row, col, element=O.HEAT, lame_lambda=0.1)
+
+
+def _q1_neumann_solve(d, f, neu_faces, value, dir_group, dir_value, tag=""):
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col)
+    rhs = O.rhs_constant_source(dim, X, C, npc, f)
+    O.rhs_neumann_q1(dim, X, C, npc, neu_faces, value, rhs=rhs)
+    g = d["group_" + dir_group]
+    fi, fv = np.zeros(N, np.uint8), np.zeros(N)
+    fi[g], fv[g] = 1, 1e30
+    rhs[g] = 1e30 * dir_value
+    O.apply_transformations(row, col, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    sfx = "_" + tag if tag else ""
+    m = dict(zip(d["node_uid"].tolist(), x))
+    got = np.array([m[u] for u in d["ref_uid" + sfx].tolist()])
+    return O.count_differences(d["ref_value" + sfx][:, 0], got, 1e-4)
+
+
+Q1_NEUMANN = [
+    # case, extra tag, f, Neumann group, value, Dirichlet group, value
+    ("circle_2D_quad", "neumann", 5.5, "curved", [-0.35, 1.65], "horizontal", 0.5),
+    ("circle_2D_quad", "scalar_neumann", 5.5, "curved", 2.0, "horizontal", 0.5),
+    ("sphere_3D_hexa", "neumann", 5.5, "curved", [0.35, 1.65, 3.75], "horizontal", 0.5),
+    ("sphere_3D_hexa", "scalar_neumann", 5.5, "curved", 3.1, "horizontal", 0.5),
+    ("cube_3D_hexa8", "", 9.8, "right", 13.9, "left", 0.5),
+]
+
+
+@pytest.mark.parametrize("case,tag,f,ngroup,value,dgroup,dvalue", Q1_NEUMANN,
+                         ids=[c[0] + "-" + (c[1] or "golden") for c in Q1_NEUMANN])
+def test_golden_poisson_q1_neumann(case, tag, f, ngroup, value, dgroup, dvalue):
+    """modules/poisson tests 2D_neumann_quad, 2D_scalar_neumann_quad,
+    3D_neumann_hexa, 3D_scalar_neumann_hexa and cube.3D.hexa.arc on the Hexa8
+    cube: Neumann with Gauss points on edges / quadrangles, faces in Arcane's
+    order (eps 1e-4).  The vector flux on Hexa8 faces misses the file when the
+    normals are all turned outward: the reference's normal follows Arcane's
+    face order."""
+    d = load_case(case)
+    faces = O.arcane_face_order(d["faces_" + ngroup], d["node_uid"])
+    assert _q1_neumann_solve(d, f, faces, value, dgroup, dvalue, tag) == 0
+    if case == "sphere_3D_hexa" and tag == "neumann":
+        X, C = d["coords"], d["cells"]
+        N = X.shape[0]
+        row, col = O.pattern(N, C, 8)
+        b_arc = O.rhs_neumann_q1(3, X, C, 8, faces, value)
+        b_out = O.rhs_neumann_q1(3, X, C, 8, faces, value, outward=True)
+        assert np.abs(b_arc - b_out).max() > 1e-3
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q1_neumann_known_answers(dim):
+    """On a jittered Q1 box: a scalar flux g adds g |boundary| in total (|F| of
+    each face integrated exactly), and with outward normals a constant flux
+    vector adds 0 (divergence theorem) while e_x adds +1 on x = 1 and -1 on
+    x = 0; each face adds g int N_j = g |F| / k on a flat parallelogram face."""
+    from arcanefem_amd import box_mesh_q1
+    X, C = box_mesh_q1(dim, 4, 0.2, seed=8)
+    npc = C.shape[1]
+    # boundary faces of the box
+    if dim == 2:
+        loc = [(0, 1), (1, 2), (2, 3), (3, 0)]
+    else:
+        loc = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    cnt = {}
+    for c in C:
+        for l in loc:
+            f = tuple(c[list(l)])
+            key = tuple(sorted(f))
+            cnt[key] = (cnt.get(key, (0, f))[0] + 1, f)
+    F = np.array([f for n, f in cnt.values() if n == 1], np.int32)
+    b = O.rhs_neumann_q1(dim, X, C, npc, F, 1.5, outward=True)
+    assert abs(b.sum() - 1.5 * 2 * dim) <= 1e-12
+    g = np.array([0.3, -0.7, 1.1])[:dim]
+    assert abs(O.rhs_neumann_q1(dim, X, C, npc, F, g, outward=True).sum()) <= 1e-12
+    ex = np.zeros(dim)
+    ex[0] = 1.0
+    bx = O.rhs_neumann_q1(dim, X, C, npc, F, ex, outward=True)
+    right = np.all(X[F][:, :, 0] == 1.0, axis=1)
+    left = np.all(X[F][:, :, 0] == 0.0, axis=1)
+    assert abs(O.rhs_neumann_q1(dim, X, C, npc, F[right], ex, outward=True).sum() - 1.0) <= 1e-12
+    assert abs(O.rhs_neumann_q1(dim, X, C, npc, F[left], ex, outward=True).sum() + 1.0) <= 1e-12
+    assert abs(bx.sum()) <= 1e-12
+    # the unit square's / cube's own face: g |F| / k at each node
+    Xu, Cu = box_mesh_q1(dim, 1)
+    Fu = np.array([Cu[0][list(loc[0])]], np.int32)
+    bu = O.rhs_neumann_q1(dim, Xu, Cu, npc, Fu, 2.0)
+    assert np.allclose(bu[Fu[0]], 2.0 / Fu.shape[1], rtol=0, atol=1e-15)
