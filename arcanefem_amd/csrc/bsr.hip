@@ -1879,6 +1879,7 @@ static void build_plan(afem_bsr* b)
   b->plan.release();
   b->cplan.release();
   b->diag_pos.release();
+  b->q1_plan.release(); // rebuilt by the first Q1 assembly
   if (b->max_row > kPlanMaxRow || !m.simplex) // the plans serve the simplex kernels
     return;
   b->plan.alloc(m.nc_items.n);

@@ -244,6 +244,10 @@ struct afem_bsr
   afem::DBuf<int64_t> eslice;        // first cell word of each workgroup
   afem::DBuf<int32_t> ecell;         // cell j of block t of workgroup w at eslice[w] + 256 j + t
   afem::DBuf<uint8_t> ecnt;          // cells per block
+  // Quad4 / Hexa8: per cell the upper triangle of the element matrix and the
+  // source vector, cell-minor (entry k of cell c at k * nb_cell + c)
+  afem::DBuf<double> q1_elem;
+  afem::DBuf<uint64_t> q1_plan;      // per (node, cell) pair: the row positions of the cell's nodes, 8 bits each
   int32_t e_nwg = 0;
   afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
   afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes

@@ -7,12 +7,16 @@
 // femutils/ArcaneFemFunctionsGpu.h:296-350, :553-587 (gradients and det J);
 // femutils/ArcaneFemFunctionsGpu.cc:393-470, :875-960 (source term).  The
 // reference scatters each element matrix with atomics (assembleBilinearAtomic)
-// or recomputes it per node with a column search (assembleBilinearAtomicFree);
-// here one thread owns one matrix row: it walks the node's cells, builds the
-// element row of the node's slot in registers and adds it into its own row,
-// so no two threads ever write the same value.
+// or recomputes it per node with a column search (assembleBilinearAtomicFree).
+// Here one thread owns one matrix row, so no two threads ever write the same
+// value: by default the element matrices are built once per cell into a
+// workspace and each row thread gathers the rows of its node's cells; the
+// variant 0 form rebuilds the element row of the node's slot per (node, cell)
+// pair in registers and needs no workspace.
 #include "common.h"
 #include "device_utils.h"
+
+#include <type_traits>
 
 namespace afem {
 
@@ -201,6 +205,144 @@ k_assemble_q1(const double* __restrict__ coords, const int32_t* __restrict__ cel
     rhs[(int64_t)i * nb_dof + dof_index] = src;
 }
 
+// Element-once form.  Plan (once per pattern): per (node, cell) pair the
+// row positions of the cell's nodes.  Pass 1, one thread per cell: the element matrix
+// sum_g (grad N_a . grad N_b) det J_g and the source vector sum_g N_a det J_g,
+// stored per cell as N rows of N values then the N source values (row s of
+// cell c is one contiguous N * 8-byte read).  Pass 2, one thread per row,
+// 256 / 128 / 64 rows per workgroup: the workgroup's rows are accumulated in
+// LDS in CSR order (each thread owns its row's slots, so no atomics), for each
+// cell of the node (slot s) lambda_c times the cell's row s, then written to
+// HBM once, coalesced; the RHS receives f times entry s of the source vector.
+// Each element matrix is built once instead of once per node (4x / 8x fewer
+// Gauss-point evaluations).
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cells, int32_t nb_cell,
+              double* __restrict__ elem)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NG = DIM == 2 ? 4 : 8;
+  constexpr int NT = N * (N + 1) / 2;
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb_cell)
+    return;
+  constexpr int stride = N > 4 ? N : 4;
+  double x[N][3];
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    const int32_t n = cells[(int64_t)stride * c + a];
+    const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)n);
+    x[a][0] = xy.x;
+    x[a][1] = xy.y;
+    x[a][2] = coords[4 * (int64_t)n + 2];
+  }
+  double K[NT], src[N]; // upper triangle, row-major
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    K[t] = 0.0;
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+    src[a] = 0.0;
+  for (int g = 0; g < NG; ++g) {
+    Q1Point<DIM> q;
+    q.eval(x, g);
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+#pragma unroll
+      for (int b = a; b < N; ++b, ++t)
+        K[t] += (q.gx[a] * q.gx[b]) * q.det + (q.gy[a] * q.gy[b]) * q.det + (q.gz[a] * q.gz[b]) * q.det;
+      src[a] += q.Nv[a] * q.det;
+    }
+  }
+  double* out = elem + (int64_t)(N * N + N) * c;
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      const int lo = b < a ? b : a, hi = b < a ? a : b;
+      out[a * N + b] = K[lo * N - lo * (lo - 1) / 2 + (hi - lo)];
+    }
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+    out[N * N + a] = src[a];
+}
+
+// Per (node, cell) pair e (node->cell order): the positions in the node's
+// row of the cell's N nodes, 8 bits each (rows of at most 255 entries).
+// *err = 1 when a column is missing from the pattern.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_q1_plan(const int32_t* __restrict__ cells, int slot_bits, const int32_t* __restrict__ nc_off,
+          const int32_t* __restrict__ nc_items, int32_t nb_node, const int32_t* __restrict__ row_index,
+          const int32_t* __restrict__ columns, uint64_t* __restrict__ plan, int32_t* __restrict__ err)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t a0 = row_index[i], len = row_index[i + 1] - a0;
+  const int32_t* rc = columns + a0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int32_t* cn = cells + (it - (it & ((1 << slot_bits) - 1)));
+    uint64_t w = 0;
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      const int pos = q1_find_col(rc, len, cn[b]);
+      if (!(pos < len && rc[pos] == cn[b]))
+        atomicOr(err, 1);
+      w |= (uint64_t)(pos & 0xff) << (8 * b);
+    }
+    plan[e] = w;
+  }
+}
+
+template <int DIM, int BS>
+__global__ void __launch_bounds__(BS)
+k_q1_gather(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+            const uint64_t* __restrict__ plan, int32_t nb_node, const int32_t* __restrict__ row_index,
+            const double* __restrict__ elem, const double* __restrict__ cell_lambda, double* __restrict__ values,
+            double f, double* __restrict__ rhs)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NE = N * N + N;
+  extern __shared__ double s_val[];
+  const int t = threadIdx.x;
+  const int32_t r0 = blockIdx.x * BS, i = r0 + t;
+  const int32_t rend = min(r0 + BS, nb_node);
+  const int32_t base = row_index[r0], total = row_index[rend] - base;
+  for (int j = t; j < total; j += BS)
+    s_val[j] = 0.0;
+  __syncthreads();
+  if (i < nb_node) {
+    double* row = s_val + (row_index[i] - base);
+    double src = 0.0;
+    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+      const int32_t it = nc_items[e];
+      const int s = it & ((1 << slot_bits) - 1);
+      const int32_t c = it >> slot_bits;
+      const uint64_t w = plan[e];
+      const double* ke = elem + (int64_t)NE * c;
+      const double lam = cell_lambda ? cell_lambda[c] : 1.0;
+#pragma unroll
+      for (int b = 0; b < N; b += 2) {
+        const double2 p = *reinterpret_cast<const double2*>(ke + s * N + b);
+        row[(w >> (8 * b)) & 0xff] += p.x * lam;
+        row[(w >> (8 * b + 8)) & 0xff] += p.y * lam;
+      }
+      if (rhs)
+        src += f * ke[N * N + s];
+    }
+    if (rhs)
+      rhs[i] = src;
+  }
+  __syncthreads();
+  for (int j = t; j < total; j += BS)
+    values[base + j] = s_val[j];
+}
+
 template <bool MAT>
 void launch_q1(const afem_mesh& m, const int32_t* row_index, const int32_t* columns, const double* cell_lambda,
                double* values, double f, const double* cell_scale, int nb_dof, int dof_index, double* rhs,
@@ -226,7 +368,61 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
   DBuf<int32_t> err(1);
   err.zero();
   ProfScope prof(m.dim == 2 ? "assemble_laplace_quad4" : "assemble_laplace_hexa8");
-  launch_q1<true>(m, b->row_index.p, b->columns.p, cell_lambda, b->values.p, f, nullptr, 1, 0, rhs, err.p);
+  if (b->variant == 0) { // element rows rebuilt per (node, cell) pair, no workspace
+    launch_q1<true>(m, b->row_index.p, b->columns.p, cell_lambda, b->values.p, f, nullptr, 1, 0, rhs, err.p);
+  }
+  else {
+    const int n = m.npc;
+    const size_t need = (size_t)(n * n + n) * m.nb_cell;
+    if (b->q1_elem.n != need)
+      b->q1_elem.alloc(need);
+    const dim3 cgrid(grid_for(m.nb_cell, 256)), block(256);
+    if (m.dim == 2)
+      hipLaunchKernelGGL(k_q1_elements<2>, cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         b->q1_elem.p);
+    else
+      hipLaunchKernelGGL(k_q1_elements<3>, cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         b->q1_elem.p);
+    check_launch("q1_elements");
+    if (b->q1_plan.n != m.nc_items.n) { // once per pattern
+      AFEM_REQUIRE(b->max_row <= 255, AFEM_ERR_CAPACITY, "Q1 matrix rows longer than 255 entries");
+      b->q1_plan.alloc(m.nc_items.n);
+      const dim3 pgrid(grid_for(m.nb_node, 256));
+      if (m.dim == 2)
+        hipLaunchKernelGGL(k_q1_plan<2>, pgrid, block, 0, stream(), m.cells.p, m.slot_bits, m.nc_offsets.p,
+                           m.nc_items.p, m.nb_node, b->row_index.p, b->columns.p, b->q1_plan.p, err.p);
+      else
+        hipLaunchKernelGGL(k_q1_plan<3>, pgrid, block, 0, stream(), m.cells.p, m.slot_bits, m.nc_offsets.p,
+                           m.nc_items.p, m.nb_node, b->row_index.p, b->columns.p, b->q1_plan.p, err.p);
+      check_launch("q1_plan");
+      int32_t h = 0;
+      err.download(&h, 1);
+      if (h != 0)
+        b->q1_plan.release();
+      AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
+    }
+    // rows per workgroup: the workgroup's values (at most rows * max_row) fit in 64 KB of LDS
+    const int rows = b->max_row <= 32 ? 256 : b->max_row <= 64 ? 128 : 64;
+    const size_t lds = (size_t)rows * b->max_row * sizeof(double);
+    AFEM_REQUIRE(lds <= 64 * 1024, AFEM_ERR_CAPACITY, "Q1 matrix rows too long for the LDS-staged gather");
+    const dim3 ngrid(grid_for(m.nb_node, rows));
+    auto gather = [&](auto dim_tag, auto bs_tag) {
+      constexpr int D = decltype(dim_tag)::value;
+      constexpr int BS = decltype(bs_tag)::value;
+      hipLaunchKernelGGL((k_q1_gather<D, BS>), ngrid, dim3(BS), lds, stream(), m.slot_bits, m.nc_offsets.p,
+                         m.nc_items.p, b->q1_plan.p, m.nb_node, b->row_index.p, b->q1_elem.p, cell_lambda,
+                         b->values.p, f, rhs);
+    };
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    if (rows == 256)
+      m.dim == 2 ? gather(I2{}, std::integral_constant<int, 256>{}) : gather(I3{}, std::integral_constant<int, 256>{});
+    else if (rows == 128)
+      m.dim == 2 ? gather(I2{}, std::integral_constant<int, 128>{}) : gather(I3{}, std::integral_constant<int, 128>{});
+    else
+      m.dim == 2 ? gather(I2{}, std::integral_constant<int, 64>{}) : gather(I3{}, std::integral_constant<int, 64>{});
+    check_launch("q1_gather");
+  }
   int32_t h = 0;
   err.download(&h, 1);
   AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");

@@ -1,6 +1,7 @@
 """GPU parity of the Quad4 / Hexa8 (Q1) path through the C ABI against the
 oracle: node->cell lists and sparsity pattern bit-exact; Laplace values and
-constant-source RHS within 1e-12 of the row's / vector's largest entry; the
+constant-source RHS within 1e-12 of the row's / vector's largest entry (both
+assembly forms); Neumann on edges and quadrangles; the
 reference's Q1 golden files (circle_2D_quad, sphere_3D_hexa: f = 5.5,
 u = 0.5 on 'horizontal', eps 1e-4) through a GPU solve; the element kernels
 that have no Q1 form refuse Q1 meshes."""
@@ -60,13 +61,18 @@ def test_q1_pattern_values_rhs_match_oracle(afem_gpu, spec):
     row, col, _, _ = b.download()
     assert np.array_equal(row, orow) and np.array_equal(col, ocol)
     lam = np.random.default_rng(5).uniform(0.2, 5.0, C.shape[0])
-    rhs = A.DeviceArray(N, np.float64)
-    b.assemble_laplace(A.DeviceArray.from_numpy(lam), 2.5, rhs)
-    _, _, val, _ = b.download()
     oval = O.assemble(dim, X, C, npc, orow, ocol, cell_lambda=lam)
-    assert row_scaled_error(orow, oval, val) <= ROW_TOL
     orhs = O.rhs_constant_source(dim, X, C, npc, 2.5)
-    assert np.abs(rhs.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max()
+    dlam = A.DeviceArray.from_numpy(lam)
+    # default: element matrices built once per cell, then gathered per row;
+    # variant 0: element rows rebuilt per (node, cell) pair
+    for variant in (-1, 0, -1):
+        b.set_assembly_variant(variant)
+        rhs = A.DeviceArray(N, np.float64)
+        b.assemble_laplace(dlam, 2.5, rhs)
+        _, _, val, _ = b.download()
+        assert row_scaled_error(orow, oval, val) <= ROW_TOL, variant
+        assert np.abs(rhs.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max(), variant
     # the standalone source kernel, with a per-cell scale, into component 1 of 2
     scale = np.random.default_rng(6).uniform(0.5, 1.5, C.shape[0])
     rhs2 = A.DeviceArray(2 * N, np.float64)
