@@ -60,7 +60,11 @@ def lib():
         L.afo_rhs_constant_source.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl,
                                               ctypes.c_int, ctypes.c_int, vp]
         L.afo_rhs_body_force.argtypes = [vp, i32, vp, vp, vp]
-        L.afo_rhs_mass_field.argtypes = [vp, i32, i32, vp, vp, dbl, ctypes.c_int, ctypes.c_int, vp]
+        L.afo_rhs_mass_field.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl, ctypes.c_int, ctypes.c_int, vp]
+        L.afo_face_measure.restype = dbl
+        L.afo_face_measure.argtypes = [ctypes.c_int, vp, vp]
+        L.afo_rhs_face_uniform.argtypes = [ctypes.c_int, vp, i32, vp, dbl, ctypes.c_int, ctypes.c_int, vp]
+        L.afo_elem_heat_q1.argtypes = [ctypes.c_int, vp, dbl, dbl, vp]
         L.afo_rhs_neumann.restype = ctypes.c_int
         L.afo_rhs_neumann.argtypes = [ctypes.c_int, vp, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
                                       ctypes.c_int, vp]
@@ -229,7 +233,8 @@ def convection_lhs(dim, coords, faces, h, row, cols, values):
     """Adds the heat module's boundary convection matrix h * M_F to values (in place)."""
     coords, faces, row, cols = _f64(coords), _i32(faces), _i32(row), _i32(cols)
     assert values.dtype == np.float64 and values.flags.c_contiguous
-    rc = lib().afo_convection_lhs(dim, _P(coords), faces.shape[0], _P(faces), h, _P(row), _P(cols), _P(values))
+    rc = lib().afo_convection_lhs(faces.shape[1], _P(coords), faces.shape[0], _P(faces), h, _P(row), _P(cols),
+                                  _P(values))
     if rc != 0:
         raise ValueError("convection: a face pair is not in the pattern")
     return values
@@ -268,6 +273,23 @@ def rhs_neumann_q1(dim, coords, cells, npc, faces, value, nb_dof=1, dof_index=0,
     return rhs
 
 
+def rhs_face_uniform(coords, faces, g, nb_dof=1, dof_index=0, rhs=None):
+    """rhs += g |F| / k at the k nodes of every face (the convection RHS)."""
+    coords, faces = _f64(coords), _i32(faces)
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0] * nb_dof)
+    lib().afo_rhs_face_uniform(faces.shape[1], _P(coords), faces.shape[0], _P(faces), g, nb_dof, dof_index, _P(rhs))
+    return rhs
+
+
+def elem_heat_q1(dim, X, lam, dt):
+    X = _f64(X)
+    n = 4 if dim == 2 else 8
+    K = np.zeros(n * n)
+    lib().afo_elem_heat_q1(dim, _P(X), lam, dt, _P(K))
+    return K.reshape(n, n)
+
+
 def max_threads():
     return int(lib().afo_max_threads())
 
@@ -284,11 +306,15 @@ def assemble_laplace_threads(coords, cells, npc, row, cols, f, nthreads):
     return vals, rhs
 
 
-def rhs_mass_field(coords, cells, npc, field, scale, nb_dof=1, dof_index=0, rhs=None):
+def rhs_mass_field(coords, cells, npc, field, scale, nb_dof=1, dof_index=0, rhs=None, dim=None):
+    """rhs += M (scale * field); dim defaults to 2 for 3-node and 3 for 4- and
+    8-node cells (pass dim = 2 for Quad4)."""
+    if dim is None:
+        dim = 2 if npc == 3 else 3
     coords, cells, field = _f64(coords), _i32(cells), _f64(field)
     if rhs is None:
         rhs = np.zeros(coords.shape[0] * nb_dof)
-    lib().afo_rhs_mass_field(_P(coords), cells.shape[0], npc, _P(cells), _P(field), scale, nb_dof, dof_index,
+    lib().afo_rhs_mass_field(dim, _P(coords), cells.shape[0], npc, _P(cells), _P(field), scale, nb_dof, dof_index,
                              _P(rhs))
     return rhs
 

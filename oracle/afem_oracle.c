@@ -281,6 +281,27 @@ void afo_source_q1(int dim, const double* X, double qdot, double* b)
       }
 }
 
+/* modules/heat/ElementMatrixHexQuad.h:33-70 (Quad4), :91-130 (Hexa8): at each
+ * Gauss point (dxU^dxU) iw lambda + (dyU^dyU) iw lambda [+ dz] + (N^N) iw / dt
+ * with iw = det J (the Q1 consistent mass matrix, no doubled diagonal). */
+void afo_elem_heat_q1(int dim, const double* X, double lambda, double dt, double* K)
+{
+  const int n = dim == 2 ? 4 : 8, ng = dim == 2 ? 1 : 2;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int k = 0; k < n * n; ++k)
+    K[k] = 0.0;
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int iz = 0; iz < ng; ++iz) {
+        double N[8], gx[8], gy[8], gz[8];
+        const double iw = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+        for (int a = 0; a < n; ++a)
+          for (int b = 0; b < n; ++b)
+            K[a * n + b] += (gx[a] * gx[b]) * iw * lambda + (gy[a] * gy[b]) * iw * lambda +
+                            (gz[a] * gz[b]) * iw * lambda + (N[a] * N[b]) * iw / dt;
+      }
+}
+
 /* modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
 {
@@ -442,7 +463,7 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
   const int nb2 = nb * nb;
   const int64_t nnz = row_index[nb_node];
   const int q1 = AFO_IS_Q1(dim, npc);
-  if (q1 && element != AFO_LAPLACE)
+  if (q1 && element == AFO_ELASTICITY)
     return -2;
   memset(values, 0, sizeof(double) * (size_t)(nnz * nb2));
   double X[24];
@@ -453,7 +474,9 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
     for (int k = 0; k < npc; ++k)
       memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
     double lambda = cell_lambda ? cell_lambda[c] : 1.0;
-    if (q1)
+    if (q1 && element == AFO_HEAT) /* lame_lambda carries dt */
+      afo_elem_heat_q1(dim, X, lambda, lame_lambda, K);
+    else if (q1)
       afo_elem_laplace_q1(dim, X, lambda, K);
     else if (element == AFO_LAPLACE) {
       if (npc == 3)
@@ -614,9 +637,37 @@ void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int
  * multiplied by scale first (the heat module does m_node_temperature_old.mult(1 / dt),
  * modules/heat/FemModule.cc:343): per cell, node i receives sum_j c_ij f_j with
  * c = area/6 | area/12 (tri) or volume/10 | volume/20 (tet) on | off the diagonal. */
-void afo_rhs_mass_field(const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+void afo_rhs_mass_field(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
                         const double* field, double scale, int nb_dof, int dof_index, double* rhs)
 {
+  if (AFO_IS_Q1(dim, npc)) {
+    /* integrateNodalFieldToRhsQuad4 / Hexa8 (femutils/ArcaneFemFunctions.h:2609-2660,
+     * :1680-1740): f_gp = sum_a N_a f_a, node i += f_gp N_i det J at each Gauss point */
+    const int ng = dim == 2 ? 1 : 2;
+    const double gp[2] = { -AFO_GP, AFO_GP };
+    for (int64_t c = 0; c < nb_cell; ++c) {
+      const int32_t* cn = cells + c * AFO_CELL_STRIDE(npc);
+      double X[24], fv[8], acc[8] = { 0 };
+      for (int k = 0; k < npc; ++k) {
+        memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+        fv[k] = field[cn[k]] * scale;
+      }
+      for (int ix = 0; ix < 2; ++ix)
+        for (int iy = 0; iy < 2; ++iy)
+          for (int iz = 0; iz < ng; ++iz) {
+            double N[8], gx[8], gy[8], gz[8];
+            const double iw = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+            double fg = 0.0;
+            for (int a = 0; a < npc; ++a)
+              fg += N[a] * fv[a];
+            for (int i = 0; i < npc; ++i)
+              acc[i] += fg * N[i] * iw;
+          }
+      for (int i = 0; i < npc; ++i)
+        rhs[(int64_t)cn[i] * nb_dof + dof_index] += acc[i];
+    }
+    return;
+  }
   for (int64_t c = 0; c < nb_cell; ++c) {
     const int32_t* cn = cells + c * 4;
     double m, cd, co;
@@ -746,29 +797,61 @@ int afo_rhs_neumann(int dim, const double* coords, int32_t nb_cell, const int32_
   return 0;
 }
 
+/* |F| of a face of k nodes: computeLengthEdge2 (femutils/ArcaneFemFunctions.h:
+ * 375-381), computeAreaTria3 (:180-188) or computeAreaQuad4 (:202-212, the two
+ * triangles n2 n1 n0 and n0 n3 n2). */
+double afo_face_measure(int k, const double* coords, const int32_t* fn)
+{
+  const double* v0 = coords + 3 * (int64_t)fn[0];
+  const double* v1 = coords + 3 * (int64_t)fn[1];
+  double e1[3], e2[3], c[3];
+  if (k == 2) {
+    sub3(v1, v0, e1);
+    return sqrt(dot3(e1, e1));
+  }
+  const double* v2 = coords + 3 * (int64_t)fn[2];
+  if (k == 3) {
+    sub3(v1, v0, e1);
+    sub3(v2, v0, e2);
+    cross3(e1, e2, c);
+    return sqrt(dot3(c, c)) / 2.0;
+  }
+  const double* v3 = coords + 3 * (int64_t)fn[3];
+  double d[3];
+  sub3(v2, v1, e1);
+  sub3(v0, v1, e2);
+  cross3(e1, e2, c);
+  sub3(v0, v3, e1);
+  sub3(v2, v3, e2);
+  cross3(e1, e2, d);
+  return 0.5 * (sqrt(dot3(c, c)) + sqrt(dot3(d, d)));
+}
+
+/* rhs[node of F] += g |F| / k for every face F (k nodes): the convection RHS
+ * (h Text) * measure / num_nodes of modules/heat/FemModule.cc:333-337, the
+ * computeFacLengthOrArea rule of femutils/ArcaneFemFunctions.h:415-440. */
+void afo_rhs_face_uniform(int k, const double* coords, int32_t nb_face, const int32_t* faces, double g,
+                          int nb_dof, int dof_index, double* rhs)
+{
+  for (int32_t f = 0; f < nb_face; ++f) {
+    const int32_t* fn = faces + (int64_t)f * k;
+    const double v = g * afo_face_measure(k, coords, fn) / k;
+    for (int a = 0; a < k; ++a)
+      rhs[(int64_t)fn[a] * nb_dof + dof_index] += v;
+  }
+}
+
 /* modules/heat/FemModule.cc:302-331: K_e = h * factor * massMatrix(U, U) * measure
  * with U all ones (FemUtils.h:584-596 doubles the diagonal), measure
  * computeLengthEdge2 (ArcaneFemFunctions.h:375-381) or computeAreaTria3
  * (:180-188), added entry by entry with matrixAddValue. */
-int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int32_t* faces, double h,
+int afo_convection_lhs(int k, const double* coords, int32_t nb_face, const int32_t* faces, double h,
                        const int32_t* row_index, const int32_t* columns, double* values)
 {
-  const int k = dim;
-  const double factor = dim == 2 ? 1 / 6. : 1 / 12.;
+  const double factor = k == 2 ? 1 / 6. : k == 3 ? 1 / 12. : 1 / 20.;
   for (int32_t f = 0; f < nb_face; ++f) {
     const int32_t* fn = faces + (int64_t)f * k;
-    const double* v0 = coords + 3 * (int64_t)fn[0];
-    const double* v1 = coords + 3 * (int64_t)fn[1];
-    double e1[3], measure;
-    sub3(v1, v0, e1);
-    if (dim == 2)
-      measure = sqrt(dot3(e1, e1));
-    else {
-      double e2[3], c[3];
-      sub3(coords + 3 * (int64_t)fn[2], v0, e2);
-      cross3(e1, e2, c);
-      measure = sqrt(dot3(c, c)) / 2.0;
-    }
+    const double measure = afo_face_measure(k, coords, fn);
     for (int a = 0; a < k; ++a)
       for (int b = 0; b < k; ++b) {
         const double m = a == b ? 2.0 : 1.0;

@@ -143,7 +143,7 @@ void afo_rhs_constant_source(int dim, const double* coords, int32_t nb_cell, int
 
 /* Mass-matrix product of a nodal field (times scale) into the RHS:
  * femutils/ArcaneFemFunctions.h:2558-2590 (tri), :1628-1660 (tet). */
-void afo_rhs_mass_field(const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+void afo_rhs_mass_field(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
                         const double* field, double scale, int nb_dof, int dof_index, double* rhs);
 void afo_elem_heat_tria3(const double* X, double lambda, double dt, double K[9]);
 void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16]);
@@ -152,13 +152,21 @@ void afo_elem_heat_tetra4(const double* X, double lambda, double dt, double K[16
 void afo_rhs_body_force(const double* coords, int32_t nb_cell, const int32_t* cells,
                         const double f[3], double* rhs);
 
-/* Convection term of the heat module on boundary faces (dim nodes each):
- * A[a][b] += h * c * m_ab * |F| for every pair of nodes of F, with m the matrix
- * of FemUtils.h:584-596 (1 off the diagonal, 2 on it) and c = 1/6 (edges) or
- * 1/12 (triangles).  modules/heat/FemModule.cc:302-331.  The RHS part of the
- * same condition, (h * Text) |F| / dim per face node, is afo_rhs_neumann with
- * the scalar g = h * Text.  Returns -1 when a pair is not in the pattern. */
-int afo_convection_lhs(int dim, const double* coords, int32_t nb_face, const int32_t* faces, double h,
+/* Convection term of the heat module on boundary faces of k nodes (2 edges,
+ * 3 triangles, 4 quadrangles): A[a][b] += h * c * m_ab * |F| for every pair
+ * of nodes of F, with m the matrix of FemUtils.h:584-596 (1 off the diagonal,
+ * 2 on it) and c = 1/6, 1/12 or 1/20.  modules/heat/FemModule.cc:302-331.
+ * The RHS part of the same condition, (h * Text) |F| / k per face node, is
+ * afo_rhs_face_uniform with g = h * Text.  |F|: afo_face_measure
+ * (computeLengthEdge2, computeAreaTria3, computeAreaQuad4).  Returns -1 when a
+ * pair is not in the pattern.  afo_elem_heat_q1: the Quad4 / Hexa8 heat
+ * element lambda K + M / dt with the Gauss-point mass matrix
+ * (modules/heat/ElementMatrixHexQuad.h:33-130). */
+double afo_face_measure(int k, const double* coords, const int32_t* face_nodes);
+void afo_rhs_face_uniform(int k, const double* coords, int32_t nb_face, const int32_t* faces, double g,
+                          int nb_dof, int dof_index, double* rhs);
+void afo_elem_heat_q1(int dim, const double* X, double lambda, double dt, double* K);
+int afo_convection_lhs(int k, const double* coords, int32_t nb_face, const int32_t* faces, double h,
                        const int32_t* row_index, const int32_t* columns, double* values);
 
 /* Neumann condition on the faces of a Quad4 mesh (edges) or a Hexa8 mesh

@@ -62,6 +62,18 @@ Golden cases (test names and tolerances from the reference's CMake files):
               check/poisson_test_ref_cube_3D_hexa8.txt (f=9.8, Dirichlet 0.5 on 'left',
               scalar Neumann 13.9 on 'right')
 
+  heat_plate_2D_quad  modules/heat/inputs/conduction.quad.arc (plate.quad.msh),
+              check/2d_conduction.quad.txt (lambda 1.75, dt 0.4, tmax 20, Tinit 30,
+              penalty 1e31 Dirichlet 10 on 'left'); extra 'convection':
+              conduction.convection.quad.arc (plus h 5, Text 15 on 'right'); extra
+              'neumann_point': conduction.neumann.point-dirichlet.quad.arc (points
+              topLeft 1.8, botRight 31, Neumann 9.6 on 'left')
+  heat_cube_3D_hexa  modules/heat/inputs/3d_conduction.convection.hexa.arc
+              (truncated_cube.hexa.msh), check/3d_conduction_convection.hexa.txt
+              (dt 0.4, tmax 20, Dirichlet 10 on 'left' with penalty 1e30, h 5, Text 15
+              on 'right'); extra 'neumann': 3d_conduction.neumann.hexa.arc (Dirichlet
+              10.6 on 'top', Neumann 1.2 on 'bottom')
+
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
 """
@@ -93,12 +105,22 @@ CASES = {
     "circle_2D_quad": ("meshes/msh/circle_cut.quad.msh", "modules/poisson/check/poisson_test_ref_circle_2D_quad.txt", 1),
     "sphere_3D_hexa": ("meshes/msh/sphere_cut.hexa.msh", "modules/poisson/check/poisson_test_ref_sphere_3D_hexa.txt", 1),
     "cube_3D_hexa8": ("meshes/msh/3x3x3_cube_hexa8.msh", "modules/poisson/check/poisson_test_ref_cube_3D_hexa8.txt", 1),
+    "heat_plate_2D_quad": ("meshes/msh/plate.quad.msh", "modules/heat/check/2d_conduction.quad.txt", 1),
+    "heat_cube_3D_hexa": ("meshes/msh/truncated_cube.hexa.msh", "modules/heat/check/3d_conduction_convection.hexa.txt",
+                          1),
 }
 
 EXTRA = {
     "circle_2D_quad": {
         "neumann": "modules/poisson/check/poisson_test_ref_circle_neumann_2D_quad.txt",
         "scalar_neumann": "modules/poisson/check/poisson_test_ref_circle_scalar_neumann_2D_quad.txt",
+    },
+    "heat_plate_2D_quad": {
+        "convection": "modules/heat/check/2d_conduction_convection.quad.txt",
+        "neumann_point": "modules/heat/check/2d_conduction_neumann_pointBC.quad.txt",
+    },
+    "heat_cube_3D_hexa": {
+        "neumann": "modules/heat/check/3d_conduction_neumann.hexa.txt",
     },
     "sphere_3D_hexa": {
         "neumann": "modules/poisson/check/poisson_test_ref_sphere_neumann_3D_hexa.txt",

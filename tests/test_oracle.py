@@ -649,6 +649,13 @@ HEAT = [
     ("heat_neumann_point_2D", 0.4, 20.0, [("topLeft", 1.8), ("botRight", 31.0)], 1e31, [], [("left", 9.6)], 52),
     ("heat_convection_point_3D", 0.1, 1.0, [("bottom", 2.6), ("center", 100.6)], 1e30, [("top", 1.9, 30.0)], [],
      12),
+    # Quad4 / Hexa8 meshes (modules/heat/inputs/*.quad.arc, *.hexa.arc)
+    ("heat_plate_2D_quad", 0.4, 20.0, [("left", 10.0)], 1e31, [], [], 52),
+    ("heat_plate_2D_quad:convection", 0.4, 20.0, [("left", 10.0)], 1e31, [("right", 5.0, 15.0)], [], 52),
+    ("heat_plate_2D_quad:neumann_point", 0.4, 20.0, [("topLeft", 1.8), ("botRight", 31.0)], 1e31, [],
+     [("left", 9.6)], 52),
+    ("heat_cube_3D_hexa", 0.4, 20.0, [("left", 10.0)], 1e30, [("right", 5.0, 15.0)], [], 52),
+    ("heat_cube_3D_hexa:neumann", 0.4, 20.0, [("top", 10.6)], 1e30, [], [("bottom", 1.2)], 52),
 ]
 
 
@@ -660,11 +667,14 @@ def heat_oracle_run(d, dt, tmax, dirichlet, penalty, convection, neumann, lam=1.
     first t >= tmax.  Returns (u, number of solves)."""
     X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
     N = X.shape[0]
+    q1 = npc != dim + 1
+    faces = {g: (O.arcane_face_order(d["faces_" + g], d["node_uid"]) if npc == 8 else d["faces_" + g])
+             for g in [c[0] for c in convection] + [c[0] for c in neumann]}
     row, col = O.pattern(N, C, npc)
     val = O.assemble(dim, X, C, npc, row, col, element=O.HEAT, cell_lambda=np.full(C.shape[0], lam),
                      lame_lambda=dt)
     for g, h, _ in convection:
-        O.convection_lhs(dim, X, d["faces_" + g], h, row, col, val)
+        O.convection_lhs(dim, X, faces[g], h, row, col, val)
     fi, fv, gv = np.zeros(N, np.uint8), np.zeros(N), np.zeros(N)
     for g, v in dirichlet:
         fi[d["group_" + g]], fv[d["group_" + g]], gv[d["group_" + g]] = 1, penalty, v
@@ -675,10 +685,13 @@ def heat_oracle_run(d, dt, tmax, dirichlet, penalty, convection, neumann, lam=1.
         last = t >= tmax
         b = np.zeros(N)
         for g, h, text in convection:
-            O.rhs_neumann(dim, X, C, d["faces_" + g], h * text, rhs=b)
-        O.rhs_mass_field(X, C, npc, u, 1.0 / dt, rhs=b)
+            O.rhs_face_uniform(X, faces[g], h * text, rhs=b)
+        O.rhs_mass_field(X, C, npc, u, 1.0 / dt, rhs=b, dim=dim)
         for g, flux in neumann:
-            O.rhs_neumann(dim, X, C, d["faces_" + g], flux, rhs=b)
+            if q1:
+                O.rhs_neumann_q1(dim, X, C, npc, faces[g], flux, rhs=b)
+            else:
+                O.rhs_neumann(dim, X, C, faces[g], flux, rhs=b)
         b[fi == 1] = penalty * gv[fi == 1]
         u = lu.solve(b)
         solves += 1
@@ -693,10 +706,14 @@ def test_golden_heat_conduction(case, dt, tmax, dirichlet, penalty, convection, 
     """modules/heat tests conduction, 3d_conduction, conduction_convection,
     conduction_neumann_pointBC and 3d_conduction_convection_pointBC: the
     reference's check files within their eps 1e-4."""
-    d = load_case(case)
+    name, _, tag = case.partition(":")
+    d = load_case(name)
     u, solves = heat_oracle_run(d, dt, tmax, dirichlet, penalty, convection, neumann)
     assert solves == nsolve
-    assert O.count_differences(d["ref_value"][:, 0], by_uid(d, u)[:, 0], 1e-4) == 0
+    sfx = "_" + tag if tag else ""
+    got = dict(zip(d["node_uid"].tolist(), u))
+    assert O.count_differences(d["ref_value" + sfx][:, 0], np.array([got[x] for x in d["ref_uid" + sfx].tolist()]),
+                               1e-4) == 0
 
 
 def test_heat_golden_needs_each_term():
@@ -825,7 +842,7 @@ def test_q1_pattern_and_assembly(dim, n):
     b = O.rhs_constant_source(dim, X, C, npc, 3.0)
     assert abs(b.sum() - 3.0) <= 1e-13
     with pytest.raises(ValueError):
-        O.assemble(dim, X, C, npc, row, col, element=O.HEAT, lame_lambda=0.1)
+        O.assemble(dim, X, C, npc, row, col, element=O.ELASTICITY, nb_dof=dim)
 
 
 def _q1_neumann_solve(d, f, neu_faces, value, dir_group, dir_value, tag=""):
@@ -916,3 +933,32 @@ def test_q1_neumann_known_answers(dim):
     Fu = np.array([Cu[0][list(loc[0])]], np.int32)
     bu = O.rhs_neumann_q1(dim, Xu, Cu, npc, Fu, 2.0)
     assert np.allclose(bu[Fu[0]], 2.0 / Fu.shape[1], rtol=0, atol=1e-15)
+
+
+def test_q1_heat_pieces_known_answers():
+    """Unit square: (K_heat - lambda K) dt is the bilinear consistent mass
+    [4 2 1 2; ...] / 36; on distorted Q1 cells the mass-weighted RHS
+    integrates constant and linear fields exactly; a quadrangle face's
+    convection block sums to h |F| (1/20 with the doubled diagonal) and its
+    measure is the sum of its two triangles."""
+    from arcanefem_amd import box_mesh_q1
+    Xq = np.array([[0.0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    M = (O.elem_heat_q1(2, Xq, 1.3, 0.25) - O.elem_laplace_q1(2, Xq, 1.3)) * 0.25
+    ref = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) / 36.0
+    assert np.abs(M - ref).max() <= 1e-14
+    for dim, n in ((2, 5), (3, 3)):
+        X, C = box_mesh_q1(dim, n, 0.25, seed=12)
+        npc = C.shape[1]
+        assert abs(O.rhs_mass_field(X, C, npc, np.ones(X.shape[0]), 2.0, dim=dim).sum() - 2.0) <= 1e-13
+        lin = 1.0 + 2.0 * X[:, 0] - X[:, 1]
+        assert abs(O.rhs_mass_field(X, C, npc, lin, 1.0, dim=dim).sum() - 1.5) <= 1e-13
+    Xh, Ch = box_mesh_q1(3, 1)
+    face = np.array([[0, 1, 3, 2]], np.int32)  # z = 0 face of the unit cube, a cycle
+    row, col = O.pattern(8, Ch, 8)
+    val = np.zeros(int(row[-1]))
+    O.convection_lhs(3, Xh * [2.0, 3.0, 1.0], face, 1.5, row, col, val)
+    A = sp.csr_matrix((val, col, row), shape=(8, 8)).toarray()
+    F = A[np.ix_(face[0], face[0])]
+    assert abs(F.sum() - 1.5 * 6.0) <= 1e-13 and np.allclose(np.diag(F), 2 * F[0, 1], rtol=1e-15)
+    b = O.rhs_face_uniform(Xh * [2.0, 3.0, 1.0], face, 4.0)
+    assert np.allclose(b[face[0]], 4.0 * 6.0 / 4, rtol=1e-15)
