@@ -102,6 +102,7 @@ def _declare(L):
         "afem_rhs_mass_field": (ci, [vp, vp, ctypes.c_double, ci, ci, vp]),
         "afem_bsr_assemble_heat": (ci, [vp, vp, ctypes.c_double]),
         "afem_bsr_add_convection": (ci, [vp, ci, vp, ctypes.c_double]),
+        "afem_rhs_face_uniform": (ci, [vp, ci, vp, ctypes.c_double, ci, ci, vp]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
         "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
@@ -508,6 +509,13 @@ def rhs_neumann(mesh, faces, value, rhs, nb_dof=1, dof_index=0):
     vals = np.ascontiguousarray(np.atleast_1d(np.asarray(value, np.float64)))
     check(lib().afem_rhs_neumann(mesh.h, int(faces.shape[0]), _ptr(faces), int(vals.size), _ptr(vals), nb_dof,
                                  dof_index, rhs.c))
+
+
+def rhs_face_uniform(mesh, faces, g, rhs, nb_dof=1, dof_index=0):
+    """rhs += g |F| / k at the k nodes of every face (the convection RHS)."""
+    faces = np.ascontiguousarray(faces, np.int32)
+    check(lib().afem_rhs_face_uniform(mesh.h, int(faces.shape[0]), _ptr(faces), float(g), nb_dof, dof_index,
+                                      rhs.c))
 
 
 def rhs_mass_field(mesh, field, scale, rhs, nb_dof=1, dof_index=0):

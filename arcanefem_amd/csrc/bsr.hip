@@ -1751,25 +1751,48 @@ k_neumann_faces_q1(int dim, int32_t nface, const int32_t* __restrict__ faces, co
     }
 }
 
-// |F| of each face: computeLengthEdge2 (femutils/ArcaneFemFunctions.h:375-381)
-// or computeAreaTria3 (:180-188).
+// |F| of each face of k nodes: computeLengthEdge2 (femutils/ArcaneFemFunctions.h:
+// 375-381), computeAreaTria3 (:180-188) or computeAreaQuad4 (:202-212: the
+// triangles n2 n1 n0 and n0 n3 n2).
 __global__ void __launch_bounds__(256)
-k_face_measure(int dim, int32_t nface, const int32_t* __restrict__ faces, const double* __restrict__ coords,
+k_face_measure(int k, int32_t nface, const int32_t* __restrict__ faces, const double* __restrict__ coords,
                double* __restrict__ measure)
 {
   const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nface)
     return;
-  const int32_t* fn = faces + (int64_t)dim * f;
-  const d3 x0 = load_node(coords, fn[0]);
-  const d3 e1 = sub(load_node(coords, fn[1]), x0);
-  if (dim == 2) {
+  const int32_t* fn = faces + (int64_t)k * f;
+  const d3 x0 = load_node(coords, fn[0]), x1 = load_node(coords, fn[1]);
+  if (k == 2) {
+    const d3 e1 = sub(x1, x0);
     measure[f] = sqrt(e1.x * e1.x + e1.y * e1.y + e1.z * e1.z);
   }
-  else {
-    const d3 c = cross(e1, sub(load_node(coords, fn[2]), x0));
+  else if (k == 3) {
+    const d3 c = cross(sub(x1, x0), sub(load_node(coords, fn[2]), x0));
     measure[f] = sqrt(c.x * c.x + c.y * c.y + c.z * c.z) / 2.0;
   }
+  else {
+    const d3 x2 = load_node(coords, fn[2]), x3 = load_node(coords, fn[3]);
+    const d3 c = cross(sub(x2, x1), sub(x0, x1)), d = cross(sub(x0, x3), sub(x2, x3));
+    measure[f] = 0.5 * (sqrt(c.x * c.x + c.y * c.y + c.z * c.z) + sqrt(d.x * d.x + d.y * d.y + d.z * d.z));
+  }
+}
+
+// rhs[node * nb_dof + dof_index] += sum over the node's faces (in face order)
+// of g |F| / k.
+__global__ void __launch_bounds__(256)
+k_face_uniform_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const int32_t* __restrict__ off,
+                      const int32_t* __restrict__ pos_of, int k, const double* __restrict__ measure, double g,
+                      int nb_dof, int dof_index, double* __restrict__ rhs)
+{
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode)
+    return;
+  double* r = rhs + (int64_t)node_ids[t] * nb_dof + dof_index;
+  double acc = *r;
+  for (int32_t j = off[t]; j < off[t + 1]; ++j)
+    acc += g * measure[pos_of[j] / k] / k;
+  *r = acc;
 }
 
 // values += hf * m_ab * |F| for the pairs (node, b) of every face of the node,
@@ -2580,10 +2603,9 @@ AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t*
   AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_ARGUMENT, "the convection term needs nb_dof = 1");
   AFEM_REQUIRE(bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called first");
   const afem_mesh& m = *bsr->mesh;
-  AFEM_REQUIRE(m.npc == m.dim + 1, AFEM_ERR_NOT_IMPLEMENTED, "convection faces of simplex meshes only");
   if (nb_face == 0)
     return AFEM_OK;
-  const int k = m.dim;
+  const int k = m.npc == 8 ? 4 : m.dim; // nodes per face
   for (size_t j = 0; j < (size_t)nb_face * k; ++j)
     AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < m.nb_node, AFEM_ERR_ARGUMENT, "convection face node out of range");
   FaceIndex fx(face_nodes, nb_face, k);
@@ -2594,7 +2616,7 @@ AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t*
   hipLaunchKernelGGL(k_face_measure, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), k, nb_face, fx.faces.p,
                      m.coords.p, measure.p);
   check_launch("face_measure");
-  const double hf = h * (k == 2 ? 1 / 6. : 1 / 12.);
+  const double hf = h * (k == 2 ? 1 / 6. : k == 3 ? 1 / 12. : 1 / 20.);
   for (int check_only = 1; check_only >= 0; --check_only) { // a failed call leaves the values as they were
     hipLaunchKernelGGL(k_convection_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
                        fx.ids.p, fx.off.p, fx.pos_of.p, fx.faces.p, k, measure.p, hf, bsr->row_index.p,
@@ -2654,13 +2676,41 @@ AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face
   AFEM_GUARD_END
 }
 
+AFEM_API int afem_rhs_face_uniform(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, double g, int nb_dof,
+                                   int dof_index, double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && rhs && (nb_face == 0 || face_nodes), AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_face >= 0, AFEM_ERR_ARGUMENT, "negative face count");
+  AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  if (nb_face == 0)
+    return AFEM_OK;
+  const int k = mesh->npc == 8 ? 4 : mesh->dim; // nodes per face
+  for (size_t j = 0; j < (size_t)nb_face * k; ++j)
+    AFEM_REQUIRE(face_nodes[j] >= 0 && face_nodes[j] < mesh->nb_node, AFEM_ERR_ARGUMENT, "face node out of range");
+  FaceIndex fx(face_nodes, nb_face, k);
+  DBuf<double> measure((size_t)nb_face);
+  ProfScope prof("rhs_face_uniform");
+  hipLaunchKernelGGL(k_face_measure, dim3(grid_for(nb_face, 256)), dim3(256), 0, stream(), k, nb_face, fx.faces.p,
+                     mesh->coords.p, measure.p);
+  check_launch("face_measure");
+  hipLaunchKernelGGL(k_face_uniform_gather, dim3(grid_for(fx.nb_node, 256)), dim3(256), 0, stream(), fx.nb_node,
+                     fx.ids.p, fx.off.p, fx.pos_of.p, k, measure.p, g, nb_dof, dof_index, rhs);
+  check_launch("face_uniform_gather");
+  AFEM_GUARD_END
+}
+
 AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, double dt)
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(bsr, AFEM_ERR_ARGUMENT, "null handle");
   AFEM_REQUIRE(bsr->nb_dof == 1, AFEM_ERR_ARGUMENT, "the heat element needs nb_dof = 1");
-  AFEM_REQUIRE(bsr->mesh->simplex, AFEM_ERR_NOT_IMPLEMENTED, "the heat element is implemented for tri3 / tet4");
   AFEM_REQUIRE(dt > 0.0, AFEM_ERR_ARGUMENT, "the time step must be positive");
+  if (!bsr->mesh->simplex) {
+    AFEM_REQUIRE(bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
+    q1_assemble_heat(bsr, cell_lambda, dt);
+    return AFEM_OK;
+  }
   assemble_laplace(bsr, cell_lambda, 0.0, nullptr);
   const afem_mesh& m = *bsr->mesh;
   ProfScope prof("assemble_heat_mass");
@@ -2676,7 +2726,11 @@ AFEM_API int afem_rhs_mass_field(afem_mesh_t mesh, const double* field, double s
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(mesh && field && rhs, AFEM_ERR_ARGUMENT, "null argument");
-  AFEM_REQUIRE(mesh->simplex, AFEM_ERR_NOT_IMPLEMENTED, "the mass-weighted field RHS is implemented for tri3 / tet4");
+  if (!mesh->simplex) {
+    AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+    q1_rhs_mass_field(*mesh, field, scale, nb_dof, dof_index, rhs);
+    return AFEM_OK;
+  }
   AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
   hipLaunchKernelGGL(k_rhs_mass_field, dim3(grid_for(mesh->nb_node, 256)), dim3(256), 0, stream(), mesh->coords.p,
                      mesh->cells.p, mesh->npc, mesh->nc_offsets.p, mesh->nc_items.p, mesh->nb_node, field, scale,

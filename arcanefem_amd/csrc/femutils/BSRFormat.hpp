@@ -168,15 +168,14 @@ namespace BoundaryConditions {
           "applyConvectionToLhs");
   }
 
-  // modules/heat/FemModule.cc:333-337, RHS part: (h Text) |F| / dim at each
-  // node of each face -- a scalar Neumann flux h * Text.
+  // modules/heat/FemModule.cc:333-337, RHS part: (h Text) |F| / k at each
+  // of the k nodes of each face.
   inline void applyConvectionToRhs(const Mesh& mesh, const std::string& group, Real h, Real t_ext,
                                    VariableDoFReal& rhs)
   {
     const std::vector<Int32>& faces = mesh.faceGroup(group);
-    const Real g = h * t_ext;
-    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), 1, &g, 1, 0,
-                           rhs.devicePointer()),
+    check(afem_rhs_face_uniform(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), h * t_ext, 1,
+                                0, rhs.devicePointer()),
           "applyConvectionToRhs");
   }
 

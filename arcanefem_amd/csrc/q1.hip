@@ -216,10 +216,12 @@ k_assemble_q1(const double* __restrict__ coords, const int32_t* __restrict__ cel
 // HBM once, coalesced; the RHS receives f times entry s of the source vector.
 // Each element matrix is built once instead of once per node (4x / 8x fewer
 // Gauss-point evaluations).
-template <int DIM>
+// HEAT: lambda_c K + M / dt with M_ab = sum_g N_a N_b det J_g
+// (modules/heat/ElementMatrixHexQuad.h:33-130); the gather then applies no lambda.
+template <int DIM, bool HEAT>
 __global__ void __launch_bounds__(256)
 k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cells, int32_t nb_cell,
-              double* __restrict__ elem)
+              const double* __restrict__ cell_lambda, double inv_dt, double* __restrict__ elem)
 {
   constexpr int N = Q1Point<DIM>::N;
   constexpr int NG = DIM == 2 ? 4 : 8;
@@ -244,6 +246,7 @@ k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cel
 #pragma unroll
   for (int a = 0; a < N; ++a)
     src[a] = 0.0;
+  const double lam = HEAT && cell_lambda ? cell_lambda[c] : 1.0;
   for (int g = 0; g < NG; ++g) {
     Q1Point<DIM> q;
     q.eval(x, g);
@@ -251,8 +254,13 @@ k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cel
 #pragma unroll
     for (int a = 0; a < N; ++a) {
 #pragma unroll
-      for (int b = a; b < N; ++b, ++t)
-        K[t] += (q.gx[a] * q.gx[b]) * q.det + (q.gy[a] * q.gy[b]) * q.det + (q.gz[a] * q.gz[b]) * q.det;
+      for (int b = a; b < N; ++b, ++t) {
+        if (HEAT)
+          K[t] += (q.gx[a] * q.gx[b]) * q.det * lam + (q.gy[a] * q.gy[b]) * q.det * lam +
+                  (q.gz[a] * q.gz[b]) * q.det * lam + (q.Nv[a] * q.Nv[b]) * q.det * inv_dt;
+        else
+          K[t] += (q.gx[a] * q.gx[b]) * q.det + (q.gy[a] * q.gy[b]) * q.det + (q.gz[a] * q.gz[b]) * q.det;
+      }
       src[a] += q.Nv[a] * q.det;
     }
   }
@@ -343,6 +351,54 @@ k_q1_gather(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t* __
     values[base + j] = s_val[j];
 }
 
+// rhs[i*nb_dof+dof_index] += (M (scale * field))_i, gathered per node over
+// its cells: sum_g N_s (sum_a N_a scale f_a) det J_g
+// (integrateNodalFieldToRhsQuad4 / Hexa8, femutils/ArcaneFemFunctions.h:
+// 2609-2660, :1680-1740).
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_q1_mass_field(const double* __restrict__ coords, const int32_t* __restrict__ cells, int slot_bits,
+                const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+                const double* __restrict__ field, double scale, int nb_dof, int dof_index, double* __restrict__ rhs)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NG = DIM == 2 ? 4 : 8;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  double acc = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int s = it & ((1 << slot_bits) - 1);
+    const int32_t* cn = cells + (it - s);
+    double x[N][3], fv[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      const int32_t n = cn[a];
+      const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)n);
+      x[a][0] = xy.x;
+      x[a][1] = xy.y;
+      x[a][2] = coords[4 * (int64_t)n + 2];
+      fv[a] = field[n] * scale;
+    }
+    double part = 0.0;
+    for (int g = 0; g < NG; ++g) {
+      Q1Point<DIM> q;
+      q.eval(x, g);
+      double fg = 0.0, ns = 0.0;
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        fg += q.Nv[a] * fv[a];
+        if (a == s)
+          ns = q.Nv[a];
+      }
+      part += fg * ns * q.det;
+    }
+    acc += part;
+  }
+  rhs[(int64_t)i * nb_dof + dof_index] += acc;
+}
+
 template <bool MAT>
 void launch_q1(const afem_mesh& m, const int32_t* row_index, const int32_t* columns, const double* cell_lambda,
                double* values, double f, const double* cell_scale, int nb_dof, int dof_index, double* rhs,
@@ -362,13 +418,15 @@ void launch_q1(const afem_mesh& m, const int32_t* row_index, const int32_t* colu
 
 } // namespace
 
-void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs)
+void q1_assemble(afem_bsr* b, const double* cell_lambda, double f, double* rhs, double inv_dt)
 {
   const afem_mesh& m = *b->mesh;
+  const bool heat = inv_dt > 0.0;
   DBuf<int32_t> err(1);
   err.zero();
-  ProfScope prof(m.dim == 2 ? "assemble_laplace_quad4" : "assemble_laplace_hexa8");
-  if (b->variant == 0) { // element rows rebuilt per (node, cell) pair, no workspace
+  ProfScope prof(heat ? (m.dim == 2 ? "assemble_heat_quad4" : "assemble_heat_hexa8")
+                      : (m.dim == 2 ? "assemble_laplace_quad4" : "assemble_laplace_hexa8"));
+  if (b->variant == 0 && !heat) { // element rows rebuilt per (node, cell) pair, no workspace
     launch_q1<true>(m, b->row_index.p, b->columns.p, cell_lambda, b->values.p, f, nullptr, 1, 0, rhs, err.p);
   }
   else {
@@ -377,12 +435,19 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
     if (b->q1_elem.n != need)
       b->q1_elem.alloc(need);
     const dim3 cgrid(grid_for(m.nb_cell, 256)), block(256);
-    if (m.dim == 2)
-      hipLaunchKernelGGL(k_q1_elements<2>, cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
-                         b->q1_elem.p);
+    if (m.dim == 2 && heat)
+      hipLaunchKernelGGL((k_q1_elements<2, true>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         cell_lambda, inv_dt, b->q1_elem.p);
+    else if (m.dim == 2)
+      hipLaunchKernelGGL((k_q1_elements<2, false>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         nullptr, 0.0, b->q1_elem.p);
+    else if (heat)
+      hipLaunchKernelGGL((k_q1_elements<3, true>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         cell_lambda, inv_dt, b->q1_elem.p);
     else
-      hipLaunchKernelGGL(k_q1_elements<3>, cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
-                         b->q1_elem.p);
+      hipLaunchKernelGGL((k_q1_elements<3, false>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+                         nullptr, 0.0, b->q1_elem.p);
+    const double* gather_lambda = heat ? nullptr : cell_lambda; // the heat element carries lambda already
     check_launch("q1_elements");
     if (b->q1_plan.n != m.nc_items.n) { // once per pattern
       AFEM_REQUIRE(b->max_row <= 255, AFEM_ERR_CAPACITY, "Q1 matrix rows longer than 255 entries");
@@ -410,7 +475,7 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
       constexpr int D = decltype(dim_tag)::value;
       constexpr int BS = decltype(bs_tag)::value;
       hipLaunchKernelGGL((k_q1_gather<D, BS>), ngrid, dim3(BS), lds, stream(), m.slot_bits, m.nc_offsets.p,
-                         m.nc_items.p, b->q1_plan.p, m.nb_node, b->row_index.p, b->q1_elem.p, cell_lambda,
+                         m.nc_items.p, b->q1_plan.p, m.nb_node, b->row_index.p, b->q1_elem.p, gather_lambda,
                          b->values.p, f, rhs);
     };
     using I2 = std::integral_constant<int, 2>;
@@ -426,6 +491,29 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
   int32_t h = 0;
   err.download(&h, 1);
   AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
+}
+
+void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs)
+{
+  q1_assemble(b, cell_lambda, f, rhs, 0.0);
+}
+
+void q1_assemble_heat(afem_bsr* b, const double* cell_lambda, double dt)
+{
+  q1_assemble(b, cell_lambda, 0.0, nullptr, 1.0 / dt);
+}
+
+void q1_rhs_mass_field(const afem_mesh& m, const double* field, double scale, int nb_dof, int dof_index, double* rhs)
+{
+  ProfScope prof("rhs_mass_field_q1");
+  const dim3 grid(grid_for(m.nb_node, 256)), block(256);
+  if (m.dim == 2)
+    hipLaunchKernelGGL(k_q1_mass_field<2>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, field, scale, nb_dof, dof_index, rhs);
+  else
+    hipLaunchKernelGGL(k_q1_mass_field<3>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, field, scale, nb_dof, dof_index, rhs);
+  check_launch("q1_mass_field");
 }
 
 void q1_rhs_constant_source(const afem_mesh& m, double f, const double* cell_scale, int nb_dof, int dof_index,

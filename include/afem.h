@@ -101,6 +101,8 @@ typedef struct afem_mesh* afem_mesh_t;
  * :291-350).  On Quad4 / Hexa8 meshes the sparsity, the Laplace assembly (one
  * kernel: no variants, no autotuning) and the constant-source RHS run; the
  * other element kernels return AFEM_ERR_NOT_IMPLEMENTED or AFEM_ERR_ARGUMENT.
+ * (Heat element, mass-weighted field, convection, Neumann and the
+ * face-uniform RHS also accept Q1 meshes; elasticity and body force do not.)
  * Fails with AFEM_ERR_MESH on out-of-range node ids. */
 AFEM_API int afem_mesh_create(int dim, int32_t nb_node, const double* coords,
                               int32_t nb_cell, int32_t nodes_per_cell, const int32_t* cells,
@@ -161,7 +163,9 @@ AFEM_API int afem_bsr_set_assembly_variant(afem_bsr_t bsr, int variant);
 /* Heat element lambda K + M / dt (modules/heat/ElementMatrix.h:28-39 tri,
  * :80-91 tet; M the consistent mass matrix, femutils/FemUtils.h:341-355):
  * the Laplace assembly (cell_lambda [device] or NULL = 1, current variant)
- * followed by a pass adding M / dt.  Values only; nb_dof = 1, dt > 0. */
+ * followed by a pass adding M / dt.  On Quad4 / Hexa8 meshes the Gauss-point
+ * element of modules/heat/ElementMatrixHexQuad.h:33-130 (M_ab = sum_g N_a N_b
+ * det J), built once per cell.  Values only; nb_dof = 1, dt > 0. */
 AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, double dt);
 
 /* Same with the 12x12 P1 linear-elasticity element
@@ -195,14 +199,21 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
                                       int nb_dof, int dof_index, double* rhs);
 /* Convection term of the heat module on boundary faces (modules/heat/
  * FemModule.cc:302-331, the matrix part, added once before the time loop):
- * for each face F (face_nodes [host]: nb_face * dim node ids) and each pair
- * (a, b) of its nodes, A[a][b] += h c m_ab |F| with m_ab = 2 (a = b) or 1 and
- * c = 1/6 (edges) or 1/12 (triangles) -- h times the face's consistent mass
- * matrix.  Summed per entry in face order (no atomics).  The RHS part,
- * (h Text) |F| / dim at each face node, is afem_rhs_neumann with the scalar
- * h * Text.  nb_dof = 1; AFEM_ERR_NOT_FOUND when a pair is outside the
+ * for each face F (face_nodes [host]: nb_face * k node ids, k = dim, or 4 on
+ * a Hexa8 mesh) and each pair (a, b) of its nodes, A[a][b] += h c m_ab |F|
+ * with m_ab = 2 (a = b) or 1 and c = 1/6 (edges), 1/12 (triangles) or 1/20
+ * (quadrangles).  Summed per entry in face order (no atomics).  The RHS
+ * part, (h Text) |F| / k at each face node, is afem_rhs_face_uniform with
+ * g = h * Text.  nb_dof = 1; AFEM_ERR_NOT_FOUND when a pair is outside the
  * pattern (a face that is not a face of the mesh), values untouched. */
 AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t* face_nodes, double h);
+/* rhs[i*nb_dof+dof_index] += g |F| / k for each node i of each face F
+ * (face_nodes [host]: nb_face * k node ids, k = dim, or 4 on a Hexa8 mesh;
+ * |F| by computeLengthEdge2 / computeAreaTria3 / computeAreaQuad4): the
+ * convection RHS (h Text) measure / num_nodes of modules/heat/FemModule.cc:
+ * 333-337 with g = h * Text.  Summed per node in face order. */
+AFEM_API int afem_rhs_face_uniform(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, double g, int nb_dof,
+                                   int dof_index, double* rhs);
 /* BoundaryConditions::applyNeumannToRhs (femutils/ArcaneFemFunctions.h:
  * 2715-2759 edges, :1793-1840 triangles; GPU femutils/ArcaneFemFunctionsGpu.cc:
  * 679-738, and :741-850 / :1146-1320 for Quad4 / Hexa8 meshes): for each
@@ -219,8 +230,9 @@ AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t*
 AFEM_API int afem_rhs_neumann(afem_mesh_t mesh, int nb_face, const int32_t* face_nodes, int nb_value,
                               const double* values, int nb_dof, int dof_index, double* rhs);
 /* integrateNodalFieldToRhs (femutils/ArcaneFemFunctions.h:2558-2590 tri,
- * :1628-1660 tet) on scale * field, gather form: rhs[i*nb_dof+dof_index] +=
- * (M (scale * field))_i with M the consistent P1 mass matrix.  field [device],
+ * :1628-1660 tet, :2609-2660 quad, :1680-1740 hexa) on scale * field, gather
+ * form: rhs[i*nb_dof+dof_index] += (M (scale * field))_i with M the
+ * consistent mass matrix (Gauss points on Quad4 / Hexa8).  field [device],
  * one value per node (the heat module's M u_old / dt with scale = 1 / dt). */
 AFEM_API int afem_rhs_mass_field(afem_mesh_t mesh, const double* field, double scale, int nb_dof,
                                  int dof_index, double* rhs);

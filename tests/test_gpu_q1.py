@@ -154,11 +154,6 @@ def test_q1_unsupported_kernels_refuse(afem_gpu):
     b.compute_sparsity()
     with pytest.raises(A.AfemError):
         b.autotune_assembly()
-    with pytest.raises(A.AfemError):
-        b.assemble_heat(None, 0.1)
-    rhs = A.DeviceArray(N, np.float64)
-    with pytest.raises(A.AfemError):
-        A.rhs_mass_field(m, rhs, 1.0, rhs)
     b3 = A.BSRFormat(m, 3, use_csr=False)
     b3.compute_sparsity()
     with pytest.raises(A.AfemError):
@@ -245,3 +240,37 @@ def test_q1_golden_neumann_gpu(afem_gpu, case, tag, f, ngroup, value, dgroup, dv
     mp = dict(zip(d["node_uid"].tolist(), s.solution().numpy()))
     got = np.array([mp[u] for u in d["ref_uid" + sfx].tolist()])
     assert O.count_differences(d["ref_value" + sfx][:, 0], got, 1e-4) == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q1_heat_pieces_match_oracle(afem_gpu, dim):
+    """Q1 heat element lambda K + M/dt (per-cell lambda), the convection matrix
+    on one side (edges / quadrangles), the face-uniform convection RHS and the
+    Gauss mass-weighted field RHS against the oracle (1e-12 / 1e-13)."""
+    A = afem_gpu
+    print(f"[case] q1 heat dim={dim}", flush=True)
+    X, C = A.box_mesh_q1(dim, 6 if dim == 2 else 4, 0.2, seed=17)
+    N, npc, dt = X.shape[0], C.shape[1], 0.3
+    m = A.Mesh(dim, X, C, npc=npc)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    orow, ocol = O.pattern(N, C, npc)
+    lam = np.random.default_rng(4).uniform(0.5, 3.0, C.shape[0])
+    b.assemble_heat(A.DeviceArray.from_numpy(lam), dt)
+    _, _, val, _ = b.download()
+    oval = O.assemble(dim, X, C, npc, orow, ocol, element=O.HEAT, cell_lambda=lam, lame_lambda=dt)
+    assert row_scaled_error(orow, oval, val) <= ROW_TOL
+    side = _box_faces(C, dim)
+    side = side[np.all(X[side][:, :, 0] == 1.0, axis=1)]
+    b.add_convection(side, 2.5)
+    O.convection_lhs(dim, X, side, 2.5, orow, ocol, oval)
+    _, _, val, _ = b.download()
+    assert row_scaled_error(orow, oval, val) <= ROW_TOL
+    u = 20.0 + np.cos(3.0 * X[:, 0]) * X[:, 1]
+    rhs = A.DeviceArray(N, np.float64)
+    rhs.zero()
+    A.rhs_face_uniform(m, side, 2.5 * 15.0, rhs)
+    A.rhs_mass_field(m, A.DeviceArray.from_numpy(u), 1.0 / dt, rhs)
+    ref = O.rhs_face_uniform(X, side, 2.5 * 15.0)
+    ref = O.rhs_mass_field(X, C, npc, u, 1.0 / dt, rhs=ref, dim=dim)
+    assert np.abs(rhs.numpy() - ref).max() <= 1e-13 * np.abs(ref).max()

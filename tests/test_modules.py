@@ -186,6 +186,18 @@ HEAT = [
                                                 "--dirichlet-point", "center=100.6"],
      "3d_conduction_convection_pointBC.txt", 12),
 ]
+HEAT += [
+    ("plate.quad.msh", "0.4", "20", "1e31", ["--dirichlet", "left=10.0"], "2d_conduction.quad.txt", 52),
+    ("plate.quad.msh", "0.4", "20", "1e31", ["--dirichlet", "left=10.0", "--convection", "right=5.0,15.0"],
+     "2d_conduction_convection.quad.txt", 52),
+    ("plate.quad.msh", "0.4", "20", "1e31", ["--dirichlet-point", "topLeft=1.8", "--dirichlet-point",
+                                             "botRight=31.0", "--neumann", "left=9.6"],
+     "2d_conduction_neumann_pointBC.quad.txt", 52),
+    ("truncated_cube.hexa.msh", "0.4", "20", "1e30", ["--dirichlet", "left=10.0", "--convection", "right=5.0,15.0"],
+     "3d_conduction_convection.hexa.txt", 52),
+    ("truncated_cube.hexa.msh", "0.4", "20", "1e30", ["--dirichlet", "top=10.6", "--neumann", "bottom=1.2"],
+     "3d_conduction_neumann.hexa.txt", 52),
+]
 HEAT_SOLVES = [("Penalty", "BSR", "fp64"), ("Penalty", "AF-BSR", "fp64"), ("RowElimination", "BSR", "fp64"),
                ("RowElimination", "BSR", "mixed")]
 
