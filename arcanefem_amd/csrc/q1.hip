@@ -426,7 +426,11 @@ void q1_assemble(afem_bsr* b, const double* cell_lambda, double f, double* rhs, 
   err.zero();
   ProfScope prof(heat ? (m.dim == 2 ? "assemble_heat_quad4" : "assemble_heat_hexa8")
                       : (m.dim == 2 ? "assemble_laplace_quad4" : "assemble_laplace_hexa8"));
-  if (b->variant == 0 && !heat) { // element rows rebuilt per (node, cell) pair, no workspace
+  // rows longer than 128 entries (a workgroup's values beyond 64 KB of LDS) or
+  // 255 (8-bit plan positions) take the per-pair form
+  const bool gather_fits = b->max_row <= 128;
+  AFEM_REQUIRE(gather_fits || !heat, AFEM_ERR_CAPACITY, "Q1 heat rows longer than 128 entries");
+  if ((b->variant == 0 || !gather_fits) && !heat) { // element rows rebuilt per (node, cell) pair, no workspace
     launch_q1<true>(m, b->row_index.p, b->columns.p, cell_lambda, b->values.p, f, nullptr, 1, 0, rhs, err.p);
   }
   else {

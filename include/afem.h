@@ -98,8 +98,10 @@ typedef struct afem_mesh* afem_mesh_t;
  * 4, or 8 for hexa8 (triangles pad slot 3 with -1).  Quad4 nodes go
  * counter-clockwise, Hexa8 the bottom face counter-clockwise then the top
  * face (the Gmsh / Arcane local order, femutils/ShapeFunctions.h:104-129,
- * :291-350).  On Quad4 / Hexa8 meshes the sparsity, the Laplace assembly (one
- * kernel: no variants, no autotuning) and the constant-source RHS run; the
+ * :291-350).  On Quad4 / Hexa8 meshes the sparsity, the Laplace assembly
+ * (element matrices once per cell, then a row gather; variant 0, and rows of
+ * more than 128 entries, rebuild each element row per (node, cell) pair; no
+ * autotuning) and the constant-source RHS run; the
  * other element kernels return AFEM_ERR_NOT_IMPLEMENTED or AFEM_ERR_ARGUMENT.
  * (Heat element, mass-weighted field, convection, Neumann and the
  * face-uniform RHS also accept Q1 meshes; elasticity and body force do not.)

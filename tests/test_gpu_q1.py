@@ -274,3 +274,31 @@ def test_q1_heat_pieces_match_oracle(afem_gpu, dim):
     ref = O.rhs_face_uniform(X, side, 2.5 * 15.0)
     ref = O.rhs_mass_field(X, C, npc, u, 1.0 / dt, rhs=ref, dim=dim)
     assert np.abs(rhs.numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_q1_long_rows_fall_back(afem_gpu):
+    """A Quad4 fan whose centre row has 141 entries: the Laplace assembly takes
+    the per-pair form and still matches the oracle; the heat element, which
+    has only the gathered form, refuses it."""
+    A = afem_gpu
+    print("[case] q1 long rows", flush=True)
+    k = 70
+    ang = 2 * np.pi * np.arange(2 * k) / (2 * k)
+    X = np.zeros((2 * k + 1, 3))
+    X[1:, 0], X[1:, 1] = np.cos(ang), np.sin(ang)
+    C = np.array([[0, 1 + 2 * i, 1 + (2 * i + 1) % (2 * k), 1 + (2 * i + 2) % (2 * k)] for i in range(k)], np.int32)
+    N = X.shape[0]
+    m = A.Mesh(2, X, C, npc=4)
+    b = A.BSRFormat(m, 1, use_csr=True)
+    b.compute_sparsity()
+    orow, ocol = O.pattern(N, C, 4)
+    rhs = A.DeviceArray(N, np.float64)
+    b.assemble_laplace(None, 1.0, rhs)
+    row, col, val, _ = b.download()
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol) and orow[1] - orow[0] == 2 * k + 1
+    oval = O.assemble(2, X, C, 4, orow, ocol)
+    assert row_scaled_error(orow, oval, val) <= ROW_TOL
+    orhs = O.rhs_constant_source(2, X, C, 4, 1.0)
+    assert np.abs(rhs.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max()
+    with pytest.raises(A.AfemError):
+        b.assemble_heat(None, 0.1)
