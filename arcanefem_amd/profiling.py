@@ -29,6 +29,7 @@ def short_name(kernel):
     """'void afem::k_spmv_sell<double, double, 0>(int, ...)' -> 'k_spmv_sell<double, double, 0>'."""
     name = kernel.strip()
     name = re.sub(r"^void\s+", "", name)
+    name = name.replace("(anonymous namespace)::", "")
     depth = 0
     for i, ch in enumerate(name):  # cut the argument list at the first top-level '('
         if ch == "<":
