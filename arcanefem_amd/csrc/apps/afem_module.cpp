@@ -1,6 +1,7 @@
-// afem_module -- the Poisson, elasticity and heat module flows of ArcaneFEM
-// (modules/poisson/FemModule.cc:75-146, modules/elasticity/FemModule.cc:57-154,
-// modules/heat/FemModule.cc:74-420) driven through the C++ DoFLinearSystem /
+// afem_module -- the Poisson, Fourier, elasticity and heat module flows of
+// ArcaneFEM (modules/poisson/FemModule.cc:75-146, modules/fourier/FemModule.cc:
+// 160-356, modules/elasticity/FemModule.cc:57-154, modules/heat/FemModule.cc:
+// 74-420) driven through the C++ DoFLinearSystem /
 // BSRFormat layer and the MI355X linear-system factory, on a reference Gmsh
 // mesh or a structured box.
 //
@@ -10,6 +11,9 @@
 //   afem_module elasticity --mesh bar_dynamic_3D.msh --E 21e5 --nu 0.28 --f -1.0 \
 //       --dirichlet surfaceleft=0.0,0.0,0.0 --dirichlet surfaceright=NULL,1.0,NULL \
 //       --check bar.3D.Dirichlet.bodyForce.txt --epsilon 1e-3 --min-value 1e-10
+//   afem_module fourier --mesh multi-material.msh --lambda 0 --material Mat1=100 \
+//       --material Mat2=1 --qdot 15 --dirichlet Left=50 --dirichlet Right=5 \
+//       --check test2_results.txt
 //   afem_module heat --mesh plate.msh --lambda 1.75 --dt 0.4 --tmax 20 --Tinit 30 \
 //       --penalty 1e31 --dirichlet left=10.0 [--convection right=1.0,20.0] \
 //       --check 2d_conduction.txt
@@ -33,6 +37,8 @@ struct Args
   unsigned long long box_seed = 0;
   std::vector<double> f;
   std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann, convection;
+  std::vector<std::pair<std::string, double>> materials; // fourier: per-volume-group lambda
+  bool has_lambda = false;
   double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
   // heat (modules/heat/Fem.axl defaults) and the Dirichlet penalty
   // (femutils/FemBoundaryConditions.axl:104-109)
@@ -116,8 +122,20 @@ Args parse(int argc, char** argv)
         usage("--convection expects group=h,Text");
       a.convection.push_back({ s.substr(0, eq), v });
     }
-    else if (key == "--lambda")
+    else if (key == "--lambda") {
       a.lambda = std::stod(next());
+      a.has_lambda = true;
+    }
+    else if (key == "--qdot") // the Fourier module's name for the source term
+      a.f = { std::stod(next()) };
+    else if (key == "--material") {
+      // group=lambda (modules/fourier/Fem.axl material-property)
+      const std::string s = next();
+      const size_t eq = s.rfind('=');
+      if (eq == std::string::npos || eq == 0 || eq + 1 == s.size())
+        usage("--material expects group=lambda");
+      a.materials.push_back({ s.substr(0, eq), std::stod(s.substr(eq + 1)) });
+    }
     else if (key == "--dt")
       a.dt = std::stod(next());
     else if (key == "--tmax")
@@ -153,6 +171,8 @@ Args parse(int argc, char** argv)
   }
   if (a.mesh.empty() && a.box_dim == 0)
     usage("give --mesh or --box");
+  if (a.physics != "fourier" && !a.materials.empty())
+    usage("--material is available for fourier");
   return a;
 }
 
@@ -299,8 +319,9 @@ int main(int argc, char** argv)
     if (a.physics == "heat")
       return run_heat(a, mesh, runner);
     const bool elasticity = a.physics == "elasticity";
-    if (!elasticity && a.physics != "poisson")
-      usage("physics must be poisson, elasticity or heat");
+    const bool fourier = a.physics == "fourier";
+    if (!elasticity && !fourier && a.physics != "poisson")
+      usage("physics must be poisson, fourier, elasticity or heat");
     if (!a.convection.empty())
       usage("--convection is available for heat");
     const int nb_dof = elasticity ? mesh.dimension() : 1; // modules/elasticity/FemModule.cc:174
@@ -323,10 +344,22 @@ int main(int argc, char** argv)
 
     // _assembleBilinearOperator
     ElementParameters ep;
+    std::unique_ptr<OwnedDoFVariable<Real>> cell_lambda;
     if (elasticity) {
       ep.lame_mu = a.E / (2 * (1 + a.nu)); // modules/elasticity/FemModule.cc:171-172
       ep.lame_lambda = a.E * a.nu / ((1 + a.nu) * (1 - 2 * a.nu));
       bsr.assembleBilinear(Element::Elasticity, ep);
+    }
+    else if (fourier) {
+      // modules/fourier/FemModule.cc:172-194: lambda everywhere, then per material group
+      std::vector<Real> lam((size_t)mesh.nbCell(), a.has_lambda ? a.lambda : 1.0);
+      for (auto& mt : a.materials)
+        for (Int32 c : mesh.cellGroup(mt.first))
+          lam[(size_t)c] = mt.second;
+      cell_lambda = std::make_unique<OwnedDoFVariable<Real>>(mesh.nbCell());
+      cell_lambda->variable().fromHost(lam);
+      ep.cell_lambda = cell_lambda->devicePointer();
+      bsr.assembleBilinear(Element::Laplace, ep);
     }
     else {
       bsr.assembleBilinear(Element::Laplace, ep);
@@ -346,7 +379,7 @@ int main(int argc, char** argv)
       BoundaryConditions::applyConstantSourceToRhs(a.f[0], mesh, 1, 0, rhs);
     }
     if (!a.neumann.empty() && elasticity)
-      usage("--neumann is available for poisson");
+      usage("--neumann is available for poisson, fourier and heat");
     for (auto& nm : a.neumann)
       BoundaryConditions::applyNeumannToRhs(mesh, nm.first, nm.second, rhs);
     apply_dirichlet(a, mesh, nb_dof, ls);

@@ -266,9 +266,19 @@ class Mesh
         npc = b.k;
       }
     const int stride = npc > 4 ? npc : 4;
+    m_cell_groups.clear();
     for (auto& b : blocks) {
       const size_t n = b.conn.size() / (b.k + 1);
       if (b.dim == dim) {
+        const Int32 first = (Int32)(cells.size() / stride);
+        for (int ph : entity_phys[{ b.dim, b.tag }]) { // volume physical names (materials)
+          auto nm = names.find({ b.dim, ph });
+          if (nm == names.end())
+            continue;
+          auto& cg = m_cell_groups[nm->second];
+          for (size_t e = 0; e < n; ++e)
+            cg.push_back(first + (Int32)e);
+        }
         for (size_t e = 0; e < n; ++e)
           for (int s = 0; s < stride; ++s)
             cells.push_back(s < b.k ? lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]) : -1);
@@ -336,6 +346,14 @@ class Mesh
       throw FatalError(AFEM_ERR_ARGUMENT, "no boundary group '" + name + "'");
     return it->second;
   }
+  // cells of a named volume group (a material), local cell ids
+  const std::vector<Int32>& cellGroup(const std::string& name) const
+  {
+    auto it = m_cell_groups.find(name);
+    if (it == m_cell_groups.end())
+      throw FatalError(AFEM_ERR_ARGUMENT, "no cell group '" + name + "'");
+    return it->second;
+  }
   // faces of a named boundary group: nodesPerFace() node ids per face
   const std::vector<Int32>& faceGroup(const std::string& name) const
   {
@@ -363,6 +381,7 @@ class Mesh
   std::vector<Int64> m_uids;
   std::map<std::string, NodeGroup> m_groups;
   std::map<std::string, std::vector<Int32>> m_face_groups;
+  std::map<std::string, std::vector<Int32>> m_cell_groups;
   afem_mesh_t m_handle = nullptr;
 };
 

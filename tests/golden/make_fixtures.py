@@ -13,6 +13,7 @@ golden solution files, and writes one .npz per case with:
   group_<name>(k,)   int32     local ids of the nodes of the boundary faces
                                carrying the physical name <name>
   faces_<name>(f, dim) int32   those faces (local node ids, file order)
+  cellgroup_<name> (c,) int32  cells carrying the volume physical name <name>
   ref_uid     (M,)   int64     uids listed in the golden file
   ref_value   (M, d) float64   golden values (d = 1 or 3)
   ref_uid_<t>, ref_value_<t>   further golden files on the same mesh (EXTRA)
@@ -74,6 +75,20 @@ Golden cases (test names and tolerances from the reference's CMake files):
               on 'right'); extra 'neumann': 3d_conduction.neumann.hexa.arc (Dirichlet
               10.6 on 'top', Neumann 1.2 on 'bottom')
 
+  fourier_*  modules/fourier (-div(lambda grad u) = qdot): inputs/conduction.arc
+              on plancher.msh (extra 'fourier' of point_dirichlet_2D, check/test1_results.txt:
+              lambda 1.75, qdot 1e5, penalty 1e12 Dirichlet Cercle 50, Bas 5, Haut 21,
+              Neumann Droite 15, Gauche 0); conduction.heterogeneous.arc on
+              multi-material.msh (fourier_multimat_2D, test2_results.txt: lambda 100 on
+              Mat1 and 1 on Mat2, qdot 15, Dirichlet Left 50, Right 5); conduction.3D.arc
+              (extra 'fourier' of bar_3D, test_conduction_3D.txt: lambda 0.023, qdot
+              1.123e-2, Dirichlet surfaceleft 55, surfaceright 12); conduction.quad.arc on
+              plancher.quad4.msh (fourier_plancher_quad, conduction_quad.txt);
+              conduction.heterogeneous.quad.arc (fourier_multimat_quad,
+              conduction_multi-mat_quad.txt); conduction.sphere-cut.hexa.arc (extra
+              'fourier' of sphere_3D_hexa, conduction_hexa.txt: lambda 23.5, qdot 1.123e-2,
+              Dirichlet horizontal 55, Neumann curved 1003.67)
+
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
 """
@@ -106,11 +121,17 @@ CASES = {
     "sphere_3D_hexa": ("meshes/msh/sphere_cut.hexa.msh", "modules/poisson/check/poisson_test_ref_sphere_3D_hexa.txt", 1),
     "cube_3D_hexa8": ("meshes/msh/3x3x3_cube_hexa8.msh", "modules/poisson/check/poisson_test_ref_cube_3D_hexa8.txt", 1),
     "heat_plate_2D_quad": ("meshes/msh/plate.quad.msh", "modules/heat/check/2d_conduction.quad.txt", 1),
+    "fourier_multimat_2D": ("meshes/msh/multi-material.msh", "modules/fourier/check/test2_results.txt", 1),
+    "fourier_multimat_quad": ("meshes/msh/multi-material.quad.msh",
+                              "modules/fourier/check/conduction_multi-mat_quad.txt", 1),
+    "fourier_plancher_quad": ("meshes/msh/plancher.quad4.msh", "modules/fourier/check/conduction_quad.txt", 1),
     "heat_cube_3D_hexa": ("meshes/msh/truncated_cube.hexa.msh", "modules/heat/check/3d_conduction_convection.hexa.txt",
                           1),
 }
 
 EXTRA = {
+    "point_dirichlet_2D": {"fourier": "modules/fourier/check/test1_results.txt"},
+    "bar_3D": {"fourier": "modules/fourier/check/test_conduction_3D.txt"},
     "circle_2D_quad": {
         "neumann": "modules/poisson/check/poisson_test_ref_circle_neumann_2D_quad.txt",
         "scalar_neumann": "modules/poisson/check/poisson_test_ref_circle_scalar_neumann_2D_quad.txt",
@@ -123,6 +144,7 @@ EXTRA = {
         "neumann": "modules/heat/check/3d_conduction_neumann.hexa.txt",
     },
     "sphere_3D_hexa": {
+        "fourier": "modules/fourier/check/conduction_hexa.txt",
         "neumann": "modules/poisson/check/poisson_test_ref_sphere_neumann_3D_hexa.txt",
         "scalar_neumann": "modules/poisson/check/poisson_test_ref_sphere_scalar_neumann_3D_hexa.txt",
     },
@@ -245,10 +267,15 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     names, entity_phys, tags, xyz, blocks = read_msh41_binary(mesh_path)
     tag_to_lid = {int(t): i for i, t in enumerate(tags)}
     dim = max(b[0] for b in blocks)
-    cells = []
-    for edim, _etag, _etype, _etags, conn in blocks:
+    cells, cell_groups, ncell = [], {}, 0
+    for edim, etag, _etype, _etags, conn in blocks:
         if edim == dim:
             cells.append(conn)
+            for ptag in entity_phys.get((edim, etag), []):
+                gname = names.get((edim, ptag))
+                if gname is not None:
+                    cell_groups.setdefault(gname, []).append(np.arange(ncell, ncell + len(conn)))
+            ncell += len(conn)
     cells = np.concatenate(cells)
     npc = cells.shape[1]
     lid = np.vectorize(tag_to_lid.__getitem__, otypes=[np.int64])(cells)
@@ -277,6 +304,8 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
         payload["group_" + gname] = np.unique(np.concatenate(lst)).astype(np.int32)
     for gname, lst in faces.items():
         payload["faces_" + gname] = np.concatenate(lst).astype(np.int32)
+    for gname, lst in cell_groups.items():
+        payload["cellgroup_" + gname] = np.concatenate(lst).astype(np.int32)
     ref = np.loadtxt(os.path.join(ref_root, check_rel), ndmin=2)
     payload["ref_uid"] = ref[:, 0].astype(np.int64)
     payload["ref_value"] = ref[:, 1:1 + ncomp].astype(np.float64)

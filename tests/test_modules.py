@@ -171,6 +171,44 @@ def test_poisson_module_q1_golden(msh, args, ref, method, fmt, precision):
     _check_ok(r)
 
 
+# modules/fourier/inputs/conduction*.arc: mesh, options, check file
+FOURIER_MODULE = [
+    ("plancher.msh", ["--lambda", "1.75", "--qdot", "1e5", "--penalty", "1e12", "--dirichlet", "Cercle=50.0",
+                      "--dirichlet", "Bas=5.0", "--dirichlet", "Haut=21.0", "--neumann", "Droite=15.0",
+                      "--neumann", "Gauche=0.0"], "test1_results.txt"),
+    ("multi-material.msh", ["--lambda", "0.0", "--material", "Mat1=100.0", "--material", "Mat2=1.0", "--qdot", "15.",
+                            "--dirichlet", "Left=50.0", "--dirichlet", "Right=5.0"], "test2_results.txt"),
+    ("bar_dynamic_3D.msh", ["--lambda", "0.023", "--qdot", "1.123e-2", "--dirichlet", "surfaceleft=55.0",
+                            "--dirichlet", "surfaceright=12.0", "--neumann", "sidesurfaces=0.0"],
+     "test_conduction_3D.txt"),
+    ("plancher.quad4.msh", ["--lambda", "1.75", "--qdot", "1e5", "--penalty", "1e12", "--dirichlet", "Cercle=50.0",
+                            "--dirichlet", "Bas=5.0", "--dirichlet", "Haut=21.0", "--neumann", "Droite=15.0",
+                            "--neumann", "Gauche=0.0"], "conduction_quad.txt"),
+    ("multi-material.quad.msh", ["--lambda", "0.0", "--material", "Mat1=100.0", "--material", "Mat2=1.0", "--qdot",
+                                 "15.", "--dirichlet", "Left=50.0", "--dirichlet", "Right=5.0"],
+     "conduction_multi-mat_quad.txt"),
+    ("sphere_cut.hexa.msh", ["--lambda", "23.5", "--qdot", "1.123e-2", "--dirichlet", "horizontal=55.0", "--neumann",
+                             "curved=1003.67"], "conduction_hexa.txt"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msh,args,ref", FOURIER_MODULE, ids=[c[2][:-4] for c in FOURIER_MODULE])
+@pytest.mark.parametrize("method,fmt,precision", [("Penalty", "BSR", "fp64"), ("RowElimination", "AF-BSR", "mixed")])
+def test_fourier_module_golden(msh, args, ref, method, fmt, precision):
+    # modules/fourier: per-cell lambda (default and material volume groups), source qdot
+    r = run(["fourier", "--mesh", os.path.join(GOLDEN, msh), "--method", method, "--format", fmt,
+             "--precision", precision, "--check", os.path.join(GOLDEN, ref), "--epsilon", "1e-4"] + args)
+    _check_ok(r)
+
+
+def test_fourier_material_usage_errors():
+    r = run(["poisson", "--box", "2", "4", "0", "0", "--material", "Mat1=1.0"])
+    assert r.returncode == 2 and "--material is available for fourier" in r.stderr
+    r = run(["fourier", "--box", "2", "4", "0", "0", "--material", "Mat1"])
+    assert r.returncode == 2 and "--material expects" in r.stderr
+
+
 # modules/heat/inputs/*.arc (lambda 1.75, Tinit 30): mesh, dt, tmax, penalty,
 # conditions, check file, number of solves
 HEAT = [

@@ -962,3 +962,67 @@ def test_q1_heat_pieces_known_answers():
     assert abs(F.sum() - 1.5 * 6.0) <= 1e-13 and np.allclose(np.diag(F), 2 * F[0, 1], rtol=1e-15)
     b = O.rhs_face_uniform(Xh * [2.0, 3.0, 1.0], face, 4.0)
     assert np.allclose(b[face[0]], 4.0 * 6.0 / 4, rtol=1e-15)
+
+
+# modules/fourier (-div(lambda grad u) = qdot): case[:extra tag], lambda default,
+# material lambdas by cell group, qdot, Dirichlet (group, value), penalty,
+# scalar Neumann (group, value)
+FOURIER = [
+    ("point_dirichlet_2D:fourier", 1.75, {}, 1e5, [("Cercle", 50.0), ("Bas", 5.0), ("Haut", 21.0)], 1e12,
+     [("Droite", 15.0), ("Gauche", 0.0)]),
+    ("fourier_multimat_2D", 0.0, {"Mat1": 100.0, "Mat2": 1.0}, 15.0, [("Left", 50.0), ("Right", 5.0)], 1e30,
+     [("Top", 0.0), ("Bot", 0.0)]),
+    ("bar_3D:fourier", 0.023, {}, 1.123e-2, [("surfaceleft", 55.0), ("surfaceright", 12.0)], 1e30,
+     [("sidesurfaces", 0.0)]),
+    ("fourier_plancher_quad", 1.75, {}, 1e5, [("Cercle", 50.0), ("Bas", 5.0), ("Haut", 21.0)], 1e12,
+     [("Droite", 15.0), ("Gauche", 0.0)]),
+    ("fourier_multimat_quad", 0.0, {"Mat1": 100.0, "Mat2": 1.0}, 15.0, [("Left", 50.0), ("Right", 5.0)], 1e30,
+     [("Top", 0.0), ("Bot", 0.0)]),
+    ("sphere_3D_hexa:fourier", 23.5, {}, 1.123e-2, [("horizontal", 55.0)], 1e30, [("curved", 1003.67)]),
+]
+
+
+def fourier_lambda(d, lam, materials):
+    cl = np.full(d["cells"].shape[0], lam)
+    for g, v in materials.items():
+        cl[d["cellgroup_" + g]] = v
+    return cl
+
+
+@pytest.mark.parametrize("case,lam,materials,qdot,dirichlet,penalty,neumann", FOURIER, ids=[c[0] for c in FOURIER])
+def test_golden_fourier(case, lam, materials, qdot, dirichlet, penalty, neumann):
+    """modules/fourier tests (inputs/conduction*.arc): per-cell lambda from the
+    default and the material volume groups, constant source qdot, penalty
+    Dirichlet, scalar Neumann; tri3, tet4, Quad4 and Hexa8 meshes; eps 1e-4."""
+    name, _, tag = case.partition(":")
+    d = load_case(name)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    cl = fourier_lambda(d, lam, materials)
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col, cell_lambda=cl)
+    rhs = O.rhs_constant_source(dim, X, C, npc, qdot)
+    for g, v in neumann:
+        if npc != dim + 1:
+            O.rhs_neumann_q1(dim, X, C, npc, d["faces_" + g], v, rhs=rhs)
+        else:
+            O.rhs_neumann(dim, X, C, d["faces_" + g], v, rhs=rhs)
+    fi, fv = np.zeros(N, np.uint8), np.zeros(N)
+    for g, v in dirichlet:
+        fi[d["group_" + g]], fv[d["group_" + g]] = 1, penalty
+        rhs[d["group_" + g]] = penalty * v
+    O.apply_transformations(row, col, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, col, row), shape=(N, N)).tocsc(), rhs)
+    sfx = "_" + tag if tag else ""
+    got = dict(zip(d["node_uid"].tolist(), x))
+    ref_uid, ref = d["ref_uid" + sfx], d["ref_value" + sfx][:, 0]
+    assert O.count_differences(ref, np.array([got[u] for u in ref_uid.tolist()]), 1e-4) == 0
+    if materials:  # one lambda for every cell misses the file
+        val1 = O.assemble(dim, X, C, npc, row, col, cell_lambda=np.full(C.shape[0], 100.0))
+        rhs1 = O.rhs_constant_source(dim, X, C, npc, qdot)
+        for g, v in dirichlet:
+            rhs1[d["group_" + g]] = penalty * v
+        O.apply_transformations(row, col, val1, rhs1, fi, fv)
+        x1 = spl.spsolve(sp.csr_matrix((val1, col, row), shape=(N, N)).tocsc(), rhs1)
+        got1 = dict(zip(d["node_uid"].tolist(), x1))
+        assert O.count_differences(ref, np.array([got1[u] for u in ref_uid.tolist()]), 1e-4) > 0
