@@ -36,7 +36,7 @@ struct Args
   double box_jitter = 0.0;
   unsigned long long box_seed = 0;
   std::vector<double> f;
-  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann, convection;
+  std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann, convection, traction;
   std::vector<std::pair<std::string, double>> materials; // fourier: per-volume-group lambda
   bool has_lambda = false;
   double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
@@ -113,6 +113,14 @@ Args parse(int argc, char** argv)
         usage("--dirichlet expects group=value[,value,value]");
       a.dirichlet.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
     }
+    else if (key == "--traction") {
+      // group=tx,ty[,tz] with NULL for a free component (elasticity)
+      const std::string s = next();
+      const size_t eq = s.rfind('=');
+      if (eq == std::string::npos || eq == 0 || eq + 1 == s.size())
+        usage("--traction expects group=tx,ty[,tz]");
+      a.traction.push_back({ s.substr(0, eq), split(s.substr(eq + 1), ',') });
+    }
     else if (key == "--convection") {
       // group=h,Text (modules/heat/Fem.axl convection-boundary-condition)
       const std::string s = next();
@@ -173,6 +181,8 @@ Args parse(int argc, char** argv)
     usage("give --mesh or --box");
   if (a.physics != "fourier" && !a.materials.empty())
     usage("--material is available for fourier");
+  if (a.physics != "elasticity" && !a.traction.empty())
+    usage("--traction is available for elasticity");
   return a;
 }
 
@@ -369,7 +379,13 @@ int main(int argc, char** argv)
     // _assembleLinearOperator: source / body force, then Dirichlet
     VariableDoFReal& rhs = ls.rhsVariable();
     rhs.fill(0.0);
-    if (elasticity) {
+    if (elasticity && mesh.dimension() == 2) {
+      // modules/elasticity/BodyForce.h:52-104: f_c area / 3 per node and component
+      for (int c = 0; c < 2 && (size_t)c < a.f.size(); ++c)
+        if (a.f[c] != 0.0)
+          BoundaryConditions::applyConstantSourceToRhs(a.f[c], mesh, 2, c, rhs);
+    }
+    else if (elasticity) {
       double f3[3] = { 0.0, 0.0, 0.0 };
       for (size_t k = 0; k < a.f.size() && k < 3; ++k)
         f3[k] = a.f[k];
@@ -377,6 +393,14 @@ int main(int argc, char** argv)
     }
     else if (!a.f.empty()) {
       BoundaryConditions::applyConstantSourceToRhs(a.f[0], mesh, 1, 0, rhs);
+    }
+    // femutils/ArcaneFemFunctions.h:2854-2885 (edges), :2188-2220 (triangles):
+    // t_c |F| / k per node and component
+    for (auto& tr : a.traction) {
+      for (int c = 0; c < nb_dof && (size_t)c < tr.second.size(); ++c) {
+        if (tr.second[c] != "NULL")
+          BoundaryConditions::applyTractionToRhs(mesh, tr.first, nb_dof, c, std::stod(tr.second[c]), rhs);
+      }
     }
     if (!a.neumann.empty() && elasticity)
       usage("--neumann is available for poisson, fourier and heat");

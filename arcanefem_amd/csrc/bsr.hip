@@ -1829,6 +1829,70 @@ k_convection_gather(int32_t nnode, const int32_t* __restrict__ node_ids, const i
   }
 }
 
+// ---- 2D elasticity (tri3, 2 DoFs per node) ----------------------------------------------
+
+// One thread per node i: rows 2i and 2i+1.  For each cell of the node (slot
+// s) and each of its nodes b, the 2x2 block of modules/elasticity/
+// ElementMatrix.h:41-58 written out per entry:
+//   K00 = (l + 2m) dx_s dx_b A + m dy_s dy_b A,  K01 = l dx_s dy_b A + m dy_s dx_b A,
+//   K10 = l dy_s dx_b A + m dx_s dy_b A,         K11 = (l + 2m) dy_s dy_b A + m dx_s dx_b A,
+// with the gradients of femutils/ArcaneFemFunctionsGpu.h:241-281 and the area
+// of :76-86.  Values in per-block or per-row (scalar CSR) order; the thread
+// owns its rows, so there are no atomics.
+__global__ void __launch_bounds__(256)
+k_assemble_elasticity_tri(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                          const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+                          const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns, int per_block,
+                          double lam, double mu, double* __restrict__ values, int32_t* __restrict__ err)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t a0 = row_index[i], len = row_index[i + 1] - a0;
+  const int32_t* rc = columns + a0;
+  double* rv = values + (int64_t)a0 * 4;
+  for (int k = 0; k < 4 * len; ++k)
+    rv[k] = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int s = it & 3;
+    const int4 cq = *reinterpret_cast<const int4*>(cells + (it - s));
+    const int32_t v[3] = { cq.x, cq.y, cq.z };
+    const d3 n0 = load_node(coords, v[0]), n1 = load_node(coords, v[1]), n2 = load_node(coords, v[2]);
+    const double A2 = (n1.x - n0.x) * (n2.y - n0.y) - (n2.x - n0.x) * (n1.y - n0.y);
+    const double dx[3] = { (n1.y - n2.y) / A2, (n2.y - n0.y) / A2, (n0.y - n1.y) / A2 };
+    const double dy[3] = { (n2.x - n1.x) / A2, (n0.x - n2.x) / A2, (n1.x - n0.x) / A2 };
+    const d3 c = cross(sub(n1, n0), sub(n2, n0));
+    const double area = sqrt(c.x * c.x + c.y * c.y + c.z * c.z) / 2.0;
+    const double dxs = s == 0 ? dx[0] : s == 1 ? dx[1] : dx[2];
+    const double dys = s == 0 ? dy[0] : s == 1 ? dy[1] : dy[2];
+    for (int b = 0; b < 3; ++b) {
+      const int pos = find_col(rc, len, v[b]);
+      if (!(pos < len && rc[pos] == v[b])) {
+        atomicOr(err, 1);
+        continue;
+      }
+      const double k00 = (lam + 2 * mu) * dxs * dx[b] * area + mu * dys * dy[b] * area;
+      const double k01 = lam * dxs * dy[b] * area + mu * dys * dx[b] * area;
+      const double k10 = lam * dys * dx[b] * area + mu * dxs * dy[b] * area;
+      const double k11 = (lam + 2 * mu) * dys * dy[b] * area + mu * dxs * dx[b] * area;
+      if (per_block) {
+        double* blk = rv + 4 * pos;
+        blk[0] += k00;
+        blk[1] += k01;
+        blk[2] += k10;
+        blk[3] += k11;
+      }
+      else { // scalar row 2i holds len * 2 entries, then row 2i + 1
+        rv[2 * pos] += k00;
+        rv[2 * pos + 1] += k01;
+        rv[2 * len + 2 * pos] += k10;
+        rv[2 * len + 2 * pos + 1] += k11;
+      }
+    }
+  }
+}
+
 // ---- heat: mass terms -----------------------------------------------------------------
 
 // |T| of cell c: |cross(n1 - n0, n2 - n0)| / 2 (tri) or |det| / 6 (tet), as
@@ -2260,8 +2324,22 @@ static void autotune_laplace(afem_bsr* b, double* best_ms, int* best_variant)
 static void assemble_elasticity(afem_bsr* b, double lam, double mu)
 {
   AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
+  if (b->nb_dof == 2 && b->mesh->dim == 2 && b->mesh->npc == 3) {
+    const afem_mesh& m = *b->mesh;
+    DBuf<int32_t> err(1);
+    err.zero();
+    ProfScope prof("assemble_elasticity_tri");
+    hipLaunchKernelGGL(k_assemble_elasticity_tri, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.coords.p,
+                       m.cells.p, m.nc_offsets.p, m.nc_items.p, m.nb_node, b->row_index.p, b->columns.p,
+                       b->per_block ? 1 : 0, lam, mu, b->values.p, err.p);
+    check_launch("assemble_elasticity_tri");
+    int32_t h = 0;
+    err.download(&h, 1);
+    AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
+    return;
+  }
   AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4 && b->mesh->dim == 3, AFEM_ERR_ARGUMENT,
-               "the elasticity element needs a tetrahedral mesh and nb_dof = 3");
+               "the elasticity element needs a triangle mesh and nb_dof = 2, or a tetrahedral mesh and nb_dof = 3");
   const afem_mesh& m = *b->mesh;
   if (b->ecell.p && b->variant != 0) {
     ProfScope prof("assemble_elasticity_tet");
@@ -2435,7 +2513,7 @@ AFEM_API int afem_bsr_create(afem_mesh_t mesh, int nb_dof, int does_linear_syste
   // femutils/BSRFormat.cc:51-55
   AFEM_REQUIRE(nb_dof > 0, AFEM_ERR_ARGUMENT, "BSRMatrix(initialize): block_size should be positive");
   AFEM_REQUIRE(nb_dof <= mesh->nb_node, AFEM_ERR_ARGUMENT, "BSRMatrix(initialize): block_size should be less than nb_row");
-  AFEM_REQUIRE(nb_dof == 1 || nb_dof == 3, AFEM_ERR_NOT_IMPLEMENTED, "block sizes 1 and 3 are supported");
+  AFEM_REQUIRE(nb_dof >= 1 && nb_dof <= 3, AFEM_ERR_NOT_IMPLEMENTED, "block sizes 1, 2 and 3 are supported");
   auto* b = new afem_bsr();
   try {
     b->mesh = mesh;

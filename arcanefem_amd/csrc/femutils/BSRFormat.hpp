@@ -14,6 +14,7 @@
 #include <cmath>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 
 namespace afem::femutils {
 
@@ -129,6 +130,18 @@ namespace BoundaryConditions {
           "integrateNodalFieldToRhs");
   }
 
+  // Traction component dof_index of an elasticity system: t |F| / k at each
+  // of the k nodes of each face of the group (femutils/ArcaneFemFunctions.h:
+  // 2854-2885 edges, :2188-2220 triangles).
+  inline void applyTractionToRhs(const Mesh& mesh, const std::string& group, int nb_dof, int dof_index, Real t,
+                                 VariableDoFReal& rhs)
+  {
+    const std::vector<Int32>& faces = mesh.faceGroup(group);
+    check(afem_rhs_face_uniform(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), t, nb_dof,
+                                dof_index, rhs.devicePointer()),
+          "applyTractionToRhs");
+  }
+
   // modules/elasticity/BodyForce.h:149-161
   inline void applyBodyForceToRhs(const Real f[3], const Mesh& mesh, VariableDoFReal& rhs)
   {
@@ -205,9 +218,11 @@ namespace BoundaryConditions {
 } // namespace BoundaryConditions
 
 // femutils/FemUtils.cc:85-171 (checkNodeResultFile): reads "uid v..." lines
-// and counts relative differences (Arcane's isNearlyEqualWithEpsilon:
-// equal, or |a-b| / (|a|+|b|) < epsilon), skipping pairs below min_value.
-// Returns the number of differing values and throws if any.
+// (the first ncomp values of each line: vector results are written as Real3,
+// so 2D files carry a zero third component) and counts relative differences
+// (Arcane's isNearlyEqualWithEpsilon: equal, or |a-b| / (|a|+|b|) < epsilon),
+// skipping pairs below min_value.  Returns the number of differing values and
+// throws if any.
 inline Int64 checkNodeResultFile(const std::string& filename, const std::vector<Int64>& uids,
                                  const std::vector<Real>& values, int ncomp, double epsilon, double min_value = 0.0)
 {
@@ -215,11 +230,15 @@ inline Int64 checkNodeResultFile(const std::string& filename, const std::vector<
   if (!f)
     throw FatalError(AFEM_ERR_ARGUMENT, "Invalid result file '" + filename + "'");
   std::map<Int64, std::vector<Real>> ref;
-  Int64 uid;
-  while (f >> uid) {
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    Int64 uid;
+    if (!(in >> uid))
+      continue; // blank line
     std::vector<Real> v((size_t)ncomp);
     for (auto& x : v)
-      if (!(f >> x))
+      if (!(in >> x))
         throw FatalError(AFEM_ERR_ARGUMENT, "Error during parsing of file '" + filename + "'");
     ref[uid] = v;
   }

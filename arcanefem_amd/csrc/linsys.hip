@@ -342,6 +342,8 @@ AFEM_API int afem_system_set_bsr(afem_system_t sys, afem_bsr_t bsr)
   AFEM_REQUIRE(bsr->has_sparsity, AFEM_ERR_STATE, "computeSparsity not called");
   AFEM_REQUIRE((int64_t)bsr->nb_row * bsr->nb_dof == sys->nb_dof, AFEM_ERR_ARGUMENT,
                "matrix size differs from the DoF count");
+  AFEM_REQUIRE(bsr->nb_dof != 2 || !bsr->per_block, AFEM_ERR_NOT_IMPLEMENTED,
+               "2x2 blocks: use the per-row (CSR) value order (does_linear_system_use_csr)");
   sys->has_matrix = true;
   sys->nb_row = bsr->nb_row * bsr->nb_dof;
   sys->nnz = bsr->nb_col * bsr->nb_dof * bsr->nb_dof;

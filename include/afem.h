@@ -129,7 +129,8 @@ typedef struct afem_bsr_info {
  * (femutils/BSRFormat.cc:350-372).  Values are ordered per row (scalar CSR
  * order) when does_linear_system_use_csr, per block otherwise.  The
  * atomic-free flag is accepted for API parity; the MI355X path always builds
- * the pattern and assembles without floating-point atomics. */
+ * the pattern and assembles without floating-point atomics.  nb_dof is 1
+ * (scalar), 2 (2D elasticity) or 3 (3D elasticity). */
 AFEM_API int afem_bsr_create(afem_mesh_t mesh, int nb_dof, int does_linear_system_use_csr,
                              int use_atomic_free, afem_bsr_t* bsr);
 AFEM_API int afem_bsr_destroy(afem_bsr_t bsr);
@@ -174,7 +175,9 @@ AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, d
  * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3.
  * One thread per nonzero block sums the block over the cells of its edge;
  * assembly variant 0 selects the row-per-thread kernel with a column search
- * instead. */
+ * instead.  On a triangle mesh with nb_dof = 2: the 6x6 2D element
+ * (modules/elasticity/ElementMatrix.h:41-58), one thread per node's two rows;
+ * a linear system takes such a matrix in the per-row (CSR) value order. */
 AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu);
 
 /* Device arrays of the matrix: row_index (nb_row+1), columns (nb_col),

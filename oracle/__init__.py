@@ -48,6 +48,7 @@ def lib():
         L.afo_elem_laplace_tria3.argtypes = [vp, dbl, vp]
         L.afo_elem_laplace_tetra4.argtypes = [vp, dbl, vp]
         L.afo_elem_elasticity_tetra4.argtypes = [vp, dbl, dbl, vp]
+        L.afo_elem_elasticity_tria3.argtypes = [vp, dbl, dbl, vp]
         L.afo_lame.argtypes = [dbl, dbl, vp, vp]
         L.afo_node_cells.restype = ctypes.c_int
         L.afo_node_cells.argtypes = [i32, i32, i32, vp, vp, vp]
@@ -134,7 +135,12 @@ def elem_laplace(X, lam=1.0):
 
 
 def elem_elasticity(X, lame_lambda, lame_mu):
+    """tet4 (X (4, 3)): 12 x 12; tri3 (X (3, 3)): 6 x 6, DoFs interleaved per node."""
     X = _f64(X)
+    if X.shape[0] == 3:
+        K = np.zeros((6, 6))
+        lib().afo_elem_elasticity_tria3(_P(X), lame_lambda, lame_mu, _P(K))
+        return K
     K = np.zeros((12, 12))
     lib().afo_elem_elasticity_tetra4(_P(X), lame_lambda, lame_mu, _P(K))
     return K

@@ -302,6 +302,31 @@ void afo_elem_heat_q1(int dim, const double* X, double lambda, double dt, double
       }
 }
 
+/* modules/elasticity/ElementMatrix.h:41-58 (tri3, DoFs interleaved ux0 uy0 ux1
+ * ...): (lambda + 2 mu)(dxUx^dxUx + dyUy^dyUy) area + lambda (dyUy^dxUx +
+ * dxUx^dyUy) area + mu ((dxUy + dyUx)^(dyUx + dxUy)) area, with the vectors
+ * restated as written there. */
+void afo_elem_elasticity_tria3(const double* X, double lam, double mu, double K[36])
+{
+  double dx[3], dy[3];
+  afo_grad_tria3(X, dx, dy);
+  const double area = afo_area_tria3(X, X + 3, X + 6);
+  double dxUx[6] = { 0 }, dyUx[6] = { 0 }, dxUy[6] = { 0 }, dyUy[6] = { 0 };
+  for (int a = 0; a < 3; ++a) {
+    dxUx[2 * a] = dx[a];
+    dyUx[2 * a] = dy[a];
+    dxUy[2 * a + 1] = dx[a];
+    dyUy[2 * a + 1] = dy[a];
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const double normal = (lam + 2 * mu) * (dxUx[i] * dxUx[j] + dyUy[i] * dyUy[j]) * area;
+      const double comp = lam * (dyUy[i] * dxUx[j] + dxUx[i] * dyUy[j]) * area;
+      const double shear = mu * ((dxUy[i] + dyUx[i]) * (dyUx[j] + dxUy[j])) * area;
+      K[i * 6 + j] = normal + comp + shear;
+    }
+}
+
 /* modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
 {
@@ -489,6 +514,9 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
         afo_elem_heat_tria3(X, lambda, lame_lambda, K);
       else
         afo_elem_heat_tetra4(X, lambda, lame_lambda, K);
+    }
+    else if (npc == 3) {
+      afo_elem_elasticity_tria3(X, lame_lambda, lame_mu, K);
     }
     else {
       afo_elem_elasticity_tetra4(X, lame_lambda, lame_mu, K);

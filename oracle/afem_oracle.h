@@ -56,6 +56,8 @@ void afo_elem_laplace_tetra4(const double* X, double lambda, double K[16]);
 /* 12x12 linear-elasticity tet matrix, DoF order (node, component) interleaved.
  * modules/elasticity/ElementMatrix.h:151-211 (computeElementMatrixTetra4Base) */
 void afo_elem_elasticity_tetra4(const double* X, double lame_lambda, double lame_mu, double K[144]);
+/* 2D (tri3, 2 DoFs per node): modules/elasticity/ElementMatrix.h:41-58 */
+void afo_elem_elasticity_tria3(const double* X, double lame_lambda, double lame_mu, double K[36]);
 
 /* Lame parameters from (E, nu).  modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu);

@@ -89,6 +89,14 @@ Golden cases (test names and tolerances from the reference's CMake files):
               'fourier' of sphere_3D_hexa, conduction_hexa.txt: lambda 23.5, qdot 1.123e-2,
               Dirichlet horizontal 55, Neumann curved 1003.67)
 
+  bar_2D  modules/elasticity/inputs/bar.2D.Dirichlet.bodyForce.arc (bar.msh, tri3, 2 DoFs
+              per node; E 21e5, nu 0.28, f (NULL, -1), u = 0 on 'left'),
+              check/bar.2D.Dirichlet.bodyForce.txt; extras 'traction' (bar.2D.Dirichlet.traction
+              .arc: traction (1, NULL) on 'right', no body force), 'traction_bodyforce'
+              (bar.2D.traction.bodyforce.arc: f (3.33, -6.66), traction (1.33, 2.13)), 'point'
+              (bar.2D.PointDirichlet.Dirichlet.bodyForce.arc: ux = 0 on 'left', ux = 1 on
+              'right', u = 0 at botLeft, uy = 0 at botRight)
+
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
 """
@@ -122,6 +130,7 @@ CASES = {
     "cube_3D_hexa8": ("meshes/msh/3x3x3_cube_hexa8.msh", "modules/poisson/check/poisson_test_ref_cube_3D_hexa8.txt", 1),
     "heat_plate_2D_quad": ("meshes/msh/plate.quad.msh", "modules/heat/check/2d_conduction.quad.txt", 1),
     "fourier_multimat_2D": ("meshes/msh/multi-material.msh", "modules/fourier/check/test2_results.txt", 1),
+    "bar_2D": ("meshes/msh/bar.msh", "modules/elasticity/check/bar.2D.Dirichlet.bodyForce.txt", 2),
     "fourier_multimat_quad": ("meshes/msh/multi-material.quad.msh",
                               "modules/fourier/check/conduction_multi-mat_quad.txt", 1),
     "fourier_plancher_quad": ("meshes/msh/plancher.quad4.msh", "modules/fourier/check/conduction_quad.txt", 1),
@@ -130,6 +139,11 @@ CASES = {
 }
 
 EXTRA = {
+    "bar_2D": {
+        "traction": "modules/elasticity/check/bar.2D.Dirichlet.traction.txt",
+        "traction_bodyforce": "modules/elasticity/check/bar.2D.traction.bodyforce.txt",
+        "point": "modules/elasticity/check/bar.2D.PointDirichlet.Dirichlet.bodyForce.txt",
+    },
     "point_dirichlet_2D": {"fourier": "modules/fourier/check/test1_results.txt"},
     "bar_3D": {"fourier": "modules/fourier/check/test_conduction_3D.txt"},
     "circle_2D_quad": {

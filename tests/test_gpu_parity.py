@@ -503,7 +503,7 @@ def test_error_paths(afem_gpu):
         A.Mesh(2, X, bad)
     assert e.value.status == A.ERR_MESH
     with pytest.raises(A.AfemError) as e:
-        A.BSRFormat(m, 2)
+        A.BSRFormat(m, 4)
     assert e.value.status == A.ERR_NOT_IMPLEMENTED
     b.compute_sparsity()
     s.set_bsr(b)
@@ -588,3 +588,35 @@ def test_heat_assembly_matches_oracle(afem_gpu, dim, n):
     ref = O.rhs_neumann(dim, X, C, side, 1.9 * 20.0)
     ref = O.rhs_mass_field(X, C, dim + 1, u, 1.0 / dt, rhs=ref)
     assert np.abs(rhs.numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("use_csr", [True, False])
+def test_elasticity_2d_matches_oracle(afem_gpu, use_csr):
+    """2D elasticity (tri3, 2x2 blocks) against the oracle in both value orders
+    (1e-12 of the row's largest entry); a linear system refuses the per-block
+    2x2 order and takes the per-row one."""
+    A = afem_gpu
+    print(f"[case] elasticity 2D csr={use_csr}", flush=True)
+    X, C = box(2, 12, 0.2)
+    N = X.shape[0]
+    m = A.Mesh(2, X, C)
+    b = A.BSRFormat(m, 2, use_csr=use_csr)
+    b.compute_sparsity()
+    lam, mu = O.lame(21e5, 0.28)
+    b.assemble_elasticity(lam, mu)
+    row, col, val, _ = b.download()
+    orow, ocol = O.pattern(N, C, 3)
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+    oval = O.assemble(2, X, C, 3, orow, ocol, element=O.ELASTICITY, nb_dof=2, order_per_block=not use_csr,
+                      lame_lambda=lam, lame_mu=mu)
+    if use_csr:
+        crow, _ = O.bsr_to_csr(N, 2, orow, ocol)
+        assert row_scaled_error(crow, oval, val) <= ROW_TOL
+    else:
+        assert row_scaled_error(orow, oval, val, nb=2) <= ROW_TOL
+    s = A.DoFLinearSystem(2 * N)
+    if use_csr:
+        s.set_bsr(b)
+    else:
+        with pytest.raises(A.AfemError):
+            s.set_bsr(b)

@@ -1026,3 +1026,72 @@ def test_golden_fourier(case, lam, materials, qdot, dirichlet, penalty, neumann)
         x1 = spl.spsolve(sp.csr_matrix((val1, col, row), shape=(N, N)).tocsc(), rhs1)
         got1 = dict(zip(d["node_uid"].tolist(), x1))
         assert O.count_differences(ref, np.array([got1[u] for u in ref_uid.tolist()]), 1e-4) > 0
+
+
+def test_elasticity_tria3_identities():
+    """2D element: symmetric, the three rigid modes (two translations, the
+    rotation) in its kernel, and u^T K u = the strain energy of a linear
+    field, area (lambda (tr e)^2 + 2 mu e:e)."""
+    rng = np.random.default_rng(31)
+    X = np.zeros((3, 3))
+    X[:, :2] = rng.normal(size=(3, 2))
+    lam, mu = 1.7, 0.6
+    K = O.elem_elasticity(X, lam, mu)
+    assert np.abs(K - K.T).max() <= 1e-14 * np.abs(K).max()
+    modes = [np.tile([1.0, 0.0], 3), np.tile([0.0, 1.0], 3), np.ravel(np.stack([-X[:, 1], X[:, 0]], axis=1))]
+    for r in modes:
+        assert np.abs(K @ r).max() <= 1e-13 * np.abs(K).max()
+    G = rng.normal(size=(2, 2))
+    u = np.ravel(X[:, :2] @ G.T)
+    e = 0.5 * (G + G.T)
+    e1, e2 = X[1, :2] - X[0, :2], X[2, :2] - X[0, :2]
+    area = abs(e1[0] * e2[1] - e1[1] * e2[0]) / 2
+    energy = area * (lam * np.trace(e) ** 2 + 2 * mu * np.sum(e * e))
+    assert abs(u @ K @ u - energy) <= 1e-12 * energy
+
+
+# modules/elasticity/inputs/bar.2D.*.arc on bar.msh: extra tag, body force,
+# tractions (group, (tx, ty)), Dirichlet (group, (ux, uy) with None = free)
+ELASTICITY_2D = [
+    ("", (0.0, -1.0), [], [("left", (0.0, 0.0))]),
+    ("traction", None, [("right", (1.0, 0.0))], [("left", (0.0, 0.0))]),
+    ("traction_bodyforce", (3.33, -6.66), [("right", (1.33, 2.13))], [("left", (0.0, 0.0))]),
+    ("point", (0.0, -1.0), [], [("left", (0.0, None)), ("right", (1.0, None)), ("botLeft", (0.0, 0.0)),
+                                 ("botRight", (None, 0.0))]),
+]
+
+
+@pytest.mark.parametrize("tag,f,tractions,dirichlet", ELASTICITY_2D, ids=[c[0] or "bodyforce" for c in ELASTICITY_2D])
+def test_golden_elasticity_2d(tag, f, tractions, dirichlet):
+    """modules/elasticity 2D tests on tri3 (E 21e5, nu 0.28, plane strain Lame
+    constants, penalty 1e30): body force f area / 3 and traction t length / 2
+    per node and component; eps 1e-3, min 1e-10 (the module's own check)."""
+    d = load_case("bar_2D")
+    X, C = d["coords"], d["cells"]
+    N = X.shape[0]
+    lam, mu = O.lame(21e5, 0.28)
+    row, col = O.pattern(N, C, 3)
+    val = O.assemble(2, X, C, 3, row, col, element=O.ELASTICITY, nb_dof=2, order_per_block=False,
+                     lame_lambda=lam, lame_mu=mu)
+    rhs = np.zeros(2 * N)
+    if f is not None:
+        for c in range(2):
+            O.rhs_constant_source(2, X, C, 3, f[c], nb_dof=2, dof_index=c, rhs=rhs)
+    for g, t in tractions:
+        for c in range(2):
+            O.rhs_face_uniform(X, d["faces_" + g], t[c], nb_dof=2, dof_index=c, rhs=rhs)
+    fi, fv = np.zeros(2 * N, np.uint8), np.zeros(2 * N)
+    for g, vals in dirichlet:
+        for c, v in enumerate(vals):
+            if v is None:
+                continue
+            dofs = 2 * d["group_" + g] + c
+            fi[dofs], fv[dofs], rhs[dofs] = 1, 1e30, 1e30 * v
+    crow, ccol = O.bsr_to_csr(N, 2, row, col)
+    O.apply_transformations(crow, ccol, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, ccol, crow), shape=(2 * N, 2 * N)).tocsc(), rhs)
+    sfx = "_" + tag if tag else ""
+    got = dict(zip(d["node_uid"].tolist(), x.reshape(-1, 2)))
+    ref = d["ref_value" + sfx][:, :2]
+    gv = np.array([got[u] for u in d["ref_uid" + sfx].tolist()])
+    assert O.count_differences(ref.ravel(), gv.ravel(), 1e-3, 1e-10) == 0
