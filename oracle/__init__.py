@@ -49,6 +49,7 @@ def lib():
         L.afo_elem_laplace_tetra4.argtypes = [vp, dbl, vp]
         L.afo_elem_elasticity_tetra4.argtypes = [vp, dbl, dbl, vp]
         L.afo_elem_elasticity_tria3.argtypes = [vp, dbl, dbl, vp]
+        L.afo_elem_elasticity_q1.argtypes = [ctypes.c_int, vp, dbl, dbl, vp]
         L.afo_lame.argtypes = [dbl, dbl, vp, vp]
         L.afo_node_cells.restype = ctypes.c_int
         L.afo_node_cells.argtypes = [i32, i32, i32, vp, vp, vp]
@@ -143,6 +144,14 @@ def elem_elasticity(X, lame_lambda, lame_mu):
         return K
     K = np.zeros((12, 12))
     lib().afo_elem_elasticity_tetra4(_P(X), lame_lambda, lame_mu, _P(K))
+    return K
+
+
+def elem_elasticity_q1(dim, X, lame_lambda, lame_mu):
+    X = _f64(X)
+    nd = (4 if dim == 2 else 8) * dim
+    K = np.zeros((nd, nd))
+    lib().afo_elem_elasticity_q1(dim, _P(X), lame_lambda, lame_mu, _P(K))
     return K
 
 

@@ -327,6 +327,45 @@ void afo_elem_elasticity_tria3(const double* X, double lam, double mu, double K[
     }
 }
 
+/* modules/elasticity/ElementMatrixHexQuad.h:50-70 (Quad4, 8 DoFs), :137-187
+ * (Hexa8, 24 DoFs), summed over the Gauss points with iw = det J: the
+ * displacement-gradient vectors restated as written there (DoFs interleaved
+ * per node). */
+void afo_elem_elasticity_q1(int dim, const double* X, double lam, double mu, double* K)
+{
+  const int n = dim == 2 ? 4 : 8, nd = n * dim, ng = dim == 2 ? 1 : 2;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int k = 0; k < nd * nd; ++k)
+    K[k] = 0.0;
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int iz = 0; iz < ng; ++iz) {
+        double N[8], g[3][8];
+        const double iw = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, g[0], g[1], g[2]);
+        /* D[d][c][i]: derivative along d of displacement component c, DoF i */
+        double D[3][3][24];
+        memset(D, 0, sizeof D);
+        for (int a = 0; a < n; ++a)
+          for (int d = 0; d < dim; ++d)
+            for (int c = 0; c < dim; ++c)
+              D[d][c][a * dim + c] = g[d][a];
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < nd; ++j) {
+            double normal = 0.0, comp = 0.0, shear = 0.0;
+            for (int c = 0; c < dim; ++c)
+              normal += D[c][c][i] * D[c][c][j];
+            for (int c = 0; c < dim; ++c)
+              for (int e = 0; e < dim; ++e)
+                if (e != c)
+                  comp += D[c][c][i] * D[e][e][j];
+            for (int c = 0; c < dim; ++c)
+              for (int e = c + 1; e < dim; ++e)
+                shear += (D[e][c][i] + D[c][e][i]) * (D[e][c][j] + D[c][e][j]);
+            K[i * nd + j] += (lam + 2 * mu) * normal * iw + lam * comp * iw + mu * shear * iw;
+          }
+      }
+}
+
 /* modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu)
 {
@@ -488,18 +527,20 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords,
   const int nb2 = nb * nb;
   const int64_t nnz = row_index[nb_node];
   const int q1 = AFO_IS_Q1(dim, npc);
-  if (q1 && element == AFO_ELASTICITY)
+  if (element == AFO_ELASTICITY && nb_dof != dim)
     return -2;
   memset(values, 0, sizeof(double) * (size_t)(nnz * nb2));
   double X[24];
-  double K[144];
+  double K[576];
   const int ke = npc * nb; /* element matrix order */
   for (int64_t c = 0; c < nb_cell; ++c) {
     const int32_t* cn = cells + c * AFO_CELL_STRIDE(npc);
     for (int k = 0; k < npc; ++k)
       memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
     double lambda = cell_lambda ? cell_lambda[c] : 1.0;
-    if (q1 && element == AFO_HEAT) /* lame_lambda carries dt */
+    if (q1 && element == AFO_ELASTICITY)
+      afo_elem_elasticity_q1(dim, X, lame_lambda, lame_mu, K);
+    else if (q1 && element == AFO_HEAT) /* lame_lambda carries dt */
       afo_elem_heat_q1(dim, X, lambda, lame_lambda, K);
     else if (q1)
       afo_elem_laplace_q1(dim, X, lambda, K);

@@ -58,6 +58,9 @@ void afo_elem_laplace_tetra4(const double* X, double lambda, double K[16]);
 void afo_elem_elasticity_tetra4(const double* X, double lame_lambda, double lame_mu, double K[144]);
 /* 2D (tri3, 2 DoFs per node): modules/elasticity/ElementMatrix.h:41-58 */
 void afo_elem_elasticity_tria3(const double* X, double lame_lambda, double lame_mu, double K[36]);
+/* Quad4 / Hexa8 (2 or 3 DoFs per node, Gauss points): modules/elasticity/
+ * ElementMatrixHexQuad.h:50-70, :137-187 */
+void afo_elem_elasticity_q1(int dim, const double* X, double lame_lambda, double lame_mu, double* K);
 
 /* Lame parameters from (E, nu).  modules/elasticity/FemModule.cc:171-172 */
 void afo_lame(double E, double nu, double* lame_lambda, double* lame_mu);

@@ -97,6 +97,13 @@ Golden cases (test names and tolerances from the reference's CMake files):
               (bar.2D.PointDirichlet.Dirichlet.bodyForce.arc: ux = 0 on 'left', ux = 1 on
               'right', u = 0 at botLeft, uy = 0 at botRight)
 
+  elasticity_plate_quad / elasticity_five_quads / elasticity_cube_hexa  modules/elasticity
+              Quad4 / Hexa8 inputs: bar.2D.Dirichlet.bodyForce.quad.arc and
+              bar.2D.traction.bodyforce.quad.arc (plate.quad.msh), 2D.dirichlet.bodyforce.quad.arc
+              and 2D.dirichlet.traction.bodyforce.quad.arc (five_quads.msh),
+              3D.dirichlet.bodyforce.hexa.arc and 3D.dirichlet.traction.bodyforce.hexa.arc
+              (truncated_cube.hexa.msh); the second of each pair is extra 'traction'
+
 The raw .msh files and the golden solution files are copied beside the .npz
 files as data fixtures (the C++ module driver reads them directly).
 """
@@ -131,6 +138,12 @@ CASES = {
     "heat_plate_2D_quad": ("meshes/msh/plate.quad.msh", "modules/heat/check/2d_conduction.quad.txt", 1),
     "fourier_multimat_2D": ("meshes/msh/multi-material.msh", "modules/fourier/check/test2_results.txt", 1),
     "bar_2D": ("meshes/msh/bar.msh", "modules/elasticity/check/bar.2D.Dirichlet.bodyForce.txt", 2),
+    "elasticity_plate_quad": ("meshes/msh/plate.quad.msh", "modules/elasticity/check/bar.2D.Dirichlet.bodyForce.quad.txt",
+                              2),
+    "elasticity_five_quads": ("meshes/msh/five_quads.msh", "modules/elasticity/check/2D.dirichlet.bodyforce.quad.txt",
+                              2),
+    "elasticity_cube_hexa": ("meshes/msh/truncated_cube.hexa.msh",
+                             "modules/elasticity/check/3D.dirichlet.bodyforce.hexa.txt", 3),
     "fourier_multimat_quad": ("meshes/msh/multi-material.quad.msh",
                               "modules/fourier/check/conduction_multi-mat_quad.txt", 1),
     "fourier_plancher_quad": ("meshes/msh/plancher.quad4.msh", "modules/fourier/check/conduction_quad.txt", 1),
@@ -139,6 +152,9 @@ CASES = {
 }
 
 EXTRA = {
+    "elasticity_plate_quad": {"traction": "modules/elasticity/check/bar.2D.traction.bodyforce.quad.txt"},
+    "elasticity_five_quads": {"traction": "modules/elasticity/check/2D.dirichlet.traction.bodyforce.quad.txt"},
+    "elasticity_cube_hexa": {"traction": "modules/elasticity/check/3D.dirichlet.traction.bodyforce.hexa.txt"},
     "bar_2D": {
         "traction": "modules/elasticity/check/bar.2D.Dirichlet.traction.txt",
         "traction_bodyforce": "modules/elasticity/check/bar.2D.traction.bodyforce.txt",

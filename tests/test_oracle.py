@@ -1095,3 +1095,82 @@ def test_golden_elasticity_2d(tag, f, tractions, dirichlet):
     ref = d["ref_value" + sfx][:, :2]
     gv = np.array([got[u] for u in d["ref_uid" + sfx].tolist()])
     assert O.count_differences(ref.ravel(), gv.ravel(), 1e-3, 1e-10) == 0
+
+
+def test_elasticity_q1_identities():
+    """Q1 elasticity on distorted cells: symmetric, rigid translations and
+    rotations in the kernel, and u^T K u = |T| (lambda (tr e)^2 + 2 mu e:e) for
+    a linear displacement (2-point Gauss integrates det J exactly)."""
+    from arcanefem_amd import box_mesh_q1
+    rng = np.random.default_rng(41)
+    for dim in (2, 3):
+        X, C = box_mesh_q1(dim, 2, 0.25, seed=3)
+        Xc = X[C[1]]
+        lam, mu = 1.7, 0.6
+        K = O.elem_elasticity_q1(dim, Xc, lam, mu)
+        n = C.shape[1]
+        assert np.abs(K - K.T).max() <= 1e-14 * np.abs(K).max()
+        for c in range(dim):
+            r = np.zeros(n * dim)
+            r[c::dim] = 1.0
+            assert np.abs(K @ r).max() <= 1e-13 * np.abs(K).max()
+        r = np.zeros(n * dim)
+        r[0::dim], r[1::dim] = -Xc[:, 1], Xc[:, 0]
+        assert np.abs(K @ r).max() <= 1e-13 * np.abs(K).max()
+        G = rng.normal(size=(dim, dim))
+        u = np.ravel(Xc[:, :dim] @ G.T)
+        e = 0.5 * (G + G.T)
+        vol = sum(O.lib().afo_q1_gradients(dim, np.ascontiguousarray(Xc).ctypes.data, *gp,
+                                           *(np.zeros(8).ctypes.data for _ in range(4))) for gp in _gauss_points(dim))
+        energy = vol * (lam * np.trace(e) ** 2 + 2 * mu * np.sum(e * e))
+        assert abs(u @ K @ u - energy) <= 1e-12 * energy
+
+
+# modules/elasticity Quad4 / Hexa8 inputs: case[:tag], E, nu, body force,
+# tractions (group, t), Dirichlet (group, values)
+ELASTICITY_Q1 = [
+    ("elasticity_plate_quad", 21e5, 0.28, (0.0, -1.0), [], [("left", (0.0, 0.0))]),
+    ("elasticity_plate_quad:traction", 21e5, 0.28, (3.33, -6.66), [("right", (1.33, 2.13))], [("left", (0.0, 0.0))]),
+    ("elasticity_five_quads", 200e9, 0.3, (-9818949214245.0, -7818949234281.0), [],
+     [("bot", (0.0, 0.0)), ("top", (1.9, 14.5))]),
+    ("elasticity_five_quads:traction", 200e9, 0.3, (9.8e9, 7.3e9), [("top", (13.3e9, 14.5e5))], [("bot", (0.0, 0.0))]),
+    ("elasticity_cube_hexa", 200e9, 0.3, (-9.8e12, -7.5e12, 5.9e12), [],
+     [("top", (1.0, 2.0, 8.0)), ("bottom", (12.9, -14.5, -18.8))]),
+    ("elasticity_cube_hexa:traction", 200e9, 0.3, (-9.8e1, -7.5e1, 5.9e1), [("bottom", (12.9e11, -14.5e11, -18.8e11))],
+     [("top", (0.0, 0.0, 0.0))]),
+]
+
+
+@pytest.mark.parametrize("case,E,nu,f,tractions,dirichlet", ELASTICITY_Q1, ids=[c[0] for c in ELASTICITY_Q1])
+def test_golden_elasticity_q1(case, E, nu, f, tractions, dirichlet):
+    """modules/elasticity on Quad4 / Hexa8 meshes: Gauss-point element, body
+    force f_c int N and traction t_c int_F N per node and component, penalty
+    1e30; eps 1e-3, min 1e-10."""
+    name, _, tag = case.partition(":")
+    d = load_case(name)
+    X, C, npc, dim = d["coords"], d["cells"], int(d["npc"]), int(d["dim"])
+    N = X.shape[0]
+    lam, mu = O.lame(E, nu)
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col, element=O.ELASTICITY, nb_dof=dim, order_per_block=False,
+                     lame_lambda=lam, lame_mu=mu)
+    rhs = np.zeros(dim * N)
+    for c in range(dim):
+        O.rhs_constant_source(dim, X, C, npc, f[c], nb_dof=dim, dof_index=c, rhs=rhs)
+    for g, t in tractions:
+        faces = O.arcane_face_order(d["faces_" + g], d["node_uid"]) if npc == 8 else d["faces_" + g]
+        for c in range(dim):
+            O.rhs_neumann_q1(dim, X, C, npc, faces, t[c], nb_dof=dim, dof_index=c, rhs=rhs)
+    fi, fv = np.zeros(dim * N, np.uint8), np.zeros(dim * N)
+    for g, vals in dirichlet:
+        for c, v in enumerate(vals):
+            dofs = dim * d["group_" + g] + c
+            fi[dofs], fv[dofs], rhs[dofs] = 1, 1e30, 1e30 * v
+    crow, ccol = O.bsr_to_csr(N, dim, row, col)
+    O.apply_transformations(crow, ccol, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, ccol, crow), shape=(dim * N, dim * N)).tocsc(), rhs)
+    sfx = "_" + tag if tag else ""
+    got = dict(zip(d["node_uid"].tolist(), x.reshape(-1, dim)))
+    ref = d["ref_value" + sfx][:, :dim]
+    gv = np.array([got[u] for u in d["ref_uid" + sfx].tolist()])
+    assert O.count_differences(ref.ravel(), gv.ravel(), 1e-3, 1e-10) == 0
