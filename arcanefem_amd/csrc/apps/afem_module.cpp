@@ -379,11 +379,13 @@ int main(int argc, char** argv)
     // _assembleLinearOperator: source / body force, then Dirichlet
     VariableDoFReal& rhs = ls.rhsVariable();
     rhs.fill(0.0);
-    if (elasticity && mesh.dimension() == 2) {
-      // modules/elasticity/BodyForce.h:52-104: f_c area / 3 per node and component
-      for (int c = 0; c < 2 && (size_t)c < a.f.size(); ++c)
+    const bool simplex_mesh = mesh.nodesPerCell() == mesh.dimension() + 1;
+    if (elasticity && (mesh.dimension() == 2 || !simplex_mesh)) {
+      // modules/elasticity/BodyForce.h:52-161: f_c area / 3 per node and
+      // component on triangles, f_c int N (Gauss points) on Quad4 / Hexa8
+      for (int c = 0; c < nb_dof && (size_t)c < a.f.size(); ++c)
         if (a.f[c] != 0.0)
-          BoundaryConditions::applyConstantSourceToRhs(a.f[c], mesh, 2, c, rhs);
+          BoundaryConditions::applyConstantSourceToRhs(a.f[c], mesh, nb_dof, c, rhs);
     }
     else if (elasticity) {
       double f3[3] = { 0.0, 0.0, 0.0 };

@@ -2324,6 +2324,10 @@ static void autotune_laplace(afem_bsr* b, double* best_ms, int* best_variant)
 static void assemble_elasticity(afem_bsr* b, double lam, double mu)
 {
   AFEM_REQUIRE(b->has_sparsity, AFEM_ERR_STATE, "computeSparsity must be called before assembly");
+  if (!b->mesh->simplex) {
+    q1_assemble_elasticity(b, lam, mu);
+    return;
+  }
   if (b->nb_dof == 2 && b->mesh->dim == 2 && b->mesh->npc == 3) {
     const afem_mesh& m = *b->mesh;
     DBuf<int32_t> err(1);

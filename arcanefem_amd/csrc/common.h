@@ -285,6 +285,7 @@ namespace afem {
 // RHS when rhs is given) and the constant-source RHS alone.
 void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, double* rhs);
 void q1_assemble_heat(afem_bsr* b, const double* cell_lambda, double dt);
+void q1_assemble_elasticity(afem_bsr* b, double lam, double mu);
 void q1_rhs_mass_field(const afem_mesh& m, const double* field, double scale, int nb_dof, int dof_index, double* rhs);
 void q1_rhs_constant_source(const afem_mesh& m, double f, const double* cell_scale, int nb_dof, int dof_index,
                             double* rhs);

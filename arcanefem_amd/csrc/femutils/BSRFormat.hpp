@@ -132,13 +132,15 @@ namespace BoundaryConditions {
 
   // Traction component dof_index of an elasticity system: t |F| / k at each
   // of the k nodes of each face of the group (femutils/ArcaneFemFunctions.h:
-  // 2854-2885 edges, :2188-2220 triangles).
+  // 2854-2885 edges, :2188-2220 triangles), t int_F N with Gauss points on
+  // Quad4 / Hexa8 meshes (:2222-2290 and the Quad4 edge form): the scalar
+  // Neumann integral of each component.
   inline void applyTractionToRhs(const Mesh& mesh, const std::string& group, int nb_dof, int dof_index, Real t,
                                  VariableDoFReal& rhs)
   {
     const std::vector<Int32>& faces = mesh.faceGroup(group);
-    check(afem_rhs_face_uniform(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), t, nb_dof,
-                                dof_index, rhs.devicePointer()),
+    check(afem_rhs_neumann(mesh.handle(), (int)(faces.size() / mesh.nodesPerFace()), faces.data(), 1, &t, nb_dof,
+                           dof_index, rhs.devicePointer()),
           "applyTractionToRhs");
   }
 

@@ -399,6 +399,97 @@ k_q1_mass_field(const double* __restrict__ coords, const int32_t* __restrict__ c
   rhs[(int64_t)i * nb_dof + dof_index] += acc;
 }
 
+// Q1 linear elasticity (modules/elasticity/ElementMatrixHexQuad.h:50-70,
+// :137-187), one thread per node i and its DIM rows.  For each cell of the
+// node (slot s), the Gauss-point sum of the DIM x DIM blocks
+//   B_ab[c][e] = lambda g_a[c] g_b[e] + mu g_a[e] g_b[c] + mu (g_a . g_b) delta_ce
+// (a = s, g the physical gradients) times det J, added into the node's rows
+// (per-block or per-row value order).  *err = 1 when a column is missing.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_q1_elasticity(const double* __restrict__ coords, const int32_t* __restrict__ cells, int slot_bits,
+                const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+                const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns, int per_block, double lam,
+                double mu, double* __restrict__ values, int32_t* __restrict__ err)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NG = DIM == 2 ? 4 : 8;
+  constexpr int B = DIM;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t a0 = row_index[i], len = row_index[i + 1] - a0;
+  const int32_t* rc = columns + a0;
+  double* rv = values + (int64_t)a0 * B * B;
+  for (int k = 0; k < len * B * B; ++k)
+    rv[k] = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int s = it & ((1 << slot_bits) - 1);
+    const int32_t* cn = cells + (it - s);
+    int32_t node[N];
+    double x[N][3];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      node[a] = cn[a];
+      const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)node[a]);
+      x[a][0] = xy.x;
+      x[a][1] = xy.y;
+      x[a][2] = coords[4 * (int64_t)node[a] + 2];
+    }
+    double blk[N][B][B];
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+#pragma unroll
+        for (int d = 0; d < B; ++d)
+          blk[b][c][d] = 0.0;
+    for (int g = 0; g < NG; ++g) {
+      Q1Point<DIM> q;
+      q.eval(x, g);
+      double gs[3] = { 0.0, 0.0, 0.0 };
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+        if (a == s) {
+          gs[0] = q.gx[a];
+          gs[1] = q.gy[a];
+          gs[2] = q.gz[a];
+        }
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        const double gb[3] = { q.gx[b], q.gy[b], q.gz[b] };
+        double dotv = 0.0;
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+          dotv += gs[c] * gb[c];
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+#pragma unroll
+          for (int d = 0; d < B; ++d)
+            blk[b][c][d] += (lam * gs[c] * gb[d] + mu * gs[d] * gb[c] + (c == d ? mu * dotv : 0.0)) * q.det;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      const int pos = q1_find_col(rc, len, node[b]);
+      if (!(pos < len && rc[pos] == node[b])) {
+        atomicOr(err, 1);
+        continue;
+      }
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+#pragma unroll
+        for (int d = 0; d < B; ++d) {
+          if (per_block)
+            rv[(int64_t)pos * B * B + c * B + d] += blk[b][c][d];
+          else // scalar row B i + c holds len * B entries
+            rv[(int64_t)c * len * B + pos * B + d] += blk[b][c][d];
+        }
+    }
+  }
+}
+
 template <bool MAT>
 void launch_q1(const afem_mesh& m, const int32_t* row_index, const int32_t* columns, const double* cell_lambda,
                double* values, double f, const double* cell_scale, int nb_dof, int dof_index, double* rhs,
@@ -505,6 +596,28 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
 void q1_assemble_heat(afem_bsr* b, const double* cell_lambda, double dt)
 {
   q1_assemble(b, cell_lambda, 0.0, nullptr, 1.0 / dt);
+}
+
+void q1_assemble_elasticity(afem_bsr* b, double lam, double mu)
+{
+  const afem_mesh& m = *b->mesh;
+  AFEM_REQUIRE(b->nb_dof == m.dim, AFEM_ERR_ARGUMENT, "Q1 elasticity needs nb_dof = dim");
+  DBuf<int32_t> err(1);
+  err.zero();
+  ProfScope prof(m.dim == 2 ? "assemble_elasticity_quad4" : "assemble_elasticity_hexa8");
+  const dim3 grid(grid_for(m.nb_node, 256)), block(256);
+  if (m.dim == 2)
+    hipLaunchKernelGGL(k_q1_elasticity<2>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, b->row_index.p, b->columns.p, b->per_block ? 1 : 0,
+                       lam, mu, b->values.p, err.p);
+  else
+    hipLaunchKernelGGL(k_q1_elasticity<3>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, b->row_index.p, b->columns.p, b->per_block ? 1 : 0,
+                       lam, mu, b->values.p, err.p);
+  check_launch("q1_elasticity");
+  int32_t h = 0;
+  err.download(&h, 1);
+  AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
 }
 
 void q1_rhs_mass_field(const afem_mesh& m, const double* field, double scale, int nb_dof, int dof_index, double* rhs)

@@ -1,7 +1,7 @@
 """GPU parity of the Quad4 / Hexa8 (Q1) path through the C ABI against the
 oracle: node->cell lists and sparsity pattern bit-exact; Laplace values and
 constant-source RHS within 1e-12 of the row's / vector's largest entry (both
-assembly forms); Neumann on edges and quadrangles; the
+assembly forms); Neumann on edges and quadrangles; heat and elasticity; the
 reference's Q1 golden files (circle_2D_quad, sphere_3D_hexa: f = 5.5,
 u = 0.5 on 'horizontal', eps 1e-4) through a GPU solve; the element kernels
 that have no Q1 form refuse Q1 meshes."""
@@ -154,10 +154,10 @@ def test_q1_unsupported_kernels_refuse(afem_gpu):
     b.compute_sparsity()
     with pytest.raises(A.AfemError):
         b.autotune_assembly()
-    b3 = A.BSRFormat(m, 3, use_csr=False)
-    b3.compute_sparsity()
-    with pytest.raises(A.AfemError):
-        b3.assemble_elasticity(1.0, 1.0)
+    b1 = A.BSRFormat(m, 1, use_csr=True)
+    b1.compute_sparsity()
+    with pytest.raises(A.AfemError):  # elasticity needs dim DoFs per node
+        b1.assemble_elasticity(1.0, 1.0)
     X2, C2 = A.box_mesh_q1(2, 3, 0.1, seed=1)
     m2 = A.Mesh(2, X2, C2, npc=4)
     f = np.array([0.0, 0.0, -1.0])
@@ -302,3 +302,30 @@ def test_q1_long_rows_fall_back(afem_gpu):
     assert np.abs(rhs.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max()
     with pytest.raises(A.AfemError):
         b.assemble_heat(None, 0.1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q1_elasticity_matches_oracle(afem_gpu, dim):
+    """Q1 elasticity blocks against the oracle in both value orders (1e-12 of
+    the scalar row's largest entry)."""
+    A = afem_gpu
+    print(f"[case] q1 elasticity dim={dim}", flush=True)
+    X, C = A.box_mesh_q1(dim, 5 if dim == 2 else 3, 0.2, seed=23)
+    N, npc = X.shape[0], C.shape[1]
+    m = A.Mesh(dim, X, C, npc=npc)
+    lam, mu = O.lame(200e9, 0.3)
+    orow, ocol = O.pattern(N, C, npc)
+    crow, _ = O.bsr_to_csr(N, dim, orow, ocol)
+    for use_csr in (True, False):
+        b = A.BSRFormat(m, dim, use_csr=use_csr)
+        b.compute_sparsity()
+        b.assemble_elasticity(lam, mu)
+        _, _, val, _ = b.download()
+        oval = O.assemble(dim, X, C, npc, orow, ocol, element=O.ELASTICITY, nb_dof=dim, order_per_block=False,
+                          lame_lambda=lam, lame_mu=mu)
+        if not use_csr:  # per-block order: compare through the oracle's per-block layout
+            oval = O.assemble(dim, X, C, npc, orow, ocol, element=O.ELASTICITY, nb_dof=dim, order_per_block=True,
+                              lame_lambda=lam, lame_mu=mu)
+            assert np.abs(val - oval).max() <= 1e-12 * np.abs(oval).max()
+        else:
+            assert row_scaled_error(crow, oval, val) <= ROW_TOL

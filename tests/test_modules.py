@@ -316,6 +316,34 @@ def test_elasticity_module_2d_golden(args, ref, method):
     _check_ok(r)
 
 
+# modules/elasticity Quad4 / Hexa8 inputs (E, nu, f, Dirichlet, traction)
+ELASTICITY_Q1 = [
+    ("plate.quad.msh", ["--E", "21e5", "--nu", "0.28", "--f", "NULL,-1.0", "--dirichlet", "left=0.0,0.0"],
+     "bar.2D.Dirichlet.bodyForce.quad.txt"),
+    ("plate.quad.msh", ["--E", "21e5", "--nu", "0.28", "--f", "3.33,-6.66", "--dirichlet", "left=0.0,0.0",
+                        "--traction", "right=1.33,2.13"], "bar.2D.traction.bodyforce.quad.txt"),
+    ("five_quads.msh", ["--E", "200e9", "--nu", "0.3", "--f", "-9818949214245.0,-7818949234281.0", "--dirichlet",
+                        "bot=0.0,0.0", "--dirichlet", "top=1.9,14.5"], "2D.dirichlet.bodyforce.quad.txt"),
+    ("five_quads.msh", ["--E", "200e9", "--nu", "0.3", "--f", "9.8e9,7.3e9", "--dirichlet", "bot=0.0,0.0",
+                        "--traction", "top=13.3e9,14.5e5"], "2D.dirichlet.traction.bodyforce.quad.txt"),
+    ("truncated_cube.hexa.msh", ["--E", "200e9", "--nu", "0.3", "--f", "-9.8e12,-7.5e12,5.9e12", "--dirichlet",
+                                 "top=1.,2.,8.", "--dirichlet", "bottom=12.9,-14.5,-18.8"],
+     "3D.dirichlet.bodyforce.hexa.txt"),
+    ("truncated_cube.hexa.msh", ["--E", "200e9", "--nu", "0.3", "--f", "-9.8e1,-7.5e1,5.9e1", "--dirichlet",
+                                 "top=0.,0.,0.", "--traction", "bottom=12.9e11,-14.5e11,-18.8e11"],
+     "3D.dirichlet.traction.bodyforce.hexa.txt"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msh,args,ref", ELASTICITY_Q1, ids=[c[2][:-4] for c in ELASTICITY_Q1])
+def test_elasticity_module_q1_golden(msh, args, ref):
+    # penalty method (the inputs' default), the module's own check (eps 1e-3, min 1e-10)
+    r = run(["elasticity", "--mesh", os.path.join(GOLDEN, msh), "--rtol", "1e-12", "--check",
+             os.path.join(GOLDEN, ref), "--epsilon", "1e-3", "--min-value", "1e-10"] + args)
+    _check_ok(r)
+
+
 def test_traction_usage_error():
     r = run(["poisson", "--box", "2", "4", "0", "0", "--traction", "x=1=1.0"])
     assert r.returncode == 2 and "--traction is available for elasticity" in r.stderr

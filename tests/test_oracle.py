@@ -841,8 +841,8 @@ def test_q1_pattern_and_assembly(dim, n):
     assert abs(u @ (A1 @ u) - a @ a) <= 1e-12
     b = O.rhs_constant_source(dim, X, C, npc, 3.0)
     assert abs(b.sum() - 3.0) <= 1e-13
-    with pytest.raises(ValueError):
-        O.assemble(dim, X, C, npc, row, col, element=O.ELASTICITY, nb_dof=dim)
+    with pytest.raises(ValueError):  # elasticity needs dim DoFs per node
+        O.assemble(dim, X, C, npc, row, col, element=O.ELASTICITY, nb_dof=1)
 
 
 def _q1_neumann_solve(d, f, neu_faces, value, dir_group, dir_value, tag=""):
