@@ -298,7 +298,13 @@ int run_heat(const Args& a, Mesh& mesh, Runner& runner)
 
 int main(int argc, char** argv)
 {
-  Args a = parse(argc, argv);
+  Args a;
+  try {
+    a = parse(argc, argv);
+  }
+  catch (const std::exception& e) { // std::stod / std::stoi on a malformed number
+    usage(std::string("malformed number in the arguments (") + e.what() + ")");
+  }
   try {
     if (a.parse_only) { // modules/poisson/CMakeLists.txt "2D_parse_and_exit"
       Mesh host_mesh;
@@ -440,6 +446,10 @@ int main(int argc, char** argv)
   catch (const FatalError& e) {
     std::fprintf(stderr, "afem_module: fatal (status %d): %s\n", e.status, e.what());
     return 1;
+  }
+  catch (const std::exception& e) { // e.g. a condition value that is not a number
+    std::fprintf(stderr, "afem_module: invalid input: %s\n", e.what());
+    return 2;
   }
   return 0;
 }

@@ -42,6 +42,7 @@ def test_binary_built():
     (["poisson", "--box", "3", "4", "0", "0", "--bogus"], "unknown option"),
     (["poisson", "--mesh", "x.msh", "--dirichlet", "nogroupvalue"], "--dirichlet expects"),
     (["heat", "--mesh", "x.msh", "--convection", "top=1.0"], "--convection expects"),
+    (["poisson", "--box", "2", "four", "0", "0"], "malformed number"),
 ])
 def test_usage_errors(args, msg):
     r = run(args)
