@@ -218,63 +218,95 @@ k_assemble_q1(const double* __restrict__ coords, const int32_t* __restrict__ cel
 // Gauss-point evaluations).
 // HEAT: lambda_c K + M / dt with M_ab = sum_g N_a N_b det J_g
 // (modules/heat/ElementMatrixHexQuad.h:33-130); the gather then applies no lambda.
+// The records (N * N + N doubles per cell) leave through LDS in chunks of CH
+// values per cell, so that a wave's stores cover contiguous runs instead of
+// one double every record length.
+constexpr int kQ1ElemBlock = 128;
+
+template <int DIM>
+struct Q1Chunk
+{
+  static constexpr int N = DIM == 2 ? 4 : 8;
+  static constexpr int NE = N * N + N;
+  static constexpr int CH = DIM == 2 ? NE : (NE + 1) / 2; // values per cell per chunk
+  static constexpr int PAD = CH + 1;                      // LDS stride per cell (bank spread)
+};
+
 template <int DIM, bool HEAT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kQ1ElemBlock)
 k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cells, int32_t nb_cell,
               const double* __restrict__ cell_lambda, double inv_dt, double* __restrict__ elem)
 {
   constexpr int N = Q1Point<DIM>::N;
   constexpr int NG = DIM == 2 ? 4 : 8;
   constexpr int NT = N * (N + 1) / 2;
-  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb_cell)
-    return;
-  constexpr int stride = N > 4 ? N : 4;
-  double x[N][3];
-#pragma unroll
-  for (int a = 0; a < N; ++a) {
-    const int32_t n = cells[(int64_t)stride * c + a];
-    const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)n);
-    x[a][0] = xy.x;
-    x[a][1] = xy.y;
-    x[a][2] = coords[4 * (int64_t)n + 2];
-  }
+  constexpr int NE = Q1Chunk<DIM>::NE, CH = Q1Chunk<DIM>::CH, PAD = Q1Chunk<DIM>::PAD;
+  __shared__ double s_rec[kQ1ElemBlock * PAD];
+  const int t = threadIdx.x;
+  const int32_t c0 = blockIdx.x * kQ1ElemBlock, c = c0 + t;
   double K[NT], src[N]; // upper triangle, row-major
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-    K[t] = 0.0;
+  for (int k = 0; k < NT; ++k)
+    K[k] = 0.0;
 #pragma unroll
   for (int a = 0; a < N; ++a)
     src[a] = 0.0;
-  const double lam = HEAT && cell_lambda ? cell_lambda[c] : 1.0;
-  for (int g = 0; g < NG; ++g) {
-    Q1Point<DIM> q;
-    q.eval(x, g);
-    int t = 0;
+  if (c < nb_cell) {
+    constexpr int stride = N > 4 ? N : 4;
+    double x[N][3];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
+      const int32_t n = cells[(int64_t)stride * c + a];
+      const double2 xy = *reinterpret_cast<const double2*>(coords + 4 * (int64_t)n);
+      x[a][0] = xy.x;
+      x[a][1] = xy.y;
+      x[a][2] = coords[4 * (int64_t)n + 2];
+    }
+    const double lam = HEAT && cell_lambda ? cell_lambda[c] : 1.0;
+    for (int g = 0; g < NG; ++g) {
+      Q1Point<DIM> q;
+      q.eval(x, g);
+      int k = 0;
 #pragma unroll
-      for (int b = a; b < N; ++b, ++t) {
-        if (HEAT)
-          K[t] += (q.gx[a] * q.gx[b]) * q.det * lam + (q.gy[a] * q.gy[b]) * q.det * lam +
-                  (q.gz[a] * q.gz[b]) * q.det * lam + (q.Nv[a] * q.Nv[b]) * q.det * inv_dt;
-        else
-          K[t] += (q.gx[a] * q.gx[b]) * q.det + (q.gy[a] * q.gy[b]) * q.det + (q.gz[a] * q.gz[b]) * q.det;
+      for (int a = 0; a < N; ++a) {
+#pragma unroll
+        for (int b = a; b < N; ++b, ++k) {
+          if (HEAT)
+            K[k] += (q.gx[a] * q.gx[b]) * q.det * lam + (q.gy[a] * q.gy[b]) * q.det * lam +
+                    (q.gz[a] * q.gz[b]) * q.det * lam + (q.Nv[a] * q.Nv[b]) * q.det * inv_dt;
+          else
+            K[k] += (q.gx[a] * q.gx[b]) * q.det + (q.gy[a] * q.gy[b]) * q.det + (q.gz[a] * q.gz[b]) * q.det;
+        }
+        src[a] += q.Nv[a] * q.det;
       }
-      src[a] += q.Nv[a] * q.det;
     }
   }
-  double* out = elem + (int64_t)(N * N + N) * c;
+  const int32_t ncell = min(kQ1ElemBlock, nb_cell - c0);
+  double* out = elem + (int64_t)NE * c0;
+  double* rec = s_rec + t * PAD;
 #pragma unroll
-  for (int a = 0; a < N; ++a)
+  for (int base = 0; base < NE; base += CH) {
+    // record entry v: v < N * N is K(a, b) with a = v / N, b = v % N; then src
 #pragma unroll
-    for (int b = 0; b < N; ++b) {
-      const int lo = b < a ? b : a, hi = b < a ? a : b;
-      out[a * N + b] = K[lo * N - lo * (lo - 1) / 2 + (hi - lo)];
+    for (int v = base; v < base + CH && v < NE; ++v) {
+      double val;
+      if (v < N * N) {
+        const int a = v / N, b = v % N, lo = b < a ? b : a, hi = b < a ? a : b;
+        val = K[lo * N - lo * (lo - 1) / 2 + (hi - lo)];
+      }
+      else {
+        val = src[v - N * N];
+      }
+      rec[v - base] = val;
     }
-#pragma unroll
-  for (int a = 0; a < N; ++a)
-    out[N * N + a] = src[a];
+    __syncthreads();
+    const int chn = (base + CH <= NE ? CH : NE - base);
+    for (int j = t; j < ncell * chn; j += kQ1ElemBlock) {
+      const int cell = j / chn, k = j - cell * chn;
+      out[(int64_t)cell * NE + base + k] = s_rec[cell * PAD + k];
+    }
+    __syncthreads();
+  }
 }
 
 // Per (node, cell) pair e (node->cell order): the positions in the node's
@@ -529,18 +561,18 @@ void q1_assemble(afem_bsr* b, const double* cell_lambda, double f, double* rhs, 
     const size_t need = (size_t)(n * n + n) * m.nb_cell;
     if (b->q1_elem.n != need)
       b->q1_elem.alloc(need);
-    const dim3 cgrid(grid_for(m.nb_cell, 256)), block(256);
+    const dim3 cgrid(grid_for(m.nb_cell, kQ1ElemBlock)), block(256), eblock(kQ1ElemBlock);
     if (m.dim == 2 && heat)
-      hipLaunchKernelGGL((k_q1_elements<2, true>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+      hipLaunchKernelGGL((k_q1_elements<2, true>), cgrid, eblock, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
                          cell_lambda, inv_dt, b->q1_elem.p);
     else if (m.dim == 2)
-      hipLaunchKernelGGL((k_q1_elements<2, false>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+      hipLaunchKernelGGL((k_q1_elements<2, false>), cgrid, eblock, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
                          nullptr, 0.0, b->q1_elem.p);
     else if (heat)
-      hipLaunchKernelGGL((k_q1_elements<3, true>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+      hipLaunchKernelGGL((k_q1_elements<3, true>), cgrid, eblock, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
                          cell_lambda, inv_dt, b->q1_elem.p);
     else
-      hipLaunchKernelGGL((k_q1_elements<3, false>), cgrid, block, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
+      hipLaunchKernelGGL((k_q1_elements<3, false>), cgrid, eblock, 0, stream(), m.coords.p, m.cells.p, m.nb_cell,
                          nullptr, 0.0, b->q1_elem.p);
     const double* gather_lambda = heat ? nullptr : cell_lambda; // the heat element carries lambda already
     check_launch("q1_elements");
