@@ -103,6 +103,8 @@ def _declare(L):
         "afem_bsr_assemble_heat": (ci, [vp, vp, ctypes.c_double]),
         "afem_bsr_add_convection": (ci, [vp, ci, vp, ctypes.c_double]),
         "afem_rhs_face_uniform": (ci, [vp, ci, vp, ctypes.c_double, ci, ci, vp]),
+        "afem_rhs_manufactured": (ci, [vp, ci, ci, ci, vp]),
+        "afem_manufactured_l2_error": (ci, [vp, ci, vp, vp]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
         "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
@@ -516,6 +518,26 @@ def rhs_face_uniform(mesh, faces, g, rhs, nb_dof=1, dof_index=0):
     faces = np.ascontiguousarray(faces, np.int32)
     check(lib().afem_rhs_face_uniform(mesh.h, int(faces.shape[0]), _ptr(faces), float(g), nb_dof, dof_index,
                                       rhs.c))
+
+
+MANUFACTURED = {"sine": 0, "poly": 1, "quadratic": 2}
+
+
+def rhs_manufactured(mesh, solution, rhs, nb_dof=1, dof_index=0):
+    """rhs = the manufactured source `solution` ("sine", "poly", "quadratic")
+    integrated against the shape functions (tri3, Quad4, Hexa8 meshes)."""
+    if solution not in MANUFACTURED:
+        raise ValueError(f"unknown manufactured solution {solution!r}")
+    check(lib().afem_rhs_manufactured(mesh.h, MANUFACTURED[solution], nb_dof, dof_index, rhs.c))
+
+
+def manufactured_l2_error(mesh, solution, u):
+    """L2 error of the nodal solution u (a DeviceArray) against `solution`."""
+    if solution not in MANUFACTURED:
+        raise ValueError(f"unknown manufactured solution {solution!r}")
+    out = ctypes.c_double(0.0)
+    check(lib().afem_manufactured_l2_error(mesh.h, MANUFACTURED[solution], u.c, ctypes.byref(out)))
+    return out.value
 
 
 def rhs_mass_field(mesh, field, scale, rhs, nb_dof=1, dof_index=0):

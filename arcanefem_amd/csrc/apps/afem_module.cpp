@@ -21,7 +21,11 @@
 // Exit status 0 when the solve converged and the check (if any) passed.
 #include "femutils/BSRFormat.hpp"
 
+#include <algorithm>
+#include <array>
 #include <chrono>
+#include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstdlib>
 
@@ -39,6 +43,9 @@ struct Args
   std::vector<std::pair<std::string, std::vector<std::string>>> dirichlet, neumann, convection, traction;
   std::vector<std::pair<std::string, double>> materials; // fourier: per-volume-group lambda
   bool has_lambda = false;
+  // modules/poisson/Fem.axl:47-53: manufactured-solution name and tolerance
+  std::string manufactured;
+  double mms_tolerance = 1.0e-1;
   double E = 21e5, nu = 0.28, rtol = 1e-12, epsilon = 1e-4, min_value = 0.0;
   // heat (modules/heat/Fem.axl defaults) and the Dirichlet penalty
   // (femutils/FemBoundaryConditions.axl:104-109)
@@ -144,6 +151,10 @@ Args parse(int argc, char** argv)
         usage("--material expects group=lambda");
       a.materials.push_back({ s.substr(0, eq), std::stod(s.substr(eq + 1)) });
     }
+    else if (key == "--manufactured")
+      a.manufactured = next();
+    else if (key == "--mms-tolerance")
+      a.mms_tolerance = std::stod(next());
     else if (key == "--dt")
       a.dt = std::stod(next());
     else if (key == "--tmax")
@@ -183,6 +194,13 @@ Args parse(int argc, char** argv)
     usage("--material is available for fourier");
   if (a.physics != "elasticity" && !a.traction.empty())
     usage("--traction is available for elasticity");
+  if (!a.manufactured.empty() && (!a.dirichlet.empty() || !a.neumann.empty() || !a.f.empty()))
+    usage("--manufactured sets its own source and boundary values");
+  if (!a.manufactured.empty() && a.physics != "poisson")
+    usage("--manufactured is available for poisson");
+  if (!a.manufactured.empty() && a.manufactured != "sine" && a.manufactured != "poly" &&
+      a.manufactured != "quadratic")
+    usage("--manufactured expects sine, poly or quadratic");
   return a;
 }
 
@@ -192,6 +210,61 @@ NodeGroup group_of(const Mesh& m, const std::string& name)
   if (name.size() > 2 && name[1] == '=' && (name[0] == 'x' || name[0] == 'y' || name[0] == 'z'))
     return m.nodesOnPlane(name[0] - 'x', std::stod(name.substr(2)));
   return m.group(name);
+}
+
+// Nodes of the outer faces (faces of exactly one cell): Arcane's
+// mesh()->outerFaces().nodeGroup(), used by the manufactured solutions
+// (modules/poisson/ManufacturedSolutions.cc:248-262).
+NodeGroup outer_face_nodes(const Mesh& mesh)
+{
+  static const int tri[3][2] = { { 0, 1 }, { 1, 2 }, { 2, 0 } };
+  static const int quad[4][2] = { { 0, 1 }, { 1, 2 }, { 2, 3 }, { 3, 0 } };
+  static const int tet[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
+  static const int hexa[6][4] = { { 0, 1, 2, 3 }, { 4, 5, 6, 7 }, { 0, 1, 5, 4 },
+                                  { 1, 2, 6, 5 }, { 2, 3, 7, 6 }, { 3, 0, 4, 7 } };
+  const int npc = mesh.nodesPerCell(), stride = npc > 4 ? npc : 4;
+  int nf, k;
+  const int* table;
+  if (mesh.dimension() == 2 && npc == 3) { nf = 3; k = 2; table = &tri[0][0]; }
+  else if (mesh.dimension() == 2 && npc == 4) { nf = 4; k = 2; table = &quad[0][0]; }
+  else if (npc == 4) { nf = 4; k = 3; table = &tet[0][0]; }
+  else { nf = 6; k = 4; table = &hexa[0][0]; }
+  std::map<std::array<Int32, 4>, int> count;
+  const std::vector<Int32>& cells = mesh.cells();
+  for (Int32 c = 0; c < mesh.nbCell(); ++c)
+    for (int f = 0; f < nf; ++f) {
+      std::array<Int32, 4> key = { -1, -1, -1, -1 };
+      for (int j = 0; j < k; ++j)
+        key[j] = cells[(size_t)c * stride + table[f * k + j]];
+      std::sort(key.begin(), key.begin() + k);
+      ++count[key];
+    }
+  std::vector<char> on((size_t)mesh.nbNode(), 0);
+  for (auto& kv : count)
+    if (kv.second == 1)
+      for (int j = 0; j < k; ++j)
+        on[(size_t)kv.first[j]] = 1;
+  NodeGroup g;
+  for (Int32 i = 0; i < mesh.nbNode(); ++i)
+    if (on[(size_t)i])
+      g.push_back(i);
+  return g;
+}
+
+// modules/poisson/ManufacturedSolutions.h:91-142 (the exact solutions)
+double manufactured_exact(const std::string& name, double x, double y)
+{
+  const double pi = 3.141592653589793238462643383279502884;
+  if (name == "sine")
+    return std::sin(pi * x) * std::sin(pi * y);
+  if (name == "poly")
+    return x * (1.0 - x) * y * (1.0 - y);
+  return x * x + y * y;
+}
+
+int manufactured_id(const std::string& name)
+{
+  return name == "sine" ? 0 : name == "poly" ? 1 : 2;
 }
 
 void apply_dirichlet(const Args& a, const Mesh& mesh, int nb_dof, DoFLinearSystem& ls)
@@ -399,6 +472,21 @@ int main(int argc, char** argv)
         f3[k] = a.f[k];
       BoundaryConditions::applyBodyForceToRhs(f3, mesh, rhs);
     }
+    else if (!a.manufactured.empty()) {
+      // _applyManufacturedSource then _applyManufacturedDirichlet
+      // (modules/poisson/FemModule.cc:181-185): the source replaces the
+      // module's RHS; every outer-face node gets the penalty diagonal and
+      // penalty * exact(x) in the RHS.
+      check(afem_rhs_manufactured(mesh.handle(), manufactured_id(a.manufactured), 1, 0, rhs.devicePointer()),
+            "applyManufacturedSource");
+      const NodeGroup outer = outer_face_nodes(mesh);
+      BoundaryConditions::applyDirichletToLhsAndRhs("Penalty", outer, 1, 0, 0.0, ls, a.penalty);
+      std::vector<Real> b = rhs.toHost();
+      for (Int32 i : outer)
+        b[(size_t)i] = a.penalty * manufactured_exact(a.manufactured, mesh.coords()[3 * (size_t)i],
+                                                      mesh.coords()[3 * (size_t)i + 1]);
+      rhs.fromHost(b);
+    }
     else if (!a.f.empty()) {
       BoundaryConditions::applyConstantSourceToRhs(a.f[0], mesh, 1, 0, rhs);
     }
@@ -420,7 +508,7 @@ int main(int argc, char** argv)
     // ||b|| meaningless, so those systems stop on the preconditioned norm
     // ||M^-1 r|| / ||M^-1 b||, in which a penalty row weighs like any other.
     auto* impl = dynamic_cast<Mi355xDoFLinearSystemImpl*>(ls.implementation());
-    if (a.method == "Penalty")
+    if (a.method == "Penalty" || !a.manufactured.empty())
       ls.setSolverCommandLineArguments(CommandLineArguments{ { "-ksp_norm_type", "preconditioned" } });
     ls.applyLinearSystemTransformationAndSolve();
     auto t1 = std::chrono::steady_clock::now();
@@ -441,6 +529,15 @@ int main(int argc, char** argv)
     if (!a.check.empty()) {
       checkNodeResultFile(a.check, mesh.uniqueIds(), u, nb_dof, a.epsilon, a.min_value);
       std::printf("CHECK OK %s\n", a.check.c_str());
+    }
+    if (!a.manufactured.empty()) { // modules/poisson/FemModule.cc:469-476
+      double l2 = 0.0;
+      check(afem_manufactured_l2_error(mesh.handle(), manufactured_id(a.manufactured),
+                                       ls.solutionVariable().devicePointer(), &l2),
+            "computeManufacturedL2Error");
+      std::printf("Manufactured solution L2 error = %.6e\n", l2);
+      if (l2 > a.mms_tolerance)
+        throw FatalError(AFEM_ERR_ARGUMENT, "Manufactured solution L2 error is greater than tolerance");
     }
   }
   catch (const FatalError& e) {

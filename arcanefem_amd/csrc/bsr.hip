@@ -2677,6 +2677,36 @@ AFEM_API int afem_rhs_constant_source(afem_mesh_t mesh, double f, const double* 
   AFEM_GUARD_END
 }
 
+AFEM_API int afem_rhs_manufactured(afem_mesh_t mesh, int solution, int nb_dof, int dof_index, double* rhs)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && rhs, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(nb_dof >= 1 && dof_index >= 0 && dof_index < nb_dof, AFEM_ERR_ARGUMENT, "bad dof index");
+  AFEM_REQUIRE(solution >= 0 && solution <= 2, AFEM_ERR_ARGUMENT, "unknown manufactured solution");
+  AFEM_REQUIRE(!mesh->simplex || (mesh->dim == 2 && mesh->npc == 3), AFEM_ERR_NOT_IMPLEMENTED,
+               "Manufactured source is not implemented for tetrahedra");
+  rhs_manufactured(*mesh, solution, nb_dof, dof_index, rhs);
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_manufactured_l2_error(afem_mesh_t mesh, int solution, const double* u, double* l2_error)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(mesh && u && l2_error, AFEM_ERR_ARGUMENT, "null argument");
+  AFEM_REQUIRE(solution >= 0 && solution <= 2, AFEM_ERR_ARGUMENT, "unknown manufactured solution");
+  AFEM_REQUIRE(!mesh->simplex || (mesh->dim == 2 && mesh->npc == 3), AFEM_ERR_NOT_IMPLEMENTED,
+               "Manufactured L2 error is not implemented for tetrahedra");
+  DBuf<double> err2((size_t)mesh->nb_cell);
+  manufactured_cell_error(*mesh, solution, u, err2.p);
+  std::vector<double> h((size_t)mesh->nb_cell);
+  err2.download(h.data(), h.size());
+  double sum = 0.0; // in cell order, as the reference's ENUMERATE_ (Cell, ...) loop
+  for (double v : h)
+    sum += v;
+  *l2_error = std::sqrt(sum);
+  AFEM_GUARD_END
+}
+
 AFEM_API int afem_bsr_add_convection(afem_bsr_t bsr, int nb_face, const int32_t* face_nodes, double h)
 {
   AFEM_GUARD_BEGIN

@@ -287,6 +287,10 @@ void q1_assemble_laplace(afem_bsr* b, const double* cell_lambda, double f, doubl
 void q1_assemble_heat(afem_bsr* b, const double* cell_lambda, double dt);
 void q1_assemble_elasticity(afem_bsr* b, double lam, double mu);
 void q1_rhs_mass_field(const afem_mesh& m, const double* field, double scale, int nb_dof, int dof_index, double* rhs);
+// manufactured-solution source on tri3 / Quad4 / Hexa8 meshes (q1.hip)
+void rhs_manufactured(const afem_mesh& m, int which, int nb_dof, int dof_index, double* rhs);
+// per-cell squared L2 error of a nodal solution against a manufactured solution
+void manufactured_cell_error(const afem_mesh& m, int which, const double* u, double* err2);
 void q1_rhs_constant_source(const afem_mesh& m, double f, const double* cell_scale, int nb_dof, int dof_index,
                             double* rhs);
 

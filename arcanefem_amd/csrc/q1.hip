@@ -652,6 +652,183 @@ void q1_assemble_elasticity(afem_bsr* b, double lam, double mu)
   AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix(findValueIndex): Value not found");
 }
 
+// Manufactured sources (modules/poisson/ManufacturedSolutions.h:91-142):
+// 0 sine 2 pi^2 sin(pi x) sin(pi y), 1 poly 2 (x(1-x) + y(1-y)), 2 quadratic -4.
+__device__ __forceinline__ double manufactured_source(int which, double x, double y)
+{
+  constexpr double Pi = 3.141592653589793238462643383279502884;
+  if (which == 0)
+    return 2.0 * Pi * Pi * (sin(Pi * x) * sin(Pi * y));
+  if (which == 1)
+    return 2.0 * (x * (1.0 - x) + y * (1.0 - y));
+  return -4.0;
+}
+
+// rhs[i*nb_dof+dof_index] = sum over the cells of node i of the manufactured
+// source against N_i: tri3 source(barycentre) |T| / 3, Quad4 / Hexa8
+// sum_g N_i(x_g) source(x_g) det J_g with x_g = sum_a N_a x_a
+// (_applyManufacturedSource, modules/poisson/ManufacturedSolutions.cc:66-236).
+// DIM = 0 selects tri3.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_rhs_manufactured(const double* __restrict__ coords, const int32_t* __restrict__ cells, int slot_bits,
+                   const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items, int32_t nb_node,
+                   int which, int nb_dof, int dof_index, double* __restrict__ rhs)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  double acc = 0.0;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int32_t it = nc_items[e];
+    const int s = it & ((1 << slot_bits) - 1);
+    const int32_t* cn = cells + (it - s);
+    if constexpr (DIM == 0) {
+      double x[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          x[a][d] = coords[4 * (int64_t)cn[a] + d];
+      const double e1[3] = { x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2] };
+      const double e2[3] = { x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2] };
+      const double v0 = e1[1] * e2[2] - e1[2] * e2[1], v1 = e1[2] * e2[0] - e1[0] * e2[2],
+                   v2 = e1[0] * e2[1] - e1[1] * e2[0];
+      const double area = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / 2.0;
+      const double bx = (x[0][0] + x[1][0] + x[2][0]) / 3.0, by = (x[0][1] + x[1][1] + x[2][1]) / 3.0;
+      acc += manufactured_source(which, bx, by) * area / 3;
+    }
+    else {
+      constexpr int N = Q1Point<DIM>::N;
+      constexpr int NG = DIM == 2 ? 4 : 8;
+      double x[N][3];
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        const int32_t n = cn[a];
+        x[a][0] = coords[4 * (int64_t)n];
+        x[a][1] = coords[4 * (int64_t)n + 1];
+        x[a][2] = coords[4 * (int64_t)n + 2];
+      }
+      for (int g = 0; g < NG; ++g) {
+        Q1Point<DIM> q;
+        q.eval(x, g);
+        double xg = 0.0, yg = 0.0, ns = 0.0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+          xg += q.Nv[a] * x[a][0];
+          yg += q.Nv[a] * x[a][1];
+          if (a == s)
+            ns = q.Nv[a];
+        }
+        acc += ns * manufactured_source(which, xg, yg) * q.det;
+      }
+    }
+  }
+  rhs[(int64_t)i * nb_dof + dof_index] = acc;
+}
+
+__device__ __forceinline__ double manufactured_exact(int which, double x, double y)
+{
+  constexpr double Pi = 3.141592653589793238462643383279502884;
+  if (which == 0)
+    return sin(Pi * x) * sin(Pi * y);
+  if (which == 1)
+    return x * (1.0 - x) * y * (1.0 - y);
+  return x * x + y * y;
+}
+
+// err2[c] = the squared L2 error of the nodal solution u on cell c: tri3
+// (mean of u - exact(barycentre))^2 |T|, Quad4 / Hexa8 sum_g (u_h(x_g) -
+// exact(x_g))^2 det J_g (_computeManufacturedL2Error,
+// modules/poisson/ManufacturedSolutions.cc:270-420).  DIM = 0 selects tri3.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+k_manufactured_cell_error(const double* __restrict__ coords, const int32_t* __restrict__ cells, int stride,
+                          int32_t nb_cell, int which, const double* __restrict__ u, double* __restrict__ err2)
+{
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb_cell)
+    return;
+  const int32_t* cn = cells + (int64_t)stride * c;
+  double e2 = 0.0;
+  if constexpr (DIM == 0) {
+    double x[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        x[a][d] = coords[4 * (int64_t)cn[a] + d];
+    const double e1[3] = { x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2] };
+    const double f1[3] = { x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2] };
+    const double v0 = e1[1] * f1[2] - e1[2] * f1[1], v1 = e1[2] * f1[0] - e1[0] * f1[2],
+                 v2 = e1[0] * f1[1] - e1[1] * f1[0];
+    const double area = sqrt(v0 * v0 + v1 * v1 + v2 * v2) / 2.0;
+    const double bx = (x[0][0] + x[1][0] + x[2][0]) / 3.0, by = (x[0][1] + x[1][1] + x[2][1]) / 3.0;
+    const double err = (u[cn[0]] + u[cn[1]] + u[cn[2]]) / 3.0 - manufactured_exact(which, bx, by);
+    e2 = err * err * area;
+  }
+  else {
+    constexpr int N = Q1Point<DIM>::N;
+    constexpr int NG = DIM == 2 ? 4 : 8;
+    double x[N][3], un[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      const int32_t n = cn[a];
+      x[a][0] = coords[4 * (int64_t)n];
+      x[a][1] = coords[4 * (int64_t)n + 1];
+      x[a][2] = coords[4 * (int64_t)n + 2];
+      un[a] = u[n];
+    }
+    for (int g = 0; g < NG; ++g) {
+      Q1Point<DIM> q;
+      q.eval(x, g);
+      double xg = 0.0, yg = 0.0, uh = 0.0;
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        uh += q.Nv[a] * un[a];
+        xg += q.Nv[a] * x[a][0];
+        yg += q.Nv[a] * x[a][1];
+      }
+      const double err = uh - manufactured_exact(which, xg, yg);
+      e2 += err * err * q.det;
+    }
+  }
+  err2[c] = e2;
+}
+
+void manufactured_cell_error(const afem_mesh& m, int which, const double* u, double* err2)
+{
+  ProfScope prof("manufactured_l2_error");
+  const dim3 grid(grid_for(m.nb_cell, 256)), block(256);
+  const int stride = 1 << m.slot_bits;
+  if (m.simplex)
+    hipLaunchKernelGGL(k_manufactured_cell_error<0>, grid, block, 0, stream(), m.coords.p, m.cells.p, stride,
+                       m.nb_cell, which, u, err2);
+  else if (m.dim == 2)
+    hipLaunchKernelGGL(k_manufactured_cell_error<2>, grid, block, 0, stream(), m.coords.p, m.cells.p, stride,
+                       m.nb_cell, which, u, err2);
+  else
+    hipLaunchKernelGGL(k_manufactured_cell_error<3>, grid, block, 0, stream(), m.coords.p, m.cells.p, stride,
+                       m.nb_cell, which, u, err2);
+  check_launch("manufactured_cell_error");
+}
+
+void rhs_manufactured(const afem_mesh& m, int which, int nb_dof, int dof_index, double* rhs)
+{
+  ProfScope prof("rhs_manufactured");
+  const dim3 grid(grid_for(m.nb_node, 256)), block(256);
+  if (m.simplex)
+    hipLaunchKernelGGL(k_rhs_manufactured<0>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, which, nb_dof, dof_index, rhs);
+  else if (m.dim == 2)
+    hipLaunchKernelGGL(k_rhs_manufactured<2>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, which, nb_dof, dof_index, rhs);
+  else
+    hipLaunchKernelGGL(k_rhs_manufactured<3>, grid, block, 0, stream(), m.coords.p, m.cells.p, m.slot_bits,
+                       m.nc_offsets.p, m.nc_items.p, m.nb_node, which, nb_dof, dof_index, rhs);
+  check_launch("rhs_manufactured");
+}
+
 void q1_rhs_mass_field(const afem_mesh& m, const double* field, double scale, int nb_dof, int dof_index, double* rhs)
 {
   ProfScope prof("rhs_mass_field_q1");

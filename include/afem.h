@@ -197,6 +197,23 @@ AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int3
 
 /* ---- right-hand side ---------------------------------------------------------- */
 
+/* FemModulePoisson::_applyManufacturedSource
+ * (modules/poisson/ManufacturedSolutions.cc:66-236): rhs[n*nb_dof+dof_index]
+ * = the manufactured source integrated against N_n, written (not added).
+ * solution: 0 sine (2 pi^2 sin(pi x) sin(pi y)), 1 poly
+ * (2 (x(1-x) + y(1-y))), 2 quadratic (-4), ManufacturedSolutions.h:91-142.
+ * Triangles: source(barycentre) |T| / 3; Quad4 / Hexa8: 2x2(x2) Gauss.
+ * Tetrahedra: AFEM_ERR_NOT_IMPLEMENTED (the reference has no branch). */
+AFEM_API int afem_rhs_manufactured(afem_mesh_t mesh, int solution, int nb_dof, int dof_index, double* rhs);
+
+/* FemModulePoisson::_computeManufacturedL2Error
+ * (modules/poisson/ManufacturedSolutions.cc:270-420): *l2_error [host] =
+ * sqrt(sum over cells of the squared error of the nodal solution u [device,
+ * one value per node]): triangles at the barycentre (mean of u) times |T|,
+ * Quad4 / Hexa8 at the 2x2(x2) Gauss points times det J.  The per-cell terms
+ * are summed on the host in cell order. */
+AFEM_API int afem_manufactured_l2_error(afem_mesh_t mesh, int solution, const double* u, double* l2_error);
+
 /* BoundaryConditions::applyConstantSourceToRhs, node-wise gather
  * (femutils/ArcaneFemFunctionsGpu.h:675-708): rhs[n*nb_dof+dof_index] =
  * sum over cells T of n of f*scale_T*|T|/npc.  cell_scale [device] or NULL. */

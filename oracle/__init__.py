@@ -77,6 +77,12 @@ def lib():
         L.afo_rhs_neumann_q1.restype = ctypes.c_int
         L.afo_rhs_neumann_q1.argtypes = [ctypes.c_int, vp, i32, i32, vp, i32, vp, ctypes.c_int, vp, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, vp]
+        L.afo_manufactured_exact.restype = dbl
+        L.afo_manufactured_exact.argtypes = [ctypes.c_int, vp]
+        L.afo_rhs_manufactured.restype = ctypes.c_int
+        L.afo_rhs_manufactured.argtypes = [ctypes.c_int, vp, i32, i32, vp, ctypes.c_int, vp]
+        L.afo_manufactured_l2_error.restype = dbl
+        L.afo_manufactured_l2_error.argtypes = [ctypes.c_int, vp, i32, i32, vp, ctypes.c_int, vp]
         L.afo_convection_lhs.restype = ctypes.c_int
         L.afo_convection_lhs.argtypes = [ctypes.c_int, vp, i32, vp, dbl, vp, vp, vp]
         L.afo_apply_transformations.restype = ctypes.c_int
@@ -303,6 +309,29 @@ def elem_heat_q1(dim, X, lam, dt):
     K = np.zeros(n * n)
     lib().afo_elem_heat_q1(dim, _P(X), lam, dt, _P(K))
     return K.reshape(n, n)
+
+
+MANUFACTURED = {"sine": 0, "poly": 1, "quadratic": 2}
+
+
+def manufactured_exact(name, X):
+    X = _f64(X)
+    return np.array([lib().afo_manufactured_exact(MANUFACTURED[name], _P(np.ascontiguousarray(x))) for x in X])
+
+
+def rhs_manufactured(dim, coords, cells, npc, name, rhs=None):
+    coords, cells = _f64(coords), _i32(cells)
+    if rhs is None:
+        rhs = np.zeros(coords.shape[0])
+    if lib().afo_rhs_manufactured(dim, _P(coords), cells.shape[0], npc, _P(cells), MANUFACTURED[name], _P(rhs)):
+        raise ValueError("manufactured solutions cover tri3, Quad4 and Hexa8 cells")
+    return rhs
+
+
+def manufactured_l2_error(dim, coords, cells, npc, name, u):
+    coords, cells, u = _f64(coords), _i32(cells), _f64(u)
+    return float(lib().afo_manufactured_l2_error(dim, _P(coords), cells.shape[0], npc, _P(cells), MANUFACTURED[name],
+                                                 _P(u)))
 
 
 def max_threads():

@@ -1019,6 +1019,116 @@ int afo_rhs_neumann_q1(int dim, const double* coords, int32_t nb_cell, int32_t n
   return 0;
 }
 
+/* Manufactured solutions of modules/poisson/ManufacturedSolutions.h:91-142:
+ * which = 0 sine (sin(pi x) sin(pi y), source 2 pi^2 u), 1 poly
+ * (x(1-x) y(1-y), source 2(x(1-x) + y(1-y))), 2 quadratic (x^2 + y^2,
+ * source -4). */
+double afo_manufactured_exact(int which, const double* x)
+{
+  const double Pi = 3.141592653589793238462643383279502884;
+  if (which == 0)
+    return sin(Pi * x[0]) * sin(Pi * x[1]);
+  if (which == 1)
+    return x[0] * (1.0 - x[0]) * x[1] * (1.0 - x[1]);
+  return x[0] * x[0] + x[1] * x[1];
+}
+
+double afo_manufactured_source(int which, const double* x)
+{
+  const double Pi = 3.141592653589793238462643383279502884;
+  if (which == 0)
+    return 2.0 * Pi * Pi * afo_manufactured_exact(0, x);
+  if (which == 1)
+    return 2.0 * (x[0] * (1.0 - x[0]) + x[1] * (1.0 - x[1]));
+  return -4.0;
+}
+
+/* modules/poisson/ManufacturedSolutions.cc:66-240 (_applyManufacturedSource):
+ * tri3 source(barycentre) |T| / 3 per node; Quad4 / Hexa8 sum over the Gauss
+ * points of N_i source(x_g) det J (x_g = sum N_a x_a).  Returns -1 for other
+ * cells (the reference has no branch for them). */
+int afo_rhs_manufactured(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                         int which, double* rhs)
+{
+  const int st = AFO_CELL_STRIDE(npc);
+  if (!(dim == 2 && npc == 3) && !AFO_IS_Q1(dim, npc))
+    return -1;
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + st * c;
+    double X[24];
+    for (int k = 0; k < npc; ++k)
+      memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+    if (npc == 3) {
+      const double area = afo_area_tria3(X, X + 3, X + 6);
+      double xb[3];
+      for (int d = 0; d < 3; ++d)
+        xb[d] = (X[d] + X[3 + d] + X[6 + d]) / 3.0;
+      const double src = afo_manufactured_source(which, xb);
+      for (int k = 0; k < 3; ++k)
+        rhs[cn[k]] += src * area / 3;
+      continue;
+    }
+    const int ng = dim == 2 ? 1 : 2;
+    for (int ix = 0; ix < 2; ++ix)
+      for (int iy = 0; iy < 2; ++iy)
+        for (int iz = 0; iz < ng; ++iz) {
+          double N[8], gx[8], gy[8], gz[8];
+          const double det = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+          double xg[3] = { 0, 0, 0 };
+          for (int a = 0; a < npc; ++a)
+            for (int d = 0; d < 3; ++d)
+              xg[d] += N[a] * X[3 * a + d];
+          const double src = afo_manufactured_source(which, xg);
+          for (int a = 0; a < npc; ++a)
+            rhs[cn[a]] += N[a] * src * (det * 1.0);
+        }
+  }
+  return 0;
+}
+
+/* _computeManufacturedL2Error (ManufacturedSolutions.cc:270-420): tri3 at the
+ * barycentre (uh the mean of the nodal values) times |T|; Quad4 / Hexa8 at the
+ * Gauss points; returns sqrt of the sum. */
+double afo_manufactured_l2_error(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                                 int which, const double* u)
+{
+  const int st = AFO_CELL_STRIDE(npc);
+  const double gp[2] = { -AFO_GP, AFO_GP };
+  double e2 = 0.0;
+  for (int64_t c = 0; c < nb_cell; ++c) {
+    const int32_t* cn = cells + st * c;
+    double X[24];
+    for (int k = 0; k < npc; ++k)
+      memcpy(X + 3 * k, coords + 3 * (int64_t)cn[k], 3 * sizeof(double));
+    if (npc == 3) {
+      const double area = afo_area_tria3(X, X + 3, X + 6);
+      double xb[3];
+      for (int d = 0; d < 3; ++d)
+        xb[d] = (X[d] + X[3 + d] + X[6 + d]) / 3.0;
+      const double err = (u[cn[0]] + u[cn[1]] + u[cn[2]]) / 3.0 - afo_manufactured_exact(which, xb);
+      e2 += err * err * area;
+      continue;
+    }
+    const int ng = dim == 2 ? 1 : 2;
+    for (int ix = 0; ix < 2; ++ix)
+      for (int iy = 0; iy < 2; ++iy)
+        for (int iz = 0; iz < ng; ++iz) {
+          double N[8], gx[8], gy[8], gz[8];
+          const double det = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+          double xg[3] = { 0, 0, 0 }, uh = 0.0;
+          for (int a = 0; a < npc; ++a) {
+            uh += N[a] * u[cn[a]];
+            for (int d = 0; d < 3; ++d)
+              xg[d] += N[a] * X[3 * a + d];
+          }
+          const double err = uh - afo_manufactured_exact(which, xg);
+          e2 += err * err * det * 1.0;
+        }
+  }
+  return sqrt(e2);
+}
+
 /* ---- transformations ------------------------------------------------------------ */
 
 int afo_apply_transformations(int32_t nb_row, const int32_t* row_index, const int32_t* columns,

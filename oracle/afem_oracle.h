@@ -174,6 +174,17 @@ void afo_elem_heat_q1(int dim, const double* X, double lambda, double dt, double
 int afo_convection_lhs(int k, const double* coords, int32_t nb_face, const int32_t* faces, double h,
                        const int32_t* row_index, const int32_t* columns, double* values);
 
+/* Manufactured solutions (modules/poisson/ManufacturedSolutions.h:91-142,
+ * ManufacturedSolutions.cc:66-420): which = 0 sine, 1 poly, 2 quadratic; the
+ * exact solution, the source, the RHS on tri3 / Quad4 / Hexa8 meshes (-1 on
+ * other cells) and the L2 error of a nodal solution. */
+double afo_manufactured_exact(int which, const double* x);
+double afo_manufactured_source(int which, const double* x);
+int afo_rhs_manufactured(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                         int which, double* rhs);
+double afo_manufactured_l2_error(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
+                                 int which, const double* u);
+
 /* Neumann condition on the faces of a Quad4 mesh (edges) or a Hexa8 mesh
  * (quadrangles, 4 nodes per face) with Gauss points on the face:
  * femutils/ArcaneFemFunctionsGpu.cc:741-850 (edges, outward normal) and

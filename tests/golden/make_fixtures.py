@@ -350,6 +350,10 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     print(f"{name}: N={len(tags)} cells={cells4.shape[0]} npc={npc} groups={sorted(groups)}")
 
 
+# meshes used as they are (manufactured-solution tests: no golden file)
+MESH_ONLY = ["meshes/msh/mms.quad4_n4.msh"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
@@ -357,6 +361,8 @@ def main():
     args = ap.parse_args()
     for name, (mesh_rel, check_rel, ncomp) in CASES.items():
         build_case(args.reference, name, mesh_rel, check_rel, ncomp, args.out)
+    for rel in MESH_ONLY:
+        shutil.copyfile(os.path.join(args.reference, rel), os.path.join(args.out, os.path.basename(rel)))
 
 
 if __name__ == "__main__":

@@ -43,6 +43,9 @@ def test_binary_built():
     (["poisson", "--mesh", "x.msh", "--dirichlet", "nogroupvalue"], "--dirichlet expects"),
     (["heat", "--mesh", "x.msh", "--convection", "top=1.0"], "--convection expects"),
     (["poisson", "--box", "2", "four", "0", "0"], "malformed number"),
+    (["fourier", "--box", "2", "4", "0", "0", "--manufactured", "sine"], "--manufactured is available for poisson"),
+    (["poisson", "--box", "2", "4", "0", "0", "--manufactured", "cosine"], "--manufactured expects"),
+    (["poisson", "--box", "2", "4", "0", "0", "--manufactured", "sine", "--f", "1"], "sets its own source"),
 ])
 def test_usage_errors(args, msg):
     r = run(args)
@@ -409,3 +412,46 @@ def test_module_row_column_matches_row_elimination(physics, dirichlet):
     assert a.keys() == b.keys() and len(a) == 11 ** 3
     scale = max(abs(v) for vs in a.values() for v in vs)
     assert max(abs(x - y) for k in a for x, y in zip(a[k], b[k])) <= 1e-9 * scale
+
+
+# modules/poisson/CMakeLists.txt 2D_mms_quad4 and 3D_mms_hexa8 with
+# inputs/manufactured.solution.arc (tolerance 8e-2), plus the default
+# triangle mesh case on boxes: the L2 error falls by about 4 when h halves.
+MMS = [("mms.quad4_n4.msh", "sine"), ("3x3x3_cube_hexa8.msh", "quadratic")]
+
+
+def _l2(r):
+    _check_ok_rc(r)
+    line = [x for x in r.stdout.splitlines() if "Manufactured solution L2 error" in x]
+    assert line, r.stdout
+    return float(line[-1].split("=")[-1])
+
+
+def _check_ok_rc(r):
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("msh,name", MMS)
+def test_poisson_module_manufactured(msh, name):
+    print(f"[case] mms {msh} {name}", flush=True)
+    e = _l2(run(["poisson", "--mesh", os.path.join(GOLDEN, msh), "--manufactured", name,
+                 "--mms-tolerance", "8e-2"]))
+    assert e < 8e-2
+
+
+@pytest.mark.gpu
+def test_poisson_module_manufactured_refinement():
+    print("[case] mms refinement tri boxes", flush=True)
+    errs = []
+    for n in (8, 16):
+        args = ["poisson", "--box", "2", str(n), "0", "0", "--manufactured", "sine", "--mms-tolerance", "8e-2"]
+        errs.append(_l2(run(args)))
+    assert 3.2 < errs[0] / errs[1] < 4.8, errs
+
+
+@pytest.mark.gpu
+def test_poisson_module_manufactured_tolerance_failure():
+    r = run(["poisson", "--mesh", os.path.join(GOLDEN, "mms.quad4_n4.msh"), "--manufactured", "sine",
+             "--mms-tolerance", "1e-9"])
+    assert r.returncode == 1 and "greater than tolerance" in r.stderr

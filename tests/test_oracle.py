@@ -1174,3 +1174,104 @@ def test_golden_elasticity_q1(case, E, nu, f, tractions, dirichlet):
     ref = d["ref_value" + sfx][:, :dim]
     gv = np.array([got[u] for u in d["ref_uid" + sfx].tolist()])
     assert O.count_differences(ref.ravel(), gv.ravel(), 1e-3, 1e-10) == 0
+
+
+# ---- manufactured solutions (modules/poisson/ManufacturedSolutions.{h,cc}) -------------
+
+def _box_q1(dim, n):
+    import arcanefem_amd as A
+    return A.box_mesh_q1(dim, n)
+
+
+def _box_tri(n, jitter=0.0):
+    import arcanefem_amd as A
+    return A.box_mesh(2, n, jitter, seed=3)
+
+
+def test_manufactured_exact_values():
+    X = np.array([[0.5, 0.5, 0.0], [0.25, 0.5, 0.0], [1.0, 2.0, 0.0], [0.0, 0.3, 0.0]])
+    assert np.allclose(O.manufactured_exact("sine", X), [1.0, np.sqrt(0.5), 0.0, 0.0], atol=1e-15)
+    assert np.allclose(O.manufactured_exact("poly", X), [1 / 16, 0.25 * 0.75 * 0.25, 1.0 * 0 * 2 * -1, 0.0])
+    assert np.allclose(O.manufactured_exact("quadratic", X), [0.5, 0.3125, 5.0, 0.09])
+
+
+def test_manufactured_rhs_constant_source_case():
+    # the quadratic case has the constant source -4: the RHS is the constant
+    # source RHS of the Poisson module (a different oracle routine)
+    for dim, (X, C), npc in [(2, _box_tri(7, 0.2), 3), (2, _box_q1(2, 6), 4), (3, _box_q1(3, 4), 8)]:
+        got = O.rhs_manufactured(dim, X, C, npc, "quadratic")
+        ref = O.rhs_constant_source(dim, X, C, npc, -4.0)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_manufactured_rhs_integrates_the_source():
+    # sum_i rhs_i = quadrature of the source (partition of unity).  On an
+    # axis-aligned Q1 box the 2-point Gauss rule is exact for the poly source
+    # (quadratic per direction): int 2(x(1-x) + y(1-y)) = 2/3 on the unit
+    # square and cube.  The sine source integrates to 2 pi^2 (2/pi)^2 = 8: the
+    # 2x2 Gauss rule errs by O(h^4), the barycentre rule of triangles by O(h^2).
+    for dim, n in [(2, 5), (3, 3)]:
+        X, C = _box_q1(dim, n)
+        assert abs(O.rhs_manufactured(dim, X, C, C.shape[1], "poly").sum() - 2 / 3) < 1e-13
+    for mk, npc, ratio in [(lambda n: _box_q1(2, n), 4, 16.0), (lambda n: _box_tri(n), 3, 4.0)]:
+        errs = []
+        for n in (8, 16):
+            X, C = mk(n)
+            errs.append(abs(O.rhs_manufactured(2, X, C, npc, "sine").sum() - 8.0))
+        assert errs[1] < 0.05 and 0.85 * ratio < errs[0] / errs[1] < 1.15 * ratio, errs
+
+
+def test_manufactured_l2_error_closed_forms():
+    # nodal values of x^2 + y^2 on a uniform Q1 grid: the bilinear interpolant
+    # exceeds x^2 by (x - x0)(x1 - x) = h^2 t (1 - t) = h^2 / 6 at both Gauss
+    # abscissae t = (1 -+ 1/sqrt 3) / 2, so the error is h^2 / 3 everywhere
+    # and the L2 error over the unit square / cube is h^2 / 3
+    for dim, n in [(2, 6), (3, 3)]:
+        X, C = _box_q1(dim, n)
+        u = X[:, 0] ** 2 + X[:, 1] ** 2
+        assert abs(O.manufactured_l2_error(dim, X, C, C.shape[1], "quadratic", u) - (1 / n) ** 2 / 3) < 1e-14
+    # triangles: mean of x_i^2 minus (mean x_i)^2 is the variance of the
+    # vertex abscissae
+    X, C = _box_tri(5, 0.25)
+    C3 = C[:, :3]
+    P = X[C3][:, :, :2]
+    var = P.var(axis=1).sum(axis=1)
+    e = P[:, 1] - P[:, 0]
+    f = P[:, 2] - P[:, 0]
+    area = 0.5 * np.abs(e[:, 0] * f[:, 1] - e[:, 1] * f[:, 0])
+    u = X[:, 0] ** 2 + X[:, 1] ** 2
+    ref = np.sqrt((var ** 2 * area).sum())
+    assert abs(O.manufactured_l2_error(2, X, C, 3, "quadratic", u) - ref) < 1e-14 * max(ref, 1.0)
+
+
+def _mms_solve(dim, X, C, npc, name):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    N = X.shape[0]
+    row, col = O.pattern(N, C, npc)
+    K = sp.csr_matrix((O.assemble(dim, X, C, npc, row, col), col, row), shape=(N, N))
+    b = O.rhs_manufactured(dim, X, C, npc, name)
+    bnd = np.any(np.isclose(X[:, :dim], 0.0) | np.isclose(X[:, :dim], 1.0), axis=1)
+    u = np.zeros(N)
+    u[bnd] = O.manufactured_exact(name, X[bnd])
+    inner = ~bnd
+    rhs = b[inner] - K[inner][:, bnd] @ u[bnd]
+    u[inner] = spl.spsolve(K[inner][:, inner].tocsc(), rhs)
+    return O.manufactured_l2_error(dim, X, C, npc, name, u)
+
+
+def test_manufactured_solve_second_order():
+    # the FE solution converges at O(h^2) in L2 and passes the reference
+    # test's tolerance 8e-2 (modules/poisson/inputs/manufactured.solution.arc)
+    for dim, mk, npc in [(2, lambda n: _box_q1(2, n), 4), (2, lambda n: _box_tri(n), 3)]:
+        e = [_mms_solve(dim, *mk(n), npc, "sine") for n in (8, 16)]
+        assert e[0] < 8e-2 and 3.3 < e[0] / e[1] < 4.7, e
+    e = [_mms_solve(3, *_box_q1(3, n), 8, "quadratic") for n in (3, 6)]
+    assert e[0] < 8e-2 and 3.3 < e[0] / e[1] < 4.7, e
+
+
+def test_manufactured_rejects_tetrahedra():
+    import arcanefem_amd as A
+    X, C = A.box_mesh(3, 2)
+    with pytest.raises(ValueError):
+        O.rhs_manufactured(3, X, C, 4, "sine")
