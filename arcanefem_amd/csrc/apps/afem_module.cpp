@@ -37,6 +37,8 @@ struct Args
 {
   std::string physics, mesh, check, method = "Penalty", format = "BSR", precision = "fp64";
   int box_dim = 0, box_n = 0;
+  std::vector<int> cart_n;      // --cartesian nx,ny[,nz] lx,ly[,lz]
+  std::vector<double> cart_len;
   double box_jitter = 0.0;
   unsigned long long box_seed = 0;
   std::vector<double> f;
@@ -96,6 +98,18 @@ Args parse(int argc, char** argv)
       a.box_n = std::stoi(next());
       a.box_jitter = std::stod(next());
       a.box_seed = std::stoull(next());
+    }
+    else if (key == "--cartesian") {
+      // Arcane's Cartesian2D / Cartesian3D generator: cells and lengths per axis
+      for (auto& t : split(next(), ','))
+        a.cart_n.push_back(std::stoi(t));
+      for (auto& t : split(next(), ','))
+        a.cart_len.push_back(std::stod(t));
+      if ((a.cart_n.size() != 2 && a.cart_n.size() != 3) || a.cart_len.size() != a.cart_n.size())
+        usage("--cartesian expects nx,ny[,nz] lx,ly[,lz]");
+      for (size_t d = 0; d < a.cart_n.size(); ++d)
+        if (a.cart_n[d] < 1 || !(a.cart_len[d] > 0.0))
+          usage("--cartesian expects positive cell counts and lengths");
     }
     else if (key == "--f") {
       for (auto& s : split(next(), ','))
@@ -188,8 +202,8 @@ Args parse(int argc, char** argv)
     else
       usage("unknown option " + key);
   }
-  if (a.mesh.empty() && a.box_dim == 0)
-    usage("give --mesh or --box");
+  if (a.mesh.empty() && a.box_dim == 0 && a.cart_n.empty())
+    usage("give --mesh, --box or --cartesian");
   if (a.physics != "fourier" && !a.materials.empty())
     usage("--material is available for fourier");
   if (a.physics != "elasticity" && !a.traction.empty())
@@ -403,6 +417,8 @@ int main(int argc, char** argv)
     Mesh mesh;
     if (!a.mesh.empty())
       mesh.readGmsh41(a.mesh);
+    else if (!a.cart_n.empty())
+      mesh.buildCartesian((int)a.cart_n.size(), a.cart_n.data(), a.cart_len.data());
     else
       mesh.buildBox(a.box_dim, a.box_n, a.box_jitter, a.box_seed);
     if (a.physics == "heat")

@@ -91,6 +91,57 @@ class Mesh
     build(dim, std::move(x), std::move(c), std::move(uid));
   }
 
+  // Arcane's Cartesian2D / Cartesian3D generator (the meshes of
+  // modules/elasticity/inputs/bar.{2D,3D}.cartesian.Dirichlet.bodyForce.arc):
+  // n[d] cells of length len[d] / n[d] per axis from the origin, node unique
+  // ids from 0 with x fastest, then y, then z; Quad4 cells counter-clockwise
+  // from the lower-left corner, Hexa8 cells the bottom face then the top face;
+  // node groups XMIN, XMAX, YMIN, YMAX (, ZMIN, ZMAX) of the boundary planes
+  // (generate-sod-groups).
+  void buildCartesian(int dim, const int* n, const double* len)
+  {
+    const int nz = dim == 3 ? n[2] : 0;
+    const Int64 sx = n[0] + 1, sy = n[1] + 1, sz = nz + 1;
+    const Int64 nn = sx * sy * sz;
+    std::vector<Real> x((size_t)nn * 3);
+    std::vector<Int64> uid((size_t)nn);
+    for (Int64 k = 0; k < sz; ++k)
+      for (Int64 j = 0; j < sy; ++j)
+        for (Int64 i = 0; i < sx; ++i) {
+          const Int64 v = i + sx * (j + sy * k);
+          x[3 * v] = len[0] * (double)i / n[0];
+          x[3 * v + 1] = len[1] * (double)j / n[1];
+          x[3 * v + 2] = dim == 3 ? len[2] * (double)k / nz : 0.0;
+          uid[(size_t)v] = v;
+        }
+    const int npc = dim == 3 ? 8 : 4;
+    std::vector<Int32> c;
+    for (Int64 k = 0; k < (dim == 3 ? nz : 1); ++k)
+      for (Int64 j = 0; j < n[1]; ++j)
+        for (Int64 i = 0; i < n[0]; ++i) {
+          const Int32 v = (Int32)(i + sx * (j + sy * k));
+          const Int32 quad[4] = { v, v + 1, (Int32)(v + sx + 1), (Int32)(v + sx) };
+          for (int a = 0; a < 4; ++a)
+            c.push_back(quad[a]);
+          if (dim == 3)
+            for (int a = 0; a < 4; ++a)
+              c.push_back((Int32)(quad[a] + sx * sy));
+        }
+    m_groups.clear();
+    const char* names[3][2] = { { "XMIN", "XMAX" }, { "YMIN", "YMAX" }, { "ZMIN", "ZMAX" } };
+    for (Int64 v = 0; v < nn; ++v) {
+      const Int64 idx[3] = { v % sx, (v / sx) % sy, v / (sx * sy) };
+      const Int64 hi[3] = { n[0], n[1], nz };
+      for (int d = 0; d < dim; ++d) {
+        if (idx[d] == 0)
+          m_groups[names[d][0]].push_back((Int32)v);
+        if (idx[d] == hi[d])
+          m_groups[names[d][1]].push_back((Int32)v);
+      }
+    }
+    build(dim, std::move(x), std::move(c), std::move(uid), true, npc);
+  }
+
   // Binary Gmsh 4.1 (size_t = 8): nodes, the cells of the highest dimension
   // (tri3 or quad4 in 2D, tet4 or hexa8 in 3D, one kind per mesh), the node
   // and face groups of the (dim-1) physical names and the node groups of the

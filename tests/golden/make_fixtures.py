@@ -350,8 +350,11 @@ def build_case(ref_root, name, mesh_rel, check_rel, ncomp, out_dir):
     print(f"{name}: N={len(tags)} cells={cells4.shape[0]} npc={npc} groups={sorted(groups)}")
 
 
-# meshes used as they are (manufactured-solution tests: no golden file)
-MESH_ONLY = ["meshes/msh/mms.quad4_n4.msh"]
+# files used as they are: the manufactured-solution mesh (no golden file) and
+# the golden files of the cases on Arcane's generated Cartesian meshes
+MESH_ONLY = ["meshes/msh/mms.quad4_n4.msh",
+             "modules/elasticity/check/bar.2D.cartesian.Dirichlet.bodyForce.txt",
+             "modules/elasticity/check/bar.3D.cartesian.Dirichlet.bodyForce.txt"]
 
 
 def main():

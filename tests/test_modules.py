@@ -37,7 +37,9 @@ def test_binary_built():
 
 @pytest.mark.parametrize("args,msg", [
     ([], "expected"),
-    (["poisson"], "give --mesh or --box"),
+    (["poisson"], "give --mesh, --box or --cartesian"),
+    (["elasticity", "--cartesian", "10,2", "1.0"], "--cartesian expects nx,ny"),
+    (["elasticity", "--cartesian", "10,0", "1.0,0.1"], "positive cell counts"),
     (["poisson", "--mesh"], "missing value"),
     (["poisson", "--box", "3", "4", "0", "0", "--bogus"], "unknown option"),
     (["poisson", "--mesh", "x.msh", "--dirichlet", "nogroupvalue"], "--dirichlet expects"),
@@ -455,3 +457,26 @@ def test_poisson_module_manufactured_tolerance_failure():
     r = run(["poisson", "--mesh", os.path.join(GOLDEN, "mms.quad4_n4.msh"), "--manufactured", "sine",
              "--mms-tolerance", "1e-9"])
     assert r.returncode == 1 and "greater than tolerance" in r.stderr
+
+
+# modules/elasticity/CMakeLists.txt 2D_Dirichlet_cartesian / 3D_Dirichlet_cartesian
+# (inputs/bar.{2D,3D}.cartesian.Dirichlet.bodyForce.arc): Arcane's Cartesian
+# generator, 10 x 2 (x 2) cells of a 1.0 x 0.1 (x 0.04) bar, f = (-1, NULL),
+# XMIN clamped, XMAX u_y = 1.  Min value 1e-9 in 3D: three |u_z| <= 5.4e-10
+# values of that golden file are cancellation residue of the reference's
+# iterative solve (1e-12 from the direct solution in absolute terms).
+CARTESIAN = [
+    (["--cartesian", "10,2", "1.0,0.1", "--f", "-1.0,NULL", "--dirichlet", "XMIN=0.0,0.0",
+      "--dirichlet", "XMAX=NULL,1.0"], "bar.2D.cartesian.Dirichlet.bodyForce.txt", "1e-10"),
+    (["--cartesian", "10,2,2", "1.0,0.1,0.04", "--f", "-1.0,NULL,NULL", "--dirichlet", "XMIN=0.0,0.0,0.0",
+      "--dirichlet", "XMAX=NULL,1.0,NULL"], "bar.3D.cartesian.Dirichlet.bodyForce.txt", "1e-9"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args,ref,min_value", CARTESIAN, ids=["2D", "3D"])
+def test_elasticity_module_cartesian_golden(args, ref, min_value):
+    print(f"[case] cartesian {ref}", flush=True)
+    r = run(["elasticity", "--E", "21e5", "--nu", "0.28", "--rtol", "1e-12", "--check", os.path.join(GOLDEN, ref),
+             "--epsilon", "1e-3", "--min-value", min_value] + args)
+    _check_ok(r)

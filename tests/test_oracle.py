@@ -1176,6 +1176,55 @@ def test_golden_elasticity_q1(case, E, nu, f, tractions, dirichlet):
     assert O.count_differences(ref.ravel(), gv.ravel(), 1e-3, 1e-10) == 0
 
 
+def _cartesian(n, length):
+    """Arcane's Cartesian generator: node uid = i + sx (j + sy k), x fastest;
+    Quad4 counter-clockwise, Hexa8 bottom then top face."""
+    dim = len(n)
+    s = [k + 1 for k in n] + ([1] if dim == 2 else [])
+    k, j, i = np.meshgrid(np.arange(s[2]), np.arange(s[1]), np.arange(s[0]), indexing="ij")
+    X = np.stack([i.ravel() * length[0] / n[0], j.ravel() * length[1] / n[1],
+                  k.ravel() * (length[2] / n[2] if dim == 3 else 0.0)], axis=1)
+    ck, cj, ci = np.meshgrid(np.arange(n[2] if dim == 3 else 1), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    v = (ci + s[0] * (cj + s[1] * ck)).ravel()
+    q = [v, v + 1, v + s[0] + 1, v + s[0]]
+    C = np.stack(q + ([a + s[0] * s[1] for a in q] if dim == 3 else []), axis=1).astype(np.int32)
+    return X, C
+
+
+# min value 1e-10 as in the reference's check, except in 3D: three z values
+# of the golden file (nodes 19-21 on the y mid-plane, |u_z| <= 5.4e-10 where
+# |u| reaches 1) are cancellation residue of the reference's iterative solve,
+# 1e-12 from the direct solution in absolute terms; 1e-9 leaves them out.
+@pytest.mark.parametrize("n,length,ref,min_value", [
+    ((10, 2), (1.0, 0.1), "bar.2D.cartesian.Dirichlet.bodyForce.txt", 1e-10),
+    ((10, 2, 2), (1.0, 0.1, 0.04), "bar.3D.cartesian.Dirichlet.bodyForce.txt", 1e-9)], ids=["2D", "3D"])
+def test_golden_elasticity_cartesian(n, length, ref, min_value):
+    """modules/elasticity/inputs/bar.{2D,3D}.cartesian.Dirichlet.bodyForce.arc:
+    E 21e5, nu 0.28, f = (-1, NULL), XMIN clamped, XMAX u_y = 1, penalty 1e30;
+    eps 1e-3.  The golden file lists node uids of the generated mesh (all 33
+    nodes in 2D, 31 of 99 in 3D)."""
+    X, C = _cartesian(n, length)
+    dim, npc, N = len(n), C.shape[1], X.shape[0]
+    lam, mu = O.lame(21e5, 0.28)
+    row, col = O.pattern(N, C, npc)
+    val = O.assemble(dim, X, C, npc, row, col, element=O.ELASTICITY, nb_dof=dim, order_per_block=False,
+                     lame_lambda=lam, lame_mu=mu)
+    rhs = np.zeros(dim * N)
+    O.rhs_constant_source(dim, X, C, npc, -1.0, nb_dof=dim, dof_index=0, rhs=rhs)
+    fi, fv = np.zeros(dim * N, np.uint8), np.zeros(dim * N)
+    xmin, xmax = np.flatnonzero(X[:, 0] == 0.0), np.flatnonzero(np.isclose(X[:, 0], length[0]))
+    for nodes, comps in [(xmin, [(c, 0.0) for c in range(dim)]), (xmax, [(1, 1.0)])]:
+        for c, v in comps:
+            dofs = dim * nodes + c
+            fi[dofs], fv[dofs], rhs[dofs] = 1, 1e30, 1e30 * v
+    crow, ccol = O.bsr_to_csr(N, dim, row, col)
+    O.apply_transformations(crow, ccol, val, rhs, fi, fv)
+    x = spl.spsolve(sp.csr_matrix((val, ccol, crow), shape=(dim * N, dim * N)).tocsc(), rhs).reshape(-1, dim)
+    data = np.loadtxt(os.path.join(GOLDEN, ref))
+    uid = data[:, 0].astype(np.int64)
+    assert O.count_differences(data[:, 1:1 + dim].ravel(), x[uid].ravel(), 1e-3, min_value) == 0
+
+
 # ---- manufactured solutions (modules/poisson/ManufacturedSolutions.{h,cc}) -------------
 
 def _box_q1(dim, n):
