@@ -3,7 +3,7 @@
 // 160-356, modules/elasticity/FemModule.cc:57-154, modules/heat/FemModule.cc:
 // 74-420) driven through the C++ DoFLinearSystem /
 // BSRFormat layer and the MI355X linear-system factory, on a reference Gmsh
-// mesh or a structured box.
+// mesh, a structured box or an Arcane Cartesian mesh.
 //
 //   afem_module poisson --mesh circle_cut.msh --f 5.5 --dirichlet horizontal=0.5 \
 //       [--neumann curved=-0.35,1.65] \
@@ -17,6 +17,9 @@
 //   afem_module heat --mesh plate.msh --lambda 1.75 --dt 0.4 --tmax 20 --Tinit 30 \
 //       --penalty 1e31 --dirichlet left=10.0 [--convection right=1.0,20.0] \
 //       --check 2d_conduction.txt
+//   afem_module poisson --mesh mms.quad4_n4.msh --manufactured sine --mms-tolerance 8e-2
+//   afem_module elasticity --cartesian 10,2 1.0,0.1 --f -1.0,NULL --dirichlet XMIN=0.0,0.0 \
+//       --dirichlet XMAX=NULL,1.0 --check bar.2D.cartesian.Dirichlet.bodyForce.txt
 //
 // Exit status 0 when the solve converged and the check (if any) passed.
 #include "femutils/BSRFormat.hpp"
