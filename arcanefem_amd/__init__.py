@@ -76,6 +76,7 @@ def _declare(L):
         "afem_memcpy_dtoh": (ci, [vp, vp, ctypes.c_size_t]),
         "afem_memcpy_dtod": (ci, [vp, vp, ctypes.c_size_t]),
         "afem_memset": (ci, [vp, ci, ctypes.c_size_t]),
+        "afem_stream_copy": (ci, [vp, vp, ctypes.c_size_t]),
         "afem_profile_enable": (ci, [ci]),
         "afem_profile_reset": (ci, []),
         "afem_profile_query": (ci, [ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(dbl)]),
@@ -107,7 +108,7 @@ def _declare(L):
         "afem_manufactured_l2_error": (ci, [vp, ci, vp, vp]),
         "afem_system_create": (ci, [i32, pp]),
         "afem_system_destroy": (ci, [vp]),
-        "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp]),
+        "afem_system_set_csr_values": (ci, [vp, i32, i64, vp, vp, vp, vp]),
         "afem_system_set_bsr": (ci, [vp, vp]),
         "afem_system_rhs": (ci, [vp, pp]),
         "afem_system_solution": (ci, [vp, pp]),
@@ -441,9 +442,15 @@ class DoFLinearSystem:
         check(lib().afem_system_elimination(self.h, ctypes.byref(i), ctypes.byref(v)))
         return DeviceArray.view(i.value, self.nb_dof, np.uint8), DeviceArray.view(v.value, self.nb_dof, np.float64)
 
-    def set_csr_values(self, row, col, val):
-        """row/col/val are DeviceArray (kept alive by the caller until solve)."""
-        check(lib().afem_system_set_csr_values(self.h, self.nb_dof, int(np.prod(col.shape)), row.c, col.c, val.c))
+    def set_csr_values(self, rows, col, val, rows_nb_column=None):
+        """setCSRValues with the reference's view layout: rows holds one start
+        per row (the last row ends at the value count; a closing entry equal
+        to it is accepted), rows_nb_column the per-row counts or None.  All
+        DeviceArray; columns and values are kept alive by the caller until
+        solve, the row starts are copied."""
+        nb_row = int(np.prod(rows.shape))
+        cnt = rows_nb_column.c if rows_nb_column is not None else None
+        check(lib().afem_system_set_csr_values(self.h, nb_row, int(np.prod(col.shape)), rows.c, cnt, col.c, val.c))
 
     def set_bsr(self, bsr):
         check(lib().afem_system_set_bsr(self.h, bsr.h))

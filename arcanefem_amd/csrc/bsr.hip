@@ -81,9 +81,12 @@ static bool pattern_pass(const afem_mesh& m, const int32_t* row_index, int32_t* 
     constexpr int CAP = decltype(cap_tag)::value;
     constexpr int BS = decltype(bs_tag)::value;
     ovf.zero();
-    hipLaunchKernelGGL((k_pattern_rows<CAP, BS>), dim3(grid_for(m.nb_node, BS)), dim3(BS), 0, stream(),
-                       m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.slot_bits, m.nb_node, row_index, out,
-                       ovf.p);
+    {
+      ProfScope prof("pattern_rows");
+      hipLaunchKernelGGL((k_pattern_rows<CAP, BS>), dim3(grid_for(m.nb_node, BS)), dim3(BS), 0, stream(),
+                         m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.slot_bits, m.nb_node, row_index, out,
+                         ovf.p);
+    }
     check_launch("pattern_rows");
     int32_t flag = 0;
     ovf.download(&flag, 1);
@@ -146,6 +149,7 @@ static void bsr_compute_sparsity(afem_bsr* b)
   b->values.alloc((size_t)b->nb_col * b->nb_dof * b->nb_dof);
   b->values.zero();
   b->has_sparsity = true;
+  ++b->pattern_gen;
 }
 
 static void build_plan(afem_bsr* b);
@@ -1404,6 +1408,181 @@ k_edge_fill(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ pla
     ecnt[k] = (uint8_t)run[k];
 }
 
+// Geometry plan of the elasticity kernel: for each edge workgroup the
+// distinct cells of its rows (its "instances", sorted), and for each (block,
+// cell) entry of the edge slices the instance index and the slots of the
+// block's two nodes in that cell.
+constexpr int kEdgeSortCap = 2048; // (row, cell) pairs of one workgroup the instance builder sorts
+constexpr int kEdgeInstCap = 1023; // instances of one workgroup (10-bit index)
+
+// Distinct cells of the rows of edge workgroup w, sorted in LDS (bitonic).
+// FILL = false: their number into count[w]; FILL = true: the cells at
+// inst + inst_off[w].  *bad = 1 when a workgroup has more pairs than the sort
+// holds.
+template <bool FILL>
+__global__ void __launch_bounds__(256)
+k_edge_instances(const int32_t* __restrict__ wg_row, const int32_t* __restrict__ nc_off,
+                 const int32_t* __restrict__ nc_items, int slot_bits, const int32_t* __restrict__ inst_off,
+                 int32_t* __restrict__ count, int32_t* __restrict__ inst, int32_t* __restrict__ bad)
+{
+  __shared__ int32_t s_key[kEdgeSortCap];
+  __shared__ int32_t s_sum[256];
+  const int32_t w = blockIdx.x;
+  const int32_t e0 = nc_off[wg_row[w]];
+  const int n = nc_off[wg_row[w + 1]] - e0;
+  if (n > kEdgeSortCap) {
+    if (threadIdx.x == 0)
+      atomicOr(bad, 1);
+    return;
+  }
+  int P = 1;
+  while (P < n)
+    P <<= 1;
+  for (int k = threadIdx.x; k < P; k += 256)
+    s_key[k] = k < n ? (nc_items[e0 + k] >> slot_bits) : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1)
+    for (int half = size >> 1; half > 0; half >>= 1) {
+      for (int k = threadIdx.x; k < P / 2; k += 256) {
+        const int i = 2 * half * (k / half) + (k % half), j = i + half;
+        const int32_t a = s_key[i], c = s_key[j];
+        if ((a > c) == ((i & size) == 0)) {
+          s_key[i] = c;
+          s_key[j] = a;
+        }
+      }
+      __syncthreads();
+    }
+  // thread t owns the 8 sorted entries [8t, 8t + 8): count its first occurrences
+  const int t = threadIdx.x;
+  int own = 0;
+  for (int k = 8 * t; k < 8 * t + 8 && k < n; ++k)
+    own += (k == 0 || s_key[k] != s_key[k - 1]) ? 1 : 0;
+  s_sum[t] = own;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = t >= off ? s_sum[t - off] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  if (!FILL) {
+    if (t == 255)
+      count[w] = s_sum[255];
+    return;
+  }
+  int at = inst_off[w] + s_sum[t] - own;
+  for (int k = 8 * t; k < 8 * t + 8 && k < n; ++k)
+    if (k == 0 || s_key[k] != s_key[k - 1])
+      inst[at++] = s_key[k];
+}
+
+// Instance index and node slots of every (block, cell) entry of workgroup w.
+__global__ void __launch_bounds__(256)
+k_edge_encode(const int32_t* __restrict__ wg_row, const int32_t* __restrict__ row_index,
+              const int32_t* __restrict__ columns, const int64_t* __restrict__ eslice,
+              const int32_t* __restrict__ ecell, const uint8_t* __restrict__ ecnt, const int32_t* __restrict__ cells,
+              const int32_t* __restrict__ inst_off, const int32_t* __restrict__ inst, uint16_t* __restrict__ egeo,
+              int32_t* __restrict__ bad)
+{
+  __shared__ int32_t s_inst[kEdgeInstCap];
+  __shared__ int32_t s_rows[257];
+  const int32_t w = blockIdx.x;
+  const int32_t p0 = wg_row[w], nr = wg_row[w + 1] - p0;
+  const int32_t b0 = row_index[p0];
+  const int32_t i0 = inst_off[w], ni = inst_off[w + 1] - i0;
+  if (ni > kEdgeInstCap) {
+    if (threadIdx.x == 0)
+      atomicOr(bad, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < ni; k += 256)
+    s_inst[k] = inst[i0 + k];
+  for (int k = threadIdx.x; k <= nr; k += 256)
+    s_rows[k] = row_index[p0 + k] - b0;
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= s_rows[nr])
+    return;
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_rows[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int32_t p = p0 + lo, q = columns[b0 + t];
+  const int64_t base = eslice[w] + t;
+  for (int j = 0; j < ecnt[b0 + t]; ++j) {
+    const int32_t c = ecell[base + 256 * (int64_t)j];
+    const int li = find_col(s_inst, ni, c);
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+    const int32_t v[4] = { cq.x, cq.y, cq.z, cq.w };
+    int sp = -1, sq = -1;
+    for (int k = 0; k < 4; ++k) {
+      if (v[k] == p)
+        sp = k;
+      if (v[k] == q)
+        sq = k;
+    }
+    if (li >= ni || s_inst[li] != c || sp < 0 || sq < 0) {
+      atomicOr(bad, 2);
+      continue;
+    }
+    egeo[base + 256 * (int64_t)j] = (uint16_t)(li | (sp << 10) | (sq << 12));
+  }
+}
+
+static void build_edge_geometry(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  b->einst_off.release();
+  b->einst.release();
+  b->egeo.release();
+  b->e_inst_max = 0;
+  const int32_t nwg = b->e_nwg;
+  if (nwg <= 0)
+    return;
+  DBuf<int32_t> cnt((size_t)nwg), bad(1);
+  bad.zero();
+  hipLaunchKernelGGL(k_edge_instances<false>, dim3(nwg), dim3(256), 0, stream(), b->ewg_row.p, m.nc_offsets.p,
+                     m.nc_items.p, m.slot_bits, nullptr, cnt.p, nullptr, bad.p);
+  check_launch("edge_instances");
+  int32_t h = 0;
+  bad.download(&h, 1);
+  if (h)
+    return;
+  std::vector<int32_t> c((size_t)nwg);
+  cnt.download(c.data(), c.size());
+  std::vector<int32_t> off((size_t)nwg + 1, 0);
+  int32_t mx = 0;
+  for (int32_t w = 0; w < nwg; ++w) {
+    off[w + 1] = off[w] + c[w];
+    mx = std::max(mx, c[w]);
+  }
+  if (mx > kEdgeInstCap)
+    return;
+  b->einst_off.alloc(off.size());
+  b->einst_off.upload(off.data(), off.size());
+  b->einst.alloc((size_t)std::max(off[nwg], 1));
+  hipLaunchKernelGGL(k_edge_instances<true>, dim3(nwg), dim3(256), 0, stream(), b->ewg_row.p, m.nc_offsets.p,
+                     m.nc_items.p, m.slot_bits, b->einst_off.p, nullptr, b->einst.p, bad.p);
+  check_launch("edge_instances_fill");
+  b->egeo.alloc(b->ecell.n);
+  hipLaunchKernelGGL(k_edge_encode, dim3(nwg), dim3(256), 0, stream(), b->ewg_row.p, b->row_index.p, b->columns.p,
+                     b->eslice.p, b->ecell.p, b->ecnt.p, m.cells.p, b->einst_off.p, b->einst.p, b->egeo.p, bad.p);
+  check_launch("edge_encode");
+  bad.download(&h, 1);
+  if (h) {
+    b->einst_off.release();
+    b->einst.release();
+    b->egeo.release();
+    return;
+  }
+  b->e_inst_max = mx;
+}
+
 static void build_edge_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
@@ -1464,6 +1643,7 @@ static void build_edge_plan(afem_bsr* b)
   check_launch("edge_fill");
   AFEM_HIP(hipStreamSynchronize(stream()));
   b->e_nwg = nwg;
+  build_edge_geometry(b);
 }
 
 // Elasticity assembly, one thread per block (p, q) of a workgroup of whole
@@ -1574,6 +1754,127 @@ k_assemble_elasticity_edge(const double* __restrict__ coords, const int32_t* __r
 #pragma unroll
       for (int s = 0; s < 3; ++s)
         values[base + 3 * (x + (int64_t)r * len) + s] = K[r * 3 + s];
+  }
+}
+
+// Elasticity assembly with the cell geometry computed once per workgroup:
+// the workgroup's distinct cells (its instances) first get their cofactor
+// vectors C_k = det grad N_k (origin at the cell's first node: C_1 = e2 x e3,
+// C_2 = e3 x e1, C_3 = e1 x e2, C_0 = -(C_1 + C_2 + C_3)) and 1 / (6 |det|)
+// staged in LDS, one thread per instance; then, as k_assemble_elasticity_edge,
+// one thread per nonzero block (p, q) sums lambda C_p (x) C_q + mu C_q (x) C_p
+// + mu (C_p . C_q) I, times 1 / (6 |det|), over the cells of its edge -- now
+// 7 LDS reads per (block, cell) instead of the cell's connectivity, two node
+// coordinates and three cross products.  The diagonal block is minus the sum
+// of the row's other blocks.  The LDS of the instances is reused for that sum.
+__global__ void __launch_bounds__(256)
+k_assemble_elasticity_geo(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+                          const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                          const int32_t* __restrict__ wg_row, const int64_t* __restrict__ eslice,
+                          const uint16_t* __restrict__ egeo, const uint8_t* __restrict__ ecnt,
+                          const int32_t* __restrict__ inst_off, const int32_t* __restrict__ inst, int cap,
+                          double lame_lambda, double lame_mu, int per_block, double* __restrict__ values)
+{
+  extern __shared__ double s_geo[]; // 13 * cap doubles (component-major), later 9 * 256
+  __shared__ int32_t s_rows[257];
+  const int32_t w = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int32_t p0 = wg_row[w], nr = wg_row[w + 1] - p0;
+  const int32_t b0 = row_index[p0];
+  const int32_t i0 = inst_off[w], ni = inst_off[w + 1] - i0;
+  for (int k = threadIdx.x; k <= nr; k += 256)
+    s_rows[k] = row_index[p0 + k] - b0;
+  for (int k = threadIdx.x; k < ni; k += 256) {
+    const int32_t c = inst[i0 + k];
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+    const d3 X0 = load_node(coords, cq.x);
+    const d3 e1 = sub(load_node(coords, cq.y), X0), e2 = sub(load_node(coords, cq.z), X0),
+             e3 = sub(load_node(coords, cq.w), X0);
+    const d3 C1 = cross(e2, e3), C2 = cross(e3, e1), C3 = cross(e1, e2);
+    const double inv6 = fast_rcp(6.0 * fabs(dot(e1, C1)));
+    s_geo[0 * cap + k] = -(C1.x + C2.x + C3.x);
+    s_geo[1 * cap + k] = -(C1.y + C2.y + C3.y);
+    s_geo[2 * cap + k] = -(C1.z + C2.z + C3.z);
+    s_geo[3 * cap + k] = C1.x;
+    s_geo[4 * cap + k] = C1.y;
+    s_geo[5 * cap + k] = C1.z;
+    s_geo[6 * cap + k] = C2.x;
+    s_geo[7 * cap + k] = C2.y;
+    s_geo[8 * cap + k] = C2.z;
+    s_geo[9 * cap + k] = C3.x;
+    s_geo[10 * cap + k] = C3.y;
+    s_geo[11 * cap + k] = C3.z;
+    s_geo[12 * cap + k] = inv6;
+  }
+  __syncthreads();
+  const int nb = s_rows[nr];
+  const int t = threadIdx.x;
+  double K[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    K[i] = 0.0;
+  int pr = 0;
+  bool diag = false;
+  if (t < nb) {
+    int lo = 0, hi = nr - 1; // the row of block t: the last row starting at or before t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_rows[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    pr = lo;
+    diag = (p0 + pr) == columns[b0 + t];
+    const int cnum = ecnt[b0 + t];
+    const int64_t base = eslice[w] + t;
+    for (int j = 0; j < cnum; ++j) {
+      const uint32_t code = egeo[base + 256 * (int64_t)j];
+      const int li = code & 1023, sp = (code >> 10) & 3, sq = (code >> 12) & 3;
+      const double av[3] = { s_geo[(3 * sp) * cap + li], s_geo[(3 * sp + 1) * cap + li],
+                             s_geo[(3 * sp + 2) * cap + li] };
+      const double bq[3] = { s_geo[(3 * sq) * cap + li], s_geo[(3 * sq + 1) * cap + li],
+                             s_geo[(3 * sq + 2) * cap + li] };
+      const double inv6 = s_geo[12 * cap + li];
+      const double la = lame_lambda * inv6, mu = lame_mu * inv6;
+      const double dd = mu * (av[0] * bq[0] + av[1] * bq[1] + av[2] * bq[2]);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; ++c2)
+          K[r * 3 + c2] += la * av[r] * bq[c2] + mu * bq[r] * av[c2] + (r == c2 ? dd : 0.0);
+    }
+  }
+  __syncthreads(); // the instances are no longer read: the LDS holds the blocks now
+  double* s_blk = s_geo;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    s_blk[i * 256 + t] = K[i];
+  __syncthreads();
+  if (t >= nb)
+    return;
+  const int x0 = s_rows[pr], len = s_rows[pr + 1] - x0;
+  if (diag) {
+    for (int u = x0; u < x0 + len; ++u)
+      if (u != t) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+          K[i] -= s_blk[i * 256 + u];
+      }
+  }
+  const int64_t blk = (int64_t)b0 + t;
+  if (per_block) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      values[9 * blk + i] = K[i];
+  }
+  else {
+    const int64_t base = 9 * ((int64_t)b0 + x0);
+    const int x = t - x0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s2 = 0; s2 < 3; ++s2)
+        values[base + 3 * (x + (int64_t)r * len) + s2] = K[r * 3 + s2];
   }
 }
 
@@ -2345,6 +2646,17 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
   AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4 && b->mesh->dim == 3, AFEM_ERR_ARGUMENT,
                "the elasticity element needs a triangle mesh and nb_dof = 2, or a tetrahedral mesh and nb_dof = 3");
   const afem_mesh& m = *b->mesh;
+  if (b->egeo.p && b->variant != 0 && b->variant != 1) {
+    // cap: component stride of the staged geometry (odd, to spread the LDS banks)
+    const int cap = b->e_inst_max | 1;
+    const size_t lds = sizeof(double) * std::max<size_t>((size_t)13 * cap, (size_t)9 * 256);
+    ProfScope prof("assemble_elasticity_tet");
+    hipLaunchKernelGGL(k_assemble_elasticity_geo, dim3(b->e_nwg), dim3(256), lds, stream(), m.coords.p, m.cells.p,
+                       b->row_index.p, b->columns.p, b->ewg_row.p, b->eslice.p, b->egeo.p, b->ecnt.p,
+                       b->einst_off.p, b->einst.p, cap, lam, mu, b->per_block ? 1 : 0, b->values.p);
+    check_launch("assemble_elasticity_geo");
+    return;
+  }
   if (b->ecell.p && b->variant != 0) {
     ProfScope prof("assemble_elasticity_tet");
     hipLaunchKernelGGL(k_assemble_elasticity_edge, dim3(b->e_nwg), dim3(256), 0, stream(), m.coords.p, m.cells.p,

@@ -212,6 +212,7 @@ struct afem_bsr
   bool use_csr = false;
   bool atomic_free = false;
   bool has_sparsity = false;
+  uint64_t pattern_gen = 0;          // counts computeSparsity calls (linear systems key derived data on it)
   int32_t nb_row = 0;
   int64_t nb_col = 0;                // stored blocks
   int32_t max_row = 0;
@@ -244,6 +245,10 @@ struct afem_bsr
   afem::DBuf<int64_t> eslice;        // first cell word of each workgroup
   afem::DBuf<int32_t> ecell;         // cell j of block t of workgroup w at eslice[w] + 256 j + t
   afem::DBuf<uint8_t> ecnt;          // cells per block
+  afem::DBuf<int32_t> einst_off;     // first instance of each workgroup in einst (nb_wg + 1)
+  afem::DBuf<int32_t> einst;         // the distinct cells of each workgroup's rows, sorted
+  afem::DBuf<uint16_t> egeo;         // as ecell: instance | slot of p << 10 | slot of q << 12
+  int32_t e_inst_max = 0;            // largest instance count of a workgroup (0: no geometry plan)
   // Quad4 / Hexa8: per cell the upper triangle of the element matrix and the
   // source vector, cell-minor (entry k of cell c at k * nb_cell + c)
   afem::DBuf<double> q1_elem;
@@ -263,16 +268,19 @@ struct afem_system
   int32_t nb_dof = 0;
   afem::DBuf<double> rhs, x, forced_value, elim_value;
   afem::DBuf<uint8_t> forced_info, elim_info;
-  // matrix view (non-owning)
+  // matrix view (non-owning, except the closed copy of a CSR view's row starts)
   bool has_matrix = false;
   int32_t nb_row = 0;
   int64_t nnz = 0;
-  const int32_t* row_index = nullptr;
+  const int32_t* row_index = nullptr; // nb_dof + 1 entries, last = nnz
+  afem::DBuf<int32_t> csr_rows;       // row_index of a view given by afem_system_set_csr_values
   const int32_t* columns = nullptr;
   double* values = nullptr;
   int block = 1;                     // 3 when the view is a per-block BSR
   afem_bsr* bsr = nullptr;
+  uint64_t bsr_gen = 0;              // pattern_gen of bsr when it was set
   bool matrix_changed = true;        // invalidates derived data (FP32 copy, diagonal)
+  bool pattern_changed = true;       // invalidates pattern-derived data (SELL layouts)
   bool has_row_column = false;       // row-column eliminations present
   bool dof0_row_column = false;      // DoF 0 is row-column eliminated (column 0 kept: non-symmetric)
   afem::DBuf<double> saved_values;   // values before the row-column elimination

@@ -96,10 +96,13 @@ class OwnedDoFVariable
   DoFVariable<T> m_var;
 };
 
-// Non-owning CSR view of device arrays (rows carry nb_row + 1 offsets here).
+// Non-owning CSR view of device arrays with the spans of
+// femutils/CsrFormatMatrixView.h:145-153: rows holds nb_row row starts (the
+// last row ends at nb_value), rows_nb_column the per-row counts (optional).
 struct CsrFormatView
 {
   const Int32* rows = nullptr;
+  const Int32* rows_nb_column = nullptr;
   const Int32* columns = nullptr;
   Real* values = nullptr;
   Int32 nb_row = 0;
@@ -263,7 +266,8 @@ class Mi355xDoFLinearSystemImpl : public IDoFLinearSystemImpl
     if (v.bsr)
       check(afem_system_set_bsr(m_sys, v.bsr), "setCSRValues");
     else
-      check(afem_system_set_csr_values(m_sys, v.nb_row, v.nb_value, v.rows, v.columns, v.values), "setCSRValues");
+      check(afem_system_set_csr_values(m_sys, v.nb_row, v.nb_value, v.rows, v.rows_nb_column, v.columns, v.values),
+            "setCSRValues");
   }
   CSRFormatView& getCSRValues() override { return m_csr; }
   VariableDoFReal& getForcedValue() override { return m_forced_value; }

@@ -162,6 +162,42 @@ __global__ void k_dirichlet(const int32_t* __restrict__ nodes, int32_t count, in
   }
 }
 
+// Closes the row starts of a CSR view (femutils/CsrFormatMatrixView.h:145-187:
+// `rows` has nbRow() entries and the last row ends at nbColumn()) into
+// nb_dof + 1 offsets, and checks the view on the way: starts in [0, nnz] and
+// non-decreasing, a closing entry (nb_given = nb_dof + 1, the scalar expansion
+// of femutils/BSRFormat.cc:117-139) equal to nnz, per-row counts (when given)
+// equal to the row extents.  bad[0] collects the violations.
+__global__ void k_close_rows(const int32_t* __restrict__ rows, int32_t nb_given, const int32_t* __restrict__ counts,
+                             int32_t n, int64_t nnz, int32_t* __restrict__ out, int32_t* __restrict__ bad)
+{
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n)
+    return;
+  const int64_t b = (r < n) ? rows[r] : nnz;
+  if (r == n) {
+    out[n] = (int32_t)nnz;
+    if (nb_given > n && rows[n] != nnz)
+      atomicOr(bad, 1);
+    return;
+  }
+  const int64_t e = (r + 1 < n) ? rows[r + 1] : (int64_t)nnz;
+  out[r] = (int32_t)b;
+  if (b < 0 || b > e || e > nnz || (r == 0 && b != 0))
+    atomicOr(bad, 2);
+  if (counts && counts[r] != e - b)
+    atomicOr(bad, 4);
+}
+
+// Column ids of a CSR view must name DoFs of the family (the solver gathers
+// x at them).
+__global__ void k_check_columns(const int32_t* __restrict__ cols, int64_t nnz, int32_t n, int32_t* __restrict__ bad)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz && (cols[k] < 0 || cols[k] >= n))
+    atomicOr(bad, 8);
+}
+
 // One (row, column) entry of the matrix view: add or set; found = 0 when the
 // entry is not in the pattern.
 template <int NB>
@@ -315,23 +351,42 @@ AFEM_API int afem_system_destroy(afem_system_t sys)
   AFEM_GUARD_END
 }
 
-AFEM_API int afem_system_set_csr_values(afem_system_t sys, int32_t nb_row, int64_t nnz, const int32_t* row_index,
-                                        const int32_t* columns, double* values)
+AFEM_API int afem_system_set_csr_values(afem_system_t sys, int32_t nb_row, int64_t nnz, const int32_t* rows,
+                                        const int32_t* rows_nb_column, const int32_t* columns, double* values)
 {
   AFEM_GUARD_BEGIN
   AFEM_REQUIRE(sys, AFEM_ERR_ARGUMENT, "null handle");
-  AFEM_REQUIRE(nb_row == sys->nb_dof, AFEM_ERR_ARGUMENT, "CSR row count differs from the DoF count");
+  const int32_t n = sys->nb_dof;
+  AFEM_REQUIRE(nb_row == n || nb_row == n + 1, AFEM_ERR_ARGUMENT,
+               "CSR row count differs from the DoF count (nb_dof, or nb_dof + 1 with a closing entry)");
   AFEM_REQUIRE(nnz > 0 && nnz < (1LL << 31), AFEM_ERR_ARGUMENT, "nnz out of range");
-  AFEM_REQUIRE(row_index && columns && values, AFEM_ERR_ARGUMENT, "null CSR array");
+  AFEM_REQUIRE(rows && columns && values, AFEM_ERR_ARGUMENT, "null CSR array");
+  // closed into a new array first: a rejected view leaves the current one in place
+  DBuf<int32_t> closed((size_t)n + 1);
+  DBuf<int32_t> bad(1);
+  bad.zero();
+  hipLaunchKernelGGL(k_close_rows, dim3(grid_for((int64_t)n + 1, 256)), dim3(256), 0, stream(), rows, nb_row,
+                     rows_nb_column, n, nnz, closed.p, bad.p);
+  check_launch("close_rows");
+  hipLaunchKernelGGL(k_check_columns, dim3(grid_for(nnz, 256)), dim3(256), 0, stream(), columns, nnz, n, bad.p);
+  check_launch("check_columns");
+  int32_t h = 0;
+  bad.download(&h, 1);
+  AFEM_REQUIRE(!(h & 1), AFEM_ERR_ARGUMENT, "CSR view: the closing row entry differs from the value count");
+  AFEM_REQUIRE(!(h & 2), AFEM_ERR_ARGUMENT, "CSR view: row starts are not non-decreasing in [0, nnz] from 0");
+  AFEM_REQUIRE(!(h & 4), AFEM_ERR_ARGUMENT, "CSR view: a row's column count differs from its extent");
+  AFEM_REQUIRE(!(h & 8), AFEM_ERR_ARGUMENT, "CSR view: a column id is outside the DoF family");
+  sys->csr_rows = std::move(closed);
   sys->has_matrix = true;
-  sys->nb_row = nb_row;
+  sys->nb_row = n;
   sys->nnz = nnz;
-  sys->row_index = row_index;
+  sys->row_index = sys->csr_rows.p;
   sys->columns = columns;
   sys->values = values;
   sys->block = 1;
   sys->bsr = nullptr;
   sys->matrix_changed = true;
+  sys->pattern_changed = true; // a caller's arrays may hold a new pattern at the same addresses
   AFEM_GUARD_END
 }
 
@@ -344,11 +399,14 @@ AFEM_API int afem_system_set_bsr(afem_system_t sys, afem_bsr_t bsr)
                "matrix size differs from the DoF count");
   AFEM_REQUIRE(bsr->nb_dof != 2 || !bsr->per_block, AFEM_ERR_NOT_IMPLEMENTED,
                "2x2 blocks: use the per-row (CSR) value order (does_linear_system_use_csr)");
+  if (sys->bsr != bsr || sys->bsr_gen != bsr->pattern_gen)
+    sys->pattern_changed = true;
   sys->has_matrix = true;
   sys->nb_row = bsr->nb_row * bsr->nb_dof;
   sys->nnz = bsr->nb_col * bsr->nb_dof * bsr->nb_dof;
   sys->values = bsr->values.p;
   sys->bsr = bsr;
+  sys->bsr_gen = bsr->pattern_gen;
   sys->matrix_changed = true;
   if (bsr->nb_dof == 1) {
     sys->block = 1;
@@ -492,6 +550,7 @@ AFEM_API int afem_system_clear_values(afem_system_t sys)
   sys->values = nullptr;
   sys->bsr = nullptr;
   sys->matrix_changed = true;
+  sys->pattern_changed = true;
   AFEM_GUARD_END
 }
 

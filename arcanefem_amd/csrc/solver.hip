@@ -1328,7 +1328,8 @@ static afem_solver_ws* workspace(afem_system* s)
     w->st.alloc(1);
     AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&w->host_st), sizeof(State), hipHostMallocDefault));
   }
-  if (w->rp != s->row_index || w->ci != s->columns || w->nnz != s->nnz) {
+  if (w->rp != s->row_index || w->ci != s->columns || w->nnz != s->nnz || s->pattern_changed) {
+    s->pattern_changed = false;
     w->rp = s->row_index;
     w->ci = s->columns;
     w->nnz = s->nnz;

@@ -11,9 +11,15 @@ Two summaries are produced from the CSV files rocprofv3 writes:
   256 MiB): factor = copy bytes / counter reading.  The result is written as
   JSON (read by bench.py for ``roofline.traffic``) and as Markdown.
 
+* ``counters``: the per-dispatch median of every counter found in one or
+  more ``--pmc`` passes (e.g. SQ_INSTS_VALU, SQ_INSTS_LDS,
+  SQ_LDS_BANK_CONFLICT, SQ_WAVES), one row per kernel, as Markdown.
+
 python -m arcanefem_amd.profiling stats --csv K.csv --command "..." --out profiles/X.md
 python -m arcanefem_amd.profiling pmc --fetch F.csv --write W.csv --workload tet128 \
     --command "..." --out profiles/X   (writes X.json and X.md)
+python -m arcanefem_amd.profiling counters --csv A.csv [B.csv ...] --match REGEX \
+    --command "..." --out profiles/X.md
 """
 
 import argparse
@@ -89,6 +95,31 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes):
     }
 
 
+def counter_table(paths, match=None):
+    """{kernel: {counter: (median per dispatch, dispatches)}} over the CSV
+    files of several --pmc passes; kernels filtered by the regex `match`."""
+    acc = {}
+    for path in paths:
+        for r in _read(path):
+            name = short_name(r["Kernel_Name"])
+            if r["Kernel_Name"].startswith(COPY_KERNEL) or (match and not re.search(match, name)):
+                continue
+            acc.setdefault(name, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: {c: (statistics.median(v), len(v)) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def counters_markdown(table, command):
+    counters = sorted({c for cs in table.values() for c in cs})
+    lines = ["# rocprofv3 counters per dispatch (median)", "", "Commands:", "", "```", command, "```", "",
+             "| kernel | dispatches | " + " | ".join(counters) + " |",
+             "|---|---:|" + "---:|" * len(counters)]
+    for k in sorted(table):
+        n = max(v[1] for v in table[k].values())
+        cells = [f"{table[k][c][0]:.4g}" if c in table[k] else "" for c in counters]
+        lines.append(f"| `{k}` | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(lines) + "\n"
+
+
 def stats_markdown(path, command, top=20):
     rows = _read(path)
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
@@ -150,8 +181,16 @@ def main():
     p.add_argument("--command", required=True)
     p.add_argument("--copy-bytes", type=int, default=256 << 20)
     p.add_argument("--out", required=True, help="output path without extension")
+    c = sub.add_parser("counters")
+    c.add_argument("--csv", nargs="+", required=True)
+    c.add_argument("--match", default=None, help="regex on the short kernel name")
+    c.add_argument("--command", required=True)
+    c.add_argument("--out", required=True)
     a = ap.parse_args()
-    if a.cmd == "stats":
+    if a.cmd == "counters":
+        with open(a.out, "w") as f:
+            f.write(counters_markdown(counter_table(a.csv, a.match), a.command))
+    elif a.cmd == "stats":
         with open(a.out, "w") as f:
             f.write(stats_markdown(a.csv, a.command, a.top))
     else:

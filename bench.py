@@ -79,15 +79,80 @@ def cpu_model():
     return platform.processor()
 
 
-def solve_best(A, sysm, reps=2, **kw):
+def cpu_solve_baseline(A, args, bsr, sysm, N):
+    """FP64 Jacobi-PCG of the same transformed system on the host (the
+    PETSc cg + jacobi defaults, femutils/PetscDoFLinearSystemFactory.axl:5-10,
+    restated by the oracle): on all host threads, 1 warm-up + median of
+    --cpu-solve-reps full solves to 1e-8; on one thread a bounded sample of
+    40 iterations (median of 5), extrapolated to the all-cores iteration
+    count."""
+    import oracle as O
+    row, col, val, _ = bsr.download()
+    b = sysm.rhs().numpy()
+    nth = O.max_threads()
+    O.pcg_jacobi_threads(row, col, val, b, nth, rtol=1e-8, max_iter=20000)
+    times, its, rr = [], None, None
+    for _ in range(args.cpu_solve_reps):
+        t0 = O.wtime()
+        _, its, rr = O.pcg_jacobi_threads(row, col, val, b, nth, rtol=1e-8, max_iter=20000)
+        times.append(O.wtime() - t0)
+    t_all = float(np.median(times))
+    sample_it = 40
+    t1 = []
+    for _ in range(5):
+        t0 = O.wtime()
+        O.pcg_jacobi_threads(row, col, val, b, 1, rtol=1e-30, max_iter=sample_it)
+        t1.append(O.wtime() - t0)
+    per_it_1 = float(np.median(t1)) / sample_it
+    out = {"kind": "port", "cpu": cpu_model(),
+           "all_cores": {"cores": nth, "solve_s": t_all, "iterations": int(its), "relres_fp64": rr,
+                         "reps": args.cpu_solve_reps},
+           "one_core": {"cores": 1, "ms_per_iteration": per_it_1 * 1e3,
+                        "solve_s_extrapolated": per_it_1 * max(int(its), 1),
+                        "sample": f"{sample_it} iterations, median of 5"}}
+    log(f"[bench] cpu jacobi-pcg fp64: {t_all:.2f} s on {nth} threads ({its} it), "
+        f"{per_it_1 * 1e3:.1f} ms/iteration on one thread")
+    return out
+
+
+def measure_copy(A, nbytes=1 << 30, reps=10):
+    """Device-to-device copy by the library's streaming kernel (event-timed):
+    the measured HBM bandwidth the roofline figures are set beside."""
+    src, dst = A.DeviceArray(nbytes, np.uint8), A.DeviceArray(nbytes, np.uint8)
+    src.zero()
+    for _ in range(3):
+        A.check(A.lib().afem_stream_copy(dst.c, src.c, nbytes))
+    A.synchronize()
+    A.profile_reset()
+    A.profile_enable(True)
+    for _ in range(reps):
+        A.check(A.lib().afem_stream_copy(dst.c, src.c, nbytes))
+    A.synchronize()
+    A.profile_enable(False)
+    ms, n = kernel_avg_ms(A, "stream_copy")
+    del src, dst
+    return {"bytes_moved": 2 * nbytes, "kernel_ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9, "reps": n,
+            "kernel": "k_stream_copy"}
+
+
+def rel_diff(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def solve_best(A, sysm, reps=2, x_fp64=None, **kw):
+    """Best of `reps` solves; with x_fp64 also ||x - x_fp64|| / ||x_fp64||
+    (the error against the FP64 Jacobi-PCG solution of the same system)."""
     best = None
     for _ in range(reps):
         r = sysm.solve(raise_on_failure=False, **kw)
         if best is None or r.solve_ms < best.solve_ms:
             best = r
-    return {"iterations": best.iterations, "solve_ms": best.solve_ms, "relres_fp64": best.relres,
-            "converged": bool(best.converged), "reliable_updates": best.reliable_updates,
-            "fallback": bool(best.fallback)}
+    out = {"iterations": best.iterations, "solve_ms": best.solve_ms, "relres_fp64": best.relres,
+           "converged": bool(best.converged), "reliable_updates": best.reliable_updates,
+           "fallback": bool(best.fallback)}
+    if x_fp64 is not None:
+        out["error_vs_fp64_solution"] = rel_diff(sysm.solution().numpy(), x_fp64)
+    return out
 
 
 def learned_section(A, args, X, C, mesh, bsr, N, nnz):
@@ -154,16 +219,17 @@ def learned_section(A, args, X, C, mesh, bsr, N, nnz):
         "tolerance": 1e-8,
     }
     kw = dict(rtol=1e-8, max_iter=40000, check_every=32)
-    res["cg_fp64"] = solve_best(A, sysm, method=A.SOLVER_CG_FP64, **kw)
     res["jacobi_pcg_fp64"] = solve_best(A, sysm, method=A.SOLVER_PCG_FP64, **kw)
-    res["jacobi_pcg_mixed"] = solve_best(A, sysm, method=A.SOLVER_PCG_MIXED, **kw)
+    x64 = sysm.solution().numpy()
+    res["cg_fp64"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_CG_FP64, **kw)
+    res["jacobi_pcg_mixed"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_PCG_MIXED, **kw)
     A.synchronize()
     t0 = time.perf_counter()
     net.apply(sysm)
     A.synchronize()
     res["learned_inference_ms"] = (time.perf_counter() - t0) * 1e3
-    res["learned_fpcg_mixed_fp32m"] = solve_best(A, sysm, method=A.SOLVER_FPCG_SPAI, **kw)
-    res["learned_fpcg_mixed"] = solve_best(A, sysm, method=A.SOLVER_FPCG_SPAI, precond_bf16=1, **kw)
+    res["learned_fpcg_mixed_fp32m"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_FPCG_SPAI, **kw)
+    res["learned_fpcg_mixed"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_FPCG_SPAI, precond_bf16=1, **kw)
     res["learned_fpcg_mixed"]["preconditioner_values"] = "bf16"
     lt = res["learned_fpcg_mixed"]["solve_ms"]
     res["speedup_vs_jacobi_pcg_fp64"] = res["jacobi_pcg_fp64"]["solve_ms"] / lt
@@ -172,8 +238,10 @@ def learned_section(A, args, X, C, mesh, bsr, N, nnz):
                                                         / (lt + res["learned_inference_ms"]))
     for name in ("cg_fp64", "jacobi_pcg_fp64", "jacobi_pcg_mixed", "learned_fpcg_mixed_fp32m", "learned_fpcg_mixed"):
         r = res[name]
+        err = r.get("error_vs_fp64_solution")
         log(f"[bench] heterogeneous {name}: {r['iterations']} it, {r['solve_ms']:.1f} ms, "
-            f"relres {r['relres_fp64']:.2e}{' (fallback)' if r['fallback'] else ''}")
+            f"relres {r['relres_fp64']:.2e}{' (fallback)' if r['fallback'] else ''}"
+            + (f", |x - x_fp64|/|x_fp64| {err:.1e}" if err is not None else ""))
     return res
 
 
@@ -416,7 +484,11 @@ def elasticity_section(A, args):
         sysm.apply_dirichlet(A.DIRICHLET_ROW_ELIMINATION, left, 3, c, 0.0, 0.0)
     sysm.apply_matrix_transformation()
     sysm.apply_rhs_transformation()
-    sol = solve_best(A, sysm, reps=1, method=A.SOLVER_PCG_MIXED, rtol=1e-8, max_iter=40000, check_every=32)
+    spmv = block_spmv_roofline(A, sysm, N, nb)
+    sol64 = solve_best(A, sysm, reps=1, method=A.SOLVER_PCG_FP64, rtol=1e-8, max_iter=40000, check_every=32)
+    x64 = sysm.solution().numpy()
+    sol = solve_best(A, sysm, reps=1, x_fp64=x64, method=A.SOLVER_PCG_MIXED, rtol=1e-8, max_iter=40000,
+                     check_every=32)
     out = {
         "workload": "3D linear elasticity P1, 96^3 hexes x 6 tets, E 21e5, nu 0.28, body force (0,0,-1), "
                     "x = 0 clamped, BSR 3x3",
@@ -424,10 +496,48 @@ def elasticity_section(A, args):
         "assembly_ms": asm_ms, "assembly_kernel_ms": k_ms, "algorithmic_bytes": alg,
         "assembly_GBs": alg / (k_ms * 1e-3) / 1e9 if k_ms == k_ms else None,
         "assembled_dofs_per_s": 3 * N / (asm_ms * 1e-3),
+        "block_jacobi_pcg_fp64": sol64,
         "block_jacobi_pcg_mixed": sol,
+        "spmv_bsr3": spmv,
     }
     log(f"[bench] elasticity 96^3: assembly {asm_ms:.2f} ms ({out['assembly_GBs']:.0f} GB/s), "
         f"PCG {sol['iterations']} it {sol['solve_ms']:.1f} ms relres {sol['relres_fp64']:.1e}")
+    return out
+
+
+def block_spmv_roofline(A, sysm, N, nb):
+    """FP64 and FP32 SpMV of a 3x3-block system (the elasticity matrix):
+    algorithmic bytes 8 nnz_s + 4 nnz_b + 4 (N + 1) + 16 DoFs (FP64 values
+    and vectors) and 4 nnz_s + 4 nnz_b + 4 (N + 1) + 8 DoFs (FP32)."""
+    d = 3 * N
+    x = A.DeviceArray.from_numpy(np.random.default_rng(1).normal(size=d))
+    y = A.DeviceArray(d, np.float64)
+    x32 = A.DeviceArray.from_numpy(np.random.default_rng(1).normal(size=d).astype(np.float32))
+    y32 = A.DeviceArray(d, np.float32)
+    for _ in range(3):
+        sysm.spmv(x, y)
+        sysm.spmv_fp32(x32, y32)
+    A.synchronize()
+    A.profile_reset()
+    A.profile_enable(True)
+    for _ in range(20):
+        sysm.spmv(x, y)
+        sysm.spmv_fp32(x32, y32)
+    A.synchronize()
+    A.profile_enable(False)
+    s64, _ = kernel_avg_ms(A, "spmv_fp64")
+    s32, _ = kernel_avg_ms(A, "spmv_fp32")
+    nnz_s = 9 * nb
+    b64 = 8 * nnz_s + 4 * nb + 4 * (N + 1) + 16 * d
+    b32 = 4 * nnz_s + 4 * nb + 4 * (N + 1) + 8 * d
+    fmt = sysm.spmv_format()
+    out = {"format": ("bsr3_csr%d" % fmt[1]) if fmt[0] == "csr" else "sell3",
+           "fp64": {"kernel_ms": s64, "algorithmic_bytes": b64, "GBs": b64 / (s64 * 1e-3) / 1e9,
+                    "frac": b64 / (s64 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "fp32": {"kernel_ms": s32, "algorithmic_bytes": b32, "GBs": b32 / (s32 * 1e-3) / 1e9,
+                    "frac": b32 / (s32 * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    log(f"[bench] elasticity spmv ({out['format']}): fp64 {s64:.3f} ms ({out['fp64']['GBs']:.0f} GB/s), "
+        f"fp32 {s32:.3f} ms ({out['fp32']['GBs']:.0f} GB/s)")
     return out
 
 
@@ -499,6 +609,12 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 256 MiB device-to-device copy first (reference for HBM byte counters)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="pattern and the timed assembly steps only (the entry point of counter runs)")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="pin the Laplace assembly variant instead of autotuning")
+    ap.add_argument("--cpu-solve-reps", type=int, default=5,
+                    help="timed repetitions of the all-cores CPU Jacobi-PCG (0 = skip the CPU solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -508,6 +624,10 @@ def main():
     import arcanefem_amd as A
 
     A.init(0)
+    copy = measure_copy(A) if not args.assembly_only else None
+    if copy:
+        log(f"[bench] stream copy: {copy['GBs']:.0f} GB/s ({copy['bytes_moved'] / 1e9:.2f} GB moved in "
+            f"{copy['kernel_ms']:.3f} ms)")
     if args.calibrate:
         nbytes = 256 << 20
         src, dst = A.DeviceArray(nbytes, np.uint8), A.DeviceArray(nbytes, np.uint8)
@@ -524,17 +644,28 @@ def main():
     mesh = A.Mesh(3, X, C)
     bsr = A.BSRFormat(mesh, 1, use_csr=True)
     A.synchronize()
+    A.profile_reset()
+    A.profile_enable(True)
     t0 = time.perf_counter()
     bsr.compute_sparsity()
     A.synchronize()
     pattern_ms = (time.perf_counter() - t0) * 1e3
+    A.profile_enable(False)
+    prow_n, prow_ms = A.profile_query("pattern_rows")
     info = bsr.info()
     nnz = int(info.nb_col)
-    log(f"[bench] pattern nnz={nnz} ({nnz / N:.2f}/row) in {pattern_ms:.1f} ms")
+    pat_bytes = 16 * T + 8 * nnz + 4 * N
+    log(f"[bench] pattern nnz={nnz} ({nnz / N:.2f}/row) in {pattern_ms:.1f} ms "
+        f"(row kernel {prow_ms:.2f} ms over {prow_n} passes, {pat_bytes / (prow_ms * 1e-3) / 1e9:.0f} GB/s)")
     sysm = A.DoFLinearSystem(N)
     rhs = sysm.rhs()
-    variant, tune_ms = bsr.autotune_assembly()
-    log(f"[bench] assembly autotuner: variant {variant} ({tune_ms:.3f} ms)")
+    if args.variant is not None:
+        bsr.set_assembly_variant(args.variant)
+        variant, tune_ms = args.variant, float("nan")
+    else:
+        variant, tune_ms = bsr.autotune_assembly()
+    log(f"[bench] assembly variant {variant} ({'pinned' if args.variant is not None else 'autotuned'}, "
+        f"{tune_ms:.3f} ms)")
 
     # ---- timed assembly steps -------------------------------------------------------------
     for _ in range(args.warmup):
@@ -586,7 +717,19 @@ def main():
             "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT),
         },
         "pattern_ms": pattern_ms,
+        "pattern": {"total_ms": pattern_ms, "row_kernel_ms": prow_ms, "row_kernel_passes": prow_n,
+                    "algorithmic_bytes": pat_bytes, "bytes_formula": "16*T + 8*nnz + 4*N (count + fill passes)",
+                    "GBs": pat_bytes / (prow_ms * 1e-3) / 1e9,
+                    "frac": pat_bytes / (prow_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "kernel": "k_pattern_rows"},
     }
+    if copy:
+        out["hbm_copy"] = copy
+        out["roofline"]["copy_measured_GBs"] = copy["GBs"]
+        out["roofline"]["frac_of_copy"] = achieved / copy["GBs"]
+    if args.assembly_only:
+        print(json.dumps(out), flush=True)
+        return
 
     # ---- SpMV and solves -------------------------------------------------------------------------
     bsr.assemble_laplace(None, 1.0, rhs)
@@ -631,23 +774,36 @@ def main():
         "fp32": {"kernel_ms": s32, "algorithmic_bytes": b32, "GBs": b32 / (s32 * 1e-3) / 1e9,
                  "frac": b32 / (s32 * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": k32,
                  "traffic": measured_traffic(workload_key, k32)},
+        "bytes_note": "algorithmic bytes are CSR-equivalent (12 nnz + 20 N FP64, 8 nnz + 12 N FP32); "
+                      "traffic_GBs is the format's measured HBM traffic over the same time",
     }
+    for key, ms in (("fp64", s64), ("fp32", s32)):
+        rec = out["spmv"][key]
+        if rec["traffic"]:
+            rec["traffic_GBs"] = rec["traffic"] / (ms * 1e-3) / 1e9
+            rec["traffic_frac"] = rec["traffic_GBs"] / HBM_PEAK_GBS
+        if copy:
+            rec["frac_of_copy"] = rec["GBs"] / copy["GBs"]
     log(f"[bench] spmv fp64 {s64:.3f} ms ({out['spmv']['fp64']['GBs']:.0f} GB/s), "
         f"fp32 {s32:.3f} ms ({out['spmv']['fp32']['GBs']:.0f} GB/s)")
 
     if not args.no_solve:
         solves = {}
-        for name, method in (("jacobi_pcg_fp64", A.SOLVER_PCG_FP64), ("mixed_pcg", A.SOLVER_PCG_MIXED)):
-            best = None
-            for rep in range(2):
-                r = sysm.solve(raise_on_failure=False, method=method, rtol=1e-8, max_iter=20000, check_every=32)
-                if best is None or r.solve_ms < best.solve_ms:
-                    best = r
-            solves[name] = {"iterations": best.iterations, "solve_ms": best.solve_ms, "relres_fp64": best.relres,
-                            "converged": bool(best.converged), "reliable_updates": best.reliable_updates}
-            log(f"[bench] {name}: {best.iterations} it, {best.solve_ms:.1f} ms, relres {best.relres:.2e}")
+        kw = dict(rtol=1e-8, max_iter=20000, check_every=32)
+        solves["jacobi_pcg_fp64"] = solve_best(A, sysm, method=A.SOLVER_PCG_FP64, **kw)
+        x64 = sysm.solution().numpy()
+        solves["mixed_pcg"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_PCG_MIXED, **kw)
+        for name in ("jacobi_pcg_fp64", "mixed_pcg"):
+            r = solves[name]
+            log(f"[bench] {name}: {r['iterations']} it, {r['solve_ms']:.1f} ms, relres {r['relres_fp64']:.2e}"
+                + (f", |x - x_fp64|/|x_fp64| {r['error_vs_fp64_solution']:.1e}"
+                   if "error_vs_fp64_solution" in r else ""))
         solves["speedup_mixed_vs_fp64"] = solves["jacobi_pcg_fp64"]["solve_ms"] / solves["mixed_pcg"]["solve_ms"]
         solves["tolerance"] = 1e-8
+        if not args.no_cpu_baseline and args.cpu_solve_reps > 0:
+            solves["cpu_jacobi_pcg_fp64"] = cpu_solve_baseline(A, args, bsr, sysm, N)
+            solves["speedup_gpu_fp64_vs_cpu_all_cores"] = (solves["cpu_jacobi_pcg_fp64"]["all_cores"]["solve_s"] * 1e3
+                                                         / solves["jacobi_pcg_fp64"]["solve_ms"])
         out["solve"] = solves
 
     # ---- heterogeneous Poisson with the learned preconditioner -------------------------------

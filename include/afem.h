@@ -21,7 +21,8 @@
  *  - Handles are not thread-safe (as DoFLinearSystem::matrixAddValue is not,
  *    femutils/AlephDoFLinearSystem.cc:125-135).
  *  - Local ids: node lid i has DoFs i*nb_dof + k (femutils/FemDoFsOnNodes.cc:101-112);
- *    row arrays carry nb_row+1 offsets.
+ *    the library's own BSR row arrays carry nb_row+1 offsets; a CSR view
+ *    handed to afem_system_set_csr_values carries the reference's nb_row.
  */
 #ifndef AFEM_H
 #define AFEM_H
@@ -65,6 +66,12 @@ AFEM_API int afem_memcpy_htod(void* dst_device, const void* src_host, size_t byt
 AFEM_API int afem_memcpy_dtoh(void* dst_host, const void* src_device, size_t bytes);
 AFEM_API int afem_memcpy_dtod(void* dst_device, const void* src_device, size_t bytes);
 AFEM_API int afem_memset(void* dptr, int value, size_t bytes);
+/* Device-to-device copy by the library's own streaming kernel (16-byte
+ * non-temporal loads and stores, grid-stride) on the library stream, timed
+ * under the profile name "stream_copy": the measured HBM bandwidth the
+ * roofline figures are set beside.  dst and src 16-byte aligned, bytes a
+ * multiple of 16.  No counterpart in the reference. */
+AFEM_API int afem_stream_copy(void* dst, const void* src, size_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the library stream around each
  * launch.  query returns the number of recorded launches of the named kernel
@@ -173,9 +180,10 @@ AFEM_API int afem_bsr_assemble_heat(afem_bsr_t bsr, const double* cell_lambda, d
 
 /* Same with the 12x12 P1 linear-elasticity element
  * (modules/elasticity/ElementMatrix.h:151-301) into 3x3 blocks; nb_dof = 3.
- * One thread per nonzero block sums the block over the cells of its edge;
- * assembly variant 0 selects the row-per-thread kernel with a column search
- * instead.  On a triangle mesh with nb_dof = 2: the 6x6 2D element
+ * One thread per nonzero block sums the block over the cells of its edge,
+ * from cell geometry staged once per workgroup in LDS; assembly variant 0
+ * selects the row-per-thread kernel with a column search instead, variant 1
+ * the block kernel that rebuilds the geometry for every (block, cell) use.  On a triangle mesh with nb_dof = 2: the 6x6 2D element
  * (modules/elasticity/ElementMatrix.h:41-58), one thread per node's two rows;
  * a linear system takes such a matrix in the per-row (CSR) value order. */
 AFEM_API int afem_bsr_assemble_elasticity(afem_bsr_t bsr, double lame_lambda, double lame_mu);
@@ -202,7 +210,8 @@ AFEM_API int afem_bsr_to_csr(afem_bsr_t bsr, const int32_t** csr_row, const int3
  * = the manufactured source integrated against N_n, written (not added).
  * solution: 0 sine (2 pi^2 sin(pi x) sin(pi y)), 1 poly
  * (2 (x(1-x) + y(1-y))), 2 quadratic (-4), ManufacturedSolutions.h:91-142.
- * Triangles: source(barycentre) |T| / 3; Quad4 / Hexa8: 2x2(x2) Gauss.
+ * Triangles: source(barycentre) |T| / 3; Quad4: 2x2 Gauss; Hexa8: 3x3x3 Gauss
+ * (ManufacturedSolutions.cc:22-40).
  * Tetrahedra: AFEM_ERR_NOT_IMPLEMENTED (the reference has no branch). */
 AFEM_API int afem_rhs_manufactured(afem_mesh_t mesh, int solution, int nb_dof, int dof_index, double* rhs);
 
@@ -210,7 +219,7 @@ AFEM_API int afem_rhs_manufactured(afem_mesh_t mesh, int solution, int nb_dof, i
  * (modules/poisson/ManufacturedSolutions.cc:270-420): *l2_error [host] =
  * sqrt(sum over cells of the squared error of the nodal solution u [device,
  * one value per node]): triangles at the barycentre (mean of u) times |T|,
- * Quad4 / Hexa8 at the 2x2(x2) Gauss points times det J.  The per-cell terms
+ * Quad4 at the 2x2, Hexa8 at the 3x3x3 Gauss points times det J w_g.  The per-cell terms
  * are summed on the host in cell order. */
 AFEM_API int afem_manufactured_l2_error(afem_mesh_t mesh, int solution, const double* u, double* l2_error);
 
@@ -277,12 +286,25 @@ enum afem_dirichlet_method {
  * zero-initialised. */
 AFEM_API int afem_system_create(int32_t nb_dof, afem_system_t* sys);
 AFEM_API int afem_system_destroy(afem_system_t sys);
-/* IDoFLinearSystemImpl::setCSRValues: stores a non-owning view of device
- * arrays that must stay valid until solve (femutils/DoFLinearSystem.h:318-325).
- * row_index has nb_row+1 entries. */
+/* IDoFLinearSystemImpl::setCSRValues(CSRFormatView) with the four spans of
+ * the reference's view (femutils/CsrFormatMatrixView.h:145-187):
+ *   rows [device] nb_row row starts, the view's rows(); the last row ends at
+ *     nnz (= nbColumn()), as CsrFormatMatrixView::rowRange takes it.  nb_row
+ *     is nb_dof, or nb_dof + 1 when the array carries a closing entry equal
+ *     to nnz (the scalar expansion of BSRMatrix::toCsr for blocks > 1,
+ *     femutils/BSRFormat.cc:117-139);
+ *   rows_nb_column [device] nb_dof per-row counts (nbColumnForRow) or NULL;
+ *     when given they are checked against the row extents;
+ *   columns [device] nnz column ids in [0, nb_dof); values [device] nnz.
+ * columns and values are kept as a non-owning view that must stay valid until
+ * solve (femutils/DoFLinearSystem.h:318-325); the row starts are copied (and
+ * closed with nnz) into an array the system owns.  Synchronous: a view whose
+ * starts are not non-decreasing from 0, whose counts or closing entry
+ * disagree, or whose columns leave the DoF family fails with
+ * AFEM_ERR_ARGUMENT. */
 AFEM_API int afem_system_set_csr_values(afem_system_t sys, int32_t nb_row, int64_t nnz,
-                                        const int32_t* row_index, const int32_t* columns,
-                                        double* values);
+                                        const int32_t* rows, const int32_t* rows_nb_column,
+                                        const int32_t* columns, double* values);
 /* setCSRValues from a BSR handle (BSRFormat::toLinearSystem, femutils/BSRFormat.cc:377-395).
  * With nb_block > 1 the solver keeps the block structure for its SpMV and
  * block-Jacobi preconditioner. */

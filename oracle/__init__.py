@@ -59,6 +59,10 @@ def lib():
         L.afo_assemble.argtypes = [ctypes.c_int, i32, vp, i32, i32, vp, vp, vp,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, dbl, dbl, vp]
         L.afo_bsr_to_csr.argtypes = [i32, ctypes.c_int, vp, vp, vp, vp]
+        L.afo_find_value_index.restype = i64
+        L.afo_pcg_jacobi_threads.restype = ctypes.c_int
+        L.afo_pcg_jacobi_threads.argtypes = [i32, vp, vp, vp, vp, vp, dbl, ctypes.c_int, ctypes.c_int, vp]
+        L.afo_find_value_index.argtypes = [i32, ctypes.c_int, ctypes.c_int, vp, i64, vp, vp, i32, i32]
         L.afo_rhs_constant_source.argtypes = [ctypes.c_int, vp, i32, i32, vp, vp, dbl,
                                               ctypes.c_int, ctypes.c_int, vp]
         L.afo_rhs_body_force.argtypes = [vp, i32, vp, vp, vp]
@@ -388,6 +392,26 @@ def apply_transformations(row, cols, vals, rhs, forced_info=None, forced_value=N
     return vals, rhs
 
 
+def find_value_index(rows, cols, row, col, nb_block=1, order_per_block=True, nnz_per_row=None):
+    """Value index of scalar entry (row, col) as BSRMatrix::findValueIndex
+    (femutils/BSRFormat.cc:79-106) finds it: rows holds one start per block
+    row (the reference's layout, no closing entry); -1 when absent."""
+    rows, cols = _i32(rows), _i32(cols)
+    nzr = _i32(nnz_per_row) if nnz_per_row is not None else None
+    return int(lib().afo_find_value_index(len(rows), nb_block, 1 if order_per_block else 0, _P(rows), len(cols),
+                                          _P(cols), _P(nzr), int(row), int(col)))
+
+
+def matrix_add_value(rows, cols, vals, row, col, value, set_value=False, **kw):
+    """matrixAddValue / matrixSetValue (femutils/internal/CsrDoFLinearSystemImpl.h:56-63)
+    in place on vals; returns False when the entry is not in the pattern."""
+    k = find_value_index(rows, cols, row, col, **kw)
+    if k < 0:
+        return False
+    vals[k] = value if set_value else vals[k] + value
+    return True
+
+
 def spmv(row, cols, vals, x):
     row, cols, vals, x = _i32(row), _i32(cols), _f64(vals), _f64(x)
     y = np.zeros(len(row) - 1)
@@ -400,6 +424,17 @@ def pcg_jacobi(row, cols, vals, b, rtol=1e-10, max_iter=100000, x0=None):
     x = np.zeros(len(b)) if x0 is None else np.array(x0, dtype=np.float64)
     rr = np.zeros(1)
     it = lib().afo_pcg_jacobi(len(b), _P(row), _P(cols), _P(vals), _P(b), _P(x), rtol, max_iter, _P(rr))
+    return x, it, float(rr[0])
+
+
+def pcg_jacobi_threads(row, cols, vals, b, nthreads, rtol=1e-10, max_iter=100000, x0=None):
+    """Jacobi-PCG on nthreads OpenMP threads (CPU timing baseline); returns
+    (x, iterations or -1 when max_iter was reached, true relative residual)."""
+    row, cols, vals, b = _i32(row), _i32(cols), _f64(vals), _f64(b)
+    x = np.zeros(len(b)) if x0 is None else np.array(x0, dtype=np.float64)
+    rr = np.zeros(1)
+    it = lib().afo_pcg_jacobi_threads(len(b), _P(row), _P(cols), _P(vals), _P(b), _P(x), rtol, max_iter,
+                                      int(nthreads), _P(rr))
     return x, it, float(rr[0])
 
 

@@ -668,6 +668,38 @@ void afo_bsr_to_csr(int32_t nb_node, int nb, const int32_t* row_index, const int
           csr_col[o++] = columns[b] * nb + k;
 }
 
+/* BSRMatrix::findValueIndex (femutils/BSRFormat.cc:79-106) over the
+ * reference's row array of nb_row starts (the last row ends at nb_col); for
+ * nb_block = 1 this is CsrDoFLinearSystemImpl::indexValue
+ * (femutils/internal/CsrDoFLinearSystemImpl.h:46-53).  Returns the value
+ * index, or -1 when (row, col) is not in the pattern (where the reference
+ * throws or indexes values[-1]). */
+int64_t afo_find_value_index(int32_t nb_row, int nb_block, int order_per_block, const int32_t* rows,
+                             int64_t nb_col, const int32_t* columns, const int32_t* nnz_per_row, int32_t row,
+                             int32_t col)
+{
+  const int32_t block_row = row / nb_block;
+  const int32_t block_col = col / nb_block;
+  int64_t block_start = rows[block_row];
+  const int64_t block_end = (block_row == nb_row - 1) ? nb_col : rows[block_row + 1];
+  const int row_offset = row % nb_block;
+  const int col_offset = col % nb_block;
+  const int64_t nb_block_sq = (int64_t)nb_block * nb_block;
+  const int64_t block_start_in_value = block_start * nb_block_sq;
+  int64_t col_index = 0;
+  while (block_start < block_end) {
+    if (columns[block_start] == block_col) {
+      if (!order_per_block)
+        return block_start_in_value + (int64_t)nb_block * col_index +
+               (int64_t)row_offset * nb_block * nnz_per_row[block_row] + col_offset;
+      return block_start * nb_block_sq + (int64_t)row_offset * nb_block + col_offset;
+    }
+    ++block_start;
+    ++col_index;
+  }
+  return -1;
+}
+
 /* ---- right-hand side ------------------------------------------------------------ */
 
 /* femutils/ArcaneFemFunctions.h:1601-1611 (tet) and :2531-2541 (tri) */
@@ -1043,9 +1075,16 @@ double afo_manufactured_source(int which, const double* x)
   return -4.0;
 }
 
+/* 3-point Gauss rule of the Hexa8 manufactured-solution integrals
+ * (modules/poisson/ManufacturedSolutions.cc:29-40): abscissae 0 and
+ * +-sqrt(3/5), weights 8/9 and 5/9. */
+static const double gp3[3] = { -0.77459666924148337704, 0.0, 0.77459666924148337704 };
+static const double gw3[3] = { 5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0 };
+
 /* modules/poisson/ManufacturedSolutions.cc:66-240 (_applyManufacturedSource):
- * tri3 source(barycentre) |T| / 3 per node; Quad4 / Hexa8 sum over the Gauss
- * points of N_i source(x_g) det J (x_g = sum N_a x_a).  Returns -1 for other
+ * tri3 source(barycentre) |T| / 3 per node; Quad4 (2x2 Gauss, weight 1) /
+ * Hexa8 (3x3x3 Gauss, xi slowest) sum over the Gauss points of
+ * N_i source(x_g) det J w_g (x_g = sum N_a x_a).  Returns -1 for other
  * cells (the reference has no branch for them). */
 int afo_rhs_manufactured(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
                          int which, double* rhs)
@@ -1069,27 +1108,30 @@ int afo_rhs_manufactured(int dim, const double* coords, int32_t nb_cell, int32_t
         rhs[cn[k]] += src * area / 3;
       continue;
     }
-    const int ng = dim == 2 ? 1 : 2;
-    for (int ix = 0; ix < 2; ++ix)
-      for (int iy = 0; iy < 2; ++iy)
-        for (int iz = 0; iz < ng; ++iz) {
+    const int np = dim == 2 ? 2 : 3;
+    for (int ix = 0; ix < np; ++ix)
+      for (int iy = 0; iy < np; ++iy)
+        for (int iz = 0; iz < (dim == 2 ? 1 : 3); ++iz) {
           double N[8], gx[8], gy[8], gz[8];
-          const double det = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+          const double* p = dim == 2 ? gp : gp3;
+          const double w = dim == 2 ? 1.0 : gw3[ix] * gw3[iy] * gw3[iz];
+          const double det = afo_q1_gradients(dim, X, p[ix], p[iy], dim == 2 ? 0.0 : p[iz], N, gx, gy, gz);
           double xg[3] = { 0, 0, 0 };
           for (int a = 0; a < npc; ++a)
             for (int d = 0; d < 3; ++d)
               xg[d] += N[a] * X[3 * a + d];
           const double src = afo_manufactured_source(which, xg);
           for (int a = 0; a < npc; ++a)
-            rhs[cn[a]] += N[a] * src * (det * 1.0);
+            rhs[cn[a]] += N[a] * src * (det * w);
         }
   }
   return 0;
 }
 
 /* _computeManufacturedL2Error (ManufacturedSolutions.cc:270-420): tri3 at the
- * barycentre (uh the mean of the nodal values) times |T|; Quad4 / Hexa8 at the
- * Gauss points; returns sqrt of the sum. */
+ * barycentre (uh the mean of the nodal values) times |T|; Quad4 at the 2x2,
+ * Hexa8 at the 3x3x3 Gauss points (the rules of the source above); returns
+ * sqrt of the sum. */
 double afo_manufactured_l2_error(int dim, const double* coords, int32_t nb_cell, int32_t npc, const int32_t* cells,
                                  int which, const double* u)
 {
@@ -1110,12 +1152,14 @@ double afo_manufactured_l2_error(int dim, const double* coords, int32_t nb_cell,
       e2 += err * err * area;
       continue;
     }
-    const int ng = dim == 2 ? 1 : 2;
-    for (int ix = 0; ix < 2; ++ix)
-      for (int iy = 0; iy < 2; ++iy)
-        for (int iz = 0; iz < ng; ++iz) {
+    const int np = dim == 2 ? 2 : 3;
+    for (int ix = 0; ix < np; ++ix)
+      for (int iy = 0; iy < np; ++iy)
+        for (int iz = 0; iz < (dim == 2 ? 1 : 3); ++iz) {
           double N[8], gx[8], gy[8], gz[8];
-          const double det = afo_q1_gradients(dim, X, gp[ix], gp[iy], dim == 2 ? 0.0 : gp[iz], N, gx, gy, gz);
+          const double* p = dim == 2 ? gp : gp3;
+          const double w = dim == 2 ? 1.0 : gw3[ix] * gw3[iy] * gw3[iz];
+          const double det = afo_q1_gradients(dim, X, p[ix], p[iy], dim == 2 ? 0.0 : p[iz], N, gx, gy, gz);
           double xg[3] = { 0, 0, 0 }, uh = 0.0;
           for (int a = 0; a < npc; ++a) {
             uh += N[a] * u[cn[a]];
@@ -1123,7 +1167,7 @@ double afo_manufactured_l2_error(int dim, const double* coords, int32_t nb_cell,
               xg[d] += N[a] * X[3 * a + d];
           }
           const double err = uh - afo_manufactured_exact(which, xg);
-          e2 += err * err * det * 1.0;
+          e2 += err * err * det * w;
         }
   }
   return sqrt(e2);
@@ -1278,6 +1322,89 @@ int afo_pcg_jacobi(int32_t n, const int32_t* row_index, const int32_t* columns,
   double rs = 0.0;
   for (int32_t i = 0; i < n; ++i)
     rs += (b[i] - q[i]) * (b[i] - q[i]);
+  if (true_relres)
+    *true_relres = bnorm > 0 ? sqrt(rs) / bnorm : sqrt(rs);
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  free(dinv);
+  return converged ? it : -1;
+}
+
+/* The same Jacobi-PCG on `nthreads` OpenMP threads (rows split statically;
+ * the dot products are OpenMP reductions, so their summation order and the
+ * iteration count can differ from the serial version in the last bits).
+ * Timing baseline only: the CPU counterpart of the PETSc cg + jacobi solve
+ * (femutils/PetscDoFLinearSystemFactory.axl:5-10) on all host cores. */
+int afo_pcg_jacobi_threads(int32_t n, const int32_t* row_index, const int32_t* columns, const double* values,
+                           const double* b, double* x, double rtol, int max_iter, int nthreads,
+                           double* true_relres)
+{
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* z = (double*)malloc(sizeof(double) * (size_t)n);
+  double* p = (double*)malloc(sizeof(double) * (size_t)n);
+  double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dinv = (double*)malloc(sizeof(double) * (size_t)n);
+  double bb = 0.0, rho = 0.0, rr = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : bb, rho, rr)
+  for (int32_t i = 0; i < n; ++i) {
+    double d = 0.0, ax = 0.0;
+    for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k) {
+      if (columns[k] == i)
+        d = values[k];
+      ax += values[k] * x[columns[k]];
+    }
+    dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
+    r[i] = b[i] - ax;
+    z[i] = dinv[i] * r[i];
+    p[i] = z[i];
+    bb += b[i] * b[i];
+    rho += r[i] * z[i];
+    rr += r[i] * r[i];
+  }
+  const double bnorm = sqrt(bb);
+  int it = 0;
+  int converged = (bnorm == 0.0) || (sqrt(rr) <= rtol * bnorm);
+  while (!converged && it < max_iter) {
+    double pq = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : pq)
+    for (int32_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k)
+        s += values[k] * p[columns[k]];
+      q[i] = s;
+      pq += p[i] * s;
+    }
+    const double alpha = rho / pq;
+    double rho_new = 0.0, rn = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rho_new, rn)
+    for (int32_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      z[i] = dinv[i] * r[i];
+      rho_new += r[i] * z[i];
+      rn += r[i] * r[i];
+    }
+    ++it;
+    if (sqrt(rn) <= rtol * bnorm) {
+      converged = 1;
+      break;
+    }
+    const double beta = rho_new / rho;
+    rho = rho_new;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int32_t i = 0; i < n; ++i)
+      p[i] = z[i] + beta * p[i];
+  }
+  double rs = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rs)
+  for (int32_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int32_t k = row_index[i]; k < row_index[i + 1]; ++k)
+      s += values[k] * x[columns[k]];
+    rs += (b[i] - s) * (b[i] - s);
+  }
   if (true_relres)
     *true_relres = bnorm > 0 ? sqrt(rs) / bnorm : sqrt(rs);
   free(r);

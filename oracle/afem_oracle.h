@@ -130,6 +130,13 @@ int afo_assemble(int dim, int32_t nb_node, const double* coords /* nb_node x 3 *
 void afo_bsr_to_csr(int32_t nb_node, int nb, const int32_t* row_index, const int32_t* columns,
                     int32_t* csr_row, int32_t* csr_col);
 
+/* BSRMatrix::findValueIndex / CsrDoFLinearSystemImpl::indexValue over the
+ * reference's nb_row-entry row array (last row ends at nb_col): the value
+ * index of (row, col) in per-block or per-row order, -1 when absent. */
+int64_t afo_find_value_index(int32_t nb_row, int nb_block, int order_per_block, const int32_t* rows,
+                             int64_t nb_col, const int32_t* columns, const int32_t* nnz_per_row, int32_t row,
+                             int32_t col);
+
 /* Laplace values (lambda = 1) and the constant-source RHS f*|T|/npc on `nthreads`
  * threads with atomic updates (timing baseline; see afem_oracle.c). */
 int afo_assemble_laplace_threads(int32_t nb_node, const double* coords, int32_t nb_cell, int32_t npc,
@@ -241,6 +248,12 @@ void afo_spmv(int32_t nb_row, const int32_t* row_index, const int32_t* columns,
 int afo_pcg_jacobi(int32_t nb_row, const int32_t* row_index, const int32_t* columns,
                    const double* values, const double* b, double* x,
                    double rtol, int max_iter, double* true_relres);
+
+/* afo_pcg_jacobi on nthreads OpenMP threads (reduction order may differ):
+ * the all-cores CPU timing baseline of the solve. */
+int afo_pcg_jacobi_threads(int32_t n, const int32_t* row_index, const int32_t* columns, const double* values,
+                           const double* b, double* x, double rtol, int max_iter, int nthreads,
+                           double* true_relres);
 
 /* ---- result comparison ------------------------------------------------------- */
 

@@ -132,7 +132,8 @@ def test_assembly_autotune_keeps_parity(afem_gpu):
     assert row_scaled_error(row, O.assemble(3, X, C, 4, row, col), val) <= ROW_TOL
 
 
-@pytest.mark.parametrize("variant", [-1, 0])  # per-block kernel, column-search kernel
+# geometry-once block kernel (default), column-search kernel, per-use-geometry block kernel
+@pytest.mark.parametrize("variant", [-1, 0, 1])
 @pytest.mark.parametrize("per_block", [True, False])
 @pytest.mark.parametrize("case", ["box3", "bar_3D"])
 def test_elasticity_assembly(afem_gpu, per_block, case, variant):

@@ -462,21 +462,62 @@ def test_poisson_module_manufactured_tolerance_failure():
 # modules/elasticity/CMakeLists.txt 2D_Dirichlet_cartesian / 3D_Dirichlet_cartesian
 # (inputs/bar.{2D,3D}.cartesian.Dirichlet.bodyForce.arc): Arcane's Cartesian
 # generator, 10 x 2 (x 2) cells of a 1.0 x 0.1 (x 0.04) bar, f = (-1, NULL),
-# XMIN clamped, XMAX u_y = 1.  Min value 1e-9 in 3D: three |u_z| <= 5.4e-10
-# values of that golden file are cancellation residue of the reference's
-# iterative solve (1e-12 from the direct solution in absolute terms).
+# XMIN clamped, XMAX u_y = 1, checked with the reference's epsilon 1e-3 and
+# min value 1e-10 (modules/elasticity/FemModule.cc:384).
 CARTESIAN = [
     (["--cartesian", "10,2", "1.0,0.1", "--f", "-1.0,NULL", "--dirichlet", "XMIN=0.0,0.0",
-      "--dirichlet", "XMAX=NULL,1.0"], "bar.2D.cartesian.Dirichlet.bodyForce.txt", "1e-10"),
+      "--dirichlet", "XMAX=NULL,1.0"], "bar.2D.cartesian.Dirichlet.bodyForce.txt", 2),
     (["--cartesian", "10,2,2", "1.0,0.1,0.04", "--f", "-1.0,NULL,NULL", "--dirichlet", "XMIN=0.0,0.0,0.0",
-      "--dirichlet", "XMAX=NULL,1.0,NULL"], "bar.3D.cartesian.Dirichlet.bodyForce.txt", "1e-9"),
+      "--dirichlet", "XMAX=NULL,1.0,NULL"], "bar.3D.cartesian.Dirichlet.bodyForce.txt", 3),
 ]
+
+# The only values of the 3D golden file outside the reference's comparator:
+# u_z of uids 19-21 (y mid-plane, |u_z| <= 5.4e-10 while |u| reaches 1).
+# They are allowed an absolute difference of 1e-11 * max |u|: cancellation
+# residue of an iterative solve at that scale, not a relative 1e-3 match.
+CARTESIAN_ABS_ONLY = {(19, 2), (20, 2), (21, 2)}
+
+
+def _printed_solution(stdout):
+    out = {}
+    for line in stdout.splitlines():
+        if line.startswith("u["):
+            uid = int(line[2:line.index("]")])
+            out[uid] = np.array([float(t) for t in line.split("=", 1)[1].split()])
+    return out
+
+
+def _golden(path, ncomp):
+    ref = {}
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if t:
+                ref[int(t[0])] = np.array([float(v) for v in t[1:1 + ncomp]])
+    return ref
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("args,ref,min_value", CARTESIAN, ids=["2D", "3D"])
-def test_elasticity_module_cartesian_golden(args, ref, min_value):
+@pytest.mark.parametrize("args,ref,dim", CARTESIAN, ids=["2D", "3D"])
+def test_elasticity_module_cartesian_golden(args, ref, dim):
+    """femutils/FemUtils.cc:105-171 at epsilon 1e-3, min value 1e-10: every
+    value passes except the listed 3D residue entries, which must agree to
+    1e-11 * max |u| in absolute terms."""
     print(f"[case] cartesian {ref}", flush=True)
-    r = run(["elasticity", "--E", "21e5", "--nu", "0.28", "--rtol", "1e-12", "--check", os.path.join(GOLDEN, ref),
-             "--epsilon", "1e-3", "--min-value", min_value] + args)
-    _check_ok(r)
+    r = run(["elasticity", "--E", "21e5", "--nu", "0.28", "--rtol", "1e-12", "--print"] + args)
+    assert r.returncode == 0 and "converged=1" in r.stdout, r.stdout + r.stderr
+    got = _printed_solution(r.stdout)
+    gold = _golden(os.path.join(GOLDEN, ref), dim)
+    assert set(gold) <= set(got)
+    umax = max(np.abs(v).max() for v in gold.values())
+    failing = set()
+    for uid, a in gold.items():
+        for c in range(dim):
+            x, y = a[c], got[uid][c]
+            if (abs(x) < 1e-10 and abs(y) < 1e-10) or x == y:
+                continue
+            if not abs(x - y) / (abs(x) + abs(y)) < 1e-3:
+                failing.add((uid, c))
+                assert (uid, c) in CARTESIAN_ABS_ONLY, (uid, c, x, y)
+                assert abs(x - y) <= 1e-11 * umax, (uid, c, x, y)
+    assert failing <= CARTESIAN_ABS_ONLY

@@ -279,6 +279,45 @@ def test_elasticity_assembly_orders_agree_with_dense():
     assert np.array_equal(np.sort(ccol[crow[5]:crow[6]]), ref.indices[ref.indptr[5]:ref.indptr[6]])
 
 
+@pytest.mark.parametrize("nb,per_block", [(1, True), (3, True), (3, False)])
+def test_find_value_index_bruteforce(nb, per_block):
+    """findValueIndex over the reference's nb_row-entry row array: every
+    scalar (row, col) of the dense matrix is found at the value whose scipy
+    reading is that entry, and every entry outside the pattern gives -1
+    (the last row included, whose end is the value count)."""
+    X, C = box(2, 3)
+    N = X.shape[0]
+    row, col = O.pattern(N, C, 3)
+    vals = RNG.normal(size=len(col) * nb * nb)
+    nzr = np.diff(row)
+    if per_block:
+        D = sp.bsr_matrix((vals.reshape(-1, nb, nb), col, row), shape=(nb * N, nb * N)).toarray()
+        S = sp.bsr_matrix((np.ones((len(col), nb, nb)), col, row), shape=(nb * N, nb * N)).toarray() != 0
+    else:
+        crow, ccol = O.bsr_to_csr(N, nb, row, col)
+        D = csr_to_dense(crow, ccol, vals, nb * N)
+        S = csr_to_dense(crow, ccol, np.ones(len(ccol)), nb * N) != 0
+    starts = row[:-1].copy()  # the reference's layout: no closing entry
+    for r in range(nb * N):
+        for c in range(nb * N):
+            k = O.find_value_index(starts, col, r, c, nb_block=nb, order_per_block=per_block, nnz_per_row=nzr)
+            if S[r, c]:
+                assert k >= 0 and vals[k] == D[r, c]
+            else:
+                assert k == -1
+    # matrixAddValue / matrixSetValue on the last row
+    r, c = nb * N - 1, nb * N - 1
+    v = vals.copy()
+    assert O.matrix_add_value(starts, col, v, r, c, 2.0, nb_block=nb, order_per_block=per_block, nnz_per_row=nzr)
+    k = O.find_value_index(starts, col, r, c, nb_block=nb, order_per_block=per_block, nnz_per_row=nzr)
+    assert v[k] == vals[k] + 2.0
+    assert O.matrix_add_value(starts, col, v, r, c, 7.0, set_value=True, nb_block=nb, order_per_block=per_block,
+                              nnz_per_row=nzr)
+    assert v[k] == 7.0
+    assert not O.matrix_add_value(starts, col, v, 0, nb * N - 1, 1.0, nb_block=nb, order_per_block=per_block,
+                                  nnz_per_row=nzr)
+
+
 def test_rhs_sources_integrate_to_domain_measure():
     for dim, n in [(2, 6), (3, 4)]:
         X, C = box(dim, n)
@@ -383,6 +422,24 @@ def test_spmv_and_pcg_against_scipy():
     assert it > 0 and rr < 1e-9
     direct = spl.spsolve(S.tocsc(), rhs)
     assert np.abs(sol - direct).max() < 1e-8 * np.abs(direct).max()
+
+
+def test_threaded_pcg_matches_serial():
+    """The all-cores CPU baseline solve is the serial Jacobi-PCG (pinned
+    against scipy above) up to the summation order of its reductions."""
+    X, C = box(3, 10, 0.15)
+    N = X.shape[0]
+    row, col = O.pattern(N, C, 4)
+    vals = O.assemble(3, X, C, 4, row, col)
+    b = O.rhs_constant_source(3, X, C, 4, 1.0)
+    ei = np.zeros(N, np.uint8)
+    ei[X[:, 2] == 0.0] = O.ELIMINATE_ROW
+    O.apply_transformations(row, col, vals, b, elim_info=ei, elim_value=np.zeros(N))
+    x1, it1, r1 = O.pcg_jacobi(row, col, vals, b, rtol=1e-10)
+    for nt in (1, 3):
+        x2, it2, r2 = O.pcg_jacobi_threads(row, col, vals, b, nt, rtol=1e-10)
+        assert abs(it1 - it2) <= 1 and r2 <= 1e-10
+        assert np.abs(x1 - x2).max() <= 1e-9 * np.abs(x1).max()
 
 
 def test_count_differences_semantics():
@@ -1268,17 +1325,31 @@ def test_manufactured_rhs_integrates_the_source():
             X, C = mk(n)
             errs.append(abs(O.rhs_manufactured(2, X, C, npc, "sine").sum() - 8.0))
         assert errs[1] < 0.05 and 0.85 * ratio < errs[0] / errs[1] < 1.15 * ratio, errs
+    # Hexa8: the 3-point rule errs by O(h^6) (the 2-point rule by O(h^4))
+    errs = []
+    for n in (4, 8):
+        X, C = _box_q1(3, n)
+        errs.append(abs(O.rhs_manufactured(3, X, C, 8, "sine").sum() - 8.0))
+    assert errs[1] < 1e-5 and 0.85 * 64 < errs[0] / errs[1] < 1.15 * 64, errs
 
 
 def test_manufactured_l2_error_closed_forms():
     # nodal values of x^2 + y^2 on a uniform Q1 grid: the bilinear interpolant
-    # exceeds x^2 by (x - x0)(x1 - x) = h^2 t (1 - t) = h^2 / 6 at both Gauss
-    # abscissae t = (1 -+ 1/sqrt 3) / 2, so the error is h^2 / 3 everywhere
-    # and the L2 error over the unit square / cube is h^2 / 3
-    for dim, n in [(2, 6), (3, 3)]:
-        X, C = _box_q1(dim, n)
+    # exceeds x^2 by (x - x0)(x1 - x) = h^2 t (1 - t).  Quad4 (2x2 Gauss): that
+    # is h^2 / 6 at both abscissae t = (1 -+ 1/sqrt 3) / 2, so the error is
+    # h^2 / 3 at every point and the L2 error over the unit square is h^2 / 3.
+    X, C = _box_q1(2, 6)
+    u = X[:, 0] ** 2 + X[:, 1] ** 2
+    assert abs(O.manufactured_l2_error(2, X, C, 4, "quadratic", u) - (1 / 6) ** 2 / 3) < 1e-14
+    # Hexa8 (3x3x3 Gauss, exact for the quartic error): per cell
+    # h int int (x(h-x) + y(h-y))^2 = h (2 h^6/30 + 2 (h^3/6)^2) = 11 h^7 / 90,
+    # so the L2 error over the unit cube is h^2 sqrt(11/90) (the 2-point
+    # rule would give h^2 / 3)
+    for n in (3, 4):
+        X, C = _box_q1(3, n)
         u = X[:, 0] ** 2 + X[:, 1] ** 2
-        assert abs(O.manufactured_l2_error(dim, X, C, C.shape[1], "quadratic", u) - (1 / n) ** 2 / 3) < 1e-14
+        ref = (1 / n) ** 2 * np.sqrt(11 / 90)
+        assert abs(O.manufactured_l2_error(3, X, C, 8, "quadratic", u) - ref) < 1e-14
     # triangles: mean of x_i^2 minus (mean x_i)^2 is the variance of the
     # vertex abscissae
     X, C = _box_tri(5, 0.25)

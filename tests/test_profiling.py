@@ -55,3 +55,18 @@ def test_pmc_summary_needs_calibration(tmp_path):
     write_csv(w, "WRITE_SIZE", [("void afem::k(int)", 64, 1.0)])
     with pytest.raises(SystemExit):
         P.pmc_summary(str(f), str(w), 256 << 20)
+
+
+def test_counter_table(tmp_path):
+    kern = "void afem::k_assemble_laplace_brick<4, 16>(double const*, int)"
+    other = "void afem::k_spmv_sell16<double, double, 0, 4, 256>(int)"
+    a = tmp_path / "a.csv"
+    b = tmp_path / "b.csv"
+    write_csv(a, "SQ_INSTS_VALU", [(kern, 64, 10.0), (kern, 64, 30.0), (kern, 64, 20.0), (other, 64, 5.0)])
+    write_csv(b, "SQ_LDS_BANK_CONFLICT", [(kern, 64, 7.0), (P.COPY_KERNEL, 64, 1.0)])
+    t = P.counter_table([str(a), str(b)], match="assemble")
+    assert set(t) == {"k_assemble_laplace_brick<4, 16>"}
+    assert t["k_assemble_laplace_brick<4, 16>"]["SQ_INSTS_VALU"] == (20.0, 3)
+    assert t["k_assemble_laplace_brick<4, 16>"]["SQ_LDS_BANK_CONFLICT"] == (7.0, 1)
+    md = P.counters_markdown(t, "cmd")
+    assert "SQ_INSTS_VALU" in md and "| 3 |" in md
