@@ -164,8 +164,10 @@ class Mesh
       return s;
     };
     auto take = [&](void* dst, size_t n) {
-      if (p + n > data.size())
+      if (n > data.size() - p)
         throw FatalError(AFEM_ERR_ARGUMENT, "truncated mesh file");
+      if (n == 0)
+        return; // dst may be the null data() of an empty vector
       std::memcpy(dst, data.data() + p, n);
       p += n;
     };
@@ -178,6 +180,24 @@ class Mesh
       uint64_t v;
       take(&v, 8);
       return v;
+    };
+    // a count of `each`-byte records read next must fit in what is left
+    auto fits = [&](uint64_t count, uint64_t each) {
+      if (count > (data.size() - p) / each)
+        throw FatalError(AFEM_ERR_ARGUMENT, "corrupt mesh file: a record count exceeds the file");
+      return (size_t)count;
+    };
+    auto to_int = [&](const std::string& t) {
+      try {
+        size_t used = 0;
+        const int v = std::stoi(t, &used);
+        if (used == 0)
+          throw std::invalid_argument(t);
+        return v;
+      }
+      catch (const std::exception&) {
+        throw FatalError(AFEM_ERR_ARGUMENT, "corrupt mesh file: '" + t + "' is not a number");
+      }
     };
     std::map<std::pair<int, int>, std::string> names;
     std::map<std::pair<int, int>, std::vector<int>> entity_phys;
@@ -205,11 +225,13 @@ class Mesh
         line(); // $EndMeshFormat
       }
       else if (head == "$PhysicalNames") {
-        int n = std::stoi(line());
+        int n = to_int(line());
         for (int k = 0; k < n; ++k) {
           std::string s = line();
-          size_t a = s.find(' '), b = s.find(' ', a + 1);
-          int d = std::stoi(s.substr(0, a)), t = std::stoi(s.substr(a + 1, b - a - 1));
+          size_t a = s.find(' '), b = a == std::string::npos ? a : s.find(' ', a + 1);
+          if (b == std::string::npos)
+            throw FatalError(AFEM_ERR_ARGUMENT, "corrupt mesh file: physical name line '" + s + "'");
+          int d = to_int(s.substr(0, a)), t = to_int(s.substr(a + 1, b - a - 1));
           std::string nm = s.substr(b + 1);
           if (nm.size() >= 2 && nm.front() == '"')
             nm = nm.substr(1, nm.size() - 2);
@@ -226,13 +248,13 @@ class Mesh
             int tag = i32();
             double box[6];
             take(box, d == 0 ? 24 : 48);
-            uint64_t np = u64();
+            const size_t np = fits(u64(), 4);
             std::vector<int> phys(np);
             for (auto& v : phys)
               v = i32();
             entity_phys[{ d, tag }] = phys;
             if (d > 0) {
-              uint64_t nb = u64();
+              const size_t nb = fits(u64(), 4);
               p += 4 * nb;
             }
           }
@@ -248,9 +270,9 @@ class Mesh
           i32();
           if (i32() != 0)
             throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "parametric Gmsh nodes");
-          uint64_t n = u64();
+          const size_t n = fits(u64(), 32); // a tag and three coordinates per node
           size_t t0 = tags.size();
-          for (uint64_t k = 0; k < n; ++k)
+          for (size_t k = 0; k < n; ++k)
             tags.push_back((Int64)u64());
           xyz.resize((t0 + n) * 3);
           take(xyz.data() + t0 * 3, n * 24);
@@ -274,7 +296,7 @@ class Mesh
           if (it == nodes_of.end())
             throw FatalError(AFEM_ERR_NOT_IMPLEMENTED, "Gmsh element type " + std::to_string(type));
           blk.k = it->second;
-          uint64_t n = u64();
+          const size_t n = fits(u64(), 8 * (uint64_t)(blk.k + 1));
           blk.conn.resize(n * (blk.k + 1));
           take(blk.conn.data(), blk.conn.size() * 8);
           blocks.push_back(std::move(blk));
@@ -294,6 +316,13 @@ class Mesh
       order[k] = k;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tags[a] < tags[b]; });
     std::map<Int64, Int32> lid;
+    auto node_of = [&lid](uint64_t tag) {
+      auto it = lid.find((Int64)tag);
+      if (it == lid.end())
+        throw FatalError(AFEM_ERR_MESH, "corrupt mesh file: an element names node " + std::to_string(tag) +
+                                          ", which the file does not define");
+      return it->second;
+    };
     std::vector<Real> coords(tags.size() * 3);
     std::vector<Int64> uids(tags.size());
     for (size_t k = 0; k < order.size(); ++k) {
@@ -332,7 +361,7 @@ class Mesh
         }
         for (size_t e = 0; e < n; ++e)
           for (int s = 0; s < stride; ++s)
-            cells.push_back(s < b.k ? lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]) : -1);
+            cells.push_back(s < b.k ? node_of(b.conn[e * (b.k + 1) + 1 + s]) : -1);
       }
       else if (b.dim == dim - 1 || (b.dim == 0 && dim > 1)) { // faces, or named points
         for (int ph : entity_phys[{ b.dim, b.tag }]) {
@@ -342,7 +371,7 @@ class Mesh
           auto& g = m_groups[nm->second];
           for (size_t e = 0; e < n; ++e)
             for (int s = 0; s < b.k; ++s)
-              g.push_back(lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]));
+              g.push_back(node_of(b.conn[e * (b.k + 1) + 1 + s]));
           if (b.dim == dim - 1 && (b.k == dim || (dim == 3 && b.k == 4))) {
             // faces in file order: edges (2D), triangles or quadrangles (3D)
             if (m_face_nodes != 0 && m_face_nodes != b.k)
@@ -352,7 +381,7 @@ class Mesh
             for (size_t e = 0; e < n; ++e) {
               Int32 f[4];
               for (int s = 0; s < b.k; ++s)
-                f[s] = lid.at((Int64)b.conn[e * (b.k + 1) + 1 + s]);
+                f[s] = node_of(b.conn[e * (b.k + 1) + 1 + s]);
               if (b.k == 4) { // quadrangles in Arcane's face order (smallest uid first, towards its smaller neighbour)
                 const Int64* u = uids.data();
                 int i0 = 0;
@@ -376,6 +405,8 @@ class Mesh
       std::sort(kv.second.begin(), kv.second.end());
       kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
     }
+    if (cells.empty() || dim < 2)
+      throw FatalError(AFEM_ERR_MESH, "mesh file without 2D or 3D cells");
     build(dim, std::move(coords), std::move(cells), std::move(uids), upload, npc);
   }
 
