@@ -53,7 +53,8 @@ class SolveResult(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("reliable_delta", ctypes.c_double), ("chunk", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("use_initial_guess", ctypes.c_int)]
+                ("use_initial_guess", ctypes.c_int), ("persistent", ctypes.c_int), ("workgroups", ctypes.c_int),
+                ("spin_limit", ctypes.c_longlong), ("timeout_ms", ctypes.c_double)]
 
 
 class BatchResult(ctypes.Structure):
@@ -640,7 +641,8 @@ def union_mesh(meshes):
 
 class BatchSolver:
     """Independent systems on the diagonal blocks of one system's matrix,
-    solved together (segmented mixed-precision PCG, hipGraph chunks)."""
+    solved together (segmented mixed-precision PCG: one persistent launch by
+    default, or hipGraph chunks of iterations with persistent=0)."""
 
     def __init__(self, sysm, segment_rows):
         seg = np.ascontiguousarray(segment_rows, dtype=np.int32)

@@ -32,6 +32,8 @@
 #include <type_traits>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <thread>
 #include <vector>
 
 namespace afem {
@@ -553,6 +555,415 @@ __global__ void k_prepare(int32_t n, const int32_t* __restrict__ rp, const int32
   }
 }
 
+// ---- persistent solve ----------------------------------------------------------------------
+//
+// One launch runs the whole mixed PCG of every active segment.  Segment s
+// owns a team of consecutive workgroups of the launch; workgroup j of the
+// team owns a strip of the segment's rows (a multiple of 64 rows, so a wave
+// of the SELL product is one slice).  The workgroups of a team synchronize
+// through a counter in global memory: every wave arrives with an agent-scope
+// release add, the first thread polls the counter and, once every wave of the
+// team has arrived, takes an agent-scope acquire fence.  The grid never
+// exceeds the number of workgroups the device holds at once, and every wait
+// is bounded: after spin_limit polls, or when the host raises its cancel flag
+// (pinned host memory), the waiting workgroup sets the abort flag, and every
+// workgroup that sees it leaves the kernel; the host then reports an error.
+//
+// The p update is folded into the product, so an iteration has two barriers:
+//   A  beta = rz / rho; p_new = z + beta p_old for the own rows (p is
+//      double-buffered by parity; every gathered column forms the same
+//      z + beta p_old with the same fma); q = A32 p_new; partial p.q
+//   -- barrier --
+//   B  alpha = rho / p.q; x += alpha p_new; r -= alpha q; z = D r; partials r.z, r.r
+//   -- barrier --
+// An iteration that takes a reliable update adds two: r64 = b - A64 x
+// (partial r64.r64) -- barrier -- r = r64, z = D r (partial r.z) -- barrier.
+// All the workgroups of a team reduce the same partials in the same order, so
+// they take the same decisions and leave the loop together.
+
+struct Team
+{
+  const int32_t* seg_of; // segment of each launched workgroup
+  const int32_t* first;  // first workgroup of each segment's team (nb_seg + 1)
+  const int32_t* rows;   // segment row offsets (nb_seg + 1)
+  const int32_t* strip;  // rows per workgroup of each segment's team (multiple of 64)
+  // SELL copy (sell != 0) or CSR arrays
+  int sell;
+  const int32_t* sslice;
+  const int32_t* soff;
+  const int16_t* sdcol;
+  const int32_t* rp;
+  const int32_t* ci;
+  const float* v32;
+  const double* v64;
+  unsigned* cnt;           // per-segment barrier counters, zero at the launch
+  int* abort_flag;         // zero at the launch
+  const int* host_cancel;  // pinned host flag (may be null)
+  long long spin_limit;
+};
+
+struct TeamVec
+{
+  double* x;
+  const double* b;
+  const float* dinv;
+  float* r;
+  float* z;
+  float* p[2];
+  float* q;
+  double* r64;
+};
+
+// one entry per launched workgroup; written by one workgroup, read by its
+// team after a barrier (plain pointers: never cached as read-only)
+struct TeamParts
+{
+  double* pq;
+  double* rz;
+  double* rr;
+  double* rr64;
+  double* rzru;
+  double* bb;
+};
+
+__device__ __forceinline__ bool team_abort(const Team& T)
+{
+  return __hip_atomic_load(T.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// The vectors workgroups of a team exchange (z, both p buffers, x, the
+// partial sums) are written with agent-scope atomic stores and read with
+// agent-scope atomic loads, which go to the device's coherence point instead
+// of the per-CU L1 and per-XCD L2; everything else (the matrix, D^-1, b, the
+// rows a workgroup alone reads and writes) uses ordinary cached accesses.  So
+// the barrier needs neither an L2 write-back (an agent-scope release) nor an
+// L2 invalidation (an agent-scope acquire, which would also drop the cached
+// matrix of every other team on the XCD): every wave waits until its stores
+// are acknowledged, the workgroup synchronizes, and its first thread arrives
+// with one relaxed add and polls the counter; the loads that follow are
+// issued only after the poll has seen every arrival.
+__device__ __forceinline__ void publish(float* p, float v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish(double* p, double v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float fetch(const float* p)
+{
+  return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double fetch(const double* p)
+{
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier of the team of segment s (nw workgroups); `phase` counts this
+// workgroup's barriers.  False: give up (abort).
+__device__ bool team_barrier(const Team& T, int s, int nw, unsigned& phase)
+{
+  __shared__ int s_ok;
+  ++phase;
+  const unsigned target = phase * (unsigned)nw;
+  __atomic_signal_fence(__ATOMIC_SEQ_CST); // keep the phase's stores above the wait
+  __builtin_amdgcn_s_waitcnt(0);           // this wave's stores are acknowledged
+  __syncthreads();                         // ... and so are every wave's
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(T.cnt + s, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    long long n = 0;
+    while (__hip_atomic_load(T.cnt + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      ++n;
+      bool give_up = n >= T.spin_limit;
+      if (!give_up && (n & 31) == 0) // the flags are polled less often than the counter
+        give_up = team_abort(T) || (T.host_cancel && __hip_atomic_load(const_cast<int*>(T.host_cancel),
+                                                                       __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_SYSTEM) != 0);
+      if (give_up) {
+        __hip_atomic_store(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  const bool ok = s_ok != 0;
+  __syncthreads(); // s_ok is rewritten by the next barrier
+  return ok;
+}
+
+// sum of a team's partials p[t0..t1), identical in every workgroup of the team
+__device__ __forceinline__ double team_sum(const double* p, int t0, int t1)
+{
+  __shared__ double lds[kBT / kWave];
+  double s = 0.0;
+  for (int i = t0 + threadIdx.x; i < t1; i += kBT)
+    s += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = wave_reduce_sum(s);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = s;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBT / kWave; ++w)
+    t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// p_new of column c: the same fma at the owner and at every gather
+__device__ __forceinline__ float p_of(const float* z, const float* pold, float beta, int64_t c)
+{
+  return __fmaf_rn(beta, fetch(pold + c), fetch(z + c));
+}
+
+// Row r of A32 p_new (p_new formed from z and p_old on the fly).
+__device__ __forceinline__ float prod_row32(const Team& T, const float* z, const float* pold, float beta, int32_t r,
+                                            int32_t sr0, int32_t sl0)
+{
+  float s0 = 0.f, s1 = 0.f;
+  if (T.sell) {
+    const int32_t local = r - sr0;
+    const int32_t sl = sl0 + (local >> 6);
+    const int64_t base = (int64_t)T.soff[sl] * 64 + (local & 63);
+    const int w = T.soff[sl + 1] - T.soff[sl];
+    float s2 = 0.f, s3 = 0.f;
+    int j = 0;
+    for (; j + 3 < w; j += 4) { // four slots' loads in flight
+      const int64_t i0 = base + (int64_t)j * 64;
+      int d[4];
+      float m[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        d[u] = T.sdcol[i0 + 64 * u];
+        m[u] = T.v32[i0 + 64 * u];
+      }
+      s0 += m[0] * p_of(z, pold, beta, (int64_t)r + d[0]);
+      s1 += m[1] * p_of(z, pold, beta, (int64_t)r + d[1]);
+      s2 += m[2] * p_of(z, pold, beta, (int64_t)r + d[2]);
+      s3 += m[3] * p_of(z, pold, beta, (int64_t)r + d[3]);
+    }
+    for (; j < w; ++j) {
+      const int64_t i0 = base + (int64_t)j * 64;
+      s0 += T.v32[i0] * p_of(z, pold, beta, (int64_t)r + T.sdcol[i0]);
+    }
+    s0 += s2;
+    s1 += s3;
+  }
+  else {
+    const int32_t e = T.rp[r + 1];
+    int32_t k = T.rp[r];
+    for (; k + 1 < e; k += 2) {
+      s0 += T.v32[k] * p_of(z, pold, beta, T.ci[k]);
+      s1 += T.v32[k + 1] * p_of(z, pold, beta, T.ci[k + 1]);
+    }
+    if (k < e)
+      s0 += T.v32[k] * p_of(z, pold, beta, T.ci[k]);
+  }
+  return s0 + s1;
+}
+
+// Row r of b - A64 x.
+__device__ __forceinline__ double residual_row64(const Team& T, const double* x, const double* b, int32_t r,
+                                                 int32_t sr0, int32_t sl0)
+{
+  double s0 = 0.0, s1 = 0.0;
+  if (T.sell) {
+    const int32_t local = r - sr0;
+    const int32_t sl = sl0 + (local >> 6);
+    const int64_t base = (int64_t)T.soff[sl] * 64 + (local & 63);
+    const int w = T.soff[sl + 1] - T.soff[sl];
+    int j = 0;
+    for (; j + 1 < w; j += 2) {
+      const int64_t i0 = base + (int64_t)j * 64, i1 = i0 + 64;
+      s0 += T.v64[i0] * fetch(x + (int64_t)r + T.sdcol[i0]);
+      s1 += T.v64[i1] * fetch(x + (int64_t)r + T.sdcol[i1]);
+    }
+    if (j < w) {
+      const int64_t i0 = base + (int64_t)j * 64;
+      s0 += T.v64[i0] * fetch(x + (int64_t)r + T.sdcol[i0]);
+    }
+  }
+  else {
+    const int32_t e = T.rp[r + 1];
+    int32_t k = T.rp[r];
+    for (; k + 1 < e; k += 2) {
+      s0 += T.v64[k] * fetch(x + T.ci[k]);
+      s1 += T.v64[k + 1] * fetch(x + T.ci[k + 1]);
+    }
+    if (k < e)
+      s0 += T.v64[k] * fetch(x + T.ci[k]);
+  }
+  return b[r] - (s0 + s1);
+}
+
+// out[gb] = the sum of v over the workgroup
+__device__ __forceinline__ void team_put(double v, double* out, int gb)
+{
+  __shared__ double lds[kBT / kWave];
+  v = wave_reduce_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    lds[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBT / kWave; ++w)
+      t += lds[w];
+    publish(out + gb, t);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBT, 5) // 5 waves per SIMD without spills: 5 workgroups per CU
+k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
+{
+  const int gb = blockIdx.x;
+  const int s = T.seg_of[gb];
+  const int t0 = T.first[s], t1 = T.first[s + 1], nw = t1 - t0;
+  const int32_t sr0 = T.rows[s], sr1 = T.rows[s + 1];
+  const int32_t r0 = sr0 + (gb - t0) * T.strip[s];
+  const int32_t r1 = min(sr1, r0 + T.strip[s]);
+  const int32_t sl0 = T.sell ? T.sslice[s] : 0;
+  const bool lead = gb == t0 && threadIdx.x == 0;
+  SegState& S = st[s];
+  if (!S.active || team_abort(T))
+    return;
+  const double tol_r2 = S.rtol2, tol_a2 = S.atol2, delta2 = S.delta2;
+  const int max_iter = S.max_iter;
+  unsigned phase = 0;
+  // init: r64 = b - A64 x; r = r64, z = D r, p_old = 0; partials r.z, r.r, b.b
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
+    const double rd = residual_row64(T, V.x, V.b, i, sr0, sl0);
+    const float rv = (float)rd;
+    const float zv = V.dinv[i] * rv;
+    V.r[i] = rv;
+    publish(V.z + i, zv);
+    publish(V.p[1] + i, 0.f);
+    rz += (double)rv * (double)zv;
+    rr += rd * rd;
+    bb += V.b[i] * V.b[i];
+  }
+  team_put(rz, P.rz, gb);
+  team_put(rr, P.rr, gb);
+  team_put(bb, P.bb, gb);
+  if (!team_barrier(T, s, nw, phase))
+    return;
+  rz = team_sum(P.rz, t0, t1);
+  rr = team_sum(P.rr, t0, t1);
+  bb = team_sum(P.bb, t0, t1);
+  const double tol2 = fmax(tol_r2 * bb, tol_a2);
+  double rho = rz, rr_ref = rr, rr_out = rr;
+  double beta = 0.0;
+  int k = 0, nb_ru = 0, converged = 0, breakdown = 0;
+  bool done = rr <= tol2 || rz == 0.0;
+  converged = done ? 1 : 0;
+  while (!done) {
+    // A: p_new = z + beta p_old; q = A32 p_new; partial p.q
+    const int par = k & 1;
+    float* pnew = V.p[par];
+    const float* pold = V.p[par ^ 1];
+    const float bt = (float)beta;
+    double pq = 0.0;
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
+      const float y = prod_row32(T, V.z, pold, bt, i, sr0, sl0);
+      const float pi = p_of(V.z, pold, bt, i);
+      V.q[i] = y;
+      publish(pnew + i, pi); // other gathers of this phase read pold, a different buffer
+      pq += (double)y * (double)pi;
+    }
+    team_put(pq, P.pq, gb);
+    if (!team_barrier(T, s, nw, phase))
+      return;
+    // B: alpha; x, r, z; partials r.z, r.r
+    pq = team_sum(P.pq, t0, t1);
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      breakdown = 1;
+      break;
+    }
+    const double alpha = rho / pq;
+    const float af = (float)alpha;
+    rz = 0.0;
+    rr = 0.0;
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
+      publish(V.x + i, fetch(V.x + i) + alpha * (double)fetch(pnew + i));
+      const float rv = V.r[i] - af * V.q[i];
+      V.r[i] = rv;
+      const float zv = V.dinv[i] * rv;
+      publish(V.z + i, zv);
+      rz += (double)rv * (double)zv;
+      rr += (double)rv * (double)rv;
+    }
+    team_put(rz, P.rz, gb);
+    team_put(rr, P.rr, gb);
+    if (!team_barrier(T, s, nw, phase))
+      return;
+    rz = team_sum(P.rz, t0, t1);
+    rr = team_sum(P.rr, t0, t1);
+    beta = (rho != 0.0) ? rz / rho : 0.0;
+    const bool ru = rr <= delta2 * rr_ref || rr <= tol2 || k + 1 >= max_iter || !isfinite(rr);
+    if (ru) {
+      double part = 0.0;
+      for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
+        const double rd = residual_row64(T, V.x, V.b, i, sr0, sl0);
+        V.r64[i] = rd;
+        part += rd * rd;
+      }
+      team_put(part, P.rr64, gb);
+      if (!team_barrier(T, s, nw, phase))
+        return;
+      const double rr64 = team_sum(P.rr64, t0, t1);
+      ++nb_ru;
+      rr_out = rr64;
+      if (rr64 <= tol2 || k + 1 >= max_iter || !isfinite(rr64)) {
+        converged = rr64 <= tol2 ? 1 : 0;
+        ++k;
+        break;
+      }
+      double rzn = 0.0;
+      for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
+        const float rv = (float)V.r64[i];
+        V.r[i] = rv;
+        const float zv = V.dinv[i] * rv;
+        publish(V.z + i, zv);
+        rzn += (double)rv * (double)zv;
+      }
+      team_put(rzn, P.rzru, gb);
+      if (!team_barrier(T, s, nw, phase))
+        return;
+      rho = team_sum(P.rzru, t0, t1);
+      rr_ref = rr64;
+    }
+    else {
+      rho = rz;
+      rr_out = rr;
+    }
+    ++k;
+  }
+  if (lead) {
+    S.done = 1;
+    S.converged = converged;
+    S.breakdown = breakdown;
+    S.iter = k;
+    S.nb_ru = nb_ru;
+    S.bb = bb;
+    S.tol2 = tol2;
+    S.rr_out = rr_out;
+  }
+}
+
+__global__ void k_persistent_clear(unsigned* cnt, int nb_seg, int* abort_flag)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb_seg; i += gridDim.x * blockDim.x)
+    cnt[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *abort_flag = 0;
+}
+
 } // namespace batch
 } // namespace afem
 
@@ -585,12 +996,30 @@ struct afem_batch
   bool prepared = false;
   hipGraphExec_t graph = nullptr;
   int64_t graph_key = -1;
+  // persistent solve: team layout, second p buffer, partials, barrier
+  // counters, abort flag, host cancel flag (pinned), its captured graph
+  int pt_budget = 0; // requested workgroups (0 = default)
+  int pt_grid = 0;   // workgroups of the layout
+  int pt_layout_budget = -1;
+  std::vector<int32_t> pt_first_host, pt_strip_host;
+  DBuf<int32_t> pt_seg_of, pt_first, pt_strip;
+  DBuf<float> p32b;
+  DBuf<double> pt_parts;
+  DBuf<unsigned> pt_cnt;
+  DBuf<int> pt_abort;
+  int* pt_cancel = nullptr; // pinned, mapped
+  hipGraphExec_t pt_graph = nullptr;
+  uint64_t pt_graph_key = 0;
   ~afem_batch()
   {
     if (graph)
       (void)hipGraphExecDestroy(graph);
+    if (pt_graph)
+      (void)hipGraphExecDestroy(pt_graph);
     if (host_st)
       (void)hipHostFree(host_st);
+    if (pt_cancel)
+      (void)hipHostFree(pt_cancel);
   }
 };
 
@@ -864,6 +1293,208 @@ RunStats run(afem_batch* b, const afem_batch_options& o, bool use_graph)
   return rs;
 }
 
+// ---- persistent solve: host side ----
+
+void drop_persistent_graph(afem_batch* b)
+{
+  if (b->pt_graph) {
+    (void)hipGraphExecDestroy(b->pt_graph);
+    b->pt_graph = nullptr;
+  }
+  b->pt_graph_key = 0;
+}
+
+// Workgroups of k_persistent the device holds at once.
+int persistent_capacity()
+{
+  int per_cu = 0;
+  AFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, kBT, 0));
+  return std::max(0, per_cu) * rt().num_cu;
+}
+
+// Teams: segment s gets m_s workgroups in proportion to rows^1.5 (its work
+// per iteration times the growth of its iteration count with its size), at
+// least one and at most one per 256 rows; strips are multiples of 64 rows.
+void persistent_layout(afem_batch* b, int budget)
+{
+  const int ns = b->nb_seg;
+  std::vector<double> w(ns);
+  double wsum = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    const double r = (double)(b->seg_rows[k + 1] - b->seg_rows[k]);
+    w[k] = r * std::sqrt(r);
+    wsum += w[k];
+  }
+  std::vector<int32_t> m(ns);
+  auto assign = [&](double scale) {
+    int64_t tot = 0;
+    for (int k = 0; k < ns; ++k) {
+      const int32_t rows = b->seg_rows[k + 1] - b->seg_rows[k];
+      const int32_t cap = std::max(1, (rows + 255) / 256);
+      m[k] = (int32_t)std::min<double>(cap, std::max(1.0, std::floor(scale * w[k] / wsum)));
+      tot += m[k];
+    }
+    return tot;
+  };
+  double scale = budget;
+  while (assign(scale) > budget && scale > 1.0)
+    scale *= 0.95;
+  b->pt_first_host.assign(ns + 1, 0);
+  b->pt_strip_host.assign(ns, 0);
+  for (int k = 0; k < ns; ++k) {
+    const int32_t rows = b->seg_rows[k + 1] - b->seg_rows[k];
+    int32_t strip = (rows + m[k] - 1) / m[k];
+    strip = (strip + 63) / 64 * 64;
+    const int32_t mk = (rows + strip - 1) / strip;
+    b->pt_strip_host[k] = strip;
+    b->pt_first_host[k + 1] = b->pt_first_host[k] + mk;
+  }
+  b->pt_grid = b->pt_first_host[ns];
+  std::vector<int32_t> seg_of(b->pt_grid);
+  for (int k = 0; k < ns; ++k)
+    for (int g = b->pt_first_host[k]; g < b->pt_first_host[k + 1]; ++g)
+      seg_of[g] = k;
+  b->pt_seg_of.alloc(seg_of.size());
+  b->pt_seg_of.upload(seg_of.data(), seg_of.size());
+  b->pt_first.alloc(ns + 1);
+  b->pt_first.upload(b->pt_first_host.data(), ns + 1);
+  b->pt_strip.alloc(ns);
+  b->pt_strip.upload(b->pt_strip_host.data(), ns);
+  b->pt_parts.alloc((size_t)6 * b->pt_grid);
+  b->pt_cnt.alloc(ns);
+  b->pt_abort.alloc(1);
+  if (!b->pt_cancel) {
+    AFEM_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->pt_cancel), sizeof(int),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+  }
+  *b->pt_cancel = 0;
+  b->pt_layout_budget = budget;
+  drop_persistent_graph(b);
+}
+
+struct PersistentArgs
+{
+  Team T;
+  TeamVec V;
+  TeamParts P;
+};
+
+PersistentArgs persistent_args(afem_batch* b, long long spin_limit)
+{
+  afem_system* s = b->sys;
+  PersistentArgs a;
+  std::memset(&a, 0, sizeof(a)); // padding too: the graph key hashes the bytes
+  Team& T = a.T;
+  T.seg_of = b->pt_seg_of.p;
+  T.first = b->pt_first.p;
+  T.rows = b->d_rows.p;
+  T.strip = b->pt_strip.p;
+  T.sell = b->sell_ok ? 1 : 0;
+  T.sslice = b->d_sslice.p;
+  T.soff = b->d_soff.p;
+  T.sdcol = b->sdcol.p;
+  T.rp = s->row_index;
+  T.ci = s->columns;
+  T.v32 = b->sell_ok ? b->sv32.p : b->v32.p;
+  T.v64 = b->sell_ok ? b->sv64.p : s->values;
+  T.cnt = b->pt_cnt.p;
+  T.abort_flag = b->pt_abort.p;
+  int* dev_cancel = nullptr;
+  AFEM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_cancel), b->pt_cancel, 0));
+  T.host_cancel = dev_cancel;
+  T.spin_limit = spin_limit;
+  a.V = TeamVec{ s->x.p, s->rhs.p, b->dinv.p, b->r32.p, b->z32.p, { b->p32.p, b->p32b.p }, b->q32.p, b->r64.p };
+  const int n = b->pt_grid;
+  double* pp = b->pt_parts.p;
+  a.P = TeamParts{ pp, pp + n, pp + 2 * n, pp + 3 * n, pp + 4 * n, pp + 5 * n };
+  return a;
+}
+
+// FNV-1a over the launch arguments: a captured graph is replayed only with
+// exactly the buffers and settings it was captured with.
+uint64_t persistent_key(const PersistentArgs& a, int grid)
+{
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&h](const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+      h ^= c[i];
+      h *= 1099511628211ULL;
+    }
+  };
+  mix(&a, sizeof(a));
+  mix(&grid, sizeof(grid));
+  return h;
+}
+
+void launch_persistent(afem_batch* b, const PersistentArgs& a)
+{
+  hipLaunchKernelGGL(k_persistent_clear, dim3(1), dim3(256), 0, stream(), b->pt_cnt.p, b->nb_seg, b->pt_abort.p);
+  hipLaunchKernelGGL(k_persistent, dim3(b->pt_grid), dim3(kBT), 0, stream(), a.T, a.V, a.P, b->st.p);
+}
+
+// Solves the active segments in one persistent launch (captured once in a
+// hipGraph and replayed); the host waits for it with a bounded wall time and
+// raises the cancel flag when that passes.
+void run_persistent(afem_batch* b, const afem_batch_options& o)
+{
+  const int cap = persistent_capacity();
+  AFEM_REQUIRE(cap >= b->nb_seg, AFEM_ERR_CAPACITY,
+               "the persistent batched solve needs one resident workgroup per system at least");
+  const int want = o.workgroups > 0 ? o.workgroups : cap;
+  const int budget = std::max(b->nb_seg, std::min(want, cap));
+  if (b->pt_layout_budget != budget || (size_t)b->pt_first_host.size() != (size_t)b->nb_seg + 1)
+    persistent_layout(b, budget);
+  if (b->p32b.n != b->p32.n) {
+    b->p32b.alloc(b->p32.n);
+    drop_persistent_graph(b);
+  }
+  const long long spin_limit = o.spin_limit > 0 ? o.spin_limit : (1LL << 21);
+  *b->pt_cancel = 0;
+  const PersistentArgs args = persistent_args(b, spin_limit);
+  if (o.use_graph) {
+    const uint64_t key = persistent_key(args, b->pt_grid);
+    if (!b->pt_graph || b->pt_graph_key != key) {
+      drop_persistent_graph(b);
+      hipGraph_t gr;
+      AFEM_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+      launch_persistent(b, args);
+      AFEM_HIP(hipStreamEndCapture(stream(), &gr));
+      const hipError_t e = hipGraphInstantiate(&b->pt_graph, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      AFEM_HIP(e);
+      b->pt_graph_key = key;
+    }
+    AFEM_HIP(hipGraphLaunch(b->pt_graph, stream()));
+  }
+  else {
+    launch_persistent(b, args);
+  }
+  check_launch("batch_persistent");
+  const auto t0 = std::chrono::steady_clock::now();
+  const double limit_ms = o.timeout_ms > 0.0 ? o.timeout_ms : 60000.0;
+  bool cancelled = false;
+  for (;;) {
+    const hipError_t q = hipStreamQuery(stream());
+    if (q == hipSuccess)
+      break;
+    AFEM_REQUIRE(q == hipErrorNotReady, AFEM_ERR_HIP, std::string("persistent solve: ") + hipGetErrorString(q));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!cancelled && ms > limit_ms) {
+      __atomic_store_n(b->pt_cancel, 1, __ATOMIC_SEQ_CST);
+      cancelled = true;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+  int aborted = 0;
+  AFEM_HIP(hipMemcpyAsync(&aborted, b->pt_abort.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  *b->pt_cancel = 0;
+  AFEM_REQUIRE(!aborted && !cancelled, AFEM_ERR_DEVICE,
+               cancelled ? "persistent batched solve cancelled after timeout_ms"
+                         : "persistent batched solve gave up a barrier wait (spin_limit)");
+}
+
 } // namespace
 
 extern "C" {
@@ -879,6 +1510,10 @@ AFEM_API void afem_batch_default_options(afem_batch_options* o)
   o->chunk = 32;
   o->use_graph = 1;
   o->use_initial_guess = 0;
+  o->persistent = 0;
+  o->workgroups = 0;
+  o->spin_limit = 0;
+  o->timeout_ms = 0.0;
 }
 
 AFEM_API int afem_batch_create(afem_system_t sys, int nb_segment, const int32_t* segment_rows, afem_batch_t* out)
@@ -958,7 +1593,10 @@ AFEM_API int afem_batch_solve(afem_batch_t b, const afem_batch_options* opt, afe
     if (!any)
       break;
     write_states(b);
-    run(b, *opt, opt->use_graph != 0);
+    if (opt->persistent)
+      run_persistent(b, *opt);
+    else
+      run(b, *opt, opt->use_graph != 0);
     // FP64 true residual of every segment that ran
     read_states(b);
     std::vector<SegState> after(b->host_st, b->host_st + b->nb_seg);

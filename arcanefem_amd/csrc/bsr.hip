@@ -9,6 +9,8 @@
 
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 namespace afem {
 
 // ---- sparsity -------------------------------------------------------------------
@@ -153,6 +155,9 @@ static void bsr_compute_sparsity(afem_bsr* b)
 }
 
 static void build_plan(afem_bsr* b);
+// derived plans, built on first use (afem_bsr::plans_tried)
+constexpr unsigned kPlansSliced = 1, kPlansBrick = 2, kPlansEdge = 4, kPlansCell = 8;
+static void ensure_plans(afem_bsr* b, unsigned what);
 
 // ---- element kernels ---------------------------------------------------------------
 
@@ -446,16 +451,16 @@ __device__ __forceinline__ int32_t plan_cell(uint64_t w) { return (int32_t)(w & 
 __device__ __forceinline__ int plan_slot(uint64_t w) { return (int)((w >> 32) & 3); }
 __device__ __forceinline__ int plan_pos(uint64_t w, int k) { return (int)((w >> (34 + 8 * k)) & 0xff); }
 
-__global__ void k_pair_plan(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
-                            const int32_t* __restrict__ cells, int npc, int32_t nb_node,
-                            const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
-                            uint64_t* __restrict__ plan, int32_t* __restrict__ diag_pos, int32_t* __restrict__ err)
+// One thread per row; the workgroup's 256 rows of columns are staged in LDS
+// when they fit (8192 entries), so the binary searches of the row positions
+// read LDS instead of re-reading the row from memory for every search.
+constexpr int kPairStage = 8192;
+
+template <class RowPtr>
+__device__ __forceinline__ bool pair_plan_row(RowPtr rc, int len, int32_t i, const int32_t* __restrict__ nc_off,
+                                              const int32_t* __restrict__ nc_items, const int32_t* __restrict__ cells,
+                                              int npc, uint64_t* __restrict__ plan, int32_t* __restrict__ diag_pos)
 {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb_node)
-    return;
-  const int32_t* rc = columns + row_index[i];
-  const int len = row_index[i + 1] - row_index[i];
   const int d = find_col(rc, len, i);
   diag_pos[i] = d;
   bool bad = (d >= len || rc[d] != i);
@@ -474,149 +479,39 @@ __global__ void k_pair_plan(const int32_t* __restrict__ nc_off, const int32_t* _
     }
     plan[e] = w;
   }
+  return bad;
+}
+
+__global__ void __launch_bounds__(256)
+k_pair_plan(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+            const int32_t* __restrict__ cells, int npc, int32_t nb_node, const int32_t* __restrict__ row_index,
+            const int32_t* __restrict__ columns, uint64_t* __restrict__ plan, int32_t* __restrict__ diag_pos,
+            int32_t* __restrict__ err)
+{
+  __shared__ int32_t s_cols[kPairStage];
+  const int32_t r0 = blockIdx.x * 256;
+  const int32_t r1 = min(r0 + 256, nb_node);
+  const int32_t a = row_index[r0];
+  const int32_t cnt = row_index[r1] - a;
+  const bool staged = cnt <= kPairStage;
+  if (staged)
+    for (int32_t k = threadIdx.x; k < cnt; k += 256)
+      s_cols[k] = columns[a + k];
+  __syncthreads();
+  const int32_t i = r0 + threadIdx.x;
+  if (i >= r1)
+    return;
+  const int len = row_index[i + 1] - row_index[i];
+  const bool bad = staged ? pair_plan_row(s_cols + (row_index[i] - a), len, i, nc_off, nc_items, cells, npc, plan,
+                                          diag_pos)
+                          : pair_plan_row(columns + row_index[i], len, i, nc_off, nc_items, cells, npc, plan, diag_pos);
   if (bad)
     atomicOr(err, 1);
 }
 
-// Compact plan: the other nodes as signed 16-bit deltas from the row node and
-// their 5-bit row positions, all in one word,
-//   bits 0-15 / 16-31 / 32-47 deltas, 48-52 / 53-57 / 58-62 positions,
-// so the assembly reads one word per (row, cell) pair and then the
-// coordinates, without touching the cell connectivity.  Built only when every
-// delta fits in 16 bits and every row has at most 32 entries.
+// Row positions stored in 5 bits (the local and brick plans): rows of at most
+// 32 entries.
 constexpr int kCompactMaxRow = 32;
-
-__device__ __forceinline__ int32_t cplan_node(uint64_t w, int32_t row, int k)
-{
-  return row + (int32_t)(int16_t)((w >> (16 * k)) & 0xffff);
-}
-__device__ __forceinline__ int cplan_pos(uint64_t w, int k) { return (int)((w >> (48 + 5 * k)) & 31); }
-
-__global__ void k_compact_plan(const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
-                               const int32_t* __restrict__ cells, int npc, int32_t nb_node,
-                               uint64_t* __restrict__ cplan, int32_t* __restrict__ unfit)
-{
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb_node)
-    return;
-  bool bad = false;
-  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
-    const uint64_t w = plan[e];
-    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)plan_cell(w));
-    const int3 o = other_nodes(cq, plan_slot(w));
-    const int32_t nodes[3] = { o.x, o.y, o.z };
-    uint64_t c = 0;
-    for (int k = 0; k < 3; ++k) {
-      const int32_t d = (k < npc - 1) ? nodes[k] - i : 0;
-      const int pos = (k < npc - 1) ? plan_pos(w, k) : 0;
-      bad = bad || d < -32768 || d > 32767 || pos >= kCompactMaxRow;
-      c |= (uint64_t)(uint16_t)(int16_t)d << (16 * k);
-      c |= (uint64_t)(pos & 31) << (48 + 5 * k);
-    }
-    cplan[e] = c;
-  }
-  if (bad)
-    atomicOr(unfit, 1);
-}
-
-static void build_compact_plan(afem_bsr* b)
-{
-  const afem_mesh& m = *b->mesh;
-  b->cplan.release();
-  if (!b->plan.p || b->max_row > kCompactMaxRow)
-    return;
-  b->cplan.alloc(b->plan.n);
-  DBuf<int32_t> unfit(1);
-  unfit.zero();
-  hipLaunchKernelGGL(k_compact_plan, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.nc_offsets.p,
-                     b->plan.p, m.cells.p, m.npc, m.nb_node, b->cplan.p, unfit.p);
-  check_launch("compact_plan");
-  int32_t h = 0;
-  unfit.download(&h, 1);
-  if (h)
-    b->cplan.release();
-}
-
-// Laplace assembly driven by the compact plan: per (row, cell) pair one
-// 8-byte word, then the three (two) neighbour coordinates.  The plan words of
-// a row are contiguous and loaded U at a time; the cell id (for a per-cell
-// conductivity) comes from the node->cell list only when lambda is given.
-template <int NPC, int U>
-__global__ void __launch_bounds__(256)
-k_assemble_laplace_cplan(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
-                         const int32_t* __restrict__ nc_items, const uint64_t* __restrict__ cplan,
-                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
-                         int32_t nb_node, int rows_per_block, const double* __restrict__ cell_lambda,
-                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
-{
-  extern __shared__ unsigned char smem[];
-  const int32_t tile = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
-  const int32_t r0 = tile * rows_per_block;
-  const int32_t r1 = min(r0 + rows_per_block, nb_node);
-  const int32_t a = row_index[r0];
-  const int32_t cnt = row_index[r1] - a;
-  double* s_val = reinterpret_cast<double*>(smem);
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
-    s_val[k] = 0.0;
-  __syncthreads();
-  const int32_t i = r0 + threadIdx.x;
-  if (threadIdx.x < rows_per_block && i < r1) {
-    double* rv = s_val + (row_index[i] - a);
-    const d3 p = load_node(coords, i);
-    const int32_t e0 = nc_off[i], e1 = nc_off[i + 1];
-    double diag = 0.0, det_sum = 0.0;
-    for (int32_t e = e0; e < e1; e += U) {
-      uint64_t w[U];
-      double lam[U];
-      d3 pa[U], pb[U], pc[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        w[u] = cplan[min(e + u, e1 - 1)];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        lam[u] = cell_lambda ? cell_lambda[nc_items[min(e + u, e1 - 1)] >> 2] : 1.0;
-        pa[u] = load_node(coords, cplan_node(w[u], i, 0));
-        pb[u] = load_node(coords, cplan_node(w[u], i, 1));
-        if constexpr (NPC == 4)
-          pc[u] = load_node(coords, cplan_node(w[u], i, 2));
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (e + u >= e1)
-          break;
-        if constexpr (NPC == 4) {
-          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p), ec = sub(pc[u], p);
-          const d3 Ca = cross(eb, ec), Cb = cross(ec, ea), Cc = cross(ea, eb);
-          const double adet = fabs(dot(ea, Ca));
-          const d3 Cp = { -(Ca.x + Cb.x + Cc.x), -(Ca.y + Cb.y + Cc.y), -(Ca.z + Cb.z + Cc.z) };
-          const double scale = lam[u] * fast_rcp(6.0 * adet);
-          diag += scale * dot(Cp, Cp);
-          rv[cplan_pos(w[u], 0)] += scale * dot(Cp, Ca);
-          rv[cplan_pos(w[u], 1)] += scale * dot(Cp, Cb);
-          rv[cplan_pos(w[u], 2)] += scale * dot(Cp, Cc);
-          det_sum += adet;
-        }
-        else {
-          const d3 ea = sub(pa[u], p), eb = sub(pb[u], p);
-          const double adet = fabs(ea.x * eb.y - eb.x * ea.y);
-          const double gax = eb.y, gay = -eb.x, gbx = -ea.y, gby = ea.x;
-          const double gpx = -(gax + gbx), gpy = -(gay + gby);
-          const double scale = lam[u] * fast_rcp(2.0 * adet);
-          diag += scale * (gpx * gpx + gpy * gpy);
-          rv[cplan_pos(w[u], 0)] += scale * (gpx * gax + gpy * gay);
-          rv[cplan_pos(w[u], 1)] += scale * (gpx * gbx + gpy * gby);
-          det_sum += adet;
-        }
-      }
-    }
-    rv[diag_pos[i]] += diag;
-    if (rhs)
-      rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
-    values[a + k] = s_val[k];
-}
 
 // ---- block-local plan ---------------------------------------------------------------------
 // Rows are split into blocks of kLocalRows consecutive rows.  The nodes a
@@ -1061,7 +956,7 @@ k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_of
     atomicOr(err, 1);
 }
 
-static uint64_t morton_spread(uint64_t v) // 21 bits -> every third bit
+__host__ __device__ inline uint64_t morton_spread(uint64_t v) // 21 bits -> every third bit
 {
   v &= 0x1fffffULL;
   v = (v | (v << 32)) & 0x1f00000000ffffULL;
@@ -1070,6 +965,155 @@ static uint64_t morton_spread(uint64_t v) // 21 bits -> every third bit
   v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
   v = (v | (v << 2)) & 0x1249249249249249ULL;
   return v;
+}
+
+// Per-workgroup min / max of the coordinates (6 doubles per workgroup).
+__global__ void __launch_bounds__(256)
+k_bbox(const double* __restrict__ coords, int32_t n, double* __restrict__ out)
+{
+  __shared__ double red[6][4];
+  double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+  for (int32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    for (int d = 0; d < 3; ++d) {
+      const double x = coords[4 * (int64_t)i + d];
+      lo[d] = fmin(lo[d], x);
+      hi[d] = fmax(hi[d], x);
+    }
+  for (int d = 0; d < 3; ++d)
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d) {
+      red[d][threadIdx.x >> 6] = lo[d];
+      red[3 + d][threadIdx.x >> 6] = hi[d];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    double v = red[q][0];
+    for (int w = 1; w < 4; ++w)
+      v = q < 3 ? fmin(v, red[q][w]) : fmax(v, red[q][w]);
+    out[6 * blockIdx.x + q] = v;
+  }
+}
+
+// Morton code of each node (21 bits per axis over the bounding box).
+__global__ void k_morton(const double* __restrict__ coords, int32_t n, double3 lo, double3 ext,
+                         uint64_t* __restrict__ code, int32_t* __restrict__ node)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double x[3] = { coords[4 * (int64_t)i], coords[4 * (int64_t)i + 1], coords[4 * (int64_t)i + 2] };
+  const double l[3] = { lo.x, lo.y, lo.z }, e[3] = { ext.x, ext.y, ext.z };
+  uint64_t c = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double t = e[d] > 0.0 ? (x[d] - l[d]) / e[d] : 0.0;
+    const uint64_t q = (uint64_t)fmin(2097151.0, fmax(0.0, t * 2097151.0));
+    c |= morton_spread(q) << d;
+  }
+  code[i] = c;
+  node[i] = i;
+}
+
+// Sorts each brick's 256 rows into increasing order (bitonic, in LDS), so the
+// row segments a wave writes at the end lie in runs of consecutive rows.
+__global__ void __launch_bounds__(256)
+k_sort_brick_rows(int32_t* __restrict__ perm, int32_t n)
+{
+  __shared__ int32_t buf[kLocalRows];
+  const int64_t slot = (int64_t)blockIdx.x * kLocalRows + threadIdx.x;
+  buf[threadIdx.x] = slot < n ? perm[slot] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= kLocalRows; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int t = threadIdx.x;
+      const int partner = t ^ stride;
+      const int32_t mine = buf[t], other = buf[partner];
+      __syncthreads();
+      const bool asc = (t & size) == 0;
+      const bool lower = t < partner;
+      buf[t] = (lower == asc) ? min(mine, other) : max(mine, other);
+      __syncthreads();
+    }
+  if (slot < n)
+    perm[slot] = buf[threadIdx.x];
+}
+
+// Slice width of each block of 256 rows (rows perm[blk * 256 + t], or
+// blk * 256 + t without a permutation): its longest node->cell list, times 256.
+__global__ void __launch_bounds__(256)
+k_slice_width(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_off, int32_t n,
+              int64_t* __restrict__ width)
+{
+  __shared__ int32_t red[4];
+  const int64_t slot = (int64_t)blockIdx.x * kLocalRows + threadIdx.x;
+  int32_t w = 0;
+  if (slot < n) {
+    const int32_t r = perm ? perm[slot] : (int32_t)slot;
+    w = nc_off[r + 1] - nc_off[r];
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    w = max(w, __shfl_xor(w, off, 64));
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    width[blockIdx.x] = (int64_t)max(max(red[0], red[1]), max(red[2], red[3])) * kLocalRows;
+}
+
+// slice[0] = 0, slice[k + 1] = slice[k] + width[k] (one workgroup; nblk is
+// the number of 256-row blocks, a few thousand to tens of thousands)
+__global__ void __launch_bounds__(1024)
+k_slice_scan(const int64_t* __restrict__ width, int32_t nblk, int64_t* __restrict__ slice)
+{
+  __shared__ int64_t carry;
+  __shared__ int64_t wsum[16];
+  if (threadIdx.x == 0) {
+    carry = 0;
+    slice[0] = 0;
+  }
+  __syncthreads();
+  for (int32_t base = 0; base < nblk; base += 1024) {
+    const int32_t k = base + threadIdx.x;
+    int64_t v = k < nblk ? width[k] : 0;
+    // inclusive scan within the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t u = __shfl_up(v, off, 64);
+      if ((threadIdx.x & 63) >= off)
+        v += u;
+    }
+    if ((threadIdx.x & 63) == 63)
+      wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int64_t before = carry;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w)
+      before += wsum[w];
+    if (k < nblk)
+      slice[k + 1] = before + v;
+    __syncthreads();
+    if (threadIdx.x == 1023)
+      carry = before + v;
+    __syncthreads();
+  }
+}
+
+// Slices of the blocks of 256 rows on the device; returns the total words.
+static int64_t device_slices(const int32_t* perm, const afem_mesh& m, int32_t nblk, DBuf<int64_t>& slice)
+{
+  DBuf<int64_t> width((size_t)nblk);
+  slice.alloc((size_t)nblk + 1);
+  hipLaunchKernelGGL(k_slice_width, dim3(nblk), dim3(kLocalRows), 0, stream(), perm, m.nc_offsets.p, m.nb_node,
+                     width.p);
+  check_launch("slice_width");
+  hipLaunchKernelGGL(k_slice_scan, dim3(1), dim3(1024), 0, stream(), width.p, nblk, slice.p);
+  check_launch("slice_scan");
+  int64_t total = 0;
+  AFEM_HIP(hipMemcpyAsync(&total, slice.p + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  return total;
 }
 
 static void build_brick_plan(afem_bsr* b)
@@ -1091,52 +1135,36 @@ static void build_brick_plan(afem_bsr* b)
   if (!b->plan.p || b->max_row > kCompactMaxRow)
     return;
   const int32_t n = m.nb_node;
-  // Morton order of the node coordinates (21 bits per axis in the bounding box)
-  std::vector<double> xyz((size_t)n * 4);
-  m.coords.download(xyz.data(), xyz.size());
+  // Morton order of the node coordinates (21 bits per axis in the bounding
+  // box), sorted on the device
+  const int nbb = std::min<int>(1024, (int)grid_for(n, 256));
+  DBuf<double> bb((size_t)6 * nbb);
+  hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, stream(), m.coords.p, n, bb.p);
+  check_launch("bbox");
+  std::vector<double> hb((size_t)6 * nbb);
+  bb.download(hb.data(), hb.size());
   double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
-  for (int32_t i = 0; i < n; ++i)
+  for (int k = 0; k < nbb; ++k)
     for (int d = 0; d < 3; ++d) {
-      lo[d] = std::min(lo[d], xyz[4 * (size_t)i + d]);
-      hi[d] = std::max(hi[d], xyz[4 * (size_t)i + d]);
+      lo[d] = std::min(lo[d], hb[6 * k + d]);
+      hi[d] = std::max(hi[d], hb[6 * k + 3 + d]);
     }
-  std::vector<uint64_t> code((size_t)n), code2((size_t)n);
-  for (int32_t i = 0; i < n; ++i) {
-    uint64_t c = 0;
-    for (int d = 0; d < 3; ++d) {
-      const double ext = hi[d] - lo[d];
-      const double t = ext > 0.0 ? (xyz[4 * (size_t)i + d] - lo[d]) / ext : 0.0;
-      const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
-      c |= morton_spread(q) << d;
-    }
-    code[i] = c;
-  }
-  // stable LSD radix sort of (code, node) by code: four 16-bit digits
-  std::vector<int32_t> perm((size_t)n), perm2((size_t)n);
-  for (int32_t i = 0; i < n; ++i)
-    perm[i] = i;
-  std::vector<int64_t> hist(65536 + 1);
-  for (int shift = 0; shift < 64; shift += 16) {
-    std::fill(hist.begin(), hist.end(), 0);
-    for (int32_t i = 0; i < n; ++i)
-      ++hist[((code[i] >> shift) & 0xffff) + 1];
-    for (int k = 0; k < 65536; ++k)
-      hist[k + 1] += hist[k];
-    for (int32_t i = 0; i < n; ++i) {
-      const int64_t at = hist[(code[i] >> shift) & 0xffff]++;
-      code2[at] = code[i];
-      perm2[at] = perm[i];
-    }
-    code.swap(code2);
-    perm.swap(perm2);
-  }
+  DBuf<uint64_t> code((size_t)n), code2((size_t)n);
+  DBuf<int32_t> node((size_t)n);
+  b->bperm.alloc((size_t)n);
+  hipLaunchKernelGGL(k_morton, dim3(grid_for(n, 256)), dim3(256), 0, stream(), m.coords.p, n,
+                     make_double3(lo[0], lo[1], lo[2]), make_double3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]),
+                     code.p, node.p);
+  check_launch("morton");
+  size_t tmp_bytes = 0;
+  AFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, code.p, code2.p, node.p, b->bperm.p, n, 0, 63,
+                                              stream()));
+  DBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
+  AFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, code.p, code2.p, node.p, b->bperm.p, n, 0, 63,
+                                              stream()));
   const int32_t nblk = (n + kLocalRows - 1) / kLocalRows;
-  // inside a brick the rows go in increasing order, so that the row segments a
-  // wave writes at the end lie in runs of consecutive rows
-  for (int32_t k = 0; k < nblk; ++k)
-    std::sort(perm.begin() + (size_t)k * kLocalRows, perm.begin() + std::min<size_t>(n, (size_t)(k + 1) * kLocalRows));
-  b->bperm.alloc(n);
-  b->bperm.upload(perm.data(), perm.size());
+  hipLaunchKernelGGL(k_sort_brick_rows, dim3(nblk), dim3(kLocalRows), 0, stream(), b->bperm.p, n);
+  check_launch("sort_brick_rows");
   b->bval_off.alloc(n);
   b->btotal.alloc(nblk);
   hipLaunchKernelGGL(k_brick_offsets, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, b->row_index.p, n,
@@ -1160,21 +1188,9 @@ static void build_brick_plan(afem_bsr* b)
     cmax = std::max(cmax, counts[k]);
   }
   // slices: width = the brick's longest node->cell list
-  std::vector<int32_t> off((size_t)n + 1);
-  m.nc_offsets.download(off.data(), off.size());
-  std::vector<int64_t> slice((size_t)nblk + 1, 0);
-  for (int32_t k = 0; k < nblk; ++k) {
-    int32_t w = 0;
-    for (int32_t t = k * kLocalRows; t < std::min(n, (k + 1) * kLocalRows); ++t) {
-      const int32_t r = perm[t];
-      w = std::max(w, off[r + 1] - off[r]);
-    }
-    slice[k + 1] = slice[k] + (int64_t)w * kLocalRows;
-  }
-  b->bslice.alloc(slice.size());
-  b->bslice.upload(slice.data(), slice.size());
-  b->bplan.alloc((size_t)slice[nblk]);
-  b->bcell.alloc((size_t)slice[nblk]);
+  const int64_t words = device_slices(b->bperm.p, m, nblk, b->bslice);
+  b->bplan.alloc((size_t)words);
+  b->bcell.alloc((size_t)words);
   DBuf<int32_t> err(1);
   err.zero();
   hipLaunchKernelGGL(k_brick_plan, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, m.nc_offsets.p, b->plan.p,
@@ -1218,33 +1234,23 @@ static void build_sliced_plan(afem_bsr* b)
   if (!b->lplan.p)
     return;
   const int32_t nblk = (m.nb_node + kLocalRows - 1) / kLocalRows;
-  std::vector<int32_t> off((size_t)m.nb_node + 1);
-  m.nc_offsets.download(off.data(), off.size());
-  std::vector<int64_t> slice((size_t)nblk + 1, 0);
-  for (int32_t k = 0; k < nblk; ++k) {
-    int32_t w = 0;
-    for (int32_t i = k * kLocalRows; i < std::min(m.nb_node, (k + 1) * kLocalRows); ++i)
-      w = std::max(w, off[i + 1] - off[i]);
-    slice[k + 1] = slice[k] + (int64_t)w * kLocalRows;
-  }
-  b->slice.alloc(slice.size());
-  b->slice.upload(slice.data(), slice.size());
-  b->lplan_sl.alloc((size_t)slice[nblk]);
-  b->lcell_sl.alloc((size_t)slice[nblk]);
+  const int64_t words = device_slices(nullptr, m, nblk, b->slice);
+  b->lplan_sl.alloc((size_t)words);
+  b->lcell_sl.alloc((size_t)words);
   hipLaunchKernelGGL(k_slice_plan, dim3(nblk), dim3(kLocalRows), 0, stream(), m.nc_offsets.p, m.nc_items.p,
                      b->lplan.p, b->slice.p, m.nb_node, b->lplan_sl.p, b->lcell_sl.p);
   check_launch("slice_plan");
 }
 
 // Laplace assembly with the block's node coordinates staged in LDS (SoA):
-// per (row, cell) pair one word from HBM, then only LDS reads.  SLICED:
-// the words (and cell ids) are stored per block of rows with word j of row
-// r at slice[blk] + j * 256 + (r - r0), so one wave load is one contiguous
-// 512-byte read; otherwise they follow the node->cell lists (nc_off).
-template <int NPC, int U, bool SLICED>
+// per (row, cell) pair one word from HBM, then only LDS reads.  The words
+// (and cell ids) are stored per block of rows with word j of row r at
+// slice[blk] + j * 256 + (r - r0), so one wave load is one contiguous
+// 512-byte read.
+template <int NPC, int U>
 __global__ void __launch_bounds__(256)
 k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
-                         const int32_t* __restrict__ nc_items, const uint64_t* __restrict__ lplan,
+                         const int32_t* __restrict__ lcell, const uint64_t* __restrict__ lplan,
                          const int64_t* __restrict__ slice, const int32_t* __restrict__ table,
                          const int32_t* __restrict__ count, const int32_t* __restrict__ diag_pos,
                          const int32_t* __restrict__ row_index, int32_t nb_node, int val_cap, int node_cap,
@@ -1277,11 +1283,10 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
     const int32_t e0 = nc_off[i];
     const int32_t ncell = nc_off[i + 1] - e0;
     // word j of this row (and its cell id for the conductivity)
-    const int64_t wbase = SLICED ? slice[blk] + threadIdx.x : (int64_t)e0;
-    const int wstride = SLICED ? kLocalRows : 1;
+    const int64_t wbase = slice[blk] + threadIdx.x;
     double diag = 0.0, det_sum = 0.0;
-    row_pairs<NPC, U, !SLICED>(lplan, nc_items, wbase, wstride, ncell, cell_lambda, s_xy, s_z, p, rv, 1, diag,
-                               det_sum);
+    row_pairs<NPC, U, false>(lplan, lcell, wbase, kLocalRows, ncell, cell_lambda, s_xy, s_z, p, rv, 1, diag,
+                             det_sum);
     rv[diag_pos[i]] += diag;
     if (rhs)
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
@@ -1340,6 +1345,396 @@ k_assemble_laplace_brick(const double* __restrict__ coords, const int32_t* __res
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
     // only this thread writes into rv, so its row is complete here
     const int32_t a = row_index[i], len = row_index[i + 1] - a;
+    for (int k = 0; k < len; ++k)
+      values[a + k] = rv[64 * k];
+  }
+}
+
+// ---- cell-once brick plan ---------------------------------------------------------------
+// Every cell with a vertex among a brick's rows is listed once for that brick
+// (an instance), so the assembly builds each cell's element matrix once per
+// brick instead of once per (row, cell) pair.  Instance word 0: the cell's
+// nodes as indices into the brick's node table (11 bits each) and, at bit 44,
+// the mask of its vertices that are rows of the brick; word 1: for each owned
+// vertex v, the row positions of the cell's other nodes, in cell order
+// (5 bits each, at bit 5 * ((npc - 1) v + k)).  A brick's instances are ordered
+// by mask, so a wave's lanes mostly share one mask and take the same
+// branches, and within a mask by a bijective hash of the cell id, so that a
+// wave's lanes are scattered cells (neighbouring cells share vertices, and
+// their LDS additions into the same rows would collide).  Per brick also the
+// table index -> brick slot map (255: not a row of the brick).
+constexpr int kCellMaxInst = kLocalSort;
+
+// Bijective scrambling of 32-bit cell ids (odd multiplier, then an xorshift
+// by half the width, which is its own inverse) and its inverse.
+__device__ __forceinline__ uint32_t cell_hash(uint32_t c)
+{
+  uint32_t x = c * 0x9E3779B1u;
+  return x ^ (x >> 16);
+}
+__device__ __forceinline__ uint32_t cell_unhash(uint32_t x)
+{
+  x ^= x >> 16;
+  return x * 0x0E8B2F51u; // the inverse of 0x9E3779B1 modulo 2^32
+}
+
+__global__ void k_invert_perm(const int32_t* __restrict__ perm, int32_t n, int32_t* __restrict__ inv)
+{
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n)
+    inv[perm[k]] = k;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns
+// the thread's offset, *total the sum.
+__device__ __forceinline__ int32_t block_scan256(int32_t v, int32_t* scan, int32_t* total)
+{
+  const int tid = threadIdx.x;
+  scan[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int32_t u = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += u;
+    __syncthreads();
+  }
+  const int32_t incl = scan[tid];
+  *total = scan[255];
+  __syncthreads();
+  return incl - v;
+}
+
+// WRITE = false: the number of distinct cells of each brick (ninst[blk], -1
+// when the brick's (row, cell) pairs exceed kCellMaxInst); WRITE = true: the
+// instance words at off[blk], the cell ids, and the slot map.
+template <bool WRITE>
+__global__ void __launch_bounds__(256)
+k_cell_instances(const int32_t* __restrict__ perm, const int32_t* __restrict__ inv,
+                 const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                 const int32_t* __restrict__ cells, int npc, int32_t nb_node,
+                 const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
+                 const int32_t* __restrict__ table, const int32_t* __restrict__ count,
+                 const int64_t* __restrict__ off, int32_t* __restrict__ ninst, uint64_t* __restrict__ words,
+                 int32_t* __restrict__ inst_cell, uint8_t* __restrict__ slot_map, int32_t* __restrict__ err)
+{
+  extern __shared__ uint32_t s_cells[]; // 2 * kCellMaxInst: the sorted (row, cell) pairs, the distinct cells
+  uint32_t* buf = s_cells;               // hashed cell ids, then the owned masks
+  uint32_t* uniq = s_cells + kCellMaxInst; // hashed cell ids
+  __shared__ int32_t scan[256];
+  const int tid = threadIdx.x;
+  const int32_t blk = blockIdx.x;
+  const int64_t slot = (int64_t)blk * kLocalRows + tid;
+  int32_t r = -1, e0 = 0, nc = 0;
+  if (slot < nb_node) {
+    r = perm[slot];
+    e0 = nc_off[r];
+    nc = nc_off[r + 1] - e0;
+  }
+  int32_t total = 0;
+  const int32_t start = block_scan256(nc, scan, &total);
+  if (total > kCellMaxInst) {
+    if (tid == 0) {
+      if (WRITE)
+        atomicOr(err, 1);
+      else
+        ninst[blk] = -1;
+    }
+    return;
+  }
+  int n2 = 256;
+  while (n2 < total)
+    n2 <<= 1;
+  for (int k = tid; k < n2; k += 256)
+    buf[k] = 0xffffffffu;
+  __syncthreads();
+  for (int k = 0; k < nc; ++k)
+    buf[start + k] = cell_hash((uint32_t)(nc_items[e0 + k] >> 2));
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = tid; k < n2 / 2; k += 256) {
+        const int i = (k / stride) * 2 * stride + (k % stride);
+        const int j = i + stride;
+        const bool asc = (i & size) == 0;
+        const uint32_t x = buf[i], y = buf[j];
+        if ((x > y) == asc) {
+          buf[i] = y;
+          buf[j] = x;
+        }
+      }
+      __syncthreads();
+    }
+  // distinct cells, in hash order (the padding sorts last: only the first
+  // `total` entries are read)
+  const int chunk = n2 / 256;
+  const int k0 = tid * chunk;
+  int mine = 0;
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < total && (k == 0 || buf[k] != buf[k - 1]))
+      ++mine;
+  int32_t nu = 0;
+  int32_t pos = block_scan256(mine, scan, &nu);
+  for (int k = k0; k < k0 + chunk; ++k)
+    if (k < total && (k == 0 || buf[k] != buf[k - 1]))
+      uniq[pos++] = buf[k];
+  __syncthreads();
+  if (!WRITE) {
+    if (tid == 0)
+      ninst[blk] = nu;
+    return;
+  }
+  const int tn = count[blk];
+  const int32_t* tab = table + (int64_t)blk * kLocalMaxNodes;
+  for (int k = tid; k < tn; k += 256)
+    slot_map[(int64_t)blk * kLocalMaxNodes + k] = 255;
+  __syncthreads();
+  if (slot < nb_node) {
+    const int l = find_node(tab, tn, r);
+    if (l < tn && tab[l] == r)
+      slot_map[(int64_t)blk * kLocalMaxNodes + l] = (uint8_t)tid;
+    else
+      atomicOr(err, 1); // the row's own node is always among its columns
+  }
+  // owned-vertex mask of each distinct cell (buf reused)
+  for (int k = tid; k < nu; k += 256) {
+    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)cell_unhash(uniq[k]));
+    const int32_t nodes[4] = { cq.x, cq.y, cq.z, cq.w };
+    int m = 0;
+    for (int v = 0; v < npc; ++v)
+      if (inv[nodes[v]] / kLocalRows == blk)
+        m |= 1 << v;
+    buf[k] = (uint32_t)m;
+  }
+  __syncthreads();
+  // order: masks with more owned vertices first, then by mask, then by hash
+  const int uchunk = (nu + 255) / 256;
+  const int u0 = tid * uchunk, u1 = min(nu, u0 + uchunk);
+  int32_t base = 0;
+  bool bad = false;
+  for (int pc = npc; pc >= 1; --pc)
+    for (int m = (1 << npc) - 1; m >= 1; --m) {
+      if (__popc(m) != pc)
+        continue;
+      int cnt = 0;
+      for (int k = u0; k < u1; ++k)
+        cnt += buf[k] == (uint32_t)m;
+      int32_t mtotal = 0;
+      int32_t at = base + block_scan256(cnt, scan, &mtotal);
+      for (int k = u0; k < u1; ++k) {
+        if (buf[k] != (uint32_t)m)
+          continue;
+        const int32_t c = (int32_t)cell_unhash(uniq[k]);
+        const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+        const int32_t nodes[4] = { cq.x, cq.y, cq.z, cq.w };
+        uint64_t w0 = (uint64_t)m << 44, w1 = 0;
+        for (int v = 0; v < npc; ++v) {
+          const int l = find_node(tab, tn, nodes[v]);
+          bad = bad || l >= tn || tab[l] != nodes[v];
+          w0 |= (uint64_t)(l & 2047) << (11 * v);
+          if (!((m >> v) & 1))
+            continue;
+          const int32_t* rc = columns + row_index[nodes[v]];
+          const int len = row_index[nodes[v] + 1] - row_index[nodes[v]];
+          int kk = 0;
+          for (int wv = 0; wv < npc; ++wv) {
+            if (wv == v)
+              continue;
+            const int p = find_col(rc, len, nodes[wv]);
+            bad = bad || p >= len || rc[p] != nodes[wv] || p >= kCompactMaxRow;
+            w1 |= (uint64_t)(p & 31) << (5 * ((npc - 1) * v + kk));
+            ++kk;
+          }
+        }
+        const int64_t g = off[blk] + at;
+        words[2 * g] = w0;
+        words[2 * g + 1] = w1;
+        inst_cell[g] = c;
+        ++at;
+      }
+      base += mtotal;
+    }
+  if (bad)
+    atomicOr(err, 1);
+}
+
+static void build_cell_plan(afem_bsr* b)
+{
+  const afem_mesh& m = *b->mesh;
+  auto release_all = [b]() {
+    b->cinst_off.release();
+    b->cwords.release();
+    b->ccell.release();
+    b->cmap.release();
+    b->cell_inst_max = 0;
+  };
+  release_all();
+  if (!b->bplan.p || !m.simplex || m.npc > 4 || b->max_row > kCompactMaxRow)
+    return;
+  const int32_t n = m.nb_node;
+  const int32_t nblk = (n + kLocalRows - 1) / kLocalRows;
+  DBuf<int32_t> inv((size_t)n), cnt((size_t)nblk), err(1);
+  hipLaunchKernelGGL(k_invert_perm, dim3(grid_for(n, 256)), dim3(256), 0, stream(), b->bperm.p, n, inv.p);
+  check_launch("invert_perm");
+  const size_t lds = 2 * kCellMaxInst * sizeof(int32_t);
+  hipLaunchKernelGGL(k_cell_instances<false>, dim3(nblk), dim3(256), lds, stream(), b->bperm.p, inv.p, m.nc_offsets.p,
+                     m.nc_items.p, m.cells.p, m.npc, n, b->row_index.p, b->columns.p, b->btable.p, b->bcount.p,
+                     (const int64_t*)nullptr, cnt.p, (uint64_t*)nullptr, (int32_t*)nullptr, (uint8_t*)nullptr, err.p);
+  check_launch("cell_instances_count");
+  std::vector<int32_t> c((size_t)nblk);
+  cnt.download(c.data(), c.size());
+  std::vector<int64_t> off((size_t)nblk + 1, 0);
+  int32_t mx = 0;
+  for (int32_t k = 0; k < nblk; ++k) {
+    if (c[k] < 0)
+      return;
+    off[k + 1] = off[k] + c[k];
+    mx = std::max(mx, c[k]);
+  }
+  b->cinst_off.alloc(off.size());
+  b->cinst_off.upload(off.data(), off.size());
+  b->cwords.alloc((size_t)std::max<int64_t>(2 * off[nblk], 2));
+  b->ccell.alloc((size_t)std::max<int64_t>(off[nblk], 1));
+  b->cmap.alloc((size_t)nblk * kLocalMaxNodes);
+  err.zero();
+  hipLaunchKernelGGL(k_cell_instances<true>, dim3(nblk), dim3(256), lds, stream(), b->bperm.p, inv.p, m.nc_offsets.p,
+                     m.nc_items.p, m.cells.p, m.npc, n, b->row_index.p, b->columns.p, b->btable.p, b->bcount.p,
+                     b->cinst_off.p, cnt.p, b->cwords.p, b->ccell.p, b->cmap.p, err.p);
+  check_launch("cell_instances");
+  int32_t h = 0;
+  err.download(&h, 1);
+  if (h) {
+    release_all();
+    return;
+  }
+  b->cell_inst_max = mx;
+}
+
+// a += v on an LDS double that other lanes may update too (ds_add_f64)
+__device__ __forceinline__ void lds_atomic_add(double* a, double v)
+{
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Cell-once Laplace assembly over the bricks: each thread takes instances of
+// the brick's cells, builds the cell's gradients once (cofactors of the
+// edges from its first vertex) and adds the off-diagonal entries of every
+// owned vertex's element row into that row's LDS slots (interleaved as in
+// k_assemble_laplace_brick) and |det| into the row's source accumulator.
+// After the workgroup barrier each row thread sets its diagonal to minus the
+// sum of its off-diagonal entries (every element row sums to zero) and writes
+// its row.  The LDS additions of different waves land in any order, so the
+// values are reproducible to rounding, not bit for bit.
+template <int NPC>
+__global__ void __launch_bounds__(256)
+k_assemble_laplace_cell(const double* __restrict__ coords, const int32_t* __restrict__ perm,
+                        const int64_t* __restrict__ inst_off, const uint64_t* __restrict__ words,
+                        const int32_t* __restrict__ inst_cell, const uint8_t* __restrict__ slot_map,
+                        const int32_t* __restrict__ table, const int32_t* __restrict__ count,
+                        const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
+                        int32_t nb_node, int max_row, int node_cap, const double* __restrict__ cell_lambda,
+                        double source_f, double* __restrict__ rhs, double* __restrict__ values)
+{
+  extern __shared__ unsigned char smem[];
+  const int32_t blk = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int val_cap = kLocalRows * max_row;
+  const int tn = count[blk];
+  double* s_val = reinterpret_cast<double*>(smem);
+  double* s_src = s_val + val_cap;                              // kLocalRows
+  double2* s_xy = reinterpret_cast<double2*>(s_src + kLocalRows); // node_cap
+  double* s_z = reinterpret_cast<double*>(s_xy + node_cap);
+  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_z + node_cap);
+  const int32_t* tab = table + (int64_t)blk * kLocalMaxNodes;
+  for (int k = threadIdx.x; k < tn; k += blockDim.x) {
+    const int64_t g = tab[k];
+    s_xy[k] = *reinterpret_cast<const double2*>(coords + 4 * g);
+    s_z[k] = coords[4 * g + 2];
+    s_map[k] = slot_map[(int64_t)blk * kLocalMaxNodes + k];
+  }
+  for (int k = threadIdx.x; k < val_cap; k += blockDim.x)
+    s_val[k] = 0.0;
+  s_src[threadIdx.x] = 0.0;
+  __syncthreads();
+  const int64_t i1 = inst_off[blk + 1];
+  for (int64_t i = inst_off[blk] + threadIdx.x; i < i1; i += kLocalRows) {
+    const uint64_t w0 = words[2 * i], w1 = words[2 * i + 1];
+    const double lam = cell_lambda ? cell_lambda[inst_cell[i]] : 1.0;
+    const int mask = (int)((w0 >> 44) & 15);
+    d3 x[NPC];
+#pragma unroll
+    for (int v = 0; v < NPC; ++v) {
+      const int l = (int)((w0 >> (11 * v)) & 2047);
+      const double2 xy = s_xy[l];
+      x[v] = d3{ xy.x, xy.y, NPC == 4 ? s_z[l] : 0.0 };
+    }
+    double K[NPC][NPC];
+    double adet;
+    if constexpr (NPC == 4) {
+      const d3 e1 = sub(x[1], x[0]), e2 = sub(x[2], x[0]), e3 = sub(x[3], x[0]);
+      d3 C[4];
+      C[1] = cross(e2, e3);
+      C[2] = cross(e3, e1);
+      C[3] = cross(e1, e2);
+      adet = fabs(dot(e1, C[1]));
+      C[0] = d3{ -(C[1].x + C[2].x + C[3].x), -(C[1].y + C[2].y + C[3].y), -(C[1].z + C[2].z + C[3].z) };
+      const double scale = lam * fast_rcp(6.0 * adet);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = a + 1; c < 4; ++c) {
+          K[a][c] = scale * dot(C[a], C[c]);
+          K[c][a] = K[a][c];
+        }
+    }
+    else {
+      const d3 e1 = sub(x[1], x[0]), e2 = sub(x[2], x[0]);
+      const double det = e1.x * e2.y - e2.x * e1.y;
+      adet = fabs(det);
+      // rotated edges: the gradients times det
+      const double g1x = e2.y, g1y = -e2.x, g2x = -e1.y, g2y = e1.x;
+      const double g0x = -(g1x + g2x), g0y = -(g1y + g2y);
+      const double gx[3] = { g0x, g1x, g2x }, gy[3] = { g0y, g1y, g2y };
+      const double scale = lam * fast_rcp(2.0 * adet);
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = a + 1; c < 3; ++c) {
+          K[a][c] = scale * (gx[a] * gx[c] + gy[a] * gy[c]);
+          K[c][a] = K[a][c];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NPC; ++v) {
+      if (!((mask >> v) & 1))
+        continue;
+      const int l = (int)((w0 >> (11 * v)) & 2047);
+      const int t = s_map[l];
+      double* rv = s_val + (t >> 6) * (64 * max_row) + (t & 63);
+      int kk = 0;
+#pragma unroll
+      for (int c = 0; c < NPC; ++c) {
+        if (c == v)
+          continue;
+        const int pos = (int)((w1 >> (5 * ((NPC - 1) * v + kk))) & 31);
+        lds_atomic_add(rv + 64 * pos, K[v][c]);
+        ++kk;
+      }
+      lds_atomic_add(s_src + t, adet);
+    }
+  }
+  __syncthreads();
+  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
+  if (slot < nb_node) {
+    const int32_t r = perm[slot];
+    double* rv = s_val + (threadIdx.x >> 6) * (64 * max_row) + (threadIdx.x & 63);
+    const int32_t a = row_index[r], len = row_index[r + 1] - a;
+    const int d = diag_pos[r];
+    double diag = 0.0;
+    for (int k = 0; k < len; ++k)
+      if (k != d)
+        diag -= rv[64 * k];
+    rv[64 * d] = diag;
+    if (rhs)
+      rhs[r] = source_f * s_src[threadIdx.x] / (NPC == 4 ? 24.0 : 6.0);
     for (int k = 0; k < len; ++k)
       values[a + k] = rv[64 * k];
   }
@@ -2265,7 +2660,6 @@ static void build_plan(afem_bsr* b)
 {
   const afem_mesh& m = *b->mesh;
   b->plan.release();
-  b->cplan.release();
   b->diag_pos.release();
   b->q1_plan.release(); // rebuilt by the first Q1 assembly
   if (b->max_row > kPlanMaxRow || !m.simplex) // the plans serve the simplex kernels
@@ -2281,11 +2675,35 @@ static void build_plan(afem_bsr* b)
   int32_t h = 0;
   err.download(&h, 1);
   AFEM_REQUIRE(h == 0, AFEM_ERR_NOT_FOUND, "BSRMatrix: a cell coupling is missing from the pattern");
-  build_compact_plan(b);
-  build_local_plan(b);
-  build_sliced_plan(b);
-  build_brick_plan(b);
-  build_edge_plan(b);
+  b->plans_tried = 0; // the derived plans are built by the first assembly that uses them
+}
+
+// Builds, once per pattern, the derived plan a kernel kind needs (kinds that
+// cannot be built leave their arrays empty, and the variant falls back).
+static void ensure_plans(afem_bsr* b, unsigned what)
+{
+  const unsigned todo = what & ~b->plans_tried;
+  if (!todo || !b->plan.p)
+    return;
+  b->plans_tried |= todo;
+  if (todo & kPlansSliced) {
+    ProfScope prof("plan_sliced");
+    build_local_plan(b);
+    build_sliced_plan(b);
+    b->lplan.release(); // only the sliced copy is read by the kernels
+  }
+  if (todo & kPlansBrick) {
+    ProfScope prof("plan_brick");
+    build_brick_plan(b);
+  }
+  if (todo & kPlansEdge) {
+    ProfScope prof("plan_edge");
+    build_edge_plan(b);
+  }
+  if (todo & kPlansCell) {
+    ProfScope prof("plan_cell");
+    build_cell_plan(b); // on the brick plan (ensured first by the caller)
+  }
 }
 
 // Laplace assembly driven by the plan: only the values are staged in LDS.
@@ -2387,16 +2805,17 @@ static bool assembly_shape(const afem_bsr& b, int bytes_per_entry, int max_rows,
   return bytes <= 160 * 1024;
 }
 
-// Laplace assembly variants: searching kernel (no plan), plan kernels and
-// compact-plan kernels over (U, rows per block).
+// Laplace assembly variants: the searching kernel (no plan), the pair-plan
+// kernel (rows of up to 255 entries), the sliced block-local plan and the
+// brick plan (rows of up to 32 entries, node tables of up to 2048 nodes),
+// each with U plan words in flight per thread.
 enum PlanKind
 {
   kSearch = 0,
   kPlan = 1,
-  kCompact = 2,
-  kLocal = 3,
-  kLocalSliced = 4,
-  kBrick = 5
+  kLocalSliced = 2,
+  kBrick = 3,
+  kCell = 4
 };
 struct LaplaceVariant
 {
@@ -2405,24 +2824,22 @@ struct LaplaceVariant
   int rows;
 };
 static const LaplaceVariant kLaplaceVariants[] = {
-  { kSearch, 4, 256 },  { kPlan, 4, 256 },  { kPlan, 8, 256 },        { kCompact, 4, 256 },
-  { kCompact, 8, 128 }, { kLocal, 4, 256 }, { kLocal, 8, 256 },       { kLocal, 16, 256 },
-  { kLocalSliced, 4, 256 }, { kLocalSliced, 8, 256 }, { kBrick, 4, 256 }, { kBrick, 8, 256 },
-  { kLocalSliced, 16, 256 }, { kBrick, 16, 256 },
+  { kSearch, 4, 256 },       { kPlan, 8, 256 },   { kLocalSliced, 8, 256 },
+  { kLocalSliced, 16, 256 }, { kBrick, 8, 256 },  { kBrick, 16, 256 },
+  { kCell, 1, 256 },
 };
 constexpr int kNumLaplaceVariants = sizeof(kLaplaceVariants) / sizeof(kLaplaceVariants[0]);
-constexpr int kDefaultLaplaceVariant = 6;
+constexpr int kDefaultLaplaceVariant = 5;
 
-template <int NPC, int U, bool SLICED>
+template <int NPC, int U>
 static void launch_laplace_lplan(const afem_bsr* b, const double* lam, double f, double* rhs)
 {
   const afem_mesh& m = *b->mesh;
   const int val_cap = (kLocalRows * b->max_row + 1) & ~1; // keeps the coordinate table 16-byte aligned
   const int node_cap = b->block_nodes_max;
   const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap) * sizeof(double);
-  hipLaunchKernelGGL((k_assemble_laplace_lplan<NPC, U, SLICED>), dim3(grid_for(m.nb_node, kLocalRows)),
-                     dim3(kLocalRows), lds, stream(), m.coords.p, m.nc_offsets.p,
-                     SLICED ? b->lcell_sl.p : m.nc_items.p, SLICED ? b->lplan_sl.p : b->lplan.p, b->slice.p,
+  hipLaunchKernelGGL((k_assemble_laplace_lplan<NPC, U>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
+                     lds, stream(), m.coords.p, m.nc_offsets.p, b->lcell_sl.p, b->lplan_sl.p, b->slice.p,
                      b->block_nodes.p, b->block_count.p, b->diag_pos.p, b->row_index.p, m.nb_node, val_cap, node_cap,
                      lam, f, rhs, b->values.p);
 }
@@ -2449,29 +2866,48 @@ static void launch_laplace_plan(const afem_bsr* b, int rows, size_t lds, const d
                      rows, lam, f, rhs, b->values.p);
 }
 
-template <int NPC, int U>
-static void launch_laplace_cplan(const afem_bsr* b, int rows, size_t lds, const double* lam, double f, double* rhs)
+static size_t cell_lds(const afem_bsr* b)
 {
-  const afem_mesh& m = *b->mesh;
-  hipLaunchKernelGGL((k_assemble_laplace_cplan<NPC, U>), dim3(grid_for(m.nb_node, rows)), dim3(rows), lds, stream(),
-                     m.coords.p, m.nc_offsets.p, m.nc_items.p, b->cplan.p, b->diag_pos.p, b->row_index.p, m.nb_node,
-                     rows, lam, f, rhs, b->values.p);
+  return ((size_t)kLocalRows * b->max_row + kLocalRows + 3 * (size_t)b->brick_nodes_max) * sizeof(double) +
+         (size_t)b->brick_nodes_max;
 }
 
-// The variant actually run: falls back to a plan kind the handle has.
-static LaplaceVariant effective_variant(const afem_bsr* b, int v)
+template <int NPC>
+static void launch_laplace_cell(const afem_bsr* b, const double* lam, double f, double* rhs)
+{
+  const afem_mesh& m = *b->mesh;
+  hipLaunchKernelGGL((k_assemble_laplace_cell<NPC>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
+                     cell_lds(b), stream(), m.coords.p, b->bperm.p, b->cinst_off.p, b->cwords.p, b->ccell.p,
+                     b->cmap.p, b->btable.p, b->bcount.p, b->diag_pos.p, b->row_index.p, m.nb_node, b->max_row,
+                     b->brick_nodes_max, lam, f, rhs, b->values.p);
+}
+
+// The variant actually run: builds the plan the variant needs on first use,
+// and falls back to a kind the handle has (brick -> sliced -> pair plan ->
+// search).
+static LaplaceVariant effective_variant(afem_bsr* b, int v)
 {
   LaplaceVariant var = kLaplaceVariants[v];
-  const size_t local_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->block_nodes_max) * sizeof(double);
-  const size_t brick_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->brick_nodes_max) * sizeof(double);
-  if (var.kind == kBrick && (!b->bplan.p || brick_lds > 160 * 1024))
-    var.kind = kLocalSliced;
-  if (var.kind == kLocalSliced && !b->lplan_sl.p)
-    var.kind = kLocal;
-  if ((var.kind == kLocal || var.kind == kLocalSliced) && (!b->lplan.p || local_lds > 160 * 1024))
-    var.kind = kCompact;
-  if (var.kind == kCompact && !b->cplan.p)
-    var.kind = kPlan;
+  if (var.kind == kCell) {
+    ensure_plans(b, kPlansBrick);
+    ensure_plans(b, kPlansCell);
+    if (!b->cwords.p || cell_lds(b) > 160 * 1024) {
+      var.kind = kBrick;
+      var.u = 16;
+    }
+  }
+  if (var.kind == kBrick) {
+    ensure_plans(b, kPlansBrick);
+    const size_t brick_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->brick_nodes_max) * sizeof(double);
+    if (!b->bplan.p || brick_lds > 160 * 1024)
+      var.kind = kLocalSliced;
+  }
+  if (var.kind == kLocalSliced) {
+    ensure_plans(b, kPlansSliced);
+    const size_t local_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->block_nodes_max) * sizeof(double);
+    if (!b->lplan_sl.p || local_lds > 160 * 1024)
+      var.kind = kPlan;
+  }
   if (var.kind == kPlan && !b->plan.p)
     var.kind = kSearch;
   return var;
@@ -2486,7 +2922,13 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
   const bool fits = assembly_shape(*b, var.kind == kSearch ? 12 : 8, var.rows, &rows, &lds);
   AFEM_REQUIRE(fits, AFEM_ERR_CAPACITY, "matrix rows too long for the LDS-staged assembly");
   ProfScope prof(m.npc == 4 ? "assemble_laplace_tet" : "assemble_laplace_tri");
-  if (var.kind == kBrick) {
+  if (var.kind == kCell) {
+    if (m.npc == 4)
+      launch_laplace_cell<4>(b, cell_lambda, f, rhs);
+    else
+      launch_laplace_cell<3>(b, cell_lambda, f, rhs);
+  }
+  else if (var.kind == kBrick) {
     if (m.npc == 4) {
       switch (var.u) {
       case 4: launch_laplace_brick<4, 4>(b, cell_lambda, f, rhs); break;
@@ -2505,46 +2947,16 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
   else if (var.kind == kLocalSliced) {
     if (m.npc == 4) {
       switch (var.u) {
-      case 4: launch_laplace_lplan<4, 4, true>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<4, 16, true>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<4, 8, true>(b, cell_lambda, f, rhs); break;
+      case 4: launch_laplace_lplan<4, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<4, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<4, 8>(b, cell_lambda, f, rhs); break;
       }
     }
     else {
       switch (var.u) {
-      case 4: launch_laplace_lplan<3, 4, true>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<3, 16, true>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<3, 8, true>(b, cell_lambda, f, rhs); break;
-      }
-    }
-  }
-  else if (var.kind == kLocal) {
-    if (m.npc == 4) {
-      switch (var.u) {
-      case 4: launch_laplace_lplan<4, 4, false>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<4, 16, false>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<4, 8, false>(b, cell_lambda, f, rhs); break;
-      }
-    }
-    else {
-      switch (var.u) {
-      case 4: launch_laplace_lplan<3, 4, false>(b, cell_lambda, f, rhs); break;
-      case 16: launch_laplace_lplan<3, 16, false>(b, cell_lambda, f, rhs); break;
-      default: launch_laplace_lplan<3, 8, false>(b, cell_lambda, f, rhs); break;
-      }
-    }
-  }
-  else if (var.kind == kCompact) {
-    if (m.npc == 4) {
-      switch (var.u) {
-      case 4: launch_laplace_cplan<4, 4>(b, rows, lds, cell_lambda, f, rhs); break;
-      default: launch_laplace_cplan<4, 8>(b, rows, lds, cell_lambda, f, rhs); break;
-      }
-    }
-    else {
-      switch (var.u) {
-      case 4: launch_laplace_cplan<3, 4>(b, rows, lds, cell_lambda, f, rhs); break;
-      default: launch_laplace_cplan<3, 8>(b, rows, lds, cell_lambda, f, rhs); break;
+      case 4: launch_laplace_lplan<3, 4>(b, cell_lambda, f, rhs); break;
+      case 16: launch_laplace_lplan<3, 16>(b, cell_lambda, f, rhs); break;
+      default: launch_laplace_lplan<3, 8>(b, cell_lambda, f, rhs); break;
       }
     }
   }
@@ -2560,16 +2972,10 @@ static void assemble_laplace_variant(afem_bsr* b, int v, const double* cell_lamb
                          rhs, b->values.p);
   }
   else if (m.npc == 4) {
-    if (var.u == 8)
-      launch_laplace_plan<4, 8>(b, rows, lds, cell_lambda, f, rhs);
-    else
-      launch_laplace_plan<4, 4>(b, rows, lds, cell_lambda, f, rhs);
+    launch_laplace_plan<4, 8>(b, rows, lds, cell_lambda, f, rhs);
   }
   else {
-    if (var.u == 8)
-      launch_laplace_plan<3, 8>(b, rows, lds, cell_lambda, f, rhs);
-    else
-      launch_laplace_plan<3, 4>(b, rows, lds, cell_lambda, f, rhs);
+    launch_laplace_plan<3, 8>(b, rows, lds, cell_lambda, f, rhs);
   }
   check_launch("assemble_laplace");
 }
@@ -2646,6 +3052,8 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
   AFEM_REQUIRE(b->nb_dof == 3 && b->mesh->npc == 4 && b->mesh->dim == 3, AFEM_ERR_ARGUMENT,
                "the elasticity element needs a triangle mesh and nb_dof = 2, or a tetrahedral mesh and nb_dof = 3");
   const afem_mesh& m = *b->mesh;
+  if (b->variant != 0)
+    ensure_plans(b, kPlansEdge);
   if (b->egeo.p && b->variant != 0 && b->variant != 1) {
     // cap: component stride of the staged geometry (odd, to spread the LDS banks)
     const int cap = b->e_inst_max | 1;

@@ -225,8 +225,8 @@ struct afem_bsr
   // assembly plan: per (node, cell) pair the cell, the slot and the row
   // positions of the other nodes; per row the diagonal position
   afem::DBuf<uint64_t> plan;
-  afem::DBuf<uint64_t> cplan;        // compact plan (node deltas + positions), when it fits
-  afem::DBuf<uint64_t> lplan;        // block-local plan (node-table indices + positions), when it fits
+  unsigned plans_tried = 0;          // derived plans built (or found unbuildable) for this pattern
+  afem::DBuf<uint64_t> lplan;        // block-local plan (node-table indices + positions): built, sliced, released
   afem::DBuf<uint64_t> lplan_sl;     // the same words sliced per block of rows (coalesced reads)
   afem::DBuf<int32_t> lcell_sl;      // cell id of each sliced word
   afem::DBuf<int64_t> slice;         // per block of rows: first sliced word
@@ -239,6 +239,13 @@ struct afem_bsr
   afem::DBuf<uint64_t> bplan;        // sliced local-plan words of the bricks
   afem::DBuf<int32_t> bcell;         // cell id of each brick plan word
   int32_t brick_val_max = 0, brick_nodes_max = 0;
+  // cell-once brick plan: per brick its cell instances (two words each), their
+  // cell ids, the table index -> brick slot map
+  afem::DBuf<int64_t> cinst_off;     // nb_brick + 1
+  afem::DBuf<uint64_t> cwords;
+  afem::DBuf<int32_t> ccell;
+  afem::DBuf<uint8_t> cmap;          // nb_brick * 2048
+  int32_t cell_inst_max = 0;
   // per nonzero block plan (elasticity): workgroups of whole rows with at most
   // 256 blocks, per block the cells sharing its edge (sliced per workgroup)
   afem::DBuf<int32_t> ewg_row;       // first row of each workgroup (nb_wg + 1)

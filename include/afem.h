@@ -160,11 +160,16 @@ AFEM_API int afem_bsr_assemble_laplace(afem_bsr_t bsr, const double* cell_lambda
                                        double source_f, double* rhs);
 
 /* Assembly kernel variants of the Laplace element (no counterpart in the
- * reference, which has one lambda per format): variant 0 searches each
- * column in the LDS-staged row; the others read the per-(node, cell) plan
- * built by afem_bsr_compute_sparsity, with different batch depths and rows
- * per workgroup.  autotune times each variant on this matrix, keeps the
- * fastest and reports it; set_assembly_variant pins one (-1 = default).
+ * reference, which has one lambda per format): 0 searches each column in the
+ * LDS-staged row; 1 reads the per-(node, cell) pair plan built by
+ * afem_bsr_compute_sparsity; 2 / 3 the sliced block-local plan and 4 / 5 the
+ * brick (Morton-ordered) plan, with 8 / 16 plan words in flight per thread;
+ * 6 builds each cell's element matrix once per brick and adds it into the
+ * rows with LDS atomics (values reproducible to rounding, not bit for bit).
+ * Plans 2-6 are built by the first assembly that needs them; a variant whose
+ * plan does not fit falls back (cell -> brick -> sliced -> pair plan -> search).
+ * autotune times each variant on this matrix, keeps the fastest and reports
+ * it; set_assembly_variant pins one (-1 = default, variant 5).
  * The autotuner assembles with lambda = 1 and no RHS: it overwrites the
  * matrix values, so assemble the system afterwards. */
 AFEM_API int afem_bsr_autotune_assembly(afem_bsr_t bsr, double* best_ms, int* variant);
@@ -479,17 +484,27 @@ AFEM_API int afem_spai_inspect(afem_spai_t spai, int what, float* out, int64_t c
  * matrix (e.g. the Laplace matrix of a disjoint union of meshes, assembled in
  * one launch by afem_bsr_assemble_laplace).  They are solved together by
  * mixed-precision Jacobi PCG with per-system scalars: FP32 matrix and vectors,
- * FP64 x, FP64 reliable-update residual deciding convergence, chunks of
- * iterations captured in a hipGraph and replayed. */
+ * FP64 x, FP64 reliable-update residual deciding convergence.  Either chunks
+ * of iterations of ordinary kernels captured in a hipGraph and replayed (the
+ * default), or one persistent kernel launch (captured in a hipGraph and
+ * replayed) in which every system gets a team of resident workgroups that
+ * synchronize through bounded barrier waits in device memory. */
 typedef struct afem_batch* afem_batch_t;
 typedef struct afem_batch_options {
   double rtol;            /* per system: ||b_s - A_s x_s|| <= max(rtol ||b_s||, atol) */
   double atol;
   int max_iter;
   double reliable_delta;  /* reliable-update reduction factor (0 = 0.1) */
-  int chunk;              /* iterations per captured graph launch (even; 0 = 32) */
-  int use_graph;          /* 1: replay a hipGraph per chunk; 0: plain launches */
+  int chunk;              /* chunked solve: iterations per captured graph launch (even; 0 = 32) */
+  int use_graph;          /* 1: replay a hipGraph (per chunk, or of the persistent launch); 0: plain launches */
   int use_initial_guess;  /* 1: start from the system's solution array */
+  int persistent;         /* 1: one persistent launch; 0: chunks of iterations (default) */
+  int workgroups;         /* persistent: workgroups of the launch (0 = 2 per CU; capped at what the
+                             device holds at once; at least one per system) */
+  long long spin_limit;   /* persistent: polls before a barrier wait gives up (0 = 2^21); the solve
+                             then fails with AFEM_ERR_DEVICE instead of waiting on */
+  double timeout_ms;      /* persistent: wall time after which the host raises the cancel flag that
+                             every barrier wait polls (0 = 60000) */
 } afem_batch_options;
 typedef struct afem_batch_result {
   int iterations;

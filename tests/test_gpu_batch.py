@@ -1,11 +1,13 @@
 """GPU tests of the batched solver: independent 2D heat/Poisson systems with
 8x8 checkerboard conductivities on meshes of different sizes, assembled in
 one launch on the disjoint union of the meshes and solved together with
-per-system scalars (segmented mixed-precision PCG in hipGraph chunks).
+per-system scalars (segmented mixed-precision PCG: hipGraph chunks of
+ordinary kernels by default, one persistent launch with persistent=1).
 
 Checked against: a direct solve of each diagonal block (scipy), the
-single-system mixed PCG of the same block (iteration counts), and the
-plain-launch path (bit-identical to the graph replays).
+single-system mixed PCG of the same block (iteration counts), the
+plain-launch path (bit-identical to the graph replays), and the persistent
+and chunked solves against each other.
 """
 
 import numpy as np
@@ -79,12 +81,13 @@ def test_batch_graph_equals_plain_launches(afem_gpu):
     print("[case] batch graph vs plain", flush=True)
     case = build_batch(A, [7, 16, 11], seed=4)
     bs = A.BatchSolver(case["sys"], case["offs"])
-    r1, _ = bs.solve(rtol=1e-8, use_graph=1, chunk=8)
-    x1 = case["sys"].solution().numpy()
-    r2, _ = bs.solve(rtol=1e-8, use_graph=0, chunk=8)
-    x2 = case["sys"].solution().numpy()
-    assert [r.iterations for r in r1] == [r.iterations for r in r2]
-    assert np.array_equal(x1, x2)
+    for persistent in (0, 1):
+        r1, _ = bs.solve(rtol=1e-8, use_graph=1, chunk=8, persistent=persistent)
+        x1 = case["sys"].solution().numpy()
+        r2, _ = bs.solve(rtol=1e-8, use_graph=0, chunk=8, persistent=persistent)
+        x2 = case["sys"].solution().numpy()
+        assert [r.iterations for r in r1] == [r.iterations for r in r2]
+        assert np.array_equal(x1, x2)
 
 
 def test_batch_iterations_match_single_solver(afem_gpu):
@@ -93,7 +96,9 @@ def test_batch_iterations_match_single_solver(afem_gpu):
     ns = [10, 24]
     case = build_batch(A, ns, seed=9)
     bs = A.BatchSolver(case["sys"], case["offs"])
-    res, _ = bs.solve(rtol=1e-8)
+    bs.set_tiles(8, 1)
+    res, _ = bs.solve(rtol=1e-8, persistent=0)  # chunked, CSR with 8 lanes: the single solver's product
+    res_p, _ = bs.solve(rtol=1e-8, persistent=1)  # SELL product and per-strip partial sums
     for s, (X, C) in enumerate(case["parts"]):
         m = A.Mesh(2, X, C)
         b = A.BSRFormat(m, 1, use_csr=True)
@@ -110,6 +115,8 @@ def test_batch_iterations_match_single_solver(afem_gpu):
         r = single.solve(method=A.SOLVER_PCG_MIXED, rtol=1e-8, max_iter=5000, spmv_lanes=8)
         assert abs(r.iterations - res[s].iterations) <= max(3, 0.05 * r.iterations), (s, r.iterations,
                                                                                        res[s].iterations)
+        assert abs(r.iterations - res_p[s].iterations) <= max(6, 0.12 * r.iterations), (s, r.iterations,
+                                                                                         res_p[s].iterations)
 
 
 def test_batch_autotune_and_tiles(afem_gpu):
@@ -139,7 +146,7 @@ def test_batch_sell_and_shrinking_grid(afem_gpu):
     sols = {}
     for lanes in (1, 8):
         bs.set_tiles(lanes, 1)
-        res, _ = bs.solve(rtol=1e-10, max_iter=5000)
+        res, _ = bs.solve(rtol=1e-10, max_iter=5000, persistent=0)
         x = case["sys"].solution().numpy().copy()
         for s_, r in enumerate(res):
             assert r.converged and r.relres <= 1e-10, (lanes, s_, r.relres)
@@ -149,7 +156,7 @@ def test_batch_sell_and_shrinking_grid(afem_gpu):
         assert res[0].iterations > 2 * max(r.iterations for r in res[1:])  # the grid did shrink
         sols[lanes] = (res, x)
     bs.set_tiles(1, 1)
-    r2, _ = bs.solve(rtol=1e-10, max_iter=5000, use_graph=0)
+    r2, _ = bs.solve(rtol=1e-10, max_iter=5000, use_graph=0, persistent=0)
     assert [r.iterations for r in r2] == [r.iterations for r in sols[1][0]]
     assert np.array_equal(case["sys"].solution().numpy(), sols[1][1])
 
@@ -220,3 +227,60 @@ def test_batch_heat_time_loop_matches_oracle(afem_gpu):
     # warm starts: the last step needs no more iterations than the first
     its = [max(r.iterations for r in res) for res, _ in out]
     assert its[-1] <= its[0]
+
+
+def test_persistent_matches_chunked_and_direct(afem_gpu):
+    """One persistent launch against the chunked graphs of ordinary kernels on
+    the same batch: both meet 1e-10 per system, match the direct solves, and
+    need about the same iterations (the partial sums are taken over
+    different workgroup strips, so FP32 rounding differs)."""
+    A = afem_gpu
+    print("[case] persistent vs chunked", flush=True)
+    case = build_batch(A, [40, 6, 7, 8, 9, 6, 7, 8, 9, 25], seed=6)
+    bs = A.BatchSolver(case["sys"], case["offs"])
+    out = {}
+    for persistent in (1, 0):
+        res, _ = bs.solve(rtol=1e-10, max_iter=5000, persistent=persistent)
+        x = case["sys"].solution().numpy().copy()
+        for s_, r in enumerate(res):
+            assert r.converged and r.relres <= 1e-10, (persistent, s_, r.relres)
+            ref = block_direct(case, s_)
+            xs = x[case["offs"][s_]:case["offs"][s_ + 1]]
+            assert np.linalg.norm(xs - ref) <= 1e-8 * np.linalg.norm(ref), (persistent, s_)
+        out[persistent] = res
+    for rp, rc in zip(out[1], out[0]):
+        assert abs(rp.iterations - rc.iterations) <= max(3, 0.05 * rc.iterations), (rp.iterations, rc.iterations)
+
+
+@pytest.mark.parametrize("workgroups", [0, 10, 64, 1024])
+def test_persistent_team_sizes(afem_gpu, workgroups):
+    """Teams from one workgroup per system (10 systems) to as many as the
+    device holds: every size converges to the same tolerance, and a graph
+    replay of the launch equals a plain launch bit for bit."""
+    A = afem_gpu
+    print(f"[case] persistent workgroups={workgroups}", flush=True)
+    case = build_batch(A, [30, 6, 7, 8, 9, 6, 7, 8, 9, 64], seed=12)
+    bs = A.BatchSolver(case["sys"], case["offs"])
+    r1, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=1, persistent=1)
+    x1 = case["sys"].solution().numpy()
+    assert all(r.converged and r.relres <= 1e-9 for r in r1)
+    r2, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=0, persistent=1)
+    x2 = case["sys"].solution().numpy()
+    assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert np.array_equal(x1, x2)
+
+
+def test_persistent_wait_limit_gives_up_then_recovers(afem_gpu):
+    """A spin limit of one poll makes the first barrier wait of a large team
+    give up (forced from the host; every workgroup still leaves the kernel
+    normally): the solve fails with AFEM_ERR_DEVICE, and the next solve with
+    the default limit converges."""
+    A = afem_gpu
+    print("[case] persistent spin limit", flush=True)
+    case = build_batch(A, [200, 150], seed=13)
+    bs = A.BatchSolver(case["sys"], case["offs"])
+    with pytest.raises(A.AfemError) as e:
+        bs.solve(rtol=1e-10, max_iter=5000, spin_limit=1, workgroups=256, use_graph=0, persistent=1)
+    assert e.value.status == A.ERR_DEVICE
+    res, _ = bs.solve(rtol=1e-10, max_iter=20000, persistent=1)
+    assert all(r.converged and r.relres <= 1e-10 for r in res)

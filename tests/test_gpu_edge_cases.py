@@ -14,7 +14,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 ROW_TOL = 1e-12
-N_LAPLACE_VARIANTS = 14
+N_LAPLACE_VARIANTS = 7
 
 
 def row_scaled_error(row, ref, got):

@@ -83,7 +83,8 @@ def test_node_cells_and_pattern_bit_exact(afem_gpu, spec):
     assert info.nb_col == len(ocol) and info.max_row_length == np.diff(orow).max()
 
 
-N_LAPLACE_VARIANTS = 14
+N_LAPLACE_VARIANTS = 7
+CELL_ONCE_VARIANT = 6
 
 
 @pytest.mark.parametrize("spec", MESHES, ids=[m[0] for m in MESHES])
@@ -111,10 +112,14 @@ def test_laplace_assembly_and_fused_rhs(afem_gpu, spec, with_lambda, variant):
     rhs2 = A.DeviceArray(N, np.float64)
     A.check(A.lib().afem_rhs_constant_source(m.h, 2.5, None, 1, 0, rhs2.c))
     assert np.abs(rhs2.numpy() - orhs).max() <= 1e-12 * np.abs(orhs).max()
-    # repeated assembly overwrites (no stale accumulation)
+    # repeated assembly overwrites (no stale accumulation); the cell-once
+    # variant adds with LDS atomics in no fixed order: equal to rounding
     b.assemble_laplace(dlam, 0.0, None)
     _, _, val2, _ = b.download()
-    assert np.array_equal(val, val2)
+    if variant == CELL_ONCE_VARIANT:
+        assert row_scaled_error(row, val, val2) <= 1e-14
+    else:
+        assert np.array_equal(val, val2)
 
 
 def test_assembly_autotune_keeps_parity(afem_gpu):
@@ -568,7 +573,7 @@ def test_heat_assembly_matches_oracle(afem_gpu, dim, n):
     lam = 1.0 + np.random.default_rng(3).random(C.shape[0])
     dlam = A.DeviceArray.from_numpy(lam)
     oval = O.assemble(dim, X, C, dim + 1, orow, ocol, element=O.HEAT, cell_lambda=lam, lame_lambda=dt)
-    for v in range(-1, 14):
+    for v in range(-1, N_LAPLACE_VARIANTS):
         b.set_assembly_variant(v)
         b.assemble_heat(dlam, dt)
         _, _, val, _ = b.download()
@@ -584,6 +589,7 @@ def test_heat_assembly_matches_oracle(afem_gpu, dim, n):
     assert np.array_equal(b.download()[2], val)
     u = 30.0 + np.sin(X[:, 0] * 3.0) * X[:, 1]
     rhs = A.DeviceArray(N, np.float64)
+    rhs.zero()  # the Neumann and mass-field terms add into it
     A.rhs_neumann(m, side, 1.9 * 20.0, rhs)
     A.rhs_mass_field(m, A.DeviceArray.from_numpy(u), 1.0 / dt, rhs)
     ref = O.rhs_neumann(dim, X, C, side, 1.9 * 20.0)
