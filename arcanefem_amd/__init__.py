@@ -122,6 +122,7 @@ def _declare(L):
         "afem_system_matrix_add_value": (ci, [vp, i32, i32, dbl, ci]),
         "afem_solver_default_options": (None, [ctypes.POINTER(SolverOptions)]),
         "afem_system_solve": (ci, [vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveResult)]),
+        "afem_system_solve_noalloc": (ci, [vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolveResult)]),
         "afem_system_spmv": (ci, [vp, vp, vp]),
         "afem_system_spmv_fp32": (ci, [vp, vp, vp]),
         "afem_system_spmv_format": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
@@ -474,6 +475,16 @@ class DoFLinearSystem:
         o = solver_options(**kw)
         r = SolveResult()
         rc = lib().afem_system_solve(self.h, ctypes.byref(o), ctypes.byref(r))
+        if rc != OK and (raise_on_failure or rc != ERR_NOT_CONVERGED):
+            check(rc)
+        return r
+
+    def solve_noalloc(self, raise_on_failure=True, **kw):
+        """The allocation-free FP64 solve (afem_system_solve_noalloc): at most
+        1048576 DoFs, FP64 methods only, CSR / 3x3 BSR SpMV on the view."""
+        o = solver_options(**kw)
+        r = SolveResult()
+        rc = lib().afem_system_solve_noalloc(self.h, ctypes.byref(o), ctypes.byref(r))
         if rc != OK and (raise_on_failure or rc != ERR_NOT_CONVERGED):
             check(rc)
         return r

@@ -530,7 +530,8 @@ __global__ void k_diag_inverse(int32_t n, const int32_t* __restrict__ rp, const 
       d = v[k];
   const double inv = (d != 0.0) ? 1.0 / d : 1.0;
   d64[r] = inv;
-  d32[r] = (float)inv;
+  if (d32)
+    d32[r] = (float)inv;
 }
 
 // 3x3 diagonal-block inverse per node (adjugate / determinant).
@@ -569,7 +570,8 @@ __global__ void k_block_inverse3(int32_t nb_node, const int32_t* __restrict__ rp
   }
   for (int q = 0; q < 9; ++q) {
     d64[9 * (int64_t)a + q] = inv[q];
-    d32[9 * (int64_t)a + q] = (float)inv[q];
+    if (d32)
+      d32[9 * (int64_t)a + q] = (float)inv[q];
   }
 }
 
@@ -1947,6 +1949,235 @@ static int solve(afem_system* s, const afem_solver_options& o, afem_solve_result
   return converged ? AFEM_OK : AFEM_ERR_NOT_CONVERGED;
 }
 
+// ---- allocation-free FP64 PCG ---------------------------------------------------------------
+//
+// The FP64 (Jacobi or unpreconditioned) CG of solve() for callers that may not
+// allocate: the SpMV runs straight on the system's matrix view (CSR with 16
+// lanes per row, or 3x3 BSR with 16 lanes per node row), and r, z, p, q, the
+// Jacobi inverse, the reduction partials and the solver state live in
+// `scratch`, a fixed device array of this file.  Each device holds its own
+// copy of the array until the process ends; every call on a device shares it
+// (calls that overlap on one device race).  The host reads the state with
+// plain copies into stack variables, so no pinned memory is needed either.
+
+constexpr int64_t kScratchMaxDof = int64_t(1) << 20;
+constexpr int64_t kScratchStateDoubles = 32;
+static_assert(sizeof(State) <= kScratchStateDoubles * sizeof(double), "State does not fit its scratch slot");
+// r, z, p, q (one double per DoF each), the Jacobi inverse (up to 3 doubles
+// per DoF for 3x3 blocks), 9 partial arrays, the state
+constexpr int64_t kScratchDoubles = 7 * kScratchMaxDof + 9 * kMaxBlocks + kScratchStateDoubles;
+static __device__ __align__(256) double scratch[kScratchDoubles]; // 56 MiB per device
+
+constexpr int kScratchMaxDevices = 64;
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current (the array's name is only a host-side stand-in).
+static double* scratch_on_device()
+{
+  ensure_init();
+  static double* ptr[kScratchMaxDevices] = {};
+  const int dev = rt().device;
+  AFEM_REQUIRE(dev >= 0 && dev < kScratchMaxDevices, AFEM_ERR_STATE, "device index out of range for the scratch table");
+  if (!ptr[dev]) {
+    int cur = -1;
+    AFEM_HIP(hipGetDevice(&cur));
+    AFEM_REQUIRE(cur == dev, AFEM_ERR_STATE, "the library's device is not the current device");
+    void* p = nullptr;
+    AFEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AFEM_REQUIRE(p != nullptr, AFEM_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    ptr[dev] = static_cast<double*>(p);
+  }
+  return ptr[dev];
+}
+
+__global__ void k_fill_zero(int64_t n, double* __restrict__ out)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = 0.0;
+}
+
+__global__ void k_state_init(State init, State* __restrict__ st)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *st = init;
+}
+
+struct NoAllocWs
+{
+  double *r, *z, *p, *q, *dinv;
+  State* st;
+  Parts P;
+};
+
+static NoAllocWs noalloc_layout(double* base)
+{
+  NoAllocWs w;
+  w.r = base;
+  w.z = base + kScratchMaxDof;
+  w.p = base + 2 * kScratchMaxDof;
+  w.q = base + 3 * kScratchMaxDof;
+  w.dinv = base + 4 * kScratchMaxDof;
+  double* parts = base + 7 * kScratchMaxDof;
+  w.P.pq = parts;
+  w.P.rz[0] = parts + kMaxBlocks;
+  w.P.rz[1] = parts + 2 * kMaxBlocks;
+  w.P.rzru[0] = parts + 3 * kMaxBlocks;
+  w.P.rzru[1] = parts + 4 * kMaxBlocks;
+  w.P.rr = parts + 5 * kMaxBlocks;
+  w.P.rr64 = parts + 6 * kMaxBlocks;
+  w.P.bb = parts + 7 * kMaxBlocks;
+  w.P.zq = parts + 8 * kMaxBlocks;
+  w.st = reinterpret_cast<State*>(parts + 9 * kMaxBlocks);
+  return w;
+}
+
+template <int MODE>
+static void noalloc_spmv(const afem_system* s, const double* x, double* y, const double* b, Gate gate,
+                         double* part_out, int nblk)
+{
+  if (s->block == 3)
+    hipLaunchKernelGGL((k_spmv_bsr3<16, double, double, MODE>), dim3(nblk), dim3(kBlock), 0, stream(),
+                       s->nb_dof / 3, s->row_index, s->columns, s->values, x, y, b, gate, part_out);
+  else
+    hipLaunchKernelGGL((k_spmv_csr<16, double, double, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                       s->row_index, s->columns, s->values, x, y, b, gate, part_out);
+  check_launch("noalloc_spmv");
+}
+
+static State noalloc_read_state(const NoAllocWs& w)
+{
+  State h{};
+  AFEM_HIP(hipMemcpyAsync(&h, w.st, sizeof(State), hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  return h;
+}
+
+template <int NB>
+static void noalloc_init(const afem_system* s, const NoAllocWs& w, int precond, int pnorm, int nblk)
+{
+  hipLaunchKernelGGL((k_init<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), (int64_t)(s->nb_dof / NB), w.r,
+                     s->rhs.p, w.r, w.z, w.p, w.dinv, precond, pnorm, w.P);
+  check_launch("noalloc_init");
+  hipLaunchKernelGGL(k_init_finalize, dim3(1), dim3(kBlock), 0, stream(), w.P, nblk, w.st);
+  check_launch("noalloc_init_finalize");
+}
+
+template <int NB>
+static void noalloc_iteration(const afem_system* s, const NoAllocWs& w, int k, int precond, int nblk)
+{
+  const int64_t nunits = s->nb_dof / NB;
+  const Gate until_done{ w.st, kUntilDone, k, nullptr, nblk };
+  noalloc_spmv<kPQ>(s, w.p, w.q, nullptr, until_done, w.P.pq, nblk);
+  hipLaunchKernelGGL((k_update<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), k, nunits, s->x.p, w.r, w.z,
+                     w.p, w.q, w.dinv, precond, w.P, nblk, w.st);
+  check_launch("noalloc_update");
+  hipLaunchKernelGGL((k_pupdate<NB, double>), dim3(nblk), dim3(kBlock), 0, stream(), k, 0, nunits, w.p, w.z, w.r,
+                     (const double*)nullptr, w.dinv, precond, w.P, nblk, w.st);
+  check_launch("noalloc_pupdate");
+}
+
+static int solve_noalloc(afem_system* s, const afem_solver_options& o, afem_solve_result* res)
+{
+  AFEM_REQUIRE(o.method == AFEM_SOLVER_PCG_FP64 || o.method == AFEM_SOLVER_CG_FP64, AFEM_ERR_NOT_IMPLEMENTED,
+               "the allocation-free solve runs the FP64 methods (AFEM_SOLVER_PCG_FP64, AFEM_SOLVER_CG_FP64)");
+  AFEM_REQUIRE(o.preconditioner == AFEM_PRECOND_JACOBI || o.preconditioner == AFEM_PRECOND_NONE, AFEM_ERR_ARGUMENT,
+               "unknown preconditioner");
+  AFEM_REQUIRE(o.rtol >= 0.0 && o.atol >= 0.0 && o.max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
+  AFEM_REQUIRE(o.norm >= 0 && o.norm <= 2, AFEM_ERR_ARGUMENT, "unknown stopping norm");
+  AFEM_REQUIRE((int64_t)s->nb_dof <= kScratchMaxDof, AFEM_ERR_ARGUMENT,
+               "the allocation-free solve takes at most 1048576 DoFs (its fixed scratch array holds 7 vectors of "
+               "that length); use afem_system_solve for larger systems");
+  system_require_matrix(s);
+  AFEM_REQUIRE(s->block == 1 || s->block == 3, AFEM_ERR_NOT_IMPLEMENTED, "block size must be 1 or 3");
+  AFEM_REQUIRE(!s->dof0_row_column, AFEM_ERR_NOT_IMPLEMENTED,
+               "DoF 0 is row-column eliminated: the transformed matrix keeps column 0 and is not symmetric, "
+               "so PCG does not apply (use RowElimination or Penalty for that DoF)");
+  auto t_start = std::chrono::steady_clock::now();
+  const NoAllocWs w = noalloc_layout(scratch_on_device());
+  const int precond = (o.method == AFEM_SOLVER_CG_FP64 || o.preconditioner == AFEM_PRECOND_NONE) ? 0 : 1;
+  const int check_every = o.check_every > 0 ? o.check_every : 16;
+  const int nblk = spmv_blocks(s);
+  // the array keeps the previous call's contents: clear the partials and the
+  // state (a fill kernel, no memset node in a captured stream)
+  hipLaunchKernelGGL(k_fill_zero, dim3(40), dim3(256), 0, stream(), (int64_t)(9 * kMaxBlocks + kScratchStateDoubles),
+                     w.P.pq);
+  check_launch("noalloc_clear");
+  if (precond) {
+    if (s->block == 3)
+      hipLaunchKernelGGL(k_block_inverse3, dim3(grid_for(s->nb_dof / 3, 256)), dim3(256), 0, stream(), s->nb_dof / 3,
+                         s->row_index, s->columns, s->values, w.dinv, (float*)nullptr);
+    else
+      hipLaunchKernelGGL(k_diag_inverse, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
+                         s->row_index, s->columns, s->values, w.dinv, (float*)nullptr);
+    check_launch("noalloc_jacobi_setup");
+  }
+  if (!o.use_initial_guess) {
+    hipLaunchKernelGGL(k_initial_guess, dim3(grid_for(s->nb_dof, 256)), dim3(256), 0, stream(), s->nb_dof,
+                       s->elim_info.p, s->elim_value.p, s->x.p);
+    check_launch("noalloc_initial_guess");
+  }
+  State init{};
+  init.rtol2 = o.rtol * o.rtol;
+  init.atol2 = o.atol * o.atol;
+  init.delta2 = 0.01;
+  init.natural = o.norm;
+  int total_iter = 0, restarts = 0;
+  bool converged = false;
+  double relres = 0.0;
+  for (;;) {
+    init.max_iter = o.max_iter - total_iter;
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, stream(), init, w.st);
+    check_launch("noalloc_state_init");
+    const Gate always{ w.st, kAlways, 0, nullptr, nblk };
+    noalloc_spmv<kResidual>(s, s->x.p, w.r, s->rhs.p, always, w.P.rr64, nblk);
+    if (s->block == 3)
+      noalloc_init<3>(s, w, precond, o.norm == 2, nblk);
+    else
+      noalloc_init<1>(s, w, precond, o.norm == 2, nblk);
+    State h = noalloc_read_state(w);
+    int k = 0;
+    while (!h.done && k < init.max_iter) {
+      for (int j = 0; j < check_every && k < init.max_iter; ++j, ++k) {
+        if (s->block == 3)
+          noalloc_iteration<3>(s, w, k, precond, nblk);
+        else
+          noalloc_iteration<1>(s, w, k, precond, nblk);
+      }
+      h = noalloc_read_state(w);
+    }
+    total_iter += h.iter;
+    const double bnorm = std::sqrt(h.bb);
+    // ||b - A x||^2 with the FP64 matrix (r overwritten)
+    noalloc_spmv<kResidual>(s, s->x.p, w.r, s->rhs.p, always, w.P.rr64, nblk);
+    hipLaunchKernelGGL(k_true_residual_finalize, dim3(1), dim3(kBlock), 0, stream(), w.P, nblk, w.st);
+    check_launch("noalloc_true_residual");
+    const State t = noalloc_read_state(w);
+    const double rr = t.rr_true;
+    relres = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+    if (o.norm == 2)
+      relres = bnorm > 0.0 ? std::sqrt(h.rr_out) / bnorm : std::sqrt(h.rr_out);
+    if (o.norm != 0) {
+      converged = h.converged != 0 && std::isfinite(relres);
+      break;
+    }
+    converged = relres <= o.rtol || std::sqrt(rr) <= o.atol;
+    if (converged || total_iter >= o.max_iter || restarts >= 8 || !std::isfinite(relres) || h.iter == 0)
+      break;
+    ++restarts;
+  }
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  auto t_end = std::chrono::steady_clock::now();
+  if (res) {
+    res->iterations = total_iter;
+    res->converged = converged ? 1 : 0;
+    res->reliable_updates = 0;
+    res->relres = relres;
+    res->solve_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    res->fallback = 0;
+  }
+  return converged ? AFEM_OK : AFEM_ERR_NOT_CONVERGED;
+}
+
 } // namespace afem
 
 using namespace afem;
@@ -1979,6 +2210,22 @@ AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt
   if (opt)
     o = *opt;
   int rc = solve(sys, o, result);
+  if (rc != AFEM_OK) {
+    set_last_error("solver reached max_iter before the tolerance");
+    return rc;
+  }
+  AFEM_GUARD_END
+}
+
+AFEM_API int afem_system_solve_noalloc(afem_system_t sys, const afem_solver_options* opt, afem_solve_result* result)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(sys, AFEM_ERR_ARGUMENT, "null handle");
+  afem_solver_options o;
+  afem_solver_default_options(&o);
+  if (opt)
+    o = *opt;
+  int rc = solve_noalloc(sys, o, result);
   if (rc != AFEM_OK) {
     set_last_error("solver reached max_iter before the tolerance");
     return rc;

@@ -403,6 +403,18 @@ AFEM_API void afem_solver_default_options(afem_solver_options* opt);
  * (with the result filled) when max_iter is reached. */
 AFEM_API int afem_system_solve(afem_system_t sys, const afem_solver_options* opt,
                                afem_solve_result* result);
+/* No counterpart in the reference.  For callers that cannot allocate device or
+ * pinned memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): afem_system_solve for the FP64 methods (AFEM_SOLVER_PCG_FP64,
+ * AFEM_SOLVER_CG_FP64; Jacobi, block-Jacobi or no preconditioner; stopping norms
+ * 0-2), with the SpMV on the system's own CSR / 3x3 BSR view (16 lanes per row)
+ * and every work vector, reduction partial and the solver state in a fixed
+ * device array of the library (56 MiB per device, held until the process ends).
+ * Allocates nothing.  At most 1048576 DoFs (AFEM_ERR_ARGUMENT beyond).  Every
+ * call on one device shares that array: calls that overlap on a device race.
+ * Synchronous, like afem_system_solve; the same result and error conventions. */
+AFEM_API int afem_system_solve_noalloc(afem_system_t sys, const afem_solver_options* opt,
+                                       afem_solve_result* result);
 
 /* ---- sparse kernels exposed for testing and benchmarking ---------------------- */
 
