@@ -144,6 +144,7 @@ def _declare(L):
         "afem_batch_default_options": (None, [ctypes.POINTER(BatchOptions)]),
         "afem_batch_create": (ci, [vp, ci, vp, pp]),
         "afem_batch_destroy": (ci, [vp]),
+        "afem_batch_persistent_mode": (ci, [vp, ctypes.POINTER(ci)]),
         "afem_batch_set_tiles": (ci, [vp, ci, ci]),
         "afem_batch_autotune": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(dbl)]),
         "afem_batch_solve": (ci, [vp, ctypes.POINTER(BatchOptions), vp, ctypes.POINTER(dbl)]),
@@ -652,8 +653,10 @@ def union_mesh(meshes):
 
 class BatchSolver:
     """Independent systems on the diagonal blocks of one system's matrix,
-    solved together (segmented mixed-precision PCG: one persistent launch by
-    default, or hipGraph chunks of iterations with persistent=0)."""
+    solved together (segmented mixed-precision PCG: hipGraph chunks of
+    iterations by default; one persistent launch with persistent=1, which
+    keeps each system's team on one XCD where the device allows it, or
+    persistent=2, which exchanges through the coherence point)."""
 
     def __init__(self, sysm, segment_rows):
         seg = np.ascontiguousarray(segment_rows, dtype=np.int32)
@@ -682,6 +685,13 @@ class BatchSolver:
         if rc != OK and (raise_on_failure or rc != ERR_NOT_CONVERGED):
             check(rc)
         return list(res), ms.value
+
+    def persistent_mode(self):
+        """Exchange of the last persistent solve: 1 XCD-local teams, 2
+        coherence point, 0 none yet."""
+        m = ctypes.c_int()
+        check(lib().afem_batch_persistent_mode(self.h, ctypes.byref(m)))
+        return m.value
 
     def __del__(self):
         try:

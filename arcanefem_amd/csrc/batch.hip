@@ -583,8 +583,9 @@ __global__ void k_prepare(int32_t n, const int32_t* __restrict__ rp, const int32
 
 struct Team
 {
-  const int32_t* seg_of; // segment of each launched workgroup
-  const int32_t* first;  // first workgroup of each segment's team (nb_seg + 1)
+  const int32_t* seg_of; // segment of each launched workgroup (-1: idle)
+  const int32_t* slot_of; // team-contiguous index of each launched workgroup (partial sums)
+  const int32_t* first;  // first team index of each segment's team (nb_seg + 1)
   const int32_t* rows;   // segment row offsets (nb_seg + 1)
   const int32_t* strip;  // rows per workgroup of each segment's team (multiple of 64)
   // SELL copy (sell != 0) or CSR arrays
@@ -600,6 +601,7 @@ struct Team
   int* abort_flag;         // zero at the launch
   const int* host_cancel;  // pinned host flag (may be null)
   long long spin_limit;
+  int xcc[8];              // XCD-local layout: XCD id of launched workgroups g with g mod 8 = c
 };
 
 struct TeamVec
@@ -659,8 +661,29 @@ __device__ __forceinline__ double fetch(const double* p)
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Exchanged vectors: XCD-local teams (LOCAL) use ordinary cached accesses --
+// every workgroup of the team shares the XCD's L2, and the L1 is written
+// through and invalidated at each barrier -- others the coherence point.
+template <bool LOCAL, class V>
+__device__ __forceinline__ void xstore(V* p, V v)
+{
+  if constexpr (LOCAL)
+    *p = v;
+  else
+    publish(p, v);
+}
+template <bool LOCAL, class V>
+__device__ __forceinline__ V xload(const V* p)
+{
+  if constexpr (LOCAL)
+    return *p;
+  else
+    return fetch(p);
+}
+
 // Barrier of the team of segment s (nw workgroups); `phase` counts this
 // workgroup's barriers.  False: give up (abort).
+template <bool LOCAL>
 __device__ bool team_barrier(const Team& T, int s, int nw, unsigned& phase)
 {
   __shared__ int s_ok;
@@ -681,12 +704,14 @@ __device__ bool team_barrier(const Team& T, int s, int nw, unsigned& phase)
                                                                        __ATOMIC_RELAXED,
                                                                        __HIP_MEMORY_SCOPE_SYSTEM) != 0);
       if (give_up) {
-        __hip_atomic_store(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_max(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    if constexpr (LOCAL) // drop the CU's L1 lines: the team's data is current in the XCD's L2
+      asm volatile("buffer_inv sc1" ::: "memory");
     s_ok = ok;
   }
   __syncthreads();
@@ -715,12 +740,14 @@ __device__ __forceinline__ double team_sum(const double* p, int t0, int t1)
 }
 
 // p_new of column c: the same fma at the owner and at every gather
+template <bool LOCAL>
 __device__ __forceinline__ float p_of(const float* z, const float* pold, float beta, int64_t c)
 {
-  return __fmaf_rn(beta, fetch(pold + c), fetch(z + c));
+  return __fmaf_rn(beta, xload<LOCAL>(pold + c), xload<LOCAL>(z + c));
 }
 
 // Row r of A32 p_new (p_new formed from z and p_old on the fly).
+template <bool LOCAL>
 __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const float* pold, float beta, int32_t r,
                                             int32_t sr0, int32_t sl0)
 {
@@ -730,6 +757,25 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
     const int32_t sl = sl0 + (local >> 6);
     const int64_t base = (int64_t)T.soff[sl] * 64 + (local & 63);
     const int w = T.soff[sl + 1] - T.soff[sl];
+    if (w <= 8) { // every slot's loads in one round, then every gather in one round
+      int d[8];
+      float m[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        d[u] = u < w ? T.sdcol[base + 64 * u] : 0;
+        m[u] = u < w ? T.v32[base + 64 * u] : 0.f;
+      }
+      float g[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        g[u] = u < w ? p_of<LOCAL>(z, pold, beta, (int64_t)r + d[u]) : 0.f;
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        s0 += m[u] * g[u];
+        s1 += m[u + 1] * g[u + 1];
+      }
+      return s0 + s1;
+    }
     float s2 = 0.f, s3 = 0.f;
     int j = 0;
     for (; j + 3 < w; j += 4) { // four slots' loads in flight
@@ -741,14 +787,14 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
         d[u] = T.sdcol[i0 + 64 * u];
         m[u] = T.v32[i0 + 64 * u];
       }
-      s0 += m[0] * p_of(z, pold, beta, (int64_t)r + d[0]);
-      s1 += m[1] * p_of(z, pold, beta, (int64_t)r + d[1]);
-      s2 += m[2] * p_of(z, pold, beta, (int64_t)r + d[2]);
-      s3 += m[3] * p_of(z, pold, beta, (int64_t)r + d[3]);
+      s0 += m[0] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[0]);
+      s1 += m[1] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[1]);
+      s2 += m[2] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[2]);
+      s3 += m[3] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[3]);
     }
     for (; j < w; ++j) {
       const int64_t i0 = base + (int64_t)j * 64;
-      s0 += T.v32[i0] * p_of(z, pold, beta, (int64_t)r + T.sdcol[i0]);
+      s0 += T.v32[i0] * p_of<LOCAL>(z, pold, beta, (int64_t)r + T.sdcol[i0]);
     }
     s0 += s2;
     s1 += s3;
@@ -757,16 +803,17 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
     const int32_t e = T.rp[r + 1];
     int32_t k = T.rp[r];
     for (; k + 1 < e; k += 2) {
-      s0 += T.v32[k] * p_of(z, pold, beta, T.ci[k]);
-      s1 += T.v32[k + 1] * p_of(z, pold, beta, T.ci[k + 1]);
+      s0 += T.v32[k] * p_of<LOCAL>(z, pold, beta, T.ci[k]);
+      s1 += T.v32[k + 1] * p_of<LOCAL>(z, pold, beta, T.ci[k + 1]);
     }
     if (k < e)
-      s0 += T.v32[k] * p_of(z, pold, beta, T.ci[k]);
+      s0 += T.v32[k] * p_of<LOCAL>(z, pold, beta, T.ci[k]);
   }
   return s0 + s1;
 }
 
 // Row r of b - A64 x.
+template <bool LOCAL>
 __device__ __forceinline__ double residual_row64(const Team& T, const double* x, const double* b, int32_t r,
                                                  int32_t sr0, int32_t sl0)
 {
@@ -779,23 +826,23 @@ __device__ __forceinline__ double residual_row64(const Team& T, const double* x,
     int j = 0;
     for (; j + 1 < w; j += 2) {
       const int64_t i0 = base + (int64_t)j * 64, i1 = i0 + 64;
-      s0 += T.v64[i0] * fetch(x + (int64_t)r + T.sdcol[i0]);
-      s1 += T.v64[i1] * fetch(x + (int64_t)r + T.sdcol[i1]);
+      s0 += T.v64[i0] * xload<LOCAL>(x + (int64_t)r + T.sdcol[i0]);
+      s1 += T.v64[i1] * xload<LOCAL>(x + (int64_t)r + T.sdcol[i1]);
     }
     if (j < w) {
       const int64_t i0 = base + (int64_t)j * 64;
-      s0 += T.v64[i0] * fetch(x + (int64_t)r + T.sdcol[i0]);
+      s0 += T.v64[i0] * xload<LOCAL>(x + (int64_t)r + T.sdcol[i0]);
     }
   }
   else {
     const int32_t e = T.rp[r + 1];
     int32_t k = T.rp[r];
     for (; k + 1 < e; k += 2) {
-      s0 += T.v64[k] * fetch(x + T.ci[k]);
-      s1 += T.v64[k + 1] * fetch(x + T.ci[k + 1]);
+      s0 += T.v64[k] * xload<LOCAL>(x + T.ci[k]);
+      s1 += T.v64[k + 1] * xload<LOCAL>(x + T.ci[k + 1]);
     }
     if (k < e)
-      s0 += T.v64[k] * fetch(x + T.ci[k]);
+      s0 += T.v64[k] * xload<LOCAL>(x + T.ci[k]);
   }
   return b[r] - (s0 + s1);
 }
@@ -818,17 +865,36 @@ __device__ __forceinline__ void team_put(double v, double* out, int gb)
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBT, 5) // 5 waves per SIMD without spills: 5 workgroups per CU
+// LOCAL: every team's workgroups run on one XCD (the device dispatches
+// launched workgroup g to XCD g mod 8; a probe launch recorded which XCD id
+// each class gets); each workgroup checks its XCD id and, on a mismatch,
+// aborts the launch with abort flag 2, after which the host restores x and
+// runs the solve again with the coherent exchange.  A team with a missing
+// member never passes its first barrier, so no result of it is kept.
+template <bool LOCAL>
+__global__ void __launch_bounds__(kBT, 4) // 4 waves per SIMD without spills: 4 workgroups per CU
 k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
 {
   const int gb = blockIdx.x;
   const int s = T.seg_of[gb];
+  if (s < 0)
+    return; // an idle slot of the XCD-local layout
+  if constexpr (LOCAL) {
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    if ((int)xcc != T.xcc[gb & 7]) {
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_max(T.abort_flag, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+  const int slot = T.slot_of[gb];
   const int t0 = T.first[s], t1 = T.first[s + 1], nw = t1 - t0;
   const int32_t sr0 = T.rows[s], sr1 = T.rows[s + 1];
-  const int32_t r0 = sr0 + (gb - t0) * T.strip[s];
+  const int32_t r0 = sr0 + (slot - t0) * T.strip[s];
   const int32_t r1 = min(sr1, r0 + T.strip[s]);
   const int32_t sl0 = T.sell ? T.sslice[s] : 0;
-  const bool lead = gb == t0 && threadIdx.x == 0;
+  const bool lead = slot == t0 && threadIdx.x == 0;
   SegState& S = st[s];
   if (!S.active || team_abort(T))
     return;
@@ -838,20 +904,20 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
   // init: r64 = b - A64 x; r = r64, z = D r, p_old = 0; partials r.z, r.r, b.b
   double rz = 0.0, rr = 0.0, bb = 0.0;
   for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
-    const double rd = residual_row64(T, V.x, V.b, i, sr0, sl0);
+    const double rd = residual_row64<LOCAL>(T, V.x, V.b, i, sr0, sl0);
     const float rv = (float)rd;
     const float zv = V.dinv[i] * rv;
     V.r[i] = rv;
-    publish(V.z + i, zv);
-    publish(V.p[1] + i, 0.f);
+    xstore<LOCAL>(V.z + i, zv);
+    xstore<LOCAL>(V.p[1] + i, 0.f);
     rz += (double)rv * (double)zv;
     rr += rd * rd;
     bb += V.b[i] * V.b[i];
   }
-  team_put(rz, P.rz, gb);
-  team_put(rr, P.rr, gb);
-  team_put(bb, P.bb, gb);
-  if (!team_barrier(T, s, nw, phase))
+  team_put(rz, P.rz, slot);
+  team_put(rr, P.rr, slot);
+  team_put(bb, P.bb, slot);
+  if (!team_barrier<LOCAL>(T, s, nw, phase))
     return;
   rz = team_sum(P.rz, t0, t1);
   rr = team_sum(P.rr, t0, t1);
@@ -869,15 +935,25 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
     const float* pold = V.p[par ^ 1];
     const float bt = (float)beta;
     double pq = 0.0;
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
-      const float y = prod_row32(T, V.z, pold, bt, i, sr0, sl0);
-      const float pi = p_of(V.z, pold, bt, i);
+    // two rows per step, so that their loads are in flight together
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 2 * kBT) {
+      const int32_t i2 = i + kBT;
+      const bool two = i2 < r1;
+      const float y = prod_row32<LOCAL>(T, V.z, pold, bt, i, sr0, sl0);
+      const float y2 = two ? prod_row32<LOCAL>(T, V.z, pold, bt, i2, sr0, sl0) : 0.f;
+      const float pi = p_of<LOCAL>(V.z, pold, bt, i);
+      const float pi2 = two ? p_of<LOCAL>(V.z, pold, bt, i2) : 0.f;
       V.q[i] = y;
-      publish(pnew + i, pi); // other gathers of this phase read pold, a different buffer
+      xstore<LOCAL>(pnew + i, pi); // other gathers of this phase read pold, a different buffer
       pq += (double)y * (double)pi;
+      if (two) {
+        V.q[i2] = y2;
+        xstore<LOCAL>(pnew + i2, pi2);
+        pq += (double)y2 * (double)pi2;
+      }
     }
-    team_put(pq, P.pq, gb);
-    if (!team_barrier(T, s, nw, phase))
+    team_put(pq, P.pq, slot);
+    if (!team_barrier<LOCAL>(T, s, nw, phase))
       return;
     // B: alpha; x, r, z; partials r.z, r.r
     pq = team_sum(P.pq, t0, t1);
@@ -889,18 +965,19 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
     const float af = (float)alpha;
     rz = 0.0;
     rr = 0.0;
+#pragma unroll 4
     for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
-      publish(V.x + i, fetch(V.x + i) + alpha * (double)fetch(pnew + i));
+      xstore<LOCAL>(V.x + i, xload<LOCAL>(V.x + i) + alpha * (double)xload<LOCAL>(pnew + i));
       const float rv = V.r[i] - af * V.q[i];
       V.r[i] = rv;
       const float zv = V.dinv[i] * rv;
-      publish(V.z + i, zv);
+      xstore<LOCAL>(V.z + i, zv);
       rz += (double)rv * (double)zv;
       rr += (double)rv * (double)rv;
     }
-    team_put(rz, P.rz, gb);
-    team_put(rr, P.rr, gb);
-    if (!team_barrier(T, s, nw, phase))
+    team_put(rz, P.rz, slot);
+    team_put(rr, P.rr, slot);
+    if (!team_barrier<LOCAL>(T, s, nw, phase))
       return;
     rz = team_sum(P.rz, t0, t1);
     rr = team_sum(P.rr, t0, t1);
@@ -909,12 +986,12 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
     if (ru) {
       double part = 0.0;
       for (int32_t i = r0 + threadIdx.x; i < r1; i += kBT) {
-        const double rd = residual_row64(T, V.x, V.b, i, sr0, sl0);
+        const double rd = residual_row64<LOCAL>(T, V.x, V.b, i, sr0, sl0);
         V.r64[i] = rd;
         part += rd * rd;
       }
-      team_put(part, P.rr64, gb);
-      if (!team_barrier(T, s, nw, phase))
+      team_put(part, P.rr64, slot);
+      if (!team_barrier<LOCAL>(T, s, nw, phase))
         return;
       const double rr64 = team_sum(P.rr64, t0, t1);
       ++nb_ru;
@@ -929,11 +1006,11 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
         const float rv = (float)V.r64[i];
         V.r[i] = rv;
         const float zv = V.dinv[i] * rv;
-        publish(V.z + i, zv);
+        xstore<LOCAL>(V.z + i, zv);
         rzn += (double)rv * (double)zv;
       }
-      team_put(rzn, P.rzru, gb);
-      if (!team_barrier(T, s, nw, phase))
+      team_put(rzn, P.rzru, slot);
+      if (!team_barrier<LOCAL>(T, s, nw, phase))
         return;
       rho = team_sum(P.rzru, t0, t1);
       rr_ref = rr64;
@@ -953,6 +1030,16 @@ k_persistent(Team T, TeamVec V, TeamParts P, SegState* st)
     S.bb = bb;
     S.tol2 = tol2;
     S.rr_out = rr_out;
+  }
+}
+
+// XCD id of each launched workgroup.
+__global__ void k_xcc_probe(int* out)
+{
+  if (threadIdx.x == 0) {
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    out[blockIdx.x] = (int)xcc;
   }
 }
 
@@ -1001,8 +1088,13 @@ struct afem_batch
   int pt_budget = 0; // requested workgroups (0 = default)
   int pt_grid = 0;   // workgroups of the layout
   int pt_layout_budget = -1;
+  int pt_layout_local = -1; // layout kind built: 1 XCD-local teams, 0 contiguous
+  int pt_local_ok = -1;     // XCD-local teams usable: -1 not probed yet
+  int pt_xcc[8] = {};
+  int pt_last_mode = 0;     // 1 XCD-local, 2 coherent, 0 none yet
   std::vector<int32_t> pt_first_host, pt_strip_host;
-  DBuf<int32_t> pt_seg_of, pt_first, pt_strip;
+  DBuf<int32_t> pt_seg_of, pt_slot_of, pt_first, pt_strip;
+  DBuf<double> pt_xsave; // x at the launch of an XCD-local solve
   DBuf<float> p32b;
   DBuf<double> pt_parts;
   DBuf<unsigned> pt_cnt;
@@ -1307,17 +1399,27 @@ void drop_persistent_graph(afem_batch* b)
 // Workgroups of k_persistent the device holds at once.
 int persistent_capacity()
 {
-  int per_cu = 0;
-  AFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, kBT, 0));
-  return std::max(0, per_cu) * rt().num_cu;
+  int per_cu = 0, per_cu_local = 0;
+  AFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent<false>, kBT, 0));
+  AFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_local, k_persistent<true>, kBT, 0));
+  return std::max(0, std::min(per_cu, per_cu_local)) * rt().num_cu;
 }
 
 // Teams: segment s gets m_s workgroups in proportion to rows^1.5 (its work
 // per iteration times the growth of its iteration count with its size), at
 // least one and at most one per 256 rows; strips are multiples of 64 rows.
-void persistent_layout(afem_batch* b, int budget)
+// Team slots (the partial-sum index of each member) are contiguous per team.
+// Contiguous layout: launched workgroup g is slot g.  XCD-local layout: the
+// teams are packed onto the 8 XCDs (largest first, onto the least loaded XCD,
+// at most cap/8 workgroups each) and the j-th workgroup of XCD x is launched
+// as g = 8 j + x; unused g are idle.  False: the XCD-local packing does not
+// fit (the caller uses the contiguous layout).
+bool persistent_layout(afem_batch* b, int budget, int cap, bool local)
 {
   const int ns = b->nb_seg;
+  const int per_xcd = cap / 8;
+  if (local && (per_xcd < 1 || ns > 8 * per_xcd))
+    return false;
   std::vector<double> w(ns);
   double wsum = 0.0;
   for (int k = 0; k < ns; ++k) {
@@ -1325,42 +1427,96 @@ void persistent_layout(afem_batch* b, int budget)
     w[k] = r * std::sqrt(r);
     wsum += w[k];
   }
-  std::vector<int32_t> m(ns);
+  std::vector<int32_t> m(ns), strip(ns);
   auto assign = [&](double scale) {
     int64_t tot = 0;
     for (int k = 0; k < ns; ++k) {
       const int32_t rows = b->seg_rows[k + 1] - b->seg_rows[k];
-      const int32_t cap = std::max(1, (rows + 255) / 256);
-      m[k] = (int32_t)std::min<double>(cap, std::max(1.0, std::floor(scale * w[k] / wsum)));
+      int32_t mk = (int32_t)std::min<double>(std::max(1, (rows + 255) / 256),
+                                             std::max(1.0, std::floor(scale * w[k] / wsum)));
+      if (local)
+        mk = std::min(mk, per_xcd);
+      int32_t st = (rows + mk - 1) / mk;
+      st = (st + 63) / 64 * 64;
+      strip[k] = st;
+      m[k] = (rows + st - 1) / st;
       tot += m[k];
     }
     return tot;
   };
+  // XCD packing of the current m: xcd_of[k], members placed per XCD
+  std::vector<int> xcd_of(ns, 0), order(ns);
+  int load[8];
+  auto pack = [&]() {
+    for (int k = 0; k < ns; ++k)
+      order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int i, int j) { return m[i] > m[j]; });
+    for (int x = 0; x < 8; ++x)
+      load[x] = 0;
+    for (int k : order) {
+      int best = 0;
+      for (int x = 1; x < 8; ++x)
+        if (load[x] < load[best])
+          best = x;
+      if (load[best] + m[k] > per_xcd)
+        return false;
+      xcd_of[k] = best;
+      load[best] += m[k];
+    }
+    return true;
+  };
   double scale = budget;
   while (assign(scale) > budget && scale > 1.0)
     scale *= 0.95;
-  b->pt_first_host.assign(ns + 1, 0);
-  b->pt_strip_host.assign(ns, 0);
-  for (int k = 0; k < ns; ++k) {
-    const int32_t rows = b->seg_rows[k + 1] - b->seg_rows[k];
-    int32_t strip = (rows + m[k] - 1) / m[k];
-    strip = (strip + 63) / 64 * 64;
-    const int32_t mk = (rows + strip - 1) / strip;
-    b->pt_strip_host[k] = strip;
-    b->pt_first_host[k + 1] = b->pt_first_host[k] + mk;
+  if (local) {
+    while (!pack()) {
+      if (scale <= 1.0)
+        return false;
+      scale *= 0.9;
+      assign(scale);
+    }
   }
-  b->pt_grid = b->pt_first_host[ns];
-  std::vector<int32_t> seg_of(b->pt_grid);
+  b->pt_first_host.assign(ns + 1, 0);
+  b->pt_strip_host.assign(strip.begin(), strip.end());
   for (int k = 0; k < ns; ++k)
-    for (int g = b->pt_first_host[k]; g < b->pt_first_host[k + 1]; ++g)
-      seg_of[g] = k;
+    b->pt_first_host[k + 1] = b->pt_first_host[k] + m[k];
+  const int members = b->pt_first_host[ns];
+  std::vector<int32_t> seg_of, slot_of;
+  if (local) {
+    int depth = 0;
+    for (int x = 0; x < 8; ++x)
+      depth = std::max(depth, load[x]);
+    b->pt_grid = 8 * depth;
+    seg_of.assign(b->pt_grid, -1);
+    slot_of.assign(b->pt_grid, -1);
+    int used[8] = {};
+    for (int k = 0; k < ns; ++k)
+      for (int j = 0; j < m[k]; ++j) {
+        const int x = xcd_of[k];
+        const int g = 8 * used[x]++ + x;
+        seg_of[g] = k;
+        slot_of[g] = b->pt_first_host[k] + j;
+      }
+  }
+  else {
+    b->pt_grid = members;
+    seg_of.resize(members);
+    slot_of.resize(members);
+    for (int k = 0; k < ns; ++k)
+      for (int g = b->pt_first_host[k]; g < b->pt_first_host[k + 1]; ++g) {
+        seg_of[g] = k;
+        slot_of[g] = g;
+      }
+  }
   b->pt_seg_of.alloc(seg_of.size());
   b->pt_seg_of.upload(seg_of.data(), seg_of.size());
+  b->pt_slot_of.alloc(slot_of.size());
+  b->pt_slot_of.upload(slot_of.data(), slot_of.size());
   b->pt_first.alloc(ns + 1);
   b->pt_first.upload(b->pt_first_host.data(), ns + 1);
   b->pt_strip.alloc(ns);
   b->pt_strip.upload(b->pt_strip_host.data(), ns);
-  b->pt_parts.alloc((size_t)6 * b->pt_grid);
+  b->pt_parts.alloc((size_t)6 * members);
   b->pt_cnt.alloc(ns);
   b->pt_abort.alloc(1);
   if (!b->pt_cancel) {
@@ -1369,7 +1525,35 @@ void persistent_layout(afem_batch* b, int budget)
   }
   *b->pt_cancel = 0;
   b->pt_layout_budget = budget;
+  b->pt_layout_local = local ? 1 : 0;
   drop_persistent_graph(b);
+  return true;
+}
+
+// XCD-local teams need the device to dispatch launched workgroup g to XCD
+// g mod 8 (8 XCDs of equal size): one probe launch of the layout's grid
+// records each workgroup's XCD id, and every class g mod 8 must show one id.
+bool probe_xcd_dispatch(afem_batch* b)
+{
+  if (rt().num_cu <= 0 || rt().num_cu % 8 != 0)
+    return false;
+  const int n = std::max(b->pt_grid, 64);
+  DBuf<int> out(n);
+  hipLaunchKernelGGL(k_xcc_probe, dim3(n), dim3(64), 0, stream(), out.p);
+  check_launch("batch_xcc_probe");
+  std::vector<int> h(n);
+  AFEM_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(int) * n, hipMemcpyDeviceToHost, stream()));
+  AFEM_HIP(hipStreamSynchronize(stream()));
+  for (int c = 0; c < 8; ++c)
+    b->pt_xcc[c] = h[c];
+  for (int g = 0; g < n; ++g)
+    if (h[g] != h[g & 7])
+      return false;
+  for (int c = 0; c < 8; ++c)
+    for (int d = 0; d < c; ++d)
+      if (h[c] == h[d])
+        return false;
+  return true;
 }
 
 struct PersistentArgs
@@ -1386,6 +1570,7 @@ PersistentArgs persistent_args(afem_batch* b, long long spin_limit)
   std::memset(&a, 0, sizeof(a)); // padding too: the graph key hashes the bytes
   Team& T = a.T;
   T.seg_of = b->pt_seg_of.p;
+  T.slot_of = b->pt_slot_of.p;
   T.first = b->pt_first.p;
   T.rows = b->d_rows.p;
   T.strip = b->pt_strip.p;
@@ -1403,8 +1588,10 @@ PersistentArgs persistent_args(afem_batch* b, long long spin_limit)
   AFEM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_cancel), b->pt_cancel, 0));
   T.host_cancel = dev_cancel;
   T.spin_limit = spin_limit;
+  for (int c = 0; c < 8; ++c)
+    T.xcc[c] = b->pt_xcc[c];
   a.V = TeamVec{ s->x.p, s->rhs.p, b->dinv.p, b->r32.p, b->z32.p, { b->p32.p, b->p32b.p }, b->q32.p, b->r64.p };
-  const int n = b->pt_grid;
+  const int n = b->pt_first_host[b->nb_seg];
   double* pp = b->pt_parts.p;
   a.P = TeamParts{ pp, pp + n, pp + 2 * n, pp + 3 * n, pp + 4 * n, pp + 5 * n };
   return a;
@@ -1412,7 +1599,7 @@ PersistentArgs persistent_args(afem_batch* b, long long spin_limit)
 
 // FNV-1a over the launch arguments: a captured graph is replayed only with
 // exactly the buffers and settings it was captured with.
-uint64_t persistent_key(const PersistentArgs& a, int grid)
+uint64_t persistent_key(const PersistentArgs& a, int grid, int local)
 {
   uint64_t h = 1469598103934665603ULL;
   auto mix = [&h](const void* p, size_t n) {
@@ -1424,41 +1611,34 @@ uint64_t persistent_key(const PersistentArgs& a, int grid)
   };
   mix(&a, sizeof(a));
   mix(&grid, sizeof(grid));
+  mix(&local, sizeof(local));
   return h;
 }
 
-void launch_persistent(afem_batch* b, const PersistentArgs& a)
+void launch_persistent(afem_batch* b, const PersistentArgs& a, bool local)
 {
   hipLaunchKernelGGL(k_persistent_clear, dim3(1), dim3(256), 0, stream(), b->pt_cnt.p, b->nb_seg, b->pt_abort.p);
-  hipLaunchKernelGGL(k_persistent, dim3(b->pt_grid), dim3(kBT), 0, stream(), a.T, a.V, a.P, b->st.p);
+  if (local)
+    hipLaunchKernelGGL(k_persistent<true>, dim3(b->pt_grid), dim3(kBT), 0, stream(), a.T, a.V, a.P, b->st.p);
+  else
+    hipLaunchKernelGGL(k_persistent<false>, dim3(b->pt_grid), dim3(kBT), 0, stream(), a.T, a.V, a.P, b->st.p);
 }
 
-// Solves the active segments in one persistent launch (captured once in a
-// hipGraph and replayed); the host waits for it with a bounded wall time and
-// raises the cancel flag when that passes.
-void run_persistent(afem_batch* b, const afem_batch_options& o)
+// Runs the persistent launch once (captured in a hipGraph and replayed); the
+// host waits for it with a bounded wall time and raises the cancel flag when
+// that passes.  Returns the abort flag (0: finished).
+int persistent_once(afem_batch* b, const afem_batch_options& o, bool local, bool& cancelled)
 {
-  const int cap = persistent_capacity();
-  AFEM_REQUIRE(cap >= b->nb_seg, AFEM_ERR_CAPACITY,
-               "the persistent batched solve needs one resident workgroup per system at least");
-  const int want = o.workgroups > 0 ? o.workgroups : cap;
-  const int budget = std::max(b->nb_seg, std::min(want, cap));
-  if (b->pt_layout_budget != budget || (size_t)b->pt_first_host.size() != (size_t)b->nb_seg + 1)
-    persistent_layout(b, budget);
-  if (b->p32b.n != b->p32.n) {
-    b->p32b.alloc(b->p32.n);
-    drop_persistent_graph(b);
-  }
   const long long spin_limit = o.spin_limit > 0 ? o.spin_limit : (1LL << 21);
   *b->pt_cancel = 0;
   const PersistentArgs args = persistent_args(b, spin_limit);
   if (o.use_graph) {
-    const uint64_t key = persistent_key(args, b->pt_grid);
+    const uint64_t key = persistent_key(args, b->pt_grid, local ? 1 : 0);
     if (!b->pt_graph || b->pt_graph_key != key) {
       drop_persistent_graph(b);
       hipGraph_t gr;
       AFEM_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-      launch_persistent(b, args);
+      launch_persistent(b, args, local);
       AFEM_HIP(hipStreamEndCapture(stream(), &gr));
       const hipError_t e = hipGraphInstantiate(&b->pt_graph, gr, nullptr, nullptr, 0);
       (void)hipGraphDestroy(gr);
@@ -1468,12 +1648,12 @@ void run_persistent(afem_batch* b, const afem_batch_options& o)
     AFEM_HIP(hipGraphLaunch(b->pt_graph, stream()));
   }
   else {
-    launch_persistent(b, args);
+    launch_persistent(b, args, local);
   }
   check_launch("batch_persistent");
   const auto t0 = std::chrono::steady_clock::now();
   const double limit_ms = o.timeout_ms > 0.0 ? o.timeout_ms : 60000.0;
-  bool cancelled = false;
+  cancelled = false;
   for (;;) {
     const hipError_t q = hipStreamQuery(stream());
     if (q == hipSuccess)
@@ -1490,6 +1670,55 @@ void run_persistent(afem_batch* b, const afem_batch_options& o)
   AFEM_HIP(hipMemcpyAsync(&aborted, b->pt_abort.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
   AFEM_HIP(hipStreamSynchronize(stream()));
   *b->pt_cancel = 0;
+  return aborted;
+}
+
+// Solves the active segments in one persistent launch.  persistent = 1 uses
+// XCD-local teams when the device's dispatch allows it (probed once per
+// batch) and the packing fits, else the coherent exchange; persistent = 2
+// always uses the coherent exchange.
+void run_persistent(afem_batch* b, const afem_batch_options& o)
+{
+  const int cap = persistent_capacity();
+  AFEM_REQUIRE(cap >= b->nb_seg, AFEM_ERR_CAPACITY,
+               "the persistent batched solve needs one resident workgroup per system at least");
+  const int want = o.workgroups > 0 ? o.workgroups : cap;
+  const int budget = std::max(b->nb_seg, std::min(want, cap));
+  bool local = o.persistent == 1 && b->pt_local_ok != 0;
+  if (b->p32b.n != b->p32.n) {
+    b->p32b.alloc(b->p32.n);
+    drop_persistent_graph(b);
+  }
+  if (b->pt_layout_budget != budget || b->pt_layout_local != (local ? 1 : 0) ||
+      (size_t)b->pt_first_host.size() != (size_t)b->nb_seg + 1) {
+    if (local && !persistent_layout(b, budget, cap, true))
+      local = false;
+    if (!local)
+      persistent_layout(b, budget, cap, false);
+  }
+  if (local && b->pt_local_ok < 0) {
+    b->pt_local_ok = probe_xcd_dispatch(b) ? 1 : 0;
+    local = b->pt_local_ok == 1;
+    if (!local)
+      persistent_layout(b, budget, cap, false);
+  }
+  afem_system* s = b->sys;
+  if (local) {
+    b->pt_xsave.reserve((size_t)s->nb_dof);
+    AFEM_HIP(hipMemcpyAsync(b->pt_xsave.p, s->x.p, sizeof(double) * s->nb_dof, hipMemcpyDeviceToDevice, stream()));
+  }
+  bool cancelled = false;
+  int aborted = persistent_once(b, o, local, cancelled);
+  if (local && aborted == 2 && !cancelled) {
+    // a workgroup ran on another XCD than probed: restore x and the states,
+    // and solve with the coherent exchange from now on
+    b->pt_local_ok = 0;
+    AFEM_HIP(hipMemcpyAsync(s->x.p, b->pt_xsave.p, sizeof(double) * s->nb_dof, hipMemcpyDeviceToDevice, stream()));
+    write_states(b);
+    persistent_layout(b, budget, cap, false);
+    aborted = persistent_once(b, o, false, cancelled);
+  }
+  b->pt_last_mode = aborted == 0 && local && b->pt_local_ok == 1 ? 1 : 2;
   AFEM_REQUIRE(!aborted && !cancelled, AFEM_ERR_DEVICE,
                cancelled ? "persistent batched solve cancelled after timeout_ms"
                          : "persistent batched solve gave up a barrier wait (spin_limit)");
@@ -1538,6 +1767,14 @@ AFEM_API int afem_batch_create(afem_system_t sys, int nb_segment, const int32_t*
   AFEM_GUARD_END
 }
 
+AFEM_API int afem_batch_persistent_mode(afem_batch_t b, int* mode)
+{
+  AFEM_GUARD_BEGIN
+  AFEM_REQUIRE(b && mode, AFEM_ERR_ARGUMENT, "null argument");
+  *mode = b->pt_last_mode;
+  AFEM_GUARD_END
+}
+
 AFEM_API int afem_batch_destroy(afem_batch_t b)
 {
   AFEM_GUARD_BEGIN
@@ -1567,6 +1804,7 @@ AFEM_API int afem_batch_solve(afem_batch_t b, const afem_batch_options* opt, afe
   system_require_matrix(s);
   AFEM_REQUIRE(s->block == 1, AFEM_ERR_NOT_IMPLEMENTED, "batched solves are for scalar systems");
   AFEM_REQUIRE(opt->rtol >= 0.0 && opt->atol >= 0.0 && opt->max_iter >= 0, AFEM_ERR_ARGUMENT, "bad tolerances");
+  AFEM_REQUIRE(opt->persistent >= 0 && opt->persistent <= 2, AFEM_ERR_ARGUMENT, "persistent must be 0, 1 or 2");
   const auto t0 = std::chrono::steady_clock::now();
   prepare(b);
   if (!opt->use_initial_guess) {

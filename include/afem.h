@@ -510,9 +510,13 @@ typedef struct afem_batch_options {
   int chunk;              /* chunked solve: iterations per captured graph launch (even; 0 = 32) */
   int use_graph;          /* 1: replay a hipGraph (per chunk, or of the persistent launch); 0: plain launches */
   int use_initial_guess;  /* 1: start from the system's solution array */
-  int persistent;         /* 1: one persistent launch; 0: chunks of iterations (default) */
-  int workgroups;         /* persistent: workgroups of the launch (0 = 2 per CU; capped at what the
-                             device holds at once; at least one per system) */
+  int persistent;         /* 0: chunks of iterations (default); 1: one persistent launch, each
+                             system's team on one XCD when the device dispatches workgroups to XCDs
+                             round-robin (probed once per batch) and the teams fit, exchanging
+                             vectors through that XCD's L2, else as 2; 2: one persistent launch,
+                             vectors exchanged through the device's coherence point */
+  int workgroups;         /* persistent: workgroups of the launch (0 = as many as the device holds
+                             at once; capped there; at least one per system) */
   long long spin_limit;   /* persistent: polls before a barrier wait gives up (0 = 2^21); the solve
                              then fails with AFEM_ERR_DEVICE instead of waiting on */
   double timeout_ms;      /* persistent: wall time after which the host raises the cancel flag that
@@ -540,6 +544,9 @@ AFEM_API int afem_batch_autotune(afem_batch_t batch, int* lanes, int* tiles, dou
  * AFEM_ERR_NOT_CONVERGED when a system misses the tolerance. */
 AFEM_API int afem_batch_solve(afem_batch_t batch, const afem_batch_options* opt,
                               afem_batch_result* results, double* total_ms);
+/* Exchange used by the last persistent solve of the batch: 1 XCD-local teams,
+ * 2 coherence point, 0 no persistent solve yet. */
+AFEM_API int afem_batch_persistent_mode(afem_batch_t batch, int* mode);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,11 @@ def test_batch_argument_errors(afem_gpu):
     bs = A.BatchSolver(case["sys"], case["offs"])
     with pytest.raises(A.AfemError):
         bs.set_tiles(3, 1)
+    for bad in (-1, 3):
+        with pytest.raises(A.AfemError) as e:
+            bs.solve(persistent=bad)
+        assert e.value.status == A.ERR_ARGUMENT
+    assert bs.persistent_mode() == 0
 
 
 def test_batch_heat_time_loop_matches_oracle(afem_gpu):
@@ -239,8 +244,12 @@ def test_persistent_matches_chunked_and_direct(afem_gpu):
     case = build_batch(A, [40, 6, 7, 8, 9, 6, 7, 8, 9, 25], seed=6)
     bs = A.BatchSolver(case["sys"], case["offs"])
     out = {}
-    for persistent in (1, 0):
+    for persistent in (1, 2, 0):
         res, _ = bs.solve(rtol=1e-10, max_iter=5000, persistent=persistent)
+        if persistent:
+            mode = bs.persistent_mode()
+            print(f"[case] persistent={persistent} exchange mode {mode}", flush=True)
+            assert mode == 2 if persistent == 2 else mode in (1, 2)
         x = case["sys"].solution().numpy().copy()
         for s_, r in enumerate(res):
             assert r.converged and r.relres <= 1e-10, (persistent, s_, r.relres)
@@ -248,23 +257,26 @@ def test_persistent_matches_chunked_and_direct(afem_gpu):
             xs = x[case["offs"][s_]:case["offs"][s_ + 1]]
             assert np.linalg.norm(xs - ref) <= 1e-8 * np.linalg.norm(ref), (persistent, s_)
         out[persistent] = res
-    for rp, rc in zip(out[1], out[0]):
-        assert abs(rp.iterations - rc.iterations) <= max(3, 0.05 * rc.iterations), (rp.iterations, rc.iterations)
+    for k in (1, 2):
+        for rp, rc in zip(out[k], out[0]):
+            assert abs(rp.iterations - rc.iterations) <= max(3, 0.05 * rc.iterations), (k, rp.iterations, rc.iterations)
 
 
+@pytest.mark.parametrize("persistent", [1, 2])
 @pytest.mark.parametrize("workgroups", [0, 10, 64, 1024])
-def test_persistent_team_sizes(afem_gpu, workgroups):
+def test_persistent_team_sizes(afem_gpu, workgroups, persistent):
     """Teams from one workgroup per system (10 systems) to as many as the
-    device holds: every size converges to the same tolerance, and a graph
-    replay of the launch equals a plain launch bit for bit."""
+    device holds, with XCD-local teams (where the device allows them) and
+    with the coherent exchange: every size converges to the same tolerance,
+    and a graph replay of the launch equals a plain launch bit for bit."""
     A = afem_gpu
-    print(f"[case] persistent workgroups={workgroups}", flush=True)
+    print(f"[case] persistent={persistent} workgroups={workgroups}", flush=True)
     case = build_batch(A, [30, 6, 7, 8, 9, 6, 7, 8, 9, 64], seed=12)
     bs = A.BatchSolver(case["sys"], case["offs"])
-    r1, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=1, persistent=1)
+    r1, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=1, persistent=persistent)
     x1 = case["sys"].solution().numpy()
     assert all(r.converged and r.relres <= 1e-9 for r in r1)
-    r2, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=0, persistent=1)
+    r2, _ = bs.solve(rtol=1e-9, max_iter=5000, workgroups=workgroups, use_graph=0, persistent=persistent)
     x2 = case["sys"].solution().numpy()
     assert [r.iterations for r in r1] == [r.iterations for r in r2]
     assert np.array_equal(x1, x2)
