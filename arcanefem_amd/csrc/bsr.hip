@@ -956,6 +956,59 @@ k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_of
     atomicOr(err, 1);
 }
 
+// Per brick slot, what the brick assembly needs of its row, gathered once
+// per pattern: the row id, its value offset, its length, diagonal position
+// and cell count (packed), and the row node's index in the brick's node
+// table.  err is set when a field does not fit its bits or the row node is
+// missing from the table.
+__global__ void __launch_bounds__(256)
+k_brick_records(const int32_t* __restrict__ perm, const int32_t* __restrict__ row_index,
+                const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ nc_off,
+                const int32_t* __restrict__ table, const int32_t* __restrict__ count, int32_t nb_node,
+                int4* __restrict__ rec, int32_t* __restrict__ err)
+{
+  __shared__ int32_t tab[kLocalMaxNodes];
+  const int32_t blk = blockIdx.x;
+  const int tn = count[blk];
+  if (tn < 0 || tn > kLocalMaxNodes) {
+    if (threadIdx.x == 0)
+      atomicOr(err, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < tn; k += 256)
+    tab[k] = table[(int64_t)blk * kLocalMaxNodes + k];
+  __syncthreads();
+  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
+  int4 r = make_int4(0, 0, 0, 0);
+  if (slot < nb_node) {
+    const int32_t i = perm[slot];
+    const int32_t a = row_index[i], len = row_index[i + 1] - a;
+    const int32_t d = diag_pos[i];
+    const int32_t nc = nc_off[i + 1] - nc_off[i];
+    const int li = find_node(tab, tn, i);
+    if (li >= tn || tab[li] != i || len > 255 || d < 0 || d > 255 || nc < 0 || nc > 65535)
+      atomicOr(err, 1);
+    r = make_int4(i, a, len | (d << 8) | (nc << 16), li);
+  }
+  rec[slot] = r;
+}
+
+// Per brick, the coordinates of its node table (xy pairs, then z) at
+// off[blk], so the assembly stages them with contiguous loads.
+__global__ void __launch_bounds__(256)
+k_brick_coords(const int32_t* __restrict__ table, const int32_t* __restrict__ count,
+               const double* __restrict__ coords, const int64_t* __restrict__ off, double* __restrict__ out)
+{
+  const int32_t blk = blockIdx.x;
+  const int tn = count[blk];
+  double* base = out + off[blk];
+  for (int k = threadIdx.x; k < tn; k += 256) {
+    const int64_t g = table[(int64_t)blk * kLocalMaxNodes + k];
+    reinterpret_cast<double2*>(base)[k] = *reinterpret_cast<const double2*>(coords + 4 * g);
+    base[2 * tn + k] = coords[4 * g + 2];
+  }
+}
+
 __host__ __device__ inline uint64_t morton_spread(uint64_t v) // 21 bits -> every third bit
 {
   v &= 0x1fffffULL;
@@ -1128,6 +1181,9 @@ static void build_brick_plan(afem_bsr* b)
     b->bslice.release();
     b->bplan.release();
     b->bcell.release();
+    b->brec.release();
+    b->bxyz.release();
+    b->bxyz_off.release();
     b->brick_val_max = 0;
     b->brick_nodes_max = 0;
   };
@@ -1196,6 +1252,21 @@ static void build_brick_plan(afem_bsr* b)
   hipLaunchKernelGGL(k_brick_plan, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, m.nc_offsets.p, b->plan.p,
                      m.cells.p, m.npc, n, b->btable.p, b->bcount.p, b->bslice.p, b->bplan.p, b->bcell.p, err.p);
   check_launch("brick_plan");
+  // per-slot row records and per-brick coordinate blocks (even offsets, so
+  // every block starts 16-byte aligned)
+  std::vector<int64_t> xoff((size_t)nblk + 1, 0);
+  for (int32_t k = 0; k < nblk; ++k)
+    xoff[k + 1] = xoff[k] + ((3 * (int64_t)counts[k] + 1) & ~(int64_t)1);
+  b->bxyz_off.alloc(xoff.size());
+  b->bxyz_off.upload(xoff.data(), xoff.size());
+  b->bxyz.alloc((size_t)std::max<int64_t>(xoff[nblk], 2));
+  b->brec.alloc((size_t)nblk * kLocalRows);
+  hipLaunchKernelGGL(k_brick_records, dim3(nblk), dim3(256), 0, stream(), b->bperm.p, b->row_index.p, b->diag_pos.p,
+                     m.nc_offsets.p, b->btable.p, b->bcount.p, n, b->brec.p, err.p);
+  check_launch("brick_records");
+  hipLaunchKernelGGL(k_brick_coords, dim3(nblk), dim3(256), 0, stream(), b->btable.p, b->bcount.p, m.coords.p,
+                     b->bxyz_off.p, b->bxyz.p);
+  check_launch("brick_coords");
   int32_t h = 0;
   err.download(&h, 1);
   if (h) {
@@ -1296,55 +1367,59 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
     values[a + k] = s_val[k];
 }
 
-// Brick-ordered Laplace assembly: thread t of brick blk owns row
-// perm[blk * 256 + t] and writes its segment of the value array at the end;
+// Brick-ordered Laplace assembly: thread t of brick blk owns the row of
+// slot blk * 256 + t and writes its segment of the value array at the end;
 // otherwise as k_assemble_laplace_lplan (SLICED).  The row accumulators are
 // interleaved in LDS -- slot k of row t at (t / 64) * 64 * max_row + k * 64 +
 // t % 64 -- so the 64 lanes of a wave always hit 64 consecutive doubles
-// whatever their column positions (no bank conflicts).
+// whatever their column positions (no bank conflicts).  Everything a row
+// needs besides its plan words comes from one 16-byte slot record, and the
+// brick's node coordinates from one contiguous block (both gathered once per
+// pattern; the mesh's coordinates do not change), so no load depends on
+// another before the pair loop.
 template <int NPC, int U>
 __global__ void __launch_bounds__(256)
-k_assemble_laplace_brick(const double* __restrict__ coords, const int32_t* __restrict__ nc_off,
-                         const int32_t* __restrict__ perm, const int32_t* __restrict__ bval_off,
-                         const int32_t* __restrict__ total, const uint64_t* __restrict__ bplan,
-                         const int32_t* __restrict__ bcell, const int64_t* __restrict__ slice,
-                         const int32_t* __restrict__ table, const int32_t* __restrict__ count,
-                         const int32_t* __restrict__ diag_pos, const int32_t* __restrict__ row_index,
-                         int32_t nb_node, int val_cap, int node_cap, const double* __restrict__ cell_lambda,
-                         double source_f, double* __restrict__ rhs, double* __restrict__ values)
+k_assemble_laplace_brick(const int4* __restrict__ brec, const double* __restrict__ bxyz,
+                         const int64_t* __restrict__ bxyz_off, const int32_t* __restrict__ count,
+                         const uint64_t* __restrict__ bplan, const int32_t* __restrict__ bcell,
+                         const int64_t* __restrict__ slice, int32_t nb_node, int val_cap,
+                         const double* __restrict__ cell_lambda, double source_f, double* __restrict__ rhs,
+                         double* __restrict__ values)
 {
   extern __shared__ unsigned char smem[];
   const int32_t blk = (int32_t)xcd_tile(blockIdx.x, gridDim.x);
-  const int cnt = val_cap; // kLocalRows * max_row interleaved slots
   const int max_row = val_cap / kLocalRows;
   const int tn = count[blk];
   double* s_val = reinterpret_cast<double*>(smem);
-  double2* s_xy = reinterpret_cast<double2*>(s_val + val_cap); // val_cap is even
-  double* s_z = reinterpret_cast<double*>(s_xy + node_cap);
-  const int32_t* tab = table + (int64_t)blk * kLocalMaxNodes;
-  for (int k = threadIdx.x; k < tn; k += blockDim.x) {
-    const int64_t g = tab[k];
-    s_xy[k] = *reinterpret_cast<const double2*>(coords + 4 * g);
-    s_z[k] = coords[4 * g + 2];
+  double* s_coord = s_val + val_cap; // val_cap is even: 16-byte aligned
+  const double2* s_xy = reinterpret_cast<const double2*>(s_coord);
+  const double* s_z = s_coord + 2 * tn;
+  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
+  const int4 rec = brec[slot];
+  {
+    const double2* src = reinterpret_cast<const double2*>(bxyz + bxyz_off[blk]);
+    double2* dst = reinterpret_cast<double2*>(s_coord);
+    const int n2 = (3 * tn + 1) >> 1; // the block is padded to an even count
+    for (int k = threadIdx.x; k < n2; k += blockDim.x)
+      dst[k] = src[k];
   }
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x)
+  for (int k = threadIdx.x; k < val_cap; k += blockDim.x)
     s_val[k] = 0.0;
   __syncthreads();
-  const int64_t slot = (int64_t)blk * kLocalRows + threadIdx.x;
   if (slot < nb_node) {
-    const int32_t i = perm[slot];
+    const int32_t i = rec.x, a = rec.y;
+    const int len = rec.z & 255, dpos = (rec.z >> 8) & 255;
+    const int32_t ncell = (int32_t)((uint32_t)rec.z >> 16);
     double* rv = s_val + (threadIdx.x >> 6) * (64 * max_row) + (threadIdx.x & 63); // slot k at rv[64 k]
-    const d3 p = load_node(coords, i);
-    const int32_t ncell = nc_off[i + 1] - nc_off[i];
+    const d3 p = { s_xy[rec.w].x, s_xy[rec.w].y, s_z[rec.w] };
     const int64_t wbase = slice[blk] + threadIdx.x;
     double diag = 0.0, det_sum = 0.0;
     row_pairs<NPC, U, false>(bplan, bcell, wbase, kLocalRows, ncell, cell_lambda, s_xy, s_z, p, rv, 64, diag,
                              det_sum);
-    rv[64 * diag_pos[i]] += diag;
+    rv[64 * dpos] += diag;
     if (rhs)
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
     // only this thread writes into rv, so its row is complete here
-    const int32_t a = row_index[i], len = row_index[i + 1] - a;
     for (int k = 0; k < len; ++k)
       values[a + k] = rv[64 * k];
   }
@@ -2850,11 +2925,11 @@ static void launch_laplace_brick(const afem_bsr* b, const double* lam, double f,
   const afem_mesh& m = *b->mesh;
   const int val_cap = kLocalRows * b->max_row; // interleaved row slots (even)
   const int node_cap = b->brick_nodes_max;
-  const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap) * sizeof(double);
+  // + 1: the coordinate block is copied in 16-byte pieces, padded to an even count
+  const size_t lds = ((size_t)val_cap + 3 * (size_t)node_cap + 1) * sizeof(double);
   hipLaunchKernelGGL((k_assemble_laplace_brick<NPC, U>), dim3(grid_for(m.nb_node, kLocalRows)), dim3(kLocalRows),
-                     lds, stream(), m.coords.p, m.nc_offsets.p, b->bperm.p, b->bval_off.p, b->btotal.p, b->bplan.p,
-                     b->bcell.p, b->bslice.p, b->btable.p, b->bcount.p, b->diag_pos.p, b->row_index.p, m.nb_node,
-                     val_cap, node_cap, lam, f, rhs, b->values.p);
+                     lds, stream(), b->brec.p, b->bxyz.p, b->bxyz_off.p, b->bcount.p, b->bplan.p, b->bcell.p,
+                     b->bslice.p, m.nb_node, val_cap, lam, f, rhs, b->values.p);
 }
 
 template <int NPC, int U>
@@ -2898,8 +2973,8 @@ static LaplaceVariant effective_variant(afem_bsr* b, int v)
   }
   if (var.kind == kBrick) {
     ensure_plans(b, kPlansBrick);
-    const size_t brick_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->brick_nodes_max) * sizeof(double);
-    if (!b->bplan.p || brick_lds > 160 * 1024)
+    const size_t brick_lds = ((size_t)kLocalRows * b->max_row + 3 * (size_t)b->brick_nodes_max + 1) * sizeof(double);
+    if (!b->bplan.p || !b->brec.p || brick_lds > 160 * 1024)
       var.kind = kLocalSliced;
   }
   if (var.kind == kLocalSliced) {

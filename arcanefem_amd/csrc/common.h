@@ -238,6 +238,9 @@ struct afem_bsr
   afem::DBuf<int64_t> bslice;        // per brick: first sliced plan word
   afem::DBuf<uint64_t> bplan;        // sliced local-plan words of the bricks
   afem::DBuf<int32_t> bcell;         // cell id of each brick plan word
+  afem::DBuf<int4> brec;             // per brick slot: row, value offset, len | diag << 8 | cells << 16, table index
+  afem::DBuf<double> bxyz;           // per brick: its node table's coordinates, xy pairs then z
+  afem::DBuf<int64_t> bxyz_off;      // per brick: offset of its coordinates in bxyz (doubles, even)
   int32_t brick_val_max = 0, brick_nodes_max = 0;
   // cell-once brick plan: per brick its cell instances (two words each), their
   // cell ids, the table index -> brick slot map
