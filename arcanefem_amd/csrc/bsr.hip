@@ -910,7 +910,8 @@ k_brick_nodes(const int32_t* __restrict__ perm, const int32_t* __restrict__ val_
 }
 
 // Sliced local-plan words of each brick (table indices + row positions, as
-// k_local_plan) and the cell ids, word j of slot t at slice[blk] + j*256 + t.
+// k_local_plan) and the cell ids, in pairs: words 2q and 2q + 1 of slot t at
+// slice[blk] + q * 512 + 2 t (+ 1), so one 16-byte load per lane reads two.
 __global__ void __launch_bounds__(256)
 k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_off, const uint64_t* __restrict__ plan,
              const int32_t* __restrict__ cells, int npc, int32_t nb_node, const int32_t* __restrict__ table,
@@ -932,7 +933,7 @@ k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_of
   if (slot >= nb_node)
     return;
   const int32_t i = perm[slot];
-  const int64_t base = slice[blk] + threadIdx.x;
+  const int64_t base = slice[blk] + 2 * threadIdx.x;
   bool bad = false;
   for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
     const uint64_t w = plan[e];
@@ -948,7 +949,8 @@ k_brick_plan(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_of
       lw |= (uint64_t)(l & 2047) << (11 * k);
       lw |= (uint64_t)(pos & 31) << (33 + 5 * k);
     }
-    const int64_t at = base + (int64_t)(e - nc_off[i]) * kLocalRows;
+    const int32_t j = e - nc_off[i];
+    const int64_t at = base + (int64_t)(j >> 1) * (2 * kLocalRows) + (j & 1);
     out_plan[at] = lw;
     out_cell[at] = cell;
   }
@@ -1098,7 +1100,7 @@ k_sort_brick_rows(int32_t* __restrict__ perm, int32_t n)
 // Slice width of each block of 256 rows (rows perm[blk * 256 + t], or
 // blk * 256 + t without a permutation): its longest node->cell list, times 256.
 __global__ void __launch_bounds__(256)
-k_slice_width(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_off, int32_t n,
+k_slice_width(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_off, int32_t n, int even,
               int64_t* __restrict__ width)
 {
   __shared__ int32_t red[4];
@@ -1113,8 +1115,12 @@ k_slice_width(const int32_t* __restrict__ perm, const int32_t* __restrict__ nc_o
   if ((threadIdx.x & 63) == 0)
     red[threadIdx.x >> 6] = w;
   __syncthreads();
-  if (threadIdx.x == 0)
-    width[blockIdx.x] = (int64_t)max(max(red[0], red[1]), max(red[2], red[3])) * kLocalRows;
+  if (threadIdx.x == 0) {
+    int32_t wmax = max(max(red[0], red[1]), max(red[2], red[3]));
+    if (even)
+      wmax = (wmax + 1) & ~1;
+    width[blockIdx.x] = (int64_t)wmax * kLocalRows;
+  }
 }
 
 // slice[0] = 0, slice[k + 1] = slice[k] + width[k] (one workgroup; nblk is
@@ -1153,13 +1159,15 @@ k_slice_scan(const int64_t* __restrict__ width, int32_t nblk, int64_t* __restric
   }
 }
 
-// Slices of the blocks of 256 rows on the device; returns the total words.
-static int64_t device_slices(const int32_t* perm, const afem_mesh& m, int32_t nblk, DBuf<int64_t>& slice)
+// Slices of the blocks of 256 rows on the device (widths rounded up to an
+// even count when `even`); returns the total words.
+static int64_t device_slices(const int32_t* perm, const afem_mesh& m, int32_t nblk, DBuf<int64_t>& slice,
+                             bool even = false)
 {
   DBuf<int64_t> width((size_t)nblk);
   slice.alloc((size_t)nblk + 1);
   hipLaunchKernelGGL(k_slice_width, dim3(nblk), dim3(kLocalRows), 0, stream(), perm, m.nc_offsets.p, m.nb_node,
-                     width.p);
+                     even ? 1 : 0, width.p);
   check_launch("slice_width");
   hipLaunchKernelGGL(k_slice_scan, dim3(1), dim3(1024), 0, stream(), width.p, nblk, slice.p);
   check_launch("slice_scan");
@@ -1244,7 +1252,7 @@ static void build_brick_plan(afem_bsr* b)
     cmax = std::max(cmax, counts[k]);
   }
   // slices: width = the brick's longest node->cell list
-  const int64_t words = device_slices(b->bperm.p, m, nblk, b->bslice);
+  const int64_t words = device_slices(b->bperm.p, m, nblk, b->bslice, true);
   b->bplan.alloc((size_t)words);
   b->bcell.alloc((size_t)words);
   DBuf<int32_t> err(1);
@@ -1367,6 +1375,68 @@ k_assemble_laplace_lplan(const double* __restrict__ coords, const int32_t* __res
     values[a + k] = s_val[k];
 }
 
+// Words 2q, 2q + 1 of a paired slice (and their conductivities; the second
+// word of a row's last pair may be padding: its cell is not read).
+__device__ __forceinline__ void load_word_pair(const uint64_t* __restrict__ plan, const int32_t* __restrict__ cell_of,
+                                               const double* __restrict__ cell_lambda, int64_t wbase, int32_t j,
+                                               int32_t ncell, uint64_t& w0, uint64_t& w1, double& l0, double& l1)
+{
+  const int64_t at = wbase + (int64_t)(j >> 1) * (2 * kLocalRows);
+  const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(plan + at);
+  w0 = v.x;
+  w1 = v.y;
+  l0 = 1.0;
+  l1 = 1.0;
+  if (cell_lambda) {
+    const int2 c = *reinterpret_cast<const int2*>(cell_of + at);
+    l0 = cell_lambda[c.x];
+    if (j + 1 < ncell)
+      l1 = cell_lambda[c.y];
+  }
+}
+
+// All the (row, cell) pairs of one row from a paired brick slice (as
+// row_pairs: U words in flight, the coordinates of pair u + 1 read from LDS
+// before pair u is computed).
+template <int NPC, int U>
+__device__ __forceinline__ void brick_pairs(const uint64_t* __restrict__ plan, const int32_t* __restrict__ cell_of,
+                                            int64_t wbase, int32_t ncell, const double* __restrict__ cell_lambda,
+                                            const double2* __restrict__ s_xy, const double* __restrict__ s_z, d3 p,
+                                            double* rv, double& diag, double& det_sum)
+{
+  static_assert(U % 2 == 0, "words are read in pairs");
+  int32_t e = 0;
+  for (; e + U <= ncell; e += U) {
+    uint64_t w[U];
+    double lam[U];
+#pragma unroll
+    for (int u = 0; u < U; u += 2)
+      load_word_pair(plan, cell_of, cell_lambda, wbase, e + u, ncell, w[u], w[u + 1], lam[u], lam[u + 1]);
+    PairNodes<NPC> next = pair_nodes<NPC>(s_xy, s_z, w[0]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const PairNodes<NPC> cur = next;
+      if (u + 1 < U)
+        next = pair_nodes<NPC>(s_xy, s_z, w[u + 1]);
+      pair_element<NPC>(cur, p, lam[u], w[u], rv, 64, diag, det_sum);
+    }
+  }
+  if (e < ncell) {
+    uint64_t w[U];
+    double lam[U];
+    const int32_t last = (ncell - 1) & ~1; // the row's last pair
+#pragma unroll
+    for (int u = 0; u < U; u += 2)
+      load_word_pair(plan, cell_of, cell_lambda, wbase, min(e + u, last), ncell, w[u], w[u + 1], lam[u], lam[u + 1]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (e + u >= ncell)
+        break;
+      pair_element<NPC>(pair_nodes<NPC>(s_xy, s_z, w[u]), p, lam[u], w[u], rv, 64, diag, det_sum);
+    }
+  }
+}
+
 // Brick-ordered Laplace assembly: thread t of brick blk owns the row of
 // slot blk * 256 + t and writes its segment of the value array at the end;
 // otherwise as k_assemble_laplace_lplan (SLICED).  The row accumulators are
@@ -1412,16 +1482,25 @@ k_assemble_laplace_brick(const int4* __restrict__ brec, const double* __restrict
     const int32_t ncell = (int32_t)((uint32_t)rec.z >> 16);
     double* rv = s_val + (threadIdx.x >> 6) * (64 * max_row) + (threadIdx.x & 63); // slot k at rv[64 k]
     const d3 p = { s_xy[rec.w].x, s_xy[rec.w].y, s_z[rec.w] };
-    const int64_t wbase = slice[blk] + threadIdx.x;
+    const int64_t wbase = slice[blk] + 2 * threadIdx.x;
     double diag = 0.0, det_sum = 0.0;
-    row_pairs<NPC, U, false>(bplan, bcell, wbase, kLocalRows, ncell, cell_lambda, s_xy, s_z, p, rv, 64, diag,
-                             det_sum);
+    brick_pairs<NPC, U>(bplan, bcell, wbase, ncell, cell_lambda, s_xy, s_z, p, rv, diag, det_sum);
     rv[64 * dpos] += diag;
     if (rhs)
       rhs[i] = source_f * det_sum / (NPC == 4 ? 24.0 : 6.0);
-    // only this thread writes into rv, so its row is complete here
-    for (int k = 0; k < len; ++k)
-      values[a + k] = rv[64 * k];
+    // only this thread writes into rv, so its row is complete here; pairs
+    // of values go out as 16-byte stores (half the store instructions)
+    {
+      int k = 0;
+      if (a & 1) {
+        values[a] = rv[0];
+        k = 1;
+      }
+      for (; k + 1 < len; k += 2)
+        *reinterpret_cast<double2*>(values + a + k) = make_double2(rv[64 * k], rv[64 * (k + 1)]);
+      if (k < len)
+        values[a + k] = rv[64 * k];
+    }
   }
 }
 
