@@ -876,7 +876,7 @@ def main():
             times.append(O.wtime() - t0)
         t = float(np.median(times))
         times1 = []
-        for _ in range(max(3, args.cpu_reps // 3)):
+        for _ in range(max(5, args.cpu_reps // 3)):
             t0 = O.wtime()
             O.assemble(3, X, C, 4, row, col)
             O.rhs_constant_source(3, X, C, 4, 1.0)
