@@ -2026,13 +2026,15 @@ k_edge_instances(const int32_t* __restrict__ wg_row, const int32_t* __restrict__
       inst[at++] = s_key[k];
 }
 
-// Instance index and node slots of every (block, cell) entry of workgroup w.
+// Instance index and node slots of every (block, cell) entry of workgroup w,
+// entry j of block t at eslice8[w] + (j / 8) * 2048 + 8 t + j % 8 (eight
+// entries per 16-byte load).
 __global__ void __launch_bounds__(256)
 k_edge_encode(const int32_t* __restrict__ wg_row, const int32_t* __restrict__ row_index,
               const int32_t* __restrict__ columns, const int64_t* __restrict__ eslice,
               const int32_t* __restrict__ ecell, const uint8_t* __restrict__ ecnt, const int32_t* __restrict__ cells,
-              const int32_t* __restrict__ inst_off, const int32_t* __restrict__ inst, uint16_t* __restrict__ egeo,
-              int32_t* __restrict__ bad)
+              const int32_t* __restrict__ inst_off, const int32_t* __restrict__ inst,
+              const int64_t* __restrict__ eslice8, uint16_t* __restrict__ egeo, int32_t* __restrict__ bad)
 {
   __shared__ int32_t s_inst[kEdgeInstCap];
   __shared__ int32_t s_rows[257];
@@ -2079,8 +2081,17 @@ k_edge_encode(const int32_t* __restrict__ wg_row, const int32_t* __restrict__ ro
       atomicOr(bad, 2);
       continue;
     }
-    egeo[base + 256 * (int64_t)j] = (uint16_t)(li | (sp << 10) | (sq << 12));
+    egeo[eslice8[w] + (int64_t)(j >> 3) * (8 * 256) + 8 * t + (j & 7)] = (uint16_t)(li | (sp << 10) | (sq << 12));
   }
+}
+
+// The nodes of every instance (its cell's connectivity).
+__global__ void k_edge_inst_nodes(const int32_t* __restrict__ inst, int64_t n, const int32_t* __restrict__ cells,
+                                  int4* __restrict__ out)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n)
+    out[k] = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)inst[k]);
 }
 
 static void build_edge_geometry(afem_bsr* b)
@@ -2088,7 +2099,9 @@ static void build_edge_geometry(afem_bsr* b)
   const afem_mesh& m = *b->mesh;
   b->einst_off.release();
   b->einst.release();
+  b->einst4.release();
   b->egeo.release();
+  b->eslice8.release();
   b->e_inst_max = 0;
   const int32_t nwg = b->e_nwg;
   if (nwg <= 0)
@@ -2118,15 +2131,32 @@ static void build_edge_geometry(afem_bsr* b)
   hipLaunchKernelGGL(k_edge_instances<true>, dim3(nwg), dim3(256), 0, stream(), b->ewg_row.p, m.nc_offsets.p,
                      m.nc_items.p, m.slot_bits, b->einst_off.p, nullptr, b->einst.p, bad.p);
   check_launch("edge_instances_fill");
-  b->egeo.alloc(b->ecell.n);
+  b->einst4.alloc((size_t)std::max(off[nwg], 1));
+  hipLaunchKernelGGL(k_edge_inst_nodes, dim3(grid_for(std::max(off[nwg], 1), 256)), dim3(256), 0, stream(),
+                     b->einst.p, (int64_t)off[nwg], m.cells.p, b->einst4.p);
+  check_launch("edge_inst_nodes");
+  // packed entry slices: each workgroup's width (its widest block's cell
+  // count) rounded up to a multiple of 8
+  std::vector<int64_t> sl((size_t)nwg + 1), sl8((size_t)nwg + 1, 0);
+  b->eslice.download(sl.data(), sl.size());
+  for (int32_t w = 0; w < nwg; ++w) {
+    const int64_t width = (sl[w + 1] - sl[w]) / 256;
+    sl8[w + 1] = sl8[w] + 256 * ((width + 7) / 8 * 8);
+  }
+  b->eslice8.alloc(sl8.size());
+  b->eslice8.upload(sl8.data(), sl8.size());
+  b->egeo.alloc((size_t)std::max<int64_t>(sl8[nwg], 8));
   hipLaunchKernelGGL(k_edge_encode, dim3(nwg), dim3(256), 0, stream(), b->ewg_row.p, b->row_index.p, b->columns.p,
-                     b->eslice.p, b->ecell.p, b->ecnt.p, m.cells.p, b->einst_off.p, b->einst.p, b->egeo.p, bad.p);
+                     b->eslice.p, b->ecell.p, b->ecnt.p, m.cells.p, b->einst_off.p, b->einst.p, b->eslice8.p,
+                     b->egeo.p, bad.p);
   check_launch("edge_encode");
   bad.download(&h, 1);
   if (h) {
     b->einst_off.release();
     b->einst.release();
+    b->einst4.release();
     b->egeo.release();
+    b->eslice8.release();
     return;
   }
   b->e_inst_max = mx;
@@ -2307,22 +2337,24 @@ k_assemble_elasticity_edge(const double* __restrict__ coords, const int32_t* __r
 }
 
 // Elasticity assembly with the cell geometry computed once per workgroup:
-// the workgroup's distinct cells (its instances) first get their cofactor
-// vectors C_k = det grad N_k (origin at the cell's first node: C_1 = e2 x e3,
-// C_2 = e3 x e1, C_3 = e1 x e2, C_0 = -(C_1 + C_2 + C_3)) and 1 / (6 |det|)
-// staged in LDS, one thread per instance; then, as k_assemble_elasticity_edge,
-// one thread per nonzero block (p, q) sums lambda C_p (x) C_q + mu C_q (x) C_p
-// + mu (C_p . C_q) I, times 1 / (6 |det|), over the cells of its edge -- now
-// 7 LDS reads per (block, cell) instead of the cell's connectivity, two node
-// coordinates and three cross products.  The diagonal block is minus the sum
-// of the row's other blocks.  The LDS of the instances is reused for that sum.
+// the workgroup's distinct cells (its instances, given by their four nodes)
+// first get their cofactor vectors C_k = det grad N_k (origin at the cell's
+// first node: C_1 = e2 x e3, C_2 = e3 x e1, C_3 = e1 x e2, C_0 = -(C_1 + C_2 +
+// C_3)) and 1 / (6 |det|) staged in LDS, one thread per instance; then, as
+// k_assemble_elasticity_edge, one thread per nonzero block (p, q) sums
+// lambda C_p (x) C_q + mu C_q (x) C_p + mu (C_p . C_q) I, times 1 / (6 |det|),
+// over the cells of its edge -- 7 LDS reads per (block, cell), the entries
+// read eight at a time.  The diagonal block is minus the sum of the row's
+// other blocks.  The LDS of the instances is reused for that sum and, in
+// per-block order, for a block-major copy that the workgroup writes with
+// 16-byte stores (its blocks are one contiguous range of the value array).
 __global__ void __launch_bounds__(256)
-k_assemble_elasticity_geo(const double* __restrict__ coords, const int32_t* __restrict__ cells,
+k_assemble_elasticity_geo(const double* __restrict__ coords, const int4* __restrict__ inst_nodes,
                           const int32_t* __restrict__ row_index, const int32_t* __restrict__ columns,
-                          const int32_t* __restrict__ wg_row, const int64_t* __restrict__ eslice,
+                          const int32_t* __restrict__ wg_row, const int64_t* __restrict__ eslice8,
                           const uint16_t* __restrict__ egeo, const uint8_t* __restrict__ ecnt,
-                          const int32_t* __restrict__ inst_off, const int32_t* __restrict__ inst, int cap,
-                          double lame_lambda, double lame_mu, int per_block, double* __restrict__ values)
+                          const int32_t* __restrict__ inst_off, int cap, double lame_lambda, double lame_mu,
+                          int per_block, double* __restrict__ values)
 {
   extern __shared__ double s_geo[]; // 13 * cap doubles (component-major), later 9 * 256
   __shared__ int32_t s_rows[257];
@@ -2333,8 +2365,7 @@ k_assemble_elasticity_geo(const double* __restrict__ coords, const int32_t* __re
   for (int k = threadIdx.x; k <= nr; k += 256)
     s_rows[k] = row_index[p0 + k] - b0;
   for (int k = threadIdx.x; k < ni; k += 256) {
-    const int32_t c = inst[i0 + k];
-    const int4 cq = *reinterpret_cast<const int4*>(cells + 4 * (int64_t)c);
+    const int4 cq = inst_nodes[i0 + k];
     const d3 X0 = load_node(coords, cq.x);
     const d3 e1 = sub(load_node(coords, cq.y), X0), e2 = sub(load_node(coords, cq.z), X0),
              e3 = sub(load_node(coords, cq.w), X0);
@@ -2375,22 +2406,30 @@ k_assemble_elasticity_geo(const double* __restrict__ coords, const int32_t* __re
     pr = lo;
     diag = (p0 + pr) == columns[b0 + t];
     const int cnum = ecnt[b0 + t];
-    const int64_t base = eslice[w] + t;
-    for (int j = 0; j < cnum; ++j) {
-      const uint32_t code = egeo[base + 256 * (int64_t)j];
-      const int li = code & 1023, sp = (code >> 10) & 3, sq = (code >> 12) & 3;
-      const double av[3] = { s_geo[(3 * sp) * cap + li], s_geo[(3 * sp + 1) * cap + li],
-                             s_geo[(3 * sp + 2) * cap + li] };
-      const double bq[3] = { s_geo[(3 * sq) * cap + li], s_geo[(3 * sq + 1) * cap + li],
-                             s_geo[(3 * sq + 2) * cap + li] };
-      const double inv6 = s_geo[12 * cap + li];
-      const double la = lame_lambda * inv6, mu = lame_mu * inv6;
-      const double dd = mu * (av[0] * bq[0] + av[1] * bq[1] + av[2] * bq[2]);
+    const int64_t base = eslice8[w] + 8 * t;
+    for (int j0 = 0; j0 < cnum; j0 += 8) {
+      const uint4 g = *reinterpret_cast<const uint4*>(egeo + base + (int64_t)(j0 >> 3) * (8 * 256));
+      const uint32_t codes[8] = { g.x & 0xffffu, g.x >> 16, g.y & 0xffffu, g.y >> 16,
+                                  g.z & 0xffffu, g.z >> 16, g.w & 0xffffu, g.w >> 16 };
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+      for (int u = 0; u < 8; ++u) {
+        if (j0 + u >= cnum)
+          break;
+        const uint32_t code = codes[u];
+        const int li = code & 1023, sp = (code >> 10) & 3, sq = (code >> 12) & 3;
+        const double av[3] = { s_geo[(3 * sp) * cap + li], s_geo[(3 * sp + 1) * cap + li],
+                               s_geo[(3 * sp + 2) * cap + li] };
+        const double bq[3] = { s_geo[(3 * sq) * cap + li], s_geo[(3 * sq + 1) * cap + li],
+                               s_geo[(3 * sq + 2) * cap + li] };
+        const double inv6 = s_geo[12 * cap + li];
+        const double la = lame_lambda * inv6, mu = lame_mu * inv6;
+        const double dd = mu * (av[0] * bq[0] + av[1] * bq[1] + av[2] * bq[2]);
 #pragma unroll
-        for (int c2 = 0; c2 < 3; ++c2)
-          K[r * 3 + c2] += la * av[r] * bq[c2] + mu * bq[r] * av[c2] + (r == c2 ? dd : 0.0);
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c2 = 0; c2 < 3; ++c2)
+            K[r * 3 + c2] += la * av[r] * bq[c2] + mu * bq[r] * av[c2] + (r == c2 ? dd : 0.0);
+      }
     }
   }
   __syncthreads(); // the instances are no longer read: the LDS holds the blocks now
@@ -2399,24 +2438,42 @@ k_assemble_elasticity_geo(const double* __restrict__ coords, const int32_t* __re
   for (int i = 0; i < 9; ++i)
     s_blk[i * 256 + t] = K[i];
   __syncthreads();
-  if (t >= nb)
-    return;
-  const int x0 = s_rows[pr], len = s_rows[pr + 1] - x0;
-  if (diag) {
-    for (int u = x0; u < x0 + len; ++u)
-      if (u != t) {
+  int x0 = 0, len = 0;
+  if (t < nb) {
+    x0 = s_rows[pr];
+    len = s_rows[pr + 1] - x0;
+    if (diag) {
+      for (int u = x0; u < x0 + len; ++u)
+        if (u != t) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i)
-          K[i] -= s_blk[i * 256 + u];
-      }
+          for (int i = 0; i < 9; ++i)
+            K[i] -= s_blk[i * 256 + u];
+        }
+    }
   }
-  const int64_t blk = (int64_t)b0 + t;
   if (per_block) {
+    // block-major copy in LDS, then the workgroup's 9 nb values leave as
+    // 16-byte stores (one 8-byte store first when the range starts odd)
+    __syncthreads();
+    if (t < nb) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i)
-      values[9 * blk + i] = K[i];
+      for (int i = 0; i < 9; ++i)
+        s_blk[9 * t + i] = K[i];
+    }
+    __syncthreads();
+    const int64_t start = 9 * (int64_t)b0;
+    const int n = 9 * nb;
+    const int head = (int)(start & 1);
+    if (head && t == 0)
+      values[start] = s_blk[0];
+    const int pairs = (n - head) >> 1;
+    for (int k = t; k < pairs; k += 256)
+      *reinterpret_cast<double2*>(values + start + head + 2 * k) =
+          make_double2(s_blk[head + 2 * k], s_blk[head + 2 * k + 1]);
+    if (((n - head) & 1) && t == 0)
+      values[start + n - 1] = s_blk[n - 1];
   }
-  else {
+  else if (t < nb) {
     const int64_t base = 9 * ((int64_t)b0 + x0);
     const int x = t - x0;
 #pragma unroll
@@ -3213,9 +3270,9 @@ static void assemble_elasticity(afem_bsr* b, double lam, double mu)
     const int cap = b->e_inst_max | 1;
     const size_t lds = sizeof(double) * std::max<size_t>((size_t)13 * cap, (size_t)9 * 256);
     ProfScope prof("assemble_elasticity_tet");
-    hipLaunchKernelGGL(k_assemble_elasticity_geo, dim3(b->e_nwg), dim3(256), lds, stream(), m.coords.p, m.cells.p,
-                       b->row_index.p, b->columns.p, b->ewg_row.p, b->eslice.p, b->egeo.p, b->ecnt.p,
-                       b->einst_off.p, b->einst.p, cap, lam, mu, b->per_block ? 1 : 0, b->values.p);
+    hipLaunchKernelGGL(k_assemble_elasticity_geo, dim3(b->e_nwg), dim3(256), lds, stream(), m.coords.p, b->einst4.p,
+                       b->row_index.p, b->columns.p, b->ewg_row.p, b->eslice8.p, b->egeo.p, b->ecnt.p,
+                       b->einst_off.p, cap, lam, mu, b->per_block ? 1 : 0, b->values.p);
     check_launch("assemble_elasticity_geo");
     return;
   }

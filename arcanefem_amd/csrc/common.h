@@ -257,7 +257,9 @@ struct afem_bsr
   afem::DBuf<uint8_t> ecnt;          // cells per block
   afem::DBuf<int32_t> einst_off;     // first instance of each workgroup in einst (nb_wg + 1)
   afem::DBuf<int32_t> einst;         // the distinct cells of each workgroup's rows, sorted
-  afem::DBuf<uint16_t> egeo;         // as ecell: instance | slot of p << 10 | slot of q << 12
+  afem::DBuf<uint16_t> egeo;         // instance | slot of p << 10 | slot of q << 12, eight per block together
+  afem::DBuf<int64_t> eslice8;       // per workgroup: first egeo word (widths rounded up to multiples of 8)
+  afem::DBuf<int4> einst4;           // the nodes of each instance
   int32_t e_inst_max = 0;            // largest instance count of a workgroup (0: no geometry plan)
   // Quad4 / Hexa8: per cell the upper triangle of the element matrix and the
   // source vector, cell-minor (entry k of cell c at k * nb_cell + c)
