@@ -142,6 +142,29 @@ def stats_markdown(path, command, top=20):
     return "\n".join(lines) + "\n"
 
 
+def grid_markdown(trace_csv, match, top=12):
+    """Per (kernel, grid size) durations from a rocprofv3 kernel trace for the
+    kernels whose short name matches `match`: one kernel is launched at
+    several sizes in one run, and only the row at the benchmark's grid is
+    comparable with its event-timed average."""
+    import re
+    groups = {}
+    for r in _read(trace_csv):
+        name = short_name(r["Kernel_Name"])
+        if not re.search(match, name):
+            continue
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        groups.setdefault((name, grid), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    rows = sorted(groups.items(), key=lambda kv: -sum(kv[1]))[:top]
+    lines = ["", f"Per launch size (kernels matching `{match}`):", "",
+             "| kernel | grid (work-items) | workgroups | calls | avg us | min us | max us |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for (name, grid), d in rows:
+        lines.append(f"| `{name}` | {grid} | {grid // 256 if grid % 256 == 0 else '-'} | {len(d)} | "
+                     f"{sum(d) / len(d):.1f} | {min(d):.1f} | {max(d):.1f} |")
+    return "\n".join(lines) + "\n"
+
+
 def pmc_markdown(summary, command, workload, top=60):
     lines = [
         f"# HBM traffic per dispatch ({workload})",
@@ -174,6 +197,8 @@ def main():
     s.add_argument("--command", required=True)
     s.add_argument("--out", required=True)
     s.add_argument("--top", type=int, default=60)
+    s.add_argument("--trace", default=None, help="kernel trace CSV for a per-launch-size table")
+    s.add_argument("--grid-match", default=None, help="regex on the short kernel names of that table")
     p = sub.add_parser("pmc")
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
@@ -193,6 +218,8 @@ def main():
     elif a.cmd == "stats":
         with open(a.out, "w") as f:
             f.write(stats_markdown(a.csv, a.command, a.top))
+            if a.trace and a.grid_match:
+                f.write(grid_markdown(a.trace, a.grid_match))
     else:
         summ = pmc_summary(a.fetch, a.write, a.copy_bytes)
         summ["workload"] = a.workload
