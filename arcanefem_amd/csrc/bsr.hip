@@ -73,12 +73,146 @@ k_pattern_rows(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ n
   }
 }
 
+// Sixteen lanes per node: lane l takes the node's cells l, l + 16, ... and
+// puts the other nodes of each (and, lane 0, the node itself) into its 8
+// register slots; the group's 128 keys (padding 0x7fffffff) are sorted by a
+// bitonic network -- key g = 8 lane + slot, exchanges within a lane between
+// registers and across lanes by shuffles -- and every key that differs from
+// its predecessor is a column.  Count pass: the row length; fill pass: the
+// sorted row at row_index[i].  *overflow = 1 when a node's candidates do not
+// fit its 128 slots (the caller then runs the per-thread kernel).
+constexpr int kPatternLanes = 16, kPatternSlots = 8;
+
+__global__ void __launch_bounds__(256)
+k_pattern_rows16(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+                 const int32_t* __restrict__ cells, int npc, int slot_bits, int32_t nb_node,
+                 const int32_t* __restrict__ row_index, int32_t* __restrict__ out, int32_t* __restrict__ overflow)
+{
+  constexpr int L = kPatternLanes, S = kPatternSlots;
+  const int lane = threadIdx.x & (L - 1);
+  const int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / L);
+  const bool live = i < nb_node;
+  int32_t key[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+    key[q] = 0x7fffffff;
+  bool ovf = false;
+  if (live) {
+    const int32_t e0 = nc_off[i], ncell = nc_off[i + 1] - e0;
+    const int rounds = (ncell + L - 1) / L;
+    ovf = rounds * (npc - 1) + 1 > S;
+    if (!ovf) {
+      int q = 0;
+      if (lane == 0)
+        key[q++] = i; // the diagonal
+      for (int r = 0; r < rounds; ++r) {
+        const int32_t c = lane + L * r;
+        if (c < ncell) {
+          const int32_t it = nc_items[e0 + c];
+          const int sl = it & ((1 << slot_bits) - 1);
+          const int32_t* cn = cells + (it - sl); // items are stride * cell + slot
+          for (int k = 0; k < npc; ++k)
+            if (k != sl) {
+              const int32_t v = cn[k];
+#pragma unroll
+              for (int z = 0; z < S; ++z)
+                if (z == q)
+                  key[z] = v;
+              ++q;
+            }
+        }
+      }
+    }
+  }
+  // every lane of a group takes the same decision: a group overflows as a whole
+  if (ovf)
+    atomicOr(overflow, 1);
+  // bitonic sort of the group's 128 keys, ascending in g = S * lane + slot
+#pragma unroll
+  for (int k = 2; k <= L * S; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= S) {
+        const int m = j / S; // partner lane = lane ^ m
+        const bool lower = (lane & m) == 0;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          const int g = S * lane + q;
+          const bool asc = (g & k) == 0;
+          const int32_t w = __shfl_xor(key[q], m, 64);
+          key[q] = (lower == asc) ? min(key[q], w) : max(key[q], w);
+        }
+      }
+      else {
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          if (q & j)
+            continue;
+          const int g = S * lane + q;
+          const bool asc = (g & k) == 0;
+          const int32_t a = key[q], b = key[q | j];
+          key[q] = asc ? min(a, b) : max(a, b);
+          key[q | j] = asc ? max(a, b) : min(a, b);
+        }
+      }
+    }
+  }
+  // columns: keys that differ from their predecessor (and are not padding)
+  int32_t prev = __shfl_up(key[S - 1], 1, L);
+  if (lane == 0)
+    prev = -1;
+  int mine = 0;
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const int32_t before = q == 0 ? prev : key[q - 1];
+    mine += (key[q] != before && key[q] != 0x7fffffff) ? 1 : 0;
+  }
+  int incl = mine;
+#pragma unroll
+  for (int d = 1; d < L; d <<= 1) {
+    const int u = __shfl_up(incl, d, L);
+    if (lane >= d)
+      incl += u;
+  }
+  if (!live || ovf)
+    return;
+  if (row_index == nullptr) {
+    if (lane == L - 1)
+      out[i] = incl;
+  }
+  else {
+    int32_t at = row_index[i] + incl - mine;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const int32_t before = q == 0 ? prev : key[q - 1];
+      if (key[q] != before && key[q] != 0x7fffffff)
+        out[at++] = key[q];
+    }
+  }
+}
+
 // Runs one pass (count if row_index == nullptr, fill otherwise) with the
 // smallest LDS capacity that holds every row.  Returns false on overflow at
 // the largest capacity.
 static bool pattern_pass(const afem_mesh& m, const int32_t* row_index, int32_t* out, int cap_hint)
 {
   DBuf<int32_t> ovf(1);
+  {
+    // sixteen lanes per node when every node's candidates fit 128 slots
+    ovf.zero();
+    {
+      ProfScope prof("pattern_rows");
+      const int64_t threads = (int64_t)m.nb_node * kPatternLanes;
+      hipLaunchKernelGGL(k_pattern_rows16, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream(),
+                         m.nc_offsets.p, m.nc_items.p, m.cells.p, m.npc, m.slot_bits, m.nb_node, row_index, out,
+                         ovf.p);
+    }
+    check_launch("pattern_rows16");
+    int32_t flag = 0;
+    ovf.download(&flag, 1);
+    if (flag == 0)
+      return true;
+  }
   auto run = [&](auto cap_tag, auto bs_tag) {
     constexpr int CAP = decltype(cap_tag)::value;
     constexpr int BS = decltype(bs_tag)::value;
