@@ -84,10 +84,13 @@ def fan_mesh(k):
     return X, tri_cells(t)
 
 
-@pytest.mark.parametrize("k", [40, 300])
+@pytest.mark.parametrize("k", [40, 60, 300])
 def test_high_valence_node(afem_gpu, k):
     """Rows longer than 32 (no local plans) and than 255 (no plan at all:
-    the column-search kernel) still assemble and solve."""
+    the column-search kernel) still assemble and solve.  The pattern of the
+    40-cell fan comes from the sixteen-lane sorting kernel (7 candidates per
+    lane), those of 60 and 300 cells from the per-thread fallback (128 and
+    512 slots)."""
     A = afem_gpu
     print(f"[case] fan k={k}", flush=True)
     X, C = fan_mesh(k)
