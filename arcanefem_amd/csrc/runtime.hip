@@ -211,28 +211,28 @@ int64_t exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n)
 }
 
 // ---- streaming copy ----------------------------------------------------------------
-// 16 bytes per lane per step, non-temporal on both sides, grid-stride over a
-// grid of a few workgroups per CU: the device-to-device copy the roofline
-// figures are measured against.
+// Each workgroup copies one contiguous 16 KB chunk (4 x 16 bytes per lane,
+// all loads in flight before the stores), one workgroup per chunk: the
+// device-to-device copy the roofline figures are measured against.  On
+// MI355X this form measured 5.8 TB/s for 1 GiB against 4.4-5.3 TB/s for
+// grid-stride loops over 4-32 workgroups per CU and 5.1 TB/s for
+// hipMemcpyDeviceToDevice.
 typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr int kCopyUnroll = 4;
 
 __global__ void __launch_bounds__(256)
 k_stream_copy(const v4i* __restrict__ src, v4i* __restrict__ dst, int64_t n16)
 {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const v4i a = __builtin_nontemporal_load(src + i);
-    const v4i b = __builtin_nontemporal_load(src + i + stride);
-    const v4i c = __builtin_nontemporal_load(src + i + 2 * stride);
-    const v4i d = __builtin_nontemporal_load(src + i + 3 * stride);
-    __builtin_nontemporal_store(a, dst + i);
-    __builtin_nontemporal_store(b, dst + i + stride);
-    __builtin_nontemporal_store(c, dst + i + 2 * stride);
-    __builtin_nontemporal_store(d, dst + i + 3 * stride);
-  }
-  for (; i < n16; i += stride)
-    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+  const int64_t b = (int64_t)blockIdx.x * (256 * kCopyUnroll) + threadIdx.x;
+  v4i a[kCopyUnroll];
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u)
+    if (b + 256 * u < n16)
+      a[u] = src[b + 256 * u];
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u)
+    if (b + 256 * u < n16)
+      dst[b + 256 * u] = a[u];
 }
 
 } // namespace afem
@@ -339,9 +339,9 @@ AFEM_API int afem_stream_copy(void* dst, const void* src, size_t bytes)
   const int64_t n16 = (int64_t)(bytes / 16);
   if (n16 == 0)
     return AFEM_OK;
-  const int64_t want = (n16 + 255) / 256;
-  const int64_t cap = rt().num_cu > 0 ? (int64_t)rt().num_cu * 8 : 2048;
-  const unsigned grid = (unsigned)std::min<int64_t>(want, cap);
+  const int64_t chunks = (n16 + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll);
+  AFEM_REQUIRE(chunks <= 0x7fffffff, AFEM_ERR_ARGUMENT, "stream copy too large for one launch");
+  const unsigned grid = (unsigned)chunks;
   ProfScope prof("stream_copy");
   hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, stream(), static_cast<const v4i*>(src),
                      static_cast<v4i*>(dst), n16);
