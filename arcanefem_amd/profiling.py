@@ -142,7 +142,7 @@ def stats_markdown(path, command, top=20):
     return "\n".join(lines) + "\n"
 
 
-def grid_markdown(trace_csv, match, top=12):
+def grid_markdown(trace_csv, match, top=40):
     """Per (kernel, grid size) durations from a rocprofv3 kernel trace for the
     kernels whose short name matches `match`: one kernel is launched at
     several sizes in one run, and only the row at the benchmark's grid is
