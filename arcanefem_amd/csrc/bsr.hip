@@ -3007,6 +3007,8 @@ static void build_plan(afem_bsr* b)
   b->plan.release();
   b->diag_pos.release();
   b->q1_plan.release(); // rebuilt by the first Q1 assembly
+  b->q1_plan_sl.release();
+  b->q1_item_sl.release();
   if (b->max_row > kPlanMaxRow || !m.simplex) // the plans serve the simplex kernels
     return;
   b->plan.alloc(m.nc_items.n);

@@ -265,6 +265,9 @@ struct afem_bsr
   // source vector, cell-minor (entry k of cell c at k * nb_cell + c)
   afem::DBuf<double> q1_elem;
   afem::DBuf<uint64_t> q1_plan;      // per (node, cell) pair: the row positions of the cell's nodes, 8 bits each
+  afem::DBuf<uint64_t> q1_plan_sl;   // the same words sliced per gather workgroup: pair j of row t at (w*width + j)*BS + t
+  afem::DBuf<int32_t> q1_item_sl;    // the pairs' node->cell items in the same layout
+  int32_t q1_sl_width = 0, q1_sl_rows = 0; // slice width (largest cell count of a node) and rows per workgroup
   int32_t e_nwg = 0;
   afem::DBuf<int32_t> block_nodes;   // per block of rows: sorted referenced nodes (fixed stride)
   afem::DBuf<int32_t> block_count;   // per block: number of referenced nodes

@@ -257,6 +257,94 @@ k_q1_plan(const int32_t* __restrict__ cells, int slot_bits, const int32_t* __res
   }
 }
 
+// Sliced copy of the plan words and node->cell items for the gather: pair j
+// of row t of workgroup w at (w * width + j) * BS + t (rows with fewer cells
+// leave their tail unused).
+__global__ void __launch_bounds__(256)
+k_q1_slice(const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
+           const uint64_t* __restrict__ plan, int32_t nb_node, int width, int rows,
+           uint64_t* __restrict__ plan_sl, int32_t* __restrict__ item_sl)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb_node)
+    return;
+  const int32_t w = i / rows, t = i - w * rows;
+  const int64_t base = (int64_t)w * width * rows + t;
+  for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
+    const int64_t at = base + (int64_t)(e - nc_off[i]) * rows;
+    plan_sl[at] = plan[e];
+    item_sl[at] = nc_items[e];
+  }
+}
+
+// a += v on an LDS double owned by the calling thread alone (ds_add_f64: the
+// LDS unit does the read-modify-write; one writer keeps program order)
+__device__ __forceinline__ void q1_lds_add(double* a, double v)
+{
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Gather from the sliced plan: the (item, word) pairs of four cells are
+// loaded together, the cells' element rows (four 16-byte loads each) added
+// into the row's LDS slots with ds_add_f64.
+template <int DIM, int BS>
+__global__ void __launch_bounds__(BS)
+k_q1_gather_sl(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t* __restrict__ item_sl,
+               const uint64_t* __restrict__ plan_sl, int width, int32_t nb_node, const int32_t* __restrict__ row_index,
+               const double* __restrict__ elem, const double* __restrict__ cell_lambda, double* __restrict__ values,
+               double f, double* __restrict__ rhs)
+{
+  constexpr int N = Q1Point<DIM>::N;
+  constexpr int NE = N * N + N;
+  constexpr int U = 4;
+  extern __shared__ double s_val[];
+  const int t = threadIdx.x;
+  const int32_t r0 = blockIdx.x * BS, i = r0 + t;
+  const int32_t rend = min(r0 + BS, nb_node);
+  const int32_t base = row_index[r0], total = row_index[rend] - base;
+  for (int j = t; j < total; j += BS)
+    s_val[j] = 0.0;
+  __syncthreads();
+  if (i < nb_node) {
+    double* row = s_val + (row_index[i] - base);
+    const int32_t ncell = nc_off[i + 1] - nc_off[i];
+    const int64_t sb = (int64_t)blockIdx.x * width * BS + t;
+    double src = 0.0;
+    for (int32_t e = 0; e < ncell; e += U) {
+      int32_t it[U];
+      uint64_t w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t at = sb + (int64_t)min(e + u, ncell - 1) * BS;
+        it[u] = item_sl[at];
+        w[u] = plan_sl[at];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e + u >= ncell)
+          break;
+        const int s = it[u] & ((1 << slot_bits) - 1);
+        const int32_t c = it[u] >> slot_bits;
+        const double* ke = elem + (int64_t)NE * c;
+        const double lam = cell_lambda ? cell_lambda[c] : 1.0;
+#pragma unroll
+        for (int b = 0; b < N; b += 2) {
+          const double2 p = *reinterpret_cast<const double2*>(ke + s * N + b);
+          q1_lds_add(row + ((w[u] >> (8 * b)) & 0xff), p.x * lam);
+          q1_lds_add(row + ((w[u] >> (8 * b + 8)) & 0xff), p.y * lam);
+        }
+        if (rhs)
+          src += f * ke[N * N + s];
+      }
+    }
+    if (rhs)
+      rhs[i] = src;
+  }
+  __syncthreads();
+  for (int j = t; j < total; j += BS)
+    values[base + j] = s_val[j];
+}
+
 template <int DIM, int BS>
 __global__ void __launch_bounds__(BS)
 k_q1_gather(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
@@ -497,6 +585,8 @@ void q1_assemble(afem_bsr* b, const double* cell_lambda, double f, double* rhs, 
     if (b->q1_plan.n != m.nc_items.n) { // once per pattern
       AFEM_REQUIRE(b->max_row <= 255, AFEM_ERR_CAPACITY, "Q1 matrix rows longer than 255 entries");
       b->q1_plan.alloc(m.nc_items.n);
+      b->q1_plan_sl.release(); // the sliced copy follows the plan
+      b->q1_item_sl.release();
       const dim3 pgrid(grid_for(m.nb_node, 256));
       if (m.dim == 2)
         hipLaunchKernelGGL(k_q1_plan<2>, pgrid, block, 0, stream(), m.cells.p, m.slot_bits, m.nc_offsets.p,
@@ -516,12 +606,23 @@ void q1_assemble(afem_bsr* b, const double* cell_lambda, double f, double* rhs, 
     const size_t lds = (size_t)rows * b->max_row * sizeof(double);
     AFEM_REQUIRE(lds <= 64 * 1024, AFEM_ERR_CAPACITY, "Q1 matrix rows too long for the LDS-staged gather");
     const dim3 ngrid(grid_for(m.nb_node, rows));
+    if (b->q1_sl_rows != rows || b->q1_plan_sl.n == 0) { // sliced copy, once per pattern
+      const int width = std::max(1, m.max_degree);
+      const size_t words = (size_t)ngrid.x * rows * width;
+      b->q1_plan_sl.alloc(words);
+      b->q1_item_sl.alloc(words);
+      hipLaunchKernelGGL(k_q1_slice, dim3(grid_for(m.nb_node, 256)), dim3(256), 0, stream(), m.nc_offsets.p,
+                         m.nc_items.p, b->q1_plan.p, m.nb_node, width, rows, b->q1_plan_sl.p, b->q1_item_sl.p);
+      check_launch("q1_slice");
+      b->q1_sl_width = width;
+      b->q1_sl_rows = rows;
+    }
     auto gather = [&](auto dim_tag, auto bs_tag) {
       constexpr int D = decltype(dim_tag)::value;
       constexpr int BS = decltype(bs_tag)::value;
-      hipLaunchKernelGGL((k_q1_gather<D, BS>), ngrid, dim3(BS), lds, stream(), m.slot_bits, m.nc_offsets.p,
-                         m.nc_items.p, b->q1_plan.p, m.nb_node, b->row_index.p, b->q1_elem.p, gather_lambda,
-                         b->values.p, f, rhs);
+      hipLaunchKernelGGL((k_q1_gather_sl<D, BS>), ngrid, dim3(BS), lds, stream(), m.slot_bits, m.nc_offsets.p,
+                         b->q1_item_sl.p, b->q1_plan_sl.p, b->q1_sl_width, m.nb_node, b->row_index.p, b->q1_elem.p,
+                         gather_lambda, b->values.p, f, rhs);
     };
     using I2 = std::integral_constant<int, 2>;
     using I3 = std::integral_constant<int, 3>;
