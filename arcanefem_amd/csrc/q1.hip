@@ -345,50 +345,6 @@ k_q1_gather_sl(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t*
     values[base + j] = s_val[j];
 }
 
-template <int DIM, int BS>
-__global__ void __launch_bounds__(BS)
-k_q1_gather(int slot_bits, const int32_t* __restrict__ nc_off, const int32_t* __restrict__ nc_items,
-            const uint64_t* __restrict__ plan, int32_t nb_node, const int32_t* __restrict__ row_index,
-            const double* __restrict__ elem, const double* __restrict__ cell_lambda, double* __restrict__ values,
-            double f, double* __restrict__ rhs)
-{
-  constexpr int N = Q1Point<DIM>::N;
-  constexpr int NE = N * N + N;
-  extern __shared__ double s_val[];
-  const int t = threadIdx.x;
-  const int32_t r0 = blockIdx.x * BS, i = r0 + t;
-  const int32_t rend = min(r0 + BS, nb_node);
-  const int32_t base = row_index[r0], total = row_index[rend] - base;
-  for (int j = t; j < total; j += BS)
-    s_val[j] = 0.0;
-  __syncthreads();
-  if (i < nb_node) {
-    double* row = s_val + (row_index[i] - base);
-    double src = 0.0;
-    for (int32_t e = nc_off[i]; e < nc_off[i + 1]; ++e) {
-      const int32_t it = nc_items[e];
-      const int s = it & ((1 << slot_bits) - 1);
-      const int32_t c = it >> slot_bits;
-      const uint64_t w = plan[e];
-      const double* ke = elem + (int64_t)NE * c;
-      const double lam = cell_lambda ? cell_lambda[c] : 1.0;
-#pragma unroll
-      for (int b = 0; b < N; b += 2) {
-        const double2 p = *reinterpret_cast<const double2*>(ke + s * N + b);
-        row[(w >> (8 * b)) & 0xff] += p.x * lam;
-        row[(w >> (8 * b + 8)) & 0xff] += p.y * lam;
-      }
-      if (rhs)
-        src += f * ke[N * N + s];
-    }
-    if (rhs)
-      rhs[i] = src;
-  }
-  __syncthreads();
-  for (int j = t; j < total; j += BS)
-    values[base + j] = s_val[j];
-}
-
 // rhs[i*nb_dof+dof_index] += (M (scale * field))_i, gathered per node over
 // its cells: sum_g N_s (sum_a N_a scale f_a) det J_g
 // (integrateNodalFieldToRhsQuad4 / Hexa8, femutils/ArcaneFemFunctions.h:
