@@ -219,9 +219,20 @@ k_q1_elements(const double* __restrict__ coords, const int32_t* __restrict__ cel
     }
     __syncthreads();
     const int chn = (base + CH <= NE ? CH : NE - base);
-    for (int j = t; j < ncell * chn; j += kQ1ElemBlock) {
-      const int cell = j / chn, k = j - cell * chn;
-      out[(int64_t)cell * NE + base + k] = s_rec[cell * PAD + k];
+    if ((chn & 1) == 0 && (NE & 1) == 0 && (base & 1) == 0) {
+      // pairs of values as 16-byte stores (records and chunks start even)
+      const int half = chn / 2;
+      for (int j = t; j < ncell * half; j += kQ1ElemBlock) {
+        const int cell = j / half, k = 2 * (j - cell * half);
+        *reinterpret_cast<double2*>(out + (int64_t)cell * NE + base + k) =
+            make_double2(s_rec[cell * PAD + k], s_rec[cell * PAD + k + 1]);
+      }
+    }
+    else {
+      for (int j = t; j < ncell * chn; j += kQ1ElemBlock) {
+        const int cell = j / chn, k = j - cell * chn;
+        out[(int64_t)cell * NE + base + k] = s_rec[cell * PAD + k];
+      }
     }
     __syncthreads();
   }
