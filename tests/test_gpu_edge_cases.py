@@ -157,3 +157,25 @@ def test_every_dof_eliminated(afem_gpu, method):
     r = s.solve(method=code, rtol=1e-10, max_iter=10)
     assert r.converged
     assert np.abs(s.solution().numpy() - 2.5).max() <= 1e-12
+
+
+@pytest.mark.parametrize("nbytes", [16, 16 * 1024, 16 * 1024 + 16, (1 << 20) + 48])
+def test_stream_copy_exact(afem_gpu, nbytes):
+    """The reference copy the roofline figures are set beside: one 16 KB chunk
+    per workgroup, a partial last chunk, every byte copied and none beyond."""
+    A = afem_gpu
+    print(f"[case] stream copy {nbytes} B", flush=True)
+    rng = np.random.default_rng(nbytes)
+    src = rng.integers(0, 256, nbytes + 64, dtype=np.uint8)
+    d_src = A.DeviceArray.from_numpy(src)
+    d_dst = A.DeviceArray.from_numpy(np.full(nbytes + 64, 7, np.uint8))
+    A.check(A.lib().afem_stream_copy(d_dst.c, d_src.c, nbytes))
+    out = d_dst.numpy()
+    assert np.array_equal(out[:nbytes], src[:nbytes])
+    assert np.all(out[nbytes:] == 7)
+
+
+def test_stream_copy_rejects_unaligned(afem_gpu):
+    A = afem_gpu
+    d = A.DeviceArray(64, np.uint8)
+    assert A.lib().afem_stream_copy(d.c, d.c, 24) == A.ERR_ARGUMENT
