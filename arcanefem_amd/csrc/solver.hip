@@ -29,6 +29,8 @@
 #include "linsys.h"
 
 #include <chrono>
+#include <type_traits>
+#include <vector>
 #include <cmath>
 
 namespace afem {
@@ -1006,9 +1008,34 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
     if (row >= n)
       continue;
     const int64_t s = row >> 6;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if constexpr (std::is_same<MT, uint32_t>::value) {
+      // BF16 words, 4 slots per 16-byte load (off holds the 4-slot layout)
+      typedef unsigned v4u __attribute__((ext_vector_type(4)));
+      const int64_t base = (int64_t)off[s] * 64 + 4 * (row & 63);
+      const int w4 = off[s + 1] - off[s];
+      for (int j = 0; j < w4; j += 8) {
+        const v4u a = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(mv + base + (int64_t)j * 64));
+        v4u c = { 0u, 0u, 0u, 0u };
+        if (j + 4 < w4)
+          c = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(mv + base + (int64_t)(j + 4) * 64));
+        const uint32_t wd[8] = { a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w };
+        int d[8];
+        float m[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          d[u] = (int)(int16_t)(wd[u] & 0xffffu);
+          m[u] = __uint_as_float(wd[u] & 0xffff0000u);
+        }
+        s0 += m[0] * xin(row + d[0]) + m[4] * xin(row + d[4]);
+        s1 += m[1] * xin(row + d[1]) + m[5] * xin(row + d[5]);
+        s2 += m[2] * xin(row + d[2]) + m[6] * xin(row + d[6]);
+        s3 += m[3] * xin(row + d[3]) + m[7] * xin(row + d[7]);
+      }
+    }
+    else {
     const int64_t base = (int64_t)off[s] * 64 + (row & 63);
     const int w = off[s + 1] - off[s];
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     int j = 0;
     for (; j + 7 < w; j += 8) {
       const int64_t i0 = base + (int64_t)j * 64;
@@ -1047,6 +1074,7 @@ __device__ __forceinline__ void prec_sell16_rows(int32_t n, const int32_t* __res
       s1 += m[1] * xin(row + d[1]);
       s2 += m[2] * xin(row + d[2]);
     }
+    }
     const float zrow = (s0 + s1) + (s2 + s3);
     const float rin = xin(row);
     z[row] = zrow;
@@ -1083,14 +1111,27 @@ k_prec_sell16(int k, int32_t n, const int32_t* __restrict__ off, const int16_t* 
 }
 
 // BF16 (round to nearest even) of the FP32 preconditioner values, packed
-// with the 16-bit column offsets: word = bf16 << 16 | (uint16_t)offset.
-__global__ void k_pack_bf16(int64_t n, const float* __restrict__ in, const int16_t* __restrict__ dcol,
-                            uint32_t* __restrict__ out)
+// with the 16-bit column offsets: word = bf16 << 16 | (uint16_t)offset.  The
+// words of a lane's slots 4q .. 4q + 3 are adjacent -- slot j of lane l of
+// slice s at off4[s] * 64 + 256 (j / 4) + 4 l + j % 4, widths rounded up to
+// multiples of 4 with zero words (value 0, offset 0) -- so one 16-byte load
+// reads four slots.  One 64-thread workgroup per slice.
+__global__ void __launch_bounds__(64)
+k_pack_bf16(const int32_t* __restrict__ off, const int32_t* __restrict__ off4, const float* __restrict__ in,
+            const int16_t* __restrict__ dcol, uint32_t* __restrict__ out)
 {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t u = __float_as_uint(in[i]);
-    const uint32_t h = (u & 0x7fffffffu) > 0x7f800000u ? 0x7fc0u : (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-    out[i] = (h << 16) | (uint32_t)(uint16_t)dcol[i];
+  const int64_t s = blockIdx.x;
+  const int l = threadIdx.x;
+  const int w = off[s + 1] - off[s], w4 = off4[s + 1] - off4[s];
+  for (int j = 0; j < w4; ++j) {
+    uint32_t word = 0u;
+    if (j < w) {
+      const int64_t i = ((int64_t)off[s] + j) * 64 + l;
+      const uint32_t u = __float_as_uint(in[i]);
+      const uint32_t h = (u & 0x7fffffffu) > 0x7f800000u ? 0x7fc0u : (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+      word = (h << 16) | (uint32_t)(uint16_t)dcol[i];
+    }
+    out[(int64_t)off4[s] * 64 + 256 * (j >> 2) + 4 * l + (j & 3)] = word;
   }
 }
 
@@ -1256,7 +1297,8 @@ struct afem_solver_ws
   bool forced_format = false;
   // sparse preconditioner matrix (pattern of A, FP32), CSR and SELL order
   afem::DBuf<float> spai32, spai_sell32;
-  afem::DBuf<uint32_t> spai_sell_pk; // BF16 values packed with the 16-bit column offsets
+  afem::DBuf<uint32_t> spai_sell_pk; // BF16 values packed with the 16-bit column offsets, 4 slots per lane together
+  afem::DBuf<int32_t> sell_off4;     // slice offsets of that layout (widths rounded up to multiples of 4)
   bool has_spai = false;
   bool spai_sell_valid = false;
   bool spai_bf16_valid = false;
@@ -1359,6 +1401,7 @@ static void build_sell_pattern(afem_system* s, afem_solver_ws* w)
   hipLaunchKernelGGL(k_sell_width, dim3(grid_for(nslice, 256)), dim3(256), 0, stream(), nrows, s->row_index, width.p);
   check_launch("sell_width");
   w->sell_off.alloc((size_t)nslice + 1);
+  w->sell_off4.release(); // the 4-slot layout follows the slices
   const int64_t cols = exclusive_scan_i32(width.p, w->sell_off.p, nslice);
   AFEM_REQUIRE(cols * 64 * s->block * s->block < (1LL << 31), AFEM_ERR_ARGUMENT, "matrix too large for SELL offsets");
   w->sell_col.alloc((size_t)cols * 64);
@@ -1728,9 +1771,21 @@ static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
   }
   if (bf16 && !w->spai_bf16_valid) {
     AFEM_REQUIRE(w->has_dcol, AFEM_ERR_STATE, "BF16 preconditioner values need the 16-bit offset layout");
-    w->spai_sell_pk.alloc(w->spai_sell32.n);
-    hipLaunchKernelGGL(k_pack_bf16, dim3(2048), dim3(256), 0, stream(), (int64_t)w->spai_sell32.n, w->spai_sell32.p,
-                       w->sell_dcol.p, w->spai_sell_pk.p);
+    const int64_t nslice = ((int64_t)s->nb_dof + 63) / 64;
+    if (w->sell_off4.n != (size_t)nslice + 1) { // the 4-slot layout's slice offsets, once per pattern
+      std::vector<int32_t> o((size_t)nslice + 1), o4((size_t)nslice + 1, 0);
+      w->sell_off.download(o.data(), o.size());
+      for (int64_t k = 0; k < nslice; ++k)
+        o4[k + 1] = o4[k] + (o[k + 1] - o[k] + 3) / 4 * 4;
+      w->sell_off4.alloc(o4.size());
+      w->sell_off4.upload(o4.data(), o4.size());
+    }
+    std::vector<int32_t> last(1);
+    AFEM_HIP(hipMemcpyAsync(last.data(), w->sell_off4.p + nslice, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+    AFEM_HIP(hipStreamSynchronize(stream()));
+    w->spai_sell_pk.alloc((size_t)last[0] * 64);
+    hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)nslice), dim3(64), 0, stream(), w->sell_off.p, w->sell_off4.p,
+                       w->spai_sell32.p, w->sell_dcol.p, w->spai_sell_pk.p);
     check_launch("spai_to_bf16");
     w->spai_bf16_valid = true;
   }
@@ -1747,9 +1802,9 @@ static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P
     const int bs = 256;
     auto go = [&](auto btag) {
       constexpr int BS = decltype(btag)::value;
-      if (bf16)
+      if (bf16) // the packed words carry their offsets, in the 4-slot layout
         hipLaunchKernelGGL((k_prec_sell16<INIT, uint32_t, BS>), dim3(nblk), dim3(BS), 0, stream(), k, s->nb_dof,
-                           w->sell_off.p, w->sell_dcol.p, w->spai_sell_pk.p, in, w->r32.p, w->z32.p, w->p32.p,
+                           w->sell_off4.p, w->sell_dcol.p, w->spai_sell_pk.p, in, w->r32.p, w->z32.p, w->p32.p,
                            w->q32.p, s->rhs.p, P, nblk, w->st.p);
       else
         hipLaunchKernelGGL((k_prec_sell16<INIT, float, BS>), dim3(nblk), dim3(BS), 0, stream(), k, s->nb_dof,
