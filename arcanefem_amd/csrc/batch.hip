@@ -139,15 +139,21 @@ __device__ __forceinline__ bool reliable(const SegState& st, double rr, int k)
   return rr <= st.delta2 * st.rr_ref[(k - 1) & 1] || rr <= st.tol2 || k + 1 >= st.max_iter || !isfinite(rr);
 }
 
-// Row product of G lanes: returns the row sum in lane 0 of the group.
+// Row product of G lanes: returns the row sum in lane 0 of the group.  The
+// FP32 values are in difference form (k_prepare): the diagonal slot holds the
+// row sum s_i and multiplies x_i, every other slot multiplies x_j - x_i.
 template <int G, class VT, class XT>
 __device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                           const VT* __restrict__ v, const XT* __restrict__ x, int32_t row, int lane)
 {
-  using A = typename std::conditional<sizeof(VT) == 4, float, double>::type;
+  constexpr bool DIFF = sizeof(VT) == 4;
+  using A = typename std::conditional<DIFF, float, double>::type;
+  const A xi = DIFF ? (A)x[row] : (A)0;
   A sum = 0;
-  for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += G)
-    sum += (A)v[k] * (A)x[ci[k]];
+  for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += G) {
+    const int32_t c = ci[k];
+    sum += (A)v[k] * (DIFF ? (A)x[c] - (c == row ? (A)0 : xi) : (A)x[c]);
+  }
   return (double)group_reduce_sum<G>(sum);
 }
 
@@ -157,12 +163,15 @@ template <class VT, class XT>
 __device__ __forceinline__ double sell_row(const Seg& g, const VT* __restrict__ v, const XT* __restrict__ x,
                                            int32_t r, int32_t sr0, int32_t sl0)
 {
-  using A = typename std::conditional<sizeof(VT) == 4, float, double>::type;
+  constexpr bool DIFF = sizeof(VT) == 4; // FP32 values in difference form (offset 0: the diagonal or padding)
+  using A = typename std::conditional<DIFF, float, double>::type;
   const int32_t local = r - sr0;
   const int32_t sl = sl0 + (local >> 6);
   const int64_t base = (int64_t)g.soff[sl] * 64 + (local & 63);
   const int w = g.soff[sl + 1] - g.soff[sl];
   const XT* xr = x + r;
+  const A xi = DIFF ? (A)xr[0] : (A)0;
+  auto gx = [&](int d) { return DIFF ? (A)xr[d] - (d != 0 ? xi : (A)0) : (A)xr[d]; };
   A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int j = 0;
   for (; j + 3 < w; j += 4) {
@@ -174,10 +183,10 @@ __device__ __forceinline__ double sell_row(const Seg& g, const VT* __restrict__ 
       d[u] = __builtin_nontemporal_load(g.sdcol + i0 + 64 * u);
       m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
     }
-    s0 += (A)m[0] * (A)xr[d[0]];
-    s1 += (A)m[1] * (A)xr[d[1]];
-    s2 += (A)m[2] * (A)xr[d[2]];
-    s3 += (A)m[3] * (A)xr[d[3]];
+    s0 += (A)m[0] * gx(d[0]);
+    s1 += (A)m[1] * gx(d[1]);
+    s2 += (A)m[2] * gx(d[2]);
+    s3 += (A)m[3] * gx(d[3]);
   }
   if (j < w) {
     const int64_t i0 = base + (int64_t)j * 64;
@@ -190,9 +199,9 @@ __device__ __forceinline__ double sell_row(const Seg& g, const VT* __restrict__ 
         d[u] = __builtin_nontemporal_load(g.sdcol + i0 + 64 * u);
         m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
       }
-    s0 += (A)m[0] * (A)xr[d[0]];
-    s1 += (A)m[1] * (A)xr[d[1]];
-    s2 += (A)m[2] * (A)xr[d[2]];
+    s0 += (A)m[0] * gx(d[0]);
+    s1 += (A)m[1] * gx(d[1]);
+    s2 += (A)m[2] * gx(d[2]);
   }
   return (double)((s0 + s1) + (s2 + s3));
 }
@@ -376,10 +385,11 @@ __global__ void k_sell_columns(int32_t nslice, const int32_t* __restrict__ slice
     atomicOr(overflow, 1);
 }
 
-// FP64 and FP32 values of the SELL copy (padding slots: 0).
+// FP64 and FP32 values of the SELL copy (padding slots: 0); the FP32 ones
+// from the difference-form CSR copy.
 __global__ void k_sell_values(int32_t nslice, const int32_t* __restrict__ slice_row, const int32_t* __restrict__ soff,
-                              const int32_t* __restrict__ rp, const double* __restrict__ v, double* __restrict__ v64,
-                              float* __restrict__ v32)
+                              const int32_t* __restrict__ rp, const double* __restrict__ v,
+                              const float* __restrict__ v32csr, double* __restrict__ v64, float* __restrict__ v32)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nslice * 64)
@@ -389,10 +399,9 @@ __global__ void k_sell_values(int32_t nslice, const int32_t* __restrict__ slice_
   const int32_t r = slice_row[sl] + lane;
   const int32_t len = r < slice_row[sl + 1] ? rp[r + 1] - rp[r] : 0;
   for (int j = 0; j < soff[sl + 1] - soff[sl]; ++j) {
-    const double val = j < len ? v[rp[r] + j] : 0.0;
     const int64_t e = ((int64_t)soff[sl] + j) * 64 + lane;
-    v64[e] = val;
-    v32[e] = (float)val;
+    v64[e] = j < len ? v[rp[r] + j] : 0.0;
+    v32[e] = j < len ? v32csr[rp[r] + j] : 0.f;
   }
 }
 
@@ -539,18 +548,33 @@ __global__ void k_initial_guess(int32_t n, const uint8_t* __restrict__ elim, con
     x[r] = (elim[r] == 1 || elim[r] == 2) ? elim_value[r] : 0.0;
 }
 
+// FP32 values in difference form (off-diagonal entries rounded, the
+// diagonal slot holding the row sum taken in FP64: the FP32 product is then
+// s_i x_i + sum_{j != i} a_ij (x_j - x_i), free of the cancellation between the
+// diagonal and the off-diagonal terms on smooth vectors) and the Jacobi
+// inverse.  *missing = 1 when a non-empty row has no diagonal entry.
 __global__ void k_prepare(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                          const double* __restrict__ v, int64_t nnz, float* __restrict__ v32,
-                          float* __restrict__ dinv)
+                          const double* __restrict__ v, float* __restrict__ v32, float* __restrict__ dinv,
+                          int32_t* __restrict__ missing)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t k = t; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
-    v32[k] = (float)v[k];
   for (int64_t r = t; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    double d = 0.0;
-    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
-      if (ci[k] == r)
-        d = v[k];
+    double d = 0.0, sum = 0.0;
+    int32_t dpos = -1;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+      const double a = v[k];
+      sum += a;
+      if (ci[k] == r) {
+        d = a;
+        dpos = k;
+      }
+      else
+        v32[k] = (float)a;
+    }
+    if (dpos >= 0)
+      v32[dpos] = (float)sum;
+    else if (rp[r] < rp[r + 1])
+      *missing = 1;
     dinv[r] = (float)(d != 0.0 ? 1.0 / d : 1.0);
   }
 }
@@ -746,12 +770,15 @@ __device__ __forceinline__ float p_of(const float* z, const float* pold, float b
   return __fmaf_rn(beta, xload<LOCAL>(pold + c), xload<LOCAL>(z + c));
 }
 
-// Row r of A32 p_new (p_new formed from z and p_old on the fly).
+// Row r of A32 p_new (p_new formed from z and p_old on the fly), the FP32
+// values in difference form: the diagonal slot (offset 0, also the padding,
+// whose value is 0) multiplies p_i, every other slot p_j - p_i.
 template <bool LOCAL>
 __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const float* pold, float beta, int32_t r,
                                             int32_t sr0, int32_t sl0)
 {
   float s0 = 0.f, s1 = 0.f;
+  const float pi = p_of<LOCAL>(z, pold, beta, (int64_t)r);
   if (T.sell) {
     const int32_t local = r - sr0;
     const int32_t sl = sl0 + (local >> 6);
@@ -768,7 +795,7 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
       float g[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        g[u] = u < w ? p_of<LOCAL>(z, pold, beta, (int64_t)r + d[u]) : 0.f;
+        g[u] = u < w ? p_of<LOCAL>(z, pold, beta, (int64_t)r + d[u]) - (d[u] != 0 ? pi : 0.f) : 0.f;
 #pragma unroll
       for (int u = 0; u < 8; u += 2) {
         s0 += m[u] * g[u];
@@ -787,14 +814,15 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
         d[u] = T.sdcol[i0 + 64 * u];
         m[u] = T.v32[i0 + 64 * u];
       }
-      s0 += m[0] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[0]);
-      s1 += m[1] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[1]);
-      s2 += m[2] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[2]);
-      s3 += m[3] * p_of<LOCAL>(z, pold, beta, (int64_t)r + d[3]);
+      s0 += m[0] * (p_of<LOCAL>(z, pold, beta, (int64_t)r + d[0]) - (d[0] != 0 ? pi : 0.f));
+      s1 += m[1] * (p_of<LOCAL>(z, pold, beta, (int64_t)r + d[1]) - (d[1] != 0 ? pi : 0.f));
+      s2 += m[2] * (p_of<LOCAL>(z, pold, beta, (int64_t)r + d[2]) - (d[2] != 0 ? pi : 0.f));
+      s3 += m[3] * (p_of<LOCAL>(z, pold, beta, (int64_t)r + d[3]) - (d[3] != 0 ? pi : 0.f));
     }
     for (; j < w; ++j) {
       const int64_t i0 = base + (int64_t)j * 64;
-      s0 += T.v32[i0] * p_of<LOCAL>(z, pold, beta, (int64_t)r + T.sdcol[i0]);
+      const int d0 = T.sdcol[i0];
+      s0 += T.v32[i0] * (p_of<LOCAL>(z, pold, beta, (int64_t)r + d0) - (d0 != 0 ? pi : 0.f));
     }
     s0 += s2;
     s1 += s3;
@@ -802,12 +830,13 @@ __device__ __forceinline__ float prod_row32(const Team& T, const float* z, const
   else {
     const int32_t e = T.rp[r + 1];
     int32_t k = T.rp[r];
+    auto g = [&](int32_t c) { return p_of<LOCAL>(z, pold, beta, c) - (c != r ? pi : 0.f); };
     for (; k + 1 < e; k += 2) {
-      s0 += T.v32[k] * p_of<LOCAL>(z, pold, beta, T.ci[k]);
-      s1 += T.v32[k + 1] * p_of<LOCAL>(z, pold, beta, T.ci[k + 1]);
+      s0 += T.v32[k] * g(T.ci[k]);
+      s1 += T.v32[k + 1] * g(T.ci[k + 1]);
     }
     if (k < e)
-      s0 += T.v32[k] * p_of<LOCAL>(z, pold, beta, T.ci[k]);
+      s0 += T.v32[k] * g(T.ci[k]);
   }
   return s0 + s1;
 }
@@ -1308,16 +1337,24 @@ void prepare(afem_batch* b)
     b->q32.alloc(n);
     b->r64.alloc(n);
   }
-  hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, stream(), n, s->row_index, s->columns, s->values, s->nnz,
-                     b->v32.p, b->dinv.p);
+  DBuf<int32_t> missing(1);
+  missing.zero();
+  hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, stream(), n, s->row_index, s->columns, s->values, b->v32.p,
+                     b->dinv.p, missing.p);
   check_launch("batch_prepare");
+  int32_t h = 0;
+  missing.download(&h, 1);
+  AFEM_REQUIRE(h == 0, AFEM_ERR_ARGUMENT,
+               "the batched solver needs a diagonal entry in every non-empty row (its FP32 product is in "
+               "difference form)");
   if (b->sell_rp != s->row_index) {
     drop_graph(b);
     build_sell(b);
   }
   if (b->sell_ok) {
     hipLaunchKernelGGL(k_sell_values, dim3(grid_for((int64_t)b->nslice * 64, 256)), dim3(256), 0, stream(),
-                       b->nslice, b->d_slice_row.p, b->d_soff.p, s->row_index, s->values, b->sv64.p, b->sv32.p);
+                       b->nslice, b->d_slice_row.p, b->d_soff.p, s->row_index, s->values, b->v32.p, b->sv64.p,
+                       b->sv32.p);
     check_launch("batch_sell_values");
   }
   b->prepared = true;

@@ -217,7 +217,10 @@ __device__ __forceinline__ void spmv_epilogue(int64_t row, double sum, const XT*
 }
 
 // CSR, G lanes per row, 256/G rows per tile.
-template <int G, class VT, class XT, int MODE>
+// DIFF (scalar FP32 copies in difference form, see k_to_float_rowsum): the
+// diagonal slot holds the row sum s_i and multiplies x_i, every other slot
+// multiplies x_j - x_i, so y_i = s_i x_i + sum_{j != i} a_ij (x_j - x_i).
+template <int G, class VT, class XT, int MODE, bool DIFF = false>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_csr(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const VT* __restrict__ v,
            const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
@@ -237,8 +240,16 @@ k_spmv_csr(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict_
     if (row < n) {
       const int32_t s = rp[row], e = rp[row + 1];
       A sum = 0;
-      for (int32_t k = s + lane; k < e; k += G)
-        sum += (A)v[k] * (A)x[ci[k]];
+      if constexpr (DIFF) {
+        const A xi = (A)x[row];
+        for (int32_t k = s + lane; k < e; k += G) {
+          const int32_t c = ci[k];
+          sum += (A)v[k] * ((A)x[c] - (c == row ? (A)0 : xi));
+        }
+      }
+      else
+        for (int32_t k = s + lane; k < e; k += G)
+          sum += (A)v[k] * (A)x[ci[k]];
       sum = group_reduce_sum<G>(sum);
       if (lane == 0)
         spmv_epilogue<MODE>(row, (double)sum, x, y, b, part);
@@ -251,7 +262,7 @@ k_spmv_csr(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict_
 // SELL-64: slice s holds rows [64s, 64s+64); its slots (j, lane) live at
 // (off[s] + j) * 64 + lane for j < off[s+1] - off[s]; padding slots carry a
 // zero value and the row's own column.
-template <class VT, class XT, int MODE>
+template <class VT, class XT, int MODE, bool DIFF = false>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ col, const VT* __restrict__ v,
             const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
@@ -270,16 +281,18 @@ k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restric
       const int lane = (int)(row & 63);
       const int64_t base = (int64_t)off[s] * 64 + lane;
       const int w = off[s + 1] - off[s];
+      const A xi = DIFF ? (A)x[row] : (A)0;
+      auto gx = [&](int32_t c) { return DIFF ? (A)x[c] - (c == row ? (A)0 : xi) : (A)x[c]; };
       A s0 = 0, s1 = 0;
       int j = 0;
       for (; j + 1 < w; j += 2) {
         const int64_t i0 = base + (int64_t)j * 64, i1 = i0 + 64;
-        s0 += (A)v[i0] * (A)x[col[i0]];
-        s1 += (A)v[i1] * (A)x[col[i1]];
+        s0 += (A)v[i0] * gx(col[i0]);
+        s1 += (A)v[i1] * gx(col[i1]);
       }
       if (j < w) {
         const int64_t i0 = base + (int64_t)j * 64;
-        s0 += (A)v[i0] * (A)x[col[i0]];
+        s0 += (A)v[i0] * gx(col[i0]);
       }
       spmv_epilogue<MODE>(row, (double)(s0 + s1), x, y, b, part);
     }
@@ -294,7 +307,7 @@ k_spmv_sell(int32_t n, const int32_t* __restrict__ off, const int32_t* __restric
 // to leave the L2 to the gathered vector.  BS threads per workgroup (256 or
 // 512: the grid is capped at kMaxBlocks workgroups, so 512 doubles the waves
 // in flight without more reduction partials).
-template <class VT, class XT, int MODE, int UNR = 4, int BS = kBlock>
+template <class VT, class XT, int MODE, int UNR = 4, int BS = kBlock, bool DIFF = false>
 __global__ void __launch_bounds__(BS)
 k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restrict__ dcol,
               const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
@@ -313,6 +326,9 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
       const int64_t base = (int64_t)off[s] * 64 + (row & 63);
       const int w = off[s + 1] - off[s];
       const XT* xr = x + row;
+      // offset 0 is the diagonal slot (and the padding, whose value is 0)
+      const A xi = DIFF ? (A)xr[0] : (A)0;
+      auto gx = [&](int d) { return DIFF ? (A)xr[d] - (d != 0 ? xi : (A)0) : (A)xr[d]; };
       A s0 = 0, s1 = 0, s2 = 0, s3 = 0;
       int j = 0;
       if constexpr (UNR == 8) {
@@ -326,10 +342,10 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
             d[q] = __builtin_nontemporal_load(dcol + i0 + 64 * q);
             m[q] = __builtin_nontemporal_load(v + i0 + 64 * q);
           }
-          s0 += (A)m[0] * (A)xr[d[0]] + (A)m[4] * (A)xr[d[4]];
-          s1 += (A)m[1] * (A)xr[d[1]] + (A)m[5] * (A)xr[d[5]];
-          s2 += (A)m[2] * (A)xr[d[2]] + (A)m[6] * (A)xr[d[6]];
-          s3 += (A)m[3] * (A)xr[d[3]] + (A)m[7] * (A)xr[d[7]];
+          s0 += (A)m[0] * gx(d[0]) + (A)m[4] * gx(d[4]);
+          s1 += (A)m[1] * gx(d[1]) + (A)m[5] * gx(d[5]);
+          s2 += (A)m[2] * gx(d[2]) + (A)m[6] * gx(d[6]);
+          s3 += (A)m[3] * gx(d[3]) + (A)m[7] * gx(d[7]);
         }
       }
       for (; j + 3 < w; j += 4) {
@@ -338,10 +354,10 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
         const int d1 = __builtin_nontemporal_load(dcol + i0 + 64);
         const int d2 = __builtin_nontemporal_load(dcol + i0 + 128);
         const int d3 = __builtin_nontemporal_load(dcol + i0 + 192);
-        s0 += (A)__builtin_nontemporal_load(v + i0) * (A)xr[d0];
-        s1 += (A)__builtin_nontemporal_load(v + i0 + 64) * (A)xr[d1];
-        s2 += (A)__builtin_nontemporal_load(v + i0 + 128) * (A)xr[d2];
-        s3 += (A)__builtin_nontemporal_load(v + i0 + 192) * (A)xr[d3];
+        s0 += (A)__builtin_nontemporal_load(v + i0) * gx(d0);
+        s1 += (A)__builtin_nontemporal_load(v + i0 + 64) * gx(d1);
+        s2 += (A)__builtin_nontemporal_load(v + i0 + 128) * gx(d2);
+        s3 += (A)__builtin_nontemporal_load(v + i0 + 192) * gx(d3);
       }
       if (j < w) { // the last 1-3 slots, loads issued together
         const int64_t i0 = base + (int64_t)j * 64;
@@ -354,9 +370,9 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
             d[u] = __builtin_nontemporal_load(dcol + i0 + 64 * u);
             m[u] = __builtin_nontemporal_load(v + i0 + 64 * u);
           }
-        s0 += (A)m[0] * (A)xr[d[0]];
-        s1 += (A)m[1] * (A)xr[d[1]];
-        s2 += (A)m[2] * (A)xr[d[2]];
+        s0 += (A)m[0] * gx(d[0]);
+        s1 += (A)m[1] * gx(d[1]);
+        s2 += (A)m[2] * gx(d[2]);
       }
       spmv_epilogue<MODE>(row, (double)((s0 + s1) + (s2 + s3)), x, y, b, part);
     }
@@ -581,6 +597,37 @@ __global__ void k_to_float(int64_t n, const double* __restrict__ in, float* __re
 {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = (float)in[i];
+}
+
+// FP32 copy of a scalar matrix in difference form: the off-diagonal entries
+// rounded to FP32, the diagonal entry replaced by the row sum s_i taken in
+// FP64.  The FP32 SpMV then forms A x as s_i x_i + sum_{j != i} a_ij (x_j - x_i):
+// for a smooth x (the slow modes of a diffusion operator, whose rows nearly
+// sum to zero) the terms are small differences instead of a large diagonal
+// term cancelling the off-diagonal ones, so rounding A to FP32 no longer
+// perturbs the low end of its spectrum.  *missing = 1 when a non-empty row
+// has no diagonal entry (the form needs one slot per row for s_i).
+__global__ void k_to_float_rowsum(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                  const double* __restrict__ v, float* __restrict__ out, int32_t* __restrict__ missing)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n)
+    return;
+  const int32_t e = rp[r + 1];
+  double sum = 0.0;
+  int32_t dpos = -1;
+  for (int32_t k = rp[r]; k < e; ++k) {
+    const double a = v[k];
+    sum += a;
+    if (ci[k] == r)
+      dpos = k;
+    else
+      out[k] = (float)a;
+  }
+  if (dpos >= 0)
+    out[dpos] = (float)sum;
+  else if (rp[r] < e)
+    *missing = 1;
 }
 
 // x0: zero except eliminated rows, which start at their value (keeps a
@@ -1286,6 +1333,7 @@ struct afem_solver_ws
   afem::DBuf<double> sell_v64;
   afem::DBuf<float> sell_v32;
   bool has_fp32 = false;
+  bool diff32 = false; // the FP32 copies (scalar matrices) are in difference form
   bool values_valid = false;
   bool pattern_valid = false;
   int format = -1; // chosen SpMV format
@@ -1413,7 +1461,13 @@ static void fill_sell(afem_system* s, afem_solver_ws* w, bool columns)
 {
   const int32_t nrows = s->nb_dof / s->block;
   const int64_t rows_pad = ((int64_t)nrows + 63) / 64 * 64;
-  float* v32 = w->has_fp32 ? w->sell_v32.p : nullptr;
+  // the difference-form FP32 values come from their CSR-order copy
+  float* v32 = (w->has_fp32 && !w->diff32) ? w->sell_v32.p : nullptr;
+  if (w->has_fp32 && w->diff32) {
+    hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows, s->row_index,
+                       w->vals32.p, w->sell_off.p, w->sell_v32.p);
+    check_launch("sell_fill_f32");
+  }
   if (s->block == 3)
     hipLaunchKernelGGL(k_sell_fill<3>, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows, s->row_index,
                        s->columns, s->values, w->sell_off.p, w->sell_col.p, w->sell_v64.p, v32, columns ? 1 : 0);
@@ -1452,8 +1506,21 @@ static void ensure_derived(afem_system* s, afem_solver_ws* w, bool need_fp32)
   }
   check_launch("jacobi_setup");
   if (w->has_fp32) {
-    hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, stream(), (int64_t)s->nnz, s->values, w->vals32.p);
-    check_launch("to_float");
+    w->diff32 = false;
+    if (s->block == 1) {
+      DBuf<int32_t> missing(1);
+      missing.zero();
+      hipLaunchKernelGGL(k_to_float_rowsum, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, s->row_index,
+                         s->columns, s->values, w->vals32.p, missing.p);
+      check_launch("to_float_rowsum");
+      int32_t h = 1;
+      missing.download(&h, 1);
+      w->diff32 = h == 0;
+    }
+    if (!w->diff32) {
+      hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, stream(), (int64_t)s->nnz, s->values, w->vals32.p);
+      check_launch("to_float");
+    }
   }
   if (w->pattern_valid && w->sell_v64.p) {
     if (w->has_fp32 && !w->sell_v32.p)
@@ -1509,6 +1576,8 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
     sell_vals = w->sell_v64.p;
   }
   (void)fp32;
+  // scalar FP32 values in difference form take the difference kernels
+  const bool diff = sizeof(VT) == 4 && w->diff32 && s->block == 1;
   if (s->block == 3) {
     const int32_t nb_node = s->nb_dof / 3;
     if (w->format == kFmtSell)
@@ -1519,8 +1588,12 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
                          s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
   }
   else if (w->format == kFmtSell) {
-    hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof, w->sell_off.p,
-                       w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+    if (diff)
+      hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE, true>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                         w->sell_off.p, w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+    else
+      hipLaunchKernelGGL((k_spmv_sell<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                         w->sell_off.p, w->sell_col.p, sell_vals, x, y, b, gate, part_out);
   }
   else if (w->format == kFmtSell16 || w->format == kFmtSell16x8) {
     const int fmt_unr = w->format == kFmtSell16x8 ? 8 : 4;
@@ -1528,8 +1601,12 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
     const int bs = (sizeof(VT) == 4 && w->unr32) ? w->bs32 : w->bs64;
     auto go = [&](auto utag, auto btag) {
       constexpr int U = decltype(utag)::value, BS = decltype(btag)::value;
-      hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, U, BS>), dim3(nblk), dim3(BS), 0, stream(), s->nb_dof,
-                         w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+      if (diff)
+        hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, U, BS, true>), dim3(nblk), dim3(BS), 0, stream(),
+                           s->nb_dof, w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
+      else
+        hipLaunchKernelGGL((k_spmv_sell16<VT, XT, MODE, U, BS>), dim3(nblk), dim3(BS), 0, stream(), s->nb_dof,
+                           w->sell_off.p, w->sell_dcol.p, sell_vals, x, y, b, gate, part_out);
     };
     using I4 = std::integral_constant<int, 4>;
     using I8 = std::integral_constant<int, 8>;
@@ -1543,8 +1620,12 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
   else {
     auto go = [&](auto gtag) {
       constexpr int G = decltype(gtag)::value;
-      hipLaunchKernelGGL((k_spmv_csr<G, VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
-                         s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
+      if (diff)
+        hipLaunchKernelGGL((k_spmv_csr<G, VT, XT, MODE, true>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                           s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
+      else
+        hipLaunchKernelGGL((k_spmv_csr<G, VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), s->nb_dof,
+                           s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
     };
     switch (w->lanes) {
     case 4: go(std::integral_constant<int, 4>{}); break;

