@@ -382,7 +382,7 @@ k_spmv_sell16(int32_t n, const int32_t* __restrict__ off, const int16_t* __restr
 }
 
 // BSR 3x3 blocks stored per block (9 contiguous values), G lanes per node row.
-template <int G, class VT, class XT, int MODE>
+template <int G, class VT, class XT, int MODE, bool DIFF = false>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_bsr3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const VT* __restrict__ v,
             const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b, Gate gate,
@@ -401,11 +401,18 @@ k_spmv_bsr3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __re
     const int64_t a = t * RPT + sub;
     if (a < nb_node) {
       const int32_t s = rp[a], e = rp[a + 1];
+      // DIFF: the diagonal block holds the block row sum and multiplies x_a,
+      // every other block x_c - x_a
+      const A xa0 = DIFF ? (A)x[3 * a] : (A)0, xa1 = DIFF ? (A)x[3 * a + 1] : (A)0,
+              xa2 = DIFF ? (A)x[3 * a + 2] : (A)0;
       A s0 = 0, s1 = 0, s2 = 0;
       for (int32_t k = s + lane; k < e; k += G) {
         const VT* blk = v + 9 * (int64_t)k;
-        const int64_t c = 3 * (int64_t)ci[k];
-        const A x0 = (A)x[c], x1 = (A)x[c + 1], x2 = (A)x[c + 2];
+        const int32_t cn = ci[k];
+        const int64_t c = 3 * (int64_t)cn;
+        const bool off = DIFF && cn != a;
+        const A x0 = (A)x[c] - (off ? xa0 : (A)0), x1 = (A)x[c + 1] - (off ? xa1 : (A)0),
+                x2 = (A)x[c + 2] - (off ? xa2 : (A)0);
         s0 += (A)blk[0] * x0 + (A)blk[1] * x1 + (A)blk[2] * x2;
         s1 += (A)blk[3] * x0 + (A)blk[4] * x1 + (A)blk[5] * x2;
         s2 += (A)blk[6] * x0 + (A)blk[7] * x1 + (A)blk[8] * x2;
@@ -427,7 +434,7 @@ k_spmv_bsr3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __re
 // SELL-64 with 3x3 blocks: slot (j, lane) of slice s has its column at
 // (off[s]+j)*64 + lane and value q (row-major in the block) at
 // ((off[s]+j)*9 + q)*64 + lane.
-template <class VT, class XT, int MODE>
+template <class VT, class XT, int MODE, bool DIFF = false>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_sell3(int32_t nb_node, const int32_t* __restrict__ off, const int32_t* __restrict__ col,
              const VT* __restrict__ v, const XT* __restrict__ x, XT* __restrict__ y, const double* __restrict__ b,
@@ -446,12 +453,17 @@ k_spmv_sell3(int32_t nb_node, const int32_t* __restrict__ off, const int32_t* __
       const int lane = (int)(a & 63);
       const int64_t o = off[s];
       const int w = off[s + 1] - off[s];
+      const A xa0 = DIFF ? (A)x[3 * a] : (A)0, xa1 = DIFF ? (A)x[3 * a + 1] : (A)0,
+              xa2 = DIFF ? (A)x[3 * a + 2] : (A)0;
       A s0 = 0, s1 = 0, s2 = 0;
       for (int j = 0; j < w; ++j) {
         const int64_t slot = o + j;
-        const int64_t c = 3 * (int64_t)col[slot * 64 + lane];
+        const int32_t cn = col[slot * 64 + lane];
+        const int64_t c = 3 * (int64_t)cn;
         const VT* vb = v + slot * 9 * 64 + lane;
-        const A x0 = (A)x[c], x1 = (A)x[c + 1], x2 = (A)x[c + 2];
+        const bool offd = DIFF && cn != a; // padding slots: the own column and a zero block
+        const A x0 = (A)x[c] - (offd ? xa0 : (A)0), x1 = (A)x[c + 1] - (offd ? xa1 : (A)0),
+                x2 = (A)x[c + 2] - (offd ? xa2 : (A)0);
         s0 += (A)vb[0] * x0 + (A)vb[64] * x1 + (A)vb[128] * x2;
         s1 += (A)vb[192] * x0 + (A)vb[256] * x1 + (A)vb[320] * x2;
         s2 += (A)vb[384] * x0 + (A)vb[448] * x1 + (A)vb[512] * x2;
@@ -1285,6 +1297,58 @@ k_pupdate_flex(int k, int32_t n, float* __restrict__ p, const float* __restrict_
   }
 }
 
+// k_to_float_rowsum for 3x3 blocks: the diagonal block holds the block row
+// sum (taken in FP64), the other blocks are rounded; rigid translations then
+// meet the block row sums exactly in the FP32 product.
+__global__ void k_to_float_rowsum3(int32_t nb_node, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                   const double* __restrict__ v, float* __restrict__ out, int32_t* __restrict__ missing)
+{
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nb_node)
+    return;
+  const int32_t e = rp[a + 1];
+  double sum[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+  int32_t dpos = -1;
+  for (int32_t k = rp[a]; k < e; ++k) {
+    const double* blk = v + 9 * (int64_t)k;
+    const bool diag = ci[k] == a;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      sum[q] += blk[q];
+      if (!diag)
+        out[9 * (int64_t)k + q] = (float)blk[q];
+    }
+    if (diag)
+      dpos = k;
+  }
+  if (dpos >= 0) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      out[9 * (int64_t)dpos + q] = (float)sum[q];
+  }
+  else if (rp[a] < e)
+    *missing = 1;
+}
+
+// SELL copy of FP32 3x3 blocks given in CSR block order (slot layout of
+// k_sell_fill<3>).
+__global__ void k_sell_fill_f32_b3(int32_t nrows, const int32_t* __restrict__ rp, const float* __restrict__ v,
+                                   const int32_t* __restrict__ off, float* __restrict__ out)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t rows_pad = ((int64_t)nrows + 63) / 64 * 64;
+  if (r >= rows_pad)
+    return;
+  const int64_t s = r >> 6;
+  const int lane = (int)(r & 63);
+  const int w = off[s + 1] - off[s];
+  const int32_t len = (r < nrows) ? rp[r + 1] - rp[r] : 0;
+  const int32_t b0 = (r < nrows) ? rp[r] : 0;
+  for (int j = 0; j < w; ++j)
+    for (int q = 0; q < 9; ++q)
+      out[(((int64_t)off[s] + j) * 9 + q) * 64 + lane] = (j < len) ? v[9 * ((int64_t)b0 + j) + q] : 0.f;
+}
+
 // SELL copy of FP32 values given in CSR order (same slice layout as A's).
 __global__ void k_sell_fill_f32(int32_t nrows, const int32_t* __restrict__ rp, const float* __restrict__ v,
                                 const int32_t* __restrict__ off, float* __restrict__ out)
@@ -1464,8 +1528,12 @@ static void fill_sell(afem_system* s, afem_solver_ws* w, bool columns)
   // the difference-form FP32 values come from their CSR-order copy
   float* v32 = (w->has_fp32 && !w->diff32) ? w->sell_v32.p : nullptr;
   if (w->has_fp32 && w->diff32) {
-    hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows, s->row_index,
-                       w->vals32.p, w->sell_off.p, w->sell_v32.p);
+    if (s->block == 3)
+      hipLaunchKernelGGL(k_sell_fill_f32_b3, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows,
+                         s->row_index, w->vals32.p, w->sell_off.p, w->sell_v32.p);
+    else
+      hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), nrows,
+                         s->row_index, w->vals32.p, w->sell_off.p, w->sell_v32.p);
     check_launch("sell_fill_f32");
   }
   if (s->block == 3)
@@ -1507,11 +1575,15 @@ static void ensure_derived(afem_system* s, afem_solver_ws* w, bool need_fp32)
   check_launch("jacobi_setup");
   if (w->has_fp32) {
     w->diff32 = false;
-    if (s->block == 1) {
+    if (s->block == 1 || s->block == 3) {
       DBuf<int32_t> missing(1);
       missing.zero();
-      hipLaunchKernelGGL(k_to_float_rowsum, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, s->row_index,
-                         s->columns, s->values, w->vals32.p, missing.p);
+      if (s->block == 1)
+        hipLaunchKernelGGL(k_to_float_rowsum, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, s->row_index,
+                           s->columns, s->values, w->vals32.p, missing.p);
+      else
+        hipLaunchKernelGGL(k_to_float_rowsum3, dim3(grid_for(n / 3, 256)), dim3(256), 0, stream(), n / 3,
+                           s->row_index, s->columns, s->values, w->vals32.p, missing.p);
       check_launch("to_float_rowsum");
       int32_t h = 1;
       missing.download(&h, 1);
@@ -1576,13 +1648,21 @@ static void spmv_launch(afem_system* s, afem_solver_ws* w, bool fp32, const XT* 
     sell_vals = w->sell_v64.p;
   }
   (void)fp32;
-  // scalar FP32 values in difference form take the difference kernels
-  const bool diff = sizeof(VT) == 4 && w->diff32 && s->block == 1;
+  // FP32 values in difference form take the difference kernels
+  const bool diff = sizeof(VT) == 4 && w->diff32;
   if (s->block == 3) {
     const int32_t nb_node = s->nb_dof / 3;
-    if (w->format == kFmtSell)
-      hipLaunchKernelGGL((k_spmv_sell3<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node, w->sell_off.p,
-                         w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+    if (w->format == kFmtSell) {
+      if (diff)
+        hipLaunchKernelGGL((k_spmv_sell3<VT, XT, MODE, true>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node,
+                           w->sell_off.p, w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+      else
+        hipLaunchKernelGGL((k_spmv_sell3<VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node,
+                           w->sell_off.p, w->sell_col.p, sell_vals, x, y, b, gate, part_out);
+    }
+    else if (diff)
+      hipLaunchKernelGGL((k_spmv_bsr3<16, VT, XT, MODE, true>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node,
+                         s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);
     else
       hipLaunchKernelGGL((k_spmv_bsr3<16, VT, XT, MODE>), dim3(nblk), dim3(kBlock), 0, stream(), nb_node,
                          s->row_index, s->columns, csr_vals, x, y, b, gate, part_out);

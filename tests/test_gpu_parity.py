@@ -409,6 +409,38 @@ def test_block_jacobi_pcg_elasticity(afem_gpu, fmt):
         assert np.abs(x - direct).max() <= 1e-6 * np.abs(direct).max()
 
 
+@pytest.mark.parametrize("fmt", [-1, 16])
+def test_block_fp32_difference_form(afem_gpu, fmt):
+    """The FP32 copy of a 3x3-block matrix is in difference form (diagonal
+    block = block row sum, other blocks times x_c - x_a): a rigid translation
+    meets the block row sums (zero up to FP64 rounding without Dirichlet
+    rows) to FP64 accuracy, where a plain FP32 product leaves about 1e-8 of
+    the diagonal blocks' size; a random vector matches the FP64 product to
+    FP32 accuracy in SELL-3 and BSR-16 order."""
+    A = afem_gpu
+    print(f"[case] elasticity fp32 difference form spmv_lanes={fmt}", flush=True)
+    X, C = box(3, 6, 0.15)
+    N = X.shape[0]
+    lam, mu = O.lame(21e5, 0.28)
+    m = A.Mesh(3, X, C)
+    b = A.BSRFormat(m, 3, use_csr=False)
+    b.compute_sparsity()
+    b.assemble_elasticity(lam, mu)
+    s = A.DoFLinearSystem(3 * N)
+    s.set_bsr(b)
+    s.solve(raise_on_failure=False, method=A.SOLVER_PCG_MIXED, max_iter=1, spmv_lanes=fmt)
+    assert s.spmv_format() == ({-1: "sell", 16: "csr"}[fmt], {-1: 64, 16: 16}[fmt])
+    row, col, val, _ = b.download()
+    S = sp.bsr_matrix((val.reshape(-1, 3, 3), col, row), shape=(3 * N, 3 * N)).tocsr()
+    dy32 = A.DeviceArray(3 * N, np.float32)
+    t = np.tile(np.array([1000.0, -500.0, 250.0]), N)
+    s.spmv_fp32(A.DeviceArray.from_numpy(t.astype(np.float32)), dy32)
+    assert np.abs(dy32.numpy() - S @ t).max() <= 1e-12 * np.abs(val).max() * 1000.0
+    x = np.random.default_rng(5).normal(size=3 * N)
+    s.spmv_fp32(A.DeviceArray.from_numpy(x.astype(np.float32)), dy32)
+    assert np.abs(dy32.numpy() - S @ x).max() <= 2e-5 * np.abs(val).max() * np.abs(x).max()
+
+
 # ---- the reference's golden solutions, end to end on the GPU ---------------------------------
 
 def by_uid(d, x, ncomp=1):
