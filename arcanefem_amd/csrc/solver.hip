@@ -1722,8 +1722,15 @@ static int spmv_blocks(afem_system* s)
   return solver_blocks(s->nb_dof / s->block);
 }
 
-// Picks the SpMV format: times FP64 SpMV with each candidate (3 launches,
-// best of) and keeps the fastest.  A positive spmv_lanes forces CSR with that
+// Timing of the SpMV candidates: the first of kTuneReps runs warms up, the
+// best of the others counts; each run times kTuneLaunches back-to-back
+// launches, so a single launch's jitter (a few us against 50-80 us at a
+// million rows) does not pick a slower shape.
+constexpr int kTuneReps = 5;
+constexpr int kTuneLaunches = 3;
+
+// Picks the SpMV format: times FP64 SpMV with each candidate (best of the
+// timed runs) and keeps the fastest.  A positive spmv_lanes forces CSR with that
 // many lanes per row, -1 forces SELL-64.
 static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
 {
@@ -1775,10 +1782,11 @@ static void choose_format(afem_system* s, afem_solver_ws* w, int forced)
     w->lanes = c.lanes;
     w->bs64 = c.bs;
     float t_best = 1e30f;
-    for (int rep = 0; rep < 4; ++rep) {
+    for (int rep = 0; rep < kTuneReps; ++rep) {
       AFEM_HIP(hipEventRecord(e0, stream()));
-      spmv_launch<double, double, kPlain>(s, w, false, w->p64.p, w->q64.p, nullptr, always, nullptr, spmv_blocks(s),
-                                          "spmv_tune");
+      for (int l = 0; l < kTuneLaunches; ++l)
+        spmv_launch<double, double, kPlain>(s, w, false, w->p64.p, w->q64.p, nullptr, always, nullptr,
+                                            spmv_blocks(s), "spmv_tune");
       AFEM_HIP(hipEventRecord(e1, stream()));
       AFEM_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1817,10 +1825,11 @@ static void choose_fp32(afem_system* s, afem_solver_ws* w)
       w->unr32 = u;
       w->bs32 = bs;
       float t_best = 1e30f;
-      for (int rep = 0; rep < 4; ++rep) {
+      for (int rep = 0; rep < kTuneReps; ++rep) {
         AFEM_HIP(hipEventRecord(e0, stream()));
-        spmv_launch<float, float, kPlain>(s, w, true, w->p32.p, w->q32.p, nullptr, always, nullptr, spmv_blocks(s),
-                                          "spmv_tune_fp32");
+        for (int l = 0; l < kTuneLaunches; ++l)
+          spmv_launch<float, float, kPlain>(s, w, true, w->p32.p, w->q32.p, nullptr, always, nullptr,
+                                            spmv_blocks(s), "spmv_tune_fp32");
         AFEM_HIP(hipEventRecord(e1, stream()));
         AFEM_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
