@@ -420,7 +420,12 @@ AFEM_API int afem_system_solve_noalloc(afem_system_t sys, const afem_solver_opti
 
 /* y = A x with the system's matrix (FP64). x, y [device] nb_dof values. */
 AFEM_API int afem_system_spmv(afem_system_t sys, const double* x, double* y);
-/* y = A32 x32 with the FP32 copy of the system's matrix (made on first use). */
+/* y = A32 x32 with the FP32 copy of the system's matrix (made on first use).
+ * The copy is in difference form when every non-empty row (block row) has
+ * its diagonal entry: off-diagonal entries rounded to FP32, the diagonal
+ * entry (block) replaced by the row sum taken in FP64, and the product
+ * formed as s_i x_i + sum_{j != i} a_ij (x_j - x_i) -- the same value, without
+ * the cancellation a rounded diagonal leaves on smooth vectors. */
 AFEM_API int afem_system_spmv_fp32(afem_system_t sys, const float* x, float* y);
 /* Installs a sparse preconditioner M with the pattern of the system's matrix:
  * values [device] nnz FP32 in the order of the matrix values; z = M r.  Used
@@ -541,7 +546,9 @@ AFEM_API int afem_batch_set_tiles(afem_batch_t batch, int lanes, int tiles);
 /* Times the (lanes, tiles) candidates on one chunk and keeps the fastest. */
 AFEM_API int afem_batch_autotune(afem_batch_t batch, int* lanes, int* tiles, double* ms_per_iteration);
 /* Solves every system; results [host] nb_segment entries (may be NULL).
- * AFEM_ERR_NOT_CONVERGED when a system misses the tolerance. */
+ * AFEM_ERR_NOT_CONVERGED when a system misses the tolerance;
+ * AFEM_ERR_ARGUMENT when a non-empty matrix row has no diagonal entry (the
+ * FP32 product keeps each row sum in the diagonal slot). */
 AFEM_API int afem_batch_solve(afem_batch_t batch, const afem_batch_options* opt,
                               afem_batch_result* results, double* total_ms);
 /* Exchange used by the last persistent solve of the batch: 1 XCD-local teams,
