@@ -1410,6 +1410,7 @@ struct afem_solver_ws
   // sparse preconditioner matrix (pattern of A, FP32), CSR and SELL order
   afem::DBuf<float> spai32, spai_sell32;
   afem::DBuf<uint32_t> spai_sell_pk; // BF16 values packed with the 16-bit column offsets, 4 slots per lane together
+  size_t pk_words = 0;                // its size: the 4-slot layout's slots times 64 lanes
   afem::DBuf<int32_t> sell_off4;     // slice offsets of that layout (widths rounded up to multiples of 4)
   bool has_spai = false;
   bool spai_sell_valid = false;
@@ -1931,7 +1932,8 @@ static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
   if (w->format != kFmtSell && w->format != kFmtSell16 && w->format != kFmtSell16x8)
     return;
   if (!w->spai_sell_valid) {
-    w->spai_sell32.alloc(w->sell_col.n);
+    if (w->spai_sell32.n != w->sell_col.n) // a new M on the same pattern reuses the buffers
+      w->spai_sell32.alloc(w->sell_col.n);
     const int64_t rows_pad = ((int64_t)s->nb_dof + 63) / 64 * 64;
     hipLaunchKernelGGL(k_sell_fill_f32, dim3(grid_for(rows_pad, 256)), dim3(256), 0, stream(), s->nb_dof,
                        s->row_index, w->spai32.p, w->sell_off.p, w->spai_sell32.p);
@@ -1949,11 +1951,10 @@ static void ensure_spai_sell(afem_system* s, afem_solver_ws* w, bool bf16)
         o4[k + 1] = o4[k] + (o[k + 1] - o[k] + 3) / 4 * 4;
       w->sell_off4.alloc(o4.size());
       w->sell_off4.upload(o4.data(), o4.size());
+      w->pk_words = (size_t)o4[(size_t)nslice] * 64;
     }
-    std::vector<int32_t> last(1);
-    AFEM_HIP(hipMemcpyAsync(last.data(), w->sell_off4.p + nslice, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
-    AFEM_HIP(hipStreamSynchronize(stream()));
-    w->spai_sell_pk.alloc((size_t)last[0] * 64);
+    if (w->spai_sell_pk.n != w->pk_words)
+      w->spai_sell_pk.alloc(w->pk_words);
     hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)nslice), dim3(64), 0, stream(), w->sell_off.p, w->sell_off4.p,
                        w->spai_sell32.p, w->sell_dcol.p, w->spai_sell_pk.p);
     check_launch("spai_to_bf16");

@@ -172,28 +172,54 @@ __global__ void k_features(int32_t n, const int32_t* __restrict__ rp, const int3
   const bool idi = ident[i] != 0;
   const double si = s[i];
   float f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, cnt = 0.f, f7 = 0.f;
-  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    const int32_t j = ci[k];
-    float a;
-    if (j == i)
-      a = idi ? 1.f : (float)(v[k] * si * si);
-    else
-      a = (idi || ident[j]) ? 0.f : (float)(v[k] * si * (double)s[j]);
-    shat[k] = a;
-    if (j != i && a != 0.f) {
-      const float m = fabsf(a);
-      f1 += a;
-      f2 += m;
-      f3 = fmaxf(f3, m);
-      f4 += a * a;
-      f7 = (cnt == 0.f) ? m : fminf(f7, m);
-      cnt += 1.f;
+  const int32_t e = rp[i + 1];
+  // four entries at a time: their loads, then their neighbours' scales, in flight together
+  for (int32_t k0 = rp[i]; k0 < e; k0 += 4) {
+    int32_t j[4];
+    double vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = k0 + u < e;
+      j[u] = in ? ci[k0 + u] : i;
+      vv[u] = in ? v[k0 + u] : 0.0;
+    }
+    float sj[4];
+    bool idj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sj[u] = s[j[u]];
+      idj[u] = ident[j[u]] != 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u >= e)
+        break;
+      float a;
+      if (j[u] == i)
+        a = idi ? 1.f : (float)(vv[u] * si * si);
+      else
+        a = (idi || idj[u]) ? 0.f : (float)(vv[u] * si * (double)sj[u]);
+      shat[k0 + u] = a;
+      if (j[u] != i && a != 0.f) {
+        const float m = fabsf(a);
+        f1 += a;
+        f2 += m;
+        f3 = fmaxf(f3, m);
+        f4 += a * a;
+        f7 = (cnt == 0.f) ? m : fminf(f7, m);
+        cnt += 1.f;
+      }
     }
   }
   const float f[kF] = { 1.f, f1, f2, f3, f4, cnt * (1.f / 16.f), idi ? 1.f : 0.f, f7, 0, 0, 0, 0, 0, 0, 0, 0 };
+  // the row's 16 BF16 features as two 16-byte stores
+  uint32_t w[kF / 2];
 #pragma unroll
-  for (int q = 0; q < kF; ++q)
-    x[(int64_t)i * kF + q] = f2bf(f[q]);
+  for (int q = 0; q < kF / 2; ++q)
+    w[q] = (uint32_t)f2bf(f[2 * q]) | ((uint32_t)f2bf(f[2 * q + 1]) << 16);
+  uint4* xo = reinterpret_cast<uint4*>(x + (int64_t)i * kF);
+  xo[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  xo[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
 // ---- MFMA dense layers --------------------------------------------------------------------------
@@ -433,23 +459,53 @@ k_reduce_partials(int nparts, int stride, int len, const float* __restrict__ par
 
 // ---- message passing and heads ------------------------------------------------------------------
 
-// a_i = sum_{j != i} |s_ij| h_j (BF16 in, FP32 accumulation, BF16 out); 32
-// lanes per row, one feature per lane.
+// a_i = sum_{j != i} |s_ij| h_j (BF16 in, FP32 accumulation, BF16 out).
+// Four lanes per row, eight features (one 16-byte load) per lane; the row's
+// entries are taken four at a time, so four gathers are in flight per lane
+// (the entries are added in row order, as one lane per feature would).
+constexpr int kAggLanes = 4;
+
 __global__ void __launch_bounds__(256)
 k_aggregate(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
             const float* __restrict__ shat, const uint16_t* __restrict__ h, uint16_t* __restrict__ a)
 {
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
-  const int f = threadIdx.x & 31;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kAggLanes;
+  const int q = threadIdx.x & (kAggLanes - 1); // features 8q .. 8q + 7
   if (i >= n)
     return;
-  float s = 0.f;
-  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    const int32_t j = ci[k];
-    if (j != i)
-      s += fabsf(shat[k]) * bf2f(h[(int64_t)j * 32 + f]);
+  float acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+    acc[t] = 0.f;
+  const int32_t e = rp[i + 1];
+  for (int32_t k = rp[i]; k < e; k += 4) {
+    int32_t j[4];
+    float w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = k + u < e;
+      j[u] = in ? ci[k + u] : (int32_t)i;
+      w[u] = (in && j[u] != i) ? fabsf(shat[k + u]) : 0.f;
+    }
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[u] = *reinterpret_cast<const uint4*>(h + (int64_t)j[u] * 32 + 8 * q);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t ws[4] = { v[u].x, v[u].y, v[u].z, v[u].w };
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc[2 * t] += w[u] * __uint_as_float(ws[t] << 16);
+        acc[2 * t + 1] += w[u] * __uint_as_float(ws[t] & 0xffff0000u);
+      }
+    }
   }
-  a[i * 32 + f] = f2bf(s);
+  uint32_t o[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    o[t] = (uint32_t)f2bf(acc[2 * t]) | ((uint32_t)f2bf(acc[2 * t + 1]) << 16);
+  *reinterpret_cast<uint4*>(a + i * 32 + 8 * q) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // g_out_i += sum_{j != i} |s_ij| g_in_j in FP32 (|S| off the diagonal is
@@ -486,51 +542,92 @@ __device__ __forceinline__ void load_vec32(const uint16_t* __restrict__ p, float
   }
 }
 
-__device__ __forceinline__ float dot32(const float* a, const uint16_t* __restrict__ bp)
-{
-  float b[32];
-  load_vec32(bp, b);
-  float s = 0.f;
-#pragma unroll
-  for (int q = 0; q < 32; ++q)
-    s += a[q] * b[q];
-  return s;
-}
-
 // Mh (FP32, pattern of A) and the preconditioner M = D^-1/2 Mh D^-1/2
 // (FP32) that the solver applies; pre_d keeps the diagonal pre-activation.
+// Four lanes per row, eight features per lane (one 16-byte load of u_j);
+// the row's entries are taken four at a time (four gathers in flight per
+// lane), the partial dot products summed across the four lanes by a
+// butterfly (every lane ends with the same bits, and u_i . u_j and u_j . u_i
+// are formed from the same products in the same order, so Mh stays exactly
+// symmetric), and lane u writes entry k + u.
+constexpr int kHeadLanes = 4;
+
+__device__ __forceinline__ void unpack8(uint4 v, float* f)
+{
+  const uint32_t ws[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f[2 * t] = __uint_as_float(ws[t] << 16);
+    f[2 * t + 1] = __uint_as_float(ws[t] & 0xffff0000u);
+  }
+}
+
+__device__ __forceinline__ float sum4(float v)
+{
+  v += __shfl_xor(v, 1, kHeadLanes);
+  v += __shfl_xor(v, 2, kHeadLanes);
+  return v;
+}
+
 __global__ void __launch_bounds__(256)
 k_head(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const float* __restrict__ shat,
        const float* __restrict__ s, const uint8_t* __restrict__ ident, const uint16_t* __restrict__ h3,
        const uint16_t* __restrict__ u, const float* __restrict__ params, float* __restrict__ mhat,
        float* __restrict__ mval, float* __restrict__ pre_d)
 {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n)
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHeadLanes;
+  const int q = threadIdx.x & (kHeadLanes - 1); // features 8q .. 8q + 7
+  if (i >= n) // the four lanes of a row leave together
     return;
   const bool idi = ident[i] != 0;
-  float hv[32];
-  load_vec32(h3 + (int64_t)i * 32, hv);
-  float pd = params[oBd];
+  float hv[8];
+  unpack8(*reinterpret_cast<const uint4*>(h3 + i * 32 + 8 * q), hv);
+  float pd = 0.f;
 #pragma unroll
-  for (int q = 0; q < 32; ++q)
-    pd += hv[q] * params[oWd + q];
-  pre_d[i] = pd;
-  float ui[32];
-  load_vec32(u + (int64_t)i * 32, ui);
+  for (int t = 0; t < 8; ++t)
+    pd += hv[t] * params[oWd + 8 * q + t];
+  pd = params[oBd] + sum4(pd);
+  if (q == 0)
+    pre_d[i] = pd;
+  float ui[8];
+  unpack8(*reinterpret_cast<const uint4*>(u + i * 32 + 8 * q), ui);
   const float c = params[oC];
   const float si = s[i];
-  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    const int32_t j = ci[k];
-    float m;
-    if (j == i)
-      m = idi ? 1.f : softplus(pd);
-    else {
-      const float a = shat[k];
-      m = (a != 0.f) ? a * (c + dot32(ui, u + (int64_t)j * 32)) : 0.f;
+  const float diag = idi ? 1.f : softplus(pd);
+  const int32_t e = rp[i + 1];
+  for (int32_t k = rp[i]; k < e; k += 4) {
+    int32_t j[4];
+    float a[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const bool in = k + v < e;
+      j[v] = in ? ci[k + v] : (int32_t)i;
+      a[v] = in ? shat[k + v] : 0.f;
     }
-    mhat[k] = m;
-    mval[k] = si * m * s[j];
+    uint4 w[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      w[v] = *reinterpret_cast<const uint4*>(u + (int64_t)j[v] * 32 + 8 * q);
+    float d[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float uj[8];
+      unpack8(w[v], uj);
+      float t = 0.f;
+#pragma unroll
+      for (int f = 0; f < 8; ++f)
+        t += ui[f] * uj[f];
+      d[v] = sum4(t);
+    }
+    // lane q writes entry k + q
+    const int32_t jq = q == 0 ? j[0] : q == 1 ? j[1] : q == 2 ? j[2] : j[3];
+    const float aq = q == 0 ? a[0] : q == 1 ? a[1] : q == 2 ? a[2] : a[3];
+    const float dq = q == 0 ? d[0] : q == 1 ? d[1] : q == 2 ? d[2] : d[3];
+    if (k + q < e) {
+      const float m = (jq == i) ? diag : (aq != 0.f ? aq * (c + dq) : 0.f);
+      mhat[k + q] = m;
+      mval[k + q] = si * m * s[jq];
+    }
   }
 }
 
@@ -851,7 +948,7 @@ void forward(afem_spai* m, afem_system* s)
   hipLaunchKernelGGL(k_gemm_fwd<kF>, dim3(tiles32(n)), dim3(256), 0, stream(), n, m->x.p, nullptr, W + bW0t,
                      P + oB0, nullptr, m->pre[0].p, m->h[0].p, 1);
   check_launch("spai_gemm_fwd0");
-  const unsigned ga = (unsigned)(((int64_t)n * 32 + 255) / 256);
+  const unsigned ga = (unsigned)(((int64_t)n * kAggLanes + 255) / 256);
   for (int l = 1; l <= kL; ++l) {
     hipLaunchKernelGGL(k_aggregate, dim3(ga), dim3(256), 0, stream(), n, s->row_index, s->columns, m->shat.p,
                        m->h[l - 1].p, m->a[l].p);
@@ -861,7 +958,8 @@ void forward(afem_spai* m, afem_system* s)
   }
   hipLaunchKernelGGL(k_gemm_fwd<kH>, dim3(tiles32(n)), dim3(256), 0, stream(), n, m->h[kL].p, nullptr, W + bWut,
                      nullptr, nullptr, nullptr, m->u.p, 0);
-  hipLaunchKernelGGL(k_head, dim3(g), dim3(256), 0, stream(), n, s->row_index, s->columns, m->shat.p, m->s.p,
+  hipLaunchKernelGGL(k_head, dim3((unsigned)(((int64_t)n * kHeadLanes + 255) / 256)), dim3(256), 0, stream(), n,
+                     s->row_index, s->columns, m->shat.p, m->s.p,
                      m->ident.p, m->h[kL].p, m->u.p, P, m->mhat.p, m->mval.p, m->pre_d.p);
   check_launch("spai_head");
   m->forward_valid = true;

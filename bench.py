@@ -220,11 +220,17 @@ def learned_section(A, args, X, C, mesh, bsr, N, nnz):
     x64 = sysm.solution().numpy()
     res["cg_fp64"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_CG_FP64, **kw)
     res["jacobi_pcg_mixed"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_PCG_MIXED, **kw)
-    A.synchronize()
-    t0 = time.perf_counter()
-    net.apply(sysm)
-    A.synchronize()
-    res["learned_inference_ms"] = (time.perf_counter() - t0) * 1e3
+    # inference: the network's forward pass on the system and the installation
+    # of M (its SELL copy and BF16 words, made by a zero-iteration solve).  The
+    # first application at this mesh size also allocates the activations;
+    # the second, on the same buffers, is the per-system cost.
+    for key in ("learned_inference_first_ms", "learned_inference_ms"):
+        A.synchronize()
+        t0 = time.perf_counter()
+        net.apply(sysm)
+        sysm.solve(raise_on_failure=False, method=A.SOLVER_FPCG_SPAI, precond_bf16=1, max_iter=0)
+        A.synchronize()
+        res[key] = (time.perf_counter() - t0) * 1e3
     res["learned_fpcg_mixed_fp32m"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_FPCG_SPAI, **kw)
     res["learned_fpcg_mixed"] = solve_best(A, sysm, x_fp64=x64, method=A.SOLVER_FPCG_SPAI, precond_bf16=1, **kw)
     res["learned_fpcg_mixed"]["preconditioner_values"] = "bf16"
