@@ -509,22 +509,49 @@ k_aggregate(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict
 }
 
 // g_out_i += sum_{j != i} |s_ij| g_in_j in FP32 (|S| off the diagonal is
-// symmetric, so this is also the transposed product).
+// symmetric, so this is also the transposed product).  Eight lanes per row,
+// four features (one 16-byte load) per lane, four neighbours in flight; the
+// entries are added in row order.
+constexpr int kAggGradLanes = 8;
+
 __global__ void __launch_bounds__(256)
 k_aggregate_grad(int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                  const float* __restrict__ shat, const float* __restrict__ g_in, float* __restrict__ g_out)
 {
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
-  const int f = threadIdx.x & 31;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kAggGradLanes;
+  const int q = threadIdx.x & (kAggGradLanes - 1); // features 4q .. 4q + 3
   if (i >= n)
     return;
-  float s = 0.f;
-  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    const int32_t j = ci[k];
-    if (j != i)
-      s += fabsf(shat[k]) * g_in[(int64_t)j * 32 + f];
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int32_t e = rp[i + 1];
+  for (int32_t k = rp[i]; k < e; k += 4) {
+    int32_t j[4];
+    float w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = k + u < e;
+      j[u] = in ? ci[k + u] : (int32_t)i;
+      w[u] = (in && j[u] != i) ? fabsf(shat[k + u]) : 0.f;
+    }
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[u] = *reinterpret_cast<const float4*>(g_in + (int64_t)j[u] * 32 + 4 * q);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc.x += w[u] * v[u].x;
+      acc.y += w[u] * v[u].y;
+      acc.z += w[u] * v[u].z;
+      acc.w += w[u] * v[u].w;
+    }
   }
-  g_out[i * 32 + f] += s;
+  float4* out = reinterpret_cast<float4*>(g_out + i * 32 + 4 * q);
+  float4 o = *out;
+  o.x += acc.x;
+  o.y += acc.y;
+  o.z += acc.z;
+  o.w += acc.w;
+  *out = o;
 }
 
 __device__ __forceinline__ void load_vec32(const uint16_t* __restrict__ p, float* v)
@@ -742,10 +769,13 @@ k_head_backward(int32_t n, const int32_t* __restrict__ rp, const int32_t* __rest
       }
       pv[kH + 1] = dc;
     }
+    // the row's 32 + 32 values as 16-byte stores
+    float4* duo = reinterpret_cast<float4*>(du + (int64_t)i * 32);
+    float4* dho = reinterpret_cast<float4*>(dh3 + (int64_t)i * 32);
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      du[(int64_t)i * 32 + q] = dui[q];
-      dh3[(int64_t)i * 32 + q] = dh[q];
+    for (int q = 0; q < 8; ++q) {
+      duo[q] = make_float4(dui[4 * q], dui[4 * q + 1], dui[4 * q + 2], dui[4 * q + 3]);
+      dho[q] = make_float4(dh[4 * q], dh[4 * q + 1], dh[4 * q + 2], dh[4 * q + 3]);
     }
   }
   block_sum_write<kHeadParts>(pv, part + (int64_t)blockIdx.x * kHeadParts);
@@ -1020,7 +1050,7 @@ void backward(afem_spai* m, afem_system* s)
   hipLaunchKernelGGL(k_gemm_dx<kH>, dim3(tiles32(n)), dim3(256), 0, stream(), n, m->du.p, nullptr, W + bWu,
                      nullptr, dh, 1, nullptr);
   check_launch("spai_gemm_dx_u");
-  const unsigned ga = (unsigned)(((int64_t)n * 32 + 255) / 256);
+  const unsigned ga = (unsigned)(((int64_t)n * kAggGradLanes + 255) / 256);
   for (int l = kL; l >= 1; --l) {
     weight_grad<2 * kH>(m, n, m->h[l - 1].p, m->a[l].p, dh, m->pre[l].p, layer_w(l), layer_b(l));
     // dh_next = dh + (dpre W^T)[:, :32];  da = (dpre W^T)[:, 32:]
