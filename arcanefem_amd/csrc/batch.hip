@@ -1106,6 +1106,7 @@ struct afem_batch
   DBuf<float> sv32;
   int nslice = 0;
   DBuf<float> v32, dinv, r32, z32, p32, q32;
+  DBuf<int32_t> missing; // set by k_prepare when a non-empty row lacks its diagonal entry
   DBuf<double> r64, parts;
   DBuf<SegState> st;
   SegState* host_st = nullptr;
@@ -1337,7 +1338,9 @@ void prepare(afem_batch* b)
     b->q32.alloc(n);
     b->r64.alloc(n);
   }
-  DBuf<int32_t> missing(1);
+  if (!b->missing.p)
+    b->missing.alloc(1);
+  DBuf<int32_t>& missing = b->missing;
   missing.zero();
   hipLaunchKernelGGL(k_prepare, dim3(1024), dim3(256), 0, stream(), n, s->row_index, s->columns, s->values, b->v32.p,
                      b->dinv.p, missing.p);

@@ -1397,7 +1397,8 @@ struct afem_solver_ws
   afem::DBuf<double> sell_v64;
   afem::DBuf<float> sell_v32;
   bool has_fp32 = false;
-  bool diff32 = false; // the FP32 copies (scalar matrices) are in difference form
+  bool diff32 = false; // the FP32 copies are in difference form
+  afem::DBuf<int32_t> missing; // set by the difference-form copy when a row lacks its diagonal entry
   bool values_valid = false;
   bool pattern_valid = false;
   int format = -1; // chosen SpMV format
@@ -1577,7 +1578,9 @@ static void ensure_derived(afem_system* s, afem_solver_ws* w, bool need_fp32)
   if (w->has_fp32) {
     w->diff32 = false;
     if (s->block == 1 || s->block == 3) {
-      DBuf<int32_t> missing(1);
+      if (!w->missing.p)
+        w->missing.alloc(1);
+      DBuf<int32_t>& missing = w->missing;
       missing.zero();
       if (s->block == 1)
         hipLaunchKernelGGL(k_to_float_rowsum, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, s->row_index,
