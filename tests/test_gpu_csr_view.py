@@ -318,3 +318,41 @@ def test_malformed_views_rejected(afem_gpu):
     y = A.DeviceArray(N, np.float64)
     s.spmv(x, y)
     assert np.array_equal(y.numpy(), good_y.numpy())
+
+
+def test_fp32_copy_without_diagonal_entry(afem_gpu):
+    """A CSR view with a non-empty row that lacks its diagonal entry cannot
+    hold the difference form (the row sum needs the diagonal slot): the
+    single-system FP32 product keeps the plain rounded copy and still equals
+    the FP64 product to FP32 accuracy, and the batched solver refuses the
+    matrix with AFEM_ERR_ARGUMENT."""
+    A = afem_gpu
+    print("[case] fp32 copy of a view without a diagonal entry", flush=True)
+    rng = np.random.default_rng(3)
+    N = 40
+    dense = np.zeros((N, N))
+    for i in range(N):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < N and (i < 20) == (j < 20):  # two diagonal blocks, rows 0-19 and 20-39
+                dense[i, j] = rng.uniform(1.0, 2.0) * (4.0 if i == j else -1.0)
+    dense[7, 7] = 0.0  # row 7 keeps its off-diagonal entries only
+    row = np.zeros(N + 1, np.int32)
+    cols, vals = [], []
+    for i in range(N):
+        nz = np.nonzero(dense[i])[0]
+        cols += nz.tolist()
+        vals += dense[i, nz].tolist()
+        row[i + 1] = row[i] + len(nz)
+    col, val = np.array(cols, np.int32), np.array(vals)
+    rows, dcol, dval, _ = reference_view(A, row, col, val)
+    s = A.DoFLinearSystem(N)
+    s.set_csr_values(rows, dcol, dval)
+    x = rng.normal(size=N)
+    y32 = A.DeviceArray(N, np.float32)
+    s.spmv_fp32(A.DeviceArray.from_numpy(x.astype(np.float32)), y32)
+    ref = dense @ x
+    assert np.abs(y32.numpy() - ref).max() <= 2e-6 * np.abs(dense).max() * np.abs(x).max()
+    bs = A.BatchSolver(s, np.array([0, 20, N], np.int32))
+    with pytest.raises(A.AfemError) as e:
+        bs.solve(rtol=1e-8, max_iter=10)
+    assert e.value.status == A.ERR_ARGUMENT
