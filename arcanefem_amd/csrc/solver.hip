@@ -1971,9 +1971,10 @@ static void prec_launch(afem_system* s, afem_solver_ws* w, int k, const Parts& P
   ProfScope prof("fpcg_precond_spmv");
   const PrecIn in{ w->r32.p, w->r64.p, true };
   if (w->format == kFmtSell16 || w->format == kFmtSell16x8) {
-    // 256 threads: measured faster than 512 for this kernel at 128^3 (52.8 vs
-    // 59.8 us with BF16 values), unlike the A product
-    const int bs = 256;
+    // workgroup size measured at 128^3 in the learned solve: the packed BF16
+    // words run faster with 512 threads (104 against 108 ms), the FP32
+    // values with 256 (123 against 131 ms)
+    const int bs = bf16 ? 512 : 256;
     auto go = [&](auto btag) {
       constexpr int BS = decltype(btag)::value;
       if (bf16) // the packed words carry their offsets, in the 4-slot layout
