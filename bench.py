@@ -648,8 +648,8 @@ def main():
                     help="timed repetitions of the all-cores CPU Jacobi-PCG (0 = skip the CPU solve)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this implementation runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the path across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     import arcanefem_amd as A
