@@ -478,10 +478,11 @@ def poisson2d_section(A, args):
     return out
 
 
-def elasticity_section(A, args):
+def elasticity_section(A, args, assembly_only=False):
     """3D linear elasticity P1: 96^3 hexes x 6 tets (jitter 0.15h), E = 21e5,
     nu = 0.28, body force (0, 0, -1), face x = 0 clamped; BSR with 3x3 FP64
-    blocks; mixed-precision block-Jacobi PCG to 1e-8."""
+    blocks; mixed-precision block-Jacobi PCG to 1e-8.  assembly_only: the
+    timed assembly steps alone (the entry point of counter runs)."""
     n = 96
     X, C = A.box_mesh(3, n, 0.15, seed=args.seed)
     N, T = X.shape[0], C.shape[0]
@@ -505,6 +506,10 @@ def elasticity_section(A, args):
     A.profile_enable(False)
     k_ms, _ = kernel_avg_ms(A, "assemble_elasticity_tet")
     alg = 16 * T + 24 * N + 76 * nb + 4 * (N + 1)
+    if assembly_only:
+        return {"workload": "3D linear elasticity P1 assembly, 96^3 hexes x 6 tets", "assembly_ms": asm_ms,
+                "assembly_kernel_ms": k_ms, "algorithmic_bytes": alg,
+                "assembly_GBs": alg / (k_ms * 1e-3) / 1e9 if k_ms == k_ms else None}
     sysm = A.DoFLinearSystem(3 * N)
     rhs = sysm.rhs()
     rhs.zero()
@@ -642,6 +647,8 @@ def main():
                     help="run one 256 MiB device-to-device copy first (reference for HBM byte counters)")
     ap.add_argument("--assembly-only", action="store_true",
                     help="pattern and the timed assembly steps only (the entry point of counter runs)")
+    ap.add_argument("--elasticity-assembly-only", action="store_true",
+                    help="the timed elasticity assembly steps only (the entry point of counter runs)")
     ap.add_argument("--variant", type=int, default=None,
                     help="pin the Laplace assembly variant instead of autotuning")
     ap.add_argument("--cpu-solve-reps", type=int, default=5,
@@ -655,10 +662,6 @@ def main():
     import arcanefem_amd as A
 
     A.init(0)
-    copy = measure_copy(A) if not args.assembly_only else None
-    if copy:
-        log(f"[bench] stream copy: {copy['GBs']:.0f} GB/s ({copy['bytes_moved'] / 1e9:.2f} GB moved in "
-            f"{copy['kernel_ms']:.3f} ms)")
     if args.calibrate:
         nbytes = 256 << 20
         src, dst = A.DeviceArray(nbytes, np.uint8), A.DeviceArray(nbytes, np.uint8)
@@ -667,6 +670,13 @@ def main():
         A.synchronize()
         log(f"[bench] calibration copy of {nbytes} bytes done")
         del src, dst
+    if args.elasticity_assembly_only:
+        print(json.dumps(elasticity_section(A, args, assembly_only=True)), flush=True)
+        return
+    copy = measure_copy(A) if not args.assembly_only else None
+    if copy:
+        log(f"[bench] stream copy: {copy['GBs']:.0f} GB/s ({copy['bytes_moved'] / 1e9:.2f} GB moved in "
+            f"{copy['kernel_ms']:.3f} ms)")
     n = args.n
     t0 = time.time()
     X, C = A.box_mesh(3, n, 0.15, seed=args.seed)
