@@ -681,7 +681,13 @@ def main():
     t0 = time.time()
     X, C = A.box_mesh(3, n, 0.15, seed=args.seed)
     N, T = X.shape[0], C.shape[0]
-    log(f"[bench] mesh {n}^3: N={N} T={T} ({time.time() - t0:.1f}s)")
+    # smallest signed 6|T| over the unjittered cell's h^3: the jittered mesh stays positively oriented
+    e1, e2, e3 = X[C[:, 1]] - X[C[:, 0]], X[C[:, 2]] - X[C[:, 0]], X[C[:, 3]] - X[C[:, 0]]
+    min_det_ratio = float(np.einsum("ij,ij->i", e1, np.cross(e2, e3)).min()) * float(n) ** 3
+    del e1, e2, e3
+    assert min_det_ratio > 0.0, "inverted cell in the benchmark mesh"
+    log(f"[bench] mesh {n}^3: N={N} T={T}, smallest signed 6|T| / h^3 = {min_det_ratio:.3f} "
+        f"({time.time() - t0:.1f}s)")
     mesh = A.Mesh(3, X, C)
     bsr = A.BSRFormat(mesh, 1, use_csr=True)
     A.synchronize()
@@ -766,6 +772,7 @@ def main():
             "workload": f"3D Poisson P1 assembly (CSR values + constant-source RHS), {n}^3 hexes x 6 Kuhn tets, "
                         f"jitter 0.15h seed {args.seed}",
             "nodes": N, "tets": T, "nnz": nnz, "parallelism": "single GPU",
+            "min_signed_det_over_h3": min_det_ratio,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

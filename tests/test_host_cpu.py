@@ -81,6 +81,32 @@ def test_box_mesh_geometry(dim, n, alpha):
     assert counts.max() == 2
 
 
+def signed_measure_ratio(dim, X, C, n):
+    """Signed cell measures (2|T| or 6|T|, with the node order's sign) over the
+    unjittered cell's value h^dim."""
+    if dim == 2:
+        a, b = X[C[:, 1], :2] - X[C[:, 0], :2], X[C[:, 2], :2] - X[C[:, 0], :2]
+        det = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+    else:
+        a, b, c = X[C[:, 1]] - X[C[:, 0]], X[C[:, 2]] - X[C[:, 0]], X[C[:, 3]] - X[C[:, 0]]
+        det = np.einsum("ij,ij->i", a, np.cross(b, c))
+    return det * float(n) ** dim
+
+
+@pytest.mark.parametrize("dim,n,alpha,seed", [(3, 128, 0.15, 0), (3, 96, 0.15, 0), (3, 64, 0.15, 1),
+                                              (2, 256, 0.2, 0), (2, 509, 0.2, 163)])
+def test_benchmark_meshes_positively_oriented(dim, n, alpha, seed):
+    """Every cell of the benchmark's jittered meshes (the 128^3 Poisson and
+    96^3 elasticity meshes, the 64^3 training mesh, the 2D 256^2 mesh and a
+    mesh of the batch's largest size, 509^2) keeps a positive signed measure: the reference
+    assembles with |det|, so an inverted cell would change nothing there but
+    would be a different mesh."""
+    X, C = A.box_mesh(dim, n, alpha, seed=seed)
+    r = signed_measure_ratio(dim, X, C, n)
+    assert r.min() > 0.0
+    assert r.min() > 0.25  # at least a quarter of the unjittered cell's value
+
+
 def test_box_mesh_seeded():
     X1, _ = A.box_mesh(3, 5, 0.15, seed=1)
     X2, _ = A.box_mesh(3, 5, 0.15, seed=1)
